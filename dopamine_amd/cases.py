@@ -1,0 +1,99 @@
+"""Workload definitions: the reference's regression cases that the mesh builder can regenerate,
+and the benchmark configurations.
+
+Regression cases follow tests/regression/cases/<name>/case.input with meshes from
+tests/regression/mk_box_mesh.py (its node, hex and quad order are what mesh.box_elements emits).
+"""
+import math
+from dataclasses import dataclass, field
+from typing import Callable
+
+from . import mesh as M
+from .params import SolverParams
+
+
+@dataclass
+class Case:
+    name: str
+    make_mesh: Callable
+    params: SolverParams
+    ic: str = "taylor_green"      # 'taylor_green' | 'uniform'
+    ic_uref: float = 1.0
+    nsteps: int = 20
+    stats_every: int = 1
+    notes: str = ""
+    extra: dict = field(default_factory=dict)
+
+
+def _skew_periodic(n, amp):
+    return lambda: M.skewed_box(n, amp)
+
+
+def regression_cases():
+    """Cases whose reference CSV (tests/regression/reference/<name>_np1.csv) the builder can reproduce."""
+    cases = {}
+    # tests/regression/cases/tgv_skew/case.input
+    cases["tgv_skew"] = Case(
+        "tgv_skew", _skew_periodic(16, 0.08),
+        SolverParams(nu=6.25e-4, advection_scheme="KEP", use_sgs=False, dt=2.0e-2, cfl_max=0.6, p_tol=1.0e-10,
+                     p_max_iter=1000, n_nonortho_corr=3, rhie_chow_dual=True, cell_projection="LS",
+                     pressure_min_cos_theta=0.0),
+        nsteps=30)
+    # tests/regression/cases/tgv_skew_chk/case.input
+    cases["tgv_skew_chk"] = Case(
+        "tgv_skew_chk", _skew_periodic(16, 0.08),
+        SolverParams(nu=6.25e-4, advection_scheme="KEP", use_sgs=False, dt=2.0e-2, cfl_max=0.6, p_tol=1.0e-10,
+                     p_max_iter=1000, n_nonortho_corr=5, nonortho_check=True, rhie_chow_dual=True,
+                     cell_projection="LS", pressure_min_cos_theta=0.0),
+        nsteps=30)
+    walls = ("noSlip",) * 6
+    # tests/regression/cases/hydrowall/case.input
+    cases["hydrowall"] = Case(
+        "hydrowall", lambda: M.skewed_box(12, 0.04, group_types=walls),
+        SolverParams(nu=1.0e-2, f_const=(0.0, 0.0, -1.0), advection_scheme="CDS", use_sgs=False, dt=2.0e-3,
+                     cfl_max=0.5, p_tol=1.0e-12, p_max_iter=1000, n_nonortho_corr=4, rhie_chow_dual=False,
+                     cell_projection="LS", pressure_min_cos_theta=0.0),
+        ic="uniform", ic_uref=0.0, nsteps=20, stats_every=10)
+    outl = ("periodic", "periodic", "periodic", "periodic", "outlet", "outlet")
+    # tests/regression/cases/hydro/case.input
+    cases["hydro"] = Case(
+        "hydro", lambda: M.skewed_box(12, 0.04, group_types=outl),
+        SolverParams(nu=1.0e-2, f_const=(0.0, 0.0, -1.0), advection_scheme="CDS", use_sgs=False, dt=2.0e-3,
+                     cfl_max=0.5, p_tol=1.0e-12, p_max_iter=1000, n_nonortho_corr=4, rhie_chow_dual=False,
+                     cell_projection="LS", pressure_min_cos_theta=0.0, patch_p={"z0": 1.0, "zL": 0.0}),
+        ic="uniform", ic_uref=0.0, nsteps=20, stats_every=10)
+    # tests/regression/cases/pdrop/case.input
+    cases["pdrop"] = Case(
+        "pdrop", lambda: M.skewed_box(12, 0.04, group_types=outl),
+        SolverParams(nu=1.0e-2, advection_scheme="CDS", use_sgs=False, dt=2.0e-3, cfl_max=0.5, p_tol=1.0e-12,
+                     p_max_iter=1000, n_nonortho_corr=4, rhie_chow_dual=False, cell_projection="LS",
+                     pressure_min_cos_theta=0.0, patch_p={"z0": 1.0, "zL": 0.0}),
+        ic="uniform", ic_uref=0.0, nsteps=20, stats_every=10)
+    return cases
+
+
+def tgv32(n=32, c_vreman=0.025):
+    """Taylor-Green vortex, periodic [0,2pi]^3, n^3 hexes, Re 1600, KEP, Vreman, fixed dt 1e-3.
+
+    The reference hard-codes C_v = 0.025 (src/mod_parameters.f90:20); c_vreman=0.07 is the
+    benchmark's requested value and a documented deviation.
+    """
+    L = 2 * math.pi
+    return Case(
+        f"tgv{n}", lambda: M.uniform_box(n, n, n, L, L, L,
+                                         group_names=("periodic_x0", "periodic_xL", "periodic_y0", "periodic_yL",
+                                                      "periodic_z0", "periodic_zL")),
+        SolverParams(nu=1.0 / 1600.0, advection_scheme="KEP", use_sgs=True, sgs_model="VREMAN", c_vreman=c_vreman,
+                     dt=1.0e-3, cfl_max=0.5, p_tol=1.0e-8, p_max_iter=1000, n_nonortho_corr=0,
+                     rhie_chow_dual=False, cell_projection="LS", pressure_min_cos_theta=0.0),
+        nsteps=200)
+
+
+def channel(nx=128, ny=96, nz=64):
+    """Turbulent channel 2pi x 2 x pi, tanh-stretched walls, Re_tau 395 (nu = 1/395, f_x = 1), Vreman + log-law."""
+    return Case(
+        f"channel{nx}x{ny}x{nz}", lambda: M.channel_box(nx, ny, nz),
+        SolverParams(nu=1.0 / 395.0, f_const=(1.0, 0.0, 0.0), advection_scheme="KEP", use_sgs=True,
+                     sgs_model="VREMAN", dt=1.0e-3, cfl_max=0.5, p_tol=1.0e-8, p_max_iter=1000,
+                     n_nonortho_corr=0, rhie_chow_dual=False, cell_projection="LS", pressure_min_cos_theta=0.0),
+        ic="channel_parabolic", ic_uref=1.0, nsteps=500)

@@ -1,0 +1,230 @@
+"""Mesh setup front end: element-list generators and the native mesh builder.
+
+The reference reads Gmsh v4 / OpenFOAM meshes and builds its face list, geometry, periodic pairs
+and least-squares stencil once at start-up (src/mesh/mod_gmsh_reader.f90:45-116,
+src/mesh/mod_mesh.f90:215-758, src/bc/mod_bc_setup.f90:134-330). Here the same element lists are
+produced by generators (structured hex boxes, with the node/element/boundary ordering that
+tests/regression/mk_box_mesh.py writes) and handed to the native builder in libdopamine_hip.so.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from ._lib import lib
+
+# Boundary-condition ids (src/mod_parameters.f90:43-54).
+BC_INLET, BC_OUTLET, BC_WALL_NOSLIP, BC_WALL_MODEL, BC_SLIP, BC_PERIODIC, BC_SYMMETRY, \
+    BC_FIXEDTEMP, BC_ZEROGRADIENT, BC_WALL_VELOCITY, BC_ABL_INLET = range(1, 12)
+
+# patch_type names accepted by the reference input parser (src/io/mod_input_parser.f90:1203-1233).
+BC_NAMES = {
+    "velocity": BC_INLET, "inlet": BC_INLET, "fixedvalue": BC_INLET,
+    "outlet": BC_OUTLET, "fixedpressure": BC_OUTLET, "outflow": BC_OUTLET,
+    "zerogradient": BC_ZEROGRADIENT, "noslip": BC_WALL_NOSLIP,
+    "movingwall": BC_WALL_VELOCITY, "wallvelocity": BC_WALL_VELOCITY,
+    "wallmodel": BC_WALL_MODEL, "wall": BC_WALL_MODEL, "slip": BC_SLIP, "symmetry": BC_SYMMETRY,
+    "periodic": BC_PERIODIC, "cyclic": BC_PERIODIC, "fixedtemperature": BC_FIXEDTEMP,
+    "ablinlet": BC_ABL_INLET,
+}
+
+GMSH_QUAD4, GMSH_HEX8 = 3, 5
+
+_P = ctypes.POINTER
+_d, _i32, _i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+
+
+class _Input(ctypes.Structure):
+    _fields_ = [("n_nodes", _i64), ("coords", _P(_d)),
+                ("n_vol", _i64), ("vol_type", _P(_i32)), ("vol_offset", _P(_i64)), ("vol_nodes", _P(_i32)),
+                ("n_bnd", _i64), ("bnd_type", _P(_i32)), ("bnd_offset", _P(_i64)), ("bnd_nodes", _P(_i32)),
+                ("bnd_group", _P(_i32)), ("n_groups", _i32), ("group_bc", _P(_i32))]
+
+
+class _View(ctypes.Structure):
+    _fields_ = [("n_cells", _i32), ("n_faces", _i32), ("n_int_faces", _i32), ("n_bnd_faces", _i32),
+                ("n_patches", _i32), ("n_vertices", _i32), ("n_ls", _i64), ("max_cell_faces", _i32),
+                ("ls_degenerate", _i32),
+                ("cell_centroid", _P(_d)), ("cell_volume", _P(_d)), ("cell_face_offset", _P(_i32)),
+                ("cell_faces", _P(_i32)),
+                ("face_owner", _P(_i32)), ("face_neighbour", _P(_i32)), ("face_patch", _P(_i32)),
+                ("face_area", _P(_d)), ("face_centre", _P(_d)), ("face_normal", _P(_d)), ("face_sf", _P(_d)),
+                ("face_dcf", _P(_d)), ("face_gc", _P(_d)), ("face_skew", _P(_d)), ("face_eps", _P(_d)),
+                ("patch_bc", _P(_i32)), ("patch_start", _P(_i32)), ("patch_nfaces", _P(_i32)),
+                ("periodic_pair", _P(_i32)), ("periodic_pair_dcf", _P(_d)), ("periodic_gc", _P(_d)),
+                ("ls_offset", _P(_i32)), ("ls_cell", _P(_i32)), ("ls_face", _P(_i32)), ("ls_weight", _P(_d)),
+                ("vertex_coords", _P(_d)), ("cell_vertex_offset", _P(_i64)), ("cell_vertices", _P(_i32))]
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(_P(ct))
+
+
+def _copy(p, n, shape=None):
+    if n == 0:
+        a = np.zeros(0, dtype=np.dtype(p._type_))
+    else:
+        a = np.ctypeslib.as_array(p, shape=(n,)).copy()
+    return a.reshape(shape) if shape else a
+
+
+class Mesh:
+    """A built mesh: plain numpy arrays (0-based), the data every hot-path kernel reads.
+
+    Attribute names follow the reference's Mesh SoA mirror (src/mesh/mod_mesh.f90:164-186).
+    """
+
+    ARRAYS = ("cell_centroid", "cell_volume", "cell_face_offset", "cell_faces", "face_owner",
+              "face_neighbour", "face_patch", "face_area", "face_centre", "face_normal", "face_sf",
+              "face_dcf", "face_gc", "face_skew", "face_eps", "patch_bc", "patch_start", "patch_nfaces",
+              "periodic_pair", "periodic_pair_dcf", "periodic_gc", "ls_offset", "ls_cell", "ls_face",
+              "ls_weight")
+
+    def __init__(self, elements):
+        L = lib()
+        L.dm_mesh_build.restype = ctypes.c_void_p
+        L.dm_mesh_build.argtypes = [ctypes.POINTER(_Input)]
+        L.dm_mesh_get_view.argtypes = [ctypes.c_void_p, ctypes.POINTER(_View)]
+        L.dm_mesh_free.argtypes = [ctypes.c_void_p]
+        L.dm_last_error.restype = ctypes.c_char_p
+        e = elements
+        keep = {k: np.ascontiguousarray(v) for k, v in e.items() if isinstance(v, np.ndarray)}
+        inp = _Input(
+            n_nodes=len(keep["coords"]) // 3, coords=_ptr(keep["coords"], _d),
+            n_vol=len(keep["vol_type"]), vol_type=_ptr(keep["vol_type"], _i32),
+            vol_offset=_ptr(keep["vol_offset"], _i64), vol_nodes=_ptr(keep["vol_nodes"], _i32),
+            n_bnd=len(keep["bnd_type"]), bnd_type=_ptr(keep["bnd_type"], _i32),
+            bnd_offset=_ptr(keep["bnd_offset"], _i64), bnd_nodes=_ptr(keep["bnd_nodes"], _i32),
+            bnd_group=_ptr(keep["bnd_group"], _i32), n_groups=len(keep["group_bc"]),
+            group_bc=_ptr(keep["group_bc"], _i32))
+        h = L.dm_mesh_build(ctypes.byref(inp))
+        if not h:
+            raise RuntimeError("mesh build failed: " + L.dm_last_error().decode())
+        try:
+            v = _View()
+            L.dm_mesh_get_view(h, ctypes.byref(v))
+            nc, nf, nb, npch = v.n_cells, v.n_faces, v.n_bnd_faces, v.n_patches
+            self.n_cells, self.n_faces, self.n_int_faces, self.n_bnd_faces = nc, nf, v.n_int_faces, nb
+            self.n_patches, self.n_ls, self.max_cell_faces = npch, v.n_ls, v.max_cell_faces
+            self.ls_degenerate = v.ls_degenerate
+            self.cell_centroid = _copy(v.cell_centroid, 3 * nc, (nc, 3))
+            self.cell_volume = _copy(v.cell_volume, nc)
+            self.cell_face_offset = _copy(v.cell_face_offset, nc + 1)
+            self.cell_faces = _copy(v.cell_faces, int(self.cell_face_offset[-1]))
+            for name in ("face_owner", "face_neighbour", "face_patch", "face_area", "face_dcf", "face_gc",
+                         "face_skew"):
+                setattr(self, name, _copy(getattr(v, name), nf))
+            for name in ("face_centre", "face_normal", "face_sf", "face_eps"):
+                setattr(self, name, _copy(getattr(v, name), 3 * nf, (nf, 3)))
+            for name in ("patch_bc", "patch_start", "patch_nfaces"):
+                setattr(self, name, _copy(getattr(v, name), npch))
+            for name in ("periodic_pair", "periodic_pair_dcf", "periodic_gc"):
+                setattr(self, name, _copy(getattr(v, name), nb))
+            self.ls_offset = _copy(v.ls_offset, nc + 1)
+            self.ls_cell = _copy(v.ls_cell, v.n_ls)
+            self.ls_face = _copy(v.ls_face, v.n_ls)
+            self.ls_weight = _copy(v.ls_weight, 3 * v.n_ls, (v.n_ls, 3))
+            self.vertex_coords = _copy(v.vertex_coords, 3 * v.n_vertices, (v.n_vertices, 3))
+        finally:
+            L.dm_mesh_free(h)
+        self.patch_names = list(elements.get("group_names", []))
+        # group index of each patch (patches are the non-empty groups in group order)
+        self.bounds_min = self.vertex_coords.min(axis=0)
+        self.bounds_max = self.vertex_coords.max(axis=0)
+
+    def patch_of_name(self, name):
+        return self.patch_names.index(name)
+
+
+def box_elements(nx, ny, nz, node_xyz, group_names=("x0", "xL", "y0", "yL", "z0", "zL"),
+                 group_types=("periodic",) * 6):
+    """Element lists of a structured nx*ny*nz hex box.
+
+    Node, hex and boundary-quad order follow tests/regression/mk_box_mesh.py (node id
+    i + (nx+1)*(j + (ny+1)*k), hexes k-j-i nested, quads per surface x0,xL,y0,yL,z0,zL with the
+    first in-surface index outermost). ``node_xyz(I, J, K)`` maps integer node indices to
+    coordinates (arrays).
+    """
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    # node id order: i fastest, then j, then k
+    I, J, K = (a.transpose(2, 1, 0).reshape(-1) for a in (I, J, K))
+    X, Y, Z = node_xyz(I, J, K)
+    coords = np.stack([X, Y, Z], axis=1).reshape(-1).astype(np.float64)
+
+    def nid(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.reshape(-1), j.reshape(-1), k.reshape(-1)
+    hexes = np.stack([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                      nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)],
+                     axis=1).astype(np.int32)
+
+    def surf(n1, n2, fn):
+        a, b = np.meshgrid(np.arange(n1), np.arange(n2), indexing="ij")
+        a, b = a.reshape(-1), b.reshape(-1)
+        return np.stack(fn(a, b), axis=1)
+
+    quads = [
+        surf(ny, nz, lambda a, b: [nid(0, a, b), nid(0, a + 1, b), nid(0, a + 1, b + 1), nid(0, a, b + 1)]),
+        surf(ny, nz, lambda a, b: [nid(nx, a, b), nid(nx, a + 1, b), nid(nx, a + 1, b + 1), nid(nx, a, b + 1)]),
+        surf(nx, nz, lambda a, b: [nid(a, 0, b), nid(a + 1, 0, b), nid(a + 1, 0, b + 1), nid(a, 0, b + 1)]),
+        surf(nx, nz, lambda a, b: [nid(a, ny, b), nid(a + 1, ny, b), nid(a + 1, ny, b + 1), nid(a, ny, b + 1)]),
+        surf(nx, ny, lambda a, b: [nid(a, b, 0), nid(a + 1, b, 0), nid(a + 1, b + 1, 0), nid(a, b + 1, 0)]),
+        surf(nx, ny, lambda a, b: [nid(a, b, nz), nid(a + 1, b, nz), nid(a + 1, b + 1, nz), nid(a, b + 1, nz)]),
+    ]
+    bnd = np.concatenate(quads).astype(np.int32)
+    bnd_group = np.concatenate([np.full(len(q), g, dtype=np.int32) for g, q in enumerate(quads)])
+    nv, nq = len(hexes), len(bnd)
+    return {
+        "coords": coords,
+        "vol_type": np.full(nv, GMSH_HEX8, dtype=np.int32),
+        "vol_offset": (np.arange(nv + 1, dtype=np.int64) * 8),
+        "vol_nodes": hexes.reshape(-1),
+        "bnd_type": np.full(nq, GMSH_QUAD4, dtype=np.int32),
+        "bnd_offset": (np.arange(nq + 1, dtype=np.int64) * 4),
+        "bnd_nodes": bnd.reshape(-1),
+        "bnd_group": bnd_group,
+        "group_bc": np.array([BC_NAMES[t.lower()] for t in group_types], dtype=np.int32),
+        "group_names": list(group_names),
+    }
+
+
+def _round16g(a):
+    """Round-trip through '%.16g', as a Gmsh ASCII file written with that format stores it."""
+    return np.array([float(f"{x:.16g}") for x in np.asarray(a, dtype=np.float64).reshape(-1)]).reshape(np.shape(a))
+
+
+def skewed_box(n, amp, group_types=("periodic",) * 6, group_names=("x0", "xL", "y0", "yL", "z0", "zL")):
+    """Curvilinear unit box of tests/regression/mk_box_mesh.py (N^3 hexes, node displacement amp)."""
+    def xyz(I, J, K):
+        out = np.empty((3, len(I)))
+        for q, (i, j, k) in enumerate(zip(I.tolist(), J.tolist(), K.tolist())):
+            X, Y, Z = i / n, j / n, k / n
+            out[0, q] = X + amp * math.sin(math.pi * X) * math.sin(2 * math.pi * Y) * math.sin(math.pi * Z)
+            out[1, q] = Y + amp * math.sin(math.pi * Y) * math.sin(2 * math.pi * Z) * math.sin(math.pi * X)
+            out[2, q] = Z + amp * math.sin(math.pi * Z) * math.sin(2 * math.pi * X) * math.sin(math.pi * Y)
+        return _round16g(out[0]), _round16g(out[1]), _round16g(out[2])
+    return Mesh(box_elements(n, n, n, xyz, group_names, group_types))
+
+
+def uniform_box(nx, ny, nz, lx, ly, lz, group_types=("periodic",) * 6,
+                group_names=("x0", "xL", "y0", "yL", "z0", "zL")):
+    """Axis-aligned box [0,lx]x[0,ly]x[0,lz] with uniform spacing."""
+    def xyz(I, J, K):
+        return I * (lx / nx), J * (ly / ny), K * (lz / nz)
+    return Mesh(box_elements(nx, ny, nz, xyz, group_names, group_types))
+
+
+def channel_box(nx, ny, nz, lx=2 * math.pi, ly=2.0, lz=math.pi, stretch=2.0):
+    """Channel box with tanh wall-normal (y) stretching: y_j = ly/2 (1 + tanh(s(2j/ny-1))/tanh(s)).
+
+    Periodic in x and z, log-law wall-model walls at y = 0 and y = ly.
+    """
+    yn = 0.5 * ly * (1.0 + np.tanh(stretch * (2.0 * np.arange(ny + 1) / ny - 1.0)) / math.tanh(stretch))
+    yn[0], yn[-1] = 0.0, ly
+
+    def xyz(I, J, K):
+        return I * (lx / nx), yn[J], K * (lz / nz)
+    return Mesh(box_elements(nx, ny, nz, xyz, ("x0", "xL", "bottom_wall", "top_wall", "z0", "zL"),
+                             ("periodic", "periodic", "wallModel", "wallModel", "periodic", "periodic")))

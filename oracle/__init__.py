@@ -1,0 +1,177 @@
+"""ORACLE — test infrastructure only.
+
+Python binding of liboracle.so, the FP64 CPU restatement of the reference's RK3W hot path
+(dopamine_oracle.c). It is the parity checker for the HIP product path: only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import it. Nothing it does is ever
+used by dopamine_amd.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_P = ctypes.POINTER
+_d, _i32, _i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [("n_cells", _i32), ("n_faces", _i32), ("n_int_faces", _i32), ("n_bnd_faces", _i32),
+                ("n_patches", _i32)] + [(n, _P(t)) for n, t in [
+                    ("cell_centroid", _d), ("cell_volume", _d), ("cell_face_offset", _i32), ("cell_faces", _i32),
+                    ("face_owner", _i32), ("face_neighbour", _i32), ("face_patch", _i32),
+                    ("face_area", _d), ("face_centre", _d), ("face_normal", _d), ("face_sf", _d), ("face_dcf", _d),
+                    ("face_gc", _d), ("face_skew", _d), ("face_eps", _d),
+                    ("patch_bc", _i32), ("patch_start", _i32), ("patch_nfaces", _i32),
+                    ("periodic_pair", _i32), ("periodic_pair_dcf", _d), ("periodic_gc", _d),
+                    ("ls_offset", _i32), ("ls_cell", _i32), ("ls_face", _i32), ("ls_weight", _d)]]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("nu", _d), ("f_const", _d * 3), ("scheme", ctypes.c_int), ("gradient", ctypes.c_int),
+                ("gg_iterations", ctypes.c_int), ("sgs", ctypes.c_int), ("c_vreman", _d), ("cs_smag", _d),
+                ("wall_model", ctypes.c_int), ("wall_model_height", _d), ("n_nonortho_corr", ctypes.c_int),
+                ("min_cos_theta", _d), ("rhie_chow_dual", ctypes.c_int), ("nonortho_check", ctypes.c_int),
+                ("rotational", ctypes.c_int), ("p_tol", _d), ("p_max_iter", ctypes.c_int), ("cfl_max", _d),
+                ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.om_create.restype = ctypes.c_void_p
+        L.om_create.argtypes = [_P(_Mesh), _P(_Params)]
+        L.om_free.argtypes = [ctypes.c_void_p]
+        L.om_array.restype = _P(_d)
+        L.om_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _P(_i64)]
+        L.om_apply_all_bc.argtypes = [ctypes.c_void_p]
+        L.om_project_once.argtypes = [ctypes.c_void_p, _d]
+        L.om_advance.argtypes = [ctypes.c_void_p, _d, _P(_d)]
+        L.om_stats.argtypes = [ctypes.c_void_p, _d * 8]
+        L.om_time.restype = _d
+        L.om_time.argtypes = [ctypes.c_void_p]
+        L.om_counters.argtypes = [ctypes.c_void_p, _i64 * 3]
+        L.om_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int * 3]
+        L.om_cg_jacobi.argtypes = [ctypes.c_int, ctypes.c_int, _P(_i32), _P(_i32), _P(_d), _P(_d), _P(_d), _P(_d),
+                                   _d, _d, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(_d)]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(_P(t))
+
+
+STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
+
+
+class Solver:
+    """Oracle solver on a dopamine_amd.mesh.Mesh with dopamine_amd.params.SolverParams."""
+
+    def __init__(self, mesh, params):
+        params.validate()
+        L = lib()
+        self.mesh = mesh
+        self._keep = []
+
+        def arr(a, t):
+            a = np.ascontiguousarray(a)
+            self._keep.append(a)
+            return _p(a, t)
+
+        m = mesh
+        self._m = _Mesh(m.n_cells, m.n_faces, m.n_int_faces, m.n_bnd_faces, m.n_patches,
+                        arr(m.cell_centroid, _d), arr(m.cell_volume, _d), arr(m.cell_face_offset, _i32),
+                        arr(m.cell_faces, _i32), arr(m.face_owner, _i32), arr(m.face_neighbour, _i32),
+                        arr(m.face_patch, _i32), arr(m.face_area, _d), arr(m.face_centre, _d),
+                        arr(m.face_normal, _d), arr(m.face_sf, _d), arr(m.face_dcf, _d), arr(m.face_gc, _d),
+                        arr(m.face_skew, _d), arr(m.face_eps, _d), arr(m.patch_bc, _i32), arr(m.patch_start, _i32),
+                        arr(m.patch_nfaces, _i32), arr(m.periodic_pair, _i32), arr(m.periodic_pair_dcf, _d),
+                        arr(m.periodic_gc, _d), arr(m.ls_offset, _i32), arr(m.ls_cell, _i32), arr(m.ls_face, _i32),
+                        arr(m.ls_weight, _d))
+        pv, uv = params.patch_arrays(mesh)
+        from dopamine_amd.params import WALL_MODEL_IDS
+        self._pr = _Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
+                           0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
+                           params.sgs_id(), params.c_vreman, params.cs_smagorinsky,
+                           WALL_MODEL_IDS[params.wall_model], params.wall_model_height, params.n_nonortho_corr,
+                           params.pressure_min_cos_theta, int(params.rhie_chow_dual), int(params.nonortho_check),
+                           int(params.use_rotational_projection), params.p_tol, params.p_max_iter, params.cfl_max,
+                           params.dt, arr(pv, _d), arr(uv.reshape(-1), _d))
+        self.params = params
+        self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().om_free(self.h)
+            self.h = None
+
+    def array(self, name):
+        n = _i64(0)
+        ptr = lib().om_array(self.h, name.encode(), ctypes.byref(n))
+        if not ptr:
+            raise KeyError(name)
+        return np.ctypeslib.as_array(ptr, shape=(n.value,))
+
+    @property
+    def U(self):
+        return self.array("U").reshape(-1, 3)
+
+    @property
+    def p(self):
+        return self.array("p")
+
+    def set_velocity(self, U):
+        self.array("U")[:] = np.asarray(U, dtype=np.float64).reshape(-1)
+        lib().om_apply_all_bc(self.h)
+
+    def project_once(self, dt):
+        return lib().om_project_once(self.h, dt)
+
+    def advance(self, dt=None):
+        used = _d(0.0)
+        rc = lib().om_advance(self.h, self.params.dt if dt is None else dt, ctypes.byref(used))
+        if rc != 0:
+            raise RuntimeError(f"oracle advance failed ({rc})")
+        return used.value
+
+    @property
+    def time(self):
+        return lib().om_time(self.h)
+
+    def stats(self):
+        out = (_d * 8)()
+        lib().om_stats(self.h, out)
+        return dict(zip(STATS_COLUMNS, list(out)))
+
+    def counters(self):
+        out = (_i64 * 3)()
+        lib().om_counters(self.h, out)
+        return {"solves": out[0], "iterations": out[1], "last": out[2]}
+
+    def nonortho_info(self):
+        out = (ctypes.c_int * 3)()
+        lib().om_nonortho_info(self.h, out)
+        return {"skip_nonortho": out[0], "skip_grad_p": out[1], "n_nonortho_corr": out[2]}
+
+
+def cg_jacobi(owner, neighbour, coeff, diag, b, x0, rtol, atol, max_it, nullspace):
+    """Standalone Jacobi CG with the reference operator convention (0-based faces)."""
+    x = np.array(x0, dtype=np.float64, copy=True)
+    its = ctypes.c_int(0)
+    res = _d(0.0)
+    own = np.ascontiguousarray(owner, dtype=np.int32)
+    nbr = np.ascontiguousarray(neighbour, dtype=np.int32)
+    co = np.ascontiguousarray(coeff, dtype=np.float64)
+    dg = np.ascontiguousarray(diag, dtype=np.float64)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    rc = lib().om_cg_jacobi(len(bb), len(own), _p(own, _i32), _p(nbr, _i32), _p(co, _d), _p(dg, _d), _p(bb, _d),
+                            _p(x, _d), rtol, atol, max_it, int(nullspace), ctypes.byref(its), ctypes.byref(res))
+    return x, its.value, res.value, rc

@@ -1,0 +1,1655 @@
+/* ORACLE — test infrastructure only (see dopamine_oracle.h).
+ *
+ * FP64 CPU restatement of one DOPAMINE RK3W step and its pressure projection, restated from the
+ * reference Fortran as sequential face-scatter loops. Every routine cites what it follows.
+ * parity pins: tests/test_oracle_regression.py (reference regression CSVs), tests/test_oracle_units.py
+ * (closed forms and reference unit-test known answers).
+ */
+#include "dopamine_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SMALL 1.0e-30
+#define NONORTHO_SIN2_TOL 1.0e-10
+#define SKEW_BLEND_LO 0.15
+#define SKEW_BLEND_HI 0.25
+#define REFINE_SKEW_TOL 0.15
+#define KAPPA 0.41
+#define BLOG 5.2
+#define NU_SGS_STAB_FRAC 0.4
+
+enum {
+  BC_INLET = 1, BC_OUTLET = 2, BC_WALL_NOSLIP = 3, BC_WALL_MODEL = 4, BC_SLIP = 5, BC_PERIODIC = 6,
+  BC_SYMMETRY = 7, BC_FIXEDTEMP = 8, BC_ZEROGRADIENT = 9, BC_WALL_VELOCITY = 10, BC_ABL_INLET = 11
+};
+
+/* Wray RK3 coefficients (src/mod_parameters.f90:64-71). */
+static const double RK3_A[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double RK3_B[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+static const double RK3_G[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+static const double RK3_C[3] = {0.0, 8.0 / 15.0, 2.0 / 3.0};
+
+/* grad(i,j,c) = d u_j / d x_i, column-major like the reference grad_U(3,3,nc). */
+#define GI(c, i, j) (9 * (size_t)(c) + 3 * (j) + (i))
+
+struct om_solver {
+  om_mesh m;
+  om_params p;
+  int nc, nf, nint, nb, npatch;
+  double *U, *P, *p_old, *nu_sgs, *U_bnd, *p_bnd, *phi_prev, *phi_prev_bnd, *tau_w;
+  double *grad_U, *gU_prev, *face_flux, *U_face, *inv_vol;
+  double *H, *H0, *conv, *visc, *rhs_p, *div_w, *phi_it;
+  double *gp, *gp_rc, *gp_cell, *p_tot, *p_tot_bnd, *gp_prev;
+  double *Tinv, *outlet_pb, *Gf, *ws_y, *div_diag;
+  int *per_nb, *per_bf, *nper_bf, n_per_bf, n_nper_bf, n_wm_patches, n_outlet;
+  int *ws_cell;
+  char *built;
+  /* pressure operator (src/solver/mod_pressure_solver.f90:76-96) */
+  int npf, n_dir;
+  int *ps_owner, *ps_nb, *ps_fmesh, *ps_skip, *dir_rows;
+  double *ps_coeff, *ps_diag, *dir_vals;
+  double *k_int, *k_bnd;
+  char *act_int, *act_bnd;
+  int skip_nonortho, skip_grad_p, n_corr, diag_interface;
+  double V_min, V_total, p_atol_factor, time;
+  int u_face_ready;
+  int64_t n_solves, n_iters, last_iters;
+  double *cg_r, *cg_z, *cg_p, *cg_q, *cg_b;
+  double *dt_flux, *dt_diff;
+};
+
+static double *dz(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+static int *iz(size_t n) { return (int *)calloc(n ? n : 1, sizeof(int)); }
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static int bc_of_face(const om_solver *s, int f) {
+  const int pid = s->m.face_patch[f];
+  if (pid >= 0 && pid < s->npatch) return s->m.patch_bc[pid];
+  return BC_ZEROGRADIENT;
+}
+
+/* refine_blend_weight (src/mesh/mod_mesh.f90:491-500) */
+static double refine_blend_weight(double skew) {
+  double t = (skew - SKEW_BLEND_LO) / (SKEW_BLEND_HI - SKEW_BLEND_LO);
+  t = fmax(0.0, fmin(1.0, t));
+  return 1.0 - t * t * (3.0 - 2.0 * t);
+}
+
+/* nonortho_kfactor_limited (src/numerics/mod_nonortho.f90:65-98) */
+static void kfactor_limited(const double sf[3], const double d_on[3], double min_cos, double k[3]) {
+  const double d_on_len = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]);
+  if (d_on_len < 1.0e-30) {
+    k[0] = k[1] = k[2] = 0.0;
+    return;
+  }
+  const double inv_len = 1.0 / d_on_len;
+  const double s_dot_dhat = dot3(sf, d_on) * inv_len;
+  double d_len = s_dot_dhat * d_on_len;
+  const double sf_sq = sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2];
+  if (min_cos > 0.0) d_len = fmax(d_len, min_cos * sqrt(sf_sq) * d_on_len);
+  if (d_len > 1.0e-30) {
+    for (int i = 0; i < 3; ++i) k[i] = sf[i] - (sf_sq / d_len) * d_on[i];
+  } else {
+    for (int i = 0; i < 3; ++i) k[i] = sf[i] - (s_dot_dhat * inv_len) * d_on[i];
+  }
+}
+
+/* nonortho_cross_flux (src/numerics/mod_nonortho.f90:120-128) */
+static double cross_flux(const double *go, const double *gn, double gc, const double *k) {
+  double v[3];
+  for (int i = 0; i < 3; ++i) v[i] = gc * go[i] + (1.0 - gc) * gn[i];
+  return dot3(v, k);
+}
+
+/* resolve_periodic_nb (src/bc/mod_bc.f90:822-859), single rank: partner through the pair. */
+static int resolve_periodic_nb(const om_solver *s, int b) {
+  const int o = s->m.face_owner[s->nint + b];
+  const int pair = s->m.periodic_pair[b];
+  if (pair >= 0 && pair < s->nb) {
+    const int nb = s->m.face_owner[s->nint + pair];
+    if (nb >= 0 && nb != o) return nb;
+  }
+  return -1;
+}
+
+/* ---------------------------------------------------------------- boundary conditions */
+
+/* apply_velocity_bc (src/bc/mod_bc.f90:196-379). filter: 0 all, 1 non-periodic, 2 periodic only. */
+static void apply_velocity_bc(om_solver *s, int filter, const double *gU) {
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b;
+    const int bc = bc_of_face(s, f);
+    if (filter == 2 && bc != BC_PERIODIC) continue;
+    if (filter == 1 && bc == BC_PERIODIC) continue;
+    const int o = s->m.face_owner[f], pid = s->m.face_patch[f];
+    const double *n = &s->m.face_normal[3 * f];
+    double *ub = &s->U_bnd[3 * b];
+    const double *uo = &s->U[3 * o];
+    switch (bc) {
+      case BC_INLET:
+        for (int k = 0; k < 3; ++k) ub[k] = s->p.patch_U_val[3 * pid + k];
+        break;
+      case BC_OUTLET:
+      case BC_ZEROGRADIENT: {
+        double u[3] = {uo[0], uo[1], uo[2]};
+        if (gU) {
+          const double d[3] = {s->m.face_centre[3 * f] - s->m.cell_centroid[3 * o],
+                               s->m.face_centre[3 * f + 1] - s->m.cell_centroid[3 * o + 1],
+                               s->m.face_centre[3 * f + 2] - s->m.cell_centroid[3 * o + 2]};
+          for (int k = 0; k < 3; ++k)
+            u[k] = u[k] + gU[GI(o, 0, k)] * d[0] + gU[GI(o, 1, k)] * d[1] + gU[GI(o, 2, k)] * d[2];
+        }
+        for (int k = 0; k < 3; ++k) ub[k] = u[k];
+        break;
+      }
+      case BC_WALL_NOSLIP:
+      case BC_WALL_MODEL:
+        ub[0] = ub[1] = ub[2] = 0.0;
+        break;
+      case BC_WALL_VELOCITY: {
+        double u[3];
+        for (int k = 0; k < 3; ++k) u[k] = s->p.patch_U_val[3 * pid + k];
+        const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+        for (int k = 0; k < 3; ++k) ub[k] = u[k] - un * n[k];
+        break;
+      }
+      case BC_PERIODIC: {
+        double u[3] = {uo[0], uo[1], uo[2]};
+        const int nb = s->per_nb[b];
+        if (nb >= 0) {
+          const double w = s->m.periodic_gc[b];
+          for (int k = 0; k < 3; ++k) u[k] = w * u[k] + (1.0 - w) * s->U[3 * nb + k];
+        }
+        for (int k = 0; k < 3; ++k) ub[k] = u[k];
+        break;
+      }
+      case BC_SLIP:
+      case BC_SYMMETRY: {
+        const double un = uo[0] * n[0] + uo[1] * n[1] + uo[2] * n[2];
+        for (int k = 0; k < 3; ++k) ub[k] = uo[k] - un * n[k];
+        break;
+      }
+      default:
+        break;
+    }
+  }
+}
+
+/* apply_pressure_bc (src/bc/mod_bc.f90:383-486). */
+static void apply_pressure_bc(om_solver *s, const double *p, double *pb, int filter, const double *gp,
+                              double outlet_scale, const double *pob) {
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b;
+    const int bc = bc_of_face(s, f);
+    if (filter == 2 && bc != BC_PERIODIC) continue;
+    if (filter == 1 && bc == BC_PERIODIC) continue;
+    const int o = s->m.face_owner[f], pid = s->m.face_patch[f];
+    switch (bc) {
+      case BC_OUTLET:
+        pb[b] = outlet_scale * (pob ? pob[b] : s->p.patch_p_val[pid]);
+        break;
+      case BC_PERIODIC: {
+        const int nb = s->per_nb[b];
+        pb[b] = (nb >= 0) ? s->m.periodic_gc[b] * p[o] + (1.0 - s->m.periodic_gc[b]) * p[nb] : p[o];
+        break;
+      }
+      case BC_WALL_NOSLIP:
+      case BC_WALL_VELOCITY:
+      case BC_WALL_MODEL:
+      case BC_SLIP:
+      case BC_SYMMETRY: {
+        /* Kim-Moin Neumann with a_wall = 0 (BCConfig default) */
+        const double *n = &s->m.face_normal[3 * f];
+        const double an = 0.0 * n[0] + 0.0 * n[1] + 0.0 * n[2];
+        double d[3];
+        for (int i = 0; i < 3; ++i) d[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+        const double d_ortho = fabs(d[0] * n[0] + d[1] * n[1] + d[2] * n[2]);
+        pb[b] = p[o] - an * d_ortho;
+        break;
+      }
+      default:
+        if (gp) {
+          double d[3];
+          for (int i = 0; i < 3; ++i) d[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+          pb[b] = p[o] + gp[3 * o] * d[0] + gp[3 * o + 1] * d[1] + gp[3 * o + 2] * d[2];
+        } else {
+          pb[b] = p[o];
+        }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- gradients */
+
+/* ls_gradient_vector (src/numerics/mod_gradient.f90:425-556, CPU branch) */
+static void ls_gradient_vector(const om_solver *s, const double *U, const double *Ub, double *g) {
+  memset(g, 0, sizeof(double) * 9 * (size_t)s->nc);
+  for (int c = 0; c < s->nc; ++c) {
+    for (int k = s->m.ls_offset[c]; k < s->m.ls_offset[c + 1]; ++k) {
+      const int nb = s->m.ls_cell[k];
+      double du[3];
+      if (nb < 0) {
+        const int f = s->m.ls_face[k];
+        const int bi = (f >= 0) ? f - s->nint : -1;
+        const double *un = (bi >= 0 && bi < s->nb) ? &Ub[3 * bi] : &U[3 * c];
+        for (int j = 0; j < 3; ++j) du[j] = un[j] - U[3 * c + j];
+      } else {
+        for (int j = 0; j < 3; ++j) du[j] = U[3 * nb + j] - U[3 * c + j];
+      }
+      const double *w = &s->m.ls_weight[3 * (size_t)k];
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) g[GI(c, i, j)] = g[GI(c, i, j)] + w[i] * du[j];
+    }
+  }
+}
+
+/* ls_gradient_scalar (src/numerics/mod_gradient.f90:307-422, CPU branch) */
+static void ls_gradient_scalar(const om_solver *s, const double *phi, const double *pb, double *g) {
+  memset(g, 0, sizeof(double) * 3 * (size_t)s->nc);
+  for (int c = 0; c < s->nc; ++c) {
+    for (int k = s->m.ls_offset[c]; k < s->m.ls_offset[c + 1]; ++k) {
+      const int nb = s->m.ls_cell[k];
+      double pn;
+      if (nb < 0) {
+        const int f = s->m.ls_face[k];
+        const int bi = (f >= 0) ? f - s->nint : -1;
+        pn = (bi >= 0 && bi < s->nb) ? pb[bi] : phi[c];
+      } else {
+        pn = phi[nb];
+      }
+      const double d = pn - phi[c];
+      const double *w = &s->m.ls_weight[3 * (size_t)k];
+      for (int i = 0; i < 3; ++i) g[3 * c + i] = g[3 * c + i] + w[i] * d;
+    }
+  }
+}
+
+/* green_gauss_gradient_vector (src/numerics/mod_gradient.f90:172-304) */
+static void gg_gradient_vector(const om_solver *s, const double *U, const double *Ub, const double *gprev, double *g) {
+  memset(g, 0, sizeof(double) * 9 * (size_t)s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const double *sf = &s->m.face_sf[3 * f];
+    double Uf[3];
+    if (nb >= 0) {
+      const double gc = s->m.face_gc[f];
+      if (gprev) {
+        const double *eps = &s->m.face_eps[3 * f];
+        for (int k = 0; k < 3; ++k) {
+          double gm[3];
+          for (int i = 0; i < 3; ++i) gm[i] = gc * gprev[GI(o, i, k)] + (1.0 - gc) * gprev[GI(nb, i, k)];
+          Uf[k] = gc * U[3 * o + k] + (1.0 - gc) * U[3 * nb + k] + dot3(gm, eps);
+        }
+        const double w = refine_blend_weight(s->m.face_skew[f]);
+        if (w < 1.0)
+          for (int k = 0; k < 3; ++k) {
+            const double pl = gc * U[3 * o + k] + (1.0 - gc) * U[3 * nb + k];
+            Uf[k] = pl + w * (Uf[k] - pl);
+          }
+      } else {
+        for (int k = 0; k < 3; ++k) Uf[k] = gc * U[3 * o + k] + (1.0 - gc) * U[3 * nb + k];
+      }
+    } else {
+      for (int k = 0; k < 3; ++k) Uf[k] = Ub[3 * (f - s->nint) + k];
+    }
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 3; ++i) {
+        g[GI(o, i, k)] = g[GI(o, i, k)] + Uf[k] * sf[i];
+        if (nb >= 0) g[GI(nb, i, k)] = g[GI(nb, i, k)] - Uf[k] * sf[i];
+      }
+  }
+  for (int c = 0; c < s->nc; ++c)
+    if (s->m.cell_volume[c] > SMALL) {
+      const double iv = 1.0 / s->m.cell_volume[c];
+      for (int q = 0; q < 9; ++q) g[9 * (size_t)c + q] = g[9 * (size_t)c + q] * iv;
+    }
+}
+
+/* green_gauss_gradient_scalar (src/numerics/mod_gradient.f90:38-168) */
+static void gg_gradient_scalar(const om_solver *s, const double *phi, const double *pb, const double *gprev, double *g) {
+  memset(g, 0, sizeof(double) * 3 * (size_t)s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const double *sf = &s->m.face_sf[3 * f];
+    double pf;
+    if (nb >= 0) {
+      const double gc = s->m.face_gc[f];
+      if (gprev) {
+        double gm[3];
+        for (int i = 0; i < 3; ++i) gm[i] = gc * gprev[3 * o + i] + (1.0 - gc) * gprev[3 * nb + i];
+        const double pl = gc * phi[o] + (1.0 - gc) * phi[nb];
+        pf = pl + dot3(gm, &s->m.face_eps[3 * f]);
+        const double w = refine_blend_weight(s->m.face_skew[f]);
+        if (w < 1.0) pf = pl + w * (pf - pl);
+      } else {
+        pf = gc * phi[o] + (1.0 - gc) * phi[nb];
+      }
+      for (int i = 0; i < 3; ++i) {
+        g[3 * o + i] = g[3 * o + i] + pf * sf[i];
+        g[3 * nb + i] = g[3 * nb + i] - pf * sf[i];
+      }
+    } else {
+      pf = pb[f - s->nint];
+      for (int i = 0; i < 3; ++i) g[3 * o + i] = g[3 * o + i] + pf * sf[i];
+    }
+  }
+  for (int c = 0; c < s->nc; ++c)
+    if (s->m.cell_volume[c] > SMALL) {
+      const double iv = 1.0 / s->m.cell_volume[c];
+      for (int i = 0; i < 3; ++i) g[3 * c + i] = g[3 * c + i] * iv;
+    }
+}
+
+/* compute_all_gradients (src/solver/mod_fv_solver.f90:1003-1077), velocity only */
+static void compute_all_gradients(om_solver *s) {
+  if (s->p.gradient == OM_GRAD_GG) {
+    gg_gradient_vector(s, s->U, s->U_bnd, NULL, s->grad_U);
+    for (int it = 2; it <= s->p.gg_iterations; ++it) {
+      memcpy(s->gU_prev, s->grad_U, sizeof(double) * 9 * (size_t)s->nc);
+      gg_gradient_vector(s, s->U, s->U_bnd, s->gU_prev, s->grad_U);
+    }
+  } else {
+    ls_gradient_vector(s, s->U, s->U_bnd, s->grad_U);
+  }
+}
+
+/* gradient_of_scalar (src/solver/mod_fv_solver.f90:2341-2369) */
+static void gradient_of_scalar(om_solver *s, const double *fld, const double *fb, double *g) {
+  if (s->p.gradient == OM_GRAD_GG) {
+    gg_gradient_scalar(s, fld, fb, NULL, g);
+    for (int it = 2; it <= s->p.gg_iterations; ++it) {
+      memcpy(s->gp_prev, g, sizeof(double) * 3 * (size_t)s->nc);
+      gg_gradient_scalar(s, fld, fb, s->gp_prev, g);
+    }
+  } else {
+    ls_gradient_scalar(s, fld, fb, g);
+  }
+}
+
+static void compute_pressure_gradient(om_solver *s) {
+  if (s->skip_grad_p) return;
+  gradient_of_scalar(s, s->P, s->p_bnd, s->gp);
+}
+
+/* refresh_halos_and_bc + apply_all_bc_subset (src/solver/mod_fv_solver.f90:985-1000, 2025-2060):
+ * single rank, so the non-periodic and periodic passes run back to back. */
+static void refresh_bc(om_solver *s) {
+  for (int pass = 1; pass <= 2; ++pass) {
+    apply_velocity_bc(s, pass, s->grad_U);
+    apply_pressure_bc(s, s->P, s->p_bnd, pass, s->skip_grad_p ? NULL : s->gp, 1.0, s->outlet_pb);
+  }
+}
+
+static void velocity_bc_both(om_solver *s) {
+  apply_velocity_bc(s, 1, s->grad_U);
+  apply_velocity_bc(s, 2, NULL);
+}
+
+/* ---------------------------------------------------------------- SGS and wall model */
+
+static double det3(const double *A) {
+  /* A column-major 3x3: A(i,j) = A[3j+i] (det3x3, src/sgs/mod_vreman.f90:38-44) */
+#define A_(i, j) A[3 * ((j)-1) + ((i)-1)]
+  return A_(1, 1) * (A_(2, 2) * A_(3, 3) - A_(2, 3) * A_(3, 2)) - A_(1, 2) * (A_(2, 1) * A_(3, 3) - A_(2, 3) * A_(3, 1)) +
+         A_(1, 3) * (A_(2, 1) * A_(3, 2) - A_(2, 2) * A_(3, 1));
+#undef A_
+}
+
+/* build_filter_tensor + smooth_filter_tensor (src/sgs/mod_vreman.f90:168-264) */
+static void build_filter_tensor(om_solver *s) {
+  const int nc = s->nc;
+  for (int c = 0; c < nc; ++c) {
+    const double V = s->m.cell_volume[c];
+    const double d2_iso = pow(V, 2.0 / 3.0);
+    double g[9] = {0};
+    for (int q = s->m.cell_face_offset[c]; q < s->m.cell_face_offset[c + 1]; ++q) {
+      const int f = s->m.cell_faces[q];
+      double dx[3];
+      for (int i = 0; i < 3; ++i) dx[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * c + i];
+      const double a = s->m.face_area[f];
+      for (int n = 0; n < 3; ++n)
+        for (int mm = 0; mm < 3; ++mm) g[3 * n + mm] = g[3 * n + mm] + a * dx[mm] * dx[n];
+    }
+    const double det = det3(g);
+    const double t = (g[0] + g[4] + g[8]) / 3.0;
+    const double tr3 = t * t * t;
+    double *G = &s->Gf[9 * (size_t)c];
+    if (tr3 > 0.0 && det > 1.0e-12 * tr3) {
+      const double sc = pow(V * V / det, 1.0 / 3.0);
+      for (int q = 0; q < 9; ++q) G[q] = sc * g[q];
+    } else {
+      for (int q = 0; q < 9; ++q) G[q] = 0.0;
+      G[0] = G[4] = G[8] = d2_iso;
+    }
+  }
+  double *gnb = dz(9 * (size_t)nc), *ws = dz(nc);
+  for (int pass = 0; pass < 2; ++pass) {
+    memset(gnb, 0, sizeof(double) * 9 * (size_t)nc);
+    memset(ws, 0, sizeof(double) * nc);
+    for (int f = 0; f < s->nf; ++f) {
+      const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+      if (nb < 0) continue;
+      const double a = s->m.face_area[f];
+      for (int q = 0; q < 9; ++q) gnb[9 * (size_t)o + q] = gnb[9 * (size_t)o + q] + a * s->Gf[9 * (size_t)nb + q];
+      ws[o] = ws[o] + a;
+      for (int q = 0; q < 9; ++q) gnb[9 * (size_t)nb + q] = gnb[9 * (size_t)nb + q] + a * s->Gf[9 * (size_t)o + q];
+      ws[nb] = ws[nb] + a;
+    }
+    for (int c = 0; c < nc; ++c)
+      if (ws[c] > SMALL)
+        for (int q = 0; q < 9; ++q)
+          s->Gf[9 * (size_t)c + q] = (1.0 - 0.5) * s->Gf[9 * (size_t)c + q] + 0.5 * gnb[9 * (size_t)c + q] / ws[c];
+  }
+  free(gnb);
+  free(ws);
+}
+
+/* vreman_sgs_viscosity (src/sgs/mod_vreman.f90:50-153) */
+static void vreman(om_solver *s, double dt_stab) {
+  for (int c = 0; c < s->nc; ++c) {
+    const double *al = &s->grad_U[9 * (size_t)c]; /* al[3j+i] = alpha(i,j) */
+    const double *G = &s->Gf[9 * (size_t)c];      /* G[3n+m] = G(m,n) */
+    double beta[9] = {0};
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i <= j; ++i) {
+        double b = 0.0;
+        for (int n = 0; n < 3; ++n)
+          for (int mm = 0; mm < 3; ++mm) b = b + G[3 * n + mm] * al[3 * i + mm] * al[3 * j + n];
+        beta[3 * j + i] = b;
+      }
+    double asq = 0.0;
+    /* alpha(1,1)**2 + alpha(1,2)**2 + alpha(1,3)**2 + alpha(2,1)**2 + ... (row-wise) */
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) asq = asq + al[3 * j + i] * al[3 * j + i];
+#define B_(i, j) beta[3 * ((j)-1) + ((i)-1)]
+    const double Bb = B_(1, 1) * B_(2, 2) - B_(1, 2) * B_(1, 2) + B_(1, 1) * B_(3, 3) - B_(1, 3) * B_(1, 3) +
+                      B_(2, 2) * B_(3, 3) - B_(2, 3) * B_(2, 3);
+#undef B_
+    double nu = (asq > SMALL && Bb > 0.0) ? s->p.c_vreman * sqrt(Bb / asq) : 0.0;
+    const double cap = NU_SGS_STAB_FRAC * pow(s->m.cell_volume[c], 2.0 / 3.0) / fmax(dt_stab, SMALL);
+    s->nu_sgs[c] = fmin(nu, cap);
+  }
+}
+
+/* smagorinsky_sgs (src/sgs/mod_vreman.f90:295-364) */
+static void smagorinsky(om_solver *s, double dt_stab) {
+  for (int c = 0; c < s->nc; ++c) {
+    const double *g = &s->grad_U[9 * (size_t)c];
+    double S[9], ss = 0.0;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) S[3 * j + i] = 0.5 * (g[3 * j + i] + g[3 * i + j]);
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) ss = ss + S[3 * j + i] * S[3 * j + i];
+    const double Sn = sqrt(2.0 * ss);
+    const double delta = fmax(pow(fmax(det3(&s->Gf[9 * (size_t)c]), 0.0), 1.0 / 6.0), SMALL);
+    const double csd = s->p.cs_smag * delta;
+    double nu = csd * csd * Sn;
+    const double cap = NU_SGS_STAB_FRAC * delta * delta / fmax(dt_stab, SMALL);
+    s->nu_sgs[c] = fmin(nu, cap);
+  }
+}
+
+/* reichardt_uplus / wall_stress_newton (src/sgs/mod_wall_model.f90:50-115) */
+static void reichardt(double yp, double cbuf, double *up, double *dup) {
+  const double A1 = 11.0, A2 = 0.33, A1I = 1.0 / 11.0;
+  const double e1 = exp(-yp * A1I), e2 = exp(-A2 * yp);
+  *up = log(1.0 + KAPPA * yp) / KAPPA + cbuf * (1.0 - e1 - yp * A1I * e2);
+  *dup = 1.0 / (1.0 + KAPPA * yp) + cbuf * (A1I * e1 - A1I * e2 + yp * A1I * A2 * e2);
+  (void)A1;
+}
+
+static double wall_tau(double u_par, double y_w, double nu) {
+  const double cbuf = BLOG - log(KAPPA) / KAPPA;
+  if (u_par <= SMALL || y_w <= SMALL) return 0.0;
+  double ut = fmax(0.01 * u_par, sqrt(nu * u_par / (y_w + SMALL)));
+  const double inv_nu = 1.0 / (nu + SMALL), ywn = y_w * inv_nu;
+  for (int it = 0; it < 50; ++it) {
+    double up, dup;
+    reichardt(ut * ywn, cbuf, &up, &dup);
+    const double F = u_par / (ut + SMALL) - up;
+    const double dF = -u_par / (ut * ut + SMALL) - dup * ywn;
+    double un = ut - F / (dF + copysign(SMALL, dF));
+    un = fmin(fmax(un, 0.1 * ut), 10.0 * ut);
+    if (fabs(un - ut) < 1.0e-8 * un) {
+      ut = un;
+      break;
+    }
+    ut = un;
+  }
+  return ut * ut;
+}
+
+/* apply_wall_model_bc without time filtering (src/sgs/mod_wall_model.f90:176-313) */
+static void wall_model(om_solver *s) {
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b;
+    if (bc_of_face(s, f) != BC_WALL_MODEL) continue;
+    const double *n = &s->m.face_normal[3 * f];
+    const int c = s->ws_cell[b];
+    const double *u = &s->U[3 * c];
+    const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    const double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    const double mag = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    const double tau = wall_tau(mag, s->ws_y[b], s->p.nu);
+    const double sc = -tau / (mag + SMALL);
+    for (int k = 0; k < 3; ++k) s->tau_w[3 * b + k] = sc * up[k];
+  }
+}
+
+/* ---------------------------------------------------------------- explicit terms */
+
+static int periodic_partner_soa(const om_solver *s, int b, int o) {
+  /* nb_per resolution of the SoA kernels (mod_advection.f90:1108-1124): periodic_neighbor is
+   * unset on one rank, so the partner comes from the pair's owner. */
+  int nb_per = -1;
+  const int pair = s->m.periodic_pair[b];
+  if (pair >= 0) nb_per = s->m.face_owner[s->nint + pair];
+  if (nb_per == o) nb_per = -1;
+  return nb_per;
+}
+
+/* advection_rhs_vector_kep_soa (src/numerics/mod_advection.f90:1013-1197) */
+static void advection_kep(om_solver *s, double *rhs) {
+  const int nc = s->nc;
+  memset(rhs, 0, sizeof(double) * 3 * (size_t)nc);
+  memset(s->div_w, 0, sizeof(double) * nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const double *sf = &s->m.face_sf[3 * f];
+    int nb_per = -1, bc = -1;
+    double Uf[3], Fn;
+    if (nb >= 0) {
+      const double gc = s->m.face_gc[f];
+      for (int k = 0; k < 3; ++k) Uf[k] = gc * s->U[3 * o + k] + (1.0 - gc) * s->U[3 * nb + k];
+      Fn = s->U_face[f];
+    } else {
+      const int b = f - s->nint;
+      bc = bc_of_face(s, f);
+      if (bc == BC_PERIODIC) {
+        nb_per = periodic_partner_soa(s, b, o);
+        if (nb_per >= 0) {
+          const double w = s->m.periodic_gc[b];
+          for (int k = 0; k < 3; ++k) Uf[k] = w * s->U[3 * o + k] + (1.0 - w) * s->U[3 * nb_per + k];
+        } else {
+          for (int k = 0; k < 3; ++k) Uf[k] = s->U_bnd[3 * b + k];
+        }
+      } else {
+        for (int k = 0; k < 3; ++k) Uf[k] = s->U_bnd[3 * b + k];
+      }
+      if (bc == BC_PERIODIC || bc == BC_OUTLET) {
+        Fn = s->U_face[f];
+      } else if (bc == BC_WALL_NOSLIP || bc == BC_WALL_MODEL || bc == BC_WALL_VELOCITY || bc == BC_SLIP ||
+                 bc == BC_SYMMETRY) {
+        Fn = 0.0;
+      } else {
+        Fn = dot3(Uf, sf);
+      }
+    }
+    s->div_w[o] = s->div_w[o] + Fn;
+    if (nb >= 0) s->div_w[nb] = s->div_w[nb] - Fn;
+    for (int k = 0; k < 3; ++k) {
+      double phi;
+      if (nb >= 0)
+        phi = 0.5 * (s->U[3 * o + k] + s->U[3 * nb + k]);
+      else if (nb_per >= 0)
+        phi = 0.5 * (s->U[3 * o + k] + s->U[3 * nb_per + k]);
+      else
+        phi = Uf[k];
+      rhs[3 * o + k] = rhs[3 * o + k] - Fn * phi * s->inv_vol[o];
+      if (nb >= 0) rhs[3 * nb + k] = rhs[3 * nb + k] + Fn * phi * s->inv_vol[nb];
+    }
+  }
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 3; ++k) rhs[3 * c + k] = rhs[3 * c + k] + 0.5 * s->U[3 * c + k] * s->div_w[c] * s->inv_vol[c];
+}
+
+/* advection_rhs_vector_cds_soa (src/numerics/mod_advection.f90:1200-1392) */
+static void advection_cds(om_solver *s, double *rhs) {
+  memset(rhs, 0, sizeof(double) * 3 * (size_t)s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const double *sf = &s->m.face_sf[3 * f];
+    int b = 0, bc = -1, nb_per = -1;
+    double Fn;
+    if (nb >= 0) {
+      Fn = s->U_face[f];
+    } else {
+      b = f - s->nint;
+      bc = bc_of_face(s, f);
+      if (bc == BC_PERIODIC || bc == BC_OUTLET)
+        Fn = s->U_face[f];
+      else if (bc == BC_WALL_NOSLIP || bc == BC_WALL_MODEL || bc == BC_WALL_VELOCITY || bc == BC_SLIP ||
+               bc == BC_SYMMETRY)
+        Fn = 0.0;
+      else
+        Fn = s->U_bnd[3 * b] * sf[0] + s->U_bnd[3 * b + 1] * sf[1] + s->U_bnd[3 * b + 2] * sf[2];
+    }
+    int has_rec = (nb >= 0);
+    if (!has_rec && bc == BC_PERIODIC) {
+      nb_per = periodic_partner_soa(s, b, o);
+      has_rec = (nb_per >= 0);
+    }
+    const int nbs = has_rec ? (nb >= 0 ? nb : nb_per) : -1;
+    for (int k = 0; k < 3; ++k) {
+      double phi;
+      if (has_rec) {
+        const double gc = (f >= s->nint) ? s->m.periodic_gc[b] : s->m.face_gc[f];
+        const double pl = gc * s->U[3 * o + k] + (1.0 - gc) * s->U[3 * nbs + k];
+        if (f < s->nint) {
+          double gm[3];
+          for (int i = 0; i < 3; ++i) gm[i] = gc * s->grad_U[GI(o, i, k)] + (1.0 - gc) * s->grad_U[GI(nbs, i, k)];
+          phi = pl + dot3(gm, &s->m.face_eps[3 * f]);
+          const double w = refine_blend_weight(s->m.face_skew[f]);
+          if (w < 1.0) phi = pl + w * (phi - pl);
+        } else {
+          phi = pl;
+        }
+      } else {
+        phi = s->U_bnd[3 * b + k];
+      }
+      if (!isfinite(phi)) phi = 0.0;
+      rhs[3 * o + k] = rhs[3 * o + k] - Fn * phi * s->inv_vol[o];
+      if (nb >= 0) rhs[3 * nb + k] = rhs[3 * nb + k] + Fn * phi * s->inv_vol[nb];
+    }
+  }
+}
+
+static double ortho_weight(double s_dot_dhat, double area) {
+  return area * fmax(0.0, fmin(1.0, s_dot_dhat / (area + SMALL)));
+}
+
+/* viscous_rhs_vector (src/numerics/mod_viscous.f90:37-283) */
+static void viscous_vector(om_solver *s, double *rhs) {
+  memset(rhs, 0, sizeof(double) * 3 * (size_t)s->nc);
+  const double nu = s->p.nu;
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const double *sf = &s->m.face_sf[3 * f];
+    const double *n_hat = &s->m.face_normal[3 * f];
+    double st[3] = {0, 0, 0};
+    int have = 1;
+    if (nb >= 0) {
+      const double gc = s->m.face_gc[f];
+      const double nue = nu + gc * s->nu_sgs[o] + (1.0 - gc) * s->nu_sgs[nb];
+      double gf[9], d_on[3], dn_perp[3];
+      for (int q = 0; q < 9; ++q) gf[q] = gc * s->grad_U[9 * (size_t)o + q] + (1.0 - gc) * s->grad_U[9 * (size_t)nb + q];
+      for (int i = 0; i < 3; ++i) d_on[i] = s->m.cell_centroid[3 * nb + i] - s->m.cell_centroid[3 * o + i];
+      const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+      const double sdd = dot3(sf, d_on) / dlen;
+      for (int i = 0; i < 3; ++i) dn_perp[i] = sf[i] - (sdd / dlen) * d_on[i];
+      const double wo = ortho_weight(sdd, s->m.face_area[f]);
+      for (int k = 0; k < 3; ++k) {
+        const double col = gf[3 * k] * dn_perp[0] + gf[3 * k + 1] * dn_perp[1] + gf[3 * k + 2] * dn_perp[2];
+        const double row = gf[k] * sf[0] + gf[3 + k] * sf[1] + gf[6 + k] * sf[2];
+        st[k] = nue * ((s->U[3 * nb + k] - s->U[3 * o + k]) * wo / dlen + col + row);
+      }
+    } else {
+      const int b = f - s->nint;
+      const int bc = bc_of_face(s, f);
+      if (bc == BC_PERIODIC) {
+        int nb_per = -1, pair_face = -1;
+        const int pair = s->m.periodic_pair[b];
+        if (pair >= 0) {
+          pair_face = s->nint + pair;
+          nb_per = s->m.face_owner[pair_face];
+        }
+        if (nb_per == o) nb_per = -1;
+        if (nb_per >= 0) {
+          double d_per, gcp;
+          if (pair_face >= 0) {
+            d_per = s->m.face_dcf[f] + s->m.face_dcf[pair_face];
+            gcp = s->m.face_dcf[pair_face] / (d_per + SMALL);
+          } else if (s->m.periodic_pair_dcf[b] > SMALL) {
+            d_per = s->m.face_dcf[f] + s->m.periodic_pair_dcf[b];
+            gcp = s->m.periodic_pair_dcf[b] / (d_per + SMALL);
+          } else {
+            d_per = 2.0 * s->m.face_dcf[f];
+            gcp = 0.5;
+          }
+          const double nue = nu + gcp * s->nu_sgs[o] + (1.0 - gcp) * s->nu_sgs[nb_per];
+          double d_on[3], dn_perp[3], gf[9];
+          for (int i = 0; i < 3; ++i) d_on[i] = d_per * n_hat[i];
+          const double dlen = d_per + SMALL;
+          const double sdd = dot3(sf, d_on) / dlen;
+          for (int i = 0; i < 3; ++i) dn_perp[i] = sf[i] - (sdd / dlen) * d_on[i];
+          for (int q = 0; q < 9; ++q)
+            gf[q] = gcp * s->grad_U[9 * (size_t)o + q] + (1.0 - gcp) * s->grad_U[9 * (size_t)nb_per + q];
+          const double wo = ortho_weight(sdd, s->m.face_area[f]);
+          for (int k = 0; k < 3; ++k) {
+            const double col = gf[3 * k] * dn_perp[0] + gf[3 * k + 1] * dn_perp[1] + gf[3 * k + 2] * dn_perp[2];
+            const double row = gf[k] * sf[0] + gf[3 + k] * sf[1] + gf[6 + k] * sf[2];
+            st[k] = nue * ((s->U[3 * nb_per + k] - s->U[3 * o + k]) * wo / dlen + col + row);
+          }
+        } else {
+          const double nue = nu + s->nu_sgs[o];
+          for (int k = 0; k < 3; ++k)
+            st[k] = nue * ((s->U_bnd[3 * b + k] - s->U[3 * o + k]) / (s->m.face_dcf[f] + SMALL)) * s->m.face_area[f];
+        }
+      } else if (bc == BC_WALL_MODEL) {
+        have = 0;
+      } else {
+        const double nue = nu + s->nu_sgs[o];
+        double d_on[3], dn_perp[3];
+        for (int i = 0; i < 3; ++i) d_on[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+        const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+        const double sdd = dot3(sf, d_on) / dlen;
+        for (int i = 0; i < 3; ++i) dn_perp[i] = sf[i] - (sdd / dlen) * d_on[i];
+        const double *go = &s->grad_U[9 * (size_t)o];
+        const double wo = ortho_weight(sdd, s->m.face_area[f]);
+        for (int k = 0; k < 3; ++k) {
+          const double col = go[3 * k] * dn_perp[0] + go[3 * k + 1] * dn_perp[1] + go[3 * k + 2] * dn_perp[2];
+          const double row = go[k] * sf[0] + go[3 + k] * sf[1] + go[6 + k] * sf[2];
+          st[k] = nue * ((s->U_bnd[3 * b + k] - s->U[3 * o + k]) * wo / dlen + col + row);
+        }
+      }
+    }
+    if (!have) continue;
+    for (int k = 0; k < 3; ++k) {
+      rhs[3 * o + k] = rhs[3 * o + k] + st[k] * s->inv_vol[o];
+      if (nb >= 0) rhs[3 * nb + k] = rhs[3 * nb + k] - st[k] * s->inv_vol[nb];
+    }
+  }
+}
+
+/* compute_explicit_rhs (src/solver/mod_fv_solver.f90:1617-1762), momentum only */
+static void explicit_rhs(om_solver *s) {
+  if (s->p.scheme == OM_SCHEME_KEP)
+    advection_kep(s, s->conv);
+  else
+    advection_cds(s, s->conv);
+  viscous_vector(s, s->visc);
+  for (int c = 0; c < s->nc; ++c)
+    for (int k = 0; k < 3; ++k) s->H[3 * c + k] = s->conv[3 * c + k] + s->visc[3 * c + k] + s->p.f_const[k];
+  /* add_wall_stress_to_rhs (:2238-2275) */
+  if (s->n_wm_patches > 0)
+    for (int b = 0; b < s->nb; ++b) {
+      const int f = s->nint + b;
+      if (bc_of_face(s, f) != BC_WALL_MODEL) continue;
+      const int o = s->m.face_owner[f];
+      const double w = s->m.face_area[f] / s->m.cell_volume[o];
+      for (int k = 0; k < 3; ++k) s->H[3 * o + k] = s->H[3 * o + k] + s->tau_w[3 * b + k] * w;
+    }
+}
+
+/* ---------------------------------------------------------------- projection */
+
+/* compute_face_fluxes_and_rhs (src/solver/mod_fv_solver.f90:2389-2752) */
+static void face_fluxes_and_rhs(om_solver *s, double dt, int incr) {
+  const int nc = s->nc;
+  if (!s->skip_grad_p) {
+    if (incr) {
+      for (int c = 0; c < nc; ++c) s->p_tot[c] = s->phi_prev[c] + s->P[c];
+      for (int b = 0; b < s->nb; ++b) s->p_tot_bnd[b] = s->phi_prev_bnd[b] + s->p_bnd[b];
+      gradient_of_scalar(s, s->p_tot, s->p_tot_bnd, s->gp);
+    } else {
+      gradient_of_scalar(s, s->P, s->p_bnd, s->gp);
+    }
+  }
+  double *rhs = s->rhs_p, *ff = s->face_flux;
+  memset(rhs, 0, sizeof(double) * nc);
+  memset(ff, 0, sizeof(double) * s->nf);
+  memset(s->built, 0, s->nf);
+  for (int idx = 0; idx < s->npf; ++idx) {
+    if (s->ps_skip[idx]) continue;
+    const int o = s->ps_owner[idx], nb = s->ps_nb[idx], fm = s->ps_fmesh[idx];
+    double gcf;
+    if (fm < s->nint) {
+      gcf = s->m.face_gc[fm];
+    } else {
+      gcf = 0.5;
+      const int b = fm - s->nint;
+      const double d_o = s->m.face_dcf[fm];
+      double d_nb = d_o;
+      if (s->m.periodic_pair_dcf[b] > SMALL) d_nb = s->m.periodic_pair_dcf[b];
+      if (d_nb <= SMALL && s->m.periodic_pair[b] >= 0) d_nb = s->m.face_dcf[s->nint + s->m.periodic_pair[b]];
+      const double d_tot = d_o + d_nb;
+      if (d_tot > SMALL) gcf = d_nb / d_tot;
+    }
+    double d_on[3], Uf[3];
+    for (int i = 0; i < 3; ++i) d_on[i] = s->m.cell_centroid[3 * nb + i] - s->m.cell_centroid[3 * o + i];
+    const double *sf = &s->m.face_sf[3 * fm];
+    double Upl[3];
+    for (int k = 0; k < 3; ++k) Upl[k] = gcf * s->U[3 * o + k] + (1.0 - gcf) * s->U[3 * nb + k];
+    if (fm < s->nint) {
+      const double wb = refine_blend_weight(s->m.face_skew[fm]);
+      const int om = s->m.face_owner[fm], nm = s->m.face_neighbour[fm];
+      const double g = s->m.face_gc[fm];
+      const double *eps = &s->m.face_eps[3 * fm];
+      double Ue[3];
+      for (int j = 0; j < 3; ++j) {
+        Ue[j] = g * s->U[3 * om + j] + (1.0 - g) * s->U[3 * nm + j];
+        double gm[3];
+        for (int i = 0; i < 3; ++i) gm[i] = g * s->grad_U[GI(om, i, j)] + (1.0 - g) * s->grad_U[GI(nm, i, j)];
+        Ue[j] = Ue[j] + gm[0] * eps[0] + gm[1] * eps[1] + gm[2] * eps[2];
+      }
+      for (int k = 0; k < 3; ++k) Uf[k] = Upl[k] + wb * (Ue[k] - Upl[k]);
+    } else {
+      for (int k = 0; k < 3; ++k) Uf[k] = Upl[k];
+    }
+    double flux = Uf[0] * sf[0] + Uf[1] * sf[1] + Uf[2] * sf[2];
+    const double coeff = s->ps_coeff[idx];
+    double krc[3] = {0.0, 0.0, 0.0};
+    if (!s->skip_nonortho && !s->p.rhie_chow_dual) {
+      double kf[3];
+      kfactor_limited(sf, d_on, s->p.min_cos_theta, kf);
+      if (kf[0] * kf[0] + kf[1] * kf[1] + kf[2] * kf[2] >= NONORTHO_SIN2_TOL * (sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]))
+        for (int i = 0; i < 3; ++i) krc[i] = kf[i];
+    }
+    /* nonortho_rhie_chow (src/numerics/mod_nonortho.f90:101-117) */
+    double gm[3], smk[3];
+    for (int i = 0; i < 3; ++i) {
+      gm[i] = gcf * s->gp_rc[3 * o + i] + (1.0 - gcf) * s->gp_rc[3 * nb + i];
+      smk[i] = sf[i] - krc[i];
+    }
+    const double rc = dot3(gm, smk) - coeff * (s->phi_prev[nb] - s->phi_prev[o]);
+    flux = flux + dt * rc;
+    ff[fm] = flux;
+    s->built[fm] = 1;
+    rhs[o] = rhs[o] - flux;
+    rhs[nb] = rhs[nb] + flux;
+  }
+  for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+    const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+    const int bc = bc_of_face(s, f);
+    double flux;
+    if (bc == BC_WALL_NOSLIP || bc == BC_WALL_MODEL || bc == BC_WALL_VELOCITY || bc == BC_SLIP || bc == BC_SYMMETRY)
+      flux = 0.0;
+    else
+      flux = s->U_bnd[3 * b] * s->m.face_sf[3 * f] + s->U_bnd[3 * b + 1] * s->m.face_sf[3 * f + 1] +
+             s->U_bnd[3 * b + 2] * s->m.face_sf[3 * f + 2];
+    ff[f] = flux;
+    s->built[f] = 1;
+    rhs[o] = rhs[o] - flux;
+  }
+  for (int ib = 0; ib < s->n_per_bf; ++ib) {
+    const int b = s->per_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+    if (s->built[f]) continue;
+    const int pair = s->m.periodic_pair[b];
+    if (pair >= 0 && s->built[s->nint + pair]) {
+      ff[f] = -ff[s->nint + pair];
+      s->built[f] = 1;
+      continue;
+    }
+    const int nb = (pair >= 0) ? s->m.face_owner[s->nint + pair] : -1;
+    const double *sf = &s->m.face_sf[3 * f];
+    if (nb >= 0)
+      ff[f] = 0.5 * ((s->U[3 * o] + s->U[3 * nb]) * sf[0] + (s->U[3 * o + 1] + s->U[3 * nb + 1]) * sf[1] +
+                     (s->U[3 * o + 2] + s->U[3 * nb + 2]) * sf[2]);
+    else
+      ff[f] = s->U[3 * o] * sf[0] + s->U[3 * o + 1] * sf[1] + s->U[3 * o + 2] * sf[2];
+    s->built[f] = 1;
+  }
+  const double rdt = 1.0 / fmax(dt, SMALL);
+  for (int c = 0; c < nc; ++c) rhs[c] = rhs[c] * rdt;
+  if (!s->skip_nonortho) {
+    for (int idx = 0; idx < s->npf; ++idx) {
+      if (s->ps_skip[idx] || !s->act_int[idx]) continue;
+      const int o = s->ps_owner[idx], nb = s->ps_nb[idx], fm = s->ps_fmesh[idx];
+      const double gcw = (fm >= s->nint) ? s->m.periodic_gc[fm - s->nint] : s->m.face_gc[fm];
+      const double cr = cross_flux(&s->gp[3 * o], &s->gp[3 * nb], gcw, &s->k_int[3 * idx]);
+      rhs[o] = rhs[o] + cr;
+      rhs[nb] = rhs[nb] - cr;
+    }
+    for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+      const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+      if (bc_of_face(s, f) != BC_OUTLET || !s->act_bnd[ib]) continue;
+      rhs[o] = rhs[o] + dot3(&s->gp[3 * o], &s->k_bnd[3 * ib]);
+    }
+  }
+}
+
+/* correct_face_fluxes (src/solver/mod_fv_solver.f90:2756-2900) */
+static void correct_face_fluxes(om_solver *s, double dt, int ptot) {
+  double *ff = s->face_flux;
+  for (int idx = 0; idx < s->npf; ++idx) {
+    if (s->ps_skip[idx]) continue;
+    const int o = s->ps_owner[idx], nb = s->ps_nb[idx], fm = s->ps_fmesh[idx];
+    ff[fm] = ff[fm] - dt * s->ps_coeff[idx] * (s->P[nb] - s->P[o]);
+    if (s->act_int[idx]) {
+      const double gc = (fm >= s->nint) ? s->m.periodic_gc[fm - s->nint] : s->m.face_gc[fm];
+      ff[fm] = ff[fm] - dt * cross_flux(&s->gp[3 * o], &s->gp[3 * nb], gc, &s->k_int[3 * idx]);
+    }
+  }
+  for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+    const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+    if (bc_of_face(s, f) != BC_OUTLET) continue;
+    const double *n = &s->m.face_normal[3 * f];
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+    const double sn = fabs(d[0] * n[0] + d[1] * n[1] + d[2] * n[2]);
+    const double coeff = s->m.face_area[f] / fmax(sn, SMALL);
+    ff[f] = ff[f] - dt * coeff * ((ptot ? s->outlet_pb[b] : 0.0) - s->P[o]);
+    if (s->act_bnd[ib]) ff[f] = ff[f] - dt * dot3(&s->gp[3 * o], &s->k_bnd[3 * ib]);
+  }
+  for (int ib = 0; ib < s->n_per_bf; ++ib) {
+    const int b = s->per_bf[ib];
+    const int pair = s->m.periodic_pair[b];
+    if (pair < 0) continue;
+    const int f = s->nint + b, pf = s->nint + pair;
+    const int os = s->m.face_owner[f], op = s->m.face_owner[pf];
+    if (os == op) continue;
+    if (os > op) ff[f] = -ff[pf];
+  }
+}
+
+/* correct_velocity_psolver_dual (src/solver/mod_fv_solver.f90:2927-3151) */
+static void correct_velocity(om_solver *s, double dt, int with_cross, int ptot) {
+  const int do_cross = with_cross && !s->skip_nonortho;
+  double *gpc = s->gp_cell;
+  memset(gpc, 0, sizeof(double) * 3 * (size_t)s->nc);
+  for (int idx = 0; idx < s->npf; ++idx) {
+    if (s->ps_skip[idx]) continue;
+    const int o = s->ps_owner[idx], nb = s->ps_nb[idx], fm = s->ps_fmesh[idx];
+    double g = s->ps_coeff[idx] * (s->P[nb] - s->P[o]);
+    if (do_cross && s->act_int[idx]) {
+      const double gc = (fm >= s->nint) ? s->m.periodic_gc[fm - s->nint] : s->m.face_gc[fm];
+      g = g + cross_flux(&s->gp[3 * o], &s->gp[3 * nb], gc, &s->k_int[3 * idx]);
+    }
+    const double *n = &s->m.face_normal[3 * fm];
+    for (int i = 0; i < 3; ++i) gpc[3 * o + i] = gpc[3 * o + i] + g * n[i];
+    for (int i = 0; i < 3; ++i) gpc[3 * nb + i] = gpc[3 * nb + i] + g * n[i];
+  }
+  for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+    const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+    if (bc_of_face(s, f) != BC_OUTLET) continue;
+    const double *n = &s->m.face_normal[3 * f];
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+    const double sn = fabs(d[0] * n[0] + d[1] * n[1] + d[2] * n[2]);
+    const double coeff = s->m.face_area[f] / fmax(sn, SMALL);
+    double g = coeff * ((ptot ? s->outlet_pb[b] : 0.0) - s->P[o]);
+    if (do_cross && s->act_bnd[ib]) g = g + dot3(&s->gp[3 * o], &s->k_bnd[3 * ib]);
+    for (int i = 0; i < 3; ++i) gpc[3 * o + i] = gpc[3 * o + i] + g * n[i];
+  }
+  if (!do_cross) {
+    const double *fb = s->p.f_const;
+    for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+      const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+      if (bc_of_face(s, f) == BC_OUTLET) continue;
+      const double *n = &s->m.face_normal[3 * f];
+      const double d1 = fb[0] - 0.0, d2 = fb[1] - 0.0, d3 = fb[2] - 0.0;
+      const double fn = (d1 * n[0] + d2 * n[1] + d3 * n[2]) * s->m.face_area[f];
+      for (int i = 0; i < 3; ++i) gpc[3 * o + i] = gpc[3 * o + i] + fn * n[i];
+    }
+  }
+  for (int c = 0; c < s->nc; ++c) {
+    const double *T = &s->Tinv[9 * (size_t)c]; /* row-major T[3i+j] */
+    double g[3];
+    for (int i = 0; i < 3; ++i) g[i] = T[3 * i] * gpc[3 * c] + T[3 * i + 1] * gpc[3 * c + 1] + T[3 * i + 2] * gpc[3 * c + 2];
+    for (int i = 0; i < 3; ++i) {
+      gpc[3 * c + i] = g[i];
+      s->U[3 * c + i] = s->U[3 * c + i] - dt * g[i];
+    }
+  }
+}
+
+/* anchor_pressure_to_outlet (src/solver/mod_fv_solver.f90:3610-3639) */
+static void anchor_pressure(om_solver *s) {
+  if (s->n_outlet > 0) return;
+  double pm = 0.0;
+  for (int c = 0; c < s->nc; ++c) pm = pm + s->P[c] * s->m.cell_volume[c];
+  const double shift = (s->V_total > 0.0) ? pm / s->V_total : 0.0;
+  for (int c = 0; c < s->nc; ++c) s->P[c] = s->P[c] - shift;
+}
+
+/* ---------------------------------------------------------------- pressure solve */
+
+static double vdot(int n, const double *a, const double *b) {
+  double r = 0.0;
+  for (int i = 0; i < n; ++i) r = r + a[i] * b[i];
+  return r;
+}
+
+static void spmv(int n, int nf, const int32_t *own, const int32_t *nbr, const double *coeff, const double *diag,
+                 const double *x, double *y) {
+  for (int i = 0; i < n; ++i) y[i] = diag[i] * x[i];
+  for (int f = 0; f < nf; ++f) {
+    if (own[f] < 0 || nbr[f] < 0) continue;
+    const double c = coeff[f];
+    if (!isfinite(c) || c == 0.0) continue;
+    y[own[f]] = y[own[f]] - c * x[nbr[f]];
+    y[nbr[f]] = y[nbr[f]] - c * x[own[f]];
+  }
+}
+
+static void remove_mean(int n, double *v) {
+  double sm = 0.0;
+  for (int i = 0; i < n; ++i) sm = sm + v[i];
+  const double mean = sm / (double)n;
+  for (int i = 0; i < n; ++i) v[i] = v[i] - mean;
+}
+
+int om_cg_jacobi(int n, int nfaces, const int32_t *owner, const int32_t *neighbour, const double *coeff,
+                 const double *diag_in, const double *b_in, double *x, double rtol, double atol, int max_it,
+                 int nullspace, int *its, double *resid) {
+  double *diag = dz(n), *b = dz(n), *r = dz(n), *z = dz(n), *p = dz(n), *q = dz(n);
+  for (int i = 0; i < n; ++i) {
+    double d = diag_in[i];
+    if (!isfinite(d) || fabs(d) < 1.0e-20) d = 1.0;
+    diag[i] = d;
+    b[i] = isfinite(b_in[i]) ? b_in[i] : 0.0;
+    if (!isfinite(x[i])) x[i] = 0.0;
+  }
+  if (nullspace) remove_mean(n, b);
+  const double bnorm = sqrt(vdot(n, b, b));
+  const double ttol = fmax(rtol * bnorm, atol);
+  spmv(n, nfaces, owner, neighbour, coeff, diag, x, q);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
+  double dp = sqrt(vdot(n, r, r));
+  int it = 0, conv = (dp <= ttol);
+  double beta = 0.0, betaold = 1.0;
+  if (!conv) {
+    for (int i = 0; i < n; ++i) z[i] = r[i] / diag[i];
+    if (nullspace) remove_mean(n, z);
+    beta = vdot(n, r, z);
+    while (it < max_it) {
+      if (it == 0) {
+        memcpy(p, z, sizeof(double) * n);
+      } else {
+        const double bb = beta / betaold;
+        for (int i = 0; i < n; ++i) p[i] = z[i] + bb * p[i];
+      }
+      spmv(n, nfaces, owner, neighbour, coeff, diag, p, q);
+      const double dpi = vdot(n, p, q);
+      const double a = beta / dpi;
+      for (int i = 0; i < n; ++i) {
+        x[i] = x[i] + a * p[i];
+        r[i] = r[i] - a * q[i];
+      }
+      it++;
+      dp = sqrt(vdot(n, r, r));
+      if (dp <= ttol) {
+        conv = 1;
+        break;
+      }
+      for (int i = 0; i < n; ++i) z[i] = r[i] / diag[i];
+      if (nullspace) remove_mean(n, z);
+      betaold = beta;
+      beta = vdot(n, r, z);
+    }
+  }
+  /* true residual against the caller's rhs (KSPBuildResidual) */
+  spmv(n, nfaces, owner, neighbour, coeff, diag, x, q);
+  double rr = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double bi = isfinite(b_in[i]) ? b_in[i] : 0.0;
+    rr = rr + (bi - q[i]) * (bi - q[i]);
+  }
+  *resid = sqrt(rr);
+  *its = it;
+  free(diag); free(b); free(r); free(z); free(p); free(q);
+  return conv ? 0 : -1;
+}
+
+/* pressure_solver_solve (src/solver/mod_pressure_solver.f90:638-806) */
+static int pressure_solve(om_solver *s, double atol, int ptot) {
+  const int nc = s->nc;
+  for (int c = 0; c < nc; ++c) s->cg_b[c] = s->rhs_p[c];
+  if (ptot)
+    for (int i = 0; i < s->n_dir; ++i) s->cg_b[s->dir_rows[i]] = s->cg_b[s->dir_rows[i]] + s->dir_vals[i];
+  int its = 0;
+  double res = 0.0;
+  const int max_it = (s->p.p_max_iter > 2000) ? s->p.p_max_iter : 2000;
+  /* skipped faces carry a zero coefficient */
+  const int rc = om_cg_jacobi(nc, s->npf, s->ps_owner, s->ps_nb, s->ps_coeff, s->ps_diag, s->cg_b, s->P, s->p.p_tol, atol,
+                              max_it, s->n_dir == 0, &its, &res);
+  s->n_solves++;
+  s->n_iters += its;
+  s->last_iters = its;
+  return rc;
+}
+
+/* ---------------------------------------------------------------- time step */
+
+/* solver_cfl_rates + solver_compute_dt (src/solver/mod_fv_solver.f90:3718-3899) */
+static double compute_dt(om_solver *s) {
+  double *fs = s->dt_flux, *ds = s->dt_diff;
+  memset(fs, 0, sizeof(double) * s->nc);
+  memset(ds, 0, sizeof(double) * s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const double *sf = &s->m.face_sf[3 * f];
+    const double area = sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+    const double uf = s->u_face_ready ? s->U_face[f] : (s->U[3 * o] * sf[0] + s->U[3 * o + 1] * sf[1] + s->U[3 * o + 2] * sf[2]);
+    fs[o] = fs[o] + fabs(uf);
+    if (nb >= 0) fs[nb] = fs[nb] + fabs(uf);
+    const double w = area / fmax(s->m.face_dcf[f], SMALL);
+    if (nb >= 0) {
+      const double gc = s->m.face_gc[f];
+      const double nue = s->p.nu + gc * fmax(0.0, s->nu_sgs[o]) + (1.0 - gc) * fmax(0.0, s->nu_sgs[nb]);
+      ds[o] = ds[o] + nue * w;
+      ds[nb] = ds[nb] + nue * w;
+    } else if (bc_of_face(s, f) != BC_WALL_MODEL) {
+      ds[o] = ds[o] + (s->p.nu + fmax(0.0, s->nu_sgs[o])) * w;
+    }
+  }
+  double cfl = 0.0, dif = 0.0;
+  for (int c = 0; c < s->nc; ++c) {
+    cfl = fmax(cfl, fs[c] / (s->m.cell_volume[c] + SMALL));
+    dif = fmax(dif, ds[c] / (s->m.cell_volume[c] + SMALL));
+  }
+  double dc = 1e308, dd = 1e308;
+  if (cfl > SMALL) dc = s->p.cfl_max / cfl;
+  if (dif > SMALL) dd = 1.0 / dif;
+  double dt = fmin(dc, dd);
+  if (!(dt > 0.0)) dt = s->p.dt;
+  dt = fmin(dt, s->p.dt);
+  const double dtmin = 1.0e-6 * s->p.dt;
+  if (!(dt > 0.0)) dt = s->p.dt;
+  else if (dt < dtmin) return -1.0;
+  else dt = fmax(dt, dtmin);
+  return dt;
+}
+
+static double atol_for(om_solver *s, double gdt) {
+  double ub1 = 0.0, ub2 = 0.0;
+  for (int c = 0; c < s->nc; ++c) {
+    const double *u = &s->U[3 * c];
+    ub1 = ub1 + sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * s->m.cell_volume[c];
+    ub2 = ub2 + s->m.cell_volume[c];
+  }
+  const double us = ub1 / fmax(ub2, SMALL);
+  return fmax(1.0e-12 * fmax(us, 1.0) * s->V_min / gdt, s->p_atol_factor);
+}
+
+/* solver_project_once (src/solver/mod_fv_solver.f90:837-917) */
+int om_project_once(om_solver *s, double dt) {
+  const int nc = s->nc;
+  if (dt <= 0.0) dt = s->p.dt;
+  memset(s->rhs_p, 0, sizeof(double) * nc);
+  memset(s->phi_prev, 0, sizeof(double) * nc);
+  memset(s->phi_prev_bnd, 0, sizeof(double) * s->nb);
+  memset(s->gp_rc, 0, sizeof(double) * 3 * (size_t)nc);
+  refresh_bc(s);
+  face_fluxes_and_rhs(s, dt, 0);
+  int rc = 0;
+  if (s->p.p_max_iter > 0) rc = pressure_solve(s, atol_for(s, dt), 1);
+  anchor_pressure(s);
+  correct_face_fluxes(s, dt, 1);
+  memcpy(s->U_face, s->face_flux, sizeof(double) * s->nf);
+  s->u_face_ready = 1;
+  apply_pressure_bc(s, s->P, s->p_bnd, 0, NULL, 1.0, s->outlet_pb);
+  correct_velocity(s, dt, 1, 1);
+  apply_velocity_bc(s, 0, NULL);
+  if (s->p.rotational) {
+    for (int c = 0; c < nc; ++c) s->P[c] = s->P[c] + (s->p.nu + s->nu_sgs[c]) * s->rhs_p[c] * dt * s->inv_vol[c];
+    apply_pressure_bc(s, s->P, s->p_bnd, 0, NULL, 1.0, s->outlet_pb);
+  }
+  memcpy(s->p_old, s->P, sizeof(double) * nc);
+  memcpy(s->phi_prev, s->P, sizeof(double) * nc);
+  memcpy(s->phi_prev_bnd, s->p_bnd, sizeof(double) * s->nb);
+  apply_pressure_bc(s, s->P, s->p_bnd, 0, NULL, 1.0, s->outlet_pb);
+  compute_pressure_gradient(s);
+  return rc;
+}
+
+/* advance_rk3w (src/solver/mod_fv_solver.f90:1180-1614) */
+static int advance_rk3w(om_solver *s, double dt) {
+  const int nc = s->nc;
+  memset(s->H, 0, sizeof(double) * 3 * (size_t)nc);
+  memset(s->H0, 0, sizeof(double) * 3 * (size_t)nc);
+  memset(s->rhs_p, 0, sizeof(double) * nc);
+  const double time_n = s->time;
+  int rc = 0;
+  for (int k = 0; k < 3; ++k) {
+    const double gdt = RK3_G[k] * dt;
+    s->time = time_n + RK3_C[k] * dt;
+    refresh_bc(s);
+    compute_all_gradients(s);
+    if (s->p.sgs == OM_SGS_VREMAN)
+      vreman(s, gdt);
+    else if (s->p.sgs == OM_SGS_SMAGORINSKY)
+      smagorinsky(s, gdt);
+    else
+      memset(s->nu_sgs, 0, sizeof(double) * nc);
+    if (s->p.wall_model == OM_WM_NONE)
+      memset(s->tau_w, 0, sizeof(double) * 3 * (size_t)s->nb);
+    else
+      wall_model(s);
+    explicit_rhs(s);
+    const double a = RK3_A[k], bb = RK3_B[k];
+    for (int c = 0; c < nc; ++c)
+      for (int q = 0; q < 3; ++q) s->U[3 * c + q] = s->U[3 * c + q] + dt * (a * s->H[3 * c + q] + bb * s->H0[3 * c + q]);
+    velocity_bc_both(s);
+    memcpy(s->p_old, s->P, sizeof(double) * nc);
+    correct_velocity(s, gdt, 0, 1);
+    memcpy(s->gp_rc, s->gp_cell, sizeof(double) * 3 * (size_t)nc);
+    velocity_bc_both(s);
+    compute_all_gradients(s);
+    memcpy(s->phi_prev, s->P, sizeof(double) * nc);
+    memcpy(s->phi_prev_bnd, s->p_bnd, sizeof(double) * s->nb);
+    {
+      const double atol = atol_for(s, gdt);
+      int nmax = s->n_corr < 3 ? s->n_corr : 3;
+      double stag = 0.9;
+      if (s->diag_interface) {
+        nmax = s->n_corr < 8 ? s->n_corr : 8;
+        stag = 0.99;
+      }
+      double dprev = 1e308, dphi_prev = 1e308;
+      if (s->p.nonortho_check) {
+        nmax = s->n_corr < 8 ? s->n_corr : 8;
+        memset(s->phi_it, 0, sizeof(double) * nc);
+      }
+      for (int jno = 1; jno <= nmax; ++jno) {
+        face_fluxes_and_rhs(s, gdt, jno > 1);
+        double dcur = 0.0;
+        for (int c = 0; c < nc; ++c) dcur = fmax(dcur, fabs(s->rhs_p[c]));
+        if (jno == 1) memset(s->P, 0, sizeof(double) * nc);
+        if (pressure_solve(s, atol, 0) != 0) rc = -2;
+        apply_pressure_bc(s, s->P, s->p_bnd, 1, s->skip_grad_p ? NULL : s->gp, 0.0, NULL);
+        apply_pressure_bc(s, s->P, s->p_bnd, 2, NULL, 0.0, NULL);
+        if (s->p.nonortho_check) {
+          double dd = 0.0, dpp = 0.0;
+          for (int c = 0; c < nc; ++c) {
+            dd = fmax(dd, fabs(s->P[c] - s->phi_it[c]));
+            dpp = fmax(dpp, fabs(s->P[c]));
+            s->phi_it[c] = s->P[c];
+          }
+          if (jno > 1 && (dd < 1.0e-3 * dpp || dd > dphi_prev * stag)) break;
+          dphi_prev = dd;
+        } else if (jno > 1 && (dcur < atol || dcur > dprev * stag)) {
+          break;
+        }
+        dprev = dcur;
+      }
+    }
+    correct_face_fluxes(s, gdt, 0);
+    memcpy(s->U_face, s->face_flux, sizeof(double) * s->nf);
+    s->u_face_ready = 1;
+    correct_velocity(s, gdt, 1, 0);
+    velocity_bc_both(s);
+    if (s->p.rotational) {
+      for (int c = 0; c < nc; ++c) s->P[c] = s->P[c] + (s->p.nu + s->nu_sgs[c]) * s->rhs_p[c] * gdt * s->inv_vol[c];
+      apply_pressure_bc(s, s->P, s->p_bnd, 1, s->skip_grad_p ? NULL : s->gp, 0.0, s->outlet_pb);
+      apply_pressure_bc(s, s->P, s->p_bnd, 2, NULL, 0.0, NULL);
+    }
+    for (int c = 0; c < nc; ++c) s->P[c] = s->P[c] + s->p_old[c];
+    anchor_pressure(s);
+    apply_pressure_bc(s, s->P, s->p_bnd, 1, s->skip_grad_p ? NULL : s->gp, 1.0, s->outlet_pb);
+    apply_pressure_bc(s, s->P, s->p_bnd, 2, NULL, 1.0, NULL);
+    compute_pressure_gradient(s);
+    memcpy(s->H0, s->H, sizeof(double) * 3 * (size_t)nc);
+  }
+  s->time = time_n;
+  return rc;
+}
+
+int om_advance(om_solver *s, double dt_req, double *dt_used) {
+  double dt = dt_req;
+  const double safe = compute_dt(s);
+  if (safe < 0.0) return -3;
+  if (dt > safe) dt = safe;
+  const int rc = advance_rk3w(s, dt);
+  /* apply_scalar_bc: no temperature or scalars here */
+  s->time = s->time + dt;
+  if (dt_used) *dt_used = dt;
+  return rc;
+}
+
+/* compute_stats_sums (src/stats/mod_stats.f90:271-336) + solver_divergence_metrics (mod_fv_solver.f90:3643-3714) */
+void om_stats(om_solver *s, double out[8]) {
+  double E = 0, V = 0, eps = 0, mom[3] = {0, 0, 0}, sgs = 0, tn = 0, ta = 0, umax = 0;
+  for (int c = 0; c < s->nc; ++c) {
+    const double vol = s->m.cell_volume[c];
+    const double *u = &s->U[3 * c];
+    const double *g = &s->grad_U[9 * (size_t)c];
+    E = E + 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * vol;
+    V = V + vol;
+    const double S11 = g[GI(0, 0, 0)], S22 = g[GI(0, 1, 1)], S33 = g[GI(0, 2, 2)];
+    const double S12 = 0.5 * (g[GI(0, 1, 0)] + g[GI(0, 0, 1)]);
+    const double S13 = 0.5 * (g[GI(0, 2, 0)] + g[GI(0, 0, 2)]);
+    const double S23 = 0.5 * (g[GI(0, 2, 1)] + g[GI(0, 1, 2)]);
+    const double ss = S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23);
+    eps = eps + (s->p.nu + s->nu_sgs[c]) * ss * vol;
+    for (int k = 0; k < 3; ++k) mom[k] = mom[k] + u[k] * vol;
+    sgs = sgs + s->nu_sgs[c] * vol;
+    const double um = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (um > umax) umax = um;
+  }
+  for (int p = 0; p < s->npatch; ++p) {
+    if (s->m.patch_bc[p] != BC_WALL_MODEL) continue;
+    for (int q = 0; q < s->m.patch_nfaces[p]; ++q) {
+      const int b = s->m.patch_start[p] + q;
+      const double *t = &s->tau_w[3 * b];
+      tn = tn + sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) * s->m.face_area[s->nint + b];
+      ta = ta + s->m.face_area[s->nint + b];
+    }
+  }
+  out[0] = E / V;
+  out[1] = 2.0 * eps / V;
+  out[2] = umax;
+  {
+    const double a = mom[0] / V, b = mom[1] / V, c = mom[2] / V;
+    out[3] = sqrt(a * a + b * b + c * c);
+  }
+  out[4] = (ta > SMALL) ? tn / ta : 0.0;
+  out[5] = sgs / V;
+  /* face-flux divergence (face_flux_divergence, :3532-3579) */
+  double *div = s->div_diag;
+  memset(div, 0, sizeof(double) * s->nc);
+  for (int idx = 0; idx < s->npf; ++idx) {
+    if (s->ps_skip[idx]) continue;
+    const int o = s->ps_owner[idx], nb = s->ps_nb[idx];
+    const double fl = s->face_flux[s->ps_fmesh[idx]];
+    div[o] = div[o] + fl * s->inv_vol[o];
+    div[nb] = div[nb] - fl * s->inv_vol[nb];
+  }
+  for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+    const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+    div[o] = div[o] + s->face_flux[f] * s->inv_vol[o];
+  }
+  double mx = 0.0, as = 0.0;
+  for (int c = 0; c < s->nc; ++c) {
+    mx = fmax(mx, fabs(div[c]));
+    as = as + fabs(div[c]) * s->m.cell_volume[c];
+  }
+  out[6] = mx;
+  out[7] = as / V;
+}
+
+/* ---------------------------------------------------------------- set-up */
+
+/* invert_sym3 (src/solver/mod_fv_solver.f90:3466-3491), row-major output */
+static void invert_sym3(const double *A, double *Ai) {
+#define A_(i, j) A[3 * ((i)-1) + ((j)-1)]
+  const double c11 = A_(2, 2) * A_(3, 3) - A_(2, 3) * A_(3, 2);
+  const double c12 = A_(1, 3) * A_(3, 2) - A_(1, 2) * A_(3, 3);
+  const double c13 = A_(1, 2) * A_(2, 3) - A_(1, 3) * A_(2, 2);
+  const double c22 = A_(1, 1) * A_(3, 3) - A_(1, 3) * A_(3, 1);
+  const double c23 = A_(1, 3) * A_(2, 1) - A_(1, 1) * A_(2, 3);
+  const double c33 = A_(1, 1) * A_(2, 2) - A_(1, 2) * A_(2, 1);
+  const double det = A_(1, 1) * c11 + A_(1, 2) * (A_(2, 3) * A_(3, 1) - A_(2, 1) * A_(3, 3)) +
+                     A_(1, 3) * (A_(2, 1) * A_(3, 2) - A_(2, 2) * A_(3, 1));
+  const double tr = A_(1, 1) + A_(2, 2) + A_(3, 3);
+#undef A_
+  const double t = fmax(tr, SMALL);
+  if (fabs(det) <= 1.0e-12 * t * t * t) {
+    for (int q = 0; q < 9; ++q) Ai[q] = 0.0;
+    Ai[0] = Ai[4] = Ai[8] = 3.0 / t;
+    return;
+  }
+  Ai[0] = c11 / det; Ai[1] = c12 / det; Ai[2] = c13 / det;
+  Ai[3] = c12 / det; Ai[4] = c22 / det; Ai[5] = c23 / det;
+  Ai[6] = c13 / det; Ai[7] = c23 / det; Ai[8] = c33 / det;
+}
+
+om_solver *om_create(const om_mesh *m, const om_params *p) {
+  om_solver *s = (om_solver *)calloc(1, sizeof(om_solver));
+  s->m = *m;
+  s->p = *p;
+  const int nc = m->n_cells, nf = m->n_faces, nint = m->n_int_faces, nb = m->n_bnd_faces;
+  s->nc = nc; s->nf = nf; s->nint = nint; s->nb = nb; s->npatch = m->n_patches;
+  s->U = dz(3 * (size_t)nc); s->P = dz(nc); s->p_old = dz(nc); s->nu_sgs = dz(nc);
+  s->U_bnd = dz(3 * (size_t)nb); s->p_bnd = dz(nb); s->phi_prev = dz(nc); s->phi_prev_bnd = dz(nb);
+  s->tau_w = dz(3 * (size_t)nb); s->grad_U = dz(9 * (size_t)nc); s->gU_prev = dz(9 * (size_t)nc);
+  s->face_flux = dz(nf); s->U_face = dz(nf); s->inv_vol = dz(nc);
+  s->H = dz(3 * (size_t)nc); s->H0 = dz(3 * (size_t)nc); s->conv = dz(3 * (size_t)nc); s->visc = dz(3 * (size_t)nc);
+  s->rhs_p = dz(nc); s->div_w = dz(nc); s->phi_it = dz(nc); s->div_diag = dz(nc);
+  s->gp = dz(3 * (size_t)nc); s->gp_rc = dz(3 * (size_t)nc); s->gp_cell = dz(3 * (size_t)nc);
+  s->p_tot = dz(nc); s->p_tot_bnd = dz(nb); s->gp_prev = dz(3 * (size_t)nc);
+  s->Tinv = dz(9 * (size_t)nc); s->outlet_pb = dz(nb); s->Gf = dz(9 * (size_t)nc);
+  s->ws_y = dz(nb); s->ws_cell = iz(nb); s->per_nb = iz(nb);
+  s->built = (char *)calloc(nf ? nf : 1, 1);
+  s->cg_b = dz(nc);
+  s->dt_flux = dz(nc); s->dt_diff = dz(nc);
+  for (int c = 0; c < nc; ++c) s->inv_vol[c] = 1.0 / fmax(m->cell_volume[c], SMALL);
+  for (int b = 0; b < nb; ++b) s->per_nb[b] = resolve_periodic_nb(s, b);
+  /* outlet_pb and patch classification (:413-422, 1766-1850) */
+  s->per_bf = iz(nb); s->nper_bf = iz(nb);
+  for (int q = 0; q < s->npatch; ++q) {
+    for (int k = 0; k < m->patch_nfaces[q]; ++k) {
+      const int b = m->patch_start[q] + k;
+      s->outlet_pb[b] = p->patch_p_val ? p->patch_p_val[q] : 0.0;
+      if (m->patch_bc[q] == BC_PERIODIC)
+        s->per_bf[s->n_per_bf++] = b;
+      else
+        s->nper_bf[s->n_nper_bf++] = b;
+    }
+    if (m->patch_bc[q] == BC_WALL_MODEL) s->n_wm_patches++;
+    if (m->patch_bc[q] == BC_OUTLET) s->n_outlet += m->patch_nfaces[q];
+  }
+  s->diag_interface = 0;
+  for (int f = 0; f < nint; ++f)
+    if (m->face_skew[f] > REFINE_SKEW_TOL) s->diag_interface = 1;
+  /* solver_init: apply_all_bc, initialize_velocity_from_bc (zero without an inlet) */
+  for (int q = 0; q < s->npatch; ++q)
+    if (m->patch_bc[q] == BC_INLET) {
+      for (int c = 0; c < nc; ++c)
+        for (int k = 0; k < 3; ++k) s->U[3 * c + k] = p->patch_U_val[3 * q + k];
+      break;
+    }
+  /* pressure_solver_init (src/solver/mod_pressure_solver.f90:100-452) */
+  int nedge = 0;
+  for (int q = 0; q < s->npatch; ++q) {
+    if (m->patch_bc[q] != BC_PERIODIC) continue;
+    for (int k = 0; k < m->patch_nfaces[q]; ++k) {
+      const int b = m->patch_start[q] + k, pair = m->periodic_pair[b];
+      if (pair < 0) continue;
+      const int nbc = m->face_owner[nint + pair], o = m->face_owner[nint + b];
+      if (nbc == o || o >= nbc) continue;
+      nedge++;
+    }
+  }
+  s->npf = nint + nedge;
+  const int npf = s->npf;
+  s->ps_owner = iz(npf); s->ps_nb = iz(npf); s->ps_fmesh = iz(npf); s->ps_skip = iz(npf);
+  s->ps_coeff = dz(npf); s->ps_diag = dz(nc);
+  for (int f = 0; f < nint; ++f) {
+    s->ps_owner[f] = m->face_owner[f];
+    s->ps_nb[f] = m->face_neighbour[f];
+    s->ps_fmesh[f] = f;
+  }
+  int idx = nint;
+  for (int q = 0; q < s->npatch; ++q) {
+    if (m->patch_bc[q] != BC_PERIODIC) continue;
+    for (int k = 0; k < m->patch_nfaces[q]; ++k) {
+      const int b = m->patch_start[q] + k, pair = m->periodic_pair[b];
+      if (pair < 0) continue;
+      const int fg = nint + b, nbc = m->face_owner[nint + pair], o = m->face_owner[fg];
+      if (nbc == o || o >= nbc) continue;
+      s->ps_owner[idx] = o;
+      s->ps_nb[idx] = nbc;
+      s->ps_fmesh[idx] = fg;
+      double pdcf = m->periodic_pair_dcf[b];
+      const int pf = nint + pair;
+      if (pdcf <= SMALL) pdcf = m->face_dcf[pf];
+      if (pdcf <= SMALL) pdcf = m->face_dcf[fg];
+      const double d_per = m->face_dcf[fg] + pdcf;
+      const double area = 0.5 * (m->face_area[fg] + m->face_area[pf]);
+      const double coeff = area / (d_per + SMALL);
+      s->ps_coeff[idx] = coeff;
+      s->ps_diag[o] = s->ps_diag[o] + coeff;
+      s->ps_diag[nbc] = s->ps_diag[nbc] + coeff;
+      idx++;
+    }
+  }
+  for (int f = 0; f < nint; ++f) {
+    const int o = m->face_owner[f], nbc = m->face_neighbour[f];
+    double d_on[3];
+    for (int i = 0; i < 3; ++i) d_on[i] = m->cell_centroid[3 * nbc + i] - m->cell_centroid[3 * o + i];
+    double d_ortho = dot3(d_on, &m->face_normal[3 * f]);
+    if (p->min_cos_theta > 0.0 && d_ortho < p->min_cos_theta * m->face_dcf[f]) d_ortho = p->min_cos_theta * m->face_dcf[f];
+    const double coeff = (d_ortho > SMALL) ? m->face_area[f] / d_ortho : m->face_area[f] / (m->face_dcf[f] + SMALL);
+    if (!isfinite(coeff) || coeff <= SMALL) {
+      s->ps_skip[f] = 1;
+      continue;
+    }
+    s->ps_coeff[f] = coeff;
+    s->ps_diag[o] = s->ps_diag[o] + coeff;
+    s->ps_diag[nbc] = s->ps_diag[nbc] + coeff;
+  }
+  s->dir_rows = iz(nb);
+  s->dir_vals = dz(nb);
+  for (int q = 0; q < s->npatch; ++q) {
+    if (m->patch_bc[q] != BC_OUTLET) continue;
+    for (int k = 0; k < m->patch_nfaces[q]; ++k) {
+      const int b = m->patch_start[q] + k, fg = nint + b, o = m->face_owner[fg];
+      double d[3];
+      for (int i = 0; i < 3; ++i) d[i] = m->face_centre[3 * fg + i] - m->cell_centroid[3 * o + i];
+      double dp = fabs(dot3(d, &m->face_normal[3 * fg]));
+      if (dp <= SMALL) dp = m->face_dcf[fg];
+      if (p->min_cos_theta > 0.0) dp = fmax(dp, p->min_cos_theta * m->face_dcf[fg]);
+      if (dp <= SMALL) continue;
+      const double cb = m->face_area[fg] / dp;
+      if (!isfinite(cb) || cb <= SMALL) continue;
+      s->ps_diag[o] = s->ps_diag[o] + cb;
+      s->dir_rows[s->n_dir] = o;
+      s->dir_vals[s->n_dir] = cb * p->patch_p_val[q];
+      s->n_dir++;
+    }
+  }
+  for (int c = 0; c < nc; ++c)
+    if (!isfinite(s->ps_diag[c]) || s->ps_diag[c] <= SMALL) s->ps_diag[c] = 1.0;
+  /* V_min / V_total / atol floor (:431-459) */
+  s->V_min = 1e308;
+  s->V_total = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    s->V_min = fmin(s->V_min, m->cell_volume[c]);
+    s->V_total = s->V_total + m->cell_volume[c];
+  }
+  s->p_atol_factor = 1.0e-15;
+  /* detect_nonortho_skip + build_nonortho_cache (:1864-1996) */
+  s->k_int = dz(3 * (size_t)npf);
+  s->act_int = (char *)calloc(npf ? npf : 1, 1);
+  s->k_bnd = dz(3 * (size_t)s->n_nper_bf);
+  s->act_bnd = (char *)calloc(s->n_nper_bf ? s->n_nper_bf : 1, 1);
+  int nact = 0, ngradbc = 0;
+  for (int i = 0; i < npf; ++i) {
+    if (s->ps_skip[i]) continue;
+    const int o = s->ps_owner[i], nbc = s->ps_nb[i], fm = s->ps_fmesh[i];
+    double d_on[3], kf[3];
+    for (int q = 0; q < 3; ++q) d_on[q] = m->cell_centroid[3 * nbc + q] - m->cell_centroid[3 * o + q];
+    const double *sf = &m->face_sf[3 * fm];
+    kfactor_limited(sf, d_on, p->min_cos_theta, kf);
+    if (kf[0] * kf[0] + kf[1] * kf[1] + kf[2] * kf[2] >= NONORTHO_SIN2_TOL * (sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2])) {
+      nact++;
+      for (int q = 0; q < 3; ++q) s->k_int[3 * i + q] = kf[q];
+      s->act_int[i] = 1;
+    }
+  }
+  for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+    const int b = s->nper_bf[ib], fg = nint + b, o = m->face_owner[fg];
+    if (bc_of_face(s, fg) != BC_OUTLET) continue;
+    double d_on[3], kf[3];
+    for (int q = 0; q < 3; ++q) d_on[q] = m->face_centre[3 * fg + q] - m->cell_centroid[3 * o + q];
+    const double *sf = &m->face_sf[3 * fg];
+    kfactor_limited(sf, d_on, p->min_cos_theta, kf);
+    if (kf[0] * kf[0] + kf[1] * kf[1] + kf[2] * kf[2] >= NONORTHO_SIN2_TOL * (sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2])) {
+      nact++;
+      for (int q = 0; q < 3; ++q) s->k_bnd[3 * ib + q] = kf[q];
+      s->act_bnd[ib] = 1;
+    }
+  }
+  for (int q = 0; q < s->npatch; ++q) {
+    const int bc = m->patch_bc[q];
+    if (!(bc == BC_OUTLET || bc == BC_PERIODIC || bc == BC_WALL_NOSLIP || bc == BC_WALL_VELOCITY || bc == BC_WALL_MODEL ||
+          bc == BC_SLIP || bc == BC_SYMMETRY))
+      ngradbc++;
+  }
+  s->skip_nonortho = (nact == 0);
+  s->skip_grad_p = s->skip_nonortho && (ngradbc == 0);
+  if (s->skip_nonortho) {
+    memset(s->k_int, 0, sizeof(double) * 3 * (size_t)npf);
+    memset(s->act_int, 0, npf);
+    memset(s->k_bnd, 0, sizeof(double) * 3 * (size_t)s->n_nper_bf);
+    memset(s->act_bnd, 0, s->n_nper_bf);
+  }
+  s->n_corr = p->n_nonortho_corr;
+  if (s->n_corr <= 0) s->n_corr = s->skip_nonortho ? 1 : 2;
+  /* build_reconstruct_operator (:3410-3462) */
+  {
+    double *T = dz(9 * (size_t)nc);
+    for (int i = 0; i < npf; ++i) {
+      if (s->ps_skip[i]) continue;
+      const int o = s->ps_owner[i], nbc = s->ps_nb[i], fm = s->ps_fmesh[i];
+      const double a = m->face_area[fm];
+      const double *n = &m->face_normal[3 * fm];
+      for (int jj = 0; jj < 3; ++jj)
+        for (int ii = 0; ii < 3; ++ii) {
+          T[9 * (size_t)o + 3 * ii + jj] = T[9 * (size_t)o + 3 * ii + jj] + a * n[ii] * n[jj];
+          T[9 * (size_t)nbc + 3 * ii + jj] = T[9 * (size_t)nbc + 3 * ii + jj] + a * n[ii] * n[jj];
+        }
+    }
+    for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+      const int fg = nint + s->nper_bf[ib], o = m->face_owner[fg];
+      const double a = m->face_area[fg];
+      const double *n = &m->face_normal[3 * fg];
+      for (int jj = 0; jj < 3; ++jj)
+        for (int ii = 0; ii < 3; ++ii) T[9 * (size_t)o + 3 * ii + jj] = T[9 * (size_t)o + 3 * ii + jj] + a * n[ii] * n[jj];
+    }
+    for (int c = 0; c < nc; ++c) invert_sym3(&T[9 * (size_t)c], &s->Tinv[9 * (size_t)c]);
+    free(T);
+  }
+  if (p->sgs != OM_SGS_NONE) build_filter_tensor(s);
+  /* wall-model sampling cache (find_sample_cell with h_wm <= 0, mod_wall_model.f90:124-173) */
+  for (int b = 0; b < nb; ++b) {
+    const int fg = nint + b, o = m->face_owner[fg];
+    s->ws_cell[b] = o;
+    const double *n = &m->face_normal[3 * fg];
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = m->cell_centroid[3 * o + i] - m->face_centre[3 * fg + i];
+    s->ws_y[b] = fabs(-(d[0] * n[0]) - d[1] * n[1] - d[2] * n[2]) + SMALL;
+  }
+  om_apply_all_bc(s);
+  return s;
+}
+
+void om_apply_all_bc(om_solver *s) {
+  apply_velocity_bc(s, 0, s->grad_U);
+  apply_pressure_bc(s, s->P, s->p_bnd, 0, s->skip_grad_p ? NULL : s->gp, 1.0, s->outlet_pb);
+}
+
+void om_free(om_solver *s) {
+  if (!s) return;
+  double *dd[] = {s->U, s->P, s->p_old, s->nu_sgs, s->U_bnd, s->p_bnd, s->phi_prev, s->phi_prev_bnd, s->tau_w,
+                  s->grad_U, s->gU_prev, s->face_flux, s->U_face, s->inv_vol, s->H, s->H0, s->conv, s->visc,
+                  s->rhs_p, s->div_w, s->phi_it, s->gp, s->gp_rc, s->gp_cell, s->p_tot, s->p_tot_bnd, s->gp_prev,
+                  s->Tinv, s->outlet_pb, s->Gf, s->ws_y, s->div_diag, s->ps_coeff, s->ps_diag, s->dir_vals,
+                  s->k_int, s->k_bnd, s->cg_b, s->dt_flux, s->dt_diff};
+  for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) free(dd[i]);
+  int *ii[] = {s->per_nb, s->per_bf, s->nper_bf, s->ws_cell, s->ps_owner, s->ps_nb, s->ps_fmesh, s->ps_skip, s->dir_rows};
+  for (size_t i = 0; i < sizeof(ii) / sizeof(ii[0]); ++i) free(ii[i]);
+  free(s->built);
+  free(s->act_int);
+  free(s->act_bnd);
+  free(s);
+}
+
+double *om_array(om_solver *s, const char *name, int64_t *len) {
+  const int64_t nc = s->nc, nb = s->nb, nf = s->nf;
+  struct {
+    const char *n;
+    double *p;
+    int64_t l;
+  } t[] = {{"U", s->U, 3 * nc}, {"p", s->P, nc}, {"p_bnd", s->p_bnd, nb}, {"U_bnd", s->U_bnd, 3 * nb},
+           {"nu_sgs", s->nu_sgs, nc}, {"grad_U", s->grad_U, 9 * nc}, {"U_face", s->U_face, nf},
+           {"face_flux", s->face_flux, nf}, {"tau_w_bnd", s->tau_w, 3 * nb}, {"w_gp", s->gp, 3 * nc},
+           {"rhs_p", s->rhs_p, nc}, {"H", s->H, 3 * nc}, {"Gf", s->Gf, 9 * nc}, {"Tinv", s->Tinv, 9 * nc},
+           {"ps_coeff", s->ps_coeff, s->npf}, {"ps_diag", s->ps_diag, nc}};
+  for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i)
+    if (strcmp(t[i].n, name) == 0) {
+      if (len) *len = t[i].l;
+      return t[i].p;
+    }
+  if (len) *len = 0;
+  return NULL;
+}
+
+double om_time(const om_solver *s) { return s->time; }
+
+void om_counters(const om_solver *s, int64_t out[3]) {
+  out[0] = s->n_solves;
+  out[1] = s->n_iters;
+  out[2] = s->last_iters;
+}
+
+int om_nonortho_info(const om_solver *s, int out[3]) {
+  out[0] = s->skip_nonortho;
+  out[1] = s->skip_grad_p;
+  out[2] = s->n_corr;
+  return 0;
+}

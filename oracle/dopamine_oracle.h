@@ -1,0 +1,95 @@
+/* ORACLE — test infrastructure only.
+ *
+ * FP64 CPU restatement of DOPAMINE's RK3W fractional-step hot path, used as the parity checker
+ * for the HIP product path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it. Nothing in dopamine_amd/ links or calls it.
+ *
+ * It is written as the reference's own sequential face-scatter loops (one face at a time,
+ * owner/neighbour accumulation), each routine citing the reference file:line it follows.
+ * Indices are 0-based; a face neighbour of -1 marks a boundary face.
+ */
+#ifndef DOPAMINE_ORACLE_H
+#define DOPAMINE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mesh arrays (caller-owned, read-only), same meaning as the mesh builder's view. */
+typedef struct om_mesh {
+  int32_t n_cells, n_faces, n_int_faces, n_bnd_faces, n_patches;
+  const double *cell_centroid, *cell_volume;
+  const int32_t *cell_face_offset, *cell_faces;
+  const int32_t *face_owner, *face_neighbour, *face_patch;
+  const double *face_area, *face_centre, *face_normal, *face_sf, *face_dcf, *face_gc, *face_skew, *face_eps;
+  const int32_t *patch_bc, *patch_start, *patch_nfaces;
+  const int32_t *periodic_pair;
+  const double *periodic_pair_dcf, *periodic_gc;
+  const int32_t *ls_offset, *ls_cell, *ls_face;
+  const double *ls_weight;
+} om_mesh;
+
+enum { OM_SCHEME_CDS = 3, OM_SCHEME_KEP = 4 };
+enum { OM_GRAD_LS = 0, OM_GRAD_GG = 1 };
+enum { OM_SGS_NONE = 0, OM_SGS_VREMAN = 1, OM_SGS_SMAGORINSKY = 2 };
+enum { OM_WM_LOGLAW = 0, OM_WM_NONE = 1 };
+
+typedef struct om_params {
+  double nu;
+  double f_const[3];
+  int scheme;            /* OM_SCHEME_* */
+  int gradient;          /* OM_GRAD_* */
+  int gg_iterations;
+  int sgs;               /* OM_SGS_* */
+  double c_vreman;       /* reference value 0.025 */
+  double cs_smag;        /* 0.1 */
+  int wall_model;        /* OM_WM_* */
+  double wall_model_height;
+  int n_nonortho_corr;   /* 0 = auto */
+  double min_cos_theta;
+  int rhie_chow_dual;
+  int nonortho_check;
+  int rotational;
+  double p_tol;
+  int p_max_iter;
+  double cfl_max;
+  double dt;             /* requested (maximum) time step */
+  const double *patch_p_val;  /* n_patches, outlet pressure */
+  const double *patch_U_val;  /* 3*n_patches, inlet / moving-wall velocity */
+} om_params;
+
+typedef struct om_solver om_solver;
+
+om_solver *om_create(const om_mesh *m, const om_params *p);
+void om_free(om_solver *s);
+/* Pointer to an internal array by name ("U","p","p_bnd","U_bnd","nu_sgs","grad_U","U_face",
+ * "face_flux","tau_w_bnd","w_gp","rhs_p"), with its length in doubles. */
+double *om_array(om_solver *s, const char *name, int64_t *len);
+/* After U (and p) were written from outside: refresh boundary values as solver_init does. */
+void om_apply_all_bc(om_solver *s);
+int om_project_once(om_solver *s, double dt);
+/* One solver_advance: CFL clamp of dt_req, RK3W step; *dt_used receives the step taken. */
+int om_advance(om_solver *s, double dt_req, double *dt_used);
+/* compute_stats_sums + divergence metrics: out = {E, eps, Umax, Ub, tauw_mean, nu_sgs_mean,
+ * max_abs_divU, volavg_abs_divU}. */
+void om_stats(om_solver *s, double out[8]);
+double om_time(const om_solver *s);
+/* Counters: total pressure solves, total CG iterations, last CG iteration count. */
+void om_counters(const om_solver *s, int64_t out[3]);
+int om_nonortho_info(const om_solver *s, int out[3]); /* skip_nonortho, skip_grad_p, n_nonortho_corr */
+
+/* Standalone pressure solve with the reference's operator assembly convention
+ * (src/solver/pressure_backend_petsc.c:296-365): A_ii = diag, A_on = A_no = -coeff;
+ * Jacobi-preconditioned CG, unpreconditioned 2-norm stop ||r|| <= max(rtol*||b||, atol);
+ * constant null space projected from b and from each preconditioned residual when
+ * nullspace != 0. Returns 0 when converged, -1 otherwise. 0-based owner/neighbour. */
+int om_cg_jacobi(int n, int nfaces, const int32_t *owner, const int32_t *neighbour, const double *coeff,
+                 const double *diag, const double *b, double *x, double rtol, double atol, int max_it,
+                 int nullspace, int *its, double *resid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
