@@ -97,3 +97,10 @@ def channel(nx=128, ny=96, nz=64):
                      sgs_model="VREMAN", dt=1.0e-3, cfl_max=0.5, p_tol=1.0e-8, p_max_iter=1000,
                      n_nonortho_corr=0, rhie_chow_dual=False, cell_projection="LS", pressure_min_cos_theta=0.0),
         ic="channel_parabolic", ic_uref=1.0, nsteps=500)
+
+
+def small_channel(nx=16, ny=12, nz=8):
+    """Reduced channel (same physics as channel()) for parity tests."""
+    c = channel(nx, ny, nz)
+    c.name = f"channel{nx}x{ny}x{nz}"
+    return c

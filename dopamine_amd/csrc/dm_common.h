@@ -1,0 +1,104 @@
+// Shared definitions for the DOPAMINE hot-path kernels: constants, the reduction value type and
+// the execution policies that run kernel bodies.
+//
+// Kernel bodies are functors with a __host__ __device__ call operator. The product library runs
+// them through DeviceExec (HIP launches on gfx950). HostExec, compiled only into the test-side
+// host-check library, runs the same bodies in a loop so their indexing can be checked on a
+// machine without a GPU; it is never part of libdopamine_hip.so.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define DM_HD __host__ __device__
+#define DM_INLINE __host__ __device__ inline
+#else
+#define DM_HD
+#define DM_INLINE inline
+#endif
+
+namespace dm {
+
+#if !defined(__HIPCC__)
+using std::copysign;
+using std::exp;
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::isfinite;
+using std::log;
+using std::pow;
+using std::sqrt;
+#endif
+
+constexpr double SMALL = 1.0e-30;  // src/mod_parameters.f90:25
+constexpr int BLOCK = 256;         // 4 wavefronts of 64 lanes
+
+enum Bc : int {
+  BC_INLET = 1, BC_OUTLET = 2, BC_WALL_NOSLIP = 3, BC_WALL_MODEL = 4, BC_SLIP = 5, BC_PERIODIC = 6,
+  BC_SYMMETRY = 7, BC_FIXEDTEMP = 8, BC_ZEROGRADIENT = 9, BC_WALL_VELOCITY = 10, BC_ABL_INLET = 11
+};
+
+// Walls and slip planes carry no mass flux (mod_advection.f90:1154-1156).
+DM_INLINE bool is_solid(int bc) {
+  return bc == BC_WALL_NOSLIP || bc == BC_WALL_MODEL || bc == BC_WALL_VELOCITY || bc == BC_SLIP || bc == BC_SYMMETRY;
+}
+
+template <int N>
+struct Red {
+  double v[N];
+};
+
+template <int N, bool MAX>
+DM_INLINE Red<N> red_identity() {
+  Red<N> r;
+  for (int k = 0; k < N; ++k) r.v[k] = MAX ? -1.0e308 : 0.0;
+  return r;
+}
+
+inline int nblocks(int64_t n, int b = BLOCK) { return int((n + b - 1) / b); }
+
+// ------------------------------------------------------------------ host execution (tests only)
+struct HostExec {
+  static constexpr bool device = false;
+  void init(int) {}
+  void destroy() {}
+  template <class T>
+  T* alloc(size_t n) {
+    T* p = static_cast<T*>(std::calloc(n ? n : 1, sizeof(T)));
+    if (!p) throw std::runtime_error("host allocation failed");
+    return p;
+  }
+  void release(void* p) { std::free(p); }
+  template <class T>
+  void upload(T* dst, const T* src, size_t n) { if (n) std::memcpy(dst, src, n * sizeof(T)); }
+  template <class T>
+  void download(T* dst, const T* src, size_t n) { if (n) std::memcpy(dst, src, n * sizeof(T)); }
+  template <class T>
+  void copy(T* dst, const T* src, size_t n) { if (n) std::memmove(dst, src, n * sizeof(T)); }
+  template <class T>
+  void zero(T* dst, size_t n) { if (n) std::memset(dst, 0, n * sizeof(T)); }
+  template <class F>
+  void for_each(int n, const F& f) {
+    for (int i = 0; i < n; ++i) f(i);
+  }
+  template <int N, bool MAX, class F, class Fin>
+  void reduce(int n, const F& f, const Fin& fin, const int* gate = nullptr) {
+    if (gate && *gate) return;
+    Red<N> s = red_identity<N, MAX>();
+    for (int i = 0; i < n; ++i) {
+      const Red<N> x = f(i);
+      for (int k = 0; k < N; ++k) s.v[k] = MAX ? std::fmax(s.v[k], x.v[k]) : s.v[k] + x.v[k];
+    }
+    fin(s);
+  }
+  void sync() {}
+};
+
+}  // namespace dm

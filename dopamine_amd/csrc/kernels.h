@@ -1,0 +1,933 @@
+// Kernel bodies of the RK3W hot path, as functors (one call = one cell or one boundary face).
+//
+// Every face term is evaluated in gather form: a cell walks its own face list and adds each
+// face's contribution with the sign of its side, in the same order as the reference's
+// face-scatter loop would add it (ascending face order for the mesh-face loops, pressure-operator
+// order for the projection loops). No atomics; each output is written once by its owner.
+#pragma once
+
+#include "dm_common.h"
+
+namespace dm {
+
+// grad(i,j) = d u_j / d x_i stored like the reference grad_U(3,3,nc): [9c + 3j + i]
+#define DM_G(c, i, j) (9 * size_t(c) + 3 * (j) + (i))
+
+struct DevMesh {
+  int nc, nf, nint, nb, npf, wA, wB, wL;
+  // cells
+  const double *vol, *iv, *ctr;                   // ctr: 3 per cell
+  const int *nA, *aface, *aoth;                   // mesh-face list, ELL wA x nc (slot-major)
+  const int *nB, *bface, *both, *bidx;            // projection-face list, ELL wB x nc
+  const int *nL, *lidx;                           // LS stencil, ELL wL x nc
+  const double* lw;                               // LS weights: plane (3k+i) of nc values
+  // faces
+  const double *sf, *gc, *area, *nrm, *eps, *wbl, *fctr, *dcf;
+  // boundary faces
+  const int *bbc, *bpatch, *bown, *pnb, *pair;
+  const double *pgc, *pdcf, *cb, *kbnd, *wsy;
+  const int* wsc;
+  // pressure operator faces (interior + canonical periodic edges)
+  const double *pcoef, *rcgc, *xgc, *kint;
+  const int *pown, *pnbr, *pfm;
+  // per-cell static operators
+  const double *Tinv, *Gf;
+  // patch values
+  const double *patchU;
+};
+
+DM_INLINE double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// refine_blend_weight (src/mesh/mod_mesh.f90:491-500)
+DM_INLINE double refine_blend_weight(double skew) {
+  double t = (skew - 0.15) / (0.25 - 0.15);
+  t = fmax(0.0, fmin(1.0, t));
+  return 1.0 - t * t * (3.0 - 2.0 * t);
+}
+
+// ortho_weight (src/numerics/mod_viscous.f90:30-34)
+DM_INLINE double ortho_weight(double s_dot_dhat, double area) {
+  return area * fmax(0.0, fmin(1.0, s_dot_dhat / (area + SMALL)));
+}
+
+// nonortho_cross_flux (src/numerics/mod_nonortho.f90:120-128)
+DM_INLINE double cross_flux(const double* go, const double* gn, double gc, const double* k) {
+  double v[3];
+  for (int i = 0; i < 3; ++i) v[i] = gc * go[i] + (1.0 - gc) * gn[i];
+  return dot3(v, k);
+}
+
+// ------------------------------------------------------------------ boundary conditions
+
+// apply_velocity_bc (src/bc/mod_bc.f90:196-379), one boundary face. filter: 0 all, 1 non-periodic,
+// 2 periodic only. gU may be null (zeroth-order extrapolation).
+struct BcVelocity {
+  DevMesh M;
+  const double *U, *gU;
+  double* Ub;
+  int filter;
+  DM_HD void operator()(int b) const {
+    const int f = M.nint + b, bc = M.bbc[b];
+    if (filter == 2 && bc != BC_PERIODIC) return;
+    if (filter == 1 && bc == BC_PERIODIC) return;
+    const int c = M.bown[b];
+    const double* n = &M.nrm[3 * size_t(f)];
+    const double* uo = &U[3 * size_t(c)];
+    double* ub = &Ub[3 * size_t(b)];
+    switch (bc) {
+      case BC_INLET: {
+        const double* uv = &M.patchU[3 * size_t(M.bpatch[b])];
+        ub[0] = uv[0]; ub[1] = uv[1]; ub[2] = uv[2];
+        break;
+      }
+      case BC_OUTLET:
+      case BC_ZEROGRADIENT: {
+        double u[3] = {uo[0], uo[1], uo[2]};
+        if (gU) {
+          const double d[3] = {M.fctr[3 * size_t(f)] - M.ctr[3 * size_t(c)], M.fctr[3 * size_t(f) + 1] - M.ctr[3 * size_t(c) + 1],
+                               M.fctr[3 * size_t(f) + 2] - M.ctr[3 * size_t(c) + 2]};
+          for (int k = 0; k < 3; ++k)
+            u[k] = u[k] + gU[DM_G(c, 0, k)] * d[0] + gU[DM_G(c, 1, k)] * d[1] + gU[DM_G(c, 2, k)] * d[2];
+        }
+        ub[0] = u[0]; ub[1] = u[1]; ub[2] = u[2];
+        break;
+      }
+      case BC_WALL_NOSLIP:
+      case BC_WALL_MODEL:
+        ub[0] = 0.0; ub[1] = 0.0; ub[2] = 0.0;
+        break;
+      case BC_WALL_VELOCITY: {
+        const double* uv = &M.patchU[3 * size_t(M.bpatch[b])];
+        const double un = uv[0] * n[0] + uv[1] * n[1] + uv[2] * n[2];
+        for (int k = 0; k < 3; ++k) ub[k] = uv[k] - un * n[k];
+        break;
+      }
+      case BC_PERIODIC: {
+        double u[3] = {uo[0], uo[1], uo[2]};
+        const int nb = M.pnb[b];
+        if (nb >= 0) {
+          const double w = M.pgc[b];
+          for (int k = 0; k < 3; ++k) u[k] = w * u[k] + (1.0 - w) * U[3 * size_t(nb) + k];
+        }
+        ub[0] = u[0]; ub[1] = u[1]; ub[2] = u[2];
+        break;
+      }
+      case BC_SLIP:
+      case BC_SYMMETRY: {
+        const double un = uo[0] * n[0] + uo[1] * n[1] + uo[2] * n[2];
+        for (int k = 0; k < 3; ++k) ub[k] = uo[k] - un * n[k];
+        break;
+      }
+      default:
+        break;
+    }
+  }
+};
+
+// apply_pressure_bc (src/bc/mod_bc.f90:383-486). phi = a + b (b null: phi = a) lets the same body
+// serve the total pressure phi_prev + increment. gp may be null; pob null -> patch p_val (scale 0
+// for an increment).
+struct BcPressure {
+  DevMesh M;
+  const double *pa, *pb_add, *gp, *pob, *patchP;
+  double* pbnd;
+  double outlet_scale;
+  int filter;
+  DM_HD double val(int c) const { return pb_add ? pa[c] + pb_add[c] : pa[c]; }
+  DM_HD void operator()(int b) const {
+    const int f = M.nint + b, bc = M.bbc[b];
+    if (filter == 2 && bc != BC_PERIODIC) return;
+    if (filter == 1 && bc == BC_PERIODIC) return;
+    const int o = M.bown[b];
+    switch (bc) {
+      case BC_OUTLET:
+        pbnd[b] = outlet_scale * (pob ? pob[b] : patchP[M.bpatch[b]]);
+        break;
+      case BC_PERIODIC: {
+        const int nb = M.pnb[b];
+        pbnd[b] = (nb >= 0) ? M.pgc[b] * val(o) + (1.0 - M.pgc[b]) * val(nb) : val(o);
+        break;
+      }
+      case BC_WALL_NOSLIP:
+      case BC_WALL_VELOCITY:
+      case BC_WALL_MODEL:
+      case BC_SLIP:
+      case BC_SYMMETRY: {
+        // Kim-Moin Neumann with the default a_wall = 0
+        const double* n = &M.nrm[3 * size_t(f)];
+        const double an = 0.0 * n[0] + 0.0 * n[1] + 0.0 * n[2];
+        double d[3];
+        for (int i = 0; i < 3; ++i) d[i] = M.fctr[3 * size_t(f) + i] - M.ctr[3 * size_t(o) + i];
+        const double d_ortho = fabs(d[0] * n[0] + d[1] * n[1] + d[2] * n[2]);
+        pbnd[b] = val(o) - an * d_ortho;
+        break;
+      }
+      default:
+        if (gp) {
+          double d[3];
+          for (int i = 0; i < 3; ++i) d[i] = M.fctr[3 * size_t(f) + i] - M.ctr[3 * size_t(o) + i];
+          pbnd[b] = val(o) + gp[3 * size_t(o)] * d[0] + gp[3 * size_t(o) + 1] * d[1] + gp[3 * size_t(o) + 2] * d[2];
+        } else {
+          pbnd[b] = val(o);
+        }
+    }
+  }
+};
+
+// ------------------------------------------------------------------ gradients and SGS
+
+// Vreman (src/sgs/mod_vreman.f90:50-153) and constant Smagorinsky (:295-364) for one cell.
+DM_INLINE double det3_colmajor(const double* A) {
+  return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) + A[6] * (A[1] * A[5] - A[4] * A[2]);
+}
+
+DM_INLINE double vreman_nu(const double* al, const double* G, double c_v, double vol, double dt_stab) {
+  double beta[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i <= j; ++i) {
+      double b = 0.0;
+      for (int n = 0; n < 3; ++n)
+        for (int m = 0; m < 3; ++m) b = b + G[3 * n + m] * al[3 * i + m] * al[3 * j + n];
+      beta[3 * j + i] = b;
+    }
+  double asq = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) asq = asq + al[3 * j + i] * al[3 * j + i];
+  const double Bb = beta[0] * beta[4] - beta[3] * beta[3] + beta[0] * beta[8] - beta[6] * beta[6] +
+                    beta[4] * beta[8] - beta[7] * beta[7];
+  const double nu = (asq > SMALL && Bb > 0.0) ? c_v * sqrt(Bb / asq) : 0.0;
+  const double cap = 0.4 * pow(vol, 2.0 / 3.0) / fmax(dt_stab, SMALL);
+  return fmin(nu, cap);
+}
+
+DM_INLINE double smagorinsky_nu(const double* g, const double* G, double cs, double dt_stab) {
+  double ss = 0.0;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      const double s = 0.5 * (g[3 * j + i] + g[3 * i + j]);
+      ss = ss + s * s;
+    }
+  const double Sn = sqrt(2.0 * ss);
+  const double delta = fmax(pow(fmax(det3_colmajor(G), 0.0), 1.0 / 6.0), SMALL);
+  const double csd = cs * delta;
+  const double nu = csd * csd * Sn;
+  const double cap = 0.4 * delta * delta / fmax(dt_stab, SMALL);
+  return fmin(nu, cap);
+}
+
+// ls_gradient_vector (src/numerics/mod_gradient.f90:425-556), optionally followed by the SGS
+// viscosity of the same cell (sgs: 0 none, 1 Vreman, 2 Smagorinsky, -1 leave nu_sgs alone).
+struct LsGradVec {
+  DevMesh M;
+  const double *U, *Ub;
+  double *grad, *nu_sgs;
+  int sgs;
+  double c_v, cs, dt_stab;
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    const double u0 = U[3 * size_t(c)], u1 = U[3 * size_t(c) + 1], u2 = U[3 * size_t(c) + 2];
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int cnt = M.nL[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int j = M.lidx[s];
+      const double* un = (j >= 0) ? &U[3 * size_t(j)] : &Ub[3 * size_t(-1 - j)];
+      const double du[3] = {un[0] - u0, un[1] - u1, un[2] - u2};
+      const double w0 = M.lw[(3 * size_t(k)) * nc + c], w1 = M.lw[(3 * size_t(k) + 1) * nc + c],
+                   w2 = M.lw[(3 * size_t(k) + 2) * nc + c];
+      for (int q = 0; q < 3; ++q) {
+        g[3 * q + 0] = g[3 * q + 0] + w0 * du[q];
+        g[3 * q + 1] = g[3 * q + 1] + w1 * du[q];
+        g[3 * q + 2] = g[3 * q + 2] + w2 * du[q];
+      }
+    }
+    for (int q = 0; q < 9; ++q) grad[9 * size_t(c) + q] = g[q];
+    if (sgs == 1) nu_sgs[c] = vreman_nu(g, &M.Gf[9 * size_t(c)], c_v, M.vol[c], dt_stab);
+    else if (sgs == 2) nu_sgs[c] = smagorinsky_nu(g, &M.Gf[9 * size_t(c)], cs, dt_stab);
+    else if (sgs == 0) nu_sgs[c] = 0.0;
+  }
+};
+
+// ls_gradient_scalar (src/numerics/mod_gradient.f90:307-422) of phi = a (+ a2) with boundary
+// values pb (+ pb2).
+struct LsGradSca {
+  DevMesh M;
+  const double *a, *a2, *pb, *pb2;
+  double* g;
+  DM_HD double cv(int c) const { return a2 ? a[c] + a2[c] : a[c]; }
+  DM_HD double bv(int b) const { return pb2 ? pb[b] + pb2[b] : pb[b]; }
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    const double pc = cv(c);
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    const int cnt = M.nL[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int j = M.lidx[s];
+      const double d = ((j >= 0) ? cv(j) : bv(-1 - j)) - pc;
+      g0 = g0 + M.lw[(3 * size_t(k)) * nc + c] * d;
+      g1 = g1 + M.lw[(3 * size_t(k) + 1) * nc + c] * d;
+      g2 = g2 + M.lw[(3 * size_t(k) + 2) * nc + c] * d;
+    }
+    g[3 * size_t(c)] = g0;
+    g[3 * size_t(c) + 1] = g1;
+    g[3 * size_t(c) + 2] = g2;
+  }
+};
+
+// green_gauss_gradient_vector (src/numerics/mod_gradient.f90:172-304), gprev null on the first sweep.
+struct GgGradVec {
+  DevMesh M;
+  const double *U, *Ub, *gprev;
+  double* grad;
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int cnt = M.nA[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
+      const double* sf = &M.sf[3 * size_t(f)];
+      double Uf[3];
+      if (oth >= 0) {
+        const int o = side ? oth : c, nb = side ? c : oth;
+        const double gc = M.gc[f];
+        if (gprev) {
+          const double* ep = &M.eps[3 * size_t(f)];
+          for (int q = 0; q < 3; ++q) {
+            double gm[3];
+            for (int i = 0; i < 3; ++i) gm[i] = gc * gprev[DM_G(o, i, q)] + (1.0 - gc) * gprev[DM_G(nb, i, q)];
+            Uf[q] = gc * U[3 * size_t(o) + q] + (1.0 - gc) * U[3 * size_t(nb) + q] + dot3(gm, ep);
+          }
+          const double w = M.wbl[f];
+          if (w < 1.0)
+            for (int q = 0; q < 3; ++q) {
+              const double pl = gc * U[3 * size_t(o) + q] + (1.0 - gc) * U[3 * size_t(nb) + q];
+              Uf[q] = pl + w * (Uf[q] - pl);
+            }
+        } else {
+          for (int q = 0; q < 3; ++q) Uf[q] = gc * U[3 * size_t(o) + q] + (1.0 - gc) * U[3 * size_t(nb) + q];
+        }
+      } else {
+        const int b = -1 - oth;
+        for (int q = 0; q < 3; ++q) Uf[q] = Ub[3 * size_t(b) + q];
+      }
+      for (int q = 0; q < 3; ++q)
+        for (int i = 0; i < 3; ++i) g[3 * q + i] = side ? g[3 * q + i] - Uf[q] * sf[i] : g[3 * q + i] + Uf[q] * sf[i];
+    }
+    if (M.vol[c] > SMALL) {
+      const double iv = 1.0 / M.vol[c];
+      for (int q = 0; q < 9; ++q) g[q] = g[q] * iv;
+    }
+    for (int q = 0; q < 9; ++q) grad[9 * size_t(c) + q] = g[q];
+  }
+};
+
+// green_gauss_gradient_scalar (src/numerics/mod_gradient.f90:38-168) of phi = a (+ a2).
+struct GgGradSca {
+  DevMesh M;
+  const double *a, *a2, *pb, *pb2, *gprev;
+  double* g;
+  DM_HD double cv(int c) const { return a2 ? a[c] + a2[c] : a[c]; }
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    double r[3] = {0, 0, 0};
+    const int cnt = M.nA[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
+      const double* sf = &M.sf[3 * size_t(f)];
+      double pf;
+      if (oth >= 0) {
+        const int o = side ? oth : c, nb = side ? c : oth;
+        const double gc = M.gc[f];
+        if (gprev) {
+          double gm[3];
+          for (int i = 0; i < 3; ++i) gm[i] = gc * gprev[3 * size_t(o) + i] + (1.0 - gc) * gprev[3 * size_t(nb) + i];
+          const double pl = gc * cv(o) + (1.0 - gc) * cv(nb);
+          pf = pl + dot3(gm, &M.eps[3 * size_t(f)]);
+          const double w = M.wbl[f];
+          if (w < 1.0) pf = pl + w * (pf - pl);
+        } else {
+          pf = gc * cv(o) + (1.0 - gc) * cv(nb);
+        }
+      } else {
+        const int b = -1 - oth;
+        pf = pb2 ? pb[b] + pb2[b] : pb[b];
+      }
+      for (int i = 0; i < 3; ++i) r[i] = side ? r[i] - pf * sf[i] : r[i] + pf * sf[i];
+    }
+    if (M.vol[c] > SMALL) {
+      const double iv = 1.0 / M.vol[c];
+      for (int i = 0; i < 3; ++i) r[i] = r[i] * iv;
+    }
+    for (int i = 0; i < 3; ++i) g[3 * size_t(c) + i] = r[i];
+  }
+};
+
+// SGS viscosity alone (after a Green-Gauss gradient).
+struct SgsCell {
+  DevMesh M;
+  const double* grad;
+  double* nu_sgs;
+  int sgs;
+  double c_v, cs, dt_stab;
+  DM_HD void operator()(int c) const {
+    const double* g = &grad[9 * size_t(c)];
+    if (sgs == 1) nu_sgs[c] = vreman_nu(g, &M.Gf[9 * size_t(c)], c_v, M.vol[c], dt_stab);
+    else if (sgs == 2) nu_sgs[c] = smagorinsky_nu(g, &M.Gf[9 * size_t(c)], cs, dt_stab);
+    else nu_sgs[c] = 0.0;
+  }
+};
+
+// ------------------------------------------------------------------ wall model
+
+// Reichardt law and Newton solve for u_tau (src/sgs/mod_wall_model.f90:50-115).
+DM_INLINE double wall_tau_mag(double u_par, double y_w, double nu) {
+  const double KAPPA = 0.41, BLOG = 5.2, A1I = 1.0 / 11.0, A2 = 0.33;
+  const double cbuf = BLOG - log(KAPPA) / KAPPA;
+  if (u_par <= SMALL || y_w <= SMALL) return 0.0;
+  double ut = fmax(0.01 * u_par, sqrt(nu * u_par / (y_w + SMALL)));
+  const double ywn = y_w * (1.0 / (nu + SMALL));
+  for (int it = 0; it < 50; ++it) {
+    const double yp = ut * ywn;
+    const double e1 = exp(-yp * A1I), e2 = exp(-A2 * yp);
+    const double up = log(1.0 + KAPPA * yp) / KAPPA + cbuf * (1.0 - e1 - yp * A1I * e2);
+    const double dup = 1.0 / (1.0 + KAPPA * yp) + cbuf * (A1I * e1 - A1I * e2 + yp * A1I * A2 * e2);
+    const double F = u_par / (ut + SMALL) - up;
+    const double dF = -u_par / (ut * ut + SMALL) - dup * ywn;
+    double un = ut - F / (dF + copysign(SMALL, dF));
+    un = fmin(fmax(un, 0.1 * ut), 10.0 * ut);
+    if (fabs(un - ut) < 1.0e-8 * un) {
+      ut = un;
+      break;
+    }
+    ut = un;
+  }
+  return ut * ut;
+}
+
+// apply_wall_model_bc without time filter (src/sgs/mod_wall_model.f90:176-313), one boundary face.
+struct WallModel {
+  DevMesh M;
+  const double* U;
+  double* tau;
+  double nu;
+  DM_HD void operator()(int b) const {
+    if (M.bbc[b] != BC_WALL_MODEL) return;
+    const int f = M.nint + b;
+    const double* n = &M.nrm[3 * size_t(f)];
+    const double* u = &U[3 * size_t(M.wsc[b])];
+    const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    const double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    const double mag = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    const double t = wall_tau_mag(mag, M.wsy[b], nu);
+    const double scl = -t / (mag + SMALL);
+    for (int k = 0; k < 3; ++k) tau[3 * size_t(b) + k] = scl * up[k];
+  }
+};
+
+// ------------------------------------------------------------------ explicit right-hand side
+
+// advection_rhs_vector_{kep,cds}_soa (src/numerics/mod_advection.f90:1013-1392) + viscous_rhs_vector
+// (src/numerics/mod_viscous.f90:37-283) + compute_explicit_rhs assembly with body force and wall
+// traction (src/solver/mod_fv_solver.f90:1617-1711, 2238-2275) + the RK3W predictor
+// u* = u + dt (alpha H + beta H_prev) (:1326-1331), for one cell. Writes H and u* (to Unew).
+template <int SCHEME>
+struct ExplicitRhs {
+  DevMesh M;
+  const double *U, *Ub, *UF, *grad, *nus, *tau, *H0;
+  double *H, *Unew;
+  double nu, f0, f1, f2, dt, a, bcoef;
+  int has_wm;
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    const double iv = M.iv[c];
+    const double* Uc = &U[3 * size_t(c)];
+    double cv[3] = {0, 0, 0}, vs[3] = {0, 0, 0};
+    double dw = 0.0;
+    const int cnt = M.nA[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
+      const double* sf = &M.sf[3 * size_t(f)];
+      if (oth >= 0) {
+        const int o = side ? oth : c, nb = side ? c : oth;
+        const double* Uo = &U[3 * size_t(o)];
+        const double* Un = &U[3 * size_t(nb)];
+        const double Fn = UF[f];
+        // convective flux
+        if (SCHEME == 4) {
+          dw = side ? dw - Fn : dw + Fn;
+          for (int q = 0; q < 3; ++q) {
+            const double phi = 0.5 * (Uo[q] + Un[q]);
+            cv[q] = side ? cv[q] + Fn * phi * iv : cv[q] - Fn * phi * iv;
+          }
+        } else {
+          const double gcf = M.gc[f];
+          const double* ep = &M.eps[3 * size_t(f)];
+          const double w = M.wbl[f];
+          for (int q = 0; q < 3; ++q) {
+            const double pl = gcf * Uo[q] + (1.0 - gcf) * Un[q];
+            double gm[3];
+            for (int i = 0; i < 3; ++i) gm[i] = gcf * grad[DM_G(o, i, q)] + (1.0 - gcf) * grad[DM_G(nb, i, q)];
+            double phi = pl + dot3(gm, ep);
+            if (w < 1.0) phi = pl + w * (phi - pl);
+            if (!isfinite(phi)) phi = 0.0;
+            cv[q] = side ? cv[q] + Fn * phi * iv : cv[q] - Fn * phi * iv;
+          }
+        }
+        // viscous stress
+        const double gc = M.gc[f];
+        const double nue = nu + gc * nus[o] + (1.0 - gc) * nus[nb];
+        double gf[9];
+        for (int q = 0; q < 9; ++q) gf[q] = gc * grad[9 * size_t(o) + q] + (1.0 - gc) * grad[9 * size_t(nb) + q];
+        double d_on[3], dnp[3];
+        for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
+        const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+        const double sdd = dot3(sf, d_on) / dlen;
+        for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dlen) * d_on[i];
+        const double wo = ortho_weight(sdd, M.area[f]);
+        for (int q = 0; q < 3; ++q) {
+          const double col = gf[3 * q] * dnp[0] + gf[3 * q + 1] * dnp[1] + gf[3 * q + 2] * dnp[2];
+          const double row = gf[q] * sf[0] + gf[3 + q] * sf[1] + gf[6 + q] * sf[2];
+          const double st = nue * ((Un[q] - Uo[q]) * wo / dlen + col + row);
+          vs[q] = side ? vs[q] - st * iv : vs[q] + st * iv;
+        }
+      } else {
+        const int b = -1 - oth;
+        const int bc = M.bbc[b];
+        const double* ub = &Ub[3 * size_t(b)];
+        const int nbp = (bc == BC_PERIODIC) ? M.pnb[b] : -1;
+        // convective flux (owner side)
+        double Fn;
+        if (bc == BC_PERIODIC || bc == BC_OUTLET) Fn = UF[f];
+        else if (is_solid(bc)) Fn = 0.0;
+        else Fn = ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2];
+        if (SCHEME == 4) {
+          dw = dw + Fn;
+          for (int q = 0; q < 3; ++q) {
+            const double phi = (nbp >= 0) ? 0.5 * (Uc[q] + U[3 * size_t(nbp) + q]) : ub[q];
+            cv[q] = cv[q] - Fn * phi * iv;
+          }
+        } else {
+          for (int q = 0; q < 3; ++q) {
+            double phi;
+            if (nbp >= 0) {
+              const double g = M.pgc[b];
+              phi = g * Uc[q] + (1.0 - g) * U[3 * size_t(nbp) + q];
+            } else {
+              phi = ub[q];
+            }
+            if (!isfinite(phi)) phi = 0.0;
+            cv[q] = cv[q] - Fn * phi * iv;
+          }
+        }
+        // viscous stress (owner side)
+        if (bc == BC_PERIODIC) {
+          if (nbp >= 0) {
+            const int pf = M.nint + M.pair[b];
+            const double d_per = M.dcf[f] + M.dcf[pf];
+            const double gcp = M.dcf[pf] / (d_per + SMALL);
+            const double nue = nu + gcp * nus[c] + (1.0 - gcp) * nus[nbp];
+            const double* nh = &M.nrm[3 * size_t(f)];
+            double d_on[3], dnp[3], gf[9];
+            for (int i = 0; i < 3; ++i) d_on[i] = d_per * nh[i];
+            const double dlen = d_per + SMALL;
+            const double sdd = dot3(sf, d_on) / dlen;
+            for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dlen) * d_on[i];
+            for (int q = 0; q < 9; ++q) gf[q] = gcp * grad[9 * size_t(c) + q] + (1.0 - gcp) * grad[9 * size_t(nbp) + q];
+            const double wo = ortho_weight(sdd, M.area[f]);
+            for (int q = 0; q < 3; ++q) {
+              const double col = gf[3 * q] * dnp[0] + gf[3 * q + 1] * dnp[1] + gf[3 * q + 2] * dnp[2];
+              const double row = gf[q] * sf[0] + gf[3 + q] * sf[1] + gf[6 + q] * sf[2];
+              const double st = nue * ((U[3 * size_t(nbp) + q] - Uc[q]) * wo / dlen + col + row);
+              vs[q] = vs[q] + st * iv;
+            }
+          } else {
+            const double nue = nu + nus[c];
+            for (int q = 0; q < 3; ++q) {
+              const double st = nue * ((ub[q] - Uc[q]) / (M.dcf[f] + SMALL)) * M.area[f];
+              vs[q] = vs[q] + st * iv;
+            }
+          }
+        } else if (bc != BC_WALL_MODEL) {
+          const double nue = nu + nus[c];
+          double d_on[3], dnp[3];
+          for (int i = 0; i < 3; ++i) d_on[i] = M.fctr[3 * size_t(f) + i] - M.ctr[3 * size_t(c) + i];
+          const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+          const double sdd = dot3(sf, d_on) / dlen;
+          for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dlen) * d_on[i];
+          const double* go = &grad[9 * size_t(c)];
+          const double wo = ortho_weight(sdd, M.area[f]);
+          for (int q = 0; q < 3; ++q) {
+            const double col = go[3 * q] * dnp[0] + go[3 * q + 1] * dnp[1] + go[3 * q + 2] * dnp[2];
+            const double row = go[q] * sf[0] + go[3 + q] * sf[1] + go[6 + q] * sf[2];
+            const double st = nue * ((ub[q] - Uc[q]) * wo / dlen + col + row);
+            vs[q] = vs[q] + st * iv;
+          }
+        }
+      }
+    }
+    if (SCHEME == 4)
+      for (int q = 0; q < 3; ++q) cv[q] = cv[q] + 0.5 * Uc[q] * dw * iv;
+    double h[3] = {cv[0] + vs[0] + f0, cv[1] + vs[1] + f1, cv[2] + vs[2] + f2};
+    if (has_wm) {
+      for (int k = 0; k < cnt; ++k) {
+        const size_t s = size_t(k) * nc + c;
+        const int oth = M.aoth[s];
+        if (oth >= 0) continue;
+        const int b = -1 - oth;
+        if (M.bbc[b] != BC_WALL_MODEL) continue;
+        const int f = M.aface[s] >> 1;
+        const double w = M.area[f] / M.vol[c];
+        for (int q = 0; q < 3; ++q) h[q] = h[q] + tau[3 * size_t(b) + q] * w;
+      }
+    }
+    for (int q = 0; q < 3; ++q) {
+      H[3 * size_t(c) + q] = h[q];
+      Unew[3 * size_t(c) + q] = Uc[q] + dt * (a * h[q] + bcoef * H0[3 * size_t(c) + q]);
+    }
+  }
+};
+
+// ------------------------------------------------------------------ projection
+
+// compute_face_fluxes_and_rhs (src/solver/mod_fv_solver.f90:2389-2752) for one cell: provisional
+// face fluxes with the Rhie-Chow term (written by the owner side), rhs_p = -(1/dt) sum q + deferred
+// non-orthogonal cross terms. Returns |rhs_p| for the max reduction.
+struct FluxRhs {
+  DevMesh M;
+  const double *U, *Ub, *grad, *gprc, *phi, *gp;
+  double *ff, *rhs;
+  double dt;
+  int nonortho;  // deferred cross terms on (mesh not orthogonal)
+  int rck;       // Rhie-Chow subtracts the k-vector (non-orthogonal mesh, rhie_chow_dual off)
+  DM_HD Red<1> operator()(int c) const {
+    const int nc = M.nc;
+    double acc = 0.0;
+    const int cnt = M.nB[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.bface[s], fm = e >> 1, side = e & 1, oth = M.both[s], idx = M.bidx[s];
+      const double* sf = &M.sf[3 * size_t(fm)];
+      if (idx >= 0) {
+        const int o = side ? oth : c, nb = side ? c : oth;
+        const double gcf = M.rcgc[idx];
+        double Upl[3], Uf[3];
+        for (int q = 0; q < 3; ++q) Upl[q] = gcf * U[3 * size_t(o) + q] + (1.0 - gcf) * U[3 * size_t(nb) + q];
+        if (fm < M.nint) {
+          const double wb = M.wbl[fm], g = M.gc[fm];
+          const double* ep = &M.eps[3 * size_t(fm)];
+          for (int j = 0; j < 3; ++j) {
+            double ue = g * U[3 * size_t(o) + j] + (1.0 - g) * U[3 * size_t(nb) + j];
+            double gm[3];
+            for (int i = 0; i < 3; ++i) gm[i] = g * grad[DM_G(o, i, j)] + (1.0 - g) * grad[DM_G(nb, i, j)];
+            ue = ue + gm[0] * ep[0] + gm[1] * ep[1] + gm[2] * ep[2];
+            Uf[j] = Upl[j] + wb * (ue - Upl[j]);
+          }
+        } else {
+          Uf[0] = Upl[0]; Uf[1] = Upl[1]; Uf[2] = Upl[2];
+        }
+        double flux = Uf[0] * sf[0] + Uf[1] * sf[1] + Uf[2] * sf[2];
+        double gm[3], smk[3];
+        const double* kr = &M.kint[3 * size_t(idx)];
+        for (int i = 0; i < 3; ++i) {
+          gm[i] = gcf * gprc[3 * size_t(o) + i] + (1.0 - gcf) * gprc[3 * size_t(nb) + i];
+          smk[i] = sf[i] - (rck ? kr[i] : 0.0);
+        }
+        const double rc = dot3(gm, smk) - M.pcoef[idx] * (phi[nb] - phi[o]);
+        flux = flux + dt * rc;
+        if (side) {
+          acc = acc + flux;
+        } else {
+          ff[fm] = flux;
+          acc = acc - flux;
+        }
+      } else {
+        const int b = -1 - oth;
+        const int bc = M.bbc[b];
+        const double* ub = &Ub[3 * size_t(b)];
+        const double flux = is_solid(bc) ? 0.0 : ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2];
+        ff[fm] = flux;
+        acc = acc - flux;
+      }
+    }
+    double r = acc * (1.0 / fmax(dt, SMALL));
+    if (nonortho) {
+      for (int k = 0; k < cnt; ++k) {
+        const size_t s = size_t(k) * nc + c;
+        const int idx = M.bidx[s];
+        if (idx < 0) continue;
+        const int side = M.bface[s] & 1, oth = M.both[s];
+        const int o = side ? oth : c, nb = side ? c : oth;
+        const double cr = cross_flux(&gp[3 * size_t(o)], &gp[3 * size_t(nb)], M.xgc[idx], &M.kint[3 * size_t(idx)]);
+        r = side ? r - cr : r + cr;
+      }
+      for (int k = 0; k < cnt; ++k) {
+        const size_t s = size_t(k) * nc + c;
+        if (M.bidx[s] >= 0) continue;
+        const int b = -1 - M.both[s];
+        if (M.bbc[b] != BC_OUTLET) continue;
+        r = r + dot3(&gp[3 * size_t(c)], &M.kbnd[3 * size_t(b)]);
+      }
+    }
+    rhs[c] = r;
+    Red<1> out;
+    out.v[0] = fabs(r);
+    return out;
+  }
+};
+
+// Non-canonical periodic faces take the negated flux of their canonical partner
+// (src/solver/mod_fv_solver.f90:2644-2672, 2871-2897). One boundary face.
+struct PeriodicAntisym {
+  DevMesh M;
+  double* ff;
+  DM_HD void operator()(int b) const {
+    if (M.bbc[b] != BC_PERIODIC) return;
+    const int pr = M.pair[b];
+    if (pr < 0) return;
+    const int f = M.nint + b, pf = M.nint + pr;
+    const int os = M.bown[b], op = M.bown[pr];
+    if (os == op) return;
+    if (os > op) ff[f] = -ff[pf];
+  }
+};
+
+// correct_face_fluxes (src/solver/mod_fv_solver.f90:2756-2870): items [0, npf) are pressure-operator
+// faces, items [npf, npf+nb) boundary faces (outlets only).
+struct CorrectFaceFlux {
+  DevMesh M;
+  const double *P, *gp, *pob;
+  double* ff;
+  double dt;
+  int ptot, nonortho;
+  DM_HD void operator()(int i) const {
+    if (i < M.npf) {
+      const int o = M.pown[i], nb = M.pnbr[i], fm = M.pfm[i];
+      if (o < 0) return;  // skipped face
+      ff[fm] = ff[fm] - dt * M.pcoef[i] * (P[nb] - P[o]);
+      if (nonortho)
+        ff[fm] = ff[fm] - dt * cross_flux(&gp[3 * size_t(o)], &gp[3 * size_t(nb)], M.xgc[i], &M.kint[3 * size_t(i)]);
+      return;
+    }
+    const int b = i - M.npf;
+    if (M.bbc[b] != BC_OUTLET) return;
+    const int f = M.nint + b, o = M.bown[b];
+    ff[f] = ff[f] - dt * M.cb[b] * ((ptot ? pob[b] : 0.0) - P[o]);
+    if (nonortho) ff[f] = ff[f] - dt * dot3(&gp[3 * size_t(o)], &M.kbnd[3 * size_t(b)]);
+  }
+};
+
+// correct_velocity_psolver_dual (src/solver/mod_fv_solver.f90:2927-3151) for one cell.
+struct CorrectVelocity {
+  DevMesh M;
+  const double *P, *gp, *pob;
+  double *U, *gpc_out;
+  double dt, fb0, fb1, fb2;
+  int do_cross, ptot;
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    double g3[3] = {0, 0, 0};
+    const int cnt = M.nB[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int idx = M.bidx[s];
+      if (idx < 0) continue;
+      const int e = M.bface[s], fm = e >> 1, side = e & 1, oth = M.both[s];
+      const int o = side ? oth : c, nb = side ? c : oth;
+      double g = M.pcoef[idx] * (P[nb] - P[o]);
+      if (do_cross) g = g + cross_flux(&gp[3 * size_t(o)], &gp[3 * size_t(nb)], M.xgc[idx], &M.kint[3 * size_t(idx)]);
+      const double* n = &M.nrm[3 * size_t(fm)];
+      for (int i = 0; i < 3; ++i) g3[i] = g3[i] + g * n[i];
+    }
+    for (int k = 0; k < cnt; ++k) {  // Dirichlet-pressure (outlet) faces
+      const size_t s = size_t(k) * nc + c;
+      if (M.bidx[s] >= 0) continue;
+      const int b = -1 - M.both[s];
+      if (M.bbc[b] != BC_OUTLET) continue;
+      const int f = M.nint + b;
+      double g = M.cb[b] * ((ptot ? pob[b] : 0.0) - P[c]);
+      if (do_cross) g = g + dot3(&gp[3 * size_t(c)], &M.kbnd[3 * size_t(b)]);
+      const double* n = &M.nrm[3 * size_t(f)];
+      for (int i = 0; i < 3; ++i) g3[i] = g3[i] + g * n[i];
+    }
+    if (!do_cross) {  // wall / slip / inlet faces carry dp/dn = f.n
+      for (int k = 0; k < cnt; ++k) {
+        const size_t s = size_t(k) * nc + c;
+        if (M.bidx[s] >= 0) continue;
+        const int b = -1 - M.both[s];
+        if (M.bbc[b] == BC_OUTLET) continue;
+        const int f = M.nint + b;
+        const double* n = &M.nrm[3 * size_t(f)];
+        const double d1 = fb0 - 0.0, d2 = fb1 - 0.0, d3 = fb2 - 0.0;
+        const double fn = (d1 * n[0] + d2 * n[1] + d3 * n[2]) * M.area[f];
+        for (int i = 0; i < 3; ++i) g3[i] = g3[i] + fn * n[i];
+      }
+    }
+    const double* T = &M.Tinv[9 * size_t(c)];
+    double g[3];
+    for (int i = 0; i < 3; ++i) g[i] = T[3 * i] * g3[0] + T[3 * i + 1] * g3[1] + T[3 * i + 2] * g3[2];
+    for (int i = 0; i < 3; ++i) {
+      if (gpc_out) gpc_out[3 * size_t(c) + i] = g[i];
+      U[3 * size_t(c) + i] = U[3 * size_t(c) + i] - dt * g[i];
+    }
+  }
+};
+
+// ------------------------------------------------------------------ reductions
+
+// Volume-weighted mean speed for the divergence tolerance (src/solver/mod_fv_solver.f90:1424-1437).
+struct BulkSpeed {
+  DevMesh M;
+  const double* U;
+  DM_HD Red<2> operator()(int c) const {
+    const double* u = &U[3 * size_t(c)];
+    Red<2> r;
+    r.v[0] = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * M.vol[c];
+    r.v[1] = M.vol[c];
+    return r;
+  }
+};
+
+// Rotational correction p += (nu + nu_t) rhs_p gk_dt / V, then p += p_old (src/solver/mod_fv_solver.f90:
+// 1549-1571); returns p V for the mean removal of anchor_pressure_to_outlet (:3610-3639).
+struct RotAccumulate {
+  DevMesh M;
+  double* P;
+  const double *p_old, *nus, *rhs;
+  double nu, gdt;
+  int rotational;
+  DM_HD Red<1> operator()(int c) const {
+    double p = P[c];
+    if (rotational) p = p + (nu + nus[c]) * rhs[c] * gdt * M.iv[c];
+    p = p + p_old[c];
+    P[c] = p;
+    Red<1> r;
+    r.v[0] = p * M.vol[c];
+    return r;
+  }
+};
+
+struct ShiftP {
+  double* P;
+  const double* shift;  // device scalar
+  DM_HD void operator()(int c) const { P[c] = P[c] - *shift; }
+};
+
+// solver_cfl_rates (src/solver/mod_fv_solver.f90:3718-3862) for one cell.
+struct CflRates {
+  DevMesh M;
+  const double *U, *UF, *nus;
+  double nu;
+  int have_uface;
+  DM_HD Red<2> operator()(int c) const {
+    const int nc = M.nc;
+    double fs = 0.0, ds = 0.0;
+    const int cnt = M.nA[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
+      const double* sf = &M.sf[3 * size_t(f)];
+      const double area = sqrt(sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2]);
+      const int o = (oth >= 0 && side) ? oth : c;
+      const double uf = have_uface ? UF[f] : (U[3 * size_t(o)] * sf[0] + U[3 * size_t(o) + 1] * sf[1] + U[3 * size_t(o) + 2] * sf[2]);
+      fs = fs + fabs(uf);
+      const double w = area / fmax(M.dcf[f], SMALL);
+      if (oth >= 0) {
+        const int nb = side ? c : oth;
+        const double gc = M.gc[f];
+        const double nue = nu + gc * fmax(0.0, nus[o]) + (1.0 - gc) * fmax(0.0, nus[nb]);
+        ds = ds + nue * w;
+      } else if (M.bbc[-1 - oth] != BC_WALL_MODEL) {
+        ds = ds + (nu + fmax(0.0, nus[c])) * w;
+      }
+    }
+    Red<2> r;
+    r.v[0] = fs / (M.vol[c] + SMALL);
+    r.v[1] = ds / (M.vol[c] + SMALL);
+    return r;
+  }
+};
+
+// compute_stats_sums (src/stats/mod_stats.f90:271-336), cell part: E, V, eps, momentum, nu_t.
+struct StatsCells {
+  DevMesh M;
+  const double *U, *grad, *nus;
+  double nu;
+  DM_HD Red<7> operator()(int c) const {
+    const double vol = M.vol[c];
+    const double* u = &U[3 * size_t(c)];
+    const double* g = &grad[9 * size_t(c)];
+    const double S11 = g[0], S22 = g[4], S33 = g[8];
+    const double S12 = 0.5 * (g[1] + g[3]), S13 = 0.5 * (g[2] + g[6]), S23 = 0.5 * (g[5] + g[7]);
+    const double ss = S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23);
+    Red<7> r;
+    r.v[0] = 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * vol;
+    r.v[1] = vol;
+    r.v[2] = (nu + nus[c]) * ss * vol;
+    r.v[3] = u[0] * vol;
+    r.v[4] = u[1] * vol;
+    r.v[5] = u[2] * vol;
+    r.v[6] = nus[c] * vol;
+    return r;
+  }
+};
+
+struct StatsMax {
+  const double* U;
+  DM_HD Red<1> operator()(int c) const {
+    const double* u = &U[3 * size_t(c)];
+    Red<1> r;
+    r.v[0] = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    return r;
+  }
+};
+
+struct StatsWall {
+  DevMesh M;
+  const double* tau;
+  DM_HD Red<2> operator()(int b) const {
+    Red<2> r;
+    r.v[0] = 0.0;
+    r.v[1] = 0.0;
+    if (M.bbc[b] != BC_WALL_MODEL) return r;
+    const double* t = &tau[3 * size_t(b)];
+    const double a = M.area[M.nint + b];
+    r.v[0] = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) * a;
+    r.v[1] = a;
+    return r;
+  }
+};
+
+// face_flux_divergence (src/solver/mod_fv_solver.f90:3532-3579) for one cell: {|div|, |div| V}.
+// |face-flux divergence| of a cell; weighted = 1 multiplies it by the cell volume (the max and the
+// volume integral are two reductions).
+struct FluxDivergence {
+  DevMesh M;
+  const double* ff;
+  int weighted;
+  DM_HD Red<1> operator()(int c) const {
+    const int nc = M.nc;
+    double d = 0.0;
+    const int cnt = M.nB[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.bface[s], fm = e >> 1, side = e & 1;
+      const double fl = ff[fm];
+      if (M.bidx[s] >= 0)
+        d = side ? d - fl * M.iv[c] : d + fl * M.iv[c];
+    }
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      if (M.bidx[s] >= 0) continue;
+      d = d + ff[M.bface[s] >> 1] * M.iv[c];
+    }
+    Red<1> r;
+    r.v[0] = weighted ? fabs(d) * M.vol[c] : fabs(d);
+    return r;
+  }
+};
+
+}  // namespace dm

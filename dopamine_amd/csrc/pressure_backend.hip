@@ -1,0 +1,198 @@
+// Pressure-backend ABI (include/fvm_pressure_backend.h): the symbols of the reference's
+// src/solver/pressure_backend_petsc.c, served by the device-resident Jacobi CG on gfx950.
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <cstdlib>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cg_impl.h"
+#include "dm_device.h"
+#include "fvm_pressure_backend.h"
+
+namespace {
+
+std::string g_solver = "boomeramg";
+std::vector<std::string> g_options;
+int g_local_rank = -1;
+int g_io_on_device = 0;
+int g_initialized = 0;
+
+// Cached solve context: rebuilt when (n_local, n_global, n_faces, reference row) change
+// (pressure_backend_petsc.c:687-715).
+struct Ctx {
+  int n_local = 0, n_global = 0, n_faces = 0, ref_row = -1;
+  dm::DeviceExec x;
+  dm::CgT<dm::DeviceExec> cg;
+  dm::EllOp A;
+  int* nbr = nullptr;
+  double *coef = nullptr, *diag = nullptr, *b = nullptr, *sol = nullptr;
+  ~Ctx() {
+    cg.destroy();
+    for (void* p : {(void*)nbr, (void*)coef, (void*)diag, (void*)b, (void*)sol}) x.release(p);
+    x.destroy();
+  }
+};
+std::unique_ptr<Ctx> g_ctx;
+
+int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner, const int* nb_local,
+              const int* nb_global, const int* skip, const double* diagonal, const double* coeff, int ref_row) {
+  if (n_global != n) return -46;  // one rank: every row is local
+  for (int i = 0; i < n; ++i)
+    if (l2g[i] != i + 1) return -45;  // compact ids on one rank are the identity
+  std::vector<std::vector<std::pair<int, double>>> rows(n);
+  for (int f = 0; f < n_faces; ++f) {
+    if (skip[f] != 0 || nb_global[f] <= 0) continue;
+    const int o = owner[f] - 1;
+    if (o < 0 || o >= n) continue;
+    const double c = coeff[f];
+    if (!std::isfinite(c) || c == 0.0) continue;
+    const int g = nb_global[f] - 1;
+    if (g < 0 || g >= n) return -16;
+    rows[o].push_back({g, c});
+    if (nb_local[f] > 0) {
+      const int nbl = nb_local[f] - 1;
+      if (nbl < 0 || nbl >= n) return -17;
+      rows[nbl].push_back({o, c});
+    }
+  }
+  int w = 0;
+  for (auto& r : rows) w = std::max<int>(w, int(r.size()));
+  std::vector<int> nbr(size_t(std::max(1, w)) * n, -1);
+  std::vector<double> cf(size_t(std::max(1, w)) * n, 0.0), dg(n);
+  for (int i = 0; i < n; ++i) {
+    for (size_t k = 0; k < rows[i].size(); ++k) {
+      nbr[k * n + i] = rows[i][k].first;
+      cf[k * n + i] = rows[i][k].second;
+    }
+    double d = diagonal[i];
+    if (!std::isfinite(d) || std::fabs(d) < 1.0e-20) d = 1.0;
+    dg[i] = d;
+  }
+  auto ctx = std::make_unique<Ctx>();
+  ctx->x.init(n);
+  ctx->nbr = ctx->x.alloc<int>(nbr.size());
+  ctx->coef = ctx->x.alloc<double>(cf.size());
+  ctx->diag = ctx->x.alloc<double>(n);
+  ctx->b = ctx->x.alloc<double>(n);
+  ctx->sol = ctx->x.alloc<double>(n);
+  ctx->x.upload(ctx->nbr, nbr.data(), nbr.size());
+  ctx->x.upload(ctx->coef, cf.data(), cf.size());
+  ctx->x.upload(ctx->diag, dg.data(), dg.size());
+  ctx->A.n = n;
+  ctx->A.w = w;
+  ctx->A.nbr = ctx->nbr;
+  ctx->A.coef = ctx->coef;
+  ctx->A.diag = ctx->diag;
+  ctx->cg.init(&ctx->x, n);
+  ctx->n_local = n;
+  ctx->n_global = n_global;
+  ctx->n_faces = n_faces;
+  ctx->ref_row = ref_row;
+  g_ctx = std::move(ctx);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_petsc_backend_available(void) {
+  int n = 0;
+  return (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
+}
+
+int fvm_petsc_backend_initialize(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return -3;
+  if (g_local_rank >= 0 && hipSetDevice(g_local_rank % n) != hipSuccess) return -3;
+  g_initialized = 1;
+  return 0;
+}
+
+int fvm_petsc_backend_finalize(void) {
+  try {
+    g_ctx.reset();
+  } catch (...) {
+    return -20;
+  }
+  g_initialized = 0;
+  return 0;
+}
+
+void fvm_petsc_set_pressure_solver(const char* name) {
+  if (name) g_solver = name;
+}
+
+void fvm_petsc_insert_option(const char* opt) {
+  if (opt && opt[0]) g_options.emplace_back(opt);
+}
+
+void fvm_petsc_set_device_local_rank(int local_rank) { g_local_rank = local_rank; }
+
+int fvm_petsc_device_io_supported(void) { return 1; }
+
+void fvm_petsc_set_io_on_device(int on) { g_io_on_device = on; }
+
+int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces, const int* local_to_global,
+                             const int* face_owner, const int* face_neighbor_local, const int* face_neighbor_global,
+                             const int* skip_face, const double* diagonal, const double* rhs,
+                             const double* face_coeff, int reference_cell, int max_iterations, double rtol,
+                             double atol, const int* dirichlet_rows, const double* dirichlet_values,
+                             int n_dirichlet_rows, double* solution, int* iteration_count, double* residual_norm) {
+  (void)dirichlet_rows;
+  (void)dirichlet_values;
+  try {
+    if (n_cells_local <= 0) return -10;
+    int ref_row = reference_cell - 1;
+    if (ref_row < 0 || ref_row >= n_cells_global) ref_row = 0;
+    if (!g_ctx || g_ctx->n_local != n_cells_local || g_ctx->n_global != n_cells_global || g_ctx->n_faces != n_faces ||
+        g_ctx->ref_row != ref_row) {
+      g_ctx.reset();
+      const int rc = build_ctx(n_cells_local, n_cells_global, n_faces, local_to_global, face_owner,
+                               face_neighbor_local, face_neighbor_global, skip_face, diagonal, face_coeff, ref_row);
+      if (rc != 0) return rc;
+    }
+    Ctx& c = *g_ctx;
+    const int n = n_cells_local;
+    const double* b = rhs;
+    double* x = solution;
+    if (!g_io_on_device) {
+      c.x.upload(c.b, rhs, n);
+      c.x.upload(c.sol, solution, n);
+      b = c.b;
+      x = c.sol;
+    }
+    int maxit = max_iterations > 2000 ? max_iterations : 2000;
+    for (const auto& o : g_options)
+      if (o.rfind("-ksp_max_it", 0) == 0) {
+        const int v = std::atoi(o.c_str() + 11);
+        if (v > 0) maxit = v;
+      }
+    int its = 0;
+    double res = 0.0;
+    const int rc = c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, n_dirichlet_rows <= 0, &its, &res);
+    if (!g_io_on_device) c.x.download(solution, c.sol, n);
+    c.x.sync();
+    *iteration_count = its;
+    *residual_norm = res;
+    if (rc == 0) return 0;
+    // iteration limit (KSP_DIVERGED_ITS = -3): soft accept near tolerance (pressure_backend_petsc.c:917-920)
+    if (res < rtol * 1.0e4 || res < atol * 1.0e3) return 0;
+    return -(1000 + (-3));
+  } catch (const std::exception&) {
+    return -33;
+  }
+}
+
+int dopamine_mpi_gpu_aware(void) {
+  const char* env = std::getenv("DOPAMINE_GPU_AWARE_MPI");
+  if (env != nullptr && env[0] != '\0') return env[0] != '0';
+  return 0;
+}
+
+void dopamine_nvtx_push(const char* name) { roctxRangePushA(name); }
+void dopamine_nvtx_pop(void) { roctxRangePop(); }
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// C ABI of the HIP solver (include/dopamine_hip.h): SolverT over DeviceExec on gfx950.
+#include <string>
+
+#include "dm_device.h"
+#include "solver_impl.h"
+
+namespace {
+thread_local std::string g_error;
+using Solver = dm::SolverT<dm::DeviceExec>;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    g_error.clear();
+    return f();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -100;
+  }
+}
+
+struct FieldRef {
+  double* p;
+  int64_t n;
+};
+
+FieldRef field(Solver* s, const std::string& name) {
+  const int64_t nc = s->nc, nb = std::max(1, s->nb), nf = s->nf;
+  if (name == "U") return {s->U, 3 * nc};
+  if (name == "p") return {s->P, nc};
+  if (name == "nu_sgs") return {s->nu_sgs, nc};
+  if (name == "U_bnd") return {s->U_bnd, 3 * nb};
+  if (name == "p_bnd") return {s->p_bnd, nb};
+  if (name == "grad_U") return {s->grad, 9 * nc};
+  if (name == "U_face") return {s->UF, nf};
+  if (name == "tau_w_bnd") return {s->tau, 3 * nb};
+  if (name == "rhs_p") return {s->rhs, nc};
+  if (name == "H") return {s->H0, 3 * nc};  // after a step the last stage's H sits in H0
+  if (name == "w_gp") return {s->gp, 3 * nc};
+  throw std::runtime_error("unknown field " + name);
+}
+}  // namespace
+
+extern "C" {
+
+const char* dopamine_last_error(void) { return g_error.c_str(); }
+
+int dopamine_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params* p) {
+  Solver* s = nullptr;
+  const int rc = guard([&] {
+    if (!m || !p) throw std::runtime_error("null mesh or params");
+    if (dopamine_device_count() <= 0) throw std::runtime_error("no HIP device available");
+    if (p->wall_model_height > 0.0) throw std::runtime_error("wall_model_height > 0 (sampling walk) is not supported");
+    if (p->scheme != 3 && p->scheme != 4) throw std::runtime_error("advection scheme must be CDS (3) or KEP (4)");
+    s = new Solver();
+    s->x.init(std::max(m->n_cells, m->n_faces + m->n_bnd_faces));
+    s->create(*m, *p);
+    return 0;
+  });
+  if (rc != 0) {
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+void dopamine_solver_destroy(void* h) {
+  auto* s = static_cast<Solver*>(h);
+  if (!s) return;
+  try {
+    s->x.sync();
+  } catch (...) {
+  }
+  dm::DeviceExec x = s->x;
+  delete s;
+  x.destroy();
+}
+
+int dopamine_solver_set_field(void* h, const char* name, const double* host) {
+  return guard([&] {
+    auto* s = static_cast<Solver*>(h);
+    FieldRef f = field(s, name);
+    s->x.upload(f.p, host, size_t(f.n));
+    if (std::string(name) == "U") s->apply_all_bc();
+    s->x.sync();
+    return 0;
+  });
+}
+
+int dopamine_solver_get_field(void* h, const char* name, double* host, int64_t len) {
+  return guard([&] {
+    auto* s = static_cast<Solver*>(h);
+    FieldRef f = field(s, name);
+    if (len < f.n) throw std::runtime_error("output buffer too small for " + std::string(name));
+    s->x.download(host, f.p, size_t(f.n));
+    return 0;
+  });
+}
+
+int dopamine_solver_project_once(void* h, double dt) {
+  return guard([&] { return static_cast<Solver*>(h)->project_once(dt); });
+}
+
+int dopamine_solver_advance(void* h, double dt_req, double* dt_used) {
+  return guard([&] { return static_cast<Solver*>(h)->advance(dt_req, dt_used); });
+}
+
+double dopamine_solver_compute_dt(void* h) {
+  double dt = -1.0;
+  guard([&] {
+    dt = static_cast<Solver*>(h)->compute_dt();
+    return 0;
+  });
+  return dt;
+}
+
+int dopamine_solver_stats(void* h, double out[8]) {
+  return guard([&] {
+    static_cast<Solver*>(h)->stats(out);
+    return 0;
+  });
+}
+
+double dopamine_solver_time(void* h) { return static_cast<Solver*>(h)->time; }
+
+int dopamine_solver_counters(void* h, int64_t out[4]) {
+  auto* s = static_cast<Solver*>(h);
+  out[0] = s->n_solves;
+  out[1] = s->n_iters;
+  out[2] = s->last_iters;
+  out[3] = s->n_sweeps;
+  return 0;
+}
+
+int dopamine_solver_nonortho_info(void* h, int32_t out[3]) {
+  auto* s = static_cast<Solver*>(h);
+  out[0] = s->skip_nonortho;
+  out[1] = s->skip_grad_p;
+  out[2] = s->n_corr;
+  return 0;
+}
+
+int dopamine_solver_sync(void* h) {
+  return guard([&] {
+    static_cast<Solver*>(h)->x.sync();
+    return 0;
+  });
+}
+
+}  // extern "C"

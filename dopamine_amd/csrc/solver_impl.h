@@ -1,0 +1,927 @@
+// RK3W fractional-step driver over the hot-path kernels, templated on the execution policy.
+//
+// Set-up (host, once): builds the two per-cell face lists (mesh-face order for the transport terms,
+// pressure-operator order for the projection terms), the LS stencil in ELL form, the pressure
+// operator with its periodic edges, the non-orthogonal k-vectors, the reconstruction operator and
+// the Vreman filter tensor, exactly as solver_init / pressure_solver_init build them
+// (src/solver/mod_fv_solver.f90:302-507, src/solver/mod_pressure_solver.f90:100-452,
+// src/sgs/mod_vreman.f90:168-264), and copies everything to the execution space once.
+//
+// Stepping: advance() follows solver_advance + advance_rk3w (src/solver/mod_fv_solver.f90:920-957,
+// 1180-1614) call for call; every array stays resident between steps (to_device semantics,
+// :3286-3316); the host reads back only the scalars that steer the step (CFL dt, corrector exit,
+// CG convergence).
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "cg_impl.h"
+#include "dopamine_hip.h"
+#include "kernels.h"
+
+namespace dm {
+
+struct SolverScalars {
+  double divmax, ub1, ub2, atol, pmean, shift, cfl, dif;
+  double st[12];
+  double vmin, vtot, gdt, pad;
+};
+
+namespace sk {
+struct DivFin {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<1>& t) const { s->divmax = t.v[0]; }
+};
+struct AtolFin {
+  SolverScalars* s;
+  double gdt;
+  DM_HD void operator()(const Red<2>& t) const {
+    s->ub1 = t.v[0];
+    s->ub2 = t.v[1];
+    const double us = t.v[0] / fmax(t.v[1], SMALL);
+    s->atol = fmax(1.0e-12 * fmax(us, 1.0) * s->vmin / gdt, 1.0e-15);
+  }
+};
+struct AnchorFin {
+  SolverScalars* s;
+  int apply;
+  DM_HD void operator()(const Red<1>& t) const {
+    s->pmean = t.v[0];
+    s->shift = (apply && s->vtot > 0.0) ? t.v[0] / s->vtot : 0.0;
+  }
+};
+struct CflFin {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<2>& t) const {
+    s->cfl = t.v[0];
+    s->dif = t.v[1];
+  }
+};
+struct StatsFin7 {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<7>& t) const {
+    for (int k = 0; k < 7; ++k) s->st[k] = t.v[k];
+  }
+};
+struct StatsFinMax {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<1>& t) const { s->st[7] = t.v[0]; }
+};
+struct StatsFinWall {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<2>& t) const {
+    s->st[8] = t.v[0];
+    s->st[9] = t.v[1];
+  }
+};
+struct StatsFinDiv {
+  SolverScalars* s;
+  int slot;
+  DM_HD void operator()(const Red<1>& t) const { s->st[slot] = t.v[0]; }
+};
+struct Copy {
+  double* d;
+  const double* s;
+  DM_HD void operator()(int i) const { d[i] = s[i]; }
+};
+}  // namespace sk
+
+static const double RK3_A[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double RK3_B[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+static const double RK3_G[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+static const double RK3_C[3] = {0.0, 8.0 / 15.0, 2.0 / 3.0};
+
+// nonortho_kfactor_limited (src/numerics/mod_nonortho.f90:65-98)
+inline void kfactor_limited(const double sf[3], const double d_on[3], double min_cos, double k[3]) {
+  const double d_on_len = std::sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]);
+  if (d_on_len < 1.0e-30) {
+    k[0] = k[1] = k[2] = 0.0;
+    return;
+  }
+  const double inv_len = 1.0 / d_on_len;
+  const double sdd = (sf[0] * d_on[0] + sf[1] * d_on[1] + sf[2] * d_on[2]) * inv_len;
+  double d_len = sdd * d_on_len;
+  const double sf_sq = sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2];
+  if (min_cos > 0.0) d_len = std::fmax(d_len, min_cos * std::sqrt(sf_sq) * d_on_len);
+  if (d_len > 1.0e-30)
+    for (int i = 0; i < 3; ++i) k[i] = sf[i] - (sf_sq / d_len) * d_on[i];
+  else
+    for (int i = 0; i < 3; ++i) k[i] = sf[i] - (sdd * inv_len) * d_on[i];
+}
+
+template <class X>
+class SolverT {
+ public:
+  X x;
+  DevMesh M{};
+  dopamine_params prm{};
+  int nc = 0, nf = 0, nint = 0, nb = 0, npf = 0, npatch = 0;
+  int has_skipped = 0;
+  int skip_nonortho = 0, skip_grad_p = 0, n_corr = 1, diag_interface = 0, n_outlet = 0, n_wm = 0, n_dir = 0;
+  double time = 0.0;
+  int u_face_ready = 0;
+  int64_t n_solves = 0, n_iters = 0, last_iters = 0, n_sweeps = 0;
+
+  // fields (execution space)
+  double *U = nullptr, *U2 = nullptr, *P = nullptr, *p_old = nullptr, *phi_prev = nullptr, *p_bnd = nullptr;
+  double *phi_prev_bnd = nullptr, *U_bnd = nullptr, *nu_sgs = nullptr, *grad = nullptr, *gprev = nullptr;
+  double *gp = nullptr, *gp_rc = nullptr, *gp_prev = nullptr, *H = nullptr, *H0 = nullptr, *rhs = nullptr;
+  double *ff = nullptr, *UF = nullptr, *tau = nullptr, *outlet_pb = nullptr, *patchP = nullptr, *cg_b = nullptr;
+  double *phi_it = nullptr;
+  SolverScalars* sc = nullptr;
+  EllOp A;
+  CgT<X> cg;
+  std::vector<void*> owned;
+  std::vector<int> dir_rows_h;
+  std::vector<double> dir_vals_h;
+  int* dir_rows = nullptr;
+  double* dir_vals = nullptr;
+
+  template <class T>
+  T* up(const std::vector<T>& v) {
+    T* p = x.template alloc<T>(v.size());
+    x.upload(p, v.data(), v.size());
+    owned.push_back(p);
+    return p;
+  }
+  template <class T>
+  T* fresh(size_t n) {
+    T* p = x.template alloc<T>(n);
+    owned.push_back(p);
+    return p;
+  }
+
+  ~SolverT() {
+    cg.destroy();
+    for (void* p : owned) x.release(p);
+  }
+
+  void create(const dopamine_mesh_desc& d, const dopamine_params& p) {
+    prm = p;
+    nc = d.n_cells;
+    nf = d.n_faces;
+    nint = d.n_int_faces;
+    nb = d.n_bnd_faces;
+    npatch = d.n_patches;
+    validate(d);
+    std::vector<double> patchP_h(std::max(1, npatch), 0.0), patchU_h(3 * size_t(std::max(1, npatch)), 0.0);
+    for (int q = 0; q < npatch; ++q) {
+      patchP_h[q] = p.patch_p_val ? p.patch_p_val[q] : 0.0;
+      for (int k = 0; k < 3; ++k) patchU_h[3 * q + k] = p.patch_U_val ? p.patch_U_val[3 * q + k] : 0.0;
+    }
+    // ---- boundary faces
+    std::vector<int> bbc(std::max(1, nb)), bpatch(std::max(1, nb)), bown(std::max(1, nb)), pnb(std::max(1, nb), -1),
+        pair(std::max(1, nb), -1);
+    std::vector<double> pgc(std::max(1, nb)), pdcf(std::max(1, nb)), cb(std::max(1, nb), 0.0), wsy(std::max(1, nb), 0.0);
+    std::vector<int> wsc(std::max(1, nb));
+    for (int b = 0; b < nb; ++b) {
+      const int f = nint + b, pid = d.face_patch[f];
+      bpatch[b] = (pid >= 0 && pid < npatch) ? pid : 0;
+      bbc[b] = (pid >= 0 && pid < npatch) ? d.patch_bc[pid] : BC_ZEROGRADIENT;
+      bown[b] = d.face_owner[f];
+      pair[b] = d.periodic_pair[b];
+      pgc[b] = d.periodic_gc[b];
+      pdcf[b] = d.periodic_pair_dcf[b];
+      // periodic partner through the pair (resolve_periodic_nb, src/bc/mod_bc.f90:822-859)
+      if (pair[b] >= 0) {
+        const int nbc = d.face_owner[nint + pair[b]];
+        if (nbc >= 0 && nbc != bown[b]) pnb[b] = nbc;
+      }
+      // Dirichlet-face coefficient A / |(x_f - x_o).n| (src/solver/mod_fv_solver.f90:2851-2857)
+      const int o = bown[b];
+      const double* n = &d.face_normal[3 * size_t(f)];
+      double dv[3];
+      for (int i = 0; i < 3; ++i) dv[i] = d.face_centre[3 * size_t(f) + i] - d.cell_centroid[3 * size_t(o) + i];
+      const double sn = std::fabs(dv[0] * n[0] + dv[1] * n[1] + dv[2] * n[2]);
+      cb[b] = d.face_area[f] / std::fmax(sn, SMALL);
+      // wall-model sampling cell and height with h_wm <= 0 (find_sample_cell, mod_wall_model.f90:124-173)
+      wsc[b] = o;
+      double dd[3];
+      for (int i = 0; i < 3; ++i) dd[i] = d.cell_centroid[3 * size_t(o) + i] - d.face_centre[3 * size_t(f) + i];
+      wsy[b] = std::fabs(-(dd[0] * n[0]) - dd[1] * n[1] - dd[2] * n[2]) + SMALL;
+      if (bbc[b] == BC_OUTLET) n_outlet++;
+    }
+    for (int q = 0; q < npatch; ++q)
+      if (d.patch_bc[q] == BC_WALL_MODEL) n_wm++;
+    for (int f = 0; f < nint; ++f)
+      if (d.face_skew[f] > 0.15) diag_interface = 1;
+    if (diag_interface)
+      throw std::runtime_error("refinement-interface faces (skew > 0.15) are not supported by this build");
+
+    // ---- mesh-face list (cell_faces CSR order)
+    std::vector<int> nA(nc);
+    int wA = 0;
+    for (int c = 0; c < nc; ++c) {
+      nA[c] = d.cell_face_offset[c + 1] - d.cell_face_offset[c];
+      wA = std::max(wA, nA[c]);
+    }
+    std::vector<int> aface(size_t(wA) * nc, 0), aoth(size_t(wA) * nc, 0);
+    for (int c = 0; c < nc; ++c)
+      for (int k = 0; k < nA[c]; ++k) {
+        const int f = d.cell_faces[d.cell_face_offset[c] + k];
+        const int o = d.face_owner[f], nbc = d.face_neighbour[f];
+        const int side = (nbc >= 0 && nbc == c && o != c) ? 1 : 0;
+        aface[size_t(k) * nc + c] = 2 * f + side;
+        aoth[size_t(k) * nc + c] = (nbc >= 0) ? (side ? o : nbc) : -1 - (f - nint);
+      }
+
+    // ---- pressure operator (pressure_solver_init)
+    std::vector<int> pown, pnbr, pfm;
+    std::vector<double> pcoef, diag(nc, 0.0);
+    pown.reserve(nint + nb);
+    for (int f = 0; f < nint; ++f) {
+      pown.push_back(d.face_owner[f]);
+      pnbr.push_back(d.face_neighbour[f]);
+      pfm.push_back(f);
+    }
+    pcoef.assign(nint, 0.0);
+    for (int q = 0; q < npatch; ++q) {
+      if (d.patch_bc[q] != BC_PERIODIC) continue;
+      for (int k = 0; k < d.patch_nfaces[q]; ++k) {
+        const int b = d.patch_start[q] + k;
+        if (pair[b] < 0) continue;
+        const int fg = nint + b, o = bown[b], nbc = d.face_owner[nint + pair[b]];
+        if (nbc == o || o >= nbc) continue;  // canonical side: lower cell id owns the edge
+        const int pf = nint + pair[b];
+        double pd = d.periodic_pair_dcf[b];
+        if (pd <= SMALL) pd = d.face_dcf[pf];
+        if (pd <= SMALL) pd = d.face_dcf[fg];
+        const double dper = d.face_dcf[fg] + pd;
+        const double area = 0.5 * (d.face_area[fg] + d.face_area[pf]);
+        const double coeff = area / (dper + SMALL);
+        pown.push_back(o);
+        pnbr.push_back(nbc);
+        pfm.push_back(fg);
+        pcoef.push_back(coeff);
+        diag[o] = diag[o] + coeff;
+        diag[nbc] = diag[nbc] + coeff;
+      }
+    }
+    npf = int(pown.size());
+    for (int f = 0; f < nint; ++f) {
+      const int o = d.face_owner[f], nbc = d.face_neighbour[f];
+      double don[3];
+      for (int i = 0; i < 3; ++i) don[i] = d.cell_centroid[3 * size_t(nbc) + i] - d.cell_centroid[3 * size_t(o) + i];
+      const double* n = &d.face_normal[3 * size_t(f)];
+      double dort = don[0] * n[0] + don[1] * n[1] + don[2] * n[2];
+      if (p.min_cos_theta > 0.0 && dort < p.min_cos_theta * d.face_dcf[f]) dort = p.min_cos_theta * d.face_dcf[f];
+      const double coeff = (dort > SMALL) ? d.face_area[f] / dort : d.face_area[f] / (d.face_dcf[f] + SMALL);
+      if (!std::isfinite(coeff) || coeff <= SMALL) {
+        pown[f] = -1;  // skipped face
+        has_skipped = 1;
+        continue;
+      }
+      pcoef[f] = coeff;
+      diag[o] = diag[o] + coeff;
+      diag[nbc] = diag[nbc] + coeff;
+    }
+    for (int q = 0; q < npatch; ++q) {
+      if (d.patch_bc[q] != BC_OUTLET) continue;
+      for (int k = 0; k < d.patch_nfaces[q]; ++k) {
+        const int b = d.patch_start[q] + k, fg = nint + b, o = bown[b];
+        double dv[3];
+        for (int i = 0; i < 3; ++i) dv[i] = d.face_centre[3 * size_t(fg) + i] - d.cell_centroid[3 * size_t(o) + i];
+        const double* n = &d.face_normal[3 * size_t(fg)];
+        double dp = std::fabs(dv[0] * n[0] + dv[1] * n[1] + dv[2] * n[2]);
+        if (dp <= SMALL) dp = d.face_dcf[fg];
+        if (p.min_cos_theta > 0.0) dp = std::fmax(dp, p.min_cos_theta * d.face_dcf[fg]);
+        if (dp <= SMALL) continue;
+        const double cbv = d.face_area[fg] / dp;
+        if (!std::isfinite(cbv) || cbv <= SMALL) continue;
+        diag[o] = diag[o] + cbv;
+        dir_rows_h.push_back(o);
+        dir_vals_h.push_back(cbv * patchP_h[q]);
+      }
+    }
+    n_dir = int(dir_rows_h.size());
+    for (int c = 0; c < nc; ++c)
+      if (!std::isfinite(diag[c]) || diag[c] <= SMALL) diag[c] = 1.0;
+    // backend repair (pressure_backend_petsc.c:304-309)
+    for (int c = 0; c < nc; ++c)
+      if (!std::isfinite(diag[c]) || std::fabs(diag[c]) < 1.0e-20) diag[c] = 1.0;
+
+    // ---- interpolation weights per operator face
+    std::vector<double> rcgc(std::max(1, npf)), xgc(std::max(1, npf));
+    for (int i = 0; i < npf; ++i) {
+      const int fm = pfm[i];
+      if (fm < nint) {
+        rcgc[i] = d.face_gc[fm];
+        xgc[i] = d.face_gc[fm];
+      } else {
+        const int b = fm - nint;
+        double g = 0.5;
+        const double d_o = d.face_dcf[fm];
+        double d_n = d_o;
+        if (d.periodic_pair_dcf[b] > SMALL) d_n = d.periodic_pair_dcf[b];
+        if (d_n <= SMALL && pair[b] >= 0) d_n = d.face_dcf[nint + pair[b]];
+        const double dt = d_o + d_n;
+        if (dt > SMALL) g = d_n / dt;
+        rcgc[i] = g;
+        xgc[i] = d.periodic_gc[b];
+      }
+    }
+
+    // ---- non-orthogonal k-vectors (detect_nonortho_skip + build_nonortho_cache)
+    std::vector<double> kint(3 * size_t(std::max(1, npf)), 0.0), kbnd(3 * size_t(std::max(1, nb)), 0.0);
+    int nact = 0, ngradbc = 0;
+    for (int i = 0; i < npf; ++i) {
+      if (pown[i] < 0) continue;
+      const int o = pown[i], nbc = pnbr[i], fm = pfm[i];
+      double don[3], kf[3];
+      for (int q = 0; q < 3; ++q) don[q] = d.cell_centroid[3 * size_t(nbc) + q] - d.cell_centroid[3 * size_t(o) + q];
+      const double* sf = &d.face_sf[3 * size_t(fm)];
+      kfactor_limited(sf, don, p.min_cos_theta, kf);
+      if (kf[0] * kf[0] + kf[1] * kf[1] + kf[2] * kf[2] >= 1.0e-10 * (sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2])) {
+        nact++;
+        for (int q = 0; q < 3; ++q) kint[3 * size_t(i) + q] = kf[q];
+      }
+    }
+    for (int b = 0; b < nb; ++b) {
+      if (bbc[b] != BC_OUTLET) continue;
+      const int fg = nint + b, o = bown[b];
+      double don[3], kf[3];
+      for (int q = 0; q < 3; ++q) don[q] = d.face_centre[3 * size_t(fg) + q] - d.cell_centroid[3 * size_t(o) + q];
+      const double* sf = &d.face_sf[3 * size_t(fg)];
+      kfactor_limited(sf, don, p.min_cos_theta, kf);
+      if (kf[0] * kf[0] + kf[1] * kf[1] + kf[2] * kf[2] >= 1.0e-10 * (sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2])) {
+        nact++;
+        for (int q = 0; q < 3; ++q) kbnd[3 * size_t(b) + q] = kf[q];
+      }
+    }
+    for (int q = 0; q < npatch; ++q) {
+      const int bc = d.patch_bc[q];
+      if (!(bc == BC_OUTLET || bc == BC_PERIODIC || bc == BC_WALL_NOSLIP || bc == BC_WALL_VELOCITY ||
+            bc == BC_WALL_MODEL || bc == BC_SLIP || bc == BC_SYMMETRY))
+        ngradbc++;
+    }
+    skip_nonortho = (nact == 0);
+    skip_grad_p = skip_nonortho && (ngradbc == 0);
+    if (skip_nonortho) {
+      std::fill(kint.begin(), kint.end(), 0.0);
+      std::fill(kbnd.begin(), kbnd.end(), 0.0);
+    }
+    n_corr = p.n_nonortho_corr;
+    if (n_corr <= 0) n_corr = skip_nonortho ? 1 : 2;
+
+    // ---- projection-face list: operator faces in operator order, then non-periodic boundary faces
+    std::vector<int> nB(nc, 0);
+    for (int i = 0; i < npf; ++i)
+      if (pown[i] >= 0) {
+        nB[pown[i]]++;
+        nB[pnbr[i]]++;
+      }
+    for (int b = 0; b < nb; ++b)
+      if (bbc[b] != BC_PERIODIC) nB[bown[b]]++;
+    int wB = 0;
+    for (int c = 0; c < nc; ++c) wB = std::max(wB, nB[c]);
+    std::vector<int> bface(size_t(std::max(1, wB)) * nc, 0), both(size_t(std::max(1, wB)) * nc, 0),
+        bidx(size_t(std::max(1, wB)) * nc, -1), fillB(nc, 0);
+    for (int i = 0; i < npf; ++i) {
+      if (pown[i] < 0) continue;
+      const int o = pown[i], nbc = pnbr[i], fm = pfm[i];
+      size_t s = size_t(fillB[o]++) * nc + o;
+      bface[s] = 2 * fm;
+      both[s] = nbc;
+      bidx[s] = i;
+      s = size_t(fillB[nbc]++) * nc + nbc;
+      bface[s] = 2 * fm + 1;
+      both[s] = o;
+      bidx[s] = i;
+    }
+    for (int b = 0; b < nb; ++b) {
+      if (bbc[b] == BC_PERIODIC) continue;
+      const int o = bown[b];
+      const size_t s = size_t(fillB[o]++) * nc + o;
+      bface[s] = 2 * (nint + b);
+      both[s] = -1 - b;
+      bidx[s] = -1;
+    }
+    // ---- CG operator rows: the operator entries of the projection list, same order
+    int wP = 0;
+    std::vector<int> nP(nc, 0);
+    for (int i = 0; i < npf; ++i)
+      if (pown[i] >= 0 && std::isfinite(pcoef[i]) && pcoef[i] != 0.0) {
+        nP[pown[i]]++;
+        nP[pnbr[i]]++;
+      }
+    for (int c = 0; c < nc; ++c) wP = std::max(wP, nP[c]);
+    std::vector<int> anbr(size_t(std::max(1, wP)) * nc, -1), fillP(nc, 0);
+    std::vector<double> acoef(size_t(std::max(1, wP)) * nc, 0.0);
+    for (int i = 0; i < npf; ++i) {
+      if (pown[i] < 0 || !std::isfinite(pcoef[i]) || pcoef[i] == 0.0) continue;
+      const int o = pown[i], nbc = pnbr[i];
+      size_t s = size_t(fillP[o]++) * nc + o;
+      anbr[s] = nbc;
+      acoef[s] = pcoef[i];
+      s = size_t(fillP[nbc]++) * nc + nbc;
+      anbr[s] = o;
+      acoef[s] = pcoef[i];
+    }
+
+    // ---- LS stencil in ELL form (boundary pseudo-neighbour b encoded as -1-b)
+    std::vector<int> nL(nc);
+    int wL = 0;
+    for (int c = 0; c < nc; ++c) {
+      nL[c] = d.ls_offset[c + 1] - d.ls_offset[c];
+      wL = std::max(wL, nL[c]);
+    }
+    std::vector<int> lidx(size_t(std::max(1, wL)) * nc, 0);
+    std::vector<double> lw(3 * size_t(std::max(1, wL)) * nc, 0.0);
+    for (int c = 0; c < nc; ++c)
+      for (int k = 0; k < nL[c]; ++k) {
+        const int e = d.ls_offset[c] + k;
+        const int cell = d.ls_cell[e];
+        int enc;
+        if (cell >= 0) {
+          enc = cell;
+        } else {
+          const int f = d.ls_face[e];
+          const int b = (f >= 0) ? f - nint : -1;
+          enc = (b >= 0 && b < nb) ? -1 - b : c;  // invalid entry: zero difference
+        }
+        lidx[size_t(k) * nc + c] = enc;
+        for (int i = 0; i < 3; ++i) lw[(3 * size_t(k) + i) * nc + c] = d.ls_weight[3 * size_t(e) + i];
+      }
+
+    // ---- reconstruction operator T^-1 (build_reconstruct_operator + invert_sym3)
+    std::vector<double> Tinv(9 * size_t(nc), 0.0);
+    {
+      std::vector<double> T(9 * size_t(nc), 0.0);
+      auto addnn = [&](int c, double a, const double* n) {
+        for (int jj = 0; jj < 3; ++jj)
+          for (int ii = 0; ii < 3; ++ii) T[9 * size_t(c) + 3 * ii + jj] = T[9 * size_t(c) + 3 * ii + jj] + a * n[ii] * n[jj];
+      };
+      for (int i = 0; i < npf; ++i) {
+        if (pown[i] < 0) continue;
+        const int fm = pfm[i];
+        addnn(pown[i], d.face_area[fm], &d.face_normal[3 * size_t(fm)]);
+        addnn(pnbr[i], d.face_area[fm], &d.face_normal[3 * size_t(fm)]);
+      }
+      for (int b = 0; b < nb; ++b)
+        if (bbc[b] != BC_PERIODIC) addnn(bown[b], d.face_area[nint + b], &d.face_normal[3 * size_t(nint + b)]);
+      for (int c = 0; c < nc; ++c) invert_sym3(&T[9 * size_t(c)], &Tinv[9 * size_t(c)]);
+    }
+    // ---- Vreman filter tensor (build_filter_tensor + smooth_filter_tensor)
+    std::vector<double> Gf(9 * size_t(nc), 0.0);
+    if (p.sgs != 0) build_filter_tensor(d, Gf);
+
+    // ---- upload
+    std::vector<double> iv(nc), wbl(nf, 1.0);
+    for (int c = 0; c < nc; ++c) iv[c] = 1.0 / std::fmax(d.cell_volume[c], SMALL);
+    for (int f = 0; f < nint; ++f) wbl[f] = refine_blend_weight(d.face_skew[f]);
+    auto vec_d = [](const double* p0, size_t n) { return std::vector<double>(p0, p0 + n); };
+    M.nc = nc; M.nf = nf; M.nint = nint; M.nb = nb; M.npf = npf; M.wA = wA; M.wB = wB; M.wL = wL;
+    M.vol = up(vec_d(d.cell_volume, nc));
+    M.iv = up(iv);
+    M.ctr = up(vec_d(d.cell_centroid, 3 * size_t(nc)));
+    M.nA = up(nA); M.aface = up(aface); M.aoth = up(aoth);
+    M.nB = up(nB); M.bface = up(bface); M.both = up(both); M.bidx = up(bidx);
+    M.nL = up(nL); M.lidx = up(lidx); M.lw = up(lw);
+    M.sf = up(vec_d(d.face_sf, 3 * size_t(nf)));
+    M.gc = up(vec_d(d.face_gc, nf));
+    M.area = up(vec_d(d.face_area, nf));
+    M.nrm = up(vec_d(d.face_normal, 3 * size_t(nf)));
+    M.eps = up(vec_d(d.face_eps, 3 * size_t(nf)));
+    M.wbl = up(wbl);
+    M.fctr = up(vec_d(d.face_centre, 3 * size_t(nf)));
+    M.dcf = up(vec_d(d.face_dcf, nf));
+    M.bbc = up(bbc); M.bpatch = up(bpatch); M.bown = up(bown); M.pnb = up(pnb); M.pair = up(pair);
+    M.pgc = up(pgc); M.pdcf = up(pdcf); M.cb = up(cb); M.kbnd = up(kbnd); M.wsy = up(wsy); M.wsc = up(wsc);
+    M.pcoef = up(pcoef.empty() ? std::vector<double>(1, 0.0) : pcoef);
+    M.rcgc = up(rcgc); M.xgc = up(xgc); M.kint = up(kint);
+    M.pown = up(pown.empty() ? std::vector<int>(1, -1) : pown);
+    M.pnbr = up(pnbr.empty() ? std::vector<int>(1, -1) : pnbr);
+    M.pfm = up(pfm.empty() ? std::vector<int>(1, 0) : pfm);
+    M.Tinv = up(Tinv); M.Gf = up(Gf);
+    M.patchU = up(patchU_h);
+    patchP = up(patchP_h);
+    A.n = nc;
+    A.w = wP;
+    A.nbr = up(anbr);
+    A.coef = up(acoef);
+    A.diag = up(diag);
+    if (n_dir) {
+      dir_rows = up(dir_rows_h);
+      dir_vals = up(dir_vals_h);
+    }
+    // outlet pressure per boundary face (patch p_val; no backflow traction)
+    std::vector<double> opb(std::max(1, nb), 0.0);
+    for (int b = 0; b < nb; ++b) opb[b] = patchP_h[bpatch[b]];
+    outlet_pb = up(opb);
+
+    U = fresh<double>(3 * size_t(nc)); U2 = fresh<double>(3 * size_t(nc)); P = fresh<double>(nc);
+    p_old = fresh<double>(nc); phi_prev = fresh<double>(nc); phi_it = fresh<double>(nc);
+    p_bnd = fresh<double>(std::max(1, nb)); phi_prev_bnd = fresh<double>(std::max(1, nb));
+    U_bnd = fresh<double>(3 * size_t(std::max(1, nb))); tau = fresh<double>(3 * size_t(std::max(1, nb)));
+    nu_sgs = fresh<double>(nc); grad = fresh<double>(9 * size_t(nc)); gprev = fresh<double>(9 * size_t(nc));
+    gp = fresh<double>(3 * size_t(nc)); gp_rc = fresh<double>(3 * size_t(nc)); gp_prev = fresh<double>(3 * size_t(nc));
+    H = fresh<double>(3 * size_t(nc)); H0 = fresh<double>(3 * size_t(nc)); rhs = fresh<double>(nc);
+    cg_b = fresh<double>(nc);
+    ff = fresh<double>(nf); UF = fresh<double>(nf);
+    sc = fresh<SolverScalars>(1);
+    SolverScalars hs{};
+    hs.vmin = 1e308;
+    for (int c = 0; c < nc; ++c) {
+      hs.vmin = std::fmin(hs.vmin, d.cell_volume[c]);
+      hs.vtot = hs.vtot + d.cell_volume[c];
+    }
+    x.upload(sc, &hs, 1);
+    // initialize_velocity_from_bc: U = first inlet's velocity (zero without an inlet)
+    for (int q = 0; q < npatch; ++q)
+      if (d.patch_bc[q] == BC_INLET) {
+        std::vector<double> u0(3 * size_t(nc));
+        for (int c = 0; c < nc; ++c)
+          for (int k = 0; k < 3; ++k) u0[3 * size_t(c) + k] = patchU_h[3 * q + k];
+        x.upload(U, u0.data(), u0.size());
+        break;
+      }
+    cg.init(&x, nc);
+    apply_all_bc();
+    x.sync();
+  }
+
+  void validate(const dopamine_mesh_desc& d) {
+    if (nc <= 0 || nf <= 0 || nint < 0 || nb < 0 || nint + nb != nf) throw std::runtime_error("inconsistent mesh sizes");
+    for (int f = 0; f < nf; ++f) {
+      const int o = d.face_owner[f], n = d.face_neighbour[f];
+      if (o < 0 || o >= nc) throw std::runtime_error("face owner out of range");
+      if (f < nint && (n < 0 || n >= nc)) throw std::runtime_error("interior face neighbour out of range");
+      if (f >= nint && n >= 0) throw std::runtime_error("boundary face with a neighbour");
+      const int pid = d.face_patch[f];
+      if (f >= nint && (pid < 0 || pid >= npatch)) throw std::runtime_error("boundary face without a patch");
+    }
+    for (int b = 0; b < nb; ++b)
+      if (d.periodic_pair[b] < -1 || d.periodic_pair[b] >= nb) throw std::runtime_error("periodic pair out of range");
+    for (int c = 0; c <= nc; ++c) {
+      if (c < nc && d.cell_face_offset[c + 1] < d.cell_face_offset[c]) throw std::runtime_error("bad cell_face_offset");
+      if (d.ls_offset[c] < 0) throw std::runtime_error("bad ls_offset");
+    }
+    for (int i = 0; i < d.cell_face_offset[nc]; ++i)
+      if (d.cell_faces[i] < 0 || d.cell_faces[i] >= nf) throw std::runtime_error("cell_faces out of range");
+    for (int i = 0; i < d.ls_offset[nc]; ++i)
+      if (d.ls_cell[i] >= nc || d.ls_face[i] >= nf) throw std::runtime_error("LS stencil entry out of range");
+  }
+
+  static double refine_blend_weight(double skew) { return dm::refine_blend_weight(skew); }
+
+  // invert_sym3 (src/solver/mod_fv_solver.f90:3466-3491), row-major in and out
+  static void invert_sym3(const double* A_, double* Ai) {
+    auto a = [&](int i, int j) { return A_[3 * (i - 1) + (j - 1)]; };
+    const double c11 = a(2, 2) * a(3, 3) - a(2, 3) * a(3, 2);
+    const double c12 = a(1, 3) * a(3, 2) - a(1, 2) * a(3, 3);
+    const double c13 = a(1, 2) * a(2, 3) - a(1, 3) * a(2, 2);
+    const double c22 = a(1, 1) * a(3, 3) - a(1, 3) * a(3, 1);
+    const double c23 = a(1, 3) * a(2, 1) - a(1, 1) * a(2, 3);
+    const double c33 = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1);
+    const double det = a(1, 1) * c11 + a(1, 2) * (a(2, 3) * a(3, 1) - a(2, 1) * a(3, 3)) +
+                       a(1, 3) * (a(2, 1) * a(3, 2) - a(2, 2) * a(3, 1));
+    const double tr = a(1, 1) + a(2, 2) + a(3, 3);
+    const double t = std::fmax(tr, SMALL);
+    if (std::fabs(det) <= 1.0e-12 * t * t * t) {
+      for (int q = 0; q < 9; ++q) Ai[q] = 0.0;
+      Ai[0] = Ai[4] = Ai[8] = 3.0 / t;
+      return;
+    }
+    Ai[0] = c11 / det; Ai[1] = c12 / det; Ai[2] = c13 / det;
+    Ai[3] = c12 / det; Ai[4] = c22 / det; Ai[5] = c23 / det;
+    Ai[6] = c13 / det; Ai[7] = c23 / det; Ai[8] = c33 / det;
+  }
+
+  void build_filter_tensor(const dopamine_mesh_desc& d, std::vector<double>& G) {
+    for (int c = 0; c < nc; ++c) {
+      const double V = d.cell_volume[c];
+      double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = d.cell_face_offset[c]; q < d.cell_face_offset[c + 1]; ++q) {
+        const int f = d.cell_faces[q];
+        double dx[3];
+        for (int i = 0; i < 3; ++i) dx[i] = d.face_centre[3 * size_t(f) + i] - d.cell_centroid[3 * size_t(c) + i];
+        const double a = d.face_area[f];
+        for (int n = 0; n < 3; ++n)
+          for (int m = 0; m < 3; ++m) g[3 * n + m] = g[3 * n + m] + a * dx[m] * dx[n];
+      }
+      const double det = det3_colmajor(g);
+      const double t = (g[0] + g[4] + g[8]) / 3.0;
+      const double tr3 = t * t * t;
+      double* Gc = &G[9 * size_t(c)];
+      if (tr3 > 0.0 && det > 1.0e-12 * tr3) {
+        const double s = std::pow(V * V / det, 1.0 / 3.0);
+        for (int q = 0; q < 9; ++q) Gc[q] = s * g[q];
+      } else {
+        for (int q = 0; q < 9; ++q) Gc[q] = 0.0;
+        Gc[0] = Gc[4] = Gc[8] = std::pow(V, 2.0 / 3.0);
+      }
+    }
+    std::vector<double> gnb(9 * size_t(nc)), ws(nc);
+    for (int pass = 0; pass < 2; ++pass) {
+      std::fill(gnb.begin(), gnb.end(), 0.0);
+      std::fill(ws.begin(), ws.end(), 0.0);
+      for (int f = 0; f < nint; ++f) {
+        const int o = d.face_owner[f], n = d.face_neighbour[f];
+        const double a = d.face_area[f];
+        for (int q = 0; q < 9; ++q) gnb[9 * size_t(o) + q] = gnb[9 * size_t(o) + q] + a * G[9 * size_t(n) + q];
+        ws[o] = ws[o] + a;
+        for (int q = 0; q < 9; ++q) gnb[9 * size_t(n) + q] = gnb[9 * size_t(n) + q] + a * G[9 * size_t(o) + q];
+        ws[n] = ws[n] + a;
+      }
+      for (int c = 0; c < nc; ++c)
+        if (ws[c] > SMALL)
+          for (int q = 0; q < 9; ++q)
+            G[9 * size_t(c) + q] = (1.0 - 0.5) * G[9 * size_t(c) + q] + 0.5 * gnb[9 * size_t(c) + q] / ws[c];
+    }
+  }
+
+  // ---------------------------------------------------------------- boundary passes
+
+  void bc_velocity(int filter, const double* gU) { x.for_each(nb, BcVelocity{M, U, gU, U_bnd, filter}); }
+  void bc_pressure(int filter, const double* g, double scale, const double* pob) {
+    x.for_each(nb, BcPressure{M, P, nullptr, g, pob, patchP, p_bnd, scale, filter});
+  }
+  void apply_all_bc() {
+    bc_velocity(0, grad);
+    bc_pressure(0, skip_grad_p ? nullptr : gp, 1.0, outlet_pb);
+  }
+  // refresh_halos_and_bc (single rank: non-periodic pass, then periodic pass)
+  void refresh_bc() {
+    for (int pass = 1; pass <= 2; ++pass) {
+      bc_velocity(pass, grad);
+      bc_pressure(pass, skip_grad_p ? nullptr : gp, 1.0, outlet_pb);
+    }
+  }
+  void velocity_bc_both() {
+    bc_velocity(1, grad);
+    bc_velocity(2, nullptr);
+  }
+
+  // ---------------------------------------------------------------- gradients
+
+  // compute_all_gradients (velocity), with the SGS viscosity of the stage fused in when sgs_dt > 0
+  void gradients(bool with_sgs, double sgs_dt) {
+    const int sgs = with_sgs ? prm.sgs : -1;
+    if (prm.gradient == 0) {
+      x.for_each(nc, LsGradVec{M, U, U_bnd, grad, nu_sgs, sgs, prm.c_vreman, prm.cs_smag, sgs_dt});
+    } else {
+      x.for_each(nc, GgGradVec{M, U, U_bnd, nullptr, grad});
+      for (int it = 2; it <= prm.gg_iterations; ++it) {
+        x.copy(gprev, grad, 9 * size_t(nc));
+        x.for_each(nc, GgGradVec{M, U, U_bnd, gprev, grad});
+      }
+      if (with_sgs) x.for_each(nc, SgsCell{M, grad, nu_sgs, prm.sgs, prm.c_vreman, prm.cs_smag, sgs_dt});
+    }
+  }
+  // gradient_of_scalar of a (+ a2) with boundary values pb (+ pb2) into gp
+  void scalar_gradient(const double* a, const double* a2, const double* pb, const double* pb2) {
+    if (prm.gradient == 0) {
+      x.for_each(nc, LsGradSca{M, a, a2, pb, pb2, gp});
+    } else {
+      x.for_each(nc, GgGradSca{M, a, a2, pb, pb2, nullptr, gp});
+      for (int it = 2; it <= prm.gg_iterations; ++it) {
+        x.copy(gp_prev, gp, 3 * size_t(nc));
+        x.for_each(nc, GgGradSca{M, a, a2, pb, pb2, gp_prev, gp});
+      }
+    }
+  }
+  void pressure_gradient() {
+    if (!skip_grad_p) scalar_gradient(P, nullptr, p_bnd, nullptr);
+  }
+
+  // ---------------------------------------------------------------- projection pieces
+
+  void face_fluxes_and_rhs(double dt, bool incr) {
+    if (has_skipped) x.zero(ff, nf);
+    if (!skip_grad_p) {
+      if (incr)
+        scalar_gradient(phi_prev, P, phi_prev_bnd, p_bnd);
+      else
+        scalar_gradient(P, nullptr, p_bnd, nullptr);
+    }
+    const int rck = (!skip_nonortho && !prm.rhie_chow_dual) ? 1 : 0;
+    x.template reduce<1, true>(nc, FluxRhs{M, U, U_bnd, grad, gp_rc, phi_prev, gp, ff, rhs, dt, skip_nonortho ? 0 : 1, rck},
+                               sk::DivFin{sc});
+    x.for_each(nb, PeriodicAntisym{M, ff});
+  }
+
+  void correct_face_fluxes(double dt, bool ptot) {
+    x.for_each(npf + nb, CorrectFaceFlux{M, P, gp, outlet_pb, ff, dt, ptot ? 1 : 0, skip_nonortho ? 0 : 1});
+    x.for_each(nb, PeriodicAntisym{M, ff});
+  }
+
+  void correct_velocity(double dt, bool with_cross, bool ptot, double* gpc_out) {
+    const int do_cross = (with_cross && !skip_nonortho) ? 1 : 0;
+    x.for_each(nc, CorrectVelocity{M, P, gp, outlet_pb, U, gpc_out, dt, prm.f_const[0], prm.f_const[1], prm.f_const[2],
+                                   do_cross, ptot ? 1 : 0});
+  }
+
+  int pressure_solve(bool ptot, const double* atol_dev, double atol) {
+    x.copy(cg_b, rhs, nc);
+    if (ptot && n_dir) {
+      // Dirichlet rhs increments c_b p_val; rows can repeat, so apply them in order on the host side
+      std::vector<double> hb(nc);
+      x.download(hb.data(), cg_b, nc);
+      for (int i = 0; i < n_dir; ++i) hb[dir_rows_h[i]] = hb[dir_rows_h[i]] + dir_vals_h[i];
+      x.upload(cg_b, hb.data(), nc);
+    }
+    const int maxit = prm.p_max_iter > 2000 ? prm.p_max_iter : 2000;
+    int its = 0;
+    double res = 0.0;
+    const int rc = cg.solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, n_dir == 0, &its, &res);
+    n_solves++;
+    n_iters += its;
+    last_iters = its;
+    return rc;
+  }
+
+  void anchor_pressure() {
+    // mean removal is folded into RotAccumulate's reduction; this variant serves project_once
+    x.template reduce<1, false>(nc, RotAccumulate{M, P, zeros(), nu_sgs, rhs, 0.0, 0.0, 0}, sk::AnchorFin{sc, n_outlet == 0});
+    x.for_each(nc, ShiftP{P, &sc->shift});
+  }
+  const double* zeros() {
+    if (!zero_) zero_ = fresh<double>(nc);
+    return zero_;
+  }
+  double* zero_ = nullptr;
+
+  // Divergence tolerance from the volume-weighted mean speed (:1424-1437), left in the execution
+  // space where the CG reads it.
+  void atol_for(double gdt) { x.template reduce<2, false>(nc, BulkSpeed{M, U}, sk::AtolFin{sc, gdt}); }
+
+  // ---------------------------------------------------------------- public steps
+
+  int project_once(double dt) {
+    if (dt <= 0.0) dt = prm.dt;
+    x.zero(rhs, nc);
+    x.zero(phi_prev, nc);
+    x.zero(phi_prev_bnd, std::max(1, nb));
+    x.zero(gp_rc, 3 * size_t(nc));
+    refresh_bc();
+    face_fluxes_and_rhs(dt, false);
+    int rc = 0;
+    if (prm.p_max_iter > 0) {
+      atol_for(dt);
+      rc = pressure_solve(true, &sc->atol, 0.0);
+    }
+    anchor_pressure();
+    correct_face_fluxes(dt, true);
+    std::swap(UF, ff);
+    u_face_ready = 1;
+    bc_pressure(0, nullptr, 1.0, outlet_pb);
+    correct_velocity(dt, true, true, nullptr);
+    bc_velocity(0, nullptr);
+    if (prm.rotational) {
+      x.template reduce<1, false>(nc, RotAccumulate{M, P, zeros(), nu_sgs, rhs, prm.nu, dt, 1}, sk::AnchorFin{sc, 0});
+      bc_pressure(0, nullptr, 1.0, outlet_pb);
+    }
+    x.copy(p_old, P, nc);
+    x.copy(phi_prev, P, nc);
+    x.copy(phi_prev_bnd, p_bnd, std::max(1, nb));
+    bc_pressure(0, nullptr, 1.0, outlet_pb);
+    pressure_gradient();
+    x.sync();
+    return rc;
+  }
+
+  double compute_dt() {
+    x.template reduce<2, true>(nc, CflRates{M, U, UF, nu_sgs, prm.nu, u_face_ready}, sk::CflFin{sc});
+    SolverScalars h;
+    x.download(&h, sc, 1);
+    double dc = 1e308, dd = 1e308;
+    if (h.cfl > SMALL) dc = prm.cfl_max / h.cfl;
+    if (h.dif > SMALL) dd = 1.0 / h.dif;
+    double dt = std::fmin(dc, dd);
+    if (!(dt > 0.0)) dt = prm.dt;
+    dt = std::fmin(dt, prm.dt);
+    const double dtmin = 1.0e-6 * prm.dt;
+    if (!(dt > 0.0)) dt = prm.dt;
+    else if (dt < dtmin) return -1.0;
+    else dt = std::fmax(dt, dtmin);
+    return dt;
+  }
+
+  int advance(double dt_req, double* dt_used) {
+    double dt = dt_req;
+    const double safe = compute_dt();
+    if (safe < 0.0) return -3;
+    if (dt > safe) dt = safe;
+    const int rc = advance_rk3w(dt);
+    time = time + dt;
+    if (dt_used) *dt_used = dt;
+    return rc;
+  }
+
+  int advance_rk3w(double dt) {
+    x.zero(H, 3 * size_t(nc));
+    x.zero(H0, 3 * size_t(nc));
+    x.zero(rhs, nc);
+    const double time_n = time;
+    int rc = 0;
+    const int scheme = prm.scheme;
+    for (int k = 0; k < 3; ++k) {
+      const double gdt = RK3_G[k] * dt;
+      time = time_n + RK3_C[k] * dt;
+      refresh_bc();
+      gradients(true, gdt);
+      if (prm.wall_model == 1)
+        x.zero(tau, 3 * size_t(std::max(1, nb)));
+      else if (n_wm)
+        x.for_each(nb, WallModel{M, U, tau, prm.nu});
+      if (scheme == 4)
+        x.for_each(nc, ExplicitRhs<4>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
+                                      prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0});
+      else
+        x.for_each(nc, ExplicitRhs<3>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
+                                      prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0});
+      std::swap(U, U2);
+      velocity_bc_both();
+      x.copy(p_old, P, nc);
+      correct_velocity(gdt, false, true, gp_rc);
+      velocity_bc_both();
+      gradients(false, 0.0);
+      x.copy(phi_prev, P, nc);
+      x.copy(phi_prev_bnd, p_bnd, std::max(1, nb));
+      {
+        atol_for(gdt);
+        int nmax = std::min(n_corr, 3);
+        const double stag = 0.9;
+        double dprev = 1e308, dphi_prev = 1e308;
+        if (prm.nonortho_check) {
+          nmax = std::min(n_corr, 8);
+          x.zero(phi_it, nc);
+        }
+        for (int jno = 1; jno <= nmax; ++jno) {
+          face_fluxes_and_rhs(gdt, jno > 1);
+          if (jno == 1) x.zero(P, nc);
+          if (pressure_solve(false, &sc->atol, 0.0) != 0) rc = -2;
+          bc_pressure(1, skip_grad_p ? nullptr : gp, 0.0, nullptr);
+          bc_pressure(2, nullptr, 0.0, nullptr);
+          if (nmax == 1) break;
+          SolverScalars h;
+          x.download(&h, sc, 1);
+          const double dcur = h.divmax;
+          if (prm.nonortho_check) {
+            const double dd = phi_change();
+            if (jno > 1 && (dd < 1.0e-3 * last_pmax_ || dd > dphi_prev * stag)) break;
+            dphi_prev = dd;
+          } else if (jno > 1 && (dcur < h.atol || dcur > dprev * stag)) {
+            break;
+          }
+          dprev = dcur;
+        }
+      }
+      correct_face_fluxes(gdt, false);
+      std::swap(UF, ff);
+      u_face_ready = 1;
+      correct_velocity(gdt, true, false, nullptr);
+      velocity_bc_both();
+      // rotational correction + accumulation + mean removal (the intermediate increment BC is
+      // overwritten before anything reads it)
+      x.template reduce<1, false>(nc, RotAccumulate{M, P, p_old, nu_sgs, rhs, prm.nu, gdt, prm.rotational},
+                                  sk::AnchorFin{sc, n_outlet == 0});
+      x.for_each(nc, ShiftP{P, &sc->shift});
+      bc_pressure(1, skip_grad_p ? nullptr : gp, 1.0, outlet_pb);
+      bc_pressure(2, nullptr, 1.0, nullptr);
+      pressure_gradient();
+      std::swap(H, H0);
+    }
+    time = time_n;
+    return rc;
+  }
+
+  // max|p - phi_it| and max|p|, then phi_it = p (nonortho_check exit test, :1495-1509)
+  double last_pmax_ = 0.0;
+  double phi_change() {
+    std::vector<double> hp(nc), hi(nc);
+    x.download(hp.data(), P, nc);
+    x.download(hi.data(), phi_it, nc);
+    double dd = 0.0, pm = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      dd = std::fmax(dd, std::fabs(hp[c] - hi[c]));
+      pm = std::fmax(pm, std::fabs(hp[c]));
+    }
+    x.copy(phi_it, P, nc);
+    last_pmax_ = pm;
+    return dd;
+  }
+
+  void stats(double out[8]) {
+    x.template reduce<7, false>(nc, StatsCells{M, U, grad, nu_sgs, prm.nu}, sk::StatsFin7{sc});
+    x.template reduce<1, true>(nc, StatsMax{U}, sk::StatsFinMax{sc});
+    x.template reduce<2, false>(nb, StatsWall{M, tau}, sk::StatsFinWall{sc});
+    x.template reduce<1, true>(nc, FluxDivergence{M, UF, 0}, sk::StatsFinDiv{sc, 10});
+    x.template reduce<1, false>(nc, FluxDivergence{M, UF, 1}, sk::StatsFinDiv{sc, 11});
+    SolverScalars h;
+    x.download(&h, sc, 1);
+    const double V = h.st[1];
+    out[0] = h.st[0] / V;
+    out[1] = 2.0 * h.st[2] / V;
+    out[2] = h.st[7];
+    const double a = h.st[3] / V, b = h.st[4] / V, c = h.st[5] / V;
+    out[3] = std::sqrt(a * a + b * b + c * c);
+    out[4] = (h.st[9] > SMALL) ? h.st[8] / h.st[9] : 0.0;
+    out[5] = h.st[6] / V;
+    out[6] = h.st[10];
+    out[7] = h.st[11] / V;
+  }
+};
+
+}  // namespace dm

@@ -1,0 +1,171 @@
+"""FVSolver: the host-side mirror of the reference's solver type, running on the MI355X.
+
+Method names follow the FVSolver type-bound procedures of src/solver/mod_fv_solver.f90:270-285
+(init / project_once / advance / compute_dt / divergence_metrics / sync_host); every call goes
+through the C ABI of libdopamine_hip.so (include/dopamine_hip.h). There is no CPU fallback: without
+the library or a HIP device, construction raises.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib
+from .params import WALL_MODEL_IDS
+
+_P = ctypes.POINTER
+_d, _i32, _i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+
+MESH_FIELDS = [("cell_centroid", _d), ("cell_volume", _d), ("cell_face_offset", _i32), ("cell_faces", _i32),
+               ("face_owner", _i32), ("face_neighbour", _i32), ("face_patch", _i32),
+               ("face_area", _d), ("face_centre", _d), ("face_normal", _d), ("face_sf", _d), ("face_dcf", _d),
+               ("face_gc", _d), ("face_skew", _d), ("face_eps", _d),
+               ("patch_bc", _i32), ("patch_start", _i32), ("patch_nfaces", _i32),
+               ("periodic_pair", _i32), ("periodic_pair_dcf", _d), ("periodic_gc", _d),
+               ("ls_offset", _i32), ("ls_cell", _i32), ("ls_face", _i32), ("ls_weight", _d)]
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("n_cells", _i32), ("n_faces", _i32), ("n_int_faces", _i32), ("n_bnd_faces", _i32),
+                ("n_patches", _i32)] + [(n, _P(t)) for n, t in MESH_FIELDS]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("nu", _d), ("f_const", _d * 3), ("scheme", _i32), ("gradient", _i32), ("gg_iterations", _i32),
+                ("sgs", _i32), ("c_vreman", _d), ("cs_smag", _d), ("wall_model", _i32), ("wall_model_height", _d),
+                ("n_nonortho_corr", _i32), ("min_cos_theta", _d), ("rhie_chow_dual", _i32),
+                ("nonortho_check", _i32), ("rotational", _i32), ("p_tol", _d), ("p_max_iter", _i32),
+                ("cfl_max", _d), ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)),
+                ("precision", _i32)]
+
+
+STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
+
+
+def build_abi(mesh, params, precision=0):
+    """ctypes mesh and params structures (plus the arrays they point into, kept alive)."""
+    keep = []
+
+    def arr(a, t):
+        a = np.ascontiguousarray(a, dtype=np.float64 if t is _d else np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(_P(t))
+
+    m = mesh
+    md = MeshDesc(m.n_cells, m.n_faces, m.n_int_faces, m.n_bnd_faces, m.n_patches,
+                  *[arr(getattr(m, n), t) for n, t in MESH_FIELDS])
+    pv, uv = params.patch_arrays(mesh)
+    pr = Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
+                0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
+                params.sgs_id(), params.c_vreman, params.cs_smagorinsky, WALL_MODEL_IDS[params.wall_model],
+                params.wall_model_height, params.n_nonortho_corr, params.pressure_min_cos_theta,
+                int(params.rhie_chow_dual), int(params.nonortho_check), int(params.use_rotational_projection),
+                params.p_tol, params.p_max_iter, params.cfl_max, params.dt, arr(pv, _d), arr(uv.reshape(-1), _d),
+                int(precision))
+    return md, pr, keep
+
+
+def _bind(L):
+    L.dopamine_solver_create.restype = ctypes.c_void_p
+    L.dopamine_solver_create.argtypes = [_P(MeshDesc), _P(Params)]
+    L.dopamine_solver_destroy.argtypes = [ctypes.c_void_p]
+    L.dopamine_last_error.restype = ctypes.c_char_p
+    L.dopamine_solver_set_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _P(_d)]
+    L.dopamine_solver_get_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _P(_d), _i64]
+    L.dopamine_solver_project_once.argtypes = [ctypes.c_void_p, _d]
+    L.dopamine_solver_advance.argtypes = [ctypes.c_void_p, _d, _P(_d)]
+    L.dopamine_solver_compute_dt.restype = _d
+    L.dopamine_solver_compute_dt.argtypes = [ctypes.c_void_p]
+    L.dopamine_solver_stats.argtypes = [ctypes.c_void_p, _d * 8]
+    L.dopamine_solver_time.restype = _d
+    L.dopamine_solver_time.argtypes = [ctypes.c_void_p]
+    L.dopamine_solver_counters.argtypes = [ctypes.c_void_p, _i64 * 4]
+    L.dopamine_solver_nonortho_info.argtypes = [ctypes.c_void_p, _i32 * 3]
+    L.dopamine_solver_sync.argtypes = [ctypes.c_void_p]
+    return L
+
+
+class FVSolver:
+    """One RK3W solver on the current HIP device."""
+
+    def __init__(self, mesh, params, precision=0):
+        params.validate()
+        self.L = _bind(lib())
+        self.mesh = mesh
+        self.params = params
+        md, pr, keep = build_abi(mesh, params, precision)
+        self.h = self.L.dopamine_solver_create(ctypes.byref(md), ctypes.byref(pr))
+        if not self.h:
+            raise RuntimeError("dopamine_solver_create failed: " + self.L.dopamine_last_error().decode())
+        self._sizes = {"U": 3 * mesh.n_cells, "p": mesh.n_cells, "nu_sgs": mesh.n_cells,
+                       "U_bnd": 3 * max(1, mesh.n_bnd_faces), "p_bnd": max(1, mesh.n_bnd_faces),
+                       "grad_U": 9 * mesh.n_cells, "U_face": mesh.n_faces, "tau_w_bnd": 3 * max(1, mesh.n_bnd_faces),
+                       "rhs_p": mesh.n_cells, "H": 3 * mesh.n_cells, "w_gp": 3 * mesh.n_cells}
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.L.dopamine_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dopamine_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get(self, name):
+        n = self._sizes[name]
+        out = np.empty(n)
+        self._check(self.L.dopamine_solver_get_field(self.h, name.encode(), out.ctypes.data_as(_P(_d)), n), "get")
+        return out
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != self._sizes[name]:
+            raise ValueError(f"{name}: expected {self._sizes[name]} values, got {a.size}")
+        self._check(self.L.dopamine_solver_set_field(self.h, name.encode(), a.ctypes.data_as(_P(_d))), "set")
+
+    @property
+    def U(self):
+        return self.get("U").reshape(-1, 3)
+
+    @property
+    def p(self):
+        return self.get("p")
+
+    def set_velocity(self, U):
+        self.set("U", U)
+
+    def project_once(self, dt):
+        self._check(self.L.dopamine_solver_project_once(self.h, dt), "project_once")
+
+    def advance(self, dt=None):
+        used = _d(0.0)
+        self._check(self.L.dopamine_solver_advance(self.h, self.params.dt if dt is None else dt, ctypes.byref(used)),
+                    "advance")
+        return used.value
+
+    def compute_dt(self):
+        return self.L.dopamine_solver_compute_dt(self.h)
+
+    @property
+    def time(self):
+        return self.L.dopamine_solver_time(self.h)
+
+    def stats(self):
+        out = (_d * 8)()
+        self._check(self.L.dopamine_solver_stats(self.h, out), "stats")
+        return dict(zip(STATS_COLUMNS, list(out)))
+
+    def counters(self):
+        out = (_i64 * 4)()
+        self.L.dopamine_solver_counters(self.h, out)
+        return {"solves": out[0], "iterations": out[1], "last": out[2], "sweeps": out[3]}
+
+    def nonortho_info(self):
+        out = (_i32 * 3)()
+        self.L.dopamine_solver_nonortho_info(self.h, out)
+        return {"skip_nonortho": out[0], "skip_grad_p": out[1], "n_nonortho_corr": out[2]}
+
+    def sync(self):
+        self._check(self.L.dopamine_solver_sync(self.h), "sync")

@@ -1,0 +1,96 @@
+/* C ABI of libdopamine_hip.so: the MI355X (gfx950) implementation of DOPAMINE's RK3W
+ * fractional-step hot path.
+ *
+ * Two levels, both plain C (no torch types):
+ *  1. The pressure-backend ABI of src/solver/pressure_backend_petsc.c, symbol for symbol
+ *     (declared in fvm_pressure_backend.h), so src/solver/mod_pressure_solver.f90 links unchanged.
+ *  2. A solver-handle ABI mirroring the FVSolver type-bound procedures of
+ *     src/solver/mod_fv_solver.f90:270-285 (init / advance / project_once / compute_dt /
+ *     to_device / sync_host / divergence_metrics) for a Fortran host that keeps its own Mesh.
+ *
+ * All mesh indices are 0-based here; the Fortran binding subtracts 1.
+ */
+#ifndef DOPAMINE_HIP_H
+#define DOPAMINE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mesh arrays as the reference's Mesh SoA mirror holds them (src/mesh/mod_mesh.f90:164-186 and
+ * the LS stencil of :105-110), 0-based, host memory, read once by dopamine_solver_create. */
+typedef struct dopamine_mesh_desc {
+  int32_t n_cells, n_faces, n_int_faces, n_bnd_faces, n_patches;
+  const double *cell_centroid, *cell_volume;           /* 3*nc, nc */
+  const int32_t *cell_face_offset, *cell_faces;        /* nc+1, CSR ascending face order */
+  const int32_t *face_owner, *face_neighbour, *face_patch;
+  const double *face_area, *face_centre, *face_normal, *face_sf, *face_dcf, *face_gc, *face_skew, *face_eps;
+  const int32_t *patch_bc, *patch_start, *patch_nfaces;
+  const int32_t *periodic_pair;                        /* per boundary face, -1 = none */
+  const double *periodic_pair_dcf, *periodic_gc;
+  const int32_t *ls_offset, *ls_cell, *ls_face;        /* LS stencil CSR; ls_cell -1 = boundary face */
+  const double *ls_weight;                             /* 3 per entry */
+} dopamine_mesh_desc;
+
+/* Numerics and physics (solver_params_t, src/solver/mod_fv_solver.f90:41-110). */
+typedef struct dopamine_params {
+  double nu;
+  double f_const[3];
+  int32_t scheme;            /* 3 CDS, 4 KEP (src/mod_parameters.f90:57-62) */
+  int32_t gradient;          /* 0 LEAST_SQUARES, 1 GREEN_GAUSS */
+  int32_t gg_iterations;
+  int32_t sgs;               /* 0 none, 1 Vreman, 2 Smagorinsky */
+  double c_vreman;           /* reference C_VREMAN = 0.025 */
+  double cs_smag;            /* 0.1 */
+  int32_t wall_model;        /* 0 log-law (Reichardt), 1 none */
+  double wall_model_height;  /* <= 0: wall-adjacent cell */
+  int32_t n_nonortho_corr;   /* 0 = auto */
+  double min_cos_theta;
+  int32_t rhie_chow_dual;
+  int32_t nonortho_check;
+  int32_t rotational;
+  double p_tol;
+  int32_t p_max_iter;
+  double cfl_max;
+  double dt;
+  const double *patch_p_val;  /* n_patches */
+  const double *patch_U_val;  /* 3*n_patches */
+  int32_t precision;          /* 0: FP64 pressure solve; 1: FP32-SpMV CG inside FP64 iterative refinement */
+} dopamine_params;
+
+/* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
+void *dopamine_solver_create(const dopamine_mesh_desc *mesh, const dopamine_params *params);
+void dopamine_solver_destroy(void *solver);
+/* Device (HIP) index the library runs on; -1 if none is usable. */
+int dopamine_device_count(void);
+const char *dopamine_last_error(void);
+
+/* Copy a host field in / out. Names: "U" (3*nc, component-fastest), "p" (nc), "nu_sgs", "U_bnd",
+ * "p_bnd", "grad_U" (9*nc, grad(i,j) = du_j/dx_i at [9c+3j+i]), "U_face" (nf), "tau_w_bnd",
+ * "rhs_p", "H". set_field("U") re-applies the boundary conditions as solver_init does. */
+int dopamine_solver_set_field(void *solver, const char *name, const double *host);
+int dopamine_solver_get_field(void *solver, const char *name, double *host, int64_t len);
+
+/* solver_project_once (src/solver/mod_fv_solver.f90:837-917). */
+int dopamine_solver_project_once(void *solver, double dt);
+/* solver_advance (src/solver/mod_fv_solver.f90:920-957): CFL clamp, one RK3W step. */
+int dopamine_solver_advance(void *solver, double dt_req, double *dt_used);
+/* solver_compute_dt (src/solver/mod_fv_solver.f90:3864-3899). */
+double dopamine_solver_compute_dt(void *solver);
+/* compute_stats_sums + solver_divergence_metrics: {E, eps, Umax, Ub, tauw_mean, nu_sgs_mean,
+ * max_abs_divU, volavg_abs_divU} (src/stats/mod_stats.f90:271-336). */
+int dopamine_solver_stats(void *solver, double out[8]);
+double dopamine_solver_time(void *solver);
+/* {pressure solves, total CG iterations, last CG iteration count, refinement sweeps}. */
+int dopamine_solver_counters(void *solver, int64_t out[4]);
+/* {skip_nonortho, skip_grad_p, n_nonortho_corr}. */
+int dopamine_solver_nonortho_info(void *solver, int32_t out[3]);
+/* Wait for all work queued by the solver. */
+int dopamine_solver_sync(void *solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -55,8 +55,13 @@ def lib():
         L.om_project_once.argtypes = [ctypes.c_void_p, _d]
         L.om_advance.argtypes = [ctypes.c_void_p, _d, _P(_d)]
         L.om_stats.argtypes = [ctypes.c_void_p, _d * 8]
+        L.om_int_array.restype = _P(_i32)
+        L.om_int_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _P(_i64)]
         L.om_time.restype = _d
         L.om_time.argtypes = [ctypes.c_void_p]
+        L.om_eval_closures.argtypes = [ctypes.c_void_p, _d]
+        L.om_wall_tau.restype = _d
+        L.om_wall_tau.argtypes = [_d, _d, _d]
         L.om_counters.argtypes = [ctypes.c_void_p, _i64 * 3]
         L.om_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int * 3]
         L.om_cg_jacobi.argtypes = [ctypes.c_int, ctypes.c_int, _P(_i32), _P(_i32), _P(_d), _P(_d), _P(_d), _P(_d),
@@ -120,6 +125,13 @@ class Solver:
             raise KeyError(name)
         return np.ctypeslib.as_array(ptr, shape=(n.value,))
 
+    def int_array(self, name):
+        n = _i64(0)
+        ptr = lib().om_int_array(self.h, name.encode(), ctypes.byref(n))
+        if not ptr:
+            raise KeyError(name)
+        return np.ctypeslib.as_array(ptr, shape=(n.value,))
+
     @property
     def U(self):
         return self.array("U").reshape(-1, 3)
@@ -131,6 +143,10 @@ class Solver:
     def set_velocity(self, U):
         self.array("U")[:] = np.asarray(U, dtype=np.float64).reshape(-1)
         lib().om_apply_all_bc(self.h)
+
+    def eval_closures(self, dt_stab):
+        """Gradients, SGS viscosity and wall traction of the current U (no time advance)."""
+        lib().om_eval_closures(self.h, dt_stab)
 
     def project_once(self, dt):
         return lib().om_project_once(self.h, dt)
@@ -160,6 +176,11 @@ class Solver:
         out = (ctypes.c_int * 3)()
         lib().om_nonortho_info(self.h, out)
         return {"skip_nonortho": out[0], "skip_grad_p": out[1], "n_nonortho_corr": out[2]}
+
+
+def wall_tau(u_par, y_w, nu):
+    """Reichardt-law wall shear stress u_tau^2 for speed u_par at wall distance y_w."""
+    return lib().om_wall_tau(u_par, y_w, nu)
 
 
 def cg_jacobi(owner, neighbour, coeff, diag, b, x0, rtol, atol, max_it, nullspace):

@@ -1187,6 +1187,30 @@ int om_project_once(om_solver *s, double dt) {
 }
 
 /* advance_rk3w (src/solver/mod_fv_solver.f90:1180-1614) */
+/* Stage-start closures of solver_rk3w_stage (src/solver/mod_fv_solver.f90): velocity gradients,
+ * SGS viscosity (stability cap from dt_stab) and wall-model traction. */
+static void stage_closures(om_solver *s, double dt_stab) {
+  compute_all_gradients(s);
+  if (s->p.sgs == OM_SGS_VREMAN)
+    vreman(s, dt_stab);
+  else if (s->p.sgs == OM_SGS_SMAGORINSKY)
+    smagorinsky(s, dt_stab);
+  else
+    memset(s->nu_sgs, 0, sizeof(double) * s->nc);
+  if (s->p.wall_model == OM_WM_NONE)
+    memset(s->tau_w, 0, sizeof(double) * 3 * (size_t)s->nb);
+  else
+    wall_model(s);
+}
+
+int om_eval_closures(om_solver *s, double dt_stab) {
+  refresh_bc(s);
+  stage_closures(s, dt_stab);
+  return 0;
+}
+
+double om_wall_tau(double u_par, double y_w, double nu) { return wall_tau(u_par, y_w, nu); }
+
 static int advance_rk3w(om_solver *s, double dt) {
   const int nc = s->nc;
   memset(s->H, 0, sizeof(double) * 3 * (size_t)nc);
@@ -1198,17 +1222,7 @@ static int advance_rk3w(om_solver *s, double dt) {
     const double gdt = RK3_G[k] * dt;
     s->time = time_n + RK3_C[k] * dt;
     refresh_bc(s);
-    compute_all_gradients(s);
-    if (s->p.sgs == OM_SGS_VREMAN)
-      vreman(s, gdt);
-    else if (s->p.sgs == OM_SGS_SMAGORINSKY)
-      smagorinsky(s, gdt);
-    else
-      memset(s->nu_sgs, 0, sizeof(double) * nc);
-    if (s->p.wall_model == OM_WM_NONE)
-      memset(s->tau_w, 0, sizeof(double) * 3 * (size_t)s->nb);
-    else
-      wall_model(s);
+    stage_closures(s, gdt);
     explicit_rhs(s);
     const double a = RK3_A[k], bb = RK3_B[k];
     for (int c = 0; c < nc; ++c)
@@ -1636,6 +1650,14 @@ double *om_array(om_solver *s, const char *name, int64_t *len) {
       return t[i].p;
     }
   if (len) *len = 0;
+  return NULL;
+}
+
+int32_t *om_int_array(om_solver *s, const char *name, int64_t *len) {
+  if (strcmp(name, "ps_owner") == 0) { *len = s->npf; return s->ps_owner; }
+  if (strcmp(name, "ps_nb") == 0) { *len = s->npf; return s->ps_nb; }
+  if (strcmp(name, "ps_skip") == 0) { *len = s->npf; return s->ps_skip; }
+  *len = 0;
   return NULL;
 }
 

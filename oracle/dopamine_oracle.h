@@ -76,6 +76,15 @@ int om_advance(om_solver *s, double dt_req, double *dt_used);
  * max_abs_divU, volavg_abs_divU}. */
 void om_stats(om_solver *s, double out[8]);
 double om_time(const om_solver *s);
+/* Boundary refresh, then the closures a RK3W stage starts with (velocity gradients into "grad_U",
+ * SGS viscosity into "nu_sgs" with stability cap 0.4 V^(2/3) / dt_stab, wall traction into
+ * "tau_w_bnd"), on the current U, without advancing. */
+int om_eval_closures(om_solver *s, double dt_stab);
+/* Wall shear stress tau_w = u_tau^2 of the Reichardt-law Newton solve for a wall-parallel speed
+ * u_par sampled at wall distance y_w (src/sgs/mod_wall_model.f90:50-115). */
+double om_wall_tau(double u_par, double y_w, double nu);
+/* Pressure-operator face lists ("ps_owner", "ps_nb", "ps_skip"), 0-based. */
+int32_t *om_int_array(om_solver *s, const char *name, int64_t *len);
 /* Counters: total pressure solves, total CG iterations, last CG iteration count. */
 void om_counters(const om_solver *s, int64_t out[3]);
 int om_nonortho_info(const om_solver *s, int out[3]); /* skip_nonortho, skip_grad_p, n_nonortho_corr */

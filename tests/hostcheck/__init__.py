@@ -1,0 +1,96 @@
+"""Host-check solver (test infrastructure): the product's kernel bodies and RK3W driver
+(dopamine_amd/csrc/solver_impl.h) compiled for the CPU, so CPU tests can compare them with the
+oracle. Never used by the product."""
+import ctypes
+import os
+
+import numpy as np
+
+from dopamine_amd.solver import MeshDesc, Params, STATS_COLUMNS, build_abi
+
+_P = ctypes.POINTER
+_d, _i64 = ctypes.c_double, ctypes.c_int64
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhostcheck.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C tests/hostcheck`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.hc_solver_create.restype = ctypes.c_void_p
+        L.hc_solver_create.argtypes = [_P(MeshDesc), _P(Params)]
+        L.hc_solver_destroy.argtypes = [ctypes.c_void_p]
+        L.hc_last_error.restype = ctypes.c_char_p
+        L.hc_solver_field.restype = _P(_d)
+        L.hc_solver_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _P(_i64)]
+        L.hc_solver_apply_all_bc.argtypes = [ctypes.c_void_p]
+        L.hc_solver_project_once.argtypes = [ctypes.c_void_p, _d]
+        L.hc_solver_advance.argtypes = [ctypes.c_void_p, _d, _P(_d)]
+        L.hc_solver_stats.argtypes = [ctypes.c_void_p, _d * 8]
+        L.hc_solver_time.restype = _d
+        L.hc_solver_time.argtypes = [ctypes.c_void_p]
+        L.hc_solver_counters.argtypes = [ctypes.c_void_p, _i64 * 4]
+        _lib = L
+    return _lib
+
+
+class HostSolver:
+    def __init__(self, mesh, params, precision=0):
+        params.validate()
+        self.params = params
+        self.md, self.pr, self._keep = build_abi(mesh, params, precision)
+        self.h = lib().hc_solver_create(ctypes.byref(self.md), ctypes.byref(self.pr))
+        if not self.h:
+            raise RuntimeError(lib().hc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().hc_solver_destroy(self.h)
+            self.h = None
+
+    def array(self, name):
+        n = _i64(0)
+        p = lib().hc_solver_field(self.h, name.encode(), ctypes.byref(n))
+        return np.ctypeslib.as_array(p, shape=(n.value,))
+
+    def get(self, name):
+        return self.array(name).copy()
+
+    @property
+    def U(self):
+        return self.array("U").reshape(-1, 3)
+
+    @property
+    def p(self):
+        return self.array("p")
+
+    def set_velocity(self, U):
+        self.array("U")[:] = np.asarray(U, dtype=np.float64).reshape(-1)
+        lib().hc_solver_apply_all_bc(self.h)
+
+    def project_once(self, dt):
+        return lib().hc_solver_project_once(self.h, dt)
+
+    def advance(self, dt=None):
+        used = _d(0.0)
+        rc = lib().hc_solver_advance(self.h, self.params.dt if dt is None else dt, ctypes.byref(used))
+        if rc != 0:
+            raise RuntimeError(f"advance failed {rc}")
+        return used.value
+
+    @property
+    def time(self):
+        return lib().hc_solver_time(self.h)
+
+    def stats(self):
+        out = (_d * 8)()
+        lib().hc_solver_stats(self.h, out)
+        return dict(zip(STATS_COLUMNS, list(out)))
+
+    def counters(self):
+        out = (_i64 * 4)()
+        lib().hc_solver_counters(self.h, out)
+        return {"solves": out[0], "iterations": out[1], "last": out[2], "sweeps": out[3]}

@@ -1,0 +1,54 @@
+// Host-check build of the product solver: the same kernel bodies and driver as
+// libdopamine_hip.so (dopamine_amd/csrc/solver_impl.h), executed by HostExec on the CPU, so the
+// CPU test suite can compare the product's indexing and operation order with the oracle without
+// a GPU. Test infrastructure only; the product library never contains HostExec.
+#include <string>
+
+#include "solver_impl.h"
+
+namespace {
+using Solver = dm::SolverT<dm::HostExec>;
+thread_local std::string g_error;
+
+double* field(Solver* s, const std::string& name, int64_t* n) {
+  const int64_t nc = s->nc, nb = std::max(1, s->nb), nf = s->nf;
+  if (name == "U") { *n = 3 * nc; return s->U; }
+  if (name == "p") { *n = nc; return s->P; }
+  if (name == "nu_sgs") { *n = nc; return s->nu_sgs; }
+  if (name == "U_bnd") { *n = 3 * nb; return s->U_bnd; }
+  if (name == "p_bnd") { *n = nb; return s->p_bnd; }
+  if (name == "grad_U") { *n = 9 * nc; return s->grad; }
+  if (name == "U_face") { *n = nf; return s->UF; }
+  if (name == "tau_w_bnd") { *n = 3 * nb; return s->tau; }
+  if (name == "rhs_p") { *n = nc; return s->rhs; }
+  if (name == "w_gp") { *n = 3 * nc; return s->gp; }
+  *n = 0;
+  return nullptr;
+}
+}  // namespace
+
+extern "C" {
+const char* hc_last_error(void) { return g_error.c_str(); }
+
+void* hc_solver_create(const dopamine_mesh_desc* m, const dopamine_params* p) {
+  try {
+    auto* s = new Solver();
+    s->create(*m, *p);
+    return s;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+void hc_solver_destroy(void* h) { delete static_cast<Solver*>(h); }
+double* hc_solver_field(void* h, const char* name, int64_t* n) { return field(static_cast<Solver*>(h), name, n); }
+void hc_solver_apply_all_bc(void* h) { static_cast<Solver*>(h)->apply_all_bc(); }
+int hc_solver_project_once(void* h, double dt) { return static_cast<Solver*>(h)->project_once(dt); }
+int hc_solver_advance(void* h, double dt, double* used) { return static_cast<Solver*>(h)->advance(dt, used); }
+void hc_solver_stats(void* h, double out[8]) { static_cast<Solver*>(h)->stats(out); }
+double hc_solver_time(void* h) { return static_cast<Solver*>(h)->time; }
+void hc_solver_counters(void* h, int64_t out[4]) {
+  auto* s = static_cast<Solver*>(h);
+  out[0] = s->n_solves; out[1] = s->n_iters; out[2] = s->last_iters; out[3] = s->n_sweeps;
+}
+}

@@ -1,0 +1,87 @@
+"""The fvm_petsc_* pressure-backend ABI on the GPU, called exactly as mod_pressure_solver.f90 calls
+it (1-based index arrays), against the oracle's Jacobi CG and a direct sparse solve."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from dopamine_amd import cases
+from dopamine_amd._lib import lib
+
+pytestmark = pytest.mark.gpu
+_P = ctypes.POINTER
+
+
+def _operator(name):
+    case = cases.regression_cases()[name]
+    mesh = case.make_mesh()
+    o = oracle.Solver(mesh, case.params)
+    own = o.int_array("ps_owner").copy()
+    nb = o.int_array("ps_nb").copy()
+    skip = o.int_array("ps_skip").copy()
+    coeff = o.array("ps_coeff").copy()
+    diag = o.array("ps_diag").copy()
+    return mesh, own, nb, skip, coeff, diag
+
+
+def _call(n, own, nb, skip, diag, rhs, coeff, x0, rtol, atol, n_dir):
+    L = lib()
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    l2g = i32(np.arange(1, n + 1))
+    fo, fnl, fng, sk = i32(own + 1), i32(nb + 1), i32(nb + 1), i32(skip)
+    dg, b, cf, x = f64(diag), f64(rhs), f64(coeff), f64(x0).copy()
+    its = ctypes.c_int(0)
+    res = ctypes.c_double(0.0)
+    drows = i32(np.zeros(max(1, n_dir)))
+    dvals = f64(np.zeros(max(1, n_dir)))
+    ip = lambda a: a.ctypes.data_as(_P(ctypes.c_int))  # noqa: E731
+    dp = lambda a: a.ctypes.data_as(_P(ctypes.c_double))  # noqa: E731
+    L.fvm_petsc_solve_pressure.restype = ctypes.c_int
+    L.fvm_petsc_solve_pressure.argtypes = [ctypes.c_int] * 3 + [_P(ctypes.c_int)] * 5 + [_P(ctypes.c_double)] * 3 + \
+        [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P(ctypes.c_int), _P(ctypes.c_double),
+         ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int), _P(ctypes.c_double)]
+    rc = L.fvm_petsc_solve_pressure(n, n, len(own), ip(l2g), ip(fo), ip(fnl), ip(fng), ip(sk), dp(dg), dp(b), dp(cf),
+                                    1, 1000, rtol, atol, ip(drows), dp(dvals), n_dir, dp(x), ctypes.byref(its),
+                                    ctypes.byref(res))
+    return rc, x, its.value, res.value
+
+
+@pytest.mark.parametrize("name", ["tgv_skew", "hydrowall", "hydro"])
+def test_fvm_petsc_solve_pressure_matches_oracle_and_direct_solve(name):
+    mesh, own, nb, skip, coeff, diag = _operator(name)
+    n = mesh.n_cells
+    L = lib()
+    assert L.fvm_petsc_backend_available() == 1
+    assert L.fvm_petsc_backend_initialize() == 0
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(n)
+    n_dir = int(np.sum(np.abs(diag - np.bincount(own[skip == 0], coeff[skip == 0], n)
+                              - np.bincount(nb[skip == 0], coeff[skip == 0], n)) > 1e-12 * diag))
+    nullspace = n_dir == 0
+    if nullspace:
+        b -= b.mean()
+    rc, x, its, res = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, n_dir)
+    print(f"[gpu] {name}: rc {rc} its {its} resid {res:.3e}", flush=True)
+    assert rc == 0
+    xo, its_o, res_o, rc_o = oracle.cg_jacobi(own[skip == 0], nb[skip == 0], coeff[skip == 0], diag, b, np.zeros(n),
+                                              1e-12, 1e-300, 2000, nullspace)
+    assert rc_o == 0
+    assert abs(its - its_o) <= 1
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9
+    keep = skip == 0
+    A = sp.coo_matrix((np.concatenate([diag, -coeff[keep], -coeff[keep]]),
+                       (np.concatenate([np.arange(n), own[keep], nb[keep]]),
+                        np.concatenate([np.arange(n), nb[keep], own[keep]]))), shape=(n, n)).tocsr()
+    if nullspace:  # pin the constant: compare mean-free parts
+        xd = spla.spsolve((A + sp.coo_matrix((np.ones(1), ([0], [0])), shape=(n, n))).tocsc(), b)
+        xd -= xd.mean()
+        xg = x - x.mean()
+    else:
+        xd = spla.spsolve(A.tocsc(), b)
+        xg = x
+    assert np.linalg.norm(xg - xd) / np.linalg.norm(xd) < 1e-8
+    assert L.fvm_petsc_backend_finalize() == 0
