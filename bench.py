@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Throughput of the RK3W fractional step on one MI355X: cell-updates/s per RK3 step.
+
+Workload: turbulent channel 2pi x 2 x pi, 128 x 96 x 64 = 786,432 hex cells, tanh wall-normal
+stretching (factor 2), periodic in x and z, Re_tau 395 (nu = 1/395, f_x = 1), KEP advection,
+Vreman SGS, log-law wall model (kappa 0.41, B 5.2), laminar parabolic profile with a seeded 10%
+uniform perturbation, pressure by Jacobi CG to relative residual 1e-8, fixed dt = 1e-3.
+
+A step is one solver_advance (three RK3W stages with their pressure solves) with every field
+resident in HBM. `value` = cells x timed steps / wall time of the timed steps. The roofline entry
+is for the dominant kernel (the CG direction/operator pass), from HIP events on the solver's
+stream over the timed steps; the CPU baseline is the FP64 C restatement of the same algorithm run
+single-threaded on one step of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s per RK3 step at p-residual 1e-8: learned/mixed vs plain FP64"
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mesh", type=int, nargs=3, default=(128, 96, 64), metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--precision", choices=("fp64",), default="fp64")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="RK3 steps of the CPU baseline (0 = skip)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def dir_pass_bytes(mesh):
+    """Algorithmic HBM bytes of one CG direction/operator pass, FP64 values and int32 indices:
+    per cell diag, z, p_old read and p, q written (5 x 8 B); per operator face one coefficient and
+    two indices (8 + 2 x 4 B)."""
+    nfp = mesh.n_int_faces + int(np.count_nonzero(mesh.periodic_pair >= 0)) // 2
+    return 40 * mesh.n_cells + 16 * nfp, nfp
+
+
+def cpu_baseline(mesh, case, U, p, steps):
+    import oracle
+    o = oracle.Solver(mesh, case.params)
+    o.array("p")[:] = p
+    o.set_velocity(U)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.advance()
+    dt = time.perf_counter() - t0
+    its = o.counters()["iterations"]
+    return {"value": mesh.n_cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK3 step(s) of the same {mesh.n_cells}-cell channel from the GPU state after the "
+                      f"timed steps; FP64 C restatement, one thread; {its} CG iterations; {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the path's kernels at one GPU's share; "
+                                   "--gpus must be 1"}))
+        sys.exit(2)
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from dopamine_amd import cases, params as P
+    from dopamine_amd.solver import FVSolver
+
+    nx, ny, nz = args.mesh
+    case = cases.channel(nx, ny, nz)
+    t0 = time.perf_counter()
+    mesh = case.make_mesh()
+    log(f"mesh {nx}x{ny}x{nz}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
+    solver = FVSolver(mesh, case.params)
+    P.initialize(solver, case, mesh)
+    solver.sync()
+    log("initial projections done")
+    for i in range(args.warmup):
+        solver.advance()
+        log(f"warmup step {i + 1}/{args.warmup}: {solver.counters()['last']} CG iterations in the last solve")
+    solver.sync()
+
+    kernels = ("cg_dir", "cg_upd", "explicit", "ls_grad")
+    for k in kernels:
+        solver.kernel_timing(k, True)
+    it0 = solver.counters()["iterations"]
+    solver.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        solver.advance()
+    solver.sync()
+    elapsed = time.perf_counter() - t0
+    its = solver.counters()["iterations"] - it0
+    ktime = {k: solver.kernel_time(k) for k in kernels}
+    for k in kernels:
+        solver.kernel_timing(k, False)
+    log(f"timed {args.steps} steps in {elapsed:.3f} s, {its} CG iterations")
+
+    nc = mesh.n_cells
+    bytes_dir, nfp = dir_pass_bytes(mesh)
+    n_dir, ms_dir = ktime["cg_dir"]
+    avg_s = ms_dir / max(n_dir, 1) / 1e3
+    achieved = bytes_dir / avg_s / 1e9 if avg_s > 0 else 0.0
+    st = solver.stats()
+    out = {
+        "metric": METRIC,
+        "value": nc * args.steps / elapsed,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh",
+        "config": {"workload": f"turbulent channel {nx}x{ny}x{nz} ({nc} hex cells, {nfp} pressure faces), "
+                               "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, plain FP64",
+                   "precision": args.precision, "dt": case.params.dt},
+        "roofline": {"bound": "hbm", "kernel": "CG direction/operator pass (k_reduce<1,false,cgk::Dir,...>)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
+                     "launches": n_dir},
+        "kernels_us": {k: (v[1] * 1e3 / v[0] if v[0] else None) for k, v in ktime.items()},
+        "cg_iterations_per_step": its / args.steps,
+        "state": {"E": st["E"], "Ub": st["Ub"], "max_abs_divU": st["max_abs_divU"]},
+        "cpu_baseline": None,
+    }
+    if args.cpu_steps > 0:
+        U, p = solver.U, solver.p
+        log("CPU baseline (oracle restatement, one thread) ...")
+        out["cpu_baseline"] = cpu_baseline(mesh, case, U, p, args.cpu_steps)
+        log(f"CPU baseline: {out['cpu_baseline']['sample']}")
+    solver.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

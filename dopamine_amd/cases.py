@@ -96,7 +96,7 @@ def channel(nx=128, ny=96, nz=64):
         SolverParams(nu=1.0 / 395.0, f_const=(1.0, 0.0, 0.0), advection_scheme="KEP", use_sgs=True,
                      sgs_model="VREMAN", dt=1.0e-3, cfl_max=0.5, p_tol=1.0e-8, p_max_iter=1000,
                      n_nonortho_corr=0, rhie_chow_dual=False, cell_projection="LS", pressure_min_cos_theta=0.0),
-        ic="channel_parabolic", ic_uref=1.0, nsteps=500)
+        ic="channel_parabolic", ic_uref=1.0, nsteps=500, extra={"noise": 0.1, "seed": 395})
 
 
 def small_channel(nx=16, ny=12, nz=8):

@@ -6,6 +6,7 @@
 // removed from b and from every preconditioned residual when no Dirichlet row exists, and the
 // true residual ||b - A x||_2 reported at the end. Two fused passes per iteration; convergence is
 // decided on the device (later passes become no-ops), so the host only polls between batches.
+// Every dot product is a fixed-order sum, so a solve is bitwise reproducible.
 //
 // Operator storage: ELLPACK rows, slot-major (slot k of every row contiguous) so that the 64 lanes
 // of a wavefront read consecutive rows. A_ii = diag_i, A_ij = -coef for each listed j.
@@ -14,13 +15,16 @@
 #include "dm_common.h"
 
 #include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
 
 namespace dm {
 
 struct CgScalars {
-  double bmean, ttol, rr, rz, rz_old, pq, zmean, resid;
+  double ttol, rr, rz_hist[2], resid;
   double rtol, atol;
-  int its, done, maxit, first, nullspace, atol_from_dev;
+  int its, done, maxit, nullspace;
 };
 
 struct EllOp {
@@ -30,12 +34,21 @@ struct EllOp {
   const double* diag = nullptr;  // n (already repaired: non-finite or tiny -> 1)
 };
 
+// Chained passes of one solve. Each reads the totals of the previous pass's partials (identical
+// in every block), derives its scalars in setup(), and returns per-cell contributions to its own
+// partials. `leader` is one thread of the grid: it publishes scalars for later passes and the
+// host. A pass that finds the solve finished returns false from setup() in every block.
 namespace cgk {
 
+struct NoScal {};
+
+// b' = finite(b), x scrubbed; partial: sum b'
 struct SumB {
   const double* b;
   double *bw, *x;
-  DM_HD Red<1> operator()(int c) const {
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<1>&, bool, Scal&) const { return true; }
+  DM_HD Red<1> operator()(int c, const Scal&) const {
     double bi = b[c];
     if (!isfinite(bi)) bi = 0.0;
     bw[c] = bi;
@@ -45,24 +58,25 @@ struct SumB {
     return r;
   }
 };
-struct SumBFin {
-  CgScalars* sc;
-  int n;
-  const double* atol_dev;
-  DM_HD void operator()(const Red<1>& t) const {
-    sc->bmean = sc->nullspace ? t.v[0] / double(n) : 0.0;
-    if (atol_dev) sc->atol = *atol_dev;
-  }
-};
 
+// b' -= mean(b') on a singular operator; r = b' - A x; partials: b'.b', r.r
 struct Init {
   EllOp A;
   double *bw, *r;
   const double* x;
-  const CgScalars* sc;
-  DM_HD Red<2> operator()(int c) const {
+  CgScalars* sc;
+  const double* atol_dev;
+  struct Scal {
+    double bmean;
+  };
+  DM_HD bool setup(const Red<1>& t, bool leader, Scal& s) const {
+    s.bmean = sc->nullspace ? t.v[0] / double(A.n) : 0.0;
+    if (leader && atol_dev) sc->atol = *atol_dev;
+    return true;
+  }
+  DM_HD Red<2> operator()(int c, const Scal& s) const {
     const int n = A.n;
-    const double bp = bw[c] - sc->bmean;
+    const double bp = bw[c] - s.bmean;
     bw[c] = bp;
     double ax = A.diag[c] * x[c];
     for (int k = 0; k < A.w; ++k) {
@@ -78,58 +92,68 @@ struct Init {
     return o;
   }
 };
-struct InitFin {
-  CgScalars* sc;
-  DM_HD void operator()(const Red<2>& t) const {
-    sc->ttol = fmax(sc->rtol * sqrt(t.v[0]), sc->atol);
-    sc->rr = t.v[1];
-    sc->its = 0;
-    sc->first = 1;
-    sc->done = (sqrt(t.v[1]) <= sc->ttol) ? 1 : 0;
-  }
-};
 
-struct Z {
+// ttol = max(rtol ||b'||, atol); z = D^-1 r; partials: r.r, r.z, sum z, sum r
+struct Z0 {
   const double *diag, *r;
   double* z;
-  DM_HD Red<3> operator()(int c) const {
+  CgScalars* sc;
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<2>& t, bool leader, Scal&) const {
+    if (leader) sc->ttol = fmax(sc->rtol * sqrt(t.v[0]), sc->atol);
+    return true;
+  }
+  DM_HD Red<4> operator()(int c, const Scal&) const {
     const double rc = r[c], zc = rc / diag[c];
     z[c] = zc;
-    Red<3> o;
-    o.v[0] = rc * zc;
-    o.v[1] = zc;
-    o.v[2] = rc;
+    Red<4> o;
+    o.v[0] = rc * rc;
+    o.v[1] = rc * zc;
+    o.v[2] = zc;
+    o.v[3] = rc;
     return o;
   }
 };
-struct ZFin {
-  CgScalars* sc;
-  int n;
-  DM_HD void operator()(const Red<3>& t) const {
-    sc->zmean = sc->nullspace ? t.v[1] / double(n) : 0.0;
-    sc->rz = t.v[0] - sc->zmean * t.v[2];
-  }
-};
 
-// p = (z - zmean) + beta p_old for this row and, recomputed on the fly, for its neighbours;
-// q = A p; partial p.q.
+// Iteration k, first pass. From the residual partials of the last update: the convergence test
+// ||r_k|| <= ttol (KSPCG tests after each update, and once before the first), the projected
+// r.z = r.z - mean(z) sum r, and beta = (r.z)_k / (r.z)_(k-1). Then p = (z - mean z) + beta p_old
+// for this row and, recomputed on the fly, for its neighbours; q = A p; partial p.q.
 struct Dir {
   EllOp A;
   const double *z, *pold;
   double *pnew, *q;
-  const CgScalars* sc;
-  DM_HD Red<1> operator()(int c) const {
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double zm, beta;
+  };
+  DM_HD bool setup(const Red<4>& t, bool leader, Scal& s) const {
+    if (sc->done) return false;
+    const double rr = t.v[0];
+    s.zm = sc->nullspace ? t.v[2] / double(A.n) : 0.0;
+    const double rz = t.v[1] - s.zm * t.v[3];
+    const int fin = (sqrt(rr) <= sc->ttol) ? 1 : (k >= sc->maxit ? 2 : 0);
+    if (leader) {
+      sc->its = k;
+      sc->rr = rr;
+      if (fin) sc->done = fin;
+      else sc->rz_hist[k & 1] = rz;
+    }
+    if (fin) return false;
+    s.beta = k == 0 ? 0.0 : rz / sc->rz_hist[(k + 1) & 1];
+    return true;
+  }
+  DM_HD Red<1> operator()(int c, const Scal& s) const {
     const int n = A.n;
-    const int first = sc->first;
-    const double zm = sc->zmean;
-    const double beta = first ? 0.0 : sc->rz / sc->rz_old;
-    const double pc = first ? (z[c] - zm) : (z[c] - zm) + beta * pold[c];
+    const bool first = k == 0;
+    const double pc = first ? (z[c] - s.zm) : (z[c] - s.zm) + s.beta * pold[c];
     double qc = A.diag[c] * pc;
-    for (int k = 0; k < A.w; ++k) {
-      const int j = A.nbr[size_t(k) * n + c];
+    for (int kk = 0; kk < A.w; ++kk) {
+      const int j = A.nbr[size_t(kk) * n + c];
       if (j < 0) continue;
-      const double pj = first ? (z[j] - zm) : (z[j] - zm) + beta * pold[j];
-      qc = qc - A.coef[size_t(k) * n + c] * pj;
+      const double pj = first ? (z[j] - s.zm) : (z[j] - s.zm) + s.beta * pold[j];
+      qc = qc - A.coef[size_t(kk) * n + c] * pj;
     }
     pnew[c] = pc;
     q[c] = qc;
@@ -138,21 +162,25 @@ struct Dir {
     return o;
   }
 };
-struct DirFin {
-  CgScalars* sc;
-  DM_HD void operator()(const Red<1>& t) const { sc->pq = t.v[0]; }
-};
 
-// x += alpha p, r -= alpha q, ||r||, z = D^-1 r and the next r.z (KSPCG order: count, residual
-// norm, test, then the next preconditioner apply).
+// Iteration k, second pass: alpha = (r.z)_k / p.q; x += alpha p, r -= alpha q, z = D^-1 r;
+// partials r.r, r.z, sum z, sum r for the next test.
 struct Upd {
   const double *diag, *p, *q;
   double *x, *r, *z;
-  const CgScalars* sc;
-  DM_HD Red<4> operator()(int c) const {
-    const double a = sc->rz / sc->pq;
-    x[c] = x[c] + a * p[c];
-    const double rc = r[c] - a * q[c];
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double a;
+  };
+  DM_HD bool setup(const Red<1>& t, bool, Scal& s) const {
+    if (sc->done) return false;
+    s.a = sc->rz_hist[k & 1] / t.v[0];
+    return true;
+  }
+  DM_HD Red<4> operator()(int c, const Scal& s) const {
+    x[c] = x[c] + s.a * p[c];
+    const double rc = r[c] - s.a * q[c];
     r[c] = rc;
     const double zc = rc / diag[c];
     z[c] = zc;
@@ -164,27 +192,14 @@ struct Upd {
     return o;
   }
 };
-struct UpdFin {
-  CgScalars* sc;
-  int n;
-  DM_HD void operator()(const Red<4>& t) const {
-    sc->its = sc->its + 1;
-    sc->rr = t.v[0];
-    if (sqrt(t.v[0]) <= sc->ttol)
-      sc->done = 1;
-    else if (sc->its >= sc->maxit)
-      sc->done = 2;
-    sc->zmean = sc->nullspace ? t.v[2] / double(n) : 0.0;
-    sc->rz_old = sc->rz;
-    sc->rz = t.v[1] - sc->zmean * t.v[3];
-    sc->first = 0;
-  }
-};
 
+// true residual ||b - A x||^2 (partials summed on the host)
 struct Resid {
   EllOp A;
   const double *b, *x;
-  DM_HD Red<1> operator()(int c) const {
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<1>&, bool, Scal&) const { return true; }
+  DM_HD Red<1> operator()(int c, const Scal&) const {
     const int n = A.n;
     double ax = A.diag[c] * x[c];
     for (int k = 0; k < A.w; ++k) {
@@ -199,10 +214,6 @@ struct Resid {
     return o;
   }
 };
-struct ResidFin {
-  CgScalars* sc;
-  DM_HD void operator()(const Red<1>& t) const { sc->resid = sqrt(t.v[0]); }
-};
 
 }  // namespace cgk
 
@@ -212,6 +223,7 @@ class CgT {
   void init(X* x, int n) {
     x_ = x;
     n_ = n;
+    g_ = x->pass_grid(n);
     bw_ = x->template alloc<double>(n);
     r_ = x->template alloc<double>(n);
     z_ = x->template alloc<double>(n);
@@ -219,10 +231,13 @@ class CgT {
     pb_ = x->template alloc<double>(n);
     q_ = x->template alloc<double>(n);
     sc_ = x->template alloc<CgScalars>(1);
+    part_ = x->template alloc<double>(size_t(PARTS) * g_);
+    hpart_.resize(g_);
   }
   void destroy() {
     if (!x_) return;
-    for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_}) x_->release(p);
+    for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_})
+      x_->release(p);
     x_ = nullptr;
   }
   // Solve A x = b (pointers in the execution space). atol_dev, when not null, points to an
@@ -232,45 +247,63 @@ class CgT {
     using namespace cgk;
     X& xx = *x_;
     const int n = A.n;
+    if (n > n_) throw std::runtime_error("CG operator larger than its workspace");
+    const int g = xx.pass_grid(n);
+    double* P0 = part_;              // 1 x g
+    double* P1 = part_ + g;          // 2 x g
+    double* D = part_ + 3 * size_t(g);   // 1 x g
+    double* R = part_ + 4 * size_t(g);   // 1 x g
+    double* U[3] = {part_ + 5 * size_t(g), part_ + 9 * size_t(g), part_ + 13 * size_t(g)};  // 4 x g each
     CgScalars h{};
     h.rtol = rtol;
     h.atol = atol;
     h.maxit = maxit;
     h.nullspace = nullspace ? 1 : 0;
     xx.upload(sc_, &h, 1);
-    xx.template reduce<1, false>(n, SumB{b, bw_, x}, SumBFin{sc_, n, atol_dev});
-    xx.template reduce<2, false>(n, Init{A, bw_, r_, x, sc_}, InitFin{sc_});
-    xx.template reduce<3, false>(n, Z{A.diag, r_, z_}, ZFin{sc_, n}, &sc_->done);
-    int launched = 0;
-    int batch = std::max(4, std::min(maxit, predict_ - 1));
+    xx.template pass<1, 1>(n, SumB{b, bw_, x}, nullptr, 0, P0);
+    xx.template pass<1, 2>(n, Init{A, bw_, r_, x, sc_, atol_dev}, P0, g, P1);
+    xx.template pass<2, 4>(n, Z0{A.diag, r_, z_, sc_}, P1, g, U[2]);
+    int k = 0;
+    int batch = std::max(4, std::min(maxit + 1, predict_));
     double* pold = pa_;
     double* pnew = pb_;
     for (;;) {
-      for (int i = 0; i < batch; ++i) {
-        xx.template reduce<1, false>(n, Dir{A, z_, pold, pnew, q_, sc_}, DirFin{sc_}, &sc_->done);
-        xx.template reduce<4, false>(n, Upd{A.diag, pnew, q_, x, r_, z_, sc_}, UpdFin{sc_, n}, &sc_->done);
+      for (int i = 0; i < batch; ++i, ++k) {
+        xx.tbegin(K_CG_DIR);
+        xx.template pass<4, 1>(n, Dir{A, z_, pold, pnew, q_, sc_, k}, U[(k + 2) % 3], g, D);
+        xx.tend(K_CG_DIR);
+        xx.tbegin(K_CG_UPD);
+        xx.template pass<1, 4>(n, Upd{A.diag, pnew, q_, x, r_, z_, sc_, k}, D, g, U[k % 3]);
+        xx.tend(K_CG_UPD);
         std::swap(pold, pnew);
       }
-      launched += batch;
       xx.download(&h, sc_, 1);
-      if (h.done || launched > maxit + 1) break;
+      if (h.done || k > maxit + 1) break;
       batch = std::max(2, std::min(16, (h.its + 7) / 8));
     }
-    xx.template reduce<1, false>(n, Resid{A, b, x}, ResidFin{sc_});
+    xx.template pass<1, 1>(n, Resid{A, b, x}, nullptr, 0, R);
+    xx.download(hpart_.data(), R, size_t(g));
     xx.download(&h, sc_, 1);
+    double rr = 0.0;
+    for (int q = 0; q < g; ++q) rr = rr + hpart_[q];
+    xx.tcommit(K_CG_DIR, h.its);
+    xx.tcommit(K_CG_UPD, h.its);
     last_its_ = h.its;
-    predict_ = std::max(2, h.its);
+    predict_ = std::max(2, h.its + 1);
     if (its) *its = h.its;
-    if (resid) *resid = h.resid;
+    if (resid) *resid = std::sqrt(rr);
     return h.done == 1 ? 0 : -1;
   }
   int last_its() const { return last_its_; }
 
  private:
+  static constexpr int PARTS = 17;  // partial quantities per block: P0 1, P1 2, D 1, R 1, U 3 x 4
   X* x_ = nullptr;
-  int n_ = 0;
+  int n_ = 0, g_ = 1;
   double *bw_ = nullptr, *r_ = nullptr, *z_ = nullptr, *pa_ = nullptr, *pb_ = nullptr, *q_ = nullptr;
+  double* part_ = nullptr;
   CgScalars* sc_ = nullptr;
+  std::vector<double> hpart_;
   int last_its_ = 0, predict_ = 24;
 };
 

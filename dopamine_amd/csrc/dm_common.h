@@ -52,7 +52,7 @@ DM_INLINE bool is_solid(int bc) {
 
 template <int N>
 struct Red {
-  double v[N];
+  double v[N > 0 ? N : 1];
 };
 
 template <int N, bool MAX>
@@ -98,7 +98,30 @@ struct HostExec {
     }
     fin(s);
   }
+  int pass_grid(int) const { return 1; }
+  // chained pass with one block (see DeviceExec::pass)
+  template <int NIN, int NOUT, class F>
+  void pass(int n, const F& f, const double* in_part, int in_g, double* out_part) {
+    Red<NIN> tot = red_identity<NIN, false>();
+    for (int q = 0; q < in_g; ++q)
+      for (int k = 0; k < NIN; ++k) tot.v[k] = tot.v[k] + in_part[size_t(k) * in_g + q];
+    typename F::Scal sc;
+    if (!f.setup(tot, true, sc)) return;
+    Red<NOUT> acc = red_identity<NOUT, false>();
+    for (int c = 0; c < n; ++c) {
+      const Red<NOUT> v = f(c, sc);
+      for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
+    }
+    for (int k = 0; k < NOUT; ++k) out_part[k] = acc.v[k];
+  }
   void sync() {}
+  // kernel timing is a device feature; on the host these are no-ops
+  void tbegin(int) {}
+  void tend(int) {}
+  void tcommit(int, int) {}
 };
+
+// Kernel classes with device timing (dopamine_solver_kernel_timing).
+enum KernelClass { K_CG_DIR = 0, K_CG_UPD = 1, K_EXPLICIT = 2, K_LS_GRAD = 3, K_NCLASS = 4 };
 
 }  // namespace dm

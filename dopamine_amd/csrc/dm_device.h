@@ -5,6 +5,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 namespace dm {
 
 inline void check(hipError_t e, const char* what) {
@@ -34,10 +37,11 @@ __device__ inline Red<N> wave_reduce(Red<N> x) {
   return x;
 }
 
-// Fixed-shape block reduction (wavefront shuffles, then the 4 wavefront sums in order).
-template <int N, bool MAX>
+// Fixed-shape block reduction (wavefront shuffles, then the wavefront sums in order). Every
+// thread returns the block total, and every block of any grid sums in the same order.
+template <int N, bool MAX, int NT = BLOCK>
 __device__ inline Red<N> block_reduce(Red<N> x) {
-  __shared__ double sh[BLOCK / 64][N];
+  __shared__ double sh[NT / 64][N];
   x = wave_reduce<N, MAX>(x);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0)
@@ -49,49 +53,11 @@ __device__ inline Red<N> block_reduce(Red<N> x) {
   for (int k = 0; k < N; ++k) {
     double a = sh[0][k];
 #pragma unroll
-    for (int q = 1; q < BLOCK / 64; ++q) a = MAX ? fmax(a, sh[q][k]) : a + sh[q][k];
+    for (int q = 1; q < NT / 64; ++q) a = MAX ? fmax(a, sh[q][k]) : a + sh[q][k];
     r.v[k] = a;
   }
   __syncthreads();
   return r;
-}
-
-// Each block writes one partial per quantity; the last block to take a ticket sums the partials
-// in block order (agent-scope atomics keep them coherent across the XCDs' L2s) and returns true.
-// Every addition happens in a fixed order, so the result does not depend on block scheduling.
-template <int N, bool MAX>
-__device__ inline bool grid_reduce(Red<N> x, double* partials, unsigned* ticket, Red<N>* total) {
-  __shared__ bool last;
-  __shared__ double fin[N];
-  Red<N> b = block_reduce<N, MAX>(x);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-      __hip_atomic_store(&partials[size_t(blockIdx.x) * N + k], b.v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  Red<N> s = red_identity<N, MAX>();
-  for (int q = threadIdx.x; q < int(gridDim.x); q += BLOCK)
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const double pv = __hip_atomic_load(&partials[size_t(q) * N + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s.v[k] = MAX ? fmax(s.v[k], pv) : s.v[k] + pv;
-    }
-  s = block_reduce<N, MAX>(s);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) fin[k] = s.v[k];
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; ++k) total->v[k] = fin[k];
-  return true;
 }
 
 template <class F>
@@ -100,14 +66,64 @@ __global__ void __launch_bounds__(BLOCK) k_for(int n, F f) {
   if (i < n) f(i);
 }
 
-template <int N, bool MAX, class F, class Fin>
-__global__ void __launch_bounds__(BLOCK) k_reduce(int n, F f, Fin fin, double* part, unsigned* ticket, const int* gate) {
+// Grid reductions are two launches: every block stores its partials (layout [quantity][block]),
+// then one block sums them in block order and runs the finaliser. The kernel boundary orders the
+// two, so no block needs a device-scope fence (on gfx950 that is an L2 write-back and invalidate
+// per block). The result does not depend on block scheduling.
+template <int N, bool MAX, class F>
+__global__ void __launch_bounds__(BLOCK) k_partial(int n, F f, double* part, const int* gate) {
   if (gate && *gate) return;  // uniform across the grid
-  const int i = xcd_block(gridDim.x) * BLOCK + threadIdx.x;
+  const int b = xcd_block(gridDim.x);
+  const int i = b * BLOCK + threadIdx.x;
   Red<N> v = red_identity<N, MAX>();
   if (i < n) v = f(i);
-  Red<N> t;
-  if (grid_reduce<N, MAX>(v, part, ticket, &t) && threadIdx.x == 0) fin(t);
+  v = block_reduce<N, MAX>(v);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) part[size_t(k) * gridDim.x + b] = v.v[k];
+}
+
+template <int N, bool MAX, class Fin>
+__global__ void __launch_bounds__(BLOCK) k_final(int g, Fin fin, const double* part, const int* gate) {
+  if (gate && *gate) return;
+  Red<N> s = red_identity<N, MAX>();
+  for (int q = threadIdx.x; q < g; q += BLOCK)
+#pragma unroll
+    for (int k = 0; k < N; ++k) s.v[k] = MAX ? fmax(s.v[k], part[size_t(k) * g + q]) : s.v[k] + part[size_t(k) * g + q];
+  s = block_reduce<N, MAX>(s);
+  if (threadIdx.x == 0) fin(s);
+}
+
+// Chained reduction passes (the CG's iterations): a pass first sums the previous pass's block
+// partials -- every block redundantly, in the same order, so all blocks hold bit-identical totals
+// without any cross-block synchronisation -- lets the functor derive its scalars (and stop the
+// whole grid uniformly), then streams a contiguous chunk of cells per block and stores its own
+// block partials for the next pass.
+constexpr int PBLOCK = 512;
+constexpr int PGRID_MAX = 512;  // two blocks per CU
+
+template <int NIN, int NOUT, class F>
+__global__ void __launch_bounds__(PBLOCK) k_pass(int n, int chunk, F f, const double* in_part, int in_g,
+                                                 double* out_part) {
+  Red<NIN> tot = red_identity<NIN, false>();
+  for (int q = threadIdx.x; q < in_g; q += PBLOCK)
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) tot.v[k] = tot.v[k] + in_part[size_t(k) * in_g + q];
+  tot = block_reduce<NIN, false, PBLOCK>(tot);
+  const int b = xcd_block(gridDim.x);
+  typename F::Scal sc;
+  if (!f.setup(tot, b == 0 && threadIdx.x == 0, sc)) return;  // uniform across the grid
+  Red<NOUT> acc = red_identity<NOUT, false>();
+  const int lo = b * chunk, hi = min(n, lo + chunk);
+  for (int c = lo + int(threadIdx.x); c < hi; c += PBLOCK) {
+    const Red<NOUT> v = f(c, sc);
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
+  }
+  acc = block_reduce<NOUT, false, PBLOCK>(acc);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) out_part[size_t(k) * gridDim.x + b] = acc.v[k];
 }
 
 template <class T>
@@ -116,27 +132,105 @@ struct ZeroF {
   DM_HD void operator()(int i) const { p[i] = T(0); }
 };
 
+// Device timing of selected kernel classes: a pair of HIP events on the launching stream around
+// each launch, resolved whenever the stream has been synchronised (or the pool is full).
+struct KernelTimer {
+  static constexpr int POOL = 1024;  // event pairs
+  bool on[K_NCLASS] = {};
+  int64_t launches[K_NCLASS] = {};
+  double total_ms[K_NCLASS] = {};
+  hipEvent_t ev[2 * POOL] = {};
+  int cls[POOL] = {};
+  int used = 0;
+  bool created = false;
+  // Classes whose launches are kept only when committed: the CG's passes after convergence are
+  // gated no-ops, so a solve commits just its first `iterations` launches.
+  bool deferred[K_NCLASS] = {true, true, false, false};
+  std::vector<float> pending[K_NCLASS];
+
+  void create() {
+    if (created) return;
+    for (auto& e : ev) DM_CHECK(hipEventCreate(&e));
+    created = true;
+  }
+  void destroy() {
+    if (!created) return;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    created = false;
+  }
+  // all recorded events have completed (the stream was synchronised)
+  void resolve() {
+    for (int i = 0; i < used; ++i) {
+      float ms = 0.0f;
+      DM_CHECK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+      if (deferred[cls[i]]) {
+        pending[cls[i]].push_back(ms);
+      } else {
+        launches[cls[i]] += 1;
+        total_ms[cls[i]] += double(ms);
+      }
+    }
+    used = 0;
+  }
+  void commit(int c, int keep) {
+    const int k = std::min<int>(keep, int(pending[c].size()));
+    for (int i = 0; i < k; ++i) total_ms[c] += double(pending[c][i]);
+    launches[c] += k;
+    pending[c].clear();
+  }
+};
+
 struct DeviceExec {
   static constexpr bool device = true;
   hipStream_t stream = nullptr;
-  double* part = nullptr;    // reduction partials (4 per block, sized for the largest launch)
-  unsigned* ticket = nullptr;
+  double* part = nullptr;    // reduction partials (8 per block, sized for the largest launch)
   int part_blocks = 0;
+  KernelTimer* kt = nullptr;
+  int open_ = -1;            // class of the launch being timed
 
   void init(int max_n) {
     DM_CHECK(hipStreamCreate(&stream));  // default priority
     part_blocks = nblocks(max_n > 0 ? max_n : 1);
     DM_CHECK(hipMalloc(&part, sizeof(double) * 8 * size_t(part_blocks)));
-    DM_CHECK(hipMalloc(&ticket, sizeof(unsigned)));
-    DM_CHECK(hipMemsetAsync(ticket, 0, sizeof(unsigned), stream));
+    kt = new KernelTimer();
   }
   void destroy() {
+    if (kt) {
+      kt->destroy();
+      delete kt;
+    }
     if (part) (void)hipFree(part);
-    if (ticket) (void)hipFree(ticket);
     if (stream) (void)hipStreamDestroy(stream);
+    kt = nullptr;
     part = nullptr;
-    ticket = nullptr;
     stream = nullptr;
+  }
+  void timing(int c, bool enable) {
+    if (c < 0 || c >= K_NCLASS) throw std::runtime_error("unknown kernel class");
+    if (enable) {
+      kt->create();
+      sync();
+      kt->launches[c] = 0;
+      kt->total_ms[c] = 0.0;
+      kt->pending[c].clear();
+    }
+    kt->on[c] = enable;
+  }
+  void tbegin(int c) {
+    if (!kt || !kt->on[c]) return;
+    if (kt->used == KernelTimer::POOL) sync();
+    DM_CHECK(hipEventRecord(kt->ev[2 * kt->used], stream));
+    open_ = c;
+  }
+  // keep the first `keep` timed launches of class c since the last commit (stream synchronised)
+  void tcommit(int c, int keep) {
+    if (kt && kt->on[c]) kt->commit(c, keep);
+  }
+  void tend(int c) {
+    if (open_ != c) return;
+    DM_CHECK(hipEventRecord(kt->ev[2 * kt->used + 1], stream));
+    kt->cls[kt->used++] = c;
+    open_ = -1;
   }
   template <class T>
   T* alloc(size_t n) {
@@ -154,7 +248,7 @@ struct DeviceExec {
   template <class T>
   void download(T* dst, const T* src, size_t n) {
     if (n) DM_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
-    DM_CHECK(hipStreamSynchronize(stream));
+    sync();
   }
   template <class T>
   void copy(T* dst, const T* src, size_t n) {
@@ -175,9 +269,36 @@ struct DeviceExec {
     static_assert(N <= 8, "partials hold 8 values per block");
     const int g = nblocks(n > 0 ? n : 1);
     if (g > part_blocks) throw std::runtime_error("reduction larger than the partials buffer");
-    k_reduce<N, MAX, F, Fin><<<g, BLOCK, 0, stream>>>(n, f, fin, part, ticket, gate);
+    k_partial<N, MAX, F><<<g, BLOCK, 0, stream>>>(n, f, part, gate);
+    k_final<N, MAX, Fin><<<1, BLOCK, 0, stream>>>(g, fin, part, gate);
   }
-  void sync() { DM_CHECK(hipStreamSynchronize(stream)); }
+  // Geometry of the chained passes over n items: grid size and cells per block (a multiple of
+  // the wavefront, so every wave's loads start aligned).
+  static void pass_shape(int n, int* grid, int* chunk) {
+    const int n1 = n > 0 ? n : 1;
+    int g = std::min(PGRID_MAX, nblocks(n1, PBLOCK));
+    int c = (n1 + g - 1) / g;
+    c = (c + 63) / 64 * 64;
+    *chunk = c;
+    *grid = (n1 + c - 1) / c;
+  }
+  int pass_grid(int n) const {
+    int g, c;
+    pass_shape(n, &g, &c);
+    return g;
+  }
+  // One chained pass (see k_pass): in_part holds in_g partials per input quantity, out_part
+  // receives pass_grid(n) per output quantity.
+  template <int NIN, int NOUT, class F>
+  void pass(int n, const F& f, const double* in_part, int in_g, double* out_part) {
+    int g, c;
+    pass_shape(n, &g, &c);
+    k_pass<NIN, NOUT, F><<<g, PBLOCK, 0, stream>>>(n, c, f, in_part, in_g, out_part);
+  }
+  void sync() {
+    DM_CHECK(hipStreamSynchronize(stream));
+    if (kt && kt->used) kt->resolve();
+  }
 };
 
 }  // namespace dm

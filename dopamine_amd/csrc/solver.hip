@@ -58,6 +58,7 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (dopamine_device_count() <= 0) throw std::runtime_error("no HIP device available");
     if (p->wall_model_height > 0.0) throw std::runtime_error("wall_model_height > 0 (sampling walk) is not supported");
     if (p->scheme != 3 && p->scheme != 4) throw std::runtime_error("advection scheme must be CDS (3) or KEP (4)");
+    if (p->precision != 0) throw std::runtime_error("precision must be 0 (FP64 pressure solve)");
     s = new Solver();
     s->x.init(std::max(m->n_cells, m->n_faces + m->n_bnd_faces));
     s->create(*m, *p);
@@ -144,6 +145,24 @@ int dopamine_solver_nonortho_info(void* h, int32_t out[3]) {
   out[1] = s->skip_grad_p;
   out[2] = s->n_corr;
   return 0;
+}
+
+int dopamine_solver_kernel_timing(void* h, int kernel, int enable) {
+  return guard([&] {
+    static_cast<Solver*>(h)->x.timing(kernel, enable != 0);
+    return 0;
+  });
+}
+
+int dopamine_solver_kernel_time(void* h, int kernel, int64_t* launches, double* total_ms) {
+  return guard([&] {
+    auto* s = static_cast<Solver*>(h);
+    if (kernel < 0 || kernel >= dm::K_NCLASS) throw std::runtime_error("unknown kernel class");
+    s->x.sync();
+    *launches = s->x.kt->launches[kernel];
+    *total_ms = s->x.kt->total_ms[kernel];
+    return 0;
+  });
 }
 
 int dopamine_solver_sync(void* h) {

@@ -659,7 +659,9 @@ class SolverT {
   void gradients(bool with_sgs, double sgs_dt) {
     const int sgs = with_sgs ? prm.sgs : -1;
     if (prm.gradient == 0) {
+      x.tbegin(K_LS_GRAD);
       x.for_each(nc, LsGradVec{M, U, U_bnd, grad, nu_sgs, sgs, prm.c_vreman, prm.cs_smag, sgs_dt});
+      x.tend(K_LS_GRAD);
     } else {
       x.for_each(nc, GgGradVec{M, U, U_bnd, nullptr, grad});
       for (int it = 2; it <= prm.gg_iterations; ++it) {
@@ -825,12 +827,14 @@ class SolverT {
         x.zero(tau, 3 * size_t(std::max(1, nb)));
       else if (n_wm)
         x.for_each(nb, WallModel{M, U, tau, prm.nu});
+      x.tbegin(K_EXPLICIT);
       if (scheme == 4)
         x.for_each(nc, ExplicitRhs<4>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
                                       prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0});
       else
         x.for_each(nc, ExplicitRhs<3>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
                                       prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0});
+      x.tend(K_EXPLICIT);
       std::swap(U, U2);
       velocity_bc_both();
       x.copy(p_old, P, nc);

@@ -81,7 +81,13 @@ def _bind(L):
     L.dopamine_solver_counters.argtypes = [ctypes.c_void_p, _i64 * 4]
     L.dopamine_solver_nonortho_info.argtypes = [ctypes.c_void_p, _i32 * 3]
     L.dopamine_solver_sync.argtypes = [ctypes.c_void_p]
+    L.dopamine_solver_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.dopamine_solver_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _P(_i64), _P(_d)]
     return L
+
+
+# Kernel classes with device timing (include/dopamine_hip.h).
+KERNELS = {"cg_dir": 0, "cg_upd": 1, "explicit": 2, "ls_grad": 3}
 
 
 class FVSolver:
@@ -169,3 +175,14 @@ class FVSolver:
 
     def sync(self):
         self._check(self.L.dopamine_solver_sync(self.h), "sync")
+
+    def kernel_timing(self, kernel, enable=True):
+        """Start (clearing the totals) or stop HIP-event timing of a kernel class."""
+        self._check(self.L.dopamine_solver_kernel_timing(self.h, KERNELS[kernel], int(enable)), "kernel_timing")
+
+    def kernel_time(self, kernel):
+        """(launches, summed device milliseconds) since timing of the class was enabled."""
+        n, ms = _i64(0), _d(0.0)
+        self._check(self.L.dopamine_solver_kernel_time(self.h, KERNELS[kernel], ctypes.byref(n), ctypes.byref(ms)),
+                    "kernel_time")
+        return n.value, ms.value

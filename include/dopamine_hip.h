@@ -90,6 +90,21 @@ int dopamine_solver_nonortho_info(void *solver, int32_t out[3]);
 /* Wait for all work queued by the solver. */
 int dopamine_solver_sync(void *solver);
 
+/* Kernel classes with device timing. */
+enum {
+  DOPAMINE_K_CG_DIR = 0,   /* pressure CG: direction update + operator apply + p.q */
+  DOPAMINE_K_CG_UPD = 1,   /* pressure CG: x/r update, Jacobi apply, r.r and r.z */
+  DOPAMINE_K_EXPLICIT = 2, /* advection + viscous + forcing + RK3 predictor */
+  DOPAMINE_K_LS_GRAD = 3   /* least-squares velocity gradient with the fused SGS viscosity */
+};
+/* Device timing of one kernel class: a HIP event pair on the solver's stream around each launch.
+ * enable != 0 clears the totals and starts recording, 0 stops. Measurement only: no counterpart
+ * in the reference (its NVTX ranges, src/io/nvtx_shim.c, are host-side markers). */
+int dopamine_solver_kernel_timing(void *solver, int kernel, int enable);
+/* Launches timed since enabling and their summed device duration in milliseconds. CG passes that
+ * run after convergence (gated no-ops) are not counted. */
+int dopamine_solver_kernel_time(void *solver, int kernel, int64_t *launches, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,7 @@ def lib():
         L.hc_solver_time.restype = _d
         L.hc_solver_time.argtypes = [ctypes.c_void_p]
         L.hc_solver_counters.argtypes = [ctypes.c_void_p, _i64 * 4]
+        L.hc_solver_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int32 * 3]
         _lib = L
     return _lib
 
@@ -89,6 +90,11 @@ class HostSolver:
         out = (_d * 8)()
         lib().hc_solver_stats(self.h, out)
         return dict(zip(STATS_COLUMNS, list(out)))
+
+    def nonortho_info(self):
+        out = (ctypes.c_int32 * 3)()
+        lib().hc_solver_nonortho_info(self.h, out)
+        return {"skip_nonortho": out[0], "skip_grad_p": out[1], "n_nonortho_corr": out[2]}
 
     def counters(self):
         out = (_i64 * 4)()
