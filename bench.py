@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mesh", type=int, nargs=3, default=(128, 96, 64), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--precision", choices=("fp64",), default="fp64")
+    ap.add_argument("--cg-variant", type=int, choices=(0, 1), default=1,
+                    help="pressure CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction")
     ap.add_argument("--cpu-steps", type=int, default=1, help="RK3 steps of the CPU baseline (0 = skip)")
     return ap.parse_args()
 
@@ -39,12 +41,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def dir_pass_bytes(mesh):
-    """Algorithmic HBM bytes of one CG direction/operator pass, FP64 values and int32 indices:
-    per cell diag, z, p_old read and p, q written (5 x 8 B); per operator face one coefficient and
-    two indices (8 + 2 x 4 B)."""
+def cg_pass_bytes(mesh, variant):
+    """Algorithmic HBM bytes of the CG's first pass per iteration, FP64 values and int32 indices,
+    one coefficient and two indices per operator face (8 + 2 x 4 B).
+    Two-pass recurrence: diag, z, p_old read, p, q written (5 x 8 B per cell).
+    Single-reduction recurrence (the whole iteration): diag, p, s, x, r, w read and p, s, x, r, w
+    written (11 x 8 B per cell)."""
     nfp = mesh.n_int_faces + int(np.count_nonzero(mesh.periodic_pair >= 0)) // 2
-    return 40 * mesh.n_cells + 16 * nfp, nfp
+    per_cell = 88 if variant == 1 else 40
+    return per_cell * mesh.n_cells + 16 * nfp, nfp
 
 
 def cpu_baseline(mesh, case, U, p, steps):
@@ -74,6 +79,7 @@ def main():
 
     nx, ny, nz = args.mesh
     case = cases.channel(nx, ny, nz)
+    case.params.cg_variant = args.cg_variant
     t0 = time.perf_counter()
     mesh = case.make_mesh()
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
@@ -103,7 +109,7 @@ def main():
     log(f"timed {args.steps} steps in {elapsed:.3f} s, {its} CG iterations")
 
     nc = mesh.n_cells
-    bytes_dir, nfp = dir_pass_bytes(mesh)
+    bytes_dir, nfp = cg_pass_bytes(mesh, args.cg_variant)
     n_dir, ms_dir = ktime["cg_dir"]
     avg_s = ms_dir / max(n_dir, 1) / 1e3
     achieved = bytes_dir / avg_s / 1e9 if avg_s > 0 else 0.0
@@ -123,8 +129,10 @@ def main():
         "data": "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh",
         "config": {"workload": f"turbulent channel {nx}x{ny}x{nz} ({nc} hex cells, {nfp} pressure faces), "
                                "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, plain FP64",
-                   "precision": args.precision, "dt": case.params.dt},
-        "roofline": {"bound": "hbm", "kernel": "CG direction/operator pass (k_reduce<1,false,cgk::Dir,...>)",
+                   "precision": args.precision, "cg_variant": args.cg_variant, "dt": case.params.dt},
+        "roofline": {"bound": "hbm",
+                     "kernel": ("CG iteration pass k_pass<6,6,cgk::Fused>" if args.cg_variant == 1
+                                else "CG direction/operator pass k_pass<4,1,cgk::Dir>"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
                      "launches": n_dir},

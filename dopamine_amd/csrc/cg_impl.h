@@ -24,6 +24,7 @@ namespace dm {
 struct CgScalars {
   double ttol, rr, rz_hist[2], resid;
   double rtol, atol;
+  double gamma_hist[2], alpha_hist[2];  // single-reduction variant
   int its, done, maxit, nullspace;
 };
 
@@ -215,6 +216,121 @@ struct Resid {
   }
 };
 
+// ---- single-reduction variant (Chronopoulos & Gear, J. Comput. Appl. Math. 25 (1989) 153-168):
+// the same Krylov iterates as Hestenes-Stiefel CG in exact arithmetic, with s = A p carried by
+// recurrence, so one pass per iteration: gamma = (r,u), delta = (w,u) with u = D^-1 r and
+// w = A u come out of the same reduction. On a singular operator u = D^-1 r - mean(D^-1 r); since
+// A 1 = 0, w = A D^-1 r and the mean enters only through the scalars:
+// gamma = (r, D^-1 r) - mean * sum r, delta = (w, D^-1 r) - mean * sum w.
+
+// w0 = A D^-1 r0 and ttol; partials: r.r, r.u, w.u, sum u, sum r, sum w (u = D^-1 r)
+struct W0 {
+  EllOp A;
+  const double* r;
+  double* w;
+  CgScalars* sc;
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<2>& t, bool leader, Scal&) const {
+    if (leader) sc->ttol = fmax(sc->rtol * sqrt(t.v[0]), sc->atol);
+    return true;
+  }
+  DM_HD Red<6> operator()(int c, const Scal&) const {
+    const int n = A.n;
+    const double uc = r[c] / A.diag[c];
+    double wc = A.diag[c] * uc;
+    for (int k = 0; k < A.w; ++k) {
+      const int j = A.nbr[size_t(k) * n + c];
+      if (j < 0) continue;
+      wc = wc - A.coef[size_t(k) * n + c] * (r[j] / A.diag[j]);
+    }
+    w[c] = wc;
+    Red<6> o;
+    o.v[0] = r[c] * r[c];
+    o.v[1] = r[c] * uc;
+    o.v[2] = wc * uc;
+    o.v[3] = uc;
+    o.v[4] = r[c];
+    o.v[5] = wc;
+    return o;
+  }
+};
+
+// Iteration k: test ||r_k||; beta_k = gamma_k / gamma_(k-1), alpha_k = gamma_k / (delta_k -
+// beta_k gamma_k / alpha_(k-1)); p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
+// w' = A D^-1 r' with the neighbours' r' recomputed on the fly from their old r, s, w.
+// Old r, s, w are read from one buffer set and the new ones written to the other.
+struct Fused {
+  EllOp A;
+  const double *r0, *s0, *w0;
+  double *r1, *s1, *w1, *p, *x;
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double um, beta, alpha;
+  };
+  DM_HD bool setup(const Red<6>& t, bool leader, Scal& s) const {
+    if (sc->done) return false;
+    const double rr = t.v[0];
+    s.um = sc->nullspace ? t.v[3] / double(A.n) : 0.0;
+    const double gamma = t.v[1] - s.um * t.v[4];
+    const double delta = t.v[2] - s.um * t.v[5];
+    const int fin = (sqrt(rr) <= sc->ttol) ? 1 : (k >= sc->maxit ? 2 : 0);
+    if (fin) {
+      if (leader) {
+        sc->its = k;
+        sc->rr = rr;
+        sc->done = fin;
+      }
+      return false;
+    }
+    if (k == 0) {
+      s.beta = 0.0;
+      s.alpha = gamma / delta;
+    } else {
+      s.beta = gamma / sc->gamma_hist[(k + 1) & 1];
+      s.alpha = gamma / (delta - s.beta * gamma / sc->alpha_hist[(k + 1) & 1]);
+    }
+    if (leader) {
+      sc->its = k;
+      sc->rr = rr;
+      sc->gamma_hist[k & 1] = gamma;
+      sc->alpha_hist[k & 1] = s.alpha;
+    }
+    return true;
+  }
+  DM_HD Red<6> operator()(int c, const Scal& s) const {
+    const int n = A.n;
+    const bool first = k == 0;
+    const double dc = A.diag[c];
+    const double uc = r0[c] / dc - s.um;
+    const double pc = first ? uc : uc + s.beta * p[c];
+    const double sc_ = first ? w0[c] : w0[c] + s.beta * s0[c];
+    x[c] = x[c] + s.alpha * pc;
+    const double rc = r0[c] - s.alpha * sc_;
+    p[c] = pc;
+    s1[c] = sc_;
+    r1[c] = rc;
+    const double un = rc / dc;
+    double wc = dc * un;
+    for (int kk = 0; kk < A.w; ++kk) {
+      const int j = A.nbr[size_t(kk) * n + c];
+      if (j < 0) continue;
+      const double sj = first ? w0[j] : w0[j] + s.beta * s0[j];
+      const double rj = r0[j] - s.alpha * sj;
+      wc = wc - A.coef[size_t(kk) * n + c] * (rj / A.diag[j]);
+    }
+    w1[c] = wc;
+    Red<6> o;
+    o.v[0] = rc * rc;
+    o.v[1] = rc * un;
+    o.v[2] = wc * un;
+    o.v[3] = un;
+    o.v[4] = rc;
+    o.v[5] = wc;
+    return o;
+  }
+};
+
 }  // namespace cgk
 
 template <class X>
@@ -232,11 +348,20 @@ class CgT {
     q_ = x->template alloc<double>(n);
     sc_ = x->template alloc<CgScalars>(1);
     part_ = x->template alloc<double>(size_t(PARTS) * g_);
+    r2_ = x->template alloc<double>(n);
+    s_[0] = x->template alloc<double>(n);
+    s_[1] = x->template alloc<double>(n);
+    w_[0] = x->template alloc<double>(n);
+    w_[1] = x->template alloc<double>(n);
     hpart_.resize(g_);
   }
+  // Hestenes-Stiefel two passes per iteration (PETSc KSPCG's recurrence) or the single-reduction
+  // variant, one pass per iteration.
+  void set_single_reduction(bool on) { single_ = on; }
   void destroy() {
     if (!x_) return;
-    for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_})
+    for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
+                    (void*)r2_, (void*)s_[0], (void*)s_[1], (void*)w_[0], (void*)w_[1]})
       x_->release(p);
     x_ = nullptr;
   }
@@ -262,24 +387,43 @@ class CgT {
     xx.upload(sc_, &h, 1);
     xx.template pass<1, 1>(n, SumB{b, bw_, x}, nullptr, 0, P0);
     xx.template pass<1, 2>(n, Init{A, bw_, r_, x, sc_, atol_dev}, P0, g, P1);
-    xx.template pass<2, 4>(n, Z0{A.diag, r_, z_, sc_}, P1, g, U[2]);
     int k = 0;
     int batch = std::max(4, std::min(maxit + 1, predict_));
-    double* pold = pa_;
-    double* pnew = pb_;
-    for (;;) {
-      for (int i = 0; i < batch; ++i, ++k) {
-        xx.tbegin(K_CG_DIR);
-        xx.template pass<4, 1>(n, Dir{A, z_, pold, pnew, q_, sc_, k}, U[(k + 2) % 3], g, D);
-        xx.tend(K_CG_DIR);
-        xx.tbegin(K_CG_UPD);
-        xx.template pass<1, 4>(n, Upd{A.diag, pnew, q_, x, r_, z_, sc_, k}, D, g, U[k % 3]);
-        xx.tend(K_CG_UPD);
-        std::swap(pold, pnew);
+    if (single_) {
+      double* r[2] = {r_, r2_};
+      double* F[2] = {U[0], U[1]};  // 6 x g each (U[0] and U[1] are contiguous 4 x g blocks)
+      F[1] = U[0] + 6 * size_t(g);
+      xx.template pass<2, 6>(n, W0{A, r_, w_[0], sc_}, P1, g, F[0]);
+      for (;;) {
+        for (int i = 0; i < batch; ++i, ++k) {
+          const int a = k & 1, b2 = a ^ 1;
+          xx.tbegin(K_CG_DIR);
+          xx.template pass<6, 6>(n, Fused{A, r[a], s_[a], w_[a], r[b2], s_[b2], w_[b2], pa_, x, sc_, k}, F[a], g,
+                                 F[b2]);
+          xx.tend(K_CG_DIR);
+        }
+        xx.download(&h, sc_, 1);
+        if (h.done || k > maxit + 1) break;
+        batch = std::max(2, std::min(32, (h.its + 7) / 8));
       }
-      xx.download(&h, sc_, 1);
-      if (h.done || k > maxit + 1) break;
-      batch = std::max(2, std::min(16, (h.its + 7) / 8));
+    } else {
+      xx.template pass<2, 4>(n, Z0{A.diag, r_, z_, sc_}, P1, g, U[2]);
+      double* pold = pa_;
+      double* pnew = pb_;
+      for (;;) {
+        for (int i = 0; i < batch; ++i, ++k) {
+          xx.tbegin(K_CG_DIR);
+          xx.template pass<4, 1>(n, Dir{A, z_, pold, pnew, q_, sc_, k}, U[(k + 2) % 3], g, D);
+          xx.tend(K_CG_DIR);
+          xx.tbegin(K_CG_UPD);
+          xx.template pass<1, 4>(n, Upd{A.diag, pnew, q_, x, r_, z_, sc_, k}, D, g, U[k % 3]);
+          xx.tend(K_CG_UPD);
+          std::swap(pold, pnew);
+        }
+        xx.download(&h, sc_, 1);
+        if (h.done || k > maxit + 1) break;
+        batch = std::max(2, std::min(16, (h.its + 7) / 8));
+      }
     }
     xx.template pass<1, 1>(n, Resid{A, b, x}, nullptr, 0, R);
     xx.download(hpart_.data(), R, size_t(g));
@@ -301,6 +445,8 @@ class CgT {
   X* x_ = nullptr;
   int n_ = 0, g_ = 1;
   double *bw_ = nullptr, *r_ = nullptr, *z_ = nullptr, *pa_ = nullptr, *pb_ = nullptr, *q_ = nullptr;
+  double *r2_ = nullptr, *s_[2] = {nullptr, nullptr}, *w_[2] = {nullptr, nullptr};
+  bool single_ = false;
   double* part_ = nullptr;
   CgScalars* sc_ = nullptr;
   std::vector<double> hpart_;

@@ -3,6 +3,7 @@
 
 #include "dm_common.h"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -216,21 +217,30 @@ struct DeviceExec {
     }
     kt->on[c] = enable;
   }
+  // The next launch after tbegin(c) carries an event pair that the dispatch itself stamps at the
+  // kernel's start and end (hipExtLaunchKernelGGL), so the duration excludes queueing gaps.
   void tbegin(int c) {
     if (!kt || !kt->on[c]) return;
     if (kt->used == KernelTimer::POOL) sync();
-    DM_CHECK(hipEventRecord(kt->ev[2 * kt->used], stream));
     open_ = c;
+  }
+  void tend(int c) {
+    if (open_ == c) open_ = -1;  // nothing was launched
   }
   // keep the first `keep` timed launches of class c since the last commit (stream synchronised)
   void tcommit(int c, int keep) {
     if (kt && kt->on[c]) kt->commit(c, keep);
   }
-  void tend(int c) {
-    if (open_ != c) return;
-    DM_CHECK(hipEventRecord(kt->ev[2 * kt->used + 1], stream));
-    kt->cls[kt->used++] = c;
-    open_ = -1;
+  template <class K, class... A>
+  void launch(K kernel, int grid, int block, const A&... args) {
+    if (open_ >= 0) {
+      const int i = kt->used++;
+      kt->cls[i] = open_;
+      open_ = -1;
+      hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, kt->ev[2 * i], kt->ev[2 * i + 1], 0, args...);
+    } else {
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, args...);
+    }
   }
   template <class T>
   T* alloc(size_t n) {
@@ -262,7 +272,7 @@ struct DeviceExec {
   template <class F>
   void for_each(int n, const F& f) {
     if (n <= 0) return;
-    k_for<F><<<nblocks(n), BLOCK, 0, stream>>>(n, f);
+    launch(k_for<F>, nblocks(n), BLOCK, n, f);
   }
   template <int N, bool MAX, class F, class Fin>
   void reduce(int n, const F& f, const Fin& fin, const int* gate = nullptr) {
@@ -293,7 +303,7 @@ struct DeviceExec {
   void pass(int n, const F& f, const double* in_part, int in_g, double* out_part) {
     int g, c;
     pass_shape(n, &g, &c);
-    k_pass<NIN, NOUT, F><<<g, PBLOCK, 0, stream>>>(n, c, f, in_part, in_g, out_part);
+    launch(k_pass<NIN, NOUT, F>, g, PBLOCK, n, c, f, in_part, in_g, out_part);
   }
   void sync() {
     DM_CHECK(hipStreamSynchronize(stream));

@@ -165,11 +165,17 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
       x = c.sol;
     }
     int maxit = max_iterations > 2000 ? max_iterations : 2000;
-    for (const auto& o : g_options)
+    bool single = false;
+    for (const auto& o : g_options) {
       if (o.rfind("-ksp_max_it", 0) == 0) {
         const int v = std::atoi(o.c_str() + 11);
         if (v > 0) maxit = v;
       }
+      // PETSc's single-synchronisation CG types select the one-pass recurrence
+      if (o.rfind("-ksp_type", 0) == 0)
+        single = o.find("pipecg") != std::string::npos || o.find("groppcg") != std::string::npos;
+    }
+    c.cg.set_single_reduction(single);
     int its = 0;
     double res = 0.0;
     const int rc = c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, n_dirichlet_rows <= 0, &its, &res);

@@ -537,6 +537,7 @@ class SolverT {
         break;
       }
     cg.init(&x, nc);
+    cg.set_single_reduction(p.cg_variant == 1);
     apply_all_bc();
     x.sync();
   }

@@ -40,6 +40,8 @@ class SolverParams:
     cfl_max: float = 0.4
     dt: float = 1.0e-3
     adapt_dt: bool = False
+    # pressure CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction
+    cg_variant: int = 0
     # per-patch values, by patch name: p_val (outlet), U_val (inlet / moving wall)
     patch_p: dict = field(default_factory=dict)
     patch_U: dict = field(default_factory=dict)

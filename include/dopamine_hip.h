@@ -58,6 +58,8 @@ typedef struct dopamine_params {
   const double *patch_p_val;  /* n_patches */
   const double *patch_U_val;  /* 3*n_patches */
   int32_t precision;          /* 0: FP64 pressure solve; 1: FP32-SpMV CG inside FP64 iterative refinement */
+  int32_t cg_variant;         /* 0: two passes per iteration (KSPCG's Hestenes-Stiefel recurrence);
+                                 1: one pass per iteration (single-reduction recurrence) */
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
@@ -92,12 +94,14 @@ int dopamine_solver_sync(void *solver);
 
 /* Kernel classes with device timing. */
 enum {
-  DOPAMINE_K_CG_DIR = 0,   /* pressure CG: direction update + operator apply + p.q */
-  DOPAMINE_K_CG_UPD = 1,   /* pressure CG: x/r update, Jacobi apply, r.r and r.z */
+  DOPAMINE_K_CG_DIR = 0,   /* pressure CG, first pass of an iteration: p update + operator apply
+                              + p.q (cg_variant 0), or the whole iteration (cg_variant 1) */
+  DOPAMINE_K_CG_UPD = 1,   /* pressure CG, second pass (cg_variant 0): x/r update, Jacobi, dots */
   DOPAMINE_K_EXPLICIT = 2, /* advection + viscous + forcing + RK3 predictor */
   DOPAMINE_K_LS_GRAD = 3   /* least-squares velocity gradient with the fused SGS viscosity */
 };
-/* Device timing of one kernel class: a HIP event pair on the solver's stream around each launch.
+/* Device timing of one kernel class: an event pair on the solver's stream that the dispatch of
+ * each launch stamps at the kernel's start and end (hipExtLaunchKernelGGL).
  * enable != 0 clears the totals and starts recording, 0 stops. Measurement only: no counterpart
  * in the reference (its NVTX ranges, src/io/nvtx_shim.c, are host-side markers). */
 int dopamine_solver_kernel_timing(void *solver, int kernel, int enable);

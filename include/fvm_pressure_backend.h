@@ -25,8 +25,9 @@ int fvm_petsc_backend_finalize(void);
  * accepted for input compatibility and map to the Jacobi CG of this backend; "mixed" selects the
  * FP32-operator CG inside FP64 iterative refinement. */
 void fvm_petsc_set_pressure_solver(const char *solver_name);
-/* PETSc-style option string (pressure_backend_petsc.c:543-551); "-pc_type jacobi" and
- * "-ksp_max_it N" are understood, other options are recorded and ignored. */
+/* PETSc-style option string (pressure_backend_petsc.c:543-551); "-ksp_max_it N" and
+ * "-ksp_type pipecg|groppcg" (the one-pass single-reduction CG recurrence; default "cg", two
+ * passes) are understood, other options are recorded and ignored. */
 void fvm_petsc_insert_option(const char *opt_str);
 /* Node-local rank used to pick the HIP device (local_rank % device count) at initialize
  * (pressure_backend_petsc.c:563-565). */

@@ -34,22 +34,27 @@ def _pair(case, mesh, nsteps):
 
 # Field parity after N steps. Tolerance: 1e-10 relative L2 on u and p (the plain-FP64 bar); the
 # pressure solve runs to rtol 1e-12 so solver round-off stays below it.
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("name", ["tgv_skew", "hydrowall", "pdrop"])
-def test_gpu_fields_match_oracle_regression_meshes(name):
+def test_gpu_fields_match_oracle_regression_meshes(name, variant):
     case = REG[name]
     case.params.p_tol = 1e-12
+    case.params.cg_variant = variant
     mesh = case.make_mesh()
     print(f"[gpu] field parity {name}", flush=True)
     g, o = _pair(case, mesh, 5)
     du, dp = rel_l2(o.U, g.U), rel_l2(o.p, g.p)
-    print(f"[gpu] {name}: rel-L2 U {du:.2e} p {dp:.2e}", flush=True)
+    print(f"[gpu] {name} cg_variant {variant}: rel-L2 U {du:.2e} p {dp:.2e}", flush=True)
     assert du < 1e-10 and dp < 1e-10
+    assert g.counters()["iterations"] == o.counters()["iterations"]
 
 
-def test_gpu_channel_wallmodel_vreman_matches_oracle():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_gpu_channel_wallmodel_vreman_matches_oracle(variant):
     """Stretched channel, log-law wall model, Vreman SGS, body force, seeded 10% noise."""
     case = cases.small_channel(16, 12, 8)
     case.params.p_tol = 1e-12
+    case.params.cg_variant = variant
     case.extra["noise"] = 0.1
     mesh = case.make_mesh()
     print("[gpu] channel 16x12x8", flush=True)

@@ -50,13 +50,16 @@ def _call(n, own, nb, skip, diag, rhs, coeff, x0, rtol, atol, n_dir):
     return rc, x, its.value, res.value
 
 
+@pytest.mark.parametrize("ksp_type", ["cg", "pipecg"])
 @pytest.mark.parametrize("name", ["tgv_skew", "hydrowall", "hydro"])
-def test_fvm_petsc_solve_pressure_matches_oracle_and_direct_solve(name):
+def test_fvm_petsc_solve_pressure_matches_oracle_and_direct_solve(name, ksp_type):
     mesh, own, nb, skip, coeff, diag = _operator(name)
     n = mesh.n_cells
     L = lib()
     assert L.fvm_petsc_backend_available() == 1
     assert L.fvm_petsc_backend_initialize() == 0
+    L.fvm_petsc_insert_option.argtypes = [ctypes.c_char_p]
+    L.fvm_petsc_insert_option(f"-ksp_type {ksp_type}".encode())
     rng = np.random.default_rng(7)
     b = rng.standard_normal(n)
     n_dir = int(np.sum(np.abs(diag - np.bincount(own[skip == 0], coeff[skip == 0], n)

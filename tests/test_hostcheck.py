@@ -20,15 +20,18 @@ def _pair(case, mesh, nsteps):
     return h, o
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("name", ["tgv_skew", "tgv_skew_chk", "hydrowall", "hydro", "pdrop"])
-def test_host_kernels_match_oracle_fields(name):
+def test_host_kernels_match_oracle_fields(name, variant):
     case = REG[name]
     case.params.p_tol = 1e-12
+    case.params.cg_variant = variant
     mesh = case.make_mesh()
     h, o = _pair(case, mesh, 4)
     assert rel_l2(o.U, h.U) < 1e-10
     assert rel_l2(o.p, h.p) < 1e-10
     assert h.nonortho_info() == o.nonortho_info()
+    assert h.counters()["iterations"] == o.counters()["iterations"]
 
 
 def test_host_kernels_match_reference_csv():
