@@ -30,10 +30,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mesh", type=int, nargs=3, default=(128, 96, 64), metavar=("NX", "NY", "NZ"))
-    ap.add_argument("--precision", choices=("fp64",), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--cg-variant", type=int, choices=(0, 1), default=1,
                     help="pressure CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction")
     ap.add_argument("--cpu-steps", type=int, default=1, help="RK3 steps of the CPU baseline (0 = skip)")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="time the steps without the per-launch event pairs (no roofline entry)")
     return ap.parse_args()
 
 
@@ -41,13 +43,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cg_pass_bytes(mesh, variant):
-    """Algorithmic HBM bytes of the CG's first pass per iteration, FP64 values and int32 indices,
-    one coefficient and two indices per operator face (8 + 2 x 4 B).
-    Two-pass recurrence: diag, z, p_old read, p, q written (5 x 8 B per cell).
-    Single-reduction recurrence (the whole iteration): diag, p, s, x, r, w read and p, s, x, r, w
-    written (11 x 8 B per cell)."""
+def cg_pass_bytes(mesh, variant, mixed):
+    """Algorithmic HBM bytes of the CG's first pass per iteration: one coefficient and two int32
+    indices per operator face, and per cell:
+    FP64 two-pass recurrence: diag, z, p_old read, p, q written (5 x 8 B);
+    FP64 single-reduction recurrence (the whole iteration): diag, p, s, x, r, w read and p, s, x,
+    r, w written (11 x 8 B);
+    mixed precision (FP32 single-reduction sweep): diag, p, s, r, w read and p, s, r, w written
+    in FP32 (9 x 4 B), x read and written in FP64 (2 x 8 B); FP32 coefficients."""
     nfp = mesh.n_int_faces + int(np.count_nonzero(mesh.periodic_pair >= 0)) // 2
+    if mixed:
+        return 52 * mesh.n_cells + 12 * nfp, nfp
     per_cell = 88 if variant == 1 else 40
     return per_cell * mesh.n_cells + 16 * nfp, nfp
 
@@ -83,7 +89,7 @@ def main():
     t0 = time.perf_counter()
     mesh = case.make_mesh()
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
-    solver = FVSolver(mesh, case.params)
+    solver = FVSolver(mesh, case.params, precision=1 if args.precision == "mixed" else 0)
     P.initialize(solver, case, mesh)
     solver.sync()
     log("initial projections done")
@@ -94,7 +100,7 @@ def main():
 
     kernels = ("cg_dir", "cg_upd", "explicit", "ls_grad")
     for k in kernels:
-        solver.kernel_timing(k, True)
+        solver.kernel_timing(k, not args.no_kernel_timing)
     it0 = solver.counters()["iterations"]
     solver.sync()
     t0 = time.perf_counter()
@@ -109,7 +115,8 @@ def main():
     log(f"timed {args.steps} steps in {elapsed:.3f} s, {its} CG iterations")
 
     nc = mesh.n_cells
-    bytes_dir, nfp = cg_pass_bytes(mesh, args.cg_variant)
+    mixed = args.precision == "mixed"
+    bytes_dir, nfp = cg_pass_bytes(mesh, args.cg_variant, mixed)
     n_dir, ms_dir = ktime["cg_dir"]
     avg_s = ms_dir / max(n_dir, 1) / 1e3
     achieved = bytes_dir / avg_s / 1e9 if avg_s > 0 else 0.0
@@ -125,19 +132,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh",
         "config": {"workload": f"turbulent channel {nx}x{ny}x{nz} ({nc} hex cells, {nfp} pressure faces), "
-                               "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, plain FP64",
+                               "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, "
+                               + ("FP32 CG sweeps in FP64 iterative refinement" if mixed else "plain FP64"),
                    "precision": args.precision, "cg_variant": args.cg_variant, "dt": case.params.dt},
         "roofline": {"bound": "hbm",
-                     "kernel": ("CG iteration pass k_pass<6,6,cgk::Fused>" if args.cg_variant == 1
+                     "kernel": ("CG FP32 sweep iteration k_pass<6,6,cgk::Fused32>" if mixed else
+                                "CG iteration pass k_pass<6,6,cgk::Fused>" if args.cg_variant == 1
                                 else "CG direction/operator pass k_pass<4,1,cgk::Dir>"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
                      "launches": n_dir},
         "kernels_us": {k: (v[1] * 1e3 / v[0] if v[0] else None) for k, v in ktime.items()},
         "cg_iterations_per_step": its / args.steps,
+        "kernel_timing": not args.no_kernel_timing,
         "state": {"E": st["E"], "Ub": st["Ub"], "max_abs_divU": st["max_abs_divU"]},
         "cpu_baseline": None,
     }

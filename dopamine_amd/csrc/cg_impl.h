@@ -25,7 +25,9 @@ struct CgScalars {
   double ttol, rr, rz_hist[2], resid;
   double rtol, atol;
   double gamma_hist[2], alpha_hist[2];  // single-reduction variant
+  double inner_tol2;                    // mixed precision: inner stop on ||r_in||^2
   int its, done, maxit, nullspace;
+  int inner_done, inner_its;            // mixed precision: inner solve state
 };
 
 struct EllOp {
@@ -33,6 +35,14 @@ struct EllOp {
   const int* nbr = nullptr;      // w*n, -1 = empty slot
   const double* coef = nullptr;  // w*n
   const double* diag = nullptr;  // n (already repaired: non-finite or tiny -> 1)
+};
+
+// FP32 copy of the operator for the mixed-precision inner solves (same rows and slots).
+struct EllOp32 {
+  int n = 0, w = 0;
+  const int* nbr = nullptr;
+  const float* coef = nullptr;
+  const float* diag = nullptr;
 };
 
 // Chained passes of one solve. Each reads the totals of the previous pass's partials (identical
@@ -331,6 +341,157 @@ struct Fused {
   }
 };
 
+// ---- mixed precision: FP64 iterative refinement around FP32 single-reduction CG sweeps.
+// The outer loop keeps x and the true residual r = b' - A x in FP64 (FP64 operator); each sweep
+// solves A e = r in FP32 (FP32 operator and vectors, FP64 dot-product accumulation) to a relative
+// reduction tau, adding alpha p straight into the FP64 x.
+
+// r = b' - A x in FP64; partials (0, r.r) in the layout Init produces
+struct IrResid {
+  EllOp A;
+  const double *bw, *x;
+  double* r;
+  CgScalars* sc;
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<1>&, bool, Scal&) const { return !sc->done; }
+  DM_HD Red<2> operator()(int c, const Scal&) const {
+    const int n = A.n;
+    double ax = A.diag[c] * x[c];
+    for (int k = 0; k < A.w; ++k) {
+      const int j = A.nbr[size_t(k) * n + c];
+      if (j < 0) continue;
+      ax = ax - A.coef[size_t(k) * n + c] * x[j];
+    }
+    const double rc = bw[c] - ax;
+    r[c] = rc;
+    Red<2> o;
+    o.v[0] = 0.0;
+    o.v[1] = rc * rc;
+    return o;
+  }
+};
+
+// Start of a sweep: the outer test ||r|| <= ttol on the FP64 residual, then r32 = r, w = A u with
+// u = D^-1 r32 (FP32), partials as W0. The first sweep also fixes ttol from ||b'||.
+struct IrStart {
+  EllOp32 A;
+  const double* r64;
+  float *r, *w;
+  CgScalars* sc;
+  int sweep, max_sweeps;
+  double tau;
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<2>& t, bool leader, Scal&) const {
+    if (sc->done) return false;
+    const double ttol = sweep == 0 ? fmax(sc->rtol * sqrt(t.v[0]), sc->atol) : sc->ttol;
+    const double rr = t.v[1];
+    const int fin = (sqrt(rr) <= ttol) ? 1 : ((sweep >= max_sweeps || sc->its >= sc->maxit) ? 2 : 0);
+    if (leader) {
+      if (sweep == 0) sc->ttol = ttol;
+      sc->rr = rr;
+      sc->inner_done = 0;
+      sc->inner_its = 0;
+      // a sweep stops at tau ||r|| or at half the outer tolerance, whichever is larger
+      sc->inner_tol2 = fmax(tau * tau * rr, 0.25 * ttol * ttol);
+      if (fin) sc->done = fin;
+    }
+    return fin == 0;
+  }
+  DM_HD Red<6> operator()(int c, const Scal&) const {
+    const int n = A.n;
+    const float rc = float(r64[c]);
+    const float uc = rc / A.diag[c];
+    float wc = A.diag[c] * uc;
+    for (int k = 0; k < A.w; ++k) {
+      const int j = A.nbr[size_t(k) * n + c];
+      if (j < 0) continue;
+      wc = wc - A.coef[size_t(k) * n + c] * (float(r64[j]) / A.diag[j]);
+    }
+    r[c] = rc;
+    w[c] = wc;
+    Red<6> o;
+    o.v[0] = double(rc) * double(rc);
+    o.v[1] = double(rc) * double(uc);
+    o.v[2] = double(wc) * double(uc);
+    o.v[3] = double(uc);
+    o.v[4] = double(rc);
+    o.v[5] = double(wc);
+    return o;
+  }
+};
+
+// One FP32 single-reduction iteration of a sweep (as Fused), x accumulated in FP64.
+struct Fused32 {
+  EllOp32 A;
+  const float *r0, *s0, *w0;
+  float *r1, *s1, *w1, *p;
+  double* x;
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double um, beta, alpha;
+  };
+  DM_HD bool setup(const Red<6>& t, bool leader, Scal& s) const {
+    if (sc->done || sc->inner_done) return false;
+    const double rr = t.v[0];
+    s.um = sc->nullspace ? t.v[3] / double(A.n) : 0.0;
+    const double gamma = t.v[1] - s.um * t.v[4];
+    const double delta = t.v[2] - s.um * t.v[5];
+    if (k > 0 && (rr <= sc->inner_tol2 || sc->its + k >= sc->maxit)) {
+      if (leader) {
+        sc->inner_done = 1;
+        sc->inner_its = k;
+      }
+      return false;
+    }
+    if (k == 0) {
+      s.beta = 0.0;
+      s.alpha = gamma / delta;
+    } else {
+      s.beta = gamma / sc->gamma_hist[(k + 1) & 1];
+      s.alpha = gamma / (delta - s.beta * gamma / sc->alpha_hist[(k + 1) & 1]);
+    }
+    if (leader) {
+      sc->inner_its = k;
+      sc->gamma_hist[k & 1] = gamma;
+      sc->alpha_hist[k & 1] = s.alpha;
+    }
+    return true;
+  }
+  DM_HD Red<6> operator()(int c, const Scal& s) const {
+    const int n = A.n;
+    const bool first = k == 0;
+    const float beta = float(s.beta), alpha = float(s.alpha), um = float(s.um);
+    const float dc = A.diag[c];
+    const float uc = r0[c] / dc - um;
+    const float pc = first ? uc : uc + beta * p[c];
+    const float sc_ = first ? w0[c] : w0[c] + beta * s0[c];
+    x[c] = x[c] + s.alpha * double(pc);
+    const float rc = r0[c] - alpha * sc_;
+    p[c] = pc;
+    s1[c] = sc_;
+    r1[c] = rc;
+    const float un = rc / dc;
+    float wc = dc * un;
+    for (int kk = 0; kk < A.w; ++kk) {
+      const int j = A.nbr[size_t(kk) * n + c];
+      if (j < 0) continue;
+      const float sj = first ? w0[j] : w0[j] + beta * s0[j];
+      const float rj = r0[j] - alpha * sj;
+      wc = wc - A.coef[size_t(kk) * n + c] * (rj / A.diag[j]);
+    }
+    w1[c] = wc;
+    Red<6> o;
+    o.v[0] = double(rc) * double(rc);
+    o.v[1] = double(rc) * double(un);
+    o.v[2] = double(wc) * double(un);
+    o.v[3] = double(un);
+    o.v[4] = double(rc);
+    o.v[5] = double(wc);
+    return o;
+  }
+};
+
 }  // namespace cgk
 
 template <class X>
@@ -358,11 +519,25 @@ class CgT {
   // Hestenes-Stiefel two passes per iteration (PETSc KSPCG's recurrence) or the single-reduction
   // variant, one pass per iteration.
   void set_single_reduction(bool on) { single_ = on; }
+  // Mixed precision: FP64 iterative refinement around FP32 sweeps on A32 (same rows as the FP64
+  // operator) that each reduce the residual by tau.
+  void set_mixed(const EllOp32& A32, double tau) {
+    A32_ = A32;
+    tau_ = tau;
+    mixed_ = true;
+    if (!f32_[0])
+      for (auto*& q : f32_) q = x_->template alloc<float>(n_);
+  }
+  int last_sweeps() const { return last_sweeps_; }
   void destroy() {
     if (!x_) return;
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)r2_, (void*)s_[0], (void*)s_[1], (void*)w_[0], (void*)w_[1]})
       x_->release(p);
+    for (auto*& q : f32_) {
+      if (q) x_->release(q);
+      q = nullptr;
+    }
     x_ = nullptr;
   }
   // Solve A x = b (pointers in the execution space). atol_dev, when not null, points to an
@@ -389,7 +564,44 @@ class CgT {
     xx.template pass<1, 2>(n, Init{A, bw_, r_, x, sc_, atol_dev}, P0, g, P1);
     int k = 0;
     int batch = std::max(4, std::min(maxit + 1, predict_));
-    if (single_) {
+    last_sweeps_ = 0;
+    if (mixed_) {
+      // f32_: r (2), s (2), w (2), p
+      float* r[2] = {f32_[0], f32_[1]};
+      float* s[2] = {f32_[2], f32_[3]};
+      float* w[2] = {f32_[4], f32_[5]};
+      double* F[2] = {U[0], U[0] + 6 * size_t(g)};
+      const double* in = P1;
+      int total = 0, ibatch = std::max(4, predict_in_);
+      for (int sweep = 0;; ++sweep) {
+        xx.template pass<2, 6>(n, IrStart{A32_, r_, r[0], w[0], sc_, sweep, MAX_SWEEPS, tau_}, in, g, F[0]);
+        int kin = 0;
+        for (;;) {
+          for (int i = 0; i < ibatch; ++i, ++kin) {
+            const int a = kin & 1, b2 = a ^ 1;
+            xx.tbegin(K_CG_DIR);
+            xx.template pass<6, 6>(
+                n, Fused32{A32_, r[a], s[a], w[a], r[b2], s[b2], w[b2], f32_[6], x, sc_, kin}, F[a], g, F[b2]);
+            xx.tend(K_CG_DIR);
+          }
+          xx.download(&h, sc_, 1);
+          if (h.done || h.inner_done || kin > maxit + 1) break;
+          ibatch = std::max(2, std::min(32, (h.inner_its + 7) / 8));
+        }
+        xx.tcommit(K_CG_DIR, h.inner_its);
+        if (h.done) break;
+        total += h.inner_its;
+        predict_in_ = h.inner_its + 1;
+        ibatch = predict_in_;
+        last_sweeps_ = sweep + 1;
+        h.its = total;
+        xx.upload(&sc_->its, &total, 1);
+        xx.template pass<1, 2>(n, IrResid{A, bw_, x, r_, sc_}, nullptr, 0, P1);
+        in = P1;
+        if (sweep + 1 > MAX_SWEEPS) break;
+      }
+      h.its = total;
+    } else if (single_) {
       double* r[2] = {r_, r2_};
       double* F[2] = {U[0], U[1]};  // 6 x g each (U[0] and U[1] are contiguous 4 x g blocks)
       F[1] = U[0] + 6 * size_t(g);
@@ -430,14 +642,17 @@ class CgT {
     xx.download(&h, sc_, 1);
     double rr = 0.0;
     for (int q = 0; q < g; ++q) rr = rr + hpart_[q];
-    xx.tcommit(K_CG_DIR, h.its);
-    xx.tcommit(K_CG_UPD, h.its);
+    if (!mixed_) {
+      xx.tcommit(K_CG_DIR, h.its);
+      xx.tcommit(K_CG_UPD, h.its);
+    }
     last_its_ = h.its;
     predict_ = std::max(2, h.its + 1);
     if (its) *its = h.its;
     if (resid) *resid = std::sqrt(rr);
     return h.done == 1 ? 0 : -1;
   }
+  static constexpr int MAX_SWEEPS = 40;
   int last_its() const { return last_its_; }
 
  private:
@@ -446,7 +661,11 @@ class CgT {
   int n_ = 0, g_ = 1;
   double *bw_ = nullptr, *r_ = nullptr, *z_ = nullptr, *pa_ = nullptr, *pb_ = nullptr, *q_ = nullptr;
   double *r2_ = nullptr, *s_[2] = {nullptr, nullptr}, *w_[2] = {nullptr, nullptr};
-  bool single_ = false;
+  bool single_ = false, mixed_ = false;
+  EllOp32 A32_;
+  double tau_ = 1e-3;
+  float* f32_[7] = {};
+  int predict_in_ = 24, last_sweeps_ = 0;
   double* part_ = nullptr;
   CgScalars* sc_ = nullptr;
   std::vector<double> hpart_;

@@ -26,11 +26,15 @@ struct Ctx {
   dm::DeviceExec x;
   dm::CgT<dm::DeviceExec> cg;
   dm::EllOp A;
+  dm::EllOp32 A32;
   int* nbr = nullptr;
   double *coef = nullptr, *diag = nullptr, *b = nullptr, *sol = nullptr;
+  float *coef32 = nullptr, *diag32 = nullptr;
+  bool mixed = false;
   ~Ctx() {
     cg.destroy();
-    for (void* p : {(void*)nbr, (void*)coef, (void*)diag, (void*)b, (void*)sol}) x.release(p);
+    for (void* p : {(void*)nbr, (void*)coef, (void*)diag, (void*)b, (void*)sol, (void*)coef32, (void*)diag32})
+      x.release(p);
     x.destroy();
   }
 };
@@ -86,6 +90,20 @@ int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner
   ctx->A.coef = ctx->coef;
   ctx->A.diag = ctx->diag;
   ctx->cg.init(&ctx->x, n);
+  if (g_solver == "mixed") {  // FP32 operator copy for the mixed-precision sweeps
+    std::vector<float> c32(cf.begin(), cf.end()), d32(dg.begin(), dg.end());
+    ctx->coef32 = ctx->x.alloc<float>(c32.size());
+    ctx->diag32 = ctx->x.alloc<float>(n);
+    ctx->x.upload(ctx->coef32, c32.data(), c32.size());
+    ctx->x.upload(ctx->diag32, d32.data(), d32.size());
+    ctx->A32.n = n;
+    ctx->A32.w = w;
+    ctx->A32.nbr = ctx->nbr;
+    ctx->A32.coef = ctx->coef32;
+    ctx->A32.diag = ctx->diag32;
+    ctx->cg.set_mixed(ctx->A32, 1.0e-4);
+    ctx->mixed = true;
+  }
   ctx->n_local = n;
   ctx->n_global = n_global;
   ctx->n_faces = n_faces;
@@ -148,7 +166,7 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
     int ref_row = reference_cell - 1;
     if (ref_row < 0 || ref_row >= n_cells_global) ref_row = 0;
     if (!g_ctx || g_ctx->n_local != n_cells_local || g_ctx->n_global != n_cells_global || g_ctx->n_faces != n_faces ||
-        g_ctx->ref_row != ref_row) {
+        g_ctx->ref_row != ref_row || g_ctx->mixed != (g_solver == "mixed")) {
       g_ctx.reset();
       const int rc = build_ctx(n_cells_local, n_cells_global, n_faces, local_to_global, face_owner,
                                face_neighbor_local, face_neighbor_global, skip_face, diagonal, face_coeff, ref_row);

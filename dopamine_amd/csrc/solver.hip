@@ -58,7 +58,7 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (dopamine_device_count() <= 0) throw std::runtime_error("no HIP device available");
     if (p->wall_model_height > 0.0) throw std::runtime_error("wall_model_height > 0 (sampling walk) is not supported");
     if (p->scheme != 3 && p->scheme != 4) throw std::runtime_error("advection scheme must be CDS (3) or KEP (4)");
-    if (p->precision != 0) throw std::runtime_error("precision must be 0 (FP64 pressure solve)");
+    if (p->precision != 0 && p->precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
     if (p->cg_variant != 0 && p->cg_variant != 1) throw std::runtime_error("cg_variant must be 0 or 1");
     s = new Solver();
     s->x.init(std::max(m->n_cells, m->n_faces + m->n_bnd_faces));

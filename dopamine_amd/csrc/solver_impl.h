@@ -13,6 +13,8 @@
 // CG convergence).
 #pragma once
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <vector>
 
@@ -131,6 +133,13 @@ class SolverT {
   double *phi_it = nullptr;
   SolverScalars* sc = nullptr;
   EllOp A;
+  EllOp32 A32;
+  // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
+  static double mixed_tau() {
+    const char* e = std::getenv("DOPAMINE_MIXED_TAU");
+    const double t = e ? std::atof(e) : 1.0e-4;
+    return (t > 0.0 && t < 1.0) ? t : 1.0e-4;
+  }
   CgT<X> cg;
   std::vector<void*> owned;
   std::vector<int> dir_rows_h;
@@ -501,6 +510,19 @@ class SolverT {
     A.nbr = up(anbr);
     A.coef = up(acoef);
     A.diag = up(diag);
+    if (p.precision == 1) {  // FP32 copy of the operator for the mixed-precision sweeps
+      std::vector<float> c32(acoef.begin(), acoef.end()), d32(diag.size());
+      for (size_t i = 0; i < diag.size(); ++i) {
+        double d = diag[i];
+        if (!std::isfinite(d) || std::fabs(d) < 1.0e-20) d = 1.0;
+        d32[i] = float(d);
+      }
+      A32.n = nc;
+      A32.w = wP;
+      A32.nbr = A.nbr;
+      A32.coef = up(c32);
+      A32.diag = up(d32);
+    }
     if (n_dir) {
       dir_rows = up(dir_rows_h);
       dir_vals = up(dir_vals_h);
@@ -538,6 +560,7 @@ class SolverT {
       }
     cg.init(&x, nc);
     cg.set_single_reduction(p.cg_variant == 1);
+    if (p.precision == 1) cg.set_mixed(A32, mixed_tau());
     apply_all_bc();
     x.sync();
   }
@@ -731,6 +754,7 @@ class SolverT {
     n_solves++;
     n_iters += its;
     last_iters = its;
+    n_sweeps += cg.last_sweeps();
     return rc;
   }
 

@@ -22,8 +22,9 @@ int fvm_petsc_backend_initialize(void);
 /* Releases the cached solve context; 0 on success (pressure_backend_petsc.c:618-646). */
 int fvm_petsc_backend_finalize(void);
 /* Preconditioner selector (pressure_backend_petsc.c:534-539). "boomeramg", "gamg" and "gmres" are
- * accepted for input compatibility and map to the Jacobi CG of this backend; "mixed" selects the
- * FP32-operator CG inside FP64 iterative refinement. */
+ * accepted for input compatibility and map to the Jacobi CG of this backend; "mixed" selects
+ * FP32-operator Jacobi-CG sweeps inside FP64 iterative refinement (same stopping test on the FP64
+ * residual). A change of selector rebuilds the solve context. */
 void fvm_petsc_set_pressure_solver(const char *solver_name);
 /* PETSc-style option string (pressure_backend_petsc.c:543-551); "-ksp_max_it N" and
  * "-ksp_type pipecg|groppcg" (the one-pass single-reduction CG recurrence; default "cg", two

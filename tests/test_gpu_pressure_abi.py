@@ -88,3 +88,31 @@ def test_fvm_petsc_solve_pressure_matches_oracle_and_direct_solve(name, ksp_type
         xg = x
     assert np.linalg.norm(xg - xd) / np.linalg.norm(xd) < 1e-8
     assert L.fvm_petsc_backend_finalize() == 0
+
+
+@pytest.mark.parametrize("name", ["tgv_skew", "hydrowall", "hydro"])
+def test_mixed_precision_solve_meets_the_fp64_residual(name):
+    """FP32 sweeps in FP64 refinement: the returned true FP64 residual meets rtol ||b||."""
+    mesh, own, nb, skip, coeff, diag = _operator(name)
+    n = mesh.n_cells
+    L = lib()
+    assert L.fvm_petsc_backend_initialize() == 0
+    L.fvm_petsc_set_pressure_solver.argtypes = [ctypes.c_char_p]
+    L.fvm_petsc_set_pressure_solver(b"mixed")
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(n)
+    n_dir = 0 if name != "hydro" else 1
+    if n_dir == 0:
+        b -= b.mean()
+    for rtol in (1e-8, 1e-10):
+        rc, x, its, res = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), rtol, 1e-300, n_dir)
+        print(f"[gpu] mixed {name} rtol {rtol}: rc {rc} its {its} resid {res:.3e}", flush=True)
+        assert rc == 0
+        assert res <= rtol * np.linalg.norm(b) * 1.001
+        xo, _, _, _ = oracle.cg_jacobi(own[skip == 0], nb[skip == 0], coeff[skip == 0], diag, b, np.zeros(n),
+                                       rtol * 1e-3, 1e-300, 5000, n_dir == 0)
+        if n_dir == 0:
+            x, xo = x - x.mean(), xo - xo.mean()
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e3 * rtol
+    L.fvm_petsc_set_pressure_solver(b"boomeramg")
+    assert L.fvm_petsc_backend_finalize() == 0

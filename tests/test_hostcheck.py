@@ -72,3 +72,16 @@ def test_host_rotational_off_and_cds_match_oracle():
     mesh = case.make_mesh()
     h, o = _pair(case, mesh, 3)
     assert rel_l2(o.U, h.U) < 1e-10 and rel_l2(o.p, h.p) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["tgv_skew", "hydrowall"])
+def test_host_mixed_precision_matches_oracle_to_solver_tolerance(name):
+    case = REG[name]
+    case.params.p_tol = 1e-8
+    mesh = case.make_mesh()
+    h, o = HostSolver(mesh, case.params, precision=1), oracle.Solver(mesh, case.params)
+    for s in (h, o):
+        run_case(s, case, mesh, nsteps=3, stats_every=3)
+    assert rel_l2(o.U, h.U) < 1e-7 and rel_l2(o.p, h.p) < 1e-6
+    assert h.counters()["sweeps"] > 0
+    assert h.stats()["max_abs_divU"] < max(10 * o.stats()["max_abs_divU"], 1e-10)
