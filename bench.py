@@ -151,6 +151,17 @@ def main():
         "state": {"E": st["E"], "Ub": st["Ub"], "max_abs_divU": st["max_abs_divU"]},
         "cpu_baseline": None,
     }
+    # plain copy kernel of known size: achievable HBM bandwidth, and the calibration of the traffic
+    # counters (FETCH_SIZE / WRITE_SIZE) for the same process
+    import ctypes
+    from dopamine_amd._lib import lib as _native
+    L = _native()
+    L.dopamine_hbm_copy_probe.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    ncopy = 8 * nc
+    cms = ctypes.c_double(0.0)
+    if L.dopamine_hbm_copy_probe(ncopy, 20, ctypes.byref(cms)) == 0 and cms.value > 0:
+        out["hbm_copy"] = {"bytes_per_launch": 16 * ncopy, "avg_launch_us": cms.value * 1e3,
+                           "GBs": 16 * ncopy / (cms.value * 1e-3) / 1e9}
     if args.cpu_steps > 0:
         U, p = solver.U, solver.p
         log("CPU baseline (oracle restatement, one thread) ...")

@@ -46,6 +46,15 @@ def regression_cases():
                      p_max_iter=1000, n_nonortho_corr=5, nonortho_check=True, rhie_chow_dual=True,
                      cell_projection="LS", pressure_min_cos_theta=0.0),
         nsteps=30)
+    # tests/regression/cases/tgv/case.input: QUICK on the 32^3 periodic box of
+    # examples/taylor_green/taylor_green.geo (uniform transfinite hexes on [0, 2pi]^3)
+    L = 6.283185307179586
+    cases["tgv"] = Case(
+        "tgv", lambda: M.uniform_box(32, 32, 32, L, L, L, group_names=(
+            "periodic_x0", "periodic_xL", "periodic_y0", "periodic_yL", "periodic_z0", "periodic_zL")),
+        SolverParams(nu=6.25e-4, advection_scheme="QUICK", use_sgs=False, dt=1.0e-2, cfl_max=0.6, p_tol=1.0e-10,
+                     p_max_iter=1000, rhie_chow_dual=False, cell_projection="LS", pressure_min_cos_theta=0.0),
+        nsteps=20)
     walls = ("noSlip",) * 6
     # tests/regression/cases/hydrowall/case.input
     cases["hydrowall"] = Case(

@@ -23,6 +23,7 @@ struct DevMesh {
   const double* lw;                               // LS weights: plane (3k+i) of nc values
   // faces
   const double *sf, *gc, *area, *nrm, *eps, *wbl, *fctr, *dcf;
+  const int *fown, *fnb;                          // face owner / neighbour (-1 on boundary faces)
   // boundary faces
   const int *bbc, *bpatch, *bown, *pnb, *pair;
   const double *pgc, *pdcf, *cb, *kbnd, *wsy;
@@ -215,14 +216,42 @@ DM_INLINE double smagorinsky_nu(const double* g, const double* G, double cs, dou
   return fmin(nu, cap);
 }
 
+// amd_point_viscosity (src/sgs/mod_amd.f90:31-66) with the stability cap: al[3j+i] = alpha(i,j),
+// G[3j+i] = G(i,j).
+DM_INLINE double amd_nu(const double* al, const double* G, double c_amd, double vol, double dt_stab) {
+  double asq = 0.0;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) asq = asq + al[3 * j + i] * al[3 * j + i];
+  double num = 0.0;
+  for (int n = 0; n < 3; ++n)
+    for (int m = 0; m < 3; ++m) {
+      double beta = 0.0;
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) beta = beta + G[3 * j + i] * al[3 * m + i] * al[3 * n + j];
+      num = num - beta * al[3 * n + m];
+    }
+  const double trg = G[0] + G[4] + G[8];
+  const double nu = (asq > 1.0e-20 && num > 1.0e-12 * trg * asq * sqrt(asq)) ? c_amd * num / asq : 0.0;
+  const double cap = 0.4 * pow(vol, 2.0 / 3.0) / fmax(dt_stab, SMALL);
+  return fmin(nu, cap);
+}
+
+DM_INLINE double sgs_nu(int sgs, const double* g, const double* G, double c_v, double cs, double c_amd, double vol,
+                        double dt_stab) {
+  if (sgs == 1) return vreman_nu(g, G, c_v, vol, dt_stab);
+  if (sgs == 2) return smagorinsky_nu(g, G, cs, dt_stab);
+  if (sgs == 3) return amd_nu(g, G, c_amd, vol, dt_stab);
+  return 0.0;
+}
+
 // ls_gradient_vector (src/numerics/mod_gradient.f90:425-556), optionally followed by the SGS
-// viscosity of the same cell (sgs: 0 none, 1 Vreman, 2 Smagorinsky, -1 leave nu_sgs alone).
+// viscosity of the same cell (sgs: 0 none, 1 Vreman, 2 Smagorinsky, 3 AMD, -1 leave nu_sgs alone).
 struct LsGradVec {
   DevMesh M;
   const double *U, *Ub;
   double *grad, *nu_sgs;
   int sgs;
-  double c_v, cs, dt_stab;
+  double c_v, cs, c_amd, dt_stab;
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double u0 = U[3 * size_t(c)], u1 = U[3 * size_t(c) + 1], u2 = U[3 * size_t(c) + 2];
@@ -242,9 +271,7 @@ struct LsGradVec {
       }
     }
     for (int q = 0; q < 9; ++q) grad[9 * size_t(c) + q] = g[q];
-    if (sgs == 1) nu_sgs[c] = vreman_nu(g, &M.Gf[9 * size_t(c)], c_v, M.vol[c], dt_stab);
-    else if (sgs == 2) nu_sgs[c] = smagorinsky_nu(g, &M.Gf[9 * size_t(c)], cs, dt_stab);
-    else if (sgs == 0) nu_sgs[c] = 0.0;
+    if (sgs >= 0) nu_sgs[c] = sgs_nu(sgs, g, &M.Gf[9 * size_t(c)], c_v, cs, c_amd, M.vol[c], dt_stab);
   }
 };
 
@@ -371,19 +398,16 @@ struct SgsCell {
   const double* grad;
   double* nu_sgs;
   int sgs;
-  double c_v, cs, dt_stab;
+  double c_v, cs, c_amd, dt_stab;
   DM_HD void operator()(int c) const {
-    const double* g = &grad[9 * size_t(c)];
-    if (sgs == 1) nu_sgs[c] = vreman_nu(g, &M.Gf[9 * size_t(c)], c_v, M.vol[c], dt_stab);
-    else if (sgs == 2) nu_sgs[c] = smagorinsky_nu(g, &M.Gf[9 * size_t(c)], cs, dt_stab);
-    else nu_sgs[c] = 0.0;
+    nu_sgs[c] = sgs_nu(sgs, &grad[9 * size_t(c)], &M.Gf[9 * size_t(c)], c_v, cs, c_amd, M.vol[c], dt_stab);
   }
 };
 
 // ------------------------------------------------------------------ wall model
 
 // Reichardt law and Newton solve for u_tau (src/sgs/mod_wall_model.f90:50-115).
-DM_INLINE double wall_tau_mag(double u_par, double y_w, double nu) {
+DM_INLINE double wall_utau(double u_par, double y_w, double nu) {
   const double KAPPA = 0.41, BLOG = 5.2, A1I = 1.0 / 11.0, A2 = 0.33;
   const double cbuf = BLOG - log(KAPPA) / KAPPA;
   if (u_par <= SMALL || y_w <= SMALL) return 0.0;
@@ -404,8 +428,51 @@ DM_INLINE double wall_tau_mag(double u_par, double y_w, double nu) {
     }
     ut = un;
   }
+  return ut;
+}
+
+DM_INLINE double wall_tau_mag(double u_par, double y_w, double nu) {
+  const double ut = wall_utau(u_par, y_w, nu);
   return ut * ut;
 }
+
+// AMD in wall-model cells (src/sgs/mod_amd.f90:142-199): the cell's alpha with the wall-parallel
+// part of its wall-normal derivative replaced by u_tau / (kappa y), for each wall-model face of the
+// cell in boundary order; the last valid face's value stands. One wall cell per item.
+struct AmdWall {
+  DevMesh M;
+  const int *cells, *off, *bfaces;
+  const double *U, *grad;
+  double* nu_sgs;
+  double nu, c_amd, dt_stab;
+  DM_HD void operator()(int i) const {
+    const int o = cells[i];
+    for (int q = off[i]; q < off[i + 1]; ++q) {
+      const int b = bfaces[q], f = M.nint + b;
+      double nrm[3], dx[3];
+      for (int k = 0; k < 3; ++k) {
+        nrm[k] = -M.nrm[3 * size_t(f) + k];
+        dx[k] = M.fctr[3 * size_t(f) + k] - M.ctr[3 * size_t(o) + k];
+      }
+      const double y_w = fabs(dot3(dx, nrm));
+      const double* u = &U[3 * size_t(o)];
+      const double un = dot3(u, nrm);
+      double ut[3];
+      for (int k = 0; k < 3; ++k) ut[k] = u[k] - un * nrm[k];
+      const double ut_mag = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+      if (ut_mag <= 1.0e-14 || y_w <= SMALL) continue;
+      const double g_wall = wall_utau(ut_mag, y_w, nu) / (0.41 * y_w);
+      double al[9], d_old[3], d_t[3];
+      for (int k = 0; k < 9; ++k) al[k] = grad[9 * size_t(o) + k];
+      for (int j = 0; j < 3; ++j) d_old[j] = nrm[0] * al[3 * j + 0] + nrm[1] * al[3 * j + 1] + nrm[2] * al[3 * j + 2];
+      const double dn = dot3(d_old, nrm);
+      for (int j = 0; j < 3; ++j) d_t[j] = g_wall * ut[j] / ut_mag - (d_old[j] - dn * nrm[j]);
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) al[3 * j + k] = al[3 * j + k] + nrm[k] * d_t[j];
+      nu_sgs[o] = amd_nu(al, &M.Gf[9 * size_t(o)], c_amd, M.vol[o], dt_stab);
+    }
+  }
+};
 
 // apply_wall_model_bc without time filter (src/sgs/mod_wall_model.f90:176-313), one boundary face.
 struct WallModel {
@@ -427,6 +494,261 @@ struct WallModel {
   }
 };
 
+// ------------------------------------------------------------------ reconstruction schemes
+
+// QUICK / van Leer TVD / Koren TVD face values of the vector SoA path
+// (advection_rhs_vector_recon_soa, src/numerics/mod_advection.f90:1396-1635, helpers :2334-2791):
+// upwind/downwind context along the face, far-upwind search over the upwind cell's faces, the
+// limited reconstruction and its non-finite fallbacks. One face per item; the three components of
+// the face value go to phiF, which the explicit-RHS kernel gathers.
+enum { SCHEME_VANLEER = 1, SCHEME_KOREN = 2, SCHEME_CDS = 3, SCHEME_KEP = 4, SCHEME_QUICK = 5 };
+
+struct ReconCtx {
+  int up, down;
+  double x_up[3], x_down[3], up_shift[3], dir_hat[3], s_face, s_down;
+};
+
+template <int SCHEME>
+struct ReconFace {
+  DevMesh M;
+  const double *U, *Ub, *UF, *grad;
+  double* phiF;
+
+  DM_HD static double clampv(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+  DM_HD static double nrm3(const double* v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+  // soa_resolve_periodic_partner_dev (:2362-2414)
+  DM_HD int partner(int f, double* shift) const {
+    shift[0] = shift[1] = shift[2] = 0.0;
+    const int b = f - M.nint;
+    if (M.bbc[b] != BC_PERIODIC) return -1;
+    const int owner = M.bown[b];
+    int pc = M.pnb[b], pf = -1;
+    if (M.pair[b] >= 0) {
+      pf = M.nint + M.pair[b];
+      if (pc < 0) pc = M.bown[M.pair[b]];
+    }
+    if (pc < 0) return -1;
+    double d_per;
+    if (pf >= 0) d_per = M.dcf[f] + M.dcf[pf];
+    else if (M.pdcf[b] > SMALL) d_per = M.dcf[f] + M.pdcf[b];
+    else d_per = 2.0 * M.dcf[f];
+    for (int i = 0; i < 3; ++i)
+      shift[i] = (M.ctr[3 * size_t(owner) + i] + d_per * M.nrm[3 * size_t(f) + i]) - M.ctr[3 * size_t(pc) + i];
+    return pc;
+  }
+  // soa_resolve_reconstruction_context_dev (:2417-2498)
+  DM_HD bool context(int f, double Fn, ReconCtx& c) const {
+    c.up = c.down = 0;
+    c.s_face = c.s_down = 0.0;
+    for (int i = 0; i < 3; ++i) c.x_up[i] = c.x_down[i] = c.up_shift[i] = c.dir_hat[i] = 0.0;
+    const double* xf = &M.fctr[3 * size_t(f)];
+    const int owner = M.fown[f], nb = M.fnb[f];
+    if (nb >= 0) {
+      c.up = Fn >= 0.0 ? owner : nb;
+      c.down = Fn >= 0.0 ? nb : owner;
+      for (int i = 0; i < 3; ++i) {
+        c.x_up[i] = M.ctr[3 * size_t(c.up) + i];
+        c.x_down[i] = M.ctr[3 * size_t(c.down) + i];
+      }
+    } else {
+      double sh[3];
+      const int pc = partner(f, sh);
+      if (pc < 0) return false;
+      if (Fn >= 0.0) {
+        c.up = owner;
+        c.down = pc;
+        for (int i = 0; i < 3; ++i) {
+          c.x_up[i] = M.ctr[3 * size_t(owner) + i];
+          c.x_down[i] = M.ctr[3 * size_t(pc) + i] + sh[i];
+        }
+      } else {
+        c.up = pc;
+        c.down = owner;
+        for (int i = 0; i < 3; ++i) {
+          c.up_shift[i] = sh[i];
+          c.x_up[i] = M.ctr[3 * size_t(pc) + i] + c.up_shift[i];
+          c.x_down[i] = M.ctr[3 * size_t(owner) + i];
+        }
+      }
+    }
+    double dv[3], tf[3], td[3];
+    for (int i = 0; i < 3; ++i) dv[i] = xf[i] - c.x_up[i];
+    double dn = nrm3(dv);
+    if (dn <= SMALL) {
+      for (int i = 0; i < 3; ++i) dv[i] = c.x_down[i] - c.x_up[i];
+      dn = nrm3(dv);
+    }
+    if (dn <= SMALL) return false;
+    for (int i = 0; i < 3; ++i) {
+      c.dir_hat[i] = dv[i] / dn;
+      tf[i] = xf[i] - c.x_up[i];
+      td[i] = c.x_down[i] - c.x_up[i];
+    }
+    c.s_face = dot3(tf, c.dir_hat);
+    c.s_down = dot3(td, c.dir_hat);
+    if (c.s_face <= SMALL || c.s_down <= c.s_face) {
+      for (int i = 0; i < 3; ++i) dv[i] = c.x_down[i] - c.x_up[i];
+      dn = nrm3(dv);
+      if (dn <= SMALL) return false;
+      for (int i = 0; i < 3; ++i) c.dir_hat[i] = dv[i] / dn;
+      c.s_face = dot3(tf, c.dir_hat);
+      c.s_down = dot3(td, c.dir_hat);
+    }
+    if (c.s_down <= SMALL) return false;
+    c.s_face = fmin(fmax(c.s_face, 0.0), c.s_down);
+    return true;
+  }
+  // soa_find_far_upwind_dev (:2501-2549), the upwind cell's faces in its own face order
+  DM_HD int far_upwind(const ReconCtx& c) const {
+    const int nc = M.nc;
+    int far_up = -1;
+    double best_align = -1.0, best_perp2 = 1.0e308;
+    const int cnt = M.nA[c.up];
+    for (int k = 0; k < cnt; ++k) {
+      const int oth = M.aoth[size_t(k) * nc + c.up];
+      int nb = oth;
+      if (oth < 0) {
+        const int b = -1 - oth;
+        nb = (M.bbc[b] == BC_PERIODIC) ? M.pnb[b] : -1;
+      }
+      if (nb == c.down || nb < 0) continue;
+      double d[3];
+      for (int i = 0; i < 3; ++i) d[i] = M.ctr[3 * size_t(nb) + i] + c.up_shift[i] - c.x_up[i];
+      const double sd = dot3(d, c.dir_hat);
+      if (sd >= -1.0e-12) continue;
+      const double dist2 = fmax(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], SMALL);
+      const double align = -sd / sqrt(dist2);
+      const double perp2 = fmax(0.0, dist2 - sd * sd);
+      if (align > best_align + 1.0e-12 || (fabs(align - best_align) <= 1.0e-12 && perp2 < best_perp2 - 1.0e-12) ||
+          (fabs(align - best_align) <= 1.0e-12 && fabs(perp2 - best_perp2) <= 1.0e-12 && nb < far_up)) {
+        best_align = align;
+        best_perp2 = perp2;
+        far_up = nb;
+      }
+    }
+    return far_up;
+  }
+  DM_HD static double axial(double pf, double pu, const double* g, const double* dv, double sv, const double* dir) {
+    double t[3];
+    for (int i = 0; i < 3; ++i) t[i] = dv[i] - sv * dir[i];
+    return pf - pu - dot3(g, t);
+  }
+  // soa_quick_reconstruction3_dev (:2632-2713)
+  DM_HD void quick3(const ReconCtx& c, double* out) const {
+    const int nc = M.nc;
+    double pU[3], pD[3], b1[3], b2[3], ln[3], gup[3][3], dl[3];
+    for (int k = 0; k < 3; ++k) {
+      pU[k] = U[3 * size_t(c.up) + k];
+      pD[k] = U[3 * size_t(c.down) + k];
+      for (int i = 0; i < 3; ++i) gup[k][i] = grad[DM_G(c.up, i, k)];
+    }
+    double dv[3];
+    for (int i = 0; i < 3; ++i) dv[i] = c.x_down[i] - c.x_up[i];
+    double dist2 = fmax(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2], SMALL);
+    double w = 1.0 / dist2;
+    double a11 = w * c.s_down * c.s_down;
+    double a12 = a11 * c.s_down;
+    double a22 = a12 * c.s_down;
+    double lden = a11;
+    int nfit = 1;
+    for (int k = 0; k < 3; ++k) {
+      dl[k] = axial(pD[k], pU[k], gup[k], dv, c.s_down, c.dir_hat);
+      b1[k] = w * c.s_down * dl[k];
+      b2[k] = w * c.s_down * c.s_down * dl[k];
+      ln[k] = b1[k];
+    }
+    const int cnt = M.nA[c.up];
+    for (int q = 0; q < cnt; ++q) {
+      const int nb = M.aoth[size_t(q) * nc + c.up];
+      if (nb < 0 || nb == c.down) continue;
+      for (int i = 0; i < 3; ++i) dv[i] = (M.ctr[3 * size_t(nb) + i] + c.up_shift[i]) - c.x_up[i];
+      const double sv = dot3(dv, c.dir_hat);
+      if (sv >= -1.0e-12) continue;
+      dist2 = fmax(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2], SMALL);
+      if (sv * sv < 0.25 * dist2) continue;
+      const double align = fmin(1.0, -sv / sqrt(dist2));
+      w = (0.25 + align * align) / dist2;
+      a11 = a11 + w * sv * sv;
+      a12 = a12 + w * sv * sv * sv;
+      a22 = a22 + w * sv * sv * sv * sv;
+      lden = lden + w * sv * sv;
+      for (int k = 0; k < 3; ++k) {
+        dl[k] = axial(U[3 * size_t(nb) + k], pU[k], gup[k], dv, sv, c.dir_hat);
+        b1[k] = b1[k] + w * sv * dl[k];
+        b2[k] = b2[k] + w * sv * sv * dl[k];
+        ln[k] = ln[k] + w * sv * dl[k];
+      }
+      nfit = nfit + 1;
+    }
+    const double det = a11 * a22 - a12 * a12;
+    for (int k = 0; k < 3; ++k) {
+      double a1, a2;
+      if (nfit >= 2 && fabs(det) > SMALL) {
+        a1 = (b1[k] * a22 - b2[k] * a12) / det;
+        a2 = (-b1[k] * a12 + b2[k] * a11) / det;
+      } else if (lden > SMALL) {
+        a1 = ln[k] / lden;
+        a2 = 0.0;
+      } else {
+        a1 = (pD[k] - pU[k]) / fmax(c.s_down, SMALL);
+        a2 = 0.0;
+      }
+      out[k] = clampv(pU[k] + a1 * c.s_face + a2 * c.s_face * c.s_face, fmin(pU[k], pD[k]), fmax(pU[k], pD[k]));
+    }
+  }
+  // soa_vanleer_tvd_ctx_dev / soa_koren_tvd_ctx_dev (:2716-2786)
+  DM_HD double tvd(const ReconCtx& c, int far_up, int k) const {
+    const double pU = U[3 * size_t(c.up) + k], pD = U[3 * size_t(c.down) + k];
+    const double slope_dn = (pD - pU) / fmax(c.s_down, SMALL);
+    double psi = 0.0;
+    if (far_up >= 0 && fabs(slope_dn) > SMALL) {
+      double t[3];
+      for (int i = 0; i < 3; ++i) t[i] = M.ctr[3 * size_t(far_up) + i] + c.up_shift[i] - c.x_up[i];
+      const double s_up = -dot3(t, c.dir_hat);
+      if (s_up > SMALL) {
+        const double r = ((pU - U[3 * size_t(far_up) + k]) / s_up) / slope_dn;
+        if (SCHEME == SCHEME_KOREN) {
+          psi = fmax(0.0, fmin(fmin(2.0 * r, (1.0 + 2.0 * r) / 3.0), 2.0));
+        } else {
+          psi = (r + fabs(r)) / (1.0 + fabs(r));
+          psi = fmax(0.0, fmin(psi, 2.0));
+        }
+      }
+    }
+    return clampv(pU + psi * slope_dn * c.s_face, fmin(pU, pD), fmax(pU, pD));
+  }
+  DM_HD void operator()(int f) const {
+    const int o = M.fown[f], nb = M.fnb[f];
+    double Fn;
+    int b = 0;
+    if (nb >= 0) {
+      Fn = UF[f];
+    } else {
+      b = f - M.nint;
+      Fn = is_solid(M.bbc[b]) ? 0.0 : UF[f];
+    }
+    ReconCtx c;
+    const bool ok = context(f, Fn, c);
+    int far_up = -1;
+    if (ok && SCHEME != SCHEME_QUICK) far_up = far_upwind(c);
+    if (!isfinite(Fn)) Fn = 0.0;
+    double q[3] = {0.0, 0.0, 0.0};
+    if (ok && SCHEME == SCHEME_QUICK) quick3(c, q);
+    for (int k = 0; k < 3; ++k) {
+      double phi;
+      if (ok) {
+        phi = (SCHEME == SCHEME_QUICK) ? q[k] : tvd(c, far_up, k);
+        if (!isfinite(phi)) phi = (Fn >= 0.0 || nb < 0) ? U[3 * size_t(o) + k] : U[3 * size_t(nb) + k];
+      } else {
+        phi = Ub[3 * size_t(b) + k];
+      }
+      if (!isfinite(phi)) phi = 0.0;
+      phiF[3 * size_t(f) + k] = phi;
+    }
+  }
+};
+
 // ------------------------------------------------------------------ explicit right-hand side
 
 // advection_rhs_vector_{kep,cds}_soa (src/numerics/mod_advection.f90:1013-1392) + viscous_rhs_vector
@@ -440,6 +762,7 @@ struct ExplicitRhs {
   double *H, *Unew;
   double nu, f0, f1, f2, dt, a, bcoef;
   int has_wm;
+  const double* phiF;  // reconstruction schemes: face values from ReconFace
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -462,6 +785,12 @@ struct ExplicitRhs {
           for (int q = 0; q < 3; ++q) {
             const double phi = 0.5 * (Uo[q] + Un[q]);
             cv[q] = side ? cv[q] + Fn * phi * iv : cv[q] - Fn * phi * iv;
+          }
+        } else if (SCHEME != 3) {
+          const double Fr = isfinite(Fn) ? Fn : 0.0;
+          for (int q = 0; q < 3; ++q) {
+            const double phi = phiF[3 * size_t(f) + q];
+            cv[q] = side ? cv[q] + Fr * phi * iv : cv[q] - Fr * phi * iv;
           }
         } else {
           const double gcf = M.gc[f];
@@ -503,8 +832,12 @@ struct ExplicitRhs {
         double Fn;
         if (bc == BC_PERIODIC || bc == BC_OUTLET) Fn = UF[f];
         else if (is_solid(bc)) Fn = 0.0;
+        else if (SCHEME != 3 && SCHEME != 4) Fn = UF[f];
         else Fn = ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2];
-        if (SCHEME == 4) {
+        if (SCHEME != 3 && SCHEME != 4) {
+          if (!isfinite(Fn)) Fn = 0.0;
+          for (int q = 0; q < 3; ++q) cv[q] = cv[q] - Fn * phiF[3 * size_t(f) + q] * iv;
+        } else if (SCHEME == 4) {
           dw = dw + Fn;
           for (int q = 0; q < 3; ++q) {
             const double phi = (nbp >= 0) ? 0.5 * (Uc[q] + U[3 * size_t(nbp) + q]) : ub[q];

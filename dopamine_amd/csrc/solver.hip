@@ -57,7 +57,8 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (!m || !p) throw std::runtime_error("null mesh or params");
     if (dopamine_device_count() <= 0) throw std::runtime_error("no HIP device available");
     if (p->wall_model_height > 0.0) throw std::runtime_error("wall_model_height > 0 (sampling walk) is not supported");
-    if (p->scheme != 3 && p->scheme != 4) throw std::runtime_error("advection scheme must be CDS (3) or KEP (4)");
+    if (p->scheme < 1 || p->scheme > 5)
+      throw std::runtime_error("advection scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3), KEP (4) or QUICK (5)");
     if (p->precision != 0 && p->precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
     if (p->cg_variant != 0 && p->cg_variant != 1) throw std::runtime_error("cg_variant must be 0 or 1");
     s = new Solver();
@@ -174,3 +175,43 @@ int dopamine_solver_sync(void* h) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- measurement utility
+
+namespace {
+__global__ void __launch_bounds__(256) k_hbm_copy(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) b[i] = a[i];
+}
+}  // namespace
+
+extern "C" int dopamine_hbm_copy_probe(int64_t n, int reps, double* avg_ms) {
+  return guard([&] {
+    if (n <= 0 || reps <= 0) throw std::runtime_error("hbm copy probe: n and reps must be positive");
+    double *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DM_CHECK(hipMalloc(&a, size_t(n) * sizeof(double)));
+    DM_CHECK(hipMalloc(&b, size_t(n) * sizeof(double)));
+    DM_CHECK(hipStreamCreate(&st));
+    DM_CHECK(hipEventCreate(&e0));
+    DM_CHECK(hipEventCreate(&e1));
+    DM_CHECK(hipMemsetAsync(a, 0, size_t(n) * sizeof(double), st));
+    const int grid = 1024;
+    hipLaunchKernelGGL(k_hbm_copy, dim3(grid), dim3(256), 0, st, a, b, n);  // warm
+    float total = 0.0f;
+    for (int r = 0; r < reps; ++r) {
+      float ms = 0.0f;
+      hipExtLaunchKernelGGL(k_hbm_copy, dim3(grid), dim3(256), 0, st, e0, e1, 0, (const double*)a, b, n);
+      DM_CHECK(hipEventSynchronize(e1));
+      DM_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      total += ms;
+    }
+    *avg_ms = double(total) / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return 0;
+  });
+}

@@ -134,6 +134,9 @@ class SolverT {
   SolverScalars* sc = nullptr;
   EllOp A;
   EllOp32 A32;
+  double* phiF = nullptr;  // reconstruction schemes: 3 face values per face
+  int n_amdw = 0;
+  int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
   static double mixed_tau() {
     const char* e = std::getenv("DOPAMINE_MIXED_TAU");
@@ -495,6 +498,11 @@ class SolverT {
     M.wbl = up(wbl);
     M.fctr = up(vec_d(d.face_centre, 3 * size_t(nf)));
     M.dcf = up(vec_d(d.face_dcf, nf));
+    {
+      std::vector<int> fo(d.face_owner, d.face_owner + nf), fn(d.face_neighbour, d.face_neighbour + nf);
+      M.fown = up(fo);
+      M.fnb = up(fn);
+    }
     M.bbc = up(bbc); M.bpatch = up(bpatch); M.bown = up(bown); M.pnb = up(pnb); M.pair = up(pair);
     M.pgc = up(pgc); M.pdcf = up(pdcf); M.cb = up(cb); M.kbnd = up(kbnd); M.wsy = up(wsy); M.wsc = up(wsc);
     M.pcoef = up(pcoef.empty() ? std::vector<double>(1, 0.0) : pcoef);
@@ -527,6 +535,30 @@ class SolverT {
       dir_rows = up(dir_rows_h);
       dir_vals = up(dir_vals_h);
     }
+    {  // AMD wall cells: cells owning wall-model faces, their faces in boundary order
+      std::vector<int> first(nc, -1), cells, offs(1, 0), faces;
+      std::vector<std::vector<int>> per;
+      for (int b = 0; b < nb; ++b) {
+        if (bbc[b] != BC_WALL_MODEL) continue;
+        const int o = bown[b];
+        if (first[o] < 0) {
+          first[o] = int(cells.size());
+          cells.push_back(o);
+          per.emplace_back();
+        }
+        per[first[o]].push_back(b);
+      }
+      for (auto& v : per) {
+        faces.insert(faces.end(), v.begin(), v.end());
+        offs.push_back(int(faces.size()));
+      }
+      n_amdw = int(cells.size());
+      if (n_amdw) {
+        amdw_cell = up(cells);
+        amdw_off = up(offs);
+        amdw_face = up(faces);
+      }
+    }
     // outlet pressure per boundary face (patch p_val; no backflow traction)
     std::vector<double> opb(std::max(1, nb), 0.0);
     for (int b = 0; b < nb; ++b) opb[b] = patchP_h[bpatch[b]];
@@ -558,6 +590,8 @@ class SolverT {
         x.upload(U, u0.data(), u0.size());
         break;
       }
+    if (p.scheme == SCHEME_QUICK || p.scheme == SCHEME_VANLEER || p.scheme == SCHEME_KOREN)
+      phiF = fresh<double>(3 * size_t(nf));
     cg.init(&x, nc);
     cg.set_single_reduction(p.cg_variant == 1);
     if (p.precision == 1) cg.set_mixed(A32, mixed_tau());
@@ -680,11 +714,17 @@ class SolverT {
   // ---------------------------------------------------------------- gradients
 
   // compute_all_gradients (velocity), with the SGS viscosity of the stage fused in when sgs_dt > 0
+  template <int SCHEME>
+  void explicit_rhs(double dt, int k) {
+    x.for_each(nc, ExplicitRhs<SCHEME>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
+                                       prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0, phiF});
+  }
+
   void gradients(bool with_sgs, double sgs_dt) {
     const int sgs = with_sgs ? prm.sgs : -1;
     if (prm.gradient == 0) {
       x.tbegin(K_LS_GRAD);
-      x.for_each(nc, LsGradVec{M, U, U_bnd, grad, nu_sgs, sgs, prm.c_vreman, prm.cs_smag, sgs_dt});
+      x.for_each(nc, LsGradVec{M, U, U_bnd, grad, nu_sgs, sgs, prm.c_vreman, prm.cs_smag, prm.c_amd, sgs_dt});
       x.tend(K_LS_GRAD);
     } else {
       x.for_each(nc, GgGradVec{M, U, U_bnd, nullptr, grad});
@@ -692,8 +732,11 @@ class SolverT {
         x.copy(gprev, grad, 9 * size_t(nc));
         x.for_each(nc, GgGradVec{M, U, U_bnd, gprev, grad});
       }
-      if (with_sgs) x.for_each(nc, SgsCell{M, grad, nu_sgs, prm.sgs, prm.c_vreman, prm.cs_smag, sgs_dt});
+      if (with_sgs)
+        x.for_each(nc, SgsCell{M, grad, nu_sgs, prm.sgs, prm.c_vreman, prm.cs_smag, prm.c_amd, sgs_dt});
     }
+    if (with_sgs && prm.sgs == 3 && prm.amd_wall_gradient && n_amdw > 0)
+      x.for_each(n_amdw, AmdWall{M, amdw_cell, amdw_off, amdw_face, U, grad, nu_sgs, prm.nu, prm.c_amd, sgs_dt});
   }
   // gradient_of_scalar of a (+ a2) with boundary values pb (+ pb2) into gp
   void scalar_gradient(const double* a, const double* a2, const double* pb, const double* pb2) {
@@ -852,13 +895,17 @@ class SolverT {
         x.zero(tau, 3 * size_t(std::max(1, nb)));
       else if (n_wm)
         x.for_each(nb, WallModel{M, U, tau, prm.nu});
+      if (scheme == SCHEME_QUICK) x.for_each(nf, ReconFace<SCHEME_QUICK>{M, U, U_bnd, UF, grad, phiF});
+      if (scheme == SCHEME_VANLEER) x.for_each(nf, ReconFace<SCHEME_VANLEER>{M, U, U_bnd, UF, grad, phiF});
+      if (scheme == SCHEME_KOREN) x.for_each(nf, ReconFace<SCHEME_KOREN>{M, U, U_bnd, UF, grad, phiF});
       x.tbegin(K_EXPLICIT);
-      if (scheme == 4)
-        x.for_each(nc, ExplicitRhs<4>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
-                                      prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0});
-      else
-        x.for_each(nc, ExplicitRhs<3>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
-                                      prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0});
+      switch (scheme) {
+        case SCHEME_KEP: explicit_rhs<SCHEME_KEP>(dt, k); break;
+        case SCHEME_CDS: explicit_rhs<SCHEME_CDS>(dt, k); break;
+        case SCHEME_QUICK: explicit_rhs<SCHEME_QUICK>(dt, k); break;
+        case SCHEME_VANLEER: explicit_rhs<SCHEME_VANLEER>(dt, k); break;
+        default: explicit_rhs<SCHEME_KOREN>(dt, k); break;
+      }
       x.tend(K_EXPLICIT);
       std::swap(U, U2);
       velocity_bc_both();

@@ -8,8 +8,9 @@ import math
 
 import numpy as np
 
-SCHEME_IDS = {"CDS": 3, "CENTRAL": 3, "KEP": 4, "SKEW": 4}
-SGS_IDS = {"NONE": 0, "VREMAN": 1, "SMAGORINSKY": 2}
+SCHEME_IDS = {"VANLEER_TVD": 1, "VANLEER": 1, "KOREN_TVD": 2, "KOREN": 2, "CDS": 3, "CENTRAL": 3, "KEP": 4,
+              "SKEW": 4, "QUICK": 5}
+SGS_IDS = {"NONE": 0, "VREMAN": 1, "SMAGORINSKY": 2, "AMD": 3}
 WALL_MODEL_IDS = {"loglaw": 0, "none": 1}
 
 # Wray RK3 coefficients (src/mod_parameters.f90:64-71)
@@ -27,6 +28,8 @@ class SolverParams:
     sgs_model: str = "VREMAN"
     c_vreman: float = 0.025                  # C_VREMAN, src/mod_parameters.f90:20
     cs_smagorinsky: float = 0.1
+    c_amd: float = 1.0 / 3.0                 # AMD_C_DEFAULT, src/mod_parameters.f90:23
+    amd_wall_gradient: bool = True
     wall_model: str = "loglaw"
     wall_model_height: float = 0.0
     n_nonortho_corr: int = 0                 # 0 = auto
@@ -51,11 +54,12 @@ class SolverParams:
             raise ValueError("cell_projection must be 'LS' (the EXACT V-orthogonal projection is not part of this "
                              "hot path)")
         if self.advection_scheme.upper() not in SCHEME_IDS:
-            raise ValueError(f"advection_scheme {self.advection_scheme!r} not supported (KEP, CDS)")
+            raise ValueError(f"advection_scheme {self.advection_scheme!r} not supported "
+                             "(KOREN_TVD, VANLEER_TVD, QUICK, KEP, CDS)")
         if self.gradient_scheme.upper() not in ("LEAST_SQUARES", "GREEN_GAUSS"):
             raise ValueError("gradient_scheme must be LEAST_SQUARES or GREEN_GAUSS")
-        if self.use_sgs and self.sgs_model.upper() not in ("VREMAN", "SMAGORINSKY"):
-            raise ValueError(f"sgs_model {self.sgs_model!r} not supported (VREMAN, SMAGORINSKY)")
+        if self.use_sgs and self.sgs_model.upper() not in ("VREMAN", "SMAGORINSKY", "AMD"):
+            raise ValueError(f"sgs_model {self.sgs_model!r} not supported (VREMAN, SMAGORINSKY, AMD)")
         if self.adapt_dt:
             raise ValueError("adapt_dt is not supported; the fixed-dt CFL clamp of solver_advance applies")
 

@@ -35,7 +35,7 @@ class Params(ctypes.Structure):
                 ("n_nonortho_corr", _i32), ("min_cos_theta", _d), ("rhie_chow_dual", _i32),
                 ("nonortho_check", _i32), ("rotational", _i32), ("p_tol", _d), ("p_max_iter", _i32),
                 ("cfl_max", _d), ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)),
-                ("precision", _i32), ("cg_variant", _i32)]
+                ("precision", _i32), ("cg_variant", _i32), ("c_amd", _d), ("amd_wall_gradient", _i32)]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -60,7 +60,7 @@ def build_abi(mesh, params, precision=0):
                 params.wall_model_height, params.n_nonortho_corr, params.pressure_min_cos_theta,
                 int(params.rhie_chow_dual), int(params.nonortho_check), int(params.use_rotational_projection),
                 params.p_tol, params.p_max_iter, params.cfl_max, params.dt, arr(pv, _d), arr(uv.reshape(-1), _d),
-                int(precision), int(params.cg_variant))
+                int(precision), int(params.cg_variant), params.c_amd, int(params.amd_wall_gradient))
     return md, pr, keep
 
 

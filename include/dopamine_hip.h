@@ -38,10 +38,10 @@ typedef struct dopamine_mesh_desc {
 typedef struct dopamine_params {
   double nu;
   double f_const[3];
-  int32_t scheme;            /* 3 CDS, 4 KEP (src/mod_parameters.f90:57-62) */
+  int32_t scheme;            /* 1 VANLEER_TVD, 2 KOREN_TVD, 3 CDS, 4 KEP, 5 QUICK (src/mod_parameters.f90:54-60) */
   int32_t gradient;          /* 0 LEAST_SQUARES, 1 GREEN_GAUSS */
   int32_t gg_iterations;
-  int32_t sgs;               /* 0 none, 1 Vreman, 2 Smagorinsky */
+  int32_t sgs;               /* 0 none, 1 Vreman, 2 Smagorinsky, 3 AMD */
   double c_vreman;           /* reference C_VREMAN = 0.025 */
   double cs_smag;            /* 0.1 */
   int32_t wall_model;        /* 0 log-law (Reichardt), 1 none */
@@ -60,6 +60,8 @@ typedef struct dopamine_params {
   int32_t precision;          /* 0: FP64 pressure solve; 1: FP32-SpMV CG inside FP64 iterative refinement */
   int32_t cg_variant;         /* 0: two passes per iteration (KSPCG's Hestenes-Stiefel recurrence);
                                  1: one pass per iteration (single-reduction recurrence) */
+  double c_amd;               /* AMD constant (sgs 3), reference AMD_C_DEFAULT = 1/3 */
+  int32_t amd_wall_gradient;  /* AMD: log-law wall-normal gradient in wall-model cells */
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
@@ -108,6 +110,10 @@ int dopamine_solver_kernel_timing(void *solver, int kernel, int enable);
 /* Launches timed since enabling and their summed device duration in milliseconds. CG passes that
  * run after convergence (gated no-ops) are not counted. */
 int dopamine_solver_kernel_time(void *solver, int kernel, int64_t *launches, double *total_ms);
+/* Measurement utility (no counterpart in the reference): copies n doubles between two fresh HBM
+ * buffers reps times with a plain grid-stride kernel and returns the mean kernel duration; the
+ * known 16 n bytes per launch calibrate the HBM traffic counters and the achievable bandwidth. */
+int dopamine_hbm_copy_probe(int64_t n, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,8 @@ class _Params(ctypes.Structure):
                 ("wall_model", ctypes.c_int), ("wall_model_height", _d), ("n_nonortho_corr", ctypes.c_int),
                 ("min_cos_theta", _d), ("rhie_chow_dual", ctypes.c_int), ("nonortho_check", ctypes.c_int),
                 ("rotational", ctypes.c_int), ("p_tol", _d), ("p_max_iter", ctypes.c_int), ("cfl_max", _d),
-                ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d))]
+                ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)), ("c_amd", _d),
+                ("amd_wall_gradient", ctypes.c_int)]
 
 
 _lib = None
@@ -62,6 +63,8 @@ def lib():
         L.om_eval_closures.argtypes = [ctypes.c_void_p, _d]
         L.om_wall_tau.restype = _d
         L.om_wall_tau.argtypes = [_d, _d, _d]
+        L.om_amd_point.restype = _d
+        L.om_amd_point.argtypes = [_d * 9, _d * 9, _d]
         L.om_counters.argtypes = [ctypes.c_void_p, _i64 * 3]
         L.om_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int * 3]
         L.om_cg_jacobi.argtypes = [ctypes.c_int, ctypes.c_int, _P(_i32), _P(_i32), _P(_d), _P(_d), _P(_d), _P(_d),
@@ -109,7 +112,8 @@ class Solver:
                            WALL_MODEL_IDS[params.wall_model], params.wall_model_height, params.n_nonortho_corr,
                            params.pressure_min_cos_theta, int(params.rhie_chow_dual), int(params.nonortho_check),
                            int(params.use_rotational_projection), params.p_tol, params.p_max_iter, params.cfl_max,
-                           params.dt, arr(pv, _d), arr(uv.reshape(-1), _d))
+                           params.dt, arr(pv, _d), arr(uv.reshape(-1), _d), params.c_amd,
+                           int(params.amd_wall_gradient))
         self.params = params
         self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))
 
@@ -181,6 +185,13 @@ class Solver:
 def wall_tau(u_par, y_w, nu):
     """Reichardt-law wall shear stress u_tau^2 for speed u_par at wall distance y_w."""
     return lib().om_wall_tau(u_par, y_w, nu)
+
+
+def amd_point(alpha, G, c_amd):
+    """AMD closure at one point; alpha[i][j] = du_j/dx_i, G[i][j] (no cap)."""
+    a = np.asarray(alpha, dtype=np.float64)
+    g = np.asarray(G, dtype=np.float64)
+    return lib().om_amd_point((_d * 9)(*a.T.reshape(-1)), (_d * 9)(*g.T.reshape(-1)), c_amd)
 
 
 def cg_jacobi(owner, neighbour, coeff, diag, b, x0, rtol, atol, max_it, nullspace):

@@ -492,6 +492,70 @@ static void smagorinsky(om_solver *s, double dt_stab) {
   }
 }
 
+/* amd_point_viscosity (src/sgs/mod_amd.f90:31-66): al[3j+i] = alpha(i,j), G[3j+i] = G(i,j) */
+static double amd_point(const double *al, const double *G, double c_amd) {
+  double asq = 0.0;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) asq = asq + al[3 * j + i] * al[3 * j + i];
+  double num = 0.0;
+  for (int n = 0; n < 3; ++n)
+    for (int m = 0; m < 3; ++m) {
+      double beta = 0.0;
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) beta = beta + G[3 * j + i] * al[3 * m + i] * al[3 * n + j];
+      num = num - beta * al[3 * n + m];
+    }
+  const double trg = G[0] + G[4] + G[8];
+  if (asq > 1.0e-20 && num > 1.0e-12 * trg * asq * sqrt(asq)) return c_amd * num / asq;
+  return 0.0;
+}
+
+double om_amd_point(const double alpha[9], const double G[9], double c_amd) { return amd_point(alpha, G, c_amd); }
+
+static double wall_utau(double u_par, double y_w, double nu);
+
+/* amd_sgs_viscosity (src/sgs/mod_amd.f90:69-207): point closure and cap in every cell, then, for
+ * wall-model faces in boundary order, the owner's alpha with its wall-parallel wall-normal
+ * derivative swapped for the log-law u_tau / (kappa y) (last face wins). */
+static void amd(om_solver *s, double dt_stab) {
+  const double c = s->p.c_amd;
+  for (int cc = 0; cc < s->nc; ++cc) {
+    double nu = amd_point(&s->grad_U[9 * (size_t)cc], &s->Gf[9 * (size_t)cc], c);
+    const double cap = NU_SGS_STAB_FRAC * pow(s->m.cell_volume[cc], 2.0 / 3.0) / fmax(dt_stab, SMALL);
+    s->nu_sgs[cc] = fmin(nu, cap);
+  }
+  if (!s->p.amd_wall_gradient) return;
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b;
+    if (bc_of_face(s, f) != BC_WALL_MODEL) continue;
+    const int o = s->m.face_owner[f];
+    double nrm[3], dx[3];
+    for (int i = 0; i < 3; ++i) {
+      nrm[i] = -s->m.face_normal[3 * f + i];
+      dx[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+    }
+    const double y_w = fabs(dot3(dx, nrm));
+    const double *u = &s->U[3 * (size_t)o];
+    const double un = dot3(u, nrm);
+    double ut[3];
+    for (int i = 0; i < 3; ++i) ut[i] = u[i] - un * nrm[i];
+    const double ut_mag = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+    if (ut_mag <= 1.0e-14 || y_w <= SMALL) continue;
+    const double u_tau = wall_utau(ut_mag, y_w, s->p.nu);
+    const double g_wall = u_tau / (KAPPA * y_w);
+    double al[9], d_old[3], d_t[3];
+    memcpy(al, &s->grad_U[9 * (size_t)o], sizeof(al));
+    for (int j = 0; j < 3; ++j) d_old[j] = nrm[0] * al[3 * j + 0] + nrm[1] * al[3 * j + 1] + nrm[2] * al[3 * j + 2];
+    const double dn = dot3(d_old, nrm);
+    for (int j = 0; j < 3; ++j) d_t[j] = g_wall * ut[j] / ut_mag - (d_old[j] - dn * nrm[j]);
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) al[3 * j + i] = al[3 * j + i] + nrm[i] * d_t[j];
+    const double nu = amd_point(al, &s->Gf[9 * (size_t)o], c);
+    const double cap = NU_SGS_STAB_FRAC * pow(s->m.cell_volume[o], 2.0 / 3.0) / fmax(dt_stab, SMALL);
+    s->nu_sgs[o] = fmin(nu, cap);
+  }
+}
+
 /* reichardt_uplus / wall_stress_newton (src/sgs/mod_wall_model.f90:50-115) */
 static void reichardt(double yp, double cbuf, double *up, double *dup) {
   const double A1 = 11.0, A2 = 0.33, A1I = 1.0 / 11.0;
@@ -501,7 +565,7 @@ static void reichardt(double yp, double cbuf, double *up, double *dup) {
   (void)A1;
 }
 
-static double wall_tau(double u_par, double y_w, double nu) {
+static double wall_utau(double u_par, double y_w, double nu) {
   const double cbuf = BLOG - log(KAPPA) / KAPPA;
   if (u_par <= SMALL || y_w <= SMALL) return 0.0;
   double ut = fmax(0.01 * u_par, sqrt(nu * u_par / (y_w + SMALL)));
@@ -519,6 +583,11 @@ static double wall_tau(double u_par, double y_w, double nu) {
     }
     ut = un;
   }
+  return ut;
+}
+
+static double wall_tau(double u_par, double y_w, double nu) {
+  const double ut = wall_utau(u_par, y_w, nu);
   return ut * ut;
 }
 
@@ -657,6 +726,287 @@ static void advection_cds(om_solver *s, double *rhs) {
   }
 }
 
+/* ---- reconstruction schemes (QUICK, van Leer TVD, Koren TVD), vector SoA path
+ * advection_rhs_vector_recon_soa (src/numerics/mod_advection.f90:1396-1635) and its device helpers
+ * (:2334-2791). */
+
+static double clamp_value(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+
+/* soa_resolve_periodic_partner_dev (:2362-2414) */
+static int periodic_partner(const om_solver *s, int f, double shift[3]) {
+  shift[0] = shift[1] = shift[2] = 0.0;
+  if (s->m.face_neighbour[f] >= 0 || bc_of_face(s, f) != BC_PERIODIC) return -1;
+  const int b = f - s->nint, owner = s->m.face_owner[f];
+  int pair_cell = resolve_periodic_nb(s, b), pair_face = -1; /* soa_resolve_periodic_nb_dev (:2334-2359) */
+  const int pb = s->m.periodic_pair[b];
+  if (pb >= 0) {
+    pair_face = s->nint + pb;
+    if (pair_cell < 0) pair_cell = s->m.face_owner[pair_face];
+  }
+  if (pair_cell < 0) return -1;
+  double d_per;
+  if (pair_face >= 0)
+    d_per = s->m.face_dcf[f] + s->m.face_dcf[pair_face];
+  else if (s->m.periodic_pair_dcf[b] > SMALL)
+    d_per = s->m.face_dcf[f] + s->m.periodic_pair_dcf[b];
+  else
+    d_per = 2.0 * s->m.face_dcf[f];
+  for (int i = 0; i < 3; ++i)
+    shift[i] = (s->m.cell_centroid[3 * owner + i] + d_per * s->m.face_normal[3 * f + i]) -
+               s->m.cell_centroid[3 * pair_cell + i];
+  return pair_cell;
+}
+
+typedef struct {
+  int up, down;
+  double x_up[3], x_down[3], x_face[3], up_shift[3], dir_hat[3], s_face, s_down;
+} recon_ctx;
+
+static double norm3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* soa_resolve_reconstruction_context_dev (:2417-2498) */
+static int recon_context(const om_solver *s, int f, double Fn, recon_ctx *c) {
+  memset(c, 0, sizeof(*c));
+  for (int i = 0; i < 3; ++i) c->x_face[i] = s->m.face_centre[3 * f + i];
+  const int owner = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+  if (nb >= 0) {
+    if (Fn >= 0.0) {
+      c->up = owner;
+      c->down = nb;
+    } else {
+      c->up = nb;
+      c->down = owner;
+    }
+    for (int i = 0; i < 3; ++i) {
+      c->x_up[i] = s->m.cell_centroid[3 * c->up + i];
+      c->x_down[i] = s->m.cell_centroid[3 * c->down + i];
+    }
+  } else {
+    double sh[3];
+    const int pc = periodic_partner(s, f, sh);
+    if (pc < 0) return 0;
+    if (Fn >= 0.0) {
+      c->up = owner;
+      c->down = pc;
+      for (int i = 0; i < 3; ++i) {
+        c->x_up[i] = s->m.cell_centroid[3 * owner + i];
+        c->x_down[i] = s->m.cell_centroid[3 * pc + i] + sh[i];
+      }
+    } else {
+      c->up = pc;
+      c->down = owner;
+      for (int i = 0; i < 3; ++i) {
+        c->up_shift[i] = sh[i];
+        c->x_up[i] = s->m.cell_centroid[3 * pc + i] + c->up_shift[i];
+        c->x_down[i] = s->m.cell_centroid[3 * owner + i];
+      }
+    }
+  }
+  double dvec[3];
+  for (int i = 0; i < 3; ++i) dvec[i] = c->x_face[i] - c->x_up[i];
+  double dn = norm3(dvec);
+  if (dn <= SMALL) {
+    for (int i = 0; i < 3; ++i) dvec[i] = c->x_down[i] - c->x_up[i];
+    dn = norm3(dvec);
+  }
+  if (dn <= SMALL) return 0;
+  double tf[3], td[3];
+  for (int i = 0; i < 3; ++i) {
+    c->dir_hat[i] = dvec[i] / dn;
+    tf[i] = c->x_face[i] - c->x_up[i];
+    td[i] = c->x_down[i] - c->x_up[i];
+  }
+  c->s_face = dot3(tf, c->dir_hat);
+  c->s_down = dot3(td, c->dir_hat);
+  if (c->s_face <= SMALL || c->s_down <= c->s_face) {
+    for (int i = 0; i < 3; ++i) dvec[i] = c->x_down[i] - c->x_up[i];
+    dn = norm3(dvec);
+    if (dn <= SMALL) return 0;
+    for (int i = 0; i < 3; ++i) c->dir_hat[i] = dvec[i] / dn;
+    c->s_face = dot3(tf, c->dir_hat);
+    c->s_down = dot3(td, c->dir_hat);
+  }
+  if (c->s_down <= SMALL) return 0;
+  c->s_face = fmin(fmax(c->s_face, 0.0), c->s_down);
+  return 1;
+}
+
+/* the other cell across face fj of cell `cell` (owner / neighbour), -1 on a boundary face */
+static int across(const om_solver *s, int fj, int cell) {
+  int nb = s->m.face_owner[fj];
+  if (nb == cell) nb = s->m.face_neighbour[fj];
+  return nb;
+}
+
+/* soa_find_far_upwind_dev (:2501-2549) */
+static int find_far_upwind(const om_solver *s, const recon_ctx *c) {
+  int far_up = -1;
+  double best_align = -1.0, best_perp2 = 1.0e308;
+  for (int q = s->m.cell_face_offset[c->up]; q < s->m.cell_face_offset[c->up + 1]; ++q) {
+    const int fj = s->m.cell_faces[q];
+    int nb = across(s, fj, c->up);
+    if (nb < 0 && fj >= s->nint && bc_of_face(s, fj) == BC_PERIODIC) nb = resolve_periodic_nb(s, fj - s->nint);
+    if (nb == c->down || nb < 0) continue;
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = s->m.cell_centroid[3 * nb + i] + c->up_shift[i] - c->x_up[i];
+    const double sd = dot3(d, c->dir_hat);
+    if (sd >= -1.0e-12) continue;
+    const double dist2 = fmax(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], SMALL);
+    const double align = -sd / sqrt(dist2);
+    const double perp2 = fmax(0.0, dist2 - sd * sd);
+    if (align > best_align + 1.0e-12 || (fabs(align - best_align) <= 1.0e-12 && perp2 < best_perp2 - 1.0e-12) ||
+        (fabs(align - best_align) <= 1.0e-12 && fabs(perp2 - best_perp2) <= 1.0e-12 && nb < far_up)) {
+      best_align = align;
+      best_perp2 = perp2;
+      far_up = nb;
+    }
+  }
+  return far_up;
+}
+
+/* axial_delta (:1682-1686) */
+static double axial_delta(double phi_far, double phi_up, const double *g, const double *dvec, double sv,
+                          const double *dir) {
+  double t[3];
+  for (int i = 0; i < 3; ++i) t[i] = dvec[i] - sv * dir[i];
+  return phi_far - phi_up - dot3(g, t);
+}
+
+/* soa_quick_reconstruction3_dev (:2632-2713) */
+static void quick3(const om_solver *s, const recon_ctx *c, double out[3]) {
+  const double *U = s->U;
+  double phiU[3], phiD[3], b1[3], b2[3], ln[3], gup[3][3], dl[3];
+  for (int k = 0; k < 3; ++k) {
+    phiU[k] = U[3 * c->up + k];
+    phiD[k] = U[3 * c->down + k];
+    for (int i = 0; i < 3; ++i) gup[k][i] = s->grad_U[GI(c->up, i, k)];
+  }
+  double dvec[3];
+  for (int i = 0; i < 3; ++i) dvec[i] = c->x_down[i] - c->x_up[i];
+  double dist2 = fmax(dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2], SMALL);
+  double w = 1.0 / dist2;
+  double a11 = w * c->s_down * c->s_down;
+  double a12 = a11 * c->s_down;
+  double a22 = a12 * c->s_down;
+  double lden = a11;
+  int nfit = 1;
+  for (int k = 0; k < 3; ++k) {
+    dl[k] = axial_delta(phiD[k], phiU[k], gup[k], dvec, c->s_down, c->dir_hat);
+    b1[k] = w * c->s_down * dl[k];
+    b2[k] = w * c->s_down * c->s_down * dl[k];
+    ln[k] = b1[k];
+  }
+  for (int q = s->m.cell_face_offset[c->up]; q < s->m.cell_face_offset[c->up + 1]; ++q) {
+    const int fj = s->m.cell_faces[q];
+    const int nb = across(s, fj, c->up);
+    if (nb < 0 || nb == c->down) continue;
+    double xnb[3];
+    for (int i = 0; i < 3; ++i) {
+      xnb[i] = s->m.cell_centroid[3 * nb + i] + c->up_shift[i];
+      dvec[i] = xnb[i] - c->x_up[i];
+    }
+    const double sv = dot3(dvec, c->dir_hat);
+    if (sv >= -1.0e-12) continue;
+    dist2 = fmax(dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2], SMALL);
+    if (sv * sv < 0.25 * dist2) continue;
+    const double align = fmin(1.0, -sv / sqrt(dist2));
+    w = (0.25 + align * align) / dist2;
+    a11 = a11 + w * sv * sv;
+    a12 = a12 + w * sv * sv * sv;
+    a22 = a22 + w * sv * sv * sv * sv;
+    lden = lden + w * sv * sv;
+    for (int k = 0; k < 3; ++k) {
+      dl[k] = axial_delta(U[3 * nb + k], phiU[k], gup[k], dvec, sv, c->dir_hat);
+      b1[k] = b1[k] + w * sv * dl[k];
+      b2[k] = b2[k] + w * sv * sv * dl[k];
+      ln[k] = ln[k] + w * sv * dl[k];
+    }
+    nfit = nfit + 1;
+  }
+  const double det = a11 * a22 - a12 * a12;
+  for (int k = 0; k < 3; ++k) {
+    double a1, a2;
+    if (nfit >= 2 && fabs(det) > SMALL) {
+      a1 = (b1[k] * a22 - b2[k] * a12) / det;
+      a2 = (-b1[k] * a12 + b2[k] * a11) / det;
+    } else if (lden > SMALL) {
+      a1 = ln[k] / lden;
+      a2 = 0.0;
+    } else {
+      a1 = (phiD[k] - phiU[k]) / fmax(c->s_down, SMALL);
+      a2 = 0.0;
+    }
+    out[k] = phiU[k] + a1 * c->s_face + a2 * c->s_face * c->s_face;
+    out[k] = clamp_value(out[k], fmin(phiU[k], phiD[k]), fmax(phiU[k], phiD[k]));
+  }
+}
+
+/* soa_vanleer_tvd_ctx_dev / soa_koren_tvd_ctx_dev (:2716-2786), one component */
+static double tvd(const om_solver *s, const recon_ctx *c, int far_up, int k, int koren) {
+  const double phiU = s->U[3 * c->up + k], phiD = s->U[3 * c->down + k];
+  const double slope_dn = (phiD - phiU) / fmax(c->s_down, SMALL);
+  double psi = 0.0;
+  if (far_up >= 0 && fabs(slope_dn) > SMALL) {
+    double t[3];
+    for (int i = 0; i < 3; ++i) t[i] = s->m.cell_centroid[3 * far_up + i] + c->up_shift[i] - c->x_up[i];
+    const double s_up = -dot3(t, c->dir_hat);
+    if (s_up > SMALL) {
+      const double slope_up = (phiU - s->U[3 * far_up + k]) / s_up;
+      const double r = slope_up / slope_dn;
+      if (koren) {
+        psi = fmax(0.0, fmin(fmin(2.0 * r, (1.0 + 2.0 * r) / 3.0), 2.0));
+      } else {
+        psi = (r + fabs(r)) / (1.0 + fabs(r));
+        psi = fmax(0.0, fmin(psi, 2.0));
+      }
+    }
+  }
+  return clamp_value(phiU + psi * slope_dn * c->s_face, fmin(phiU, phiD), fmax(phiU, phiD));
+}
+
+static void advection_recon(om_solver *s, double *rhs) {
+  const int scheme = s->p.scheme;
+  memset(rhs, 0, sizeof(double) * 3 * (size_t)s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    int b = 0;
+    double Fn;
+    if (nb >= 0) {
+      Fn = s->U_face[f];
+    } else {
+      b = f - s->nint;
+      const int bc = bc_of_face(s, f);
+      if (bc == BC_WALL_NOSLIP || bc == BC_WALL_MODEL || bc == BC_WALL_VELOCITY || bc == BC_SLIP ||
+          bc == BC_SYMMETRY)
+        Fn = 0.0;
+      else
+        Fn = s->U_face[f];
+    }
+    recon_ctx c;
+    const int ok = recon_context(s, f, Fn, &c);
+    int far_up = -1;
+    if (ok && scheme != OM_SCHEME_QUICK) far_up = find_far_upwind(s, &c);
+    if (!isfinite(Fn)) Fn = 0.0;
+    double q[3] = {0.0, 0.0, 0.0};
+    if (ok && scheme == OM_SCHEME_QUICK) quick3(s, &c, q);
+    for (int k = 0; k < 3; ++k) {
+      double phi;
+      if (ok) {
+        if (scheme == OM_SCHEME_QUICK)
+          phi = q[k];
+        else
+          phi = tvd(s, &c, far_up, k, scheme == OM_SCHEME_KOREN);
+        if (!isfinite(phi)) phi = (Fn >= 0.0 || nb < 0) ? s->U[3 * o + k] : s->U[3 * nb + k];
+      } else {
+        phi = s->U_bnd[3 * b + k];
+      }
+      if (!isfinite(phi)) phi = 0.0;
+      rhs[3 * o + k] = rhs[3 * o + k] - Fn * phi * s->inv_vol[o];
+      if (nb >= 0) rhs[3 * nb + k] = rhs[3 * nb + k] + Fn * phi * s->inv_vol[nb];
+    }
+  }
+}
+
 static double ortho_weight(double s_dot_dhat, double area) {
   return area * fmax(0.0, fmin(1.0, s_dot_dhat / (area + SMALL)));
 }
@@ -758,8 +1108,10 @@ static void viscous_vector(om_solver *s, double *rhs) {
 static void explicit_rhs(om_solver *s) {
   if (s->p.scheme == OM_SCHEME_KEP)
     advection_kep(s, s->conv);
-  else
+  else if (s->p.scheme == OM_SCHEME_CDS)
     advection_cds(s, s->conv);
+  else
+    advection_recon(s, s->conv);
   viscous_vector(s, s->visc);
   for (int c = 0; c < s->nc; ++c)
     for (int k = 0; k < 3; ++k) s->H[3 * c + k] = s->conv[3 * c + k] + s->visc[3 * c + k] + s->p.f_const[k];
@@ -1195,6 +1547,8 @@ static void stage_closures(om_solver *s, double dt_stab) {
     vreman(s, dt_stab);
   else if (s->p.sgs == OM_SGS_SMAGORINSKY)
     smagorinsky(s, dt_stab);
+  else if (s->p.sgs == OM_SGS_AMD)
+    amd(s, dt_stab);
   else
     memset(s->nu_sgs, 0, sizeof(double) * s->nc);
   if (s->p.wall_model == OM_WM_NONE)

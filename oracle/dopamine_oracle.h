@@ -31,9 +31,9 @@ typedef struct om_mesh {
   const double *ls_weight;
 } om_mesh;
 
-enum { OM_SCHEME_CDS = 3, OM_SCHEME_KEP = 4 };
+enum { OM_SCHEME_VANLEER = 1, OM_SCHEME_KOREN = 2, OM_SCHEME_CDS = 3, OM_SCHEME_KEP = 4, OM_SCHEME_QUICK = 5 };
 enum { OM_GRAD_LS = 0, OM_GRAD_GG = 1 };
-enum { OM_SGS_NONE = 0, OM_SGS_VREMAN = 1, OM_SGS_SMAGORINSKY = 2 };
+enum { OM_SGS_NONE = 0, OM_SGS_VREMAN = 1, OM_SGS_SMAGORINSKY = 2, OM_SGS_AMD = 3 };
 enum { OM_WM_LOGLAW = 0, OM_WM_NONE = 1 };
 
 typedef struct om_params {
@@ -58,6 +58,8 @@ typedef struct om_params {
   double dt;             /* requested (maximum) time step */
   const double *patch_p_val;  /* n_patches, outlet pressure */
   const double *patch_U_val;  /* 3*n_patches, inlet / moving-wall velocity */
+  double c_amd;               /* AMD constant, reference 1/3 */
+  int amd_wall_gradient;      /* AMD: log-law wall-normal gradient in wall-model cells */
 } om_params;
 
 typedef struct om_solver om_solver;
@@ -83,6 +85,9 @@ int om_eval_closures(om_solver *s, double dt_stab);
 /* Wall shear stress tau_w = u_tau^2 of the Reichardt-law Newton solve for a wall-parallel speed
  * u_par sampled at wall distance y_w (src/sgs/mod_wall_model.f90:50-115). */
 double om_wall_tau(double u_par, double y_w, double nu);
+/* AMD closure at one point (src/sgs/mod_amd.f90:31-66), column-major alpha(i,j) = du_j/dx_i and
+ * G(i,j), no stability cap. */
+double om_amd_point(const double alpha[9], const double G[9], double c_amd);
 /* Pressure-operator face lists ("ps_owner", "ps_nb", "ps_skip"), 0-based. */
 int32_t *om_int_array(om_solver *s, const char *name, int64_t *len);
 /* Counters: total pressure solves, total CG iterations, last CG iteration count. */

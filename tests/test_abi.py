@@ -67,7 +67,7 @@ def test_params_refuse_out_of_scope_options():
     with pytest.raises(ValueError):
         SolverParams(cell_projection="EXACT").validate()
     with pytest.raises(ValueError):
-        SolverParams(cell_projection="LS", advection_scheme="QUICK").validate()
+        SolverParams(cell_projection="LS", advection_scheme="UPWIND").validate()
     with pytest.raises(ValueError):
         SolverParams(cell_projection="LS", adapt_dt=True).validate()
 

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 REG = cases.regression_cases()
 
 
-@pytest.mark.parametrize("name", ["tgv_skew", "tgv_skew_chk", "hydrowall", "hydro", "pdrop"])
+@pytest.mark.parametrize("name", ["tgv", "tgv_skew", "tgv_skew_chk", "hydrowall", "hydro", "pdrop"])
 def test_gpu_matches_reference_regression_csv(name):
     """The reference's own regression bar (compare_stats.py: rtol 1e-6, divU < 1e-9)."""
     case = REG[name]
@@ -98,3 +98,35 @@ def test_gpu_mixed_precision_matches_oracle_to_solver_tolerance(name):
     assert du < 1e-7 and dp < 1e-6
     assert dg < max(10 * do, 1e-10)
     assert g.counters()["sweeps"] > 0
+
+
+@pytest.mark.parametrize("wall_gradient", [True, False])
+def test_gpu_channel_amd_matches_oracle(wall_gradient):
+    """AMD SGS (c = 1/3) with and without the log-law wall-cell gradient, stretched channel."""
+    case = cases.small_channel(16, 12, 8)
+    case.params.p_tol = 1e-12
+    case.params.sgs_model = "AMD"
+    case.params.amd_wall_gradient = wall_gradient
+    mesh = case.make_mesh()
+    print(f"[gpu] channel AMD wall_gradient={wall_gradient}", flush=True)
+    g, o = _pair(case, mesh, 4)
+    for name in ("U", "p", "nu_sgs"):
+        a = o.array(name)
+        d = rel_l2(a, g.get(name)[: a.size])
+        print(f"[gpu] AMD {name}: rel-L2 {d:.2e}", flush=True)
+        assert d < 1e-10, name
+
+
+@pytest.mark.parametrize("scheme", ["QUICK", "VANLEER_TVD", "KOREN_TVD"])
+def test_gpu_reconstruction_schemes_match_oracle(scheme):
+    """QUICK / van Leer / Koren on the skewed periodic box and the outlet box."""
+    for name in ("tgv_skew", "pdrop"):
+        case = REG[name]
+        case.params.p_tol = 1e-12
+        case.params.advection_scheme = scheme
+        mesh = case.make_mesh()
+        print(f"[gpu] {scheme} on {name}", flush=True)
+        g, o = _pair(case, mesh, 3)
+        du, dp = rel_l2(o.U, g.U), rel_l2(o.p, g.p)
+        print(f"[gpu] {scheme} {name}: rel-L2 U {du:.2e} p {dp:.2e}", flush=True)
+        assert du < 1e-10 and dp < 1e-10

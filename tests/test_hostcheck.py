@@ -53,7 +53,22 @@ def test_host_channel_closures_match_oracle(gradient):
         assert rel_l2(a, h.get(name)[: a.size]) < 1e-10, name
 
 
-@pytest.mark.parametrize("sgs", ["VREMAN", "SMAGORINSKY"])
+@pytest.mark.parametrize("wall_gradient", [True, False])
+def test_host_channel_amd_matches_oracle(wall_gradient):
+    """AMD with and without the log-law wall-cell gradient on the stretched wall-model channel."""
+    case = cases.small_channel(16, 12, 8)
+    case.params.p_tol = 1e-12
+    case.params.sgs_model = "AMD"
+    case.params.amd_wall_gradient = wall_gradient
+    mesh = case.make_mesh()
+    h, o = _pair(case, mesh, 3)
+    assert o.array("nu_sgs").max() > 0
+    for name in ("U", "p", "nu_sgs"):
+        a = o.array(name)
+        assert rel_l2(a, h.get(name)[: a.size]) < 1e-10, name
+
+
+@pytest.mark.parametrize("sgs", ["VREMAN", "SMAGORINSKY", "AMD"])
 def test_host_tgv_sgs_matches_oracle(sgs):
     case = cases.tgv32(12)
     case.params.p_tol = 1e-12
@@ -85,3 +100,14 @@ def test_host_mixed_precision_matches_oracle_to_solver_tolerance(name):
     assert rel_l2(o.U, h.U) < 1e-7 and rel_l2(o.p, h.p) < 1e-6
     assert h.counters()["sweeps"] > 0
     assert h.stats()["max_abs_divU"] < max(10 * o.stats()["max_abs_divU"], 1e-10)
+
+
+@pytest.mark.parametrize("scheme", ["QUICK", "VANLEER_TVD", "KOREN_TVD"])
+@pytest.mark.parametrize("name", ["tgv_skew", "pdrop"])
+def test_host_reconstruction_schemes_match_oracle(scheme, name):
+    case = REG[name]
+    case.params.p_tol = 1e-12
+    case.params.advection_scheme = scheme
+    mesh = case.make_mesh()
+    h, o = _pair(case, mesh, 3)
+    assert rel_l2(o.U, h.U) < 1e-10 and rel_l2(o.p, h.p) < 1e-10

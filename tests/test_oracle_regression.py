@@ -12,7 +12,7 @@ from helpers import compare_stats, load_reference_csv, run_case
 REG = cases.regression_cases()
 
 
-@pytest.mark.parametrize("name", ["tgv_skew", "tgv_skew_chk", "hydrowall", "hydro", "pdrop"])
+@pytest.mark.parametrize("name", ["tgv", "tgv_skew", "tgv_skew_chk", "hydrowall", "hydro", "pdrop"])
 def test_oracle_matches_reference_csv(name):
     case = REG[name]
     mesh = case.make_mesh()

@@ -210,3 +210,33 @@ def test_pressure_operator_is_a_symmetric_m_matrix():
     rs = np.asarray(A.sum(axis=1)).ravel()
     assert np.all(rs >= -1e-12 * diag.max())
     assert 0 < np.count_nonzero(rs > 1e-9) < len(diag)
+
+
+# Known answers of the reference's tests/unit/test_amd.f90:17-50 (C = 1/3, tolerance 1e-12).
+C_AMD = 1.0 / 3.0
+
+
+@pytest.mark.parametrize("label,alpha,G,c,want", [
+    ("pure shear", [[0, 0, 0], [3, 0, 0], [0, 0, 0]], np.eye(3), C_AMD, 0.0),
+    ("solid-body rotation", [[0, 2, 0], [-2, 0, 0], [0, 0, 0]], np.eye(3), C_AMD, 0.0),
+    ("isotropic filter strain", np.diag([1.0, 1.0, -2.0]), np.eye(3), C_AMD, C_AMD),
+    ("anisotropic filter strain", np.diag([1.0, 1.0, -2.0]), np.diag([4.0, 1.0, 1.0]), C_AMD, 0.5 * C_AMD),
+    ("backscatter clipped", -np.diag([1.0, 1.0, -2.0]), np.eye(3), C_AMD, 0.0),
+    ("zero gradient", np.zeros((3, 3)), np.eye(3), C_AMD, 0.0),
+    ("constant scaling", np.diag([5.0, 5.0, -10.0]), np.eye(3), 2 * C_AMD, 2 * C_AMD * 5.0),
+])
+def test_amd_point_known_answers(label, alpha, G, c, want):
+    nu = oracle.amd_point(alpha, G, c)
+    assert abs(nu - want) <= 1e-12 * max(1.0, abs(want)), label
+
+
+def test_amd_on_mesh_axisymmetric_compression():
+    """u = (a x, -a y/2, -a z/2), a < 0, on cubes (G = h^2 I): beta = h^2 alpha^2,
+    -beta:alpha = -(3/4) h^2 a^3, alpha:alpha = (3/2) a^2, nu_t = -C h^2 a / 2."""
+    a = -2.0
+    mesh, o = _closures(_params(sgs_model="AMD", c_amd=C_AMD, wall_model="none"), np.diag([a, -a / 2, -a / 2]))
+    nu = o.array("nu_sgs")[_inner(mesh, 3)]
+    assert np.allclose(nu, -C_AMD * H * H * a / 2, rtol=1e-12)
+    # stretching along x (a > 0) is backscatter for AMD: clipped to zero
+    mesh, o = _closures(_params(sgs_model="AMD", c_amd=C_AMD, wall_model="none"), np.diag([-a, a / 2, a / 2]))
+    assert np.abs(o.array("nu_sgs")[_inner(mesh, 3)]).max() == 0.0
