@@ -113,3 +113,34 @@ def small_channel(nx=16, ny=12, nz=8):
     c = channel(nx, ny, nz)
     c.name = f"channel{nx}x{ny}x{nz}"
     return c
+
+
+def cavity_tets(n=55, jitter=0.15, seed=3200, shuffle_seed=1):
+    """Lid-driven cavity at Re 3200 on 6 n^3 jittered, shuffled tetrahedra (unit box, lid z = 1
+    moving at u = 1, other walls no-slip), Koren TVD advection, AMD SGS (c = 1/3), automatic
+    non-orthogonal correctors."""
+    names = ("x0", "xL", "y0", "yL", "z0", "lid")
+    types = ("noSlip",) * 5 + ("movingWall",)
+    return Case(
+        f"cavity_tets{n}",
+        lambda: M.tet_box(n, jitter=jitter, seed=seed, shuffle_seed=shuffle_seed, group_types=types,
+                          group_names=names),
+        SolverParams(nu=1.0 / 3200.0, advection_scheme="KOREN_TVD", use_sgs=True, sgs_model="AMD",
+                     wall_model="none", dt=2.0e-3, cfl_max=0.5, p_tol=1.0e-8, p_max_iter=1000,
+                     n_nonortho_corr=0, rhie_chow_dual=True, cell_projection="LS",
+                     patch_U={"lid": (1.0, 0.0, 0.0)}),
+        ic="uniform", ic_uref=0.0, nsteps=100)
+
+
+def rayleigh_benard(nx=256, ny=256, nz=32, ra=1.0e7, pr=0.7):
+    """Rayleigh-Benard convection in a 4 x 4 x 1 box, wall-clustered in z, Boussinesq with
+    beta = 1, g = (0, 0, -1), hot bottom (T = 1) and cold top (T = 0) walls, T_ref = 0.5,
+    nu = sqrt(Pr/Ra), alpha = 1/sqrt(Ra Pr); KEP momentum, van Leer TVD temperature; fluid at rest
+    with a seeded 1e-3 perturbation of the temperature."""
+    return Case(
+        f"rb{nx}x{ny}x{nz}", lambda: M.rb_box(nx, ny, nz),
+        SolverParams(nu=math.sqrt(pr / ra), advection_scheme="KEP", use_sgs=False, dt=5.0e-2, cfl_max=0.5,
+                     p_tol=1.0e-8, p_max_iter=1000, rhie_chow_dual=False, cell_projection="LS",
+                     boussinesq=True, alpha_T=1.0 / math.sqrt(ra * pr), beta=1.0, T_ref=0.5, g=(0.0, 0.0, -1.0),
+                     temperature_scheme="VANLEER_TVD", patch_T={"bottom_wall": 1.0, "top_wall": 0.0}),
+        ic="uniform", ic_uref=0.0, nsteps=200, extra={"t_noise": 1.0e-3, "seed": 1700})

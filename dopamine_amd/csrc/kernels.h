@@ -749,6 +749,258 @@ struct ReconFace {
   }
 };
 
+// ------------------------------------------------------------------ temperature
+
+// apply_temperature_bc (src/bc/mod_bc.f90:489-580), one boundary face. filter: 0 all,
+// 1 non-periodic, 2 periodic only. gT may be null (zero-gradient extrapolation).
+struct BcTemperature {
+  DevMesh M;
+  const double *T, *gT, *patchT;
+  double* Tb;
+  int filter;
+  DM_HD void operator()(int b) const {
+    const int f = M.nint + b, bc = M.bbc[b];
+    if (filter == 2 && bc != BC_PERIODIC) return;
+    if (filter == 1 && bc == BC_PERIODIC) return;
+    const int o = M.bown[b];
+    if (bc == BC_INLET || bc == BC_FIXEDTEMP) {
+      Tb[b] = patchT[M.bpatch[b]];
+    } else if (bc == BC_PERIODIC) {
+      const int nb = M.pnb[b];
+      Tb[b] = (nb >= 0) ? M.pgc[b] * T[o] + (1.0 - M.pgc[b]) * T[nb] : T[o];
+    } else if (gT) {
+      double d[3];
+      for (int i = 0; i < 3; ++i) d[i] = M.fctr[3 * size_t(f) + i] - M.ctr[3 * size_t(o) + i];
+      Tb[b] = T[o] + gT[3 * size_t(o)] * d[0] + gT[3 * size_t(o) + 1] * d[1] + gT[3 * size_t(o) + 2] * d[2];
+    } else {
+      Tb[b] = T[o];
+    }
+  }
+};
+
+// Face values of a scalar by QUICK / van Leer / Koren (advection_rhs_scalar_recon_soa,
+// src/numerics/mod_advection.f90:488-675): the context and far-upwind search of ReconFace, the
+// single-field QUICK fit (soa_quick_reconstruction_ctx_dev, :2552-2629).
+template <int SCHEME>
+struct ReconFaceScalar {
+  ReconFace<SCHEME> R;  // geometry helpers (its U/grad are unused)
+  const double *phi, *pb, *gphi;
+  double* phiF;
+  DM_HD double quick1(const ReconCtx& c) const {
+    const DevMesh& M = R.M;
+    const int nc = M.nc;
+    const double pU = phi[c.up], pD = phi[c.down];
+    const double* gup = &gphi[3 * size_t(c.up)];
+    double a11 = 0.0, a12 = 0.0, a22 = 0.0, b1 = 0.0, b2 = 0.0, ln = 0.0, ld = 0.0;
+    int nfit = 0;
+    double dv[3];
+    for (int i = 0; i < 3; ++i) dv[i] = c.x_down[i] - c.x_up[i];
+    double dist2 = fmax(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2], SMALL);
+    double w = 1.0 / dist2;
+    double dl = ReconFace<SCHEME>::axial(pD, pU, gup, dv, c.s_down, c.dir_hat);
+    const double sd = c.s_down;
+    a11 = a11 + w * sd * sd;
+    a12 = a12 + w * sd * sd * sd;
+    a22 = a22 + w * sd * sd * sd * sd;
+    b1 = b1 + w * sd * dl;
+    b2 = b2 + w * sd * sd * dl;
+    ln = ln + w * sd * dl;
+    ld = ld + w * sd * sd;
+    nfit = nfit + 1;
+    const int cnt = M.nA[c.up];
+    for (int q = 0; q < cnt; ++q) {
+      const int nb = M.aoth[size_t(q) * nc + c.up];
+      if (nb < 0 || nb == c.down) continue;
+      for (int i = 0; i < 3; ++i) dv[i] = (M.ctr[3 * size_t(nb) + i] + c.up_shift[i]) - c.x_up[i];
+      const double sv = dot3(dv, c.dir_hat);
+      if (sv >= -1.0e-12) continue;
+      dist2 = fmax(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2], SMALL);
+      if (sv * sv < 0.25 * dist2) continue;
+      const double align = fmin(1.0, -sv / sqrt(dist2));
+      w = (0.25 + align * align) / dist2;
+      dl = ReconFace<SCHEME>::axial(phi[nb], pU, gup, dv, sv, c.dir_hat);
+      a11 = a11 + w * sv * sv;
+      a12 = a12 + w * sv * sv * sv;
+      a22 = a22 + w * sv * sv * sv * sv;
+      b1 = b1 + w * sv * dl;
+      b2 = b2 + w * sv * sv * dl;
+      ln = ln + w * sv * dl;
+      ld = ld + w * sv * sv;
+      nfit = nfit + 1;
+    }
+    const double det = a11 * a22 - a12 * a12;
+    double a1, a2;
+    if (nfit >= 2 && fabs(det) > SMALL) {
+      a1 = (b1 * a22 - b2 * a12) / det;
+      a2 = (-b1 * a12 + b2 * a11) / det;
+    } else if (ld > SMALL) {
+      a1 = ln / ld;
+      a2 = 0.0;
+    } else {
+      a1 = (pD - pU) / fmax(c.s_down, SMALL);
+      a2 = 0.0;
+    }
+    return ReconFace<SCHEME>::clampv(pU + a1 * c.s_face + a2 * c.s_face * c.s_face, fmin(pU, pD), fmax(pU, pD));
+  }
+  DM_HD double tvd1(const ReconCtx& c, int far_up) const {
+    const DevMesh& M = R.M;
+    const double pU = phi[c.up], pD = phi[c.down];
+    const double slope_dn = (pD - pU) / fmax(c.s_down, SMALL);
+    double psi = 0.0;
+    if (far_up >= 0 && fabs(slope_dn) > SMALL) {
+      double t[3];
+      for (int i = 0; i < 3; ++i) t[i] = M.ctr[3 * size_t(far_up) + i] + c.up_shift[i] - c.x_up[i];
+      const double s_up = -dot3(t, c.dir_hat);
+      if (s_up > SMALL) {
+        const double r = ((pU - phi[far_up]) / s_up) / slope_dn;
+        if (SCHEME == SCHEME_KOREN) {
+          psi = fmax(0.0, fmin(fmin(2.0 * r, (1.0 + 2.0 * r) / 3.0), 2.0));
+        } else {
+          psi = (r + fabs(r)) / (1.0 + fabs(r));
+          psi = fmax(0.0, fmin(psi, 2.0));
+        }
+      }
+    }
+    return ReconFace<SCHEME>::clampv(pU + psi * slope_dn * c.s_face, fmin(pU, pD), fmax(pU, pD));
+  }
+  DM_HD void operator()(int f) const {
+    const DevMesh& M = R.M;
+    const int o = M.fown[f], nb = M.fnb[f];
+    double Fn;
+    int b = 0;
+    if (nb >= 0) {
+      Fn = R.UF[f];
+    } else {
+      b = f - M.nint;
+      Fn = is_solid(M.bbc[b]) ? 0.0 : R.UF[f];
+    }
+    ReconCtx c;
+    double v;
+    if (R.context(f, Fn, c)) {
+      v = (SCHEME == SCHEME_QUICK) ? quick1(c) : tvd1(c, R.far_upwind(c));
+      if (!isfinite(v)) v = (Fn >= 0.0 || nb < 0) ? phi[o] : phi[nb];  // before Fn is sanitised
+    } else {
+      v = pb[b];
+    }
+    if (!isfinite(v)) v = 0.0;
+    phiF[f] = v;
+  }
+};
+
+// Temperature right-hand side and predictor for one cell: advection_rhs_scalar (CDS:
+// src/numerics/mod_advection.f90:267-485; reconstruction schemes: face values phiF) +
+// viscous_rhs_scalar (src/numerics/mod_viscous.f90:286-513), H_T = conv + visc,
+// T* = T + dt (alpha H_T + beta H_T,prev) (src/solver/mod_fv_solver.f90:1326-1331).
+struct ExplicitRhsT {
+  DevMesh M;
+  const double *T, *Tb, *gT, *UF, *nus, *HT0, *phiF;
+  double *HT, *Tnew;
+  double alpha, prt, dt, a, bcoef;
+  int cds;
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    const double iv = M.iv[c];
+    double cv = 0.0, vs = 0.0;
+    const int cnt = M.nA[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
+      const double* sf = &M.sf[3 * size_t(f)];
+      if (oth >= 0) {
+        const int o = side ? oth : c, nb = side ? c : oth;
+        double Fn = UF[f];
+        double phif;
+        if (cds) {
+          const double gcf = M.gc[f];
+          const double pl = gcf * T[o] + (1.0 - gcf) * T[nb];
+          double gm[3];
+          for (int i = 0; i < 3; ++i) gm[i] = gcf * gT[3 * size_t(o) + i] + (1.0 - gcf) * gT[3 * size_t(nb) + i];
+          phif = pl + dot3(gm, &M.eps[3 * size_t(f)]);
+          const double w = M.wbl[f];
+          if (w < 1.0) phif = pl + w * (phif - pl);
+          if (!isfinite(phif)) phif = (Fn >= 0.0) ? T[o] : T[nb];
+          if (!isfinite(Fn)) Fn = 0.0;
+          if (!isfinite(phif)) phif = 0.0;
+        } else {
+          if (!isfinite(Fn)) Fn = 0.0;
+          phif = phiF[f];
+        }
+        cv = side ? cv + Fn * phif * iv : cv - Fn * phif * iv;
+        // diffusion
+        const double gc = M.gc[f];
+        const double ae = alpha + (gc * nus[o] + (1.0 - gc) * nus[nb]) / prt;
+        double d_on[3], dnp[3], gf[3];
+        for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
+        const double dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+        const double sdd = dot3(sf, d_on) / dl;
+        for (int i = 0; i < 3; ++i) {
+          dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+          gf[i] = gc * gT[3 * size_t(o) + i] + (1.0 - gc) * gT[3 * size_t(nb) + i];
+        }
+        const double flux = ae * ((T[nb] - T[o]) * ortho_weight(sdd, M.area[f]) / dl + dot3(gf, dnp));
+        vs = side ? vs - flux * iv : vs + flux * iv;
+      } else {
+        const int b = -1 - oth, bc = M.bbc[b];
+        const int nbp = (bc == BC_PERIODIC) ? M.pnb[b] : -1;
+        double Fn = is_solid(bc) ? 0.0 : UF[f];
+        double phif;
+        if (cds) {
+          if (nbp >= 0) {
+            const double g = M.pgc[b];
+            phif = g * T[c] + (1.0 - g) * T[nbp];
+            if (!isfinite(phif)) phif = T[c];
+          } else {
+            phif = Tb[b];
+          }
+          if (!isfinite(Fn)) Fn = 0.0;
+          if (!isfinite(phif)) phif = 0.0;
+        } else {
+          if (!isfinite(Fn)) Fn = 0.0;
+          phif = phiF[f];
+        }
+        cv = cv - Fn * phif * iv;
+        double flux;
+        if (nbp >= 0) {
+          double d_per, gcp;
+          if (M.pair[b] >= 0) {
+            const int pf = M.nint + M.pair[b];
+            d_per = M.dcf[f] + M.dcf[pf];
+            gcp = M.dcf[pf] / (d_per + SMALL);
+          } else if (M.pdcf[b] > SMALL) {
+            d_per = M.dcf[f] + M.pdcf[b];
+            gcp = M.pdcf[b] / (d_per + SMALL);
+          } else {
+            d_per = 2.0 * M.dcf[f];
+            gcp = 0.5;
+          }
+          const double ae = alpha + (gcp * nus[c] + (1.0 - gcp) * nus[nbp]) / prt;
+          double d_on[3], dnp[3], gf[3];
+          for (int i = 0; i < 3; ++i) d_on[i] = d_per * M.nrm[3 * size_t(f) + i];
+          const double dl = d_per + SMALL;
+          const double sdd = dot3(sf, d_on) / dl;
+          for (int i = 0; i < 3; ++i) {
+            dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+            gf[i] = gcp * gT[3 * size_t(c) + i] + (1.0 - gcp) * gT[3 * size_t(nbp) + i];
+          }
+          flux = ae * ((T[nbp] - T[c]) * ortho_weight(sdd, M.area[f]) / dl + dot3(gf, dnp));
+        } else {
+          const double ae = alpha + nus[c] / prt;
+          double d_on[3], dnp[3];
+          for (int i = 0; i < 3; ++i) d_on[i] = M.fctr[3 * size_t(f) + i] - M.ctr[3 * size_t(c) + i];
+          const double dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+          const double sdd = dot3(sf, d_on) / dl;
+          for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+          flux = ae * ((Tb[b] - T[c]) * ortho_weight(sdd, M.area[f]) / dl + dot3(&gT[3 * size_t(c)], dnp));
+        }
+        vs = vs + flux * iv;
+      }
+    }
+    const double h = cv + vs;
+    HT[c] = h;
+    Tnew[c] = T[c] + dt * (a * h + bcoef * HT0[c]);
+  }
+};
+
 // ------------------------------------------------------------------ explicit right-hand side
 
 // advection_rhs_vector_{kep,cds}_soa (src/numerics/mod_advection.f90:1013-1392) + viscous_rhs_vector
@@ -763,6 +1015,8 @@ struct ExplicitRhs {
   double nu, f0, f1, f2, dt, a, bcoef;
   int has_wm;
   const double* phiF;  // reconstruction schemes: face values from ReconFace
+  const double* T;     // Boussinesq buoyancy -bb (T - T_ref) g (bb = 0: off)
+  double bb, Tref, g0, g1, g2;
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -905,6 +1159,12 @@ struct ExplicitRhs {
     if (SCHEME == 4)
       for (int q = 0; q < 3; ++q) cv[q] = cv[q] + 0.5 * Uc[q] * dw * iv;
     double h[3] = {cv[0] + vs[0] + f0, cv[1] + vs[1] + f1, cv[2] + vs[2] + f2};
+    if (T) {
+      const double dT = T[c] - Tref;
+      h[0] = h[0] - bb * dT * g0;
+      h[1] = h[1] - bb * dT * g1;
+      h[2] = h[2] - bb * dT * g2;
+    }
     if (has_wm) {
       for (int k = 0; k < cnt; ++k) {
         const size_t s = size_t(k) * nc + c;
@@ -1060,6 +1320,8 @@ struct CorrectVelocity {
   double *U, *gpc_out;
   double dt, fb0, fb1, fb2;
   int do_cross, ptot;
+  const double* T;  // Boussinesq: the wall term's acceleration carries -bb (T_o - T_ref) g
+  double bb, Tref, g0, g1, g2;
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     double g3[3] = {0, 0, 0};
@@ -1094,7 +1356,9 @@ struct CorrectVelocity {
         if (M.bbc[b] == BC_OUTLET) continue;
         const int f = M.nint + b;
         const double* n = &M.nrm[3 * size_t(f)];
-        const double d1 = fb0 - 0.0, d2 = fb1 - 0.0, d3 = fb2 - 0.0;
+        const double tb = T ? T[c] : 0.0;
+        const double d1 = fb0 - bb * (tb - Tref) * g0, d2 = fb1 - bb * (tb - Tref) * g1,
+                     d3 = fb2 - bb * (tb - Tref) * g2;
         const double fn = (d1 * n[0] + d2 * n[1] + d3 * n[2]) * M.area[f];
         for (int i = 0; i < 3; ++i) g3[i] = g3[i] + fn * n[i];
       }

@@ -37,6 +37,11 @@ FieldRef field(Solver* s, const std::string& name) {
   if (name == "rhs_p") return {s->rhs, nc};
   if (name == "H") return {s->H0, 3 * nc};  // after a step the last stage's H sits in H0
   if (name == "w_gp") return {s->gp, 3 * nc};
+  if (s->bous) {
+    if (name == "T") return {s->T, nc};
+    if (name == "T_bnd") return {s->T_bnd, nb};
+    if (name == "grad_T") return {s->gT, 3 * nc};
+  }
   throw std::runtime_error("unknown field " + name);
 }
 }  // namespace
@@ -61,6 +66,8 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
       throw std::runtime_error("advection scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3), KEP (4) or QUICK (5)");
     if (p->precision != 0 && p->precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
     if (p->cg_variant != 0 && p->cg_variant != 1) throw std::runtime_error("cg_variant must be 0 or 1");
+    if (p->boussinesq && (p->t_scheme < 1 || p->t_scheme > 5 || p->t_scheme == 4))
+      throw std::runtime_error("temperature scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3) or QUICK (5)");
     s = new Solver();
     s->x.init(std::max(m->n_cells, m->n_faces + m->n_bnd_faces));
     s->create(*m, *p);
@@ -90,7 +97,7 @@ int dopamine_solver_set_field(void* h, const char* name, const double* host) {
     auto* s = static_cast<Solver*>(h);
     FieldRef f = field(s, name);
     s->x.upload(f.p, host, size_t(f.n));
-    if (std::string(name) == "U") s->apply_all_bc();
+    if (std::string(name) == "U" || std::string(name) == "T") s->apply_all_bc();
     s->x.sync();
     return 0;
   });

@@ -135,6 +135,10 @@ class SolverT {
   EllOp A;
   EllOp32 A32;
   double* phiF = nullptr;  // reconstruction schemes: 3 face values per face
+  // temperature (Boussinesq)
+  int bous = 0;
+  double *T = nullptr, *T2 = nullptr, *T_bnd = nullptr, *gT = nullptr, *gT_prev = nullptr, *HT = nullptr,
+         *HT0 = nullptr, *phiT = nullptr, *patchT = nullptr;
   int n_amdw = 0;
   int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
@@ -216,10 +220,10 @@ class SolverT {
     }
     for (int q = 0; q < npatch; ++q)
       if (d.patch_bc[q] == BC_WALL_MODEL) n_wm++;
+    // refinement-interface faces (build_interface_diagnostics, mod_fv_solver.f90:685-718): more
+    // correctors and a looser stagnation test
     for (int f = 0; f < nint; ++f)
       if (d.face_skew[f] > 0.15) diag_interface = 1;
-    if (diag_interface)
-      throw std::runtime_error("refinement-interface faces (skew > 0.15) are not supported by this build");
 
     // ---- mesh-face list (cell_faces CSR order)
     std::vector<int> nA(nc);
@@ -592,6 +596,18 @@ class SolverT {
       }
     if (p.scheme == SCHEME_QUICK || p.scheme == SCHEME_VANLEER || p.scheme == SCHEME_KOREN)
       phiF = fresh<double>(3 * size_t(nf));
+    bous = p.boussinesq != 0;
+    if (bous) {
+      T = fresh<double>(nc); T2 = fresh<double>(nc); T_bnd = fresh<double>(std::max(1, nb));
+      gT = fresh<double>(3 * size_t(nc)); gT_prev = fresh<double>(3 * size_t(nc));
+      HT = fresh<double>(nc); HT0 = fresh<double>(nc);
+      if (p.t_scheme != SCHEME_CDS) phiT = fresh<double>(nf);
+      std::vector<double> tref_c(nc, p.T_ref), tref_b(std::max(1, nb), p.T_ref), pt(std::max(1, npatch), p.T_ref);
+      for (int q = 0; q < npatch; ++q) pt[q] = p.patch_T_val ? p.patch_T_val[q] : p.T_ref;
+      x.upload(T, tref_c.data(), tref_c.size());
+      x.upload(T_bnd, tref_b.data(), tref_b.size());
+      patchT = up(pt);
+    }
     cg.init(&x, nc);
     cg.set_single_reduction(p.cg_variant == 1);
     if (p.precision == 1) cg.set_mixed(A32, mixed_tau());
@@ -695,15 +711,20 @@ class SolverT {
   void bc_pressure(int filter, const double* g, double scale, const double* pob) {
     x.for_each(nb, BcPressure{M, P, nullptr, g, pob, patchP, p_bnd, scale, filter});
   }
+  void bc_temperature(int filter, const double* g) {
+    if (bous) x.for_each(nb, BcTemperature{M, T, g, patchT, T_bnd, filter});
+  }
   void apply_all_bc() {
     bc_velocity(0, grad);
     bc_pressure(0, skip_grad_p ? nullptr : gp, 1.0, outlet_pb);
+    bc_temperature(0, gT);
   }
   // refresh_halos_and_bc (single rank: non-periodic pass, then periodic pass)
   void refresh_bc() {
     for (int pass = 1; pass <= 2; ++pass) {
       bc_velocity(pass, grad);
       bc_pressure(pass, skip_grad_p ? nullptr : gp, 1.0, outlet_pb);
+      bc_temperature(pass, gT);
     }
   }
   void velocity_bc_both() {
@@ -717,11 +738,37 @@ class SolverT {
   template <int SCHEME>
   void explicit_rhs(double dt, int k) {
     x.for_each(nc, ExplicitRhs<SCHEME>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
-                                       prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0, phiF});
+                                       prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0, phiF,
+                                       bous ? T : nullptr, prm.beta, prm.T_ref, prm.g[0], prm.g[1], prm.g[2]});
+  }
+  // temperature face values (reconstruction schemes), then H_T and the T predictor into T2
+  void explicit_rhs_T(double dt, int k) {
+    const ReconFace<SCHEME_QUICK> rq{M, U, U_bnd, UF, grad, nullptr};
+    const ReconFace<SCHEME_VANLEER> rv{M, U, U_bnd, UF, grad, nullptr};
+    const ReconFace<SCHEME_KOREN> rk{M, U, U_bnd, UF, grad, nullptr};
+    switch (prm.t_scheme) {
+      case SCHEME_QUICK: x.for_each(nf, ReconFaceScalar<SCHEME_QUICK>{rq, T, T_bnd, gT, phiT}); break;
+      case SCHEME_VANLEER: x.for_each(nf, ReconFaceScalar<SCHEME_VANLEER>{rv, T, T_bnd, gT, phiT}); break;
+      case SCHEME_KOREN: x.for_each(nf, ReconFaceScalar<SCHEME_KOREN>{rk, T, T_bnd, gT, phiT}); break;
+      default: break;
+    }
+    x.for_each(nc, ExplicitRhsT{M, T, T_bnd, gT, UF, nu_sgs, HT0, phiT, HT, T2, prm.alpha_T, prm.Pr_t, dt, RK3_A[k],
+                                RK3_B[k], prm.t_scheme == SCHEME_CDS ? 1 : 0});
   }
 
   void gradients(bool with_sgs, double sgs_dt) {
     const int sgs = with_sgs ? prm.sgs : -1;
+    if (bous) {  // compute_all_gradients: the temperature gradient alongside the velocity's
+      if (prm.gradient == 0) {
+        x.for_each(nc, LsGradSca{M, T, nullptr, T_bnd, nullptr, gT});
+      } else {
+        x.for_each(nc, GgGradSca{M, T, nullptr, T_bnd, nullptr, nullptr, gT});
+        for (int it = 2; it <= prm.gg_iterations; ++it) {
+          x.copy(gT_prev, gT, 3 * size_t(nc));
+          x.for_each(nc, GgGradSca{M, T, nullptr, T_bnd, nullptr, gT_prev, gT});
+        }
+      }
+    }
     if (prm.gradient == 0) {
       x.tbegin(K_LS_GRAD);
       x.for_each(nc, LsGradVec{M, U, U_bnd, grad, nu_sgs, sgs, prm.c_vreman, prm.cs_smag, prm.c_amd, sgs_dt});
@@ -778,7 +825,8 @@ class SolverT {
   void correct_velocity(double dt, bool with_cross, bool ptot, double* gpc_out) {
     const int do_cross = (with_cross && !skip_nonortho) ? 1 : 0;
     x.for_each(nc, CorrectVelocity{M, P, gp, outlet_pb, U, gpc_out, dt, prm.f_const[0], prm.f_const[1], prm.f_const[2],
-                                   do_cross, ptot ? 1 : 0});
+                                   do_cross, ptot ? 1 : 0, bous ? T : nullptr, prm.beta, prm.T_ref, prm.g[0],
+                                   prm.g[1], prm.g[2]});
   }
 
   int pressure_solve(bool ptot, const double* atol_dev, double atol) {
@@ -882,6 +930,10 @@ class SolverT {
   int advance_rk3w(double dt) {
     x.zero(H, 3 * size_t(nc));
     x.zero(H0, 3 * size_t(nc));
+    if (bous) {
+      x.zero(HT, nc);
+      x.zero(HT0, nc);
+    }
     x.zero(rhs, nc);
     const double time_n = time;
     int rc = 0;
@@ -907,7 +959,9 @@ class SolverT {
         default: explicit_rhs<SCHEME_KOREN>(dt, k); break;
       }
       x.tend(K_EXPLICIT);
+      if (bous) explicit_rhs_T(dt, k);
       std::swap(U, U2);
+      if (bous) std::swap(T, T2);
       velocity_bc_both();
       x.copy(p_old, P, nc);
       correct_velocity(gdt, false, true, gp_rc);
@@ -917,8 +971,8 @@ class SolverT {
       x.copy(phi_prev_bnd, p_bnd, std::max(1, nb));
       {
         atol_for(gdt);
-        int nmax = std::min(n_corr, 3);
-        const double stag = 0.9;
+        int nmax = std::min(n_corr, diag_interface ? 8 : 3);
+        const double stag = diag_interface ? 0.99 : 0.9;
         double dprev = 1e308, dphi_prev = 1e308;
         if (prm.nonortho_check) {
           nmax = std::min(n_corr, 8);
@@ -958,6 +1012,7 @@ class SolverT {
       bc_pressure(2, nullptr, 1.0, nullptr);
       pressure_gradient();
       std::swap(H, H0);
+      if (bous) std::swap(HT, HT0);
     }
     time = time_n;
     return rc;

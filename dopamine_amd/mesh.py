@@ -228,3 +228,93 @@ def channel_box(nx, ny, nz, lx=2 * math.pi, ly=2.0, lz=math.pi, stretch=2.0):
         return I * (lx / nx), yn[J], K * (lz / nz)
     return Mesh(box_elements(nx, ny, nz, xyz, ("x0", "xL", "bottom_wall", "top_wall", "z0", "zL"),
                              ("periodic", "periodic", "wallModel", "wallModel", "periodic", "periodic")))
+
+
+def rb_box(nx, ny, nz, lx=4.0, ly=4.0, lz=1.0, stretch=1.5):
+    """Rayleigh-Benard cell: periodic in x and y, wall-clustered tanh spacing in z
+    (z_k = lz/2 (1 + tanh(s(2k/nz-1))/tanh(s))), fixed-temperature walls "bottom_wall" (z = 0) and
+    "top_wall" (z = lz)."""
+    zn = 0.5 * lz * (1.0 + np.tanh(stretch * (2.0 * np.arange(nz + 1) / nz - 1.0)) / math.tanh(stretch))
+    zn[0], zn[-1] = 0.0, lz
+
+    def xyz(I, J, K):
+        return I * (lx / nx), J * (ly / ny), zn[K]
+    return Mesh(box_elements(nx, ny, nz, xyz, ("x0", "xL", "y0", "yL", "bottom_wall", "top_wall"),
+                             ("periodic", "periodic", "periodic", "periodic", "fixedTemperature",
+                              "fixedTemperature")))
+
+
+GMSH_TRI3, GMSH_TET4 = 2, 4
+
+# Kuhn (Freudenthal) split of a hex into 6 tets sharing the diagonal from corner (0,0,0) to
+# (1,1,1): one tet per axis order. Corner offsets (di, dj, dk) along each path.
+_KUHN_PATHS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def tet_box(n, lx=1.0, ly=1.0, lz=1.0, jitter=0.15, seed=3200, shuffle_seed=None,
+            group_types=("noSlip",) * 6, group_names=("x0", "xL", "y0", "yL", "z0", "zL")):
+    """n^3 hexes split into 6 tets each (Kuhn split, conforming across hexes), interior nodes
+    jittered by a seeded uniform +-jitter of the local spacing per coordinate, tets optionally put
+    in a seeded random order (cell ids shuffled). Boundary quads split along the same diagonals."""
+    nx = ny = nz = n
+    hx, hy, hz = lx / nx, ly / ny, lz / nz
+
+    def nid(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    I, J, K = (a.transpose(2, 1, 0).reshape(-1) for a in (I, J, K))
+    X, Y, Z = I * hx, J * hy, K * hz
+    if jitter > 0.0:
+        rng = np.random.default_rng(seed)
+        r = rng.uniform(-jitter, jitter, size=(len(X), 3))
+        inner = (I > 0) & (I < nx) & (J > 0) & (J < ny) & (K > 0) & (K < nz)
+        X = np.where(inner, X + r[:, 0] * hx, X)
+        Y = np.where(inner, Y + r[:, 1] * hy, Y)
+        Z = np.where(inner, Z + r[:, 2] * hz, Z)
+    coords = np.stack([X, Y, Z], axis=1).reshape(-1).astype(np.float64)
+
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.reshape(-1), j.reshape(-1), k.reshape(-1)
+    tets = []
+    for path in _KUHN_PATHS:
+        d = np.zeros((3, len(i)), dtype=np.int64)
+        verts = [nid(i, j, k)]
+        for ax in path:
+            d[ax] += 1
+            verts.append(nid(i + d[0], j + d[1], k + d[2]))
+        tets.append(np.stack(verts, axis=1))
+    tets = np.stack(tets, axis=1).reshape(-1, 4).astype(np.int32)  # 6 consecutive tets per hex
+    if shuffle_seed is not None:
+        tets = tets[np.random.default_rng(shuffle_seed).permutation(len(tets))]
+
+    def surf(n1, n2, fn):
+        a, b = np.meshgrid(np.arange(n1), np.arange(n2), indexing="ij")
+        a, b = a.reshape(-1), b.reshape(-1)
+        c00, c10, c11, c01 = fn(a, b)
+        # split along the (0,0)-(1,1) diagonal, as the Kuhn tets do on every face
+        return np.concatenate([np.stack([c00, c10, c11], axis=1), np.stack([c00, c01, c11], axis=1)])
+
+    tris = [
+        surf(ny, nz, lambda a, b: (nid(0, a, b), nid(0, a + 1, b), nid(0, a + 1, b + 1), nid(0, a, b + 1))),
+        surf(ny, nz, lambda a, b: (nid(nx, a, b), nid(nx, a + 1, b), nid(nx, a + 1, b + 1), nid(nx, a, b + 1))),
+        surf(nx, nz, lambda a, b: (nid(a, 0, b), nid(a + 1, 0, b), nid(a + 1, 0, b + 1), nid(a, 0, b + 1))),
+        surf(nx, nz, lambda a, b: (nid(a, ny, b), nid(a + 1, ny, b), nid(a + 1, ny, b + 1), nid(a, ny, b + 1))),
+        surf(nx, ny, lambda a, b: (nid(a, b, 0), nid(a + 1, b, 0), nid(a + 1, b + 1, 0), nid(a, b + 1, 0))),
+        surf(nx, ny, lambda a, b: (nid(a, b, nz), nid(a + 1, b, nz), nid(a + 1, b + 1, nz), nid(a, b + 1, nz))),
+    ]
+    bnd = np.concatenate(tris).astype(np.int32)
+    bnd_group = np.concatenate([np.full(len(t), g, dtype=np.int32) for g, t in enumerate(tris)])
+    nv, nt = len(tets), len(bnd)
+    return Mesh({
+        "coords": coords,
+        "vol_type": np.full(nv, GMSH_TET4, dtype=np.int32),
+        "vol_offset": np.arange(nv + 1, dtype=np.int64) * 4,
+        "vol_nodes": tets.reshape(-1),
+        "bnd_type": np.full(nt, GMSH_TRI3, dtype=np.int32),
+        "bnd_offset": np.arange(nt + 1, dtype=np.int64) * 3,
+        "bnd_nodes": bnd.reshape(-1),
+        "bnd_group": bnd_group,
+        "group_bc": np.array([BC_NAMES[t.lower()] for t in group_types], dtype=np.int32),
+        "group_names": list(group_names),
+    })

@@ -45,6 +45,15 @@ class SolverParams:
     adapt_dt: bool = False
     # pressure CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction
     cg_variant: int = 0
+    # temperature / Boussinesq (src/solver/mod_fv_solver.f90:41-110 defaults)
+    boussinesq: bool = False
+    alpha_T: float = 1.4e-5
+    beta: float = 3.4e-3
+    T_ref: float = 300.0
+    g: tuple = (0.0, 0.0, 0.0)
+    Pr_t: float = 0.85
+    temperature_scheme: str = "VANLEER_TVD"
+    patch_T: dict = field(default_factory=dict)
     # per-patch values, by patch name: p_val (outlet), U_val (inlet / moving wall)
     patch_p: dict = field(default_factory=dict)
     patch_U: dict = field(default_factory=dict)
@@ -60,6 +69,9 @@ class SolverParams:
             raise ValueError("gradient_scheme must be LEAST_SQUARES or GREEN_GAUSS")
         if self.use_sgs and self.sgs_model.upper() not in ("VREMAN", "SMAGORINSKY", "AMD"):
             raise ValueError(f"sgs_model {self.sgs_model!r} not supported (VREMAN, SMAGORINSKY, AMD)")
+        if self.boussinesq and self.temperature_scheme.upper() not in ("CDS", "CENTRAL", "QUICK", "VANLEER_TVD",
+                                                                         "VANLEER", "KOREN_TVD", "KOREN"):
+            raise ValueError(f"temperature_scheme {self.temperature_scheme!r} not supported")
         if self.adapt_dt:
             raise ValueError("adapt_dt is not supported; the fixed-dt CFL clamp of solver_advance applies")
 
@@ -69,14 +81,20 @@ class SolverParams:
     def sgs_id(self):
         return SGS_IDS[self.sgs_model.upper()] if self.use_sgs else 0
 
-    def patch_arrays(self, mesh):
-        """Per-patch outlet pressure and inlet velocity arrays in the mesh's patch order."""
+    def t_scheme_id(self):
+        return SCHEME_IDS[self.temperature_scheme.upper()]
+
+    def patch_arrays(self, mesh, with_T=False):
+        """Per-patch outlet pressure, inlet velocity (and fixed temperature) arrays in the mesh's
+        patch order."""
         pv = np.zeros(mesh.n_patches)
         uv = np.zeros((mesh.n_patches, 3))
+        tv = np.full(mesh.n_patches, self.T_ref)
         for i, name in enumerate(mesh.patch_names[:mesh.n_patches]):
             pv[i] = self.patch_p.get(name, 0.0)
             uv[i] = self.patch_U.get(name, (0.0, 0.0, 0.0))
-        return pv, uv
+            tv[i] = self.patch_T.get(name, self.T_ref)
+        return (pv, uv, tv) if with_T else (pv, uv)
 
 
 def taylor_green_ic(mesh, uref=1.0):
@@ -129,6 +147,13 @@ def initialize(solver, case, mesh):
     """apply_initial_conditions (src/init/mod_initconditions.f90:39-105): set U, then one projection
     with dt = gamma_1 dt (two when noise was added)."""
     U, twice = initial_velocity(case, mesh)
+    if case.params.boussinesq:
+        # temperature: T_ref plus a seeded Gaussian perturbation of amplitude t_noise
+        T = np.full(mesh.n_cells, case.params.T_ref)
+        tn = case.extra.get("t_noise", 0.0)
+        if tn > 0.0:
+            T = T + tn * np.random.default_rng(case.extra.get("seed", 1700)).standard_normal(mesh.n_cells)
+        solver.set_temperature(T)
     solver.set_velocity(U)
     dtp = RK3W_GAMMA1 * case.params.dt
     if twice:

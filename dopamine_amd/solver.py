@@ -35,7 +35,9 @@ class Params(ctypes.Structure):
                 ("n_nonortho_corr", _i32), ("min_cos_theta", _d), ("rhie_chow_dual", _i32),
                 ("nonortho_check", _i32), ("rotational", _i32), ("p_tol", _d), ("p_max_iter", _i32),
                 ("cfl_max", _d), ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)),
-                ("precision", _i32), ("cg_variant", _i32), ("c_amd", _d), ("amd_wall_gradient", _i32)]
+                ("precision", _i32), ("cg_variant", _i32), ("c_amd", _d), ("amd_wall_gradient", _i32),
+                ("boussinesq", _i32), ("alpha_T", _d), ("beta", _d), ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d),
+                ("t_scheme", _i32), ("patch_T_val", _P(_d))]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -53,14 +55,16 @@ def build_abi(mesh, params, precision=0):
     m = mesh
     md = MeshDesc(m.n_cells, m.n_faces, m.n_int_faces, m.n_bnd_faces, m.n_patches,
                   *[arr(getattr(m, n), t) for n, t in MESH_FIELDS])
-    pv, uv = params.patch_arrays(mesh)
+    pv, uv, tv = params.patch_arrays(mesh, with_T=True)
     pr = Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
                 0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
                 params.sgs_id(), params.c_vreman, params.cs_smagorinsky, WALL_MODEL_IDS[params.wall_model],
                 params.wall_model_height, params.n_nonortho_corr, params.pressure_min_cos_theta,
                 int(params.rhie_chow_dual), int(params.nonortho_check), int(params.use_rotational_projection),
                 params.p_tol, params.p_max_iter, params.cfl_max, params.dt, arr(pv, _d), arr(uv.reshape(-1), _d),
-                int(precision), int(params.cg_variant), params.c_amd, int(params.amd_wall_gradient))
+                int(precision), int(params.cg_variant), params.c_amd, int(params.amd_wall_gradient),
+                int(params.boussinesq), params.alpha_T, params.beta, params.T_ref, (_d * 3)(*params.g), params.Pr_t,
+                params.t_scheme_id(), arr(tv, _d))
     return md, pr, keep
 
 
@@ -105,7 +109,8 @@ class FVSolver:
         self._sizes = {"U": 3 * mesh.n_cells, "p": mesh.n_cells, "nu_sgs": mesh.n_cells,
                        "U_bnd": 3 * max(1, mesh.n_bnd_faces), "p_bnd": max(1, mesh.n_bnd_faces),
                        "grad_U": 9 * mesh.n_cells, "U_face": mesh.n_faces, "tau_w_bnd": 3 * max(1, mesh.n_bnd_faces),
-                       "rhs_p": mesh.n_cells, "H": 3 * mesh.n_cells, "w_gp": 3 * mesh.n_cells}
+                       "rhs_p": mesh.n_cells, "H": 3 * mesh.n_cells, "w_gp": 3 * mesh.n_cells,
+                       "T": mesh.n_cells, "T_bnd": max(1, mesh.n_bnd_faces), "grad_T": 3 * mesh.n_cells}
 
     def _check(self, rc, what):
         if rc != 0:
@@ -141,6 +146,9 @@ class FVSolver:
 
     def set_velocity(self, U):
         self.set("U", U)
+
+    def set_temperature(self, T):
+        self.set("T", T)
 
     def project_once(self, dt):
         self._check(self.L.dopamine_solver_project_once(self.h, dt), "project_once")

@@ -62,6 +62,11 @@ typedef struct dopamine_params {
                                  1: one pass per iteration (single-reduction recurrence) */
   double c_amd;               /* AMD constant (sgs 3), reference AMD_C_DEFAULT = 1/3 */
   int32_t amd_wall_gradient;  /* AMD: log-law wall-normal gradient in wall-model cells */
+  /* temperature: solved when boussinesq != 0 (solve_temperature, mod_fv_solver.f90:503) */
+  int32_t boussinesq;
+  double alpha_T, beta, T_ref, g[3], Pr_t;
+  int32_t t_scheme;           /* temperature_scheme: 1 VANLEER_TVD, 2 KOREN_TVD, 3 CDS, 5 QUICK */
+  const double *patch_T_val;  /* n_patches: fixedTemperature / inlet temperature */
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
@@ -73,7 +78,8 @@ const char *dopamine_last_error(void);
 
 /* Copy a host field in / out. Names: "U" (3*nc, component-fastest), "p" (nc), "nu_sgs", "U_bnd",
  * "p_bnd", "grad_U" (9*nc, grad(i,j) = du_j/dx_i at [9c+3j+i]), "U_face" (nf), "tau_w_bnd",
- * "rhs_p", "H". set_field("U") re-applies the boundary conditions as solver_init does. */
+ * "rhs_p", "H", and with boussinesq "T" (nc), "T_bnd", "grad_T" (3*nc). set_field("U") and
+ * set_field("T") re-apply the boundary conditions as solver_init does. */
 int dopamine_solver_set_field(void *solver, const char *name, const double *host);
 int dopamine_solver_get_field(void *solver, const char *name, double *host, int64_t len);
 

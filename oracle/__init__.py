@@ -35,7 +35,8 @@ class _Params(ctypes.Structure):
                 ("min_cos_theta", _d), ("rhie_chow_dual", ctypes.c_int), ("nonortho_check", ctypes.c_int),
                 ("rotational", ctypes.c_int), ("p_tol", _d), ("p_max_iter", ctypes.c_int), ("cfl_max", _d),
                 ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)), ("c_amd", _d),
-                ("amd_wall_gradient", ctypes.c_int)]
+                ("amd_wall_gradient", ctypes.c_int), ("boussinesq", ctypes.c_int), ("alpha_T", _d), ("beta", _d),
+                ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d), ("t_scheme", ctypes.c_int), ("patch_T_val", _P(_d))]
 
 
 _lib = None
@@ -104,7 +105,7 @@ class Solver:
                         arr(m.patch_nfaces, _i32), arr(m.periodic_pair, _i32), arr(m.periodic_pair_dcf, _d),
                         arr(m.periodic_gc, _d), arr(m.ls_offset, _i32), arr(m.ls_cell, _i32), arr(m.ls_face, _i32),
                         arr(m.ls_weight, _d))
-        pv, uv = params.patch_arrays(mesh)
+        pv, uv, tv = params.patch_arrays(mesh, with_T=True)
         from dopamine_amd.params import WALL_MODEL_IDS
         self._pr = _Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
                            0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
@@ -113,7 +114,8 @@ class Solver:
                            params.pressure_min_cos_theta, int(params.rhie_chow_dual), int(params.nonortho_check),
                            int(params.use_rotational_projection), params.p_tol, params.p_max_iter, params.cfl_max,
                            params.dt, arr(pv, _d), arr(uv.reshape(-1), _d), params.c_amd,
-                           int(params.amd_wall_gradient))
+                           int(params.amd_wall_gradient), int(params.boussinesq), params.alpha_T, params.beta,
+                           params.T_ref, (_d * 3)(*params.g), params.Pr_t, params.t_scheme_id(), arr(tv, _d))
         self.params = params
         self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))
 
@@ -143,6 +145,10 @@ class Solver:
     @property
     def p(self):
         return self.array("p")
+
+    def set_temperature(self, T):
+        self.array("T")[:] = np.asarray(T, dtype=np.float64).reshape(-1)
+        lib().om_apply_all_bc(self.h)
 
     def set_velocity(self, U):
         self.array("U")[:] = np.asarray(U, dtype=np.float64).reshape(-1)

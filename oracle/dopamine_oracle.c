@@ -58,6 +58,8 @@ struct om_solver {
   int64_t n_solves, n_iters, last_iters;
   double *cg_r, *cg_z, *cg_p, *cg_q, *cg_b;
   double *dt_flux, *dt_diff;
+  /* temperature */
+  double *T, *T_bnd, *grad_T, *gT_prev, *H_T, *H_T0, *conv_T, *visc_T;
 };
 
 static double *dz(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
@@ -222,6 +224,37 @@ static void apply_pressure_bc(om_solver *s, const double *p, double *pb, int fil
   }
 }
 
+/* apply_temperature_bc (src/bc/mod_bc.f90:489-580). filter: 0 all, 1 non-periodic, 2 periodic. */
+static void apply_temperature_bc(om_solver *s, int filter, const double *gT) {
+  if (!s->p.boussinesq) return;
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b;
+    const int bc = bc_of_face(s, f);
+    if (filter == 2 && bc != BC_PERIODIC) continue;
+    if (filter == 1 && bc == BC_PERIODIC) continue;
+    const int o = s->m.face_owner[f], pid = s->m.face_patch[f];
+    switch (bc) {
+      case BC_INLET:
+      case BC_FIXEDTEMP:
+        s->T_bnd[b] = s->p.patch_T_val[pid];
+        break;
+      case BC_PERIODIC: {
+        const int nb = s->per_nb[b];
+        s->T_bnd[b] = (nb >= 0) ? s->m.periodic_gc[b] * s->T[o] + (1.0 - s->m.periodic_gc[b]) * s->T[nb] : s->T[o];
+        break;
+      }
+      default:
+        if (gT) {
+          double d[3];
+          for (int i = 0; i < 3; ++i) d[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+          s->T_bnd[b] = s->T[o] + gT[3 * o] * d[0] + gT[3 * o + 1] * d[1] + gT[3 * o + 2] * d[2];
+        } else {
+          s->T_bnd[b] = s->T[o];
+        }
+    }
+  }
+}
+
 /* ---------------------------------------------------------------- gradients */
 
 /* ls_gradient_vector (src/numerics/mod_gradient.f90:425-556, CPU branch) */
@@ -345,14 +378,21 @@ static void gg_gradient_scalar(const om_solver *s, const double *phi, const doub
 
 /* compute_all_gradients (src/solver/mod_fv_solver.f90:1003-1077), velocity only */
 static void compute_all_gradients(om_solver *s) {
+  const int temp = s->p.boussinesq;
   if (s->p.gradient == OM_GRAD_GG) {
     gg_gradient_vector(s, s->U, s->U_bnd, NULL, s->grad_U);
+    if (temp) gg_gradient_scalar(s, s->T, s->T_bnd, NULL, s->grad_T);
     for (int it = 2; it <= s->p.gg_iterations; ++it) {
       memcpy(s->gU_prev, s->grad_U, sizeof(double) * 9 * (size_t)s->nc);
       gg_gradient_vector(s, s->U, s->U_bnd, s->gU_prev, s->grad_U);
+      if (temp) {
+        memcpy(s->gT_prev, s->grad_T, sizeof(double) * 3 * (size_t)s->nc);
+        gg_gradient_scalar(s, s->T, s->T_bnd, s->gT_prev, s->grad_T);
+      }
     }
   } else {
     ls_gradient_vector(s, s->U, s->U_bnd, s->grad_U);
+    if (temp) ls_gradient_scalar(s, s->T, s->T_bnd, s->grad_T);
   }
 }
 
@@ -380,6 +420,7 @@ static void refresh_bc(om_solver *s) {
   for (int pass = 1; pass <= 2; ++pass) {
     apply_velocity_bc(s, pass, s->grad_U);
     apply_pressure_bc(s, s->P, s->p_bnd, pass, s->skip_grad_p ? NULL : s->gp, 1.0, s->outlet_pb);
+    apply_temperature_bc(s, pass, s->grad_T);
   }
 }
 
@@ -1104,7 +1145,216 @@ static void viscous_vector(om_solver *s, double *rhs) {
   }
 }
 
-/* compute_explicit_rhs (src/solver/mod_fv_solver.f90:1617-1762), momentum only */
+/* ---- temperature transport */
+
+/* face flux of the scalar paths (advection_rhs_scalar_{cds,recon}_soa, mod_advection.f90:363-383) */
+static double scalar_face_flux(const om_solver *s, int f) {
+  if (s->m.face_neighbour[f] >= 0) return s->U_face[f];
+  const int bc = bc_of_face(s, f);
+  if (bc == BC_WALL_NOSLIP || bc == BC_WALL_MODEL || bc == BC_WALL_VELOCITY || bc == BC_SLIP || bc == BC_SYMMETRY)
+    return 0.0;
+  return s->U_face[f];
+}
+
+/* advection_rhs_scalar_cds_soa (src/numerics/mod_advection.f90:267-485) */
+static void advection_scalar_cds(om_solver *s, const double *phi, const double *pb, const double *gphi, double *rhs) {
+  memset(rhs, 0, sizeof(double) * s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const int b = f - s->nint;
+    double Fn = scalar_face_flux(s, f);
+    int nbs = nb;
+    if (nb < 0) nbs = (bc_of_face(s, f) == BC_PERIODIC) ? s->per_nb[b] : -1;
+    double phif;
+    if (nbs >= 0) {
+      const double gc = (f >= s->nint) ? s->m.periodic_gc[b] : s->m.face_gc[f];
+      const double pl = gc * phi[o] + (1.0 - gc) * phi[nbs];
+      if (f < s->nint) {
+        double gm[3];
+        for (int i = 0; i < 3; ++i) gm[i] = gc * gphi[3 * o + i] + (1.0 - gc) * gphi[3 * nbs + i];
+        phif = pl + dot3(gm, &s->m.face_eps[3 * f]);
+        const double w = refine_blend_weight(s->m.face_skew[f]);
+        if (w < 1.0) phif = pl + w * (phif - pl);
+      } else {
+        phif = pl;
+      }
+      if (!isfinite(phif)) phif = (Fn >= 0.0 || nb < 0) ? phi[o] : phi[nb];
+    } else {
+      phif = pb[b];
+    }
+    if (!isfinite(Fn)) Fn = 0.0;
+    if (!isfinite(phif)) phif = 0.0;
+    rhs[o] = rhs[o] - Fn * phif * s->inv_vol[o];
+    if (nb >= 0) rhs[nb] = rhs[nb] + Fn * phif * s->inv_vol[nb];
+  }
+}
+
+/* soa_quick_reconstruction_ctx_dev (src/numerics/mod_advection.f90:2552-2629), one scalar */
+static double quick_scalar(const om_solver *s, const recon_ctx *c, const double *phi, const double *gphi) {
+  const double pU = phi[c->up], pD = phi[c->down];
+  const double *gup = &gphi[3 * (size_t)c->up];
+  double a11 = 0.0, a12 = 0.0, a22 = 0.0, b1 = 0.0, b2 = 0.0, ln = 0.0, ld = 0.0;
+  int nfit = 0;
+  double dvec[3];
+  for (int i = 0; i < 3; ++i) dvec[i] = c->x_down[i] - c->x_up[i];
+  double dist2 = fmax(dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2], SMALL);
+  double w = 1.0 / dist2;
+  double dl = axial_delta(pD, pU, gup, dvec, c->s_down, c->dir_hat);
+  const double sd = c->s_down;
+  a11 = a11 + w * sd * sd;
+  a12 = a12 + w * sd * sd * sd;
+  a22 = a22 + w * sd * sd * sd * sd;
+  b1 = b1 + w * sd * dl;
+  b2 = b2 + w * sd * sd * dl;
+  ln = ln + w * sd * dl;
+  ld = ld + w * sd * sd;
+  nfit = nfit + 1;
+  for (int q = s->m.cell_face_offset[c->up]; q < s->m.cell_face_offset[c->up + 1]; ++q) {
+    const int fj = s->m.cell_faces[q];
+    const int nb = across(s, fj, c->up);
+    if (nb < 0 || nb == c->down) continue;
+    for (int i = 0; i < 3; ++i) dvec[i] = (s->m.cell_centroid[3 * nb + i] + c->up_shift[i]) - c->x_up[i];
+    const double sv = dot3(dvec, c->dir_hat);
+    if (sv >= -1.0e-12) continue;
+    dist2 = fmax(dvec[0] * dvec[0] + dvec[1] * dvec[1] + dvec[2] * dvec[2], SMALL);
+    if (sv * sv < 0.25 * dist2) continue;
+    const double align = fmin(1.0, -sv / sqrt(dist2));
+    w = (0.25 + align * align) / dist2;
+    dl = axial_delta(phi[nb], pU, gup, dvec, sv, c->dir_hat);
+    a11 = a11 + w * sv * sv;
+    a12 = a12 + w * sv * sv * sv;
+    a22 = a22 + w * sv * sv * sv * sv;
+    b1 = b1 + w * sv * dl;
+    b2 = b2 + w * sv * sv * dl;
+    ln = ln + w * sv * dl;
+    ld = ld + w * sv * sv;
+    nfit = nfit + 1;
+  }
+  const double det = a11 * a22 - a12 * a12;
+  double a1, a2;
+  if (nfit >= 2 && fabs(det) > SMALL) {
+    a1 = (b1 * a22 - b2 * a12) / det;
+    a2 = (-b1 * a12 + b2 * a11) / det;
+  } else if (ld > SMALL) {
+    a1 = ln / ld;
+    a2 = 0.0;
+  } else {
+    a1 = (pD - pU) / fmax(c->s_down, SMALL);
+    a2 = 0.0;
+  }
+  return clamp_value(pU + a1 * c->s_face + a2 * c->s_face * c->s_face, fmin(pU, pD), fmax(pU, pD));
+}
+
+/* TVD face value of one scalar field (soa_*_tvd_ctx_dev) */
+static double tvd_scalar(const recon_ctx *c, const double *phi, const double *ctr, int far_up, int koren) {
+  const double pU = phi[c->up], pD = phi[c->down];
+  const double slope_dn = (pD - pU) / fmax(c->s_down, SMALL);
+  double psi = 0.0;
+  if (far_up >= 0 && fabs(slope_dn) > SMALL) {
+    double t[3];
+    for (int i = 0; i < 3; ++i) t[i] = ctr[3 * far_up + i] + c->up_shift[i] - c->x_up[i];
+    const double s_up = -dot3(t, c->dir_hat);
+    if (s_up > SMALL) {
+      const double r = ((pU - phi[far_up]) / s_up) / slope_dn;
+      if (koren) {
+        psi = fmax(0.0, fmin(fmin(2.0 * r, (1.0 + 2.0 * r) / 3.0), 2.0));
+      } else {
+        psi = (r + fabs(r)) / (1.0 + fabs(r));
+        psi = fmax(0.0, fmin(psi, 2.0));
+      }
+    }
+  }
+  return clamp_value(pU + psi * slope_dn * c->s_face, fmin(pU, pD), fmax(pU, pD));
+}
+
+/* advection_rhs_scalar_recon_soa (src/numerics/mod_advection.f90:488-675) */
+static void advection_scalar_recon(om_solver *s, int scheme, const double *phi, const double *pb, const double *gphi,
+                                   double *rhs) {
+  memset(rhs, 0, sizeof(double) * s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const int b = f - s->nint;
+    double Fn = scalar_face_flux(s, f);
+    recon_ctx c;
+    const int ok = recon_context(s, f, Fn, &c);
+    double phif;
+    if (ok) {
+      if (scheme == OM_SCHEME_QUICK)
+        phif = quick_scalar(s, &c, phi, gphi);
+      else
+        phif = tvd_scalar(&c, phi, s->m.cell_centroid, find_far_upwind(s, &c), scheme == OM_SCHEME_KOREN);
+      if (!isfinite(phif)) phif = (Fn >= 0.0 || nb < 0) ? phi[o] : phi[nb];
+    } else {
+      phif = pb[b];
+    }
+    if (!isfinite(Fn)) Fn = 0.0;
+    if (!isfinite(phif)) phif = 0.0;
+    rhs[o] = rhs[o] - Fn * phif * s->inv_vol[o];
+    if (nb >= 0) rhs[nb] = rhs[nb] + Fn * phif * s->inv_vol[nb];
+  }
+}
+
+/* viscous_rhs_scalar (src/numerics/mod_viscous.f90:286-513) */
+static void viscous_scalar(om_solver *s, const double *phi, const double *pb, const double *gphi, double alpha,
+                           double *rhs) {
+  const double prt = s->p.Pr_t;
+  memset(rhs, 0, sizeof(double) * s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    const double *sf = &s->m.face_sf[3 * f];
+    double flux, d_on[3], dnp[3], gf[3];
+    if (nb >= 0) {
+      const double gc = s->m.face_gc[f];
+      const double ae = alpha + (gc * s->nu_sgs[o] + (1.0 - gc) * s->nu_sgs[nb]) / prt;
+      for (int i = 0; i < 3; ++i) d_on[i] = s->m.cell_centroid[3 * nb + i] - s->m.cell_centroid[3 * o + i];
+      const double dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+      const double sdd = dot3(sf, d_on) / dl;
+      for (int i = 0; i < 3; ++i) {
+        dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+        gf[i] = gc * gphi[3 * o + i] + (1.0 - gc) * gphi[3 * nb + i];
+      }
+      flux = ae * ((phi[nb] - phi[o]) * ortho_weight(sdd, s->m.face_area[f]) / dl + dot3(gf, dnp));
+    } else {
+      const int b = f - s->nint;
+      const int nbp = (bc_of_face(s, f) == BC_PERIODIC) ? s->per_nb[b] : -1;
+      if (nbp >= 0) {
+        const int pb2 = s->m.periodic_pair[b];
+        double d_per, gcp;
+        if (pb2 >= 0) {
+          const int pf = s->nint + pb2;
+          d_per = s->m.face_dcf[f] + s->m.face_dcf[pf];
+          gcp = s->m.face_dcf[pf] / (d_per + SMALL);
+        } else if (s->m.periodic_pair_dcf[b] > SMALL) {
+          d_per = s->m.face_dcf[f] + s->m.periodic_pair_dcf[b];
+          gcp = s->m.periodic_pair_dcf[b] / (d_per + SMALL);
+        } else {
+          d_per = 2.0 * s->m.face_dcf[f];
+          gcp = 0.5;
+        }
+        const double ae = alpha + (gcp * s->nu_sgs[o] + (1.0 - gcp) * s->nu_sgs[nbp]) / prt;
+        for (int i = 0; i < 3; ++i) d_on[i] = d_per * s->m.face_normal[3 * f + i];
+        const double dl = d_per + SMALL;
+        const double sdd = dot3(sf, d_on) / dl;
+        for (int i = 0; i < 3; ++i) {
+          dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+          gf[i] = gcp * gphi[3 * o + i] + (1.0 - gcp) * gphi[3 * nbp + i];
+        }
+        flux = ae * ((phi[nbp] - phi[o]) * ortho_weight(sdd, s->m.face_area[f]) / dl + dot3(gf, dnp));
+      } else {
+        const double ae = alpha + s->nu_sgs[o] / prt;
+        for (int i = 0; i < 3; ++i) d_on[i] = s->m.face_centre[3 * f + i] - s->m.cell_centroid[3 * o + i];
+        const double dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+        const double sdd = dot3(sf, d_on) / dl;
+        for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+        flux = ae * ((pb[b] - phi[o]) * ortho_weight(sdd, s->m.face_area[f]) / dl + dot3(&gphi[3 * o], dnp));
+      }
+    }
+    rhs[o] = rhs[o] + flux * s->inv_vol[o];
+    if (nb >= 0) rhs[nb] = rhs[nb] - flux * s->inv_vol[nb];
+  }
+}
+
+/* compute_explicit_rhs (src/solver/mod_fv_solver.f90:1617-1762) */
 static void explicit_rhs(om_solver *s) {
   if (s->p.scheme == OM_SCHEME_KEP)
     advection_kep(s, s->conv);
@@ -1113,8 +1363,20 @@ static void explicit_rhs(om_solver *s) {
   else
     advection_recon(s, s->conv);
   viscous_vector(s, s->visc);
-  for (int c = 0; c < s->nc; ++c)
+  const int bous = s->p.boussinesq;
+  if (bous) {
+    if (s->p.t_scheme == OM_SCHEME_CDS)
+      advection_scalar_cds(s, s->T, s->T_bnd, s->grad_T, s->conv_T);
+    else
+      advection_scalar_recon(s, s->p.t_scheme, s->T, s->T_bnd, s->grad_T, s->conv_T);
+    viscous_scalar(s, s->T, s->T_bnd, s->grad_T, s->p.alpha_T, s->visc_T);
+  }
+  for (int c = 0; c < s->nc; ++c) {
     for (int k = 0; k < 3; ++k) s->H[3 * c + k] = s->conv[3 * c + k] + s->visc[3 * c + k] + s->p.f_const[k];
+    if (bous)
+      for (int k = 0; k < 3; ++k) s->H[3 * c + k] = s->H[3 * c + k] - s->p.beta * (s->T[c] - s->p.T_ref) * s->p.g[k];
+    if (bous) s->H_T[c] = s->conv_T[c] + s->visc_T[c];
+  }
   /* add_wall_stress_to_rhs (:2238-2275) */
   if (s->n_wm_patches > 0)
     for (int b = 0; b < s->nb; ++b) {
@@ -1318,11 +1580,14 @@ static void correct_velocity(om_solver *s, double dt, int with_cross, int ptot) 
   }
   if (!do_cross) {
     const double *fb = s->p.f_const;
+    const double bb = s->p.boussinesq ? s->p.beta : 0.0;
     for (int ib = 0; ib < s->n_nper_bf; ++ib) {
       const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
       if (bc_of_face(s, f) == BC_OUTLET) continue;
       const double *n = &s->m.face_normal[3 * f];
-      const double d1 = fb[0] - 0.0, d2 = fb[1] - 0.0, d3 = fb[2] - 0.0;
+      const double tb = s->p.boussinesq ? s->T[o] : 0.0;
+      const double d1 = fb[0] - bb * (tb - s->p.T_ref) * s->p.g[0], d2 = fb[1] - bb * (tb - s->p.T_ref) * s->p.g[1],
+                   d3 = fb[2] - bb * (tb - s->p.T_ref) * s->p.g[2];
       const double fn = (d1 * n[0] + d2 * n[1] + d3 * n[2]) * s->m.face_area[f];
       for (int i = 0; i < 3; ++i) gpc[3 * o + i] = gpc[3 * o + i] + fn * n[i];
     }
@@ -1569,6 +1834,8 @@ static int advance_rk3w(om_solver *s, double dt) {
   const int nc = s->nc;
   memset(s->H, 0, sizeof(double) * 3 * (size_t)nc);
   memset(s->H0, 0, sizeof(double) * 3 * (size_t)nc);
+  memset(s->H_T, 0, sizeof(double) * nc);
+  memset(s->H_T0, 0, sizeof(double) * nc);
   memset(s->rhs_p, 0, sizeof(double) * nc);
   const double time_n = s->time;
   int rc = 0;
@@ -1579,8 +1846,10 @@ static int advance_rk3w(om_solver *s, double dt) {
     stage_closures(s, gdt);
     explicit_rhs(s);
     const double a = RK3_A[k], bb = RK3_B[k];
-    for (int c = 0; c < nc; ++c)
+    for (int c = 0; c < nc; ++c) {
       for (int q = 0; q < 3; ++q) s->U[3 * c + q] = s->U[3 * c + q] + dt * (a * s->H[3 * c + q] + bb * s->H0[3 * c + q]);
+      if (s->p.boussinesq) s->T[c] = s->T[c] + dt * (a * s->H_T[c] + bb * s->H_T0[c]);
+    }
     velocity_bc_both(s);
     memcpy(s->p_old, s->P, sizeof(double) * nc);
     correct_velocity(s, gdt, 0, 1);
@@ -1641,6 +1910,7 @@ static int advance_rk3w(om_solver *s, double dt) {
     apply_pressure_bc(s, s->P, s->p_bnd, 2, NULL, 1.0, NULL);
     compute_pressure_gradient(s);
     memcpy(s->H0, s->H, sizeof(double) * 3 * (size_t)nc);
+    if (s->p.boussinesq) memcpy(s->H_T0, s->H_T, sizeof(double) * nc);
   }
   s->time = time_n;
   return rc;
@@ -1756,6 +2026,10 @@ om_solver *om_create(const om_mesh *m, const om_params *p) {
   s->tau_w = dz(3 * (size_t)nb); s->grad_U = dz(9 * (size_t)nc); s->gU_prev = dz(9 * (size_t)nc);
   s->face_flux = dz(nf); s->U_face = dz(nf); s->inv_vol = dz(nc);
   s->H = dz(3 * (size_t)nc); s->H0 = dz(3 * (size_t)nc); s->conv = dz(3 * (size_t)nc); s->visc = dz(3 * (size_t)nc);
+  s->T = dz(nc); s->T_bnd = dz(nb); s->grad_T = dz(3 * (size_t)nc); s->gT_prev = dz(3 * (size_t)nc);
+  s->H_T = dz(nc); s->H_T0 = dz(nc); s->conv_T = dz(nc); s->visc_T = dz(nc);
+  for (int c = 0; c < nc; ++c) s->T[c] = p->T_ref;
+  for (int b = 0; b < nb; ++b) s->T_bnd[b] = p->T_ref;
   s->rhs_p = dz(nc); s->div_w = dz(nc); s->phi_it = dz(nc); s->div_diag = dz(nc);
   s->gp = dz(3 * (size_t)nc); s->gp_rc = dz(3 * (size_t)nc); s->gp_cell = dz(3 * (size_t)nc);
   s->p_tot = dz(nc); s->p_tot_bnd = dz(nb); s->gp_prev = dz(3 * (size_t)nc);
@@ -1969,6 +2243,7 @@ om_solver *om_create(const om_mesh *m, const om_params *p) {
 void om_apply_all_bc(om_solver *s) {
   apply_velocity_bc(s, 0, s->grad_U);
   apply_pressure_bc(s, s->P, s->p_bnd, 0, s->skip_grad_p ? NULL : s->gp, 1.0, s->outlet_pb);
+  apply_temperature_bc(s, 0, s->grad_T);
 }
 
 void om_free(om_solver *s) {
@@ -1977,7 +2252,8 @@ void om_free(om_solver *s) {
                   s->grad_U, s->gU_prev, s->face_flux, s->U_face, s->inv_vol, s->H, s->H0, s->conv, s->visc,
                   s->rhs_p, s->div_w, s->phi_it, s->gp, s->gp_rc, s->gp_cell, s->p_tot, s->p_tot_bnd, s->gp_prev,
                   s->Tinv, s->outlet_pb, s->Gf, s->ws_y, s->div_diag, s->ps_coeff, s->ps_diag, s->dir_vals,
-                  s->k_int, s->k_bnd, s->cg_b, s->dt_flux, s->dt_diff};
+                  s->k_int, s->k_bnd, s->cg_b, s->dt_flux, s->dt_diff, s->T, s->T_bnd, s->grad_T, s->gT_prev,
+                  s->H_T, s->H_T0, s->conv_T, s->visc_T};
   for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) free(dd[i]);
   int *ii[] = {s->per_nb, s->per_bf, s->nper_bf, s->ws_cell, s->ps_owner, s->ps_nb, s->ps_fmesh, s->ps_skip, s->dir_rows};
   for (size_t i = 0; i < sizeof(ii) / sizeof(ii[0]); ++i) free(ii[i]);
@@ -1997,7 +2273,8 @@ double *om_array(om_solver *s, const char *name, int64_t *len) {
            {"nu_sgs", s->nu_sgs, nc}, {"grad_U", s->grad_U, 9 * nc}, {"U_face", s->U_face, nf},
            {"face_flux", s->face_flux, nf}, {"tau_w_bnd", s->tau_w, 3 * nb}, {"w_gp", s->gp, 3 * nc},
            {"rhs_p", s->rhs_p, nc}, {"H", s->H, 3 * nc}, {"Gf", s->Gf, 9 * nc}, {"Tinv", s->Tinv, 9 * nc},
-           {"ps_coeff", s->ps_coeff, s->npf}, {"ps_diag", s->ps_diag, nc}};
+           {"ps_coeff", s->ps_coeff, s->npf}, {"ps_diag", s->ps_diag, nc}, {"T", s->T, nc}, {"T_bnd", s->T_bnd, nb},
+           {"grad_T", s->grad_T, 3 * nc}};
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i)
     if (strcmp(t[i].n, name) == 0) {
       if (len) *len = t[i].l;

@@ -60,6 +60,11 @@ typedef struct om_params {
   const double *patch_U_val;  /* 3*n_patches, inlet / moving-wall velocity */
   double c_amd;               /* AMD constant, reference 1/3 */
   int amd_wall_gradient;      /* AMD: log-law wall-normal gradient in wall-model cells */
+  /* temperature (solved when boussinesq != 0) */
+  int boussinesq;
+  double alpha_T, beta, T_ref, g[3], Pr_t;
+  int t_scheme;               /* OM_SCHEME_* (CDS, QUICK, VANLEER, KOREN) */
+  const double *patch_T_val;  /* n_patches, fixed-temperature / inlet value */
 } om_params;
 
 typedef struct om_solver om_solver;
@@ -67,7 +72,7 @@ typedef struct om_solver om_solver;
 om_solver *om_create(const om_mesh *m, const om_params *p);
 void om_free(om_solver *s);
 /* Pointer to an internal array by name ("U","p","p_bnd","U_bnd","nu_sgs","grad_U","U_face",
- * "face_flux","tau_w_bnd","w_gp","rhs_p"), with its length in doubles. */
+ * "face_flux","tau_w_bnd","w_gp","rhs_p","T","T_bnd","grad_T"), with its length in doubles. */
 double *om_array(om_solver *s, const char *name, int64_t *len);
 /* After U (and p) were written from outside: refresh boundary values as solver_init does. */
 void om_apply_all_bc(om_solver *s);

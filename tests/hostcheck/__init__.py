@@ -72,6 +72,10 @@ class HostSolver:
         self.array("U")[:] = np.asarray(U, dtype=np.float64).reshape(-1)
         lib().hc_solver_apply_all_bc(self.h)
 
+    def set_temperature(self, T):
+        self.array("T")[:] = np.asarray(T, dtype=np.float64).reshape(-1)
+        lib().hc_solver_apply_all_bc(self.h)
+
     def project_once(self, dt):
         return lib().hc_solver_project_once(self.h, dt)
 

@@ -22,6 +22,9 @@ double* field(Solver* s, const std::string& name, int64_t* n) {
   if (name == "tau_w_bnd") { *n = 3 * nb; return s->tau; }
   if (name == "rhs_p") { *n = nc; return s->rhs; }
   if (name == "w_gp") { *n = 3 * nc; return s->gp; }
+  if (s->bous && name == "T") { *n = nc; return s->T; }
+  if (s->bous && name == "T_bnd") { *n = nb; return s->T_bnd; }
+  if (s->bous && name == "grad_T") { *n = 3 * nc; return s->gT; }
   *n = 0;
   return nullptr;
 }

@@ -111,3 +111,17 @@ def test_host_reconstruction_schemes_match_oracle(scheme, name):
     mesh = case.make_mesh()
     h, o = _pair(case, mesh, 3)
     assert rel_l2(o.U, h.U) < 1e-10 and rel_l2(o.p, h.p) < 1e-10
+
+
+def test_host_tet_cavity_matches_oracle():
+    """Jittered, shuffled tetrahedra (refinement-interface skew > 0.15: up to 8 correctors),
+    Koren TVD, AMD SGS, moving lid."""
+    case = cases.cavity_tets(6)
+    case.params.p_tol = 1e-12
+    mesh = case.make_mesh()
+    assert mesh.face_skew.max() > 0.15
+    h, o = _pair(case, mesh, 4)
+    assert h.nonortho_info() == o.nonortho_info()
+    for name in ("U", "p", "nu_sgs"):
+        a = o.array(name)
+        assert rel_l2(a, h.get(name)[: a.size]) < 1e-10, name
