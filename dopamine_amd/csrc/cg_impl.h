@@ -37,6 +37,49 @@ struct EllOp {
   const double* diag = nullptr;  // n (already repaired: non-finite or tiny -> 1)
 };
 
+// Packed per-cell state of the single-reduction recurrence: one 32-byte (16-byte in FP32) load
+// fetches everything a neighbour contributes.
+struct alignas(32) St4 {
+  double r, s, w, p;
+};
+struct alignas(16) St4f {
+  float r, s, w, p;
+};
+
+// Operator in the form the single-reduction passes use: chat = coef / diag(column) and
+// dinv = 1 / diag, so A D^-1 r = r_c - sum chat r_j needs no division.
+struct JacobiScaled {
+  int n = 0, w = 0;
+  const int* nbr = nullptr;
+  const double* chat = nullptr;
+  const double* dinv = nullptr;
+  const float* chat32 = nullptr;
+  const float* dinv32 = nullptr;
+};
+
+namespace cgk {
+struct Scale {
+  int n;
+  const int* nbr;
+  const double *coef, *diag;
+  double *chat, *dinv;
+  float *chat32, *dinv32;  // may be null
+  int w;
+  DM_HD void operator()(int c) const {
+    const double di = 1.0 / diag[c];
+    dinv[c] = di;
+    if (dinv32) dinv32[c] = float(di);
+    for (int k = 0; k < w; ++k) {
+      const size_t q = size_t(k) * n + c;
+      const int j = nbr[q];
+      const double v = j >= 0 ? coef[q] / diag[j] : 0.0;
+      chat[q] = v;
+      if (chat32) chat32[q] = float(v);
+    }
+  }
+};
+}  // namespace cgk
+
 // FP32 copy of the operator for the mixed-precision inner solves (same rows and slots).
 struct EllOp32 {
   int n = 0, w = 0;
@@ -233,11 +276,12 @@ struct Resid {
 // A 1 = 0, w = A D^-1 r and the mean enters only through the scalars:
 // gamma = (r, D^-1 r) - mean * sum r, delta = (w, D^-1 r) - mean * sum w.
 
-// w0 = A D^-1 r0 and ttol; partials: r.r, r.u, w.u, sum u, sum r, sum w (u = D^-1 r)
+// w0 = A D^-1 r0 and ttol; packed state {r0, 0, w0, 0}; partials: r.r, r.u, w.u, sum u, sum r,
+// sum w (u = D^-1 r; A D^-1 r = r_c - sum chat r_j)
 struct W0 {
-  EllOp A;
+  JacobiScaled J;
   const double* r;
-  double* w;
+  St4* st;
   CgScalars* sc;
   using Scal = NoScal;
   DM_HD bool setup(const Red<2>& t, bool leader, Scal&) const {
@@ -245,21 +289,22 @@ struct W0 {
     return true;
   }
   DM_HD Red<6> operator()(int c, const Scal&) const {
-    const int n = A.n;
-    const double uc = r[c] / A.diag[c];
-    double wc = A.diag[c] * uc;
-    for (int k = 0; k < A.w; ++k) {
-      const int j = A.nbr[size_t(k) * n + c];
+    const int n = J.n;
+    const double rc = r[c], uc = rc * J.dinv[c];
+    double wc = rc;
+    for (int k = 0; k < J.w; ++k) {
+      const size_t q = size_t(k) * n + c;
+      const int j = J.nbr[q];
       if (j < 0) continue;
-      wc = wc - A.coef[size_t(k) * n + c] * (r[j] / A.diag[j]);
+      wc = wc - J.chat[q] * r[j];
     }
-    w[c] = wc;
+    st[c] = St4{rc, 0.0, wc, 0.0};
     Red<6> o;
-    o.v[0] = r[c] * r[c];
-    o.v[1] = r[c] * uc;
+    o.v[0] = rc * rc;
+    o.v[1] = rc * uc;
     o.v[2] = wc * uc;
     o.v[3] = uc;
-    o.v[4] = r[c];
+    o.v[4] = rc;
     o.v[5] = wc;
     return o;
   }
@@ -268,11 +313,12 @@ struct W0 {
 // Iteration k: test ||r_k||; beta_k = gamma_k / gamma_(k-1), alpha_k = gamma_k / (delta_k -
 // beta_k gamma_k / alpha_(k-1)); p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
 // w' = A D^-1 r' with the neighbours' r' recomputed on the fly from their old r, s, w.
-// Old r, s, w are read from one buffer set and the new ones written to the other.
+// The old packed state is read from one buffer and the new one written to the other.
 struct Fused {
-  EllOp A;
-  const double *r0, *s0, *w0;
-  double *r1, *s1, *w1, *p, *x;
+  JacobiScaled J;
+  const St4* st0;
+  St4* st1;
+  double* x;
   CgScalars* sc;
   int k;
   struct Scal {
@@ -281,7 +327,7 @@ struct Fused {
   DM_HD bool setup(const Red<6>& t, bool leader, Scal& s) const {
     if (sc->done) return false;
     const double rr = t.v[0];
-    s.um = sc->nullspace ? t.v[3] / double(A.n) : 0.0;
+    s.um = sc->nullspace ? t.v[3] / double(J.n) : 0.0;
     const double gamma = t.v[1] - s.um * t.v[4];
     const double delta = t.v[2] - s.um * t.v[5];
     const int fin = (sqrt(rr) <= sc->ttol) ? 1 : (k >= sc->maxit ? 2 : 0);
@@ -309,41 +355,40 @@ struct Fused {
     return true;
   }
   DM_HD Red<6> operator()(int c, const Scal& s) const {
-    const int n = A.n;
+    const int n = J.n;
     const bool first = k == 0;
-    const double dc = A.diag[c];
-    const double uc = r0[c] / dc - s.um;
-    const double pc = first ? uc : uc + s.beta * p[c];
-    const double sc_ = first ? w0[c] : w0[c] + s.beta * s0[c];
+    const St4 o = st0[c];
+    const double di = J.dinv[c];
+    const double uc = o.r * di - s.um;
+    const double pc = first ? uc : uc + s.beta * o.p;
+    const double sc_ = first ? o.w : o.w + s.beta * o.s;
     x[c] = x[c] + s.alpha * pc;
-    const double rc = r0[c] - s.alpha * sc_;
-    p[c] = pc;
-    s1[c] = sc_;
-    r1[c] = rc;
-    const double un = rc / dc;
-    double wc = dc * un;
-    for (int kk = 0; kk < A.w; ++kk) {
-      const int j = A.nbr[size_t(kk) * n + c];
+    const double rc = o.r - s.alpha * sc_;
+    const double un = rc * di;
+    double wc = rc;
+    for (int kk = 0; kk < J.w; ++kk) {
+      const size_t q = size_t(kk) * n + c;
+      const int j = J.nbr[q];
       if (j < 0) continue;
-      const double sj = first ? w0[j] : w0[j] + s.beta * s0[j];
-      const double rj = r0[j] - s.alpha * sj;
-      wc = wc - A.coef[size_t(kk) * n + c] * (rj / A.diag[j]);
+      const St4 nj = st0[j];
+      const double sj = first ? nj.w : nj.w + s.beta * nj.s;
+      wc = wc - J.chat[q] * (nj.r - s.alpha * sj);
     }
-    w1[c] = wc;
-    Red<6> o;
-    o.v[0] = rc * rc;
-    o.v[1] = rc * un;
-    o.v[2] = wc * un;
-    o.v[3] = un;
-    o.v[4] = rc;
-    o.v[5] = wc;
-    return o;
+    st1[c] = St4{rc, sc_, wc, pc};
+    Red<6> out;
+    out.v[0] = rc * rc;
+    out.v[1] = rc * un;
+    out.v[2] = wc * un;
+    out.v[3] = un;
+    out.v[4] = rc;
+    out.v[5] = wc;
+    return out;
   }
 };
 
 // ---- mixed precision: FP64 iterative refinement around FP32 single-reduction CG sweeps.
 // The outer loop keeps x and the true residual r = b' - A x in FP64 (FP64 operator); each sweep
-// solves A e = r in FP32 (FP32 operator and vectors, FP64 dot-product accumulation) to a relative
+// solves A e = r in FP32 (FP32 operator and state, FP64 dot-product accumulation) to a relative
 // reduction tau, adding alpha p straight into the FP64 x.
 
 // r = b' - A x in FP64; partials (0, r.r) in the layout Init produces
@@ -371,12 +416,12 @@ struct IrResid {
   }
 };
 
-// Start of a sweep: the outer test ||r|| <= ttol on the FP64 residual, then r32 = r, w = A u with
-// u = D^-1 r32 (FP32), partials as W0. The first sweep also fixes ttol from ||b'||.
+// Start of a sweep: the outer test ||r|| <= ttol on the FP64 residual, then the FP32 packed state
+// {r, 0, w, 0} with w = A D^-1 r, partials as W0. The first sweep also fixes ttol from ||b'||.
 struct IrStart {
-  EllOp32 A;
+  JacobiScaled J;
   const double* r64;
-  float *r, *w;
+  St4f* st;
   CgScalars* sc;
   int sweep, max_sweeps;
   double tau;
@@ -398,17 +443,17 @@ struct IrStart {
     return fin == 0;
   }
   DM_HD Red<6> operator()(int c, const Scal&) const {
-    const int n = A.n;
+    const int n = J.n;
     const float rc = float(r64[c]);
-    const float uc = rc / A.diag[c];
-    float wc = A.diag[c] * uc;
-    for (int k = 0; k < A.w; ++k) {
-      const int j = A.nbr[size_t(k) * n + c];
+    const float uc = rc * J.dinv32[c];
+    float wc = rc;
+    for (int k = 0; k < J.w; ++k) {
+      const size_t q = size_t(k) * n + c;
+      const int j = J.nbr[q];
       if (j < 0) continue;
-      wc = wc - A.coef[size_t(k) * n + c] * (float(r64[j]) / A.diag[j]);
+      wc = wc - J.chat32[q] * float(r64[j]);
     }
-    r[c] = rc;
-    w[c] = wc;
+    st[c] = St4f{rc, 0.0f, wc, 0.0f};
     Red<6> o;
     o.v[0] = double(rc) * double(rc);
     o.v[1] = double(rc) * double(uc);
@@ -422,9 +467,9 @@ struct IrStart {
 
 // One FP32 single-reduction iteration of a sweep (as Fused), x accumulated in FP64.
 struct Fused32 {
-  EllOp32 A;
-  const float *r0, *s0, *w0;
-  float *r1, *s1, *w1, *p;
+  JacobiScaled J;
+  const St4f* st0;
+  St4f* st1;
   double* x;
   CgScalars* sc;
   int k;
@@ -434,7 +479,7 @@ struct Fused32 {
   DM_HD bool setup(const Red<6>& t, bool leader, Scal& s) const {
     if (sc->done || sc->inner_done) return false;
     const double rr = t.v[0];
-    s.um = sc->nullspace ? t.v[3] / double(A.n) : 0.0;
+    s.um = sc->nullspace ? t.v[3] / double(J.n) : 0.0;
     const double gamma = t.v[1] - s.um * t.v[4];
     const double delta = t.v[2] - s.um * t.v[5];
     if (k > 0 && (rr <= sc->inner_tol2 || sc->its + k >= sc->maxit)) {
@@ -459,36 +504,35 @@ struct Fused32 {
     return true;
   }
   DM_HD Red<6> operator()(int c, const Scal& s) const {
-    const int n = A.n;
+    const int n = J.n;
     const bool first = k == 0;
     const float beta = float(s.beta), alpha = float(s.alpha), um = float(s.um);
-    const float dc = A.diag[c];
-    const float uc = r0[c] / dc - um;
-    const float pc = first ? uc : uc + beta * p[c];
-    const float sc_ = first ? w0[c] : w0[c] + beta * s0[c];
+    const St4f o = st0[c];
+    const float di = J.dinv32[c];
+    const float uc = o.r * di - um;
+    const float pc = first ? uc : uc + beta * o.p;
+    const float sc_ = first ? o.w : o.w + beta * o.s;
     x[c] = x[c] + s.alpha * double(pc);
-    const float rc = r0[c] - alpha * sc_;
-    p[c] = pc;
-    s1[c] = sc_;
-    r1[c] = rc;
-    const float un = rc / dc;
-    float wc = dc * un;
-    for (int kk = 0; kk < A.w; ++kk) {
-      const int j = A.nbr[size_t(kk) * n + c];
+    const float rc = o.r - alpha * sc_;
+    const float un = rc * di;
+    float wc = rc;
+    for (int kk = 0; kk < J.w; ++kk) {
+      const size_t q = size_t(kk) * n + c;
+      const int j = J.nbr[q];
       if (j < 0) continue;
-      const float sj = first ? w0[j] : w0[j] + beta * s0[j];
-      const float rj = r0[j] - alpha * sj;
-      wc = wc - A.coef[size_t(kk) * n + c] * (rj / A.diag[j]);
+      const St4f nj = st0[j];
+      const float sj = first ? nj.w : nj.w + beta * nj.s;
+      wc = wc - J.chat32[q] * (nj.r - alpha * sj);
     }
-    w1[c] = wc;
-    Red<6> o;
-    o.v[0] = double(rc) * double(rc);
-    o.v[1] = double(rc) * double(un);
-    o.v[2] = double(wc) * double(un);
-    o.v[3] = double(un);
-    o.v[4] = double(rc);
-    o.v[5] = double(wc);
-    return o;
+    st1[c] = St4f{rc, sc_, wc, pc};
+    Red<6> out;
+    out.v[0] = double(rc) * double(rc);
+    out.v[1] = double(rc) * double(un);
+    out.v[2] = double(wc) * double(un);
+    out.v[3] = double(un);
+    out.v[4] = double(rc);
+    out.v[5] = double(wc);
+    return out;
   }
 };
 
@@ -509,11 +553,6 @@ class CgT {
     q_ = x->template alloc<double>(n);
     sc_ = x->template alloc<CgScalars>(1);
     part_ = x->template alloc<double>(size_t(PARTS) * g_);
-    r2_ = x->template alloc<double>(n);
-    s_[0] = x->template alloc<double>(n);
-    s_[1] = x->template alloc<double>(n);
-    w_[0] = x->template alloc<double>(n);
-    w_[1] = x->template alloc<double>(n);
     hpart_.resize(g_);
   }
   // Hestenes-Stiefel two passes per iteration (PETSc KSPCG's recurrence) or the single-reduction
@@ -522,22 +561,21 @@ class CgT {
   // Mixed precision: FP64 iterative refinement around FP32 sweeps on A32 (same rows as the FP64
   // operator) that each reduce the residual by tau.
   void set_mixed(const EllOp32& A32, double tau) {
-    A32_ = A32;
+    (void)A32;  // the FP32 operator is derived from the FP64 one on the device
     tau_ = tau;
     mixed_ = true;
-    if (!f32_[0])
-      for (auto*& q : f32_) q = x_->template alloc<float>(n_);
   }
   int last_sweeps() const { return last_sweeps_; }
   void destroy() {
     if (!x_) return;
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
-                    (void*)r2_, (void*)s_[0], (void*)s_[1], (void*)w_[0], (void*)w_[1]})
-      x_->release(p);
-    for (auto*& q : f32_) {
-      if (q) x_->release(q);
-      q = nullptr;
-    }
+                    (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
+                    (void*)chat32_, (void*)dinv32_})
+      if (p) x_->release(p);
+    st_[0] = st_[1] = nullptr;
+    st32_[0] = st32_[1] = nullptr;
+    chat_ = dinv_ = nullptr;
+    chat32_ = dinv32_ = nullptr;
     x_ = nullptr;
   }
   // Solve A x = b (pointers in the execution space). atol_dev, when not null, points to an
@@ -560,28 +598,25 @@ class CgT {
     h.maxit = maxit;
     h.nullspace = nullspace ? 1 : 0;
     xx.upload(sc_, &h, 1);
+    JacobiScaled J;
+    if (single_ || mixed_) J = scaled(A);
     xx.template pass<1, 1>(n, SumB{b, bw_, x}, nullptr, 0, P0);
     xx.template pass<1, 2>(n, Init{A, bw_, r_, x, sc_, atol_dev}, P0, g, P1);
     int k = 0;
     int batch = std::max(4, std::min(maxit + 1, predict_));
     last_sweeps_ = 0;
     if (mixed_) {
-      // f32_: r (2), s (2), w (2), p
-      float* r[2] = {f32_[0], f32_[1]};
-      float* s[2] = {f32_[2], f32_[3]};
-      float* w[2] = {f32_[4], f32_[5]};
       double* F[2] = {U[0], U[0] + 6 * size_t(g)};
       const double* in = P1;
       int total = 0, ibatch = std::max(4, predict_in_);
       for (int sweep = 0;; ++sweep) {
-        xx.template pass<2, 6>(n, IrStart{A32_, r_, r[0], w[0], sc_, sweep, MAX_SWEEPS, tau_}, in, g, F[0]);
+        xx.template pass<2, 6>(n, IrStart{J, r_, st32_[0], sc_, sweep, MAX_SWEEPS, tau_}, in, g, F[0]);
         int kin = 0;
         for (;;) {
           for (int i = 0; i < ibatch; ++i, ++kin) {
             const int a = kin & 1, b2 = a ^ 1;
             xx.tbegin(K_CG_DIR);
-            xx.template pass<6, 6>(
-                n, Fused32{A32_, r[a], s[a], w[a], r[b2], s[b2], w[b2], f32_[6], x, sc_, kin}, F[a], g, F[b2]);
+            xx.template pass<6, 6>(n, Fused32{J, st32_[a], st32_[b2], x, sc_, kin}, F[a], g, F[b2]);
             xx.tend(K_CG_DIR);
           }
           xx.download(&h, sc_, 1);
@@ -602,16 +637,13 @@ class CgT {
       }
       h.its = total;
     } else if (single_) {
-      double* r[2] = {r_, r2_};
-      double* F[2] = {U[0], U[1]};  // 6 x g each (U[0] and U[1] are contiguous 4 x g blocks)
-      F[1] = U[0] + 6 * size_t(g);
-      xx.template pass<2, 6>(n, W0{A, r_, w_[0], sc_}, P1, g, F[0]);
+      double* F[2] = {U[0], U[0] + 6 * size_t(g)};  // 6 x g each, over the U blocks
+      xx.template pass<2, 6>(n, W0{J, r_, st_[0], sc_}, P1, g, F[0]);
       for (;;) {
         for (int i = 0; i < batch; ++i, ++k) {
           const int a = k & 1, b2 = a ^ 1;
           xx.tbegin(K_CG_DIR);
-          xx.template pass<6, 6>(n, Fused{A, r[a], s_[a], w_[a], r[b2], s_[b2], w_[b2], pa_, x, sc_, k}, F[a], g,
-                                 F[b2]);
+          xx.template pass<6, 6>(n, Fused{J, st_[a], st_[b2], x, sc_, k}, F[a], g, F[b2]);
           xx.tend(K_CG_DIR);
         }
         xx.download(&h, sc_, 1);
@@ -653,6 +685,50 @@ class CgT {
     return h.done == 1 ? 0 : -1;
   }
   static constexpr int MAX_SWEEPS = 40;
+
+ private:
+  // chat = coef / diag(column), dinv = 1 / diag (and FP32 copies when mixed), built on the device
+  // once per operator and kept while the operator's arrays stay the same.
+  JacobiScaled scaled(const EllOp& A) {
+    X& xx = *x_;
+    const size_t m = size_t(std::max(1, A.w)) * A.n;
+    if (!chat_ || scaled_of_ != A.coef || scaled_w_ != A.w || (mixed_ && !chat32_)) {
+      for (void* q : {(void*)chat_, (void*)dinv_, (void*)chat32_, (void*)dinv32_})
+        if (q) xx.release(q);
+      chat_ = xx.template alloc<double>(m);
+      dinv_ = xx.template alloc<double>(A.n);
+      chat32_ = mixed_ ? xx.template alloc<float>(m) : nullptr;
+      dinv32_ = mixed_ ? xx.template alloc<float>(A.n) : nullptr;
+      xx.for_each(A.n, cgk::Scale{A.n, A.nbr, A.coef, A.diag, chat_, dinv_, chat32_, dinv32_, A.w});
+      scaled_of_ = A.coef;
+      scaled_w_ = A.w;
+    }
+    if (single_ && !st_[0]) {
+      st_[0] = xx.template alloc<St4>(n_);
+      st_[1] = xx.template alloc<St4>(n_);
+    }
+    if (mixed_ && !st32_[0]) {
+      st32_[0] = xx.template alloc<St4f>(n_);
+      st32_[1] = xx.template alloc<St4f>(n_);
+    }
+    JacobiScaled J;
+    J.n = A.n;
+    J.w = A.w;
+    J.nbr = A.nbr;
+    J.chat = chat_;
+    J.dinv = dinv_;
+    J.chat32 = chat32_;
+    J.dinv32 = dinv32_;
+    return J;
+  }
+  St4* st_[2] = {nullptr, nullptr};
+  St4f* st32_[2] = {nullptr, nullptr};
+  double *chat_ = nullptr, *dinv_ = nullptr;
+  float *chat32_ = nullptr, *dinv32_ = nullptr;
+  const double* scaled_of_ = nullptr;
+  int scaled_w_ = -1;
+
+ public:
   int last_its() const { return last_its_; }
 
  private:
@@ -660,11 +736,8 @@ class CgT {
   X* x_ = nullptr;
   int n_ = 0, g_ = 1;
   double *bw_ = nullptr, *r_ = nullptr, *z_ = nullptr, *pa_ = nullptr, *pb_ = nullptr, *q_ = nullptr;
-  double *r2_ = nullptr, *s_[2] = {nullptr, nullptr}, *w_[2] = {nullptr, nullptr};
   bool single_ = false, mixed_ = false;
-  EllOp32 A32_;
   double tau_ = 1e-3;
-  float* f32_[7] = {};
   int predict_in_ = 24, last_sweeps_ = 0;
   double* part_ = nullptr;
   CgScalars* sc_ = nullptr;
