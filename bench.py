@@ -33,6 +33,10 @@ def parse():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--cg-variant", type=int, choices=(0, 1), default=1,
                     help="pressure CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction")
+    ap.add_argument("--precond", choices=("jacobi", "learned"), default="jacobi",
+                    help="learned: the warm-up steps' Jacobi solves train the message-passing network, the "
+                         "timed steps run flexible CG with it")
+    ap.add_argument("--learn-iters", type=int, default=300, help="Adam steps of the learned preconditioner")
     ap.add_argument("--cpu-steps", type=int, default=1, help="RK3 steps of the CPU baseline (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without the per-launch event pairs (no roofline entry)")
@@ -86,6 +90,11 @@ def main():
     nx, ny, nz = args.mesh
     case = cases.channel(nx, ny, nz)
     case.params.cg_variant = args.cg_variant
+    learned = args.precond == "learned"
+    if learned:
+        case.params.precond = "LEARNED"
+        case.params.learn_collect = max(1, 3 * args.warmup)
+        case.params.learn_iters = args.learn_iters
     t0 = time.perf_counter()
     mesh = case.make_mesh()
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
@@ -97,6 +106,8 @@ def main():
         solver.advance()
         log(f"warmup step {i + 1}/{args.warmup}: {solver.counters()['last']} CG iterations in the last solve")
     solver.sync()
+    if learned:
+        log(f"learned preconditioner after warm-up: {solver.learned_info()}")
 
     it0 = solver.counters()["iterations"]
     solver.sync()
@@ -130,6 +141,8 @@ def main():
     nc = mesh.n_cells
     mixed = args.precision == "mixed"
     bytes_dir, nfp = cg_pass_bytes(mesh, args.cg_variant, mixed)
+    if learned:  # hidden layer: h read and written (32 bfloat16 per cell), FP32 edge weight + index per face side
+        bytes_dir = 128 * nc + 16 * nfp
     n_dir, ms_dir = ktime["cg_dir"]
     avg_s = ms_dir / max(n_dir, 1) / 1e3
     achieved = bytes_dir / avg_s / 1e9 if avg_s > 0 else 0.0
@@ -145,14 +158,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if mixed else "f64",
+        "dtype": "f32+f64" if mixed else ("f64+bf16" if learned else "f64"),
         "data": "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh",
         "config": {"workload": f"turbulent channel {nx}x{ny}x{nz} ({nc} hex cells, {nfp} pressure faces), "
                                "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, "
-                               + ("FP32 CG sweeps in FP64 iterative refinement" if mixed else "plain FP64"),
-                   "precision": args.precision, "cg_variant": args.cg_variant, "dt": case.params.dt},
+                               + ("FP32 CG sweeps in FP64 iterative refinement" if mixed else "plain FP64")
+                               + (", flexible CG with the learned message-passing preconditioner (Jacobi CG "
+                                  "fallback)" if learned else ""),
+                   "precision": args.precision, "cg_variant": args.cg_variant, "precond": args.precond,
+                   "dt": case.params.dt},
         "roofline": {"bound": "hbm",
-                     "kernel": ("CG FP32 sweep iteration k_pass<6,6,cgk::Fused32>" if mixed else
+                     "kernel": ("learned preconditioner hidden layer k_tile<1,mp::L1<6>> (MFMA bf16)" if learned else
+                                "CG FP32 sweep iteration k_pass<6,6,cgk::Fused32>" if mixed else
                                 "CG iteration pass k_pass<6,6,cgk::Fused>" if args.cg_variant == 1
                                 else "CG direction/operator pass k_pass<4,1,cgk::Dir>"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -165,6 +182,11 @@ def main():
         "state": {"E": st["E"], "Ub": st["Ub"], "max_abs_divU": st["max_abs_divU"]},
         "cpu_baseline": None,
     }
+    if learned:
+        info = solver.learned_info()
+        loss = solver.learned_loss()
+        out["learned"] = {**info, "loss_first": float(loss[:10].mean()) if loss.size else None,
+                          "loss_last": float(loss[-10:].mean()) if loss.size else None}
     # plain copy kernel of known size: achievable HBM bandwidth, and the calibration of the traffic
     # counters (FETCH_SIZE / WRITE_SIZE) for the same process
     import ctypes

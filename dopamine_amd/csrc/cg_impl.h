@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 namespace dm {
@@ -314,7 +315,19 @@ struct W0 {
 // beta_k gamma_k / alpha_(k-1)); p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
 // w' = A D^-1 r' with the neighbours' r' recomputed on the fly from their old r, s, w.
 // The old packed state is read from one buffer and the new one written to the other.
-struct Fused {
+// The batched form (WC > 0, used on the device) reads WC neighbour slots branch-free: a slot past
+// the operator's width or without a neighbour reads the cell's own state and has a zero
+// coefficient (chat is zero on empty slots), which subtracts an exact zero. Each thread takes U
+// cells at a time, so all their loads are issued before the first result is needed. At k = 0 the
+// state's s and p are zero (W0 writes them), so beta = 0 gives the first iteration's values.
+#ifndef DM_CG_CELLS_PER_THREAD
+#define DM_CG_CELLS_PER_THREAD 1
+#endif
+constexpr int CG_U = DM_CG_CELLS_PER_THREAD;
+
+template <int WC>
+struct FusedT {
+  static constexpr int kSlots = WC;
   JacobiScaled J;
   const St4* st0;
   St4* st1;
@@ -354,27 +367,7 @@ struct Fused {
     }
     return true;
   }
-  DM_HD Red<6> operator()(int c, const Scal& s) const {
-    const int n = J.n;
-    const bool first = k == 0;
-    const St4 o = st0[c];
-    const double di = J.dinv[c];
-    const double uc = o.r * di - s.um;
-    const double pc = first ? uc : uc + s.beta * o.p;
-    const double sc_ = first ? o.w : o.w + s.beta * o.s;
-    x[c] = x[c] + s.alpha * pc;
-    const double rc = o.r - s.alpha * sc_;
-    const double un = rc * di;
-    double wc = rc;
-    for (int kk = 0; kk < J.w; ++kk) {
-      const size_t q = size_t(kk) * n + c;
-      const int j = J.nbr[q];
-      if (j < 0) continue;
-      const St4 nj = st0[j];
-      const double sj = first ? nj.w : nj.w + s.beta * nj.s;
-      wc = wc - J.chat[q] * (nj.r - s.alpha * sj);
-    }
-    st1[c] = St4{rc, sc_, wc, pc};
+  DM_HD static Red<6> sums(double rc, double un, double wc) {
     Red<6> out;
     out.v[0] = rc * rc;
     out.v[1] = rc * un;
@@ -384,7 +377,79 @@ struct Fused {
     out.v[5] = wc;
     return out;
   }
+  DM_HD Red<6> operator()(int c, const Scal& s) const {
+    const int n = J.n;
+    const St4 o = st0[c];
+    const double di = J.dinv[c];
+    const double uc = o.r * di - s.um;
+    const double pc = uc + s.beta * o.p;
+    const double sc_ = o.w + s.beta * o.s;
+    x[c] = x[c] + s.alpha * pc;
+    const double rc = o.r - s.alpha * sc_;
+    double wc = rc;
+    for (int kk = 0; kk < J.w; ++kk) {
+      const size_t q = size_t(kk) * n + c;
+      const int j = J.nbr[q];
+      if (j < 0) continue;
+      const St4 nj = st0[j];
+      wc = wc - J.chat[q] * (nj.r - s.alpha * (nj.w + s.beta * nj.s));
+    }
+    st1[c] = St4{rc, sc_, wc, pc};
+    return sums(rc, rc * di, wc);
+  }
+  DM_HD Red<6> run(int lo, int hi, int t, int nt, const Scal& s) const {
+    const int n = J.n, w = J.w;
+    Red<6> acc = red_identity<6, false>();
+    for (int c0 = lo + t; c0 < hi; c0 += CG_U * nt) {
+      int cc[CG_U];
+      St4 o[CG_U];
+      double di[CG_U], xo[CG_U], ch[CG_U][WC], g[CG_U][WC];
+      int jj[CG_U][WC];
+#pragma unroll
+      for (int u = 0; u < CG_U; ++u) {
+        cc[u] = (c0 + u * nt < hi) ? c0 + u * nt : c0;
+        o[u] = st0[cc[u]];
+        di[u] = J.dinv[cc[u]];
+        xo[u] = x[cc[u]];
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) {
+          const bool live = kk < w;
+          const size_t q = size_t(live ? kk : 0) * n + cc[u];
+          const int j = J.nbr[q];
+          const double h = J.chat[q];
+          jj[u][kk] = (live && j >= 0) ? j : cc[u];
+          ch[u][kk] = live ? h : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CG_U; ++u)
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) {
+          const St4 nj = st0[jj[u][kk]];
+          g[u][kk] = nj.r - s.alpha * (nj.w + s.beta * nj.s);
+        }
+#pragma unroll
+      for (int u = 0; u < CG_U; ++u) {
+        const double uc = o[u].r * di[u] - s.um;
+        const double pc = uc + s.beta * o[u].p;
+        const double sc_ = o[u].w + s.beta * o[u].s;
+        const double rc = o[u].r - s.alpha * sc_;
+        double wc = rc;
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) wc = wc - ch[u][kk] * g[u][kk];
+        if (c0 + u * nt < hi) {
+          x[cc[u]] = xo[u] + s.alpha * pc;
+          st1[cc[u]] = St4{rc, sc_, wc, pc};
+          const Red<6> v = sums(rc, rc * di[u], wc);
+#pragma unroll
+          for (int q = 0; q < 6; ++q) acc.v[q] = acc.v[q] + v.v[q];
+        }
+      }
+    }
+    return acc;
+  }
 };
+using Fused = FusedT<0>;
 
 // ---- mixed precision: FP64 iterative refinement around FP32 single-reduction CG sweeps.
 // The outer loop keeps x and the true residual r = b' - A x in FP64 (FP64 operator); each sweep
@@ -465,8 +530,10 @@ struct IrStart {
   }
 };
 
-// One FP32 single-reduction iteration of a sweep (as Fused), x accumulated in FP64.
-struct Fused32 {
+// One FP32 single-reduction iteration of a sweep (as FusedT), x accumulated in FP64.
+template <int WC>
+struct Fused32T {
+  static constexpr int kSlots = WC;
   JacobiScaled J;
   const St4f* st0;
   St4f* st1;
@@ -503,28 +570,7 @@ struct Fused32 {
     }
     return true;
   }
-  DM_HD Red<6> operator()(int c, const Scal& s) const {
-    const int n = J.n;
-    const bool first = k == 0;
-    const float beta = float(s.beta), alpha = float(s.alpha), um = float(s.um);
-    const St4f o = st0[c];
-    const float di = J.dinv32[c];
-    const float uc = o.r * di - um;
-    const float pc = first ? uc : uc + beta * o.p;
-    const float sc_ = first ? o.w : o.w + beta * o.s;
-    x[c] = x[c] + s.alpha * double(pc);
-    const float rc = o.r - alpha * sc_;
-    const float un = rc * di;
-    float wc = rc;
-    for (int kk = 0; kk < J.w; ++kk) {
-      const size_t q = size_t(kk) * n + c;
-      const int j = J.nbr[q];
-      if (j < 0) continue;
-      const St4f nj = st0[j];
-      const float sj = first ? nj.w : nj.w + beta * nj.s;
-      wc = wc - J.chat32[q] * (nj.r - alpha * sj);
-    }
-    st1[c] = St4f{rc, sc_, wc, pc};
+  DM_HD static Red<6> sums(float rc, float un, float wc) {
     Red<6> out;
     out.v[0] = double(rc) * double(rc);
     out.v[1] = double(rc) * double(un);
@@ -534,8 +580,200 @@ struct Fused32 {
     out.v[5] = double(wc);
     return out;
   }
+  DM_HD Red<6> operator()(int c, const Scal& s) const {
+    const int n = J.n;
+    const float beta = float(s.beta), alpha = float(s.alpha), um = float(s.um);
+    const St4f o = st0[c];
+    const float di = J.dinv32[c];
+    const float uc = o.r * di - um;
+    const float pc = uc + beta * o.p;
+    const float sc_ = o.w + beta * o.s;
+    x[c] = x[c] + s.alpha * double(pc);
+    const float rc = o.r - alpha * sc_;
+    float wc = rc;
+    for (int kk = 0; kk < J.w; ++kk) {
+      const size_t q = size_t(kk) * n + c;
+      const int j = J.nbr[q];
+      if (j < 0) continue;
+      const St4f nj = st0[j];
+      wc = wc - J.chat32[q] * (nj.r - alpha * (nj.w + beta * nj.s));
+    }
+    st1[c] = St4f{rc, sc_, wc, pc};
+    return sums(rc, rc * di, wc);
+  }
+  DM_HD Red<6> run(int lo, int hi, int t, int nt, const Scal& s) const {
+    const int n = J.n, w = J.w;
+    const float beta = float(s.beta), alpha = float(s.alpha), um = float(s.um);
+    Red<6> acc = red_identity<6, false>();
+    for (int c0 = lo + t; c0 < hi; c0 += CG_U * nt) {
+      int cc[CG_U];
+      St4f o[CG_U];
+      float di[CG_U], ch[CG_U][WC], g[CG_U][WC];
+      double xo[CG_U];
+      int jj[CG_U][WC];
+#pragma unroll
+      for (int u = 0; u < CG_U; ++u) {
+        cc[u] = (c0 + u * nt < hi) ? c0 + u * nt : c0;
+        o[u] = st0[cc[u]];
+        di[u] = J.dinv32[cc[u]];
+        xo[u] = x[cc[u]];
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) {
+          const bool live = kk < w;
+          const size_t q = size_t(live ? kk : 0) * n + cc[u];
+          const int j = J.nbr[q];
+          const float h = J.chat32[q];
+          jj[u][kk] = (live && j >= 0) ? j : cc[u];
+          ch[u][kk] = live ? h : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CG_U; ++u)
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) {
+          const St4f nj = st0[jj[u][kk]];
+          g[u][kk] = nj.r - alpha * (nj.w + beta * nj.s);
+        }
+#pragma unroll
+      for (int u = 0; u < CG_U; ++u) {
+        const float uc = o[u].r * di[u] - um;
+        const float pc = uc + beta * o[u].p;
+        const float sc_ = o[u].w + beta * o[u].s;
+        const float rc = o[u].r - alpha * sc_;
+        float wc = rc;
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) wc = wc - ch[u][kk] * g[u][kk];
+        if (c0 + u * nt < hi) {
+          x[cc[u]] = xo[u] + s.alpha * double(pc);
+          st1[cc[u]] = St4f{rc, sc_, wc, pc};
+          const Red<6> v = sums(rc, rc * di[u], wc);
+#pragma unroll
+          for (int q = 0; q < 6; ++q) acc.v[q] = acc.v[q] + v.v[q];
+        }
+      }
+    }
+    return acc;
+  }
+};
+using Fused32 = Fused32T<0>;
+
+}  // namespace cgk
+
+// ---- flexible CG around a preconditioner applied elsewhere (the learned one, device only).
+// beta = max(0, z.(r - r_old) / (z.r)_old) (src/solver/mod_cell_projection.f90:273-281 uses the
+// same flexible beta), alpha = z.r / p.A p; the stopping test is the unpreconditioned
+// ||r|| <= max(rtol ||b'||, atol), as in the Jacobi passes.
+
+// z = M(r) as a chain of passes: the first sums the residual partials rr_part (rr_g blocks) of the
+// last update, records its = k, tests ||r_k|| <= ttol and k >= maxit (setting done and stopping
+// its grid like FusedT), the last leaves z.r, z.r_old, sum z, sum r, sum r_old in out_part
+// (out_grid blocks).
+struct FcgPc {
+  virtual ~FcgPc() = default;
+  virtual void apply(int k, const double* r, const double* rold, double* z, CgScalars* sc, const double* rr_part,
+                     int rr_g, double* out_part) = 0;
+  virtual int out_grid(int n) const = 0;
 };
 
+// Chooses, solve by solve, how a pressure system is solved (the learned preconditioner's
+// collection, training and fallback policy); same contract as CgT::solve.
+struct PressurePolicy {
+  virtual ~PressurePolicy() = default;
+  virtual int solve(const EllOp& A, const double* b, double* x, double rtol, double atol, const double* atol_dev,
+                    int maxit, bool nullspace, int* its, double* resid) = 0;
+};
+
+// Receives the residual of a running Jacobi CG solve at its batch boundaries (r[c * stride]).
+struct CgCapture {
+  virtual ~CgCapture() = default;
+  virtual void residual(int k, const double* r, int stride) = 0;
+};
+
+namespace cgk {
+// ttol from the partials (b'.b', r.r) of Init; r_old = r; partial r.r
+struct FcgStart {
+  const double* r;
+  double* rold;
+  CgScalars* sc;
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<2>& t, bool leader, Scal&) const {
+    if (leader) sc->ttol = fmax(sc->rtol * sqrt(t.v[0]), sc->atol);
+    return true;
+  }
+  DM_HD Red<1> operator()(int c, const Scal&) const {
+    const double rc = r[c];
+    rold[c] = rc;
+    Red<1> o;
+    o.v[0] = rc * rc;
+    return o;
+  }
+};
+
+// z -= mean(z) on a singular operator; beta; p = z + beta p_old; q = A p (neighbours' p formed on
+// the fly); partial p.q
+struct FcgDir {
+  EllOp A;
+  const double *z, *pold;
+  double *pnew, *q;
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double zm, beta;
+  };
+  DM_HD bool setup(const Red<5>& t, bool leader, Scal& s) const {
+    if (sc->done) return false;
+    s.zm = sc->nullspace ? t.v[2] / double(A.n) : 0.0;
+    const double zr = t.v[0] - s.zm * t.v[3];
+    const double zro = t.v[1] - s.zm * t.v[4];
+    s.beta = k == 0 ? 0.0 : fmax(0.0, (zr - zro) / sc->rz_hist[(k + 1) & 1]);
+    if (leader) sc->rz_hist[k & 1] = zr;
+    return true;
+  }
+  DM_HD Red<1> operator()(int c, const Scal& s) const {
+    const int n = A.n;
+    const bool first = k == 0;
+    const double pc = first ? z[c] - s.zm : (z[c] - s.zm) + s.beta * pold[c];
+    double qc = A.diag[c] * pc;
+    for (int kk = 0; kk < A.w; ++kk) {
+      const size_t q_ = size_t(kk) * n + c;
+      const int j = A.nbr[q_];
+      if (j < 0) continue;
+      const double pj = first ? z[j] - s.zm : (z[j] - s.zm) + s.beta * pold[j];
+      qc = qc - A.coef[q_] * pj;
+    }
+    pnew[c] = pc;
+    q[c] = qc;
+    Red<1> o;
+    o.v[0] = pc * qc;
+    return o;
+  }
+};
+
+// alpha = (z.r) / (p.q); x += alpha p; r_old = r; r -= alpha q; partial r.r
+struct FcgUpd {
+  double *x, *r, *rold;
+  const double *p, *q;
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double alpha;
+  };
+  DM_HD bool setup(const Red<1>& t, bool, Scal& s) const {
+    if (sc->done) return false;
+    s.alpha = sc->rz_hist[k & 1] / t.v[0];
+    return true;
+  }
+  DM_HD Red<1> operator()(int c, const Scal& s) const {
+    x[c] = x[c] + s.alpha * p[c];
+    const double rc = r[c];
+    rold[c] = rc;
+    const double rn = rc - s.alpha * q[c];
+    r[c] = rn;
+    Red<1> o;
+    o.v[0] = rn * rn;
+    return o;
+  }
+};
 }  // namespace cgk
 
 template <class X>
@@ -558,6 +796,19 @@ class CgT {
   // Hestenes-Stiefel two passes per iteration (PETSc KSPCG's recurrence) or the single-reduction
   // variant, one pass per iteration.
   void set_single_reduction(bool on) { single_ = on; }
+  // Flexible CG with the preconditioner pc (null: back to the Jacobi passes).
+  void set_fcg(FcgPc* pc) {
+    fcg_ = pc;
+    if (pc && !ro_) ro_ = x_->template alloc<double>(n_);
+    if (pc && fpart_g_ < pc->out_grid(n_)) {
+      if (fpart_) x_->release(fpart_);
+      fpart_g_ = pc->out_grid(n_);
+      fpart_ = x_->template alloc<double>(size_t(5) * fpart_g_);
+    }
+  }
+  bool fcg() const { return fcg_ != nullptr; }
+  // Residual snapshots of the Jacobi passes at their batch boundaries (null: none).
+  void set_capture(CgCapture* c) { capture_ = c; }
   // Mixed precision: FP64 iterative refinement around FP32 sweeps on A32 (same rows as the FP64
   // operator) that each reduce the residual by tau.
   void set_mixed(const EllOp32& A32, double tau) {
@@ -570,12 +821,15 @@ class CgT {
     if (!x_) return;
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
-                    (void*)chat32_, (void*)dinv32_})
+                    (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_})
       if (p) x_->release(p);
     st_[0] = st_[1] = nullptr;
     st32_[0] = st32_[1] = nullptr;
     chat_ = dinv_ = nullptr;
     chat32_ = dinv32_ = nullptr;
+    ro_ = fpart_ = nullptr;
+    fpart_g_ = 0;
+    fcg_ = nullptr;
     x_ = nullptr;
   }
   // Solve A x = b (pointers in the execution space). atol_dev, when not null, points to an
@@ -604,8 +858,29 @@ class CgT {
     xx.template pass<1, 2>(n, Init{A, bw_, r_, x, sc_, atol_dev}, P0, g, P1);
     int k = 0;
     int batch = std::max(4, std::min(maxit + 1, predict_));
+    if (capture_) batch = std::min(batch, 16);  // residual snapshots along the whole solve
     last_sweeps_ = 0;
-    if (mixed_) {
+    if (fcg_) {
+      double* RR = U[0];
+      const int gt = fcg_->out_grid(n);
+      xx.template pass<2, 1>(n, FcgStart{r_, ro_, sc_}, P1, g, RR);
+      double* pold = pa_;
+      double* pnew = pb_;
+      batch = std::max(4, std::min(maxit + 1, predict_fcg_));
+      for (;;) {
+        for (int i = 0; i < batch; ++i, ++k) {
+          fcg_->apply(k, r_, ro_, z_, sc_, RR, g, fpart_);  // times its hidden layer as K_CG_DIR
+          xx.tbegin(K_CG_UPD);
+          xx.template pass<5, 1>(n, FcgDir{A, z_, pold, pnew, q_, sc_, k}, fpart_, gt, D);
+          xx.template pass<1, 1>(n, FcgUpd{x, r_, ro_, pnew, q_, sc_, k}, D, g, RR);
+          xx.tend(K_CG_UPD);
+          std::swap(pold, pnew);
+        }
+        xx.download(&h, sc_, 1);
+        if (h.done || k > maxit + 1) break;
+        batch = std::max(2, std::min(16, (h.its + 7) / 8));
+      }
+    } else if (mixed_) {
       double* F[2] = {U[0], U[0] + 6 * size_t(g)};
       const double* in = P1;
       int total = 0, ibatch = std::max(4, predict_in_);
@@ -616,7 +891,10 @@ class CgT {
           for (int i = 0; i < ibatch; ++i, ++kin) {
             const int a = kin & 1, b2 = a ^ 1;
             xx.tbegin(K_CG_DIR);
-            xx.template pass<6, 6>(n, Fused32{J, st32_[a], st32_[b2], x, sc_, kin}, F[a], g, F[b2]);
+            by_width(J.w, [&](auto wc) {
+              xx.template pass<6, 6>(n, Fused32T<decltype(wc)::value>{J, st32_[a], st32_[b2], x, sc_, kin}, F[a],
+                                     g, F[b2]);
+            });
             xx.tend(K_CG_DIR);
           }
           xx.download(&h, sc_, 1);
@@ -643,12 +921,15 @@ class CgT {
         for (int i = 0; i < batch; ++i, ++k) {
           const int a = k & 1, b2 = a ^ 1;
           xx.tbegin(K_CG_DIR);
-          xx.template pass<6, 6>(n, Fused{J, st_[a], st_[b2], x, sc_, k}, F[a], g, F[b2]);
+          by_width(J.w, [&](auto wc) {
+            xx.template pass<6, 6>(n, FusedT<decltype(wc)::value>{J, st_[a], st_[b2], x, sc_, k}, F[a], g, F[b2]);
+          });
           xx.tend(K_CG_DIR);
         }
         xx.download(&h, sc_, 1);
         if (h.done || k > maxit + 1) break;
-        batch = std::max(2, std::min(32, (h.its + 7) / 8));
+        if (capture_) capture_->residual(k, &st_[k & 1]->r, 4);
+        batch = std::max(2, std::min(capture_ ? 16 : 32, (h.its + 7) / 8));
       }
     } else {
       xx.template pass<2, 4>(n, Z0{A.diag, r_, z_, sc_}, P1, g, U[2]);
@@ -666,7 +947,8 @@ class CgT {
         }
         xx.download(&h, sc_, 1);
         if (h.done || k > maxit + 1) break;
-        batch = std::max(2, std::min(16, (h.its + 7) / 8));
+        if (capture_) capture_->residual(k, r_, 1);
+        batch = std::max(2, std::min(capture_ ? 8 : 16, (h.its + 7) / 8));
       }
     }
     xx.template pass<1, 1>(n, Resid{A, b, x}, nullptr, 0, R);
@@ -679,7 +961,8 @@ class CgT {
       xx.tcommit(K_CG_UPD, h.its);
     }
     last_its_ = h.its;
-    predict_ = std::max(2, h.its + 1);
+    last_ttol_ = h.ttol;
+    (fcg_ ? predict_fcg_ : predict_) = std::max(2, h.its + 1);
     if (its) *its = h.its;
     if (resid) *resid = std::sqrt(rr);
     return h.done == 1 ? 0 : -1;
@@ -687,6 +970,19 @@ class CgT {
   static constexpr int MAX_SWEEPS = 40;
 
  private:
+  // Calls fn with the slot count of the batched passes for an operator of width w (0: generic).
+  template <class Fn>
+  static void by_width(int w, Fn&& fn) {
+    if (w <= 4)
+      fn(std::integral_constant<int, 4>{});
+    else if (w <= 6)
+      fn(std::integral_constant<int, 6>{});
+    else if (w <= 8)
+      fn(std::integral_constant<int, 8>{});
+    else
+      fn(std::integral_constant<int, 0>{});
+  }
+
   // chat = coef / diag(column), dinv = 1 / diag (and FP32 copies when mixed), built on the device
   // once per operator and kept while the operator's arrays stay the same.
   JacobiScaled scaled(const EllOp& A) {
@@ -730,6 +1026,7 @@ class CgT {
 
  public:
   int last_its() const { return last_its_; }
+  double last_ttol() const { return last_ttol_; }
 
  private:
   static constexpr int PARTS = 17;  // partial quantities per block: P0 1, P1 2, D 1, R 1, U 3 x 4
@@ -737,12 +1034,18 @@ class CgT {
   int n_ = 0, g_ = 1;
   double *bw_ = nullptr, *r_ = nullptr, *z_ = nullptr, *pa_ = nullptr, *pb_ = nullptr, *q_ = nullptr;
   bool single_ = false, mixed_ = false;
+  FcgPc* fcg_ = nullptr;
+  CgCapture* capture_ = nullptr;
+  double *ro_ = nullptr, *fpart_ = nullptr;
+  int fpart_g_ = 0;
   double tau_ = 1e-3;
   int predict_in_ = 24, last_sweeps_ = 0;
   double* part_ = nullptr;
   CgScalars* sc_ = nullptr;
   std::vector<double> hpart_;
   int last_its_ = 0, predict_ = 24;
+  double last_ttol_ = 0.0;
+  int predict_fcg_ = 24;
 };
 
 }  // namespace dm

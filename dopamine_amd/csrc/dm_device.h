@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace dm {
@@ -100,8 +101,26 @@ __global__ void __launch_bounds__(BLOCK) k_final(int g, Fin fin, const double* p
 // without any cross-block synchronisation -- lets the functor derive its scalars (and stop the
 // whole grid uniformly), then streams a contiguous chunk of cells per block and stores its own
 // block partials for the next pass.
-constexpr int PBLOCK = 512;
-constexpr int PGRID_MAX = 512;  // two blocks per CU
+#ifndef DM_PBLOCK
+#define DM_PBLOCK 256
+#endif
+#ifndef DM_PGRID_MAX
+#define DM_PGRID_MAX 1024
+#endif
+constexpr int PBLOCK = DM_PBLOCK;
+constexpr int PGRID_MAX = DM_PGRID_MAX;  // four blocks of four waves per CU
+
+// A pass functor with a positive static kSlots streams its chunk itself (run(lo, hi, thread,
+// threads, scalars) returns the thread's partials, accumulated in the same cell order as the
+// generic loop); the others are called cell by cell.
+template <class F, class = void>
+struct PassSlots {
+  static constexpr int value = 0;
+};
+template <class F>
+struct PassSlots<F, std::void_t<decltype(F::kSlots)>> {
+  static constexpr int value = F::kSlots;
+};
 
 template <int NIN, int NOUT, class F>
 __global__ void __launch_bounds__(PBLOCK) k_pass(int n, int chunk, F f, const double* in_part, int in_g,
@@ -116,10 +135,14 @@ __global__ void __launch_bounds__(PBLOCK) k_pass(int n, int chunk, F f, const do
   if (!f.setup(tot, b == 0 && threadIdx.x == 0, sc)) return;  // uniform across the grid
   Red<NOUT> acc = red_identity<NOUT, false>();
   const int lo = b * chunk, hi = min(n, lo + chunk);
-  for (int c = lo + int(threadIdx.x); c < hi; c += PBLOCK) {
-    const Red<NOUT> v = f(c, sc);
+  if constexpr (PassSlots<F>::value > 0) {
+    acc = f.run(lo, hi, int(threadIdx.x), PBLOCK, sc);
+  } else {
+    for (int c = lo + int(threadIdx.x); c < hi; c += PBLOCK) {
+      const Red<NOUT> v = f(c, sc);
 #pragma unroll
-    for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
+      for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
+    }
   }
   acc = block_reduce<NOUT, false, PBLOCK>(acc);
   if (threadIdx.x == 0)

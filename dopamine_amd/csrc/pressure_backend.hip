@@ -10,6 +10,7 @@
 #include "cg_impl.h"
 #include "dm_device.h"
 #include "fvm_pressure_backend.h"
+#include "mpnn_dev.h"
 
 namespace {
 
@@ -31,7 +32,9 @@ struct Ctx {
   double *coef = nullptr, *diag = nullptr, *b = nullptr, *sol = nullptr;
   float *coef32 = nullptr, *diag32 = nullptr;
   bool mixed = false;
+  std::unique_ptr<dm::mp::LearnedPolicy> learned;  // selector "fcg-learned"
   ~Ctx() {
+    learned.reset();
     cg.destroy();
     for (void* p : {(void*)nbr, (void*)coef, (void*)diag, (void*)b, (void*)sol, (void*)coef32, (void*)diag32})
       x.release(p);
@@ -104,6 +107,24 @@ int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner
     ctx->cg.set_mixed(ctx->A32, 1.0e-4);
     ctx->mixed = true;
   }
+  if (g_solver == "fcg-learned") {
+    // options: -learn_collect N, -learn_capacity N, -learn_iters N, -learn_batch N, -learn_lr X,
+    // -learn_seed N (the solver's precond = 1 parameters)
+    dm::mp::LearnCfg cfg;
+    for (const auto& o : g_options) {
+      const auto val = [&](const char* key) -> const char* {
+        const size_t k = std::string(key).size();
+        return o.rfind(key, 0) == 0 && o.size() > k ? o.c_str() + k : nullptr;
+      };
+      if (const char* v = val("-learn_collect ")) cfg.collect = std::max(1, std::atoi(v));
+      if (const char* v = val("-learn_capacity ")) cfg.capacity = std::max(1, std::atoi(v));
+      if (const char* v = val("-learn_iters ")) cfg.iters = std::max(0, std::atoi(v));
+      if (const char* v = val("-learn_batch ")) cfg.batch = std::max(1, std::atoi(v));
+      if (const char* v = val("-learn_lr ")) cfg.lr = std::atof(v) > 0.0 ? std::atof(v) : cfg.lr;
+      if (const char* v = val("-learn_seed ")) cfg.seed = uint64_t(std::atoi(v));
+    }
+    ctx->learned.reset(new dm::mp::LearnedPolicy(&ctx->x, &ctx->cg, ctx->A, cfg));
+  }
   ctx->n_local = n;
   ctx->n_global = n_global;
   ctx->n_faces = n_faces;
@@ -166,7 +187,8 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
     int ref_row = reference_cell - 1;
     if (ref_row < 0 || ref_row >= n_cells_global) ref_row = 0;
     if (!g_ctx || g_ctx->n_local != n_cells_local || g_ctx->n_global != n_cells_global || g_ctx->n_faces != n_faces ||
-        g_ctx->ref_row != ref_row || g_ctx->mixed != (g_solver == "mixed")) {
+        g_ctx->ref_row != ref_row || g_ctx->mixed != (g_solver == "mixed") ||
+        bool(g_ctx->learned) != (g_solver == "fcg-learned")) {
       g_ctx.reset();
       const int rc = build_ctx(n_cells_local, n_cells_global, n_faces, local_to_global, face_owner,
                                face_neighbor_local, face_neighbor_global, skip_face, diagonal, face_coeff, ref_row);
@@ -196,7 +218,9 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
     c.cg.set_single_reduction(single);
     int its = 0;
     double res = 0.0;
-    const int rc = c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, n_dirichlet_rows <= 0, &its, &res);
+    const int rc = c.learned
+                       ? c.learned->solve(c.A, b, x, rtol, atol, nullptr, maxit, n_dirichlet_rows <= 0, &its, &res)
+                       : c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, n_dirichlet_rows <= 0, &its, &res);
     if (!g_io_on_device) c.x.download(solution, c.sol, n);
     c.x.sync();
     *iteration_count = its;

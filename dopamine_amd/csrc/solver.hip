@@ -2,6 +2,7 @@
 #include <string>
 
 #include "dm_device.h"
+#include "mpnn_dev.h"
 #include "solver_impl.h"
 
 namespace {
@@ -68,9 +69,25 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (p->cg_variant != 0 && p->cg_variant != 1) throw std::runtime_error("cg_variant must be 0 or 1");
     if (p->boussinesq && (p->t_scheme < 1 || p->t_scheme > 5 || p->t_scheme == 4))
       throw std::runtime_error("temperature scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3) or QUICK (5)");
+    if (p->precond != 0 && p->precond != 1) throw std::runtime_error("precond must be 0 (Jacobi) or 1 (learned)");
+    if (p->precond == 1 && (p->learn_collect < 1 || p->learn_capacity < 1 || p->learn_iters < 0 ||
+                            p->learn_batch < 1 || !(p->learn_lr > 0.0)))
+      throw std::runtime_error("learned preconditioner: learn_collect, learn_capacity, learn_batch >= 1, "
+                               "learn_iters >= 0, learn_lr > 0 required");
     s = new Solver();
     s->x.init(std::max(m->n_cells, m->n_faces + m->n_bnd_faces));
     s->create(*m, *p);
+    if (p->precond == 1) {
+      dm::mp::LearnCfg cfg;
+      cfg.collect = p->learn_collect;
+      cfg.capacity = p->learn_capacity;
+      cfg.iters = p->learn_iters;
+      cfg.batch = p->learn_batch;
+      cfg.lr = p->learn_lr;
+      cfg.seed = uint64_t(p->learn_seed);
+      s->policy.reset(new dm::mp::LearnedPolicy(&s->x, &s->cg, s->A, cfg));
+      s->x.sync();
+    }
     return 0;
   });
   if (rc != 0) {
@@ -171,6 +188,28 @@ int dopamine_solver_kernel_time(void* h, int kernel, int64_t* launches, double* 
     *launches = s->x.kt->launches[kernel];
     *total_ms = s->x.kt->total_ms[kernel];
     return 0;
+  });
+}
+
+int dopamine_solver_learned_info(void* h, double out[8]) {
+  return guard([&] {
+    auto* lp = dynamic_cast<dm::mp::LearnedPolicy*>(static_cast<Solver*>(h)->policy.get());
+    if (!lp) return -1;
+    const double v[8] = {double(lp->phase),        double(lp->collected),    double(lp->pc().samples()),
+                         double(lp->learned_solves), double(lp->learned_its), double(lp->fallbacks),
+                         lp->ref_its,                lp->train_ms};
+    for (int i = 0; i < 8; ++i) out[i] = v[i];
+    return 0;
+  });
+}
+
+int dopamine_solver_learned_loss(void* h, double* out, int max) {
+  return guard([&] {
+    auto* lp = dynamic_cast<dm::mp::LearnedPolicy*>(static_cast<Solver*>(h)->policy.get());
+    if (!lp) return -1;
+    const int m = std::min<int>(max, int(lp->loss_hist.size()));
+    for (int i = 0; i < m; ++i) out[i] = lp->loss_hist[i];
+    return m;
   });
 }
 

@@ -16,6 +16,7 @@
 #include <cstdlib>
 
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "cg_impl.h"
@@ -148,6 +149,7 @@ class SolverT {
     return (t > 0.0 && t < 1.0) ? t : 1.0e-4;
   }
   CgT<X> cg;
+  std::unique_ptr<PressurePolicy> policy;  // learned preconditioner (device only); null: CG passes
   std::vector<void*> owned;
   std::vector<int> dir_rows_h;
   std::vector<double> dir_vals_h;
@@ -169,6 +171,7 @@ class SolverT {
   }
 
   ~SolverT() {
+    policy.reset();
     cg.destroy();
     for (void* p : owned) x.release(p);
   }
@@ -841,7 +844,8 @@ class SolverT {
     const int maxit = prm.p_max_iter > 2000 ? prm.p_max_iter : 2000;
     int its = 0;
     double res = 0.0;
-    const int rc = cg.solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, n_dir == 0, &its, &res);
+    const int rc = policy ? policy->solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, n_dir == 0, &its, &res)
+                          : cg.solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, n_dir == 0, &its, &res);
     n_solves++;
     n_iters += its;
     last_iters = its;

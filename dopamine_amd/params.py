@@ -54,6 +54,15 @@ class SolverParams:
     Pr_t: float = 0.85
     temperature_scheme: str = "VANLEER_TVD"
     patch_T: dict = field(default_factory=dict)
+    # pressure preconditioner: "JACOBI", or "LEARNED" (flexible CG with the message-passing network,
+    # trained on residuals of the first learn_collect Jacobi solves; Jacobi CG is the fallback)
+    precond: str = "JACOBI"
+    learn_collect: int = 600
+    learn_capacity: int = 128
+    learn_iters: int = 300
+    learn_batch: int = 4
+    learn_lr: float = 3.0e-3
+    learn_seed: int = 1
     # per-patch values, by patch name: p_val (outlet), U_val (inlet / moving wall)
     patch_p: dict = field(default_factory=dict)
     patch_U: dict = field(default_factory=dict)
@@ -72,6 +81,8 @@ class SolverParams:
         if self.boussinesq and self.temperature_scheme.upper() not in ("CDS", "CENTRAL", "QUICK", "VANLEER_TVD",
                                                                          "VANLEER", "KOREN_TVD", "KOREN"):
             raise ValueError(f"temperature_scheme {self.temperature_scheme!r} not supported")
+        if self.precond.upper() not in ("JACOBI", "LEARNED"):
+            raise ValueError(f"precond {self.precond!r} not supported (JACOBI, LEARNED)")
         if self.adapt_dt:
             raise ValueError("adapt_dt is not supported; the fixed-dt CFL clamp of solver_advance applies")
 

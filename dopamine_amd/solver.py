@@ -37,7 +37,9 @@ class Params(ctypes.Structure):
                 ("cfl_max", _d), ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)),
                 ("precision", _i32), ("cg_variant", _i32), ("c_amd", _d), ("amd_wall_gradient", _i32),
                 ("boussinesq", _i32), ("alpha_T", _d), ("beta", _d), ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d),
-                ("t_scheme", _i32), ("patch_T_val", _P(_d))]
+                ("t_scheme", _i32), ("patch_T_val", _P(_d)), ("precond", _i32), ("learn_collect", _i32),
+                ("learn_capacity", _i32), ("learn_iters", _i32), ("learn_batch", _i32), ("learn_seed", _i32),
+                ("learn_lr", _d)]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -64,7 +66,9 @@ def build_abi(mesh, params, precision=0):
                 params.p_tol, params.p_max_iter, params.cfl_max, params.dt, arr(pv, _d), arr(uv.reshape(-1), _d),
                 int(precision), int(params.cg_variant), params.c_amd, int(params.amd_wall_gradient),
                 int(params.boussinesq), params.alpha_T, params.beta, params.T_ref, (_d * 3)(*params.g), params.Pr_t,
-                params.t_scheme_id(), arr(tv, _d))
+                params.t_scheme_id(), arr(tv, _d), 1 if params.precond.upper() == "LEARNED" else 0,
+                params.learn_collect, params.learn_capacity, params.learn_iters, params.learn_batch,
+                params.learn_seed, params.learn_lr)
     return md, pr, keep
 
 
@@ -87,6 +91,8 @@ def _bind(L):
     L.dopamine_solver_sync.argtypes = [ctypes.c_void_p]
     L.dopamine_solver_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _P(_i64), _P(_d)]
+    L.dopamine_solver_learned_info.argtypes = [ctypes.c_void_p, _d * 8]
+    L.dopamine_solver_learned_loss.argtypes = [ctypes.c_void_p, _P(_d), ctypes.c_int]
     return L
 
 
@@ -187,6 +193,23 @@ class FVSolver:
     def kernel_timing(self, kernel, enable=True):
         """Start (clearing the totals) or stop HIP-event timing of a kernel class."""
         self._check(self.L.dopamine_solver_kernel_timing(self.h, KERNELS[kernel], int(enable)), "kernel_timing")
+
+    LEARNED_INFO = ("phase", "collected", "samples", "learned_solves", "learned_iterations", "fallbacks",
+                    "jacobi_iterations_mean", "train_ms")
+
+    def learned_info(self):
+        """State of the learned preconditioner (precond LEARNED), or None."""
+        out = (_d * 8)()
+        if self.L.dopamine_solver_learned_info(self.h, out) != 0:
+            return None
+        d = dict(zip(self.LEARNED_INFO, list(out)))
+        d["phase"] = ("collecting", "in use", "off")[int(d["phase"])]
+        return d
+
+    def learned_loss(self, max_steps=100000):
+        out = np.zeros(max_steps)
+        m = self.L.dopamine_solver_learned_loss(self.h, out.ctypes.data_as(_P(_d)), max_steps)
+        return out[:max(0, m)]
 
     def kernel_time(self, kernel):
         """(launches, summed device milliseconds) since timing of the class was enabled."""

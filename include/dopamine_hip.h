@@ -67,6 +67,14 @@ typedef struct dopamine_params {
   double alpha_T, beta, T_ref, g[3], Pr_t;
   int32_t t_scheme;           /* temperature_scheme: 1 VANLEER_TVD, 2 KOREN_TVD, 3 CDS, 5 QUICK */
   const double *patch_T_val;  /* n_patches: fixedTemperature / inlet temperature */
+  /* Pressure preconditioner: 0 Jacobi (the CG passes above); 1 learned: the first learn_collect
+   * pressure solves run Jacobi CG and keep residual snapshots (learn_capacity of them), the
+   * message-passing network is then trained (learn_iters Adam steps of learn_batch residuals,
+   * rate learn_lr, seed learn_seed) and later solves run flexible CG with it, falling back to
+   * Jacobi CG when a solve fails or needs more iterations than the Jacobi solves did. */
+  int32_t precond;
+  int32_t learn_collect, learn_capacity, learn_iters, learn_batch, learn_seed;
+  double learn_lr;
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
@@ -97,6 +105,12 @@ double dopamine_solver_time(void *solver);
 int dopamine_solver_counters(void *solver, int64_t out[4]);
 /* {skip_nonortho, skip_grad_p, n_nonortho_corr}. */
 int dopamine_solver_nonortho_info(void *solver, int32_t out[3]);
+/* Learned preconditioner state: {phase (0 collecting, 1 in use, 2 switched off), solves
+ * collected, residual samples kept, learned solves, their CG iterations, fallbacks to Jacobi CG,
+ * mean Jacobi CG iterations per solve, training time in ms}. -1 when precond != 1. */
+int dopamine_solver_learned_info(void *solver, double out[8]);
+/* Mean training loss of each Adam step (up to max values); returns the count. */
+int dopamine_solver_learned_loss(void *solver, double *out, int max);
 /* Wait for all work queued by the solver. */
 int dopamine_solver_sync(void *solver);
 

@@ -24,11 +24,16 @@ int fvm_petsc_backend_finalize(void);
 /* Preconditioner selector (pressure_backend_petsc.c:534-539). "boomeramg", "gamg" and "gmres" are
  * accepted for input compatibility and map to the Jacobi CG of this backend; "mixed" selects
  * FP32-operator Jacobi-CG sweeps inside FP64 iterative refinement (same stopping test on the FP64
- * residual). A change of selector rebuilds the solve context. */
+ * residual); "fcg-learned" selects the learned preconditioner (dopamine_mpnn.h): the first solves
+ * run Jacobi CG and keep residual snapshots, the network is then trained on them and later solves
+ * run flexible CG with it, falling back to Jacobi CG when a solve fails or needs more iterations.
+ * A change of selector rebuilds the solve context. */
 void fvm_petsc_set_pressure_solver(const char *solver_name);
 /* PETSc-style option string (pressure_backend_petsc.c:543-551); "-ksp_max_it N" and
  * "-ksp_type pipecg|groppcg" (the one-pass single-reduction CG recurrence; default "cg", two
- * passes) are understood, other options are recorded and ignored. */
+ * passes) are understood, and with "fcg-learned" "-learn_collect N" (Jacobi solves before
+ * training, default 600), "-learn_capacity N" (128), "-learn_iters N" (300), "-learn_batch N" (4),
+ * "-learn_lr X" (3e-3), "-learn_seed N" (1); other options are recorded and ignored. */
 void fvm_petsc_insert_option(const char *opt_str);
 /* Node-local rank used to pick the HIP device (local_rank % device count) at initialize
  * (pressure_backend_petsc.c:563-565). */

@@ -12,7 +12,7 @@ import dopamine_amd._lib as _lib
 from dopamine_amd.params import SolverParams
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = ["dopamine_hip.h", "dopamine_mesh.h", "fvm_pressure_backend.h"]
+HEADERS = ["dopamine_hip.h", "dopamine_mesh.h", "fvm_pressure_backend.h", "dopamine_mpnn.h"]
 
 
 def declared(header):

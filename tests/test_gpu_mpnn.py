@@ -1,0 +1,187 @@
+"""Learned preconditioner on the GPU: forward pass and weight gradients against the oracle's
+statement of the same bfloat16 network (oracle/mpnn.py), Adam against the published update,
+training on captured Jacobi-CG residuals, and what the trained network gains inside flexible CG
+over Jacobi CG on the same systems."""
+import numpy as np
+import pytest
+
+from oracle import mpnn as om
+
+pytestmark = pytest.mark.gpu
+
+
+def channel_poisson(nx, ny, nz, stretch=2.0):
+    """Pressure-like operator of a 2pi x 2 x pi channel: periodic x and z, walls in y with tanh
+    stretching (singular: constant null space), ELL slot-major."""
+    Lx, Lz = 2.0 * np.pi, np.pi
+    eta = np.linspace(-1.0, 1.0, ny + 1)
+    yf = np.tanh(stretch * eta) / np.tanh(stretch) + 1.0
+    dy, yc = np.diff(yf), 0.5 * (yf[1:] + yf[:-1])
+    dx, dz = Lx / nx, Lz / nz
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    idx = lambda ii, jj, kk: (kk * ny + jj) * nx + ii
+    n = nx * ny * nz
+    nbr = -np.ones((6, n), dtype=np.int32)
+    coef = np.zeros((6, n))
+    nbr[0], coef[0] = idx((i + 1) % nx, j, k), dy[j] * dz / dx
+    nbr[1], coef[1] = idx((i - 1) % nx, j, k), dy[j] * dz / dx
+    nbr[2], coef[2] = idx(i, j, (k + 1) % nz), dy[j] * dx / dz
+    nbr[3], coef[3] = idx(i, j, (k - 1) % nz), dy[j] * dx / dz
+    up, dn = j + 1 < ny, j > 0
+    nbr[4, up] = idx(i[up], j[up] + 1, k[up])
+    coef[4, up] = dx * dz / (yc[j[up] + 1] - yc[j[up]])
+    nbr[5, dn] = idx(i[dn], j[dn] - 1, k[dn])
+    coef[5, dn] = dx * dz / (yc[j[dn]] - yc[j[dn] - 1])
+    return nbr, coef, coef.sum(axis=0)
+
+
+def random_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    w = om.init_weights(seed=seed, scale=scale)
+    for l in range(3):
+        w[f"b{l}"] = 0.1 * rng.standard_normal(32)
+    w["wo"] = scale * rng.standard_normal(32)
+    return om.flatten(w).astype(np.float32)
+
+
+def smooth_rhs(g, seed, passes=3):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal(g.diag.size)
+    for _ in range(passes):
+        u = 0.5 * u + 0.5 * g.aggregate(u[:, None])[:, 0]
+    return u - u.mean()
+
+
+@pytest.fixture(scope="module")
+def small():
+    from dopamine_amd.mpnn import LearnedPreconditioner
+    nbr, coef, diag = channel_poisson(16, 12, 7)  # 1344 cells: 42 tiles of 32
+    return LearnedPreconditioner(nbr, coef, diag, capacity=8, seed=5), om.Graph(nbr, coef, diag)
+
+
+def test_forward_matches_oracle(small):
+    net, g = small
+    flat = random_params(11)
+    net.params = flat
+    w = om.unflatten(flat.astype(np.float64))
+    r = np.random.default_rng(3).standard_normal(g.diag.size)
+    print("[gpu] mpnn forward 16x12x7", flush=True)
+    z, hs = net.forward(r, activations=True)
+    zo, hso = om.forward_bf16(w, g, r)
+    corr, corro = z - r / g.diag, zo - r / g.diag
+    err = np.linalg.norm(corr - corro) / np.linalg.norm(corro)
+    print(f"[gpu] mpnn forward: correction rel-L2 {err:.2e}", flush=True)
+    assert np.linalg.norm(corro) > 1e-3 * np.linalg.norm(r / g.diag)
+    assert err < 1e-2
+    for l in range(3):
+        same = np.mean(hs[l] == hso[l])
+        print(f"[gpu] mpnn h{l + 1}: {same:.4f} of the bfloat16 activations identical", flush=True)
+        assert same > 0.95
+        assert np.max(np.abs(hs[l] - hso[l])) <= 2.0 ** -5
+
+
+def test_untrained_network_is_jacobi(small):
+    net, g = small
+    flat = random_params(12)
+    flat[om.N_PARAMS - 32:] = 0.0  # readout wo
+    net.params = flat
+    r = np.random.default_rng(4).standard_normal(g.diag.size)
+    z = net.forward(r)
+    assert np.allclose(z, r / g.diag, rtol=1e-15, atol=0)
+
+
+def test_weight_gradients_match_oracle(small):
+    net, g = small
+    flat = random_params(13)
+    net.params = flat
+    w = om.unflatten(flat.astype(np.float64))
+    r = np.random.default_rng(5).standard_normal(g.diag.size)
+    print("[gpu] mpnn loss/gradient 16x12x7", flush=True)
+    loss, grad, hs = net.loss_grad(r, activations=True)
+    L, grads = om.backward_from(w, g, r, hs)
+    print(f"[gpu] mpnn loss {loss:.6e} oracle {L:.6e}", flush=True)
+    assert abs(loss - L) <= 1e-4 * L
+    gd = om.unflatten(grad)
+    for k in om.PARAM_ORDER:
+        e = np.linalg.norm(gd[k] - grads[k]) / np.linalg.norm(grads[k])
+        print(f"[gpu] mpnn grad {k}: rel-L2 {e:.2e}", flush=True)
+        assert e < 5e-3, k
+
+
+def test_adam_step_matches_published_update(small):
+    net, g = small
+    flat = random_params(14)
+    net.params = flat
+    r = np.random.default_rng(6).standard_normal(g.diag.size)
+    _, grad = net.loss_grad(r)
+    net.adam(2e-3, 1)
+    want = flat.astype(np.float64).copy()
+    om.adam_step(want, grad, np.zeros_like(want), np.zeros_like(want), 1, 2e-3)
+    got = net.params.astype(np.float64)
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-7)
+
+
+def test_trained_network_cuts_flexible_cg_iterations():
+    """Residuals of Jacobi CG solves on a 32x24x16 channel operator train the network; on new
+    right-hand sides flexible CG with it reaches the same 1e-8 residual in fewer iterations."""
+    from dopamine_amd.mpnn import LearnedPreconditioner
+    nbr, coef, diag = channel_poisson(32, 24, 16)
+    g = om.Graph(nbr, coef, diag)
+    net = LearnedPreconditioner(nbr, coef, diag, capacity=96, seed=1)
+    print("[gpu] mpnn training 32x24x16", flush=True)
+    for s in range(8):
+        b = smooth_rhs(g, s)
+        _, its, res, ok = net.solve(b, mode="collect", rtol=1e-8, nullspace=True)
+        assert ok
+    print(f"[gpu] mpnn {net.samples} residual samples", flush=True)
+    assert net.samples >= 40
+    hist = net.train(300, batch=4, lr=3e-3)
+    print(f"[gpu] mpnn loss first {hist[:10].mean():.3f} last {hist[-10:].mean():.3f}", flush=True)
+    assert hist[-20:].mean() < 0.7 * hist[:20].mean()
+    gains = []
+    for s in (100, 101, 102):
+        b = smooth_rhs(g, s)
+        xj, itj, resj, okj = net.solve(b, mode="jacobi", rtol=1e-8, nullspace=True)
+        xl, itl, resl, okl = net.solve(b, mode="learned", rtol=1e-8, nullspace=True)
+        bn = np.linalg.norm(b - b.mean())
+        print(f"[gpu] rhs {s}: Jacobi CG {itj} iterations, learned FCG {itl} (residual {resl / bn:.1e})", flush=True)
+        assert okj and okl
+        assert resl <= 2e-8 * bn
+        dx = (xl - xl.mean()) - (xj - xj.mean())
+        assert np.linalg.norm(dx) <= 1e-5 * np.linalg.norm(xj - xj.mean())
+        gains.append(itj / itl)
+    assert np.mean(gains) > 1.25, gains
+
+
+def test_solver_with_learned_preconditioner_tracks_the_jacobi_run():
+    """The full RK3W step with precond LEARNED: Jacobi CG solves of the first steps train the
+    network, later solves use flexible CG with it (or fall back); the fields stay within the
+    pressure tolerance of the plain Jacobi run, and the projection's divergence stays at its level."""
+    import dataclasses
+    from dopamine_amd import cases, params as P
+    from dopamine_amd.solver import FVSolver
+    case = cases.small_channel(32, 24, 16)
+    mesh = case.make_mesh()
+    out = {}
+    for pc in ("JACOBI", "LEARNED"):
+        prm = dataclasses.replace(case.params, precond=pc, learn_collect=30, learn_capacity=64, learn_iters=200,
+                                  learn_batch=4, learn_lr=3e-3)
+        s = FVSolver(mesh, prm)
+        P.initialize(s, case, mesh)
+        for _ in range(25):
+            s.advance()
+        out[pc] = (s.U.copy(), s.p.copy(), s.stats(), s.counters(), s.learned_info())
+        print(f"[gpu] channel 32x24x16 {pc}: {out[pc][3]} {out[pc][4]}", flush=True)
+        s.close()
+    info = out["LEARNED"][4]
+    assert out["JACOBI"][4] is None
+    assert info["collected"] >= 30 and info["samples"] > 0 and info["train_ms"] > 0
+    assert info["learned_solves"] > 0
+    uj, ul = out["JACOBI"][0], out["LEARNED"][0]
+    pj, pl = out["JACOBI"][1], out["LEARNED"][1]
+    du = np.linalg.norm(ul - uj) / np.linalg.norm(uj)
+    dp = np.linalg.norm((pl - pl.mean()) - (pj - pj.mean())) / np.linalg.norm(pj - pj.mean())
+    print(f"[gpu] learned vs Jacobi after 25 steps: U {du:.2e} p {dp:.2e}", flush=True)
+    assert du < 1e-6 and dp < 1e-5
+    assert out["LEARNED"][2]["max_abs_divU"] <= 5.0 * out["JACOBI"][2]["max_abs_divU"] + 1e-14
