@@ -144,3 +144,23 @@ def rayleigh_benard(nx=256, ny=256, nz=32, ra=1.0e7, pr=0.7):
                      boussinesq=True, alpha_T=1.0 / math.sqrt(ra * pr), beta=1.0, T_ref=0.5, g=(0.0, 0.0, -1.0),
                      temperature_scheme="VANLEER_TVD", patch_T={"bottom_wall": 1.0, "top_wall": 0.0}),
         ic="uniform", ic_uref=0.0, nsteps=200, extra={"t_noise": 1.0e-3, "seed": 1700})
+
+
+def most_channel(nx=16, ny=12, nz=8, dT=1.0, z0_m=1.0e-3):
+    """The channel with MOST rough walls and a stratified temperature: Vreman SGS, Boussinesq
+    (beta = 1/300, g = -9.81 along the wall normal y), van Leer TVD temperature, surface
+    temperature T_ref - dT (dT > 0: stable), seeded 1e-2 temperature noise."""
+    c = channel(nx, ny, nz)
+    c.name = f"most_channel{nx}x{ny}x{nz}"
+    p = c.params
+    p.wall_model = "most"
+    p.z0_m = z0_m
+    p.T_surface = 300.0 - dT
+    p.boussinesq = True
+    p.alpha_T = p.nu / 0.7
+    p.beta = 1.0 / 300.0
+    p.T_ref = 300.0
+    p.g = (0.0, -9.81, 0.0)
+    p.temperature_scheme = "VANLEER_TVD"
+    c.extra["t_noise"] = 1.0e-2
+    return c

@@ -494,6 +494,113 @@ struct WallModel {
   }
 };
 
+// ------------------------------------------------------------------ MOST surface layer
+// src/abl/mod_abl_wall.f90 and src/abl/mod_abl_profiles.f90 (Businger-Dyer corrections).
+constexpr double MOST_KAPPA = 0.41, MOST_G = 9.81, MOST_L_CLIP = 5000.0;
+
+// abl_psi_m (mod_abl_profiles.f90:51-68): stable -5 zeta, unstable Paulson (1970)
+DM_INLINE double most_psi_m(double zeta) {
+  if (fabs(zeta) < 1.0e-6) return 0.0;
+  if (zeta > 0.0) return -5.0 * zeta;
+  const double x = pow(1.0 - 16.0 * zeta, 0.25);
+  return 2.0 * log((1.0 + x) / 2.0) + log((1.0 + x * x) / 2.0) - 2.0 * atan(x) + 1.5707963267948966;
+}
+// abl_psi_h (:71-85)
+DM_INLINE double most_psi_h(double zeta) {
+  if (fabs(zeta) < 1.0e-6) return 0.0;
+  if (zeta > 0.0) return -5.0 * zeta;
+  return 2.0 * log((1.0 + sqrt(1.0 - 16.0 * zeta)) / 2.0);
+}
+// most_residuals (mod_abl_wall.f90:260-284)
+DM_INLINE void most_residuals(double us, double ts, double L, double u_par, double T_o, double T_s, double y,
+                              double z0m, double z0h, double* Fu, double* FT) {
+  double dm = log(y / z0m) - most_psi_m(y / L) + most_psi_m(z0m / L);
+  double dh = log(y / z0h) - most_psi_h(y / L) + most_psi_h(z0h / L);
+  if (fabs(dm) < SMALL) dm = copysign(SMALL, dm);
+  if (fabs(dh) < SMALL) dh = copysign(SMALL, dh);
+  *Fu = us - MOST_KAPPA * u_par / dm;
+  *FT = ts - MOST_KAPPA * (T_o - T_s) / dh;
+}
+// L_from (:226-240)
+DM_INLINE double most_L_from(double us, double ts, double T_ref) {
+  if (!(fabs(ts) > SMALL)) return MOST_L_CLIP;
+  double L = us * us * T_ref / (MOST_KAPPA * MOST_G * ts);
+  L = copysign(fmax(fabs(L), 0.001), L);
+  return fabs(L) > MOST_L_CLIP ? copysign(MOST_L_CLIP, L) : L;
+}
+// most_iterate (:155-223): relaxed Newton (factor 0.5) on (Fu, FT), finite-difference Jacobian
+// through L(u*, T*), at most 20 iterations, stop when |Fu| and |FT| < 1e-8.
+DM_INLINE void most_iterate(double u_par, double T_o, double T_s, double y, double z0m, double z0h, double L_init,
+                            double* us_out, double* ts_out, double* L_out) {
+  double us = fmax(SMALL, MOST_KAPPA * u_par / fmax(SMALL, log(y / z0m)));
+  double ts = 0.1 * (T_o - T_s);
+  double L = fabs(L_init) < SMALL ? MOST_L_CLIP : L_init;
+  for (int it = 0; it < 20; ++it) {
+    L = copysign(fmax(fabs(L), 0.001), L);
+    if (fabs(L) > MOST_L_CLIP) L = copysign(MOST_L_CLIP, L);
+    double Fu, FT, Fp, Tp;
+    most_residuals(us, ts, L, u_par, T_o, T_s, y, z0m, z0h, &Fu, &FT);
+    if (fabs(Fu) < 1.0e-8 && fabs(FT) < 1.0e-8) break;
+    const double du = fmax(1.0e-6, 1.0e-6 * fabs(us)), dT = fmax(1.0e-6, 1.0e-6 * fabs(ts));
+    most_residuals(us + du, ts, most_L_from(us + du, ts, T_o), u_par, T_o, T_s, y, z0m, z0h, &Fp, &Tp);
+    const double a11 = (Fp - Fu) / du, a21 = (Tp - FT) / du;
+    most_residuals(us, ts + dT, most_L_from(us, ts + dT, T_o), u_par, T_o, T_s, y, z0m, z0h, &Fp, &Tp);
+    const double a12 = (Fp - Fu) / dT, a22 = (Tp - FT) / dT;
+    const double den = a11 * a22 - a12 * a21;
+    if (fabs(den) < SMALL) break;
+    const double dus = -(a22 * Fu - a12 * FT) / den;
+    const double dts = -(a11 * FT - a21 * Fu) / den;
+    us = us + 0.5 * dus;
+    ts = ts + 0.5 * dts;
+    us = fmax(SMALL, fmin(us, 0.5 * u_par));
+    L = most_L_from(us, ts, T_o);
+  }
+  *us_out = us;
+  *ts_out = ts;
+  *L_out = L;
+}
+
+// abl_wall_model (mod_abl_wall.f90:64-153), one boundary face: the MOST fluxes from the owner
+// cell, tau_w = -u*^2 u_par/|u_par|, q_w = -u* T*, the face's L kept for its next solve. Partials
+// (L, 1) for the domain-mean L.
+struct MostWall {
+  DevMesh M;
+  const double *U, *T;
+  double T_const, z0m, z0h, Ts;
+  const double* Lcfg;
+  double *Lb, *tau, *qw;
+  DM_HD Red<2> operator()(int b) const {
+    Red<2> r;
+    r.v[0] = r.v[1] = 0.0;
+    if (M.bbc[b] != BC_WALL_MODEL) return r;
+    const int f = M.nint + b, o = M.bown[b];
+    const double* n = &M.nrm[3 * size_t(f)];
+    const double* u = &U[3 * size_t(o)];
+    const double T_o = T ? T[o] : T_const;
+    const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    const double mag = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    for (int k = 0; k < 3; ++k) up[k] = mag > SMALL ? up[k] / mag : 0.0;
+    double Lg = Lb[b];
+    if (fabs(Lg) < SMALL) Lg = *Lcfg;
+    double us, ts, L;
+    most_iterate(mag, T_o, Ts, M.wsy[b], z0m, z0h, Lg, &us, &ts, &L);
+    Lb[b] = L;
+    for (int k = 0; k < 3; ++k) tau[3 * size_t(b) + k] = -us * us * up[k];
+    qw[b] = -us * ts;
+    r.v[0] = L;
+    r.v[1] = 1.0;
+    return r;
+  }
+};
+struct MostLFin {
+  double* Lcfg;
+  int update;
+  DM_HD void operator()(const Red<2>& s) const {
+    if (update && s.v[1] > 0.0) *Lcfg = s.v[0] / s.v[1];
+  }
+};
+
 // ------------------------------------------------------------------ reconstruction schemes
 
 // QUICK / van Leer TVD / Koren TVD face values of the vector SoA path
@@ -897,6 +1004,7 @@ struct ExplicitRhsT {
   double *HT, *Tnew;
   double alpha, prt, dt, a, bcoef;
   int cds;
+  const double* qw;  // MOST wall heat flux per boundary face, or null
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -995,7 +1103,16 @@ struct ExplicitRhsT {
         vs = vs + flux * iv;
       }
     }
-    const double h = cv + vs;
+    double h = cv + vs;
+    // add_wall_heat_flux_to_rhs (src/solver/mod_fv_solver.f90:2279-2313): q_w A / V
+    if (qw)
+      for (int k = 0; k < cnt; ++k) {
+        const size_t s = size_t(k) * nc + c;
+        const int oth = M.aoth[s];
+        if (oth >= 0 || M.bbc[-1 - oth] != BC_WALL_MODEL) continue;
+        const int f = M.aface[s] >> 1;
+        h = h + qw[-1 - oth] * M.area[f] / M.vol[c];
+      }
     HT[c] = h;
     Tnew[c] = T[c] + dt * (a * h + bcoef * HT0[c]);
   }

@@ -38,6 +38,10 @@ FieldRef field(Solver* s, const std::string& name) {
   if (name == "rhs_p") return {s->rhs, nc};
   if (name == "H") return {s->H0, 3 * nc};  // after a step the last stage's H sits in H0
   if (name == "w_gp") return {s->gp, 3 * nc};
+  if (s->q_w) {
+    if (name == "q_w_bnd") return {s->q_w, nb};
+    if (name == "L_bnd") return {s->L_bnd, nb};
+  }
   if (s->bous) {
     if (name == "T") return {s->T, nc};
     if (name == "T_bnd") return {s->T_bnd, nb};
@@ -63,6 +67,8 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (!m || !p) throw std::runtime_error("null mesh or params");
     if (dopamine_device_count() <= 0) throw std::runtime_error("no HIP device available");
     if (p->wall_model_height > 0.0) throw std::runtime_error("wall_model_height > 0 (sampling walk) is not supported");
+    if (p->wall_model < 0 || p->wall_model > 2)
+      throw std::runtime_error("wall_model must be 0 (log-law), 1 (none) or 2 (MOST)");
     if (p->scheme < 1 || p->scheme > 5)
       throw std::runtime_error("advection scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3), KEP (4) or QUICK (5)");
     if (p->precision != 0 && p->precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");

@@ -140,6 +140,9 @@ class SolverT {
   int bous = 0;
   double *T = nullptr, *T2 = nullptr, *T_bnd = nullptr, *gT = nullptr, *gT_prev = nullptr, *HT = nullptr,
          *HT0 = nullptr, *phiT = nullptr, *patchT = nullptr;
+  // MOST wall model: heat flux and Obukhov length per boundary face, the configuration's L
+  double *q_w = nullptr, *L_bnd = nullptr, *L_cfg = nullptr;
+  double most_z0m = 0.0, most_z0h = 0.0;
   int n_amdw = 0;
   int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
@@ -223,6 +226,15 @@ class SolverT {
     }
     for (int q = 0; q < npatch; ++q)
       if (d.patch_bc[q] == BC_WALL_MODEL) n_wm++;
+    if (p.wall_model == 2) {
+      // MOST roughness (patch_roughness, src/abl/mod_abl_wall.f90:243-258, one value for every
+      // wall-model patch); the first cell must lie above the roughness layer (mod_fv_solver.f90:767-777)
+      most_z0m = std::fmax(p.most_z0_m, SMALL);
+      most_z0h = std::fmax(p.most_z0_h > 0.0 ? p.most_z0_h : most_z0m * std::exp(-2.0), SMALL);
+      for (int b = 0; b < nb; ++b)
+        if (bbc[b] == BC_WALL_MODEL && wsy[b] / most_z0m <= 1.0)
+          throw std::runtime_error("wall model: first cell inside the roughness layer (y_w / z0_m <= 1)");
+    }
     // refinement-interface faces (build_interface_diagnostics, mod_fv_solver.f90:685-718): more
     // correctors and a looser stagnation test
     for (int f = 0; f < nint; ++f)
@@ -599,6 +611,12 @@ class SolverT {
       }
     if (p.scheme == SCHEME_QUICK || p.scheme == SCHEME_VANLEER || p.scheme == SCHEME_KOREN)
       phiF = fresh<double>(3 * size_t(nf));
+    if (p.wall_model == 2) {
+      q_w = fresh<double>(std::max(1, nb));
+      L_bnd = fresh<double>(std::max(1, nb));
+      L_cfg = fresh<double>(1);
+      x.upload(L_cfg, &p.L_obukhov, 1);
+    }
     bous = p.boussinesq != 0;
     if (bous) {
       T = fresh<double>(nc); T2 = fresh<double>(nc); T_bnd = fresh<double>(std::max(1, nb));
@@ -756,7 +774,8 @@ class SolverT {
       default: break;
     }
     x.for_each(nc, ExplicitRhsT{M, T, T_bnd, gT, UF, nu_sgs, HT0, phiT, HT, T2, prm.alpha_T, prm.Pr_t, dt, RK3_A[k],
-                                RK3_B[k], prm.t_scheme == SCHEME_CDS ? 1 : 0});
+                                RK3_B[k], prm.t_scheme == SCHEME_CDS ? 1 : 0,
+                                (prm.wall_model == 2 && n_wm) ? q_w : nullptr});
   }
 
   void gradients(bool with_sgs, double sgs_dt) {
@@ -949,6 +968,10 @@ class SolverT {
       gradients(true, gdt);
       if (prm.wall_model == 1)
         x.zero(tau, 3 * size_t(std::max(1, nb)));
+      else if (n_wm && prm.wall_model == 2)
+        x.template reduce<2, false>(nb, MostWall{M, U, bous ? T : nullptr, prm.T_ref, most_z0m, most_z0h, prm.T_surface,
+                                                L_cfg, L_bnd, tau, q_w},
+                                    MostLFin{L_cfg, (prm.compute_L && k == 0) ? 1 : 0});
       else if (n_wm)
         x.for_each(nb, WallModel{M, U, tau, prm.nu});
       if (scheme == SCHEME_QUICK) x.for_each(nf, ReconFace<SCHEME_QUICK>{M, U, U_bnd, UF, grad, phiF});

@@ -11,7 +11,7 @@ import numpy as np
 SCHEME_IDS = {"VANLEER_TVD": 1, "VANLEER": 1, "KOREN_TVD": 2, "KOREN": 2, "CDS": 3, "CENTRAL": 3, "KEP": 4,
               "SKEW": 4, "QUICK": 5}
 SGS_IDS = {"NONE": 0, "VREMAN": 1, "SMAGORINSKY": 2, "AMD": 3}
-WALL_MODEL_IDS = {"loglaw": 0, "none": 1}
+WALL_MODEL_IDS = {"loglaw": 0, "none": 1, "most": 2}
 
 # Wray RK3 coefficients (src/mod_parameters.f90:64-71)
 RK3W_GAMMA1 = 8.0 / 15.0
@@ -32,6 +32,14 @@ class SolverParams:
     amd_wall_gradient: bool = True
     wall_model: str = "loglaw"
     wall_model_height: float = 0.0
+    # MOST surface layer (wall_model "most", src/abl/mod_abl_types.f90:22-27): roughness lengths
+    # (z0_h <= 0: z0_m exp(-2), the patch default), surface temperature, initial Obukhov length
+    # (0 = neutral), domain-mean L update at the first stage
+    z0_m: float = 0.1
+    z0_h: float = 0.0
+    T_surface: float = 300.0
+    L_obukhov: float = 0.0
+    compute_L: bool = True
     n_nonortho_corr: int = 0                 # 0 = auto
     pressure_min_cos_theta: float = 0.0
     rhie_chow_dual: bool = True
@@ -81,6 +89,8 @@ class SolverParams:
         if self.boussinesq and self.temperature_scheme.upper() not in ("CDS", "CENTRAL", "QUICK", "VANLEER_TVD",
                                                                          "VANLEER", "KOREN_TVD", "KOREN"):
             raise ValueError(f"temperature_scheme {self.temperature_scheme!r} not supported")
+        if self.wall_model not in WALL_MODEL_IDS:
+            raise ValueError(f"wall_model {self.wall_model!r} not supported (loglaw, most, none)")
         if self.precond.upper() not in ("JACOBI", "LEARNED"):
             raise ValueError(f"precond {self.precond!r} not supported (JACOBI, LEARNED)")
         if self.adapt_dt:

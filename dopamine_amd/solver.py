@@ -39,7 +39,8 @@ class Params(ctypes.Structure):
                 ("boussinesq", _i32), ("alpha_T", _d), ("beta", _d), ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d),
                 ("t_scheme", _i32), ("patch_T_val", _P(_d)), ("precond", _i32), ("learn_collect", _i32),
                 ("learn_capacity", _i32), ("learn_iters", _i32), ("learn_batch", _i32), ("learn_seed", _i32),
-                ("learn_lr", _d)]
+                ("learn_lr", _d), ("most_z0_m", _d), ("most_z0_h", _d), ("T_surface", _d), ("L_obukhov", _d),
+                ("compute_L", _i32)]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -68,7 +69,8 @@ def build_abi(mesh, params, precision=0):
                 int(params.boussinesq), params.alpha_T, params.beta, params.T_ref, (_d * 3)(*params.g), params.Pr_t,
                 params.t_scheme_id(), arr(tv, _d), 1 if params.precond.upper() == "LEARNED" else 0,
                 params.learn_collect, params.learn_capacity, params.learn_iters, params.learn_batch,
-                params.learn_seed, params.learn_lr)
+                params.learn_seed, params.learn_lr, params.z0_m, params.z0_h, params.T_surface, params.L_obukhov,
+                int(params.compute_L))
     return md, pr, keep
 
 
@@ -116,7 +118,8 @@ class FVSolver:
                        "U_bnd": 3 * max(1, mesh.n_bnd_faces), "p_bnd": max(1, mesh.n_bnd_faces),
                        "grad_U": 9 * mesh.n_cells, "U_face": mesh.n_faces, "tau_w_bnd": 3 * max(1, mesh.n_bnd_faces),
                        "rhs_p": mesh.n_cells, "H": 3 * mesh.n_cells, "w_gp": 3 * mesh.n_cells,
-                       "T": mesh.n_cells, "T_bnd": max(1, mesh.n_bnd_faces), "grad_T": 3 * mesh.n_cells}
+                       "T": mesh.n_cells, "T_bnd": max(1, mesh.n_bnd_faces), "grad_T": 3 * mesh.n_cells,
+                       "q_w_bnd": max(1, mesh.n_bnd_faces), "L_bnd": max(1, mesh.n_bnd_faces)}
 
     def _check(self, rc, what):
         if rc != 0:

@@ -44,7 +44,7 @@ typedef struct dopamine_params {
   int32_t sgs;               /* 0 none, 1 Vreman, 2 Smagorinsky, 3 AMD */
   double c_vreman;           /* reference C_VREMAN = 0.025 */
   double cs_smag;            /* 0.1 */
-  int32_t wall_model;        /* 0 log-law (Reichardt), 1 none */
+  int32_t wall_model;        /* 0 log-law (Reichardt), 1 none, 2 MOST (Monin-Obukhov, rough wall) */
   double wall_model_height;  /* <= 0: wall-adjacent cell */
   int32_t n_nonortho_corr;   /* 0 = auto */
   double min_cos_theta;
@@ -75,6 +75,12 @@ typedef struct dopamine_params {
   int32_t precond;
   int32_t learn_collect, learn_capacity, learn_iters, learn_batch, learn_seed;
   double learn_lr;
+  /* MOST wall model (wall_model 2, src/abl/mod_abl_wall.f90): roughness lengths (most_z0_h <= 0:
+   * most_z0_m e^-2), surface temperature, initial Obukhov length (0 = neutral), and whether the
+   * first stage of a step updates the domain-mean L. The cell temperature is T when boussinesq,
+   * else T_ref; the heat flux enters the temperature equation. */
+  double most_z0_m, most_z0_h, T_surface, L_obukhov;
+  int32_t compute_L;
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
@@ -86,7 +92,8 @@ const char *dopamine_last_error(void);
 
 /* Copy a host field in / out. Names: "U" (3*nc, component-fastest), "p" (nc), "nu_sgs", "U_bnd",
  * "p_bnd", "grad_U" (9*nc, grad(i,j) = du_j/dx_i at [9c+3j+i]), "U_face" (nf), "tau_w_bnd",
- * "rhs_p", "H", and with boussinesq "T" (nc), "T_bnd", "grad_T" (3*nc). set_field("U") and
+ * "rhs_p", "H", with boussinesq "T" (nc), "T_bnd", "grad_T" (3*nc), with wall_model 2 "q_w_bnd"
+ * and "L_bnd" (one per boundary face). set_field("U") and
  * set_field("T") re-apply the boundary conditions as solver_init does. */
 int dopamine_solver_set_field(void *solver, const char *name, const double *host);
 int dopamine_solver_get_field(void *solver, const char *name, double *host, int64_t len);

@@ -36,7 +36,8 @@ class _Params(ctypes.Structure):
                 ("rotational", ctypes.c_int), ("p_tol", _d), ("p_max_iter", ctypes.c_int), ("cfl_max", _d),
                 ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)), ("c_amd", _d),
                 ("amd_wall_gradient", ctypes.c_int), ("boussinesq", ctypes.c_int), ("alpha_T", _d), ("beta", _d),
-                ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d), ("t_scheme", ctypes.c_int), ("patch_T_val", _P(_d))]
+                ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d), ("t_scheme", ctypes.c_int), ("patch_T_val", _P(_d)),
+                ("most_z0_m", _d), ("most_z0_h", _d), ("T_surface", _d), ("L_obukhov", _d), ("compute_L", ctypes.c_int)]
 
 
 _lib = None
@@ -67,6 +68,11 @@ def lib():
         L.om_amd_point.restype = _d
         L.om_amd_point.argtypes = [_d * 9, _d * 9, _d]
         L.om_counters.argtypes = [ctypes.c_void_p, _i64 * 3]
+        L.om_most_iterate.argtypes = [_d, _d, _d, _d, _d, _d, _d, _d * 3]
+        L.om_psi_m.restype = _d
+        L.om_psi_m.argtypes = [_d]
+        L.om_psi_h.restype = _d
+        L.om_psi_h.argtypes = [_d]
         L.om_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int * 3]
         L.om_cg_jacobi.argtypes = [ctypes.c_int, ctypes.c_int, _P(_i32), _P(_i32), _P(_d), _P(_d), _P(_d), _P(_d),
                                    _d, _d, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(_d)]
@@ -115,9 +121,12 @@ class Solver:
                            int(params.use_rotational_projection), params.p_tol, params.p_max_iter, params.cfl_max,
                            params.dt, arr(pv, _d), arr(uv.reshape(-1), _d), params.c_amd,
                            int(params.amd_wall_gradient), int(params.boussinesq), params.alpha_T, params.beta,
-                           params.T_ref, (_d * 3)(*params.g), params.Pr_t, params.t_scheme_id(), arr(tv, _d))
+                           params.T_ref, (_d * 3)(*params.g), params.Pr_t, params.t_scheme_id(), arr(tv, _d),
+                           params.z0_m, params.z0_h, params.T_surface, params.L_obukhov, int(params.compute_L))
         self.params = params
         self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))
+        if not self.h:
+            raise ValueError("oracle om_create refused the set-up (MOST: first cell inside the roughness layer)")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -198,6 +207,21 @@ def amd_point(alpha, G, c_amd):
     a = np.asarray(alpha, dtype=np.float64)
     g = np.asarray(G, dtype=np.float64)
     return lib().om_amd_point((_d * 9)(*a.T.reshape(-1)), (_d * 9)(*g.T.reshape(-1)), c_amd)
+
+
+def most_iterate(u_par, T_o, T_surface, y_w, z0_m, z0_h, L_init):
+    """MOST surface-layer solve at one face: (u_star, T_star, L)."""
+    out = (_d * 3)()
+    lib().om_most_iterate(u_par, T_o, T_surface, y_w, z0_m, z0_h, L_init, out)
+    return tuple(out)
+
+
+def psi_m(zeta):
+    return lib().om_psi_m(zeta)
+
+
+def psi_h(zeta):
+    return lib().om_psi_h(zeta)
 
 
 def cg_jacobi(owner, neighbour, coeff, diag, b, x0, rtol, atol, max_it, nullspace):

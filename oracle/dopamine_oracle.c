@@ -60,6 +60,8 @@ struct om_solver {
   double *dt_flux, *dt_diff;
   /* temperature */
   double *T, *T_bnd, *grad_T, *gT_prev, *H_T, *H_T0, *conv_T, *visc_T;
+  /* MOST: per-face heat flux and Obukhov length, the configuration's L */
+  double *q_w, *L_bnd, L_cfg, most_z0h;
 };
 
 static double *dz(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
@@ -647,6 +649,116 @@ static void wall_model(om_solver *s) {
     const double sc = -tau / (mag + SMALL);
     for (int k = 0; k < 3; ++k) s->tau_w[3 * b + k] = sc * up[k];
   }
+}
+
+/* ---------------------------------------------------------------- MOST wall model */
+#define G_STD 9.81
+#define L_CLIP 5000.0
+#define KB_INV 2.0
+#define PI_D 3.14159265358979323846
+
+/* abl_psi_m (src/abl/mod_abl_profiles.f90:51-68): stable -5 zeta, unstable Paulson (1970) */
+double om_psi_m(double zeta) {
+  if (fabs(zeta) < 1.0e-6) return 0.0;
+  if (zeta > 0.0) return -5.0 * zeta;
+  const double x = pow(1.0 - 16.0 * zeta, 0.25);
+  return 2.0 * log((1.0 + x) / 2.0) + log((1.0 + x * x) / 2.0) - 2.0 * atan(x) + PI_D / 2.0;
+}
+
+/* abl_psi_h (:71-85) */
+double om_psi_h(double zeta) {
+  if (fabs(zeta) < 1.0e-6) return 0.0;
+  if (zeta > 0.0) return -5.0 * zeta;
+  const double y = pow(1.0 - 16.0 * zeta, 0.5);
+  return 2.0 * log((1.0 + y) / 2.0);
+}
+
+/* most_residuals (src/abl/mod_abl_wall.f90:260-284) */
+static void most_residuals(double u_star, double T_star, double L, double u_par, double T_o, double T_surf,
+                           double y_w, double z0_m, double z0_h, double *Fu, double *FT) {
+  double dm = log(y_w / z0_m) - om_psi_m(y_w / L) + om_psi_m(z0_m / L);
+  double dh = log(y_w / z0_h) - om_psi_h(y_w / L) + om_psi_h(z0_h / L);
+  if (fabs(dm) < SMALL) dm = copysign(SMALL, dm);
+  if (fabs(dh) < SMALL) dh = copysign(SMALL, dh);
+  *Fu = u_star - KAPPA * u_par / dm;
+  *FT = T_star - KAPPA * (T_o - T_surf) / dh;
+}
+
+/* L_from (:226-240): L = u*^2 T_ref / (kappa g T*), clipped to [0.001, L_CLIP] in magnitude */
+static double L_from(double u_star, double T_star, double T_ref) {
+  if (fabs(T_star) > SMALL) {
+    double L = u_star * u_star * T_ref / (KAPPA * G_STD * T_star);
+    L = copysign(fmax(fabs(L), 0.001), L);
+    if (fabs(L) > L_CLIP) L = copysign(L_CLIP, L);
+    return L;
+  }
+  return L_CLIP;
+}
+
+/* most_iterate (:155-223): relaxed Newton on (Fu, FT) with a finite-difference Jacobian */
+void om_most_iterate(double u_par, double T_o, double T_surf, double y_w, double z0_m, double z0_h, double L_init,
+                     double out[3]) {
+  const double T_ref = T_o;
+  double u_star = fmax(SMALL, KAPPA * u_par / fmax(SMALL, log(y_w / z0_m)));
+  double T_star = 0.1 * (T_o - T_surf);
+  double L = fabs(L_init) < SMALL ? L_CLIP : L_init;
+  for (int it = 0; it < 20; ++it) {
+    L = copysign(fmax(fabs(L), 0.001), L);
+    if (fabs(L) > L_CLIP) L = copysign(L_CLIP, L);
+    double Fu, FT, Fup, FTp;
+    most_residuals(u_star, T_star, L, u_par, T_o, T_surf, y_w, z0_m, z0_h, &Fu, &FT);
+    if (fabs(Fu) < 1.0e-8 && fabs(FT) < 1.0e-8) break;
+    const double du = fmax(1.0e-6, 1.0e-6 * fabs(u_star));
+    const double dT = fmax(1.0e-6, 1.0e-6 * fabs(T_star));
+    most_residuals(u_star + du, T_star, L_from(u_star + du, T_star, T_ref), u_par, T_o, T_surf, y_w, z0_m, z0_h,
+                   &Fup, &FTp);
+    const double a11 = (Fup - Fu) / du, a21 = (FTp - FT) / du;
+    most_residuals(u_star, T_star + dT, L_from(u_star, T_star + dT, T_ref), u_par, T_o, T_surf, y_w, z0_m, z0_h,
+                   &Fup, &FTp);
+    const double a12 = (Fup - Fu) / dT, a22 = (FTp - FT) / dT;
+    const double den = a11 * a22 - a12 * a21;
+    if (fabs(den) < SMALL) break;
+    const double dus = -(a22 * Fu - a12 * FT) / den;
+    const double dts = -(a11 * FT - a21 * Fu) / den;
+    u_star = u_star + 0.5 * dus;
+    T_star = T_star + 0.5 * dts;
+    u_star = fmax(SMALL, fmin(u_star, 0.5 * u_par));
+    L = L_from(u_star, T_star, T_ref);
+  }
+  out[0] = u_star;
+  out[1] = T_star;
+  out[2] = L;
+}
+
+/* abl_wall_model (src/abl/mod_abl_wall.f90:64-153): per wall-model face, the MOST fluxes from the
+ * owner cell; tau_w = -u*^2 u_par/|u_par|, q_w = -u* T*; at the first stage of a step the
+ * configuration's L becomes the mean over the wall faces (compute_L). */
+static void most_wall(om_solver *s, int update_L) {
+  double L_sum = 0.0;
+  int nwf = 0;
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b;
+    if (bc_of_face(s, f) != BC_WALL_MODEL) continue;
+    const int o = s->m.face_owner[f];
+    const double *n = &s->m.face_normal[3 * f];
+    const double y_w = s->ws_y[b];
+    const double T_o = s->p.boussinesq ? s->T[o] : s->p.T_ref;
+    const double *u = &s->U[3 * o];
+    const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    const double mag = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    for (int k = 0; k < 3; ++k) up[k] = mag > SMALL ? up[k] / mag : 0.0;
+    double L_guess = s->L_bnd[b];
+    if (fabs(L_guess) < SMALL) L_guess = s->L_cfg;
+    double r[3];
+    om_most_iterate(mag, T_o, s->p.T_surface, y_w, s->p.most_z0_m, s->most_z0h, L_guess, r);
+    s->L_bnd[b] = r[2];
+    L_sum = L_sum + r[2];
+    nwf++;
+    for (int k = 0; k < 3; ++k) s->tau_w[3 * b + k] = -r[0] * r[0] * up[k];
+    s->q_w[b] = -r[0] * r[1];
+  }
+  if (s->p.compute_L && update_L && nwf > 0) s->L_cfg = L_sum / (double)nwf;
 }
 
 /* ---------------------------------------------------------------- explicit terms */
@@ -1386,6 +1498,14 @@ static void explicit_rhs(om_solver *s) {
       const double w = s->m.face_area[f] / s->m.cell_volume[o];
       for (int k = 0; k < 3; ++k) s->H[3 * o + k] = s->H[3 * o + k] + s->tau_w[3 * b + k] * w;
     }
+  /* add_wall_heat_flux_to_rhs (:2279-2313): kinematic heat flux times area / volume */
+  if (bous && s->p.wall_model == OM_WM_MOST && s->n_wm_patches > 0)
+    for (int b = 0; b < s->nb; ++b) {
+      const int f = s->nint + b;
+      if (bc_of_face(s, f) != BC_WALL_MODEL) continue;
+      const int o = s->m.face_owner[f];
+      s->H_T[o] = s->H_T[o] + s->q_w[b] * s->m.face_area[f] / s->m.cell_volume[o];
+    }
 }
 
 /* ---------------------------------------------------------------- projection */
@@ -1806,7 +1926,7 @@ int om_project_once(om_solver *s, double dt) {
 /* advance_rk3w (src/solver/mod_fv_solver.f90:1180-1614) */
 /* Stage-start closures of solver_rk3w_stage (src/solver/mod_fv_solver.f90): velocity gradients,
  * SGS viscosity (stability cap from dt_stab) and wall-model traction. */
-static void stage_closures(om_solver *s, double dt_stab) {
+static void stage_closures(om_solver *s, double dt_stab, int stage) {
   compute_all_gradients(s);
   if (s->p.sgs == OM_SGS_VREMAN)
     vreman(s, dt_stab);
@@ -1818,13 +1938,15 @@ static void stage_closures(om_solver *s, double dt_stab) {
     memset(s->nu_sgs, 0, sizeof(double) * s->nc);
   if (s->p.wall_model == OM_WM_NONE)
     memset(s->tau_w, 0, sizeof(double) * 3 * (size_t)s->nb);
+  else if (s->p.wall_model == OM_WM_MOST)
+    most_wall(s, stage == 0);
   else
     wall_model(s);
 }
 
 int om_eval_closures(om_solver *s, double dt_stab) {
   refresh_bc(s);
-  stage_closures(s, dt_stab);
+  stage_closures(s, dt_stab, 1);
   return 0;
 }
 
@@ -1843,7 +1965,7 @@ static int advance_rk3w(om_solver *s, double dt) {
     const double gdt = RK3_G[k] * dt;
     s->time = time_n + RK3_C[k] * dt;
     refresh_bc(s);
-    stage_closures(s, gdt);
+    stage_closures(s, gdt, k);
     explicit_rhs(s);
     const double a = RK3_A[k], bb = RK3_B[k];
     for (int c = 0; c < nc; ++c) {
@@ -2236,6 +2358,20 @@ om_solver *om_create(const om_mesh *m, const om_params *p) {
     for (int i = 0; i < 3; ++i) d[i] = m->cell_centroid[3 * o + i] - m->face_centre[3 * fg + i];
     s->ws_y[b] = fabs(-(d[0] * n[0]) - d[1] * n[1] - d[2] * n[2]) + SMALL;
   }
+  s->q_w = dz(nb);
+  s->L_bnd = dz(nb);
+  s->L_cfg = p->L_obukhov;
+  if (p->wall_model == OM_WM_MOST) {
+    /* patch_roughness (src/abl/mod_abl_wall.f90:243-258) with one roughness for all wall patches;
+     * the first cell must lie above the roughness layer (mod_fv_solver.f90:767-777) */
+    s->p.most_z0_m = fmax(p->most_z0_m, SMALL);
+    s->most_z0h = fmax(p->most_z0_h > 0.0 ? p->most_z0_h : s->p.most_z0_m * exp(-KB_INV), SMALL);
+    for (int b = 0; b < nb; ++b)
+      if (bc_of_face(s, nint + b) == BC_WALL_MODEL && s->ws_y[b] / s->p.most_z0_m <= 1.0) {
+        om_free(s);
+        return NULL;
+      }
+  }
   om_apply_all_bc(s);
   return s;
 }
@@ -2248,6 +2384,8 @@ void om_apply_all_bc(om_solver *s) {
 
 void om_free(om_solver *s) {
   if (!s) return;
+  free(s->q_w);
+  free(s->L_bnd);
   double *dd[] = {s->U, s->P, s->p_old, s->nu_sgs, s->U_bnd, s->p_bnd, s->phi_prev, s->phi_prev_bnd, s->tau_w,
                   s->grad_U, s->gU_prev, s->face_flux, s->U_face, s->inv_vol, s->H, s->H0, s->conv, s->visc,
                   s->rhs_p, s->div_w, s->phi_it, s->gp, s->gp_rc, s->gp_cell, s->p_tot, s->p_tot_bnd, s->gp_prev,
@@ -2274,7 +2412,7 @@ double *om_array(om_solver *s, const char *name, int64_t *len) {
            {"face_flux", s->face_flux, nf}, {"tau_w_bnd", s->tau_w, 3 * nb}, {"w_gp", s->gp, 3 * nc},
            {"rhs_p", s->rhs_p, nc}, {"H", s->H, 3 * nc}, {"Gf", s->Gf, 9 * nc}, {"Tinv", s->Tinv, 9 * nc},
            {"ps_coeff", s->ps_coeff, s->npf}, {"ps_diag", s->ps_diag, nc}, {"T", s->T, nc}, {"T_bnd", s->T_bnd, nb},
-           {"grad_T", s->grad_T, 3 * nc}};
+           {"grad_T", s->grad_T, 3 * nc}, {"q_w_bnd", s->q_w, nb}, {"L_bnd", s->L_bnd, nb}};
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i)
     if (strcmp(t[i].n, name) == 0) {
       if (len) *len = t[i].l;

@@ -34,7 +34,7 @@ typedef struct om_mesh {
 enum { OM_SCHEME_VANLEER = 1, OM_SCHEME_KOREN = 2, OM_SCHEME_CDS = 3, OM_SCHEME_KEP = 4, OM_SCHEME_QUICK = 5 };
 enum { OM_GRAD_LS = 0, OM_GRAD_GG = 1 };
 enum { OM_SGS_NONE = 0, OM_SGS_VREMAN = 1, OM_SGS_SMAGORINSKY = 2, OM_SGS_AMD = 3 };
-enum { OM_WM_LOGLAW = 0, OM_WM_NONE = 1 };
+enum { OM_WM_LOGLAW = 0, OM_WM_NONE = 1, OM_WM_MOST = 2 };
 
 typedef struct om_params {
   double nu;
@@ -65,6 +65,10 @@ typedef struct om_params {
   double alpha_T, beta, T_ref, g[3], Pr_t;
   int t_scheme;               /* OM_SCHEME_* (CDS, QUICK, VANLEER, KOREN) */
   const double *patch_T_val;  /* n_patches, fixed-temperature / inlet value */
+  /* MOST wall model (wall_model == OM_WM_MOST): roughness lengths (z0_h <= 0: z0_m e^-2),
+   * surface temperature, initial Obukhov length (0 = neutral), domain-mean L update */
+  double most_z0_m, most_z0_h, T_surface, L_obukhov;
+  int compute_L;
 } om_params;
 
 typedef struct om_solver om_solver;
@@ -72,7 +76,8 @@ typedef struct om_solver om_solver;
 om_solver *om_create(const om_mesh *m, const om_params *p);
 void om_free(om_solver *s);
 /* Pointer to an internal array by name ("U","p","p_bnd","U_bnd","nu_sgs","grad_U","U_face",
- * "face_flux","tau_w_bnd","w_gp","rhs_p","T","T_bnd","grad_T"), with its length in doubles. */
+ * "face_flux","tau_w_bnd","w_gp","rhs_p","T","T_bnd","grad_T","q_w_bnd","L_bnd"), with its
+ * length in doubles. */
 double *om_array(om_solver *s, const char *name, int64_t *len);
 /* After U (and p) were written from outside: refresh boundary values as solver_init does. */
 void om_apply_all_bc(om_solver *s);
@@ -93,6 +98,14 @@ double om_wall_tau(double u_par, double y_w, double nu);
 /* AMD closure at one point (src/sgs/mod_amd.f90:31-66), column-major alpha(i,j) = du_j/dx_i and
  * G(i,j), no stability cap. */
 double om_amd_point(const double alpha[9], const double G[9], double c_amd);
+/* MOST surface-layer solve at one wall face (src/abl/mod_abl_wall.f90:155-223): speed u_par and
+ * cell temperature T_o at wall distance y_w over a surface at T_surface, roughness z0_m, z0_h,
+ * starting Obukhov length L_init. out = {u_star, T_star, L}. */
+void om_most_iterate(double u_par, double T_o, double T_surface, double y_w, double z0_m, double z0_h,
+                     double L_init, double out[3]);
+/* Businger-Dyer stability corrections psi_m, psi_h (src/abl/mod_abl_profiles.f90:51-85). */
+double om_psi_m(double zeta);
+double om_psi_h(double zeta);
 /* Pressure-operator face lists ("ps_owner", "ps_nb", "ps_skip"), 0-based. */
 int32_t *om_int_array(om_solver *s, const char *name, int64_t *len);
 /* Counters: total pressure solves, total CG iterations, last CG iteration count. */

@@ -22,6 +22,8 @@ double* field(Solver* s, const std::string& name, int64_t* n) {
   if (name == "tau_w_bnd") { *n = 3 * nb; return s->tau; }
   if (name == "rhs_p") { *n = nc; return s->rhs; }
   if (name == "w_gp") { *n = 3 * nc; return s->gp; }
+  if (s->q_w && name == "q_w_bnd") { *n = nb; return s->q_w; }
+  if (s->q_w && name == "L_bnd") { *n = nb; return s->L_bnd; }
   if (s->bous && name == "T") { *n = nc; return s->T; }
   if (s->bous && name == "T_bnd") { *n = nb; return s->T_bnd; }
   if (s->bous && name == "grad_T") { *n = 3 * nc; return s->gT; }

@@ -158,3 +158,19 @@ def test_gpu_tet_cavity_koren_amd_matches_oracle():
         d = rel_l2(a, g.get(name)[: a.size])
         print(f"[gpu] tets {name}: rel-L2 {d:.2e}", flush=True)
         assert d < 1e-10, name
+
+
+@pytest.mark.parametrize("dT", [2.0, -2.0])
+def test_gpu_most_channel_matches_oracle(dT):
+    """MOST rough walls over a stable (dT > 0) and an unstable surface, Boussinesq temperature with
+    the wall heat flux."""
+    case = cases.most_channel(16, 12, 8, dT=dT)
+    case.params.p_tol = 1e-12
+    mesh = case.make_mesh()
+    print(f"[gpu] MOST channel 16x12x8 dT {dT}", flush=True)
+    g, o = _pair(case, mesh, 4)
+    for name in ("U", "p", "T", "tau_w_bnd", "q_w_bnd", "L_bnd"):
+        a = o.array(name)
+        d = rel_l2(a, g.get(name)[: a.size])
+        print(f"[gpu] MOST {name}: rel-L2 {d:.2e}", flush=True)
+        assert d < 1e-10, name

@@ -125,3 +125,18 @@ def test_host_tet_cavity_matches_oracle():
     for name in ("U", "p", "nu_sgs"):
         a = o.array(name)
         assert rel_l2(a, h.get(name)[: a.size]) < 1e-10, name
+
+
+@pytest.mark.parametrize("dT", [2.0, -2.0])
+def test_host_most_channel_matches_oracle(dT):
+    """MOST rough walls (stable and unstable surface) with Boussinesq temperature and the wall heat
+    flux in the temperature equation."""
+    case = cases.most_channel(16, 12, 8, dT=dT)
+    case.params.p_tol = 1e-12
+    mesh = case.make_mesh()
+    h, o = _pair(case, mesh, 3)
+    for name in ("U", "p", "T", "nu_sgs", "tau_w_bnd", "q_w_bnd", "L_bnd"):
+        a = o.array(name)
+        assert rel_l2(a, h.get(name)[: a.size]) < 1e-10, name
+    qw = o.array("q_w_bnd")
+    assert np.count_nonzero(qw) > 0 and np.all(np.sign(qw[qw != 0]) == -np.sign(dT))
