@@ -62,6 +62,20 @@ def cg_pass_bytes(mesh, variant, mixed):
     return per_cell * mesh.n_cells + 16 * nfp, nfp
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the
+    same kernel and workload (profiles/), or None."""
+    import glob
+    for path in glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "*.md")):
+        for line in open(path):
+            line = line.strip()
+            if line.startswith("TRAFFIC "):
+                d = json.loads(line[len("TRAFFIC "):])
+                if d.get("kernel") == kernel and d.get("workload") == workload:
+                    return d["bytes_per_launch"]
+    return None
+
+
 def cpu_baseline(mesh, case, U, p, steps):
     import oracle
     o = oracle.Solver(mesh, case.params)
@@ -141,7 +155,9 @@ def main():
     nc = mesh.n_cells
     mixed = args.precision == "mixed"
     bytes_dir, nfp = cg_pass_bytes(mesh, args.cg_variant, mixed)
-    if learned:  # hidden layer: h read and written (32 bfloat16 per cell), FP32 edge weight + index per face side
+    info = solver.learned_info() if learned else None
+    timed_learned = learned and info is not None and info["phase"] != "off"
+    if timed_learned:  # hidden layer: h read and written (32 bfloat16 per cell), FP32 edge weight + index per face side
         bytes_dir = 128 * nc + 16 * nfp
     n_dir, ms_dir = ktime["cg_dir"]
     avg_s = ms_dir / max(n_dir, 1) / 1e3
@@ -168,12 +184,14 @@ def main():
                    "precision": args.precision, "cg_variant": args.cg_variant, "precond": args.precond,
                    "dt": case.params.dt},
         "roofline": {"bound": "hbm",
-                     "kernel": ("learned preconditioner hidden layer k_tile<1,mp::L1<6>> (MFMA bf16)" if learned else
+                     "kernel": ("learned preconditioner hidden layer k_tile<1,mp::L1<6>> (MFMA bf16)" if timed_learned else
                                 "CG FP32 sweep iteration k_pass<6,6,cgk::Fused32>" if mixed else
                                 "CG iteration pass k_pass<6,6,cgk::Fused>" if args.cg_variant == 1
                                 else "CG direction/operator pass k_pass<4,1,cgk::Dir>"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
+                     "traffic": (measured_traffic("FusedT<6>", f"channel {nx}x{ny}x{nz}")
+                                 if (not mixed and not timed_learned and args.cg_variant == 1) else None),
+                     "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
                      "launches": n_dir},
         "kernels_us": {k: (v[1] * 1e3 / v[0] if v[0] else None) for k, v in ktime.items()},
         "kernel_timing_pass_ms_per_step": ktimed_ms,

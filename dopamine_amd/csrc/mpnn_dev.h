@@ -43,7 +43,8 @@ constexpr int NPARAM = 4416;
 constexpr int KIN[3] = {3, 32, 32};
 
 constexpr int TBLOCK = 256;  // 4 waves, 32 cells per wave and tile
-constexpr int TGRID_MAX = 512;
+constexpr int TGRID_MAX = 512;   // output layer: its block partials feed the next pass
+constexpr int TGRID_BIG = 2048;  // input and hidden layers
 
 // tanh from its definition: tanh x = sign(x) (1 - e) / (1 + e), e = exp(-2|x|), with the Taylor
 // series x - x^3/3 + 2x^5/15 below |x| = 1/16 (truncation below 4e-9 relative there).
@@ -98,9 +99,9 @@ __device__ inline void slot(const Graph& G, int kk, int c, int& j, float& a) {
 
 // Epilogue shared by the layers: D[v] (v = 0..15) of lane (cell j, half) holds output channel
 // i = 8 (v / 4) + 4 half + v % 4; bias, tanh, bfloat16.
-__device__ inline void act(const f32x16& d, const float* bias, int half, float* h) {
+__device__ inline void act(const f32x16& d, const float* bias, float* h) {
 #pragma unroll
-  for (int v = 0; v < 16; ++v) h[v] = fl(bf(tanh_f(d[v] + bias[8 * (v >> 2) + 4 * half + (v & 3)])));
+  for (int v = 0; v < 16; ++v) h[v] = fl(bf(tanh_f(d[v] + bias[v])));
 }
 __device__ inline void store_h(__bf16* out, int j, int half, const float* h) {
 #pragma unroll
@@ -116,6 +117,28 @@ struct ScalS {
   double s, sinv;
 };
 
+// A wave's operands that do not change from tile to tile: its lanes' weight chunks (K blocks),
+// and the bias (and readout) values of the 16 output channels the lane holds.
+template <int KB>
+struct WReg {
+  bf16x8 a[KB];
+  float bias[16], wo[16];
+};
+template <int KB>
+__device__ inline WReg<KB> load_w(const bf16x8* Wl, const float* bias, const __bf16* wo, int lane) {
+  WReg<KB> r;
+  const int half = lane >> 5, row = (lane & 31) * (2 * KB);
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) r.a[kb] = Wl[row + kb * 2 + half];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int i = 8 * (v >> 2) + 4 * half + (v & 3);
+    r.bias[v] = bias[i];
+    r.wo[v] = wo ? fl(wo[i]) : 0.0f;
+  }
+  return r;
+}
+
 // Layer 0: features of the residual, their aggregate, one MFMA (K = 16: 6 inputs and zeros).
 // In the solve (sc != null) the setup is also the FCG convergence test of iteration k.
 template <int WC>
@@ -129,6 +152,8 @@ struct L0 {
   CgScalars* sc;     // null outside a solve
   int k;
   using Scal = ScalS;
+  using W_t = WReg<1>;
+  DM_DEV W_t wload(int lane) const { return load_w<1>(W.W0, W.bias, nullptr, lane); }
   DM_DEV bool setup(const Red<1>& t, bool leader, Scal& s) const {
     const double rr = t.v[0];
     if (sc) {
@@ -146,7 +171,7 @@ struct L0 {
     if (leader) *s_out = s.s;
     return true;
   }
-  DM_DEV void tile(int t, int lane, const Scal& s) const {
+  DM_DEV void tile(int t, int lane, const Scal& s, const W_t& w) const {
     const int half = lane >> 5, j = t * 32 + (lane & 31);
     const bool ok = j < G.n;
     const int jc = ok ? j : G.n - 1;
@@ -173,9 +198,9 @@ struct L0 {
       bx[5] = bf(G.mx2[jc]);
     }
     f32x16 d = {};
-    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.W0[(lane & 31) * 2 + half], bx, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.a[0], bx, d, 0, 0, 0);
     float h[16];
-    act(d, W.bias, half, h);
+    act(d, w.bias, h);
     if (ok) {
       store_h(hout, j, half, h);
       if (xsave) *reinterpret_cast<bf16x8*>(xsave + size_t(j) * 16 + 8 * half) = bx;
@@ -193,17 +218,39 @@ __device__ inline void mid_inputs(const Graph& G, const __bf16* hin, int jc, int
   float m[16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) m[e] = 0.0f;
-  for (int kk = 0; kk < Slots<WC>::count(G.w); ++kk) {
-    int nb;
-    float av;
-    slot(G, kk, jc, nb, av);
-    const __bf16* nr = hin + size_t(nb) * NCH + 8 * half;
-    const bf16x8 g0 = *reinterpret_cast<const bf16x8*>(nr);
-    const bf16x8 g1 = *reinterpret_cast<const bf16x8*>(nr + 16);
+  if constexpr (WC > 0) {
+    // all slot indices, then all neighbour rows, in flight together; summed in slot order
+    int nb[WC];
+    float av[WC];
+    bf16x8 g0[WC], g1[WC];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      m[e] = m[e] + av * fl(g0[e]);
-      m[8 + e] = m[8 + e] + av * fl(g1[e]);
+    for (int kk = 0; kk < WC; ++kk) slot(G, kk, jc, nb[kk], av[kk]);
+#pragma unroll
+    for (int kk = 0; kk < WC; ++kk) {
+      const __bf16* nr = hin + size_t(nb[kk]) * NCH + 8 * half;
+      g0[kk] = *reinterpret_cast<const bf16x8*>(nr);
+      g1[kk] = *reinterpret_cast<const bf16x8*>(nr + 16);
+    }
+#pragma unroll
+    for (int kk = 0; kk < WC; ++kk)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        m[e] = m[e] + av[kk] * fl(g0[kk][e]);
+        m[8 + e] = m[8 + e] + av[kk] * fl(g1[kk][e]);
+      }
+  } else {
+    for (int kk = 0; kk < G.w; ++kk) {
+      int nb;
+      float av;
+      slot(G, kk, jc, nb, av);
+      const __bf16* nr = hin + size_t(nb) * NCH + 8 * half;
+      const bf16x8 g0 = *reinterpret_cast<const bf16x8*>(nr);
+      const bf16x8 g1 = *reinterpret_cast<const bf16x8*>(nr + 16);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        m[e] = m[e] + av * fl(g0[e]);
+        m[8 + e] = m[8 + e] + av * fl(g1[e]);
+      }
     }
   }
 #pragma unroll
@@ -213,11 +260,10 @@ __device__ inline void mid_inputs(const Graph& G, const __bf16* hin, int jc, int
   }
 }
 
-__device__ inline f32x16 mid_mfma(const bf16x8* Wl, int lane, const bf16x8* b) {
-  const int half = lane >> 5, row = (lane & 31) * 8;
+__device__ inline f32x16 mid_mfma(const WReg<4>& w, const bf16x8* b) {
   f32x16 d = {};
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Wl[row + kb * 2 + half], b[kb], d, 0, 0, 0);
+  for (int kb = 0; kb < 4; ++kb) d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.a[kb], b[kb], d, 0, 0, 0);
   return d;
 }
 
@@ -231,16 +277,18 @@ struct L1 {
   __bf16* xsave;  // [n][64] or null
   const CgScalars* sc;
   using Scal = cgk::NoScal;
+  using W_t = WReg<4>;
+  DM_DEV W_t wload(int lane) const { return load_w<4>(W.W1, W.bias + NCH, nullptr, lane); }
   DM_DEV bool setup(const Red<1>&, bool, Scal&) const { return !(sc && sc->done); }
-  DM_DEV void tile(int t, int lane, const Scal&) const {
+  DM_DEV void tile(int t, int lane, const Scal&, const W_t& w) const {
     const int half = lane >> 5, j = t * 32 + (lane & 31);
     const bool ok = j < G.n;
     const int jc = ok ? j : G.n - 1;
     bf16x8 b[4];
     mid_inputs<WC>(G, hin, jc, half, b);
-    const f32x16 d = mid_mfma(W.W1, lane, b);
+    const f32x16 d = mid_mfma(w, b);
     float h[16];
-    act(d, W.bias + NCH, half, h);
+    act(d, w.bias, h);
     if (ok) {
       store_h(hout, j, half, h);
       if (xsave)
@@ -266,23 +314,25 @@ struct L2 {
   struct Scal {
     double s;
   };
+  using W_t = WReg<4>;
+  DM_DEV W_t wload(int lane) const { return load_w<4>(W.W2, W.bias + 2 * NCH, W.wo, lane); }
   DM_DEV bool setup(const Red<1>&, bool, Scal& s) const {
     if (sc && sc->done) return false;
     s.s = *s_in;
     return true;
   }
-  DM_DEV Red<5> tile(int t, int lane, const Scal& s) const {
+  DM_DEV Red<5> tile(int t, int lane, const Scal& s, const W_t& w) const {
     const int half = lane >> 5, j = t * 32 + (lane & 31);
     const bool ok = j < G.n;
     const int jc = ok ? j : G.n - 1;
     bf16x8 b[4];
     mid_inputs<WC>(G, hin, jc, half, b);
-    const f32x16 d = mid_mfma(W.W2, lane, b);
+    const f32x16 d = mid_mfma(w, b);
     float h[16];
-    act(d, W.bias + 2 * NCH, half, h);
+    act(d, w.bias, h);
     float op = 0.0f;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) op = op + fl(W.wo[8 * (v >> 2) + 4 * half + (v & 3)]) * h[v];
+    for (int v = 0; v < 16; ++v) op = op + w.wo[v] * h[v];
     // the partner half's 16 channels; every lane takes part in the exchange (a + b == b + a, so
     // both halves hold the same o)
     const float o = op + __shfl_xor(op, 32);
@@ -333,10 +383,11 @@ __global__ void __launch_bounds__(TBLOCK) k_tile(int ntiles, int tpb, F f, const
   if (!f.setup(tot, b == 0 && threadIdx.x == 0, sc)) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int t0 = b * tpb, t1 = min(ntiles, t0 + tpb);
+  const typename F::W_t w = f.wload(lane);
   if constexpr (NOUT > 0) {
     Red<NOUT> acc = red_identity<NOUT, false>();
     for (int t = t0 + wave; t < t1; t += TBLOCK / 64) {
-      const Red<NOUT> v = f.tile(t, lane, sc);
+      const Red<NOUT> v = f.tile(t, lane, sc, w);
 #pragma unroll
       for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
     }
@@ -345,7 +396,7 @@ __global__ void __launch_bounds__(TBLOCK) k_tile(int ntiles, int tpb, F f, const
 #pragma unroll
       for (int k = 0; k < NOUT; ++k) out_part[size_t(k) * gridDim.x + b] = acc.v[k];
   } else {
-    for (int t = t0 + wave; t < t1; t += TBLOCK / 64) f.tile(t, lane, sc);
+    for (int t = t0 + wave; t < t1; t += TBLOCK / 64) f.tile(t, lane, sc, w);
   }
 }
 
@@ -886,12 +937,13 @@ class LearnedPc : public FcgPc, public CgCapture {
   void layers(const double* r, const double* rold, double* z, CgScalars* sc, int k, const double* rr_part, int rr_g,
               double* out_part, bool train) {
     const int nt = (n_ + 31) / 32, gt = out_grid(n_), tpb = (nt + gt - 1) / gt;
+    const int gb = std::max(1, std::min(TGRID_BIG, (nt + 3) / 4)), tpbb = (nt + gb - 1) / gb;
     by_width(G_.w, [&](auto wc) {
       constexpr int WC = decltype(wc)::value;
-      x_->launch(k_tile<1, L0<WC>>, gt, TBLOCK, nt, tpb,
+      x_->launch(k_tile<1, L0<WC>>, gb, TBLOCK, nt, tpbb,
                  L0<WC>{G_, W_, r, h1_, train ? x0s_ : nullptr, s_, sc, k}, rr_part, rr_g, (double*)nullptr);
       x_->tbegin(K_CG_DIR);  // the hidden layer is the timed kernel of a preconditioned iteration
-      x_->launch(k_tile<1, L1<WC>>, gt, TBLOCK, nt, tpb, L1<WC>{G_, W_, h1_, h2_, train ? x1s_ : nullptr, sc},
+      x_->launch(k_tile<1, L1<WC>>, gb, TBLOCK, nt, tpbb, L1<WC>{G_, W_, h1_, h2_, train ? x1s_ : nullptr, sc},
                  (const double*)nullptr, 0, (double*)nullptr);
       x_->tend(K_CG_DIR);
       x_->launch(k_tile<1, L2<WC>>, gt, TBLOCK, nt, tpb,
