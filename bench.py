@@ -53,11 +53,11 @@ def cg_pass_bytes(mesh, variant, mixed):
     FP64 two-pass recurrence: diag, z, p_old read, p, q written (5 x 8 B);
     FP64 single-reduction recurrence (the whole iteration): diag, p, s, x, r, w read and p, s, x,
     r, w written (11 x 8 B);
-    mixed precision (FP32 single-reduction sweep): diag, p, s, r, w read and p, s, r, w written
-    in FP32 (9 x 4 B), x read and written in FP64 (2 x 8 B); FP32 coefficients."""
+    mixed precision (FP32 single-reduction sweep): 1/diag, p, s, r, w and the correction e read
+    and p, s, r, w, e written, all FP32 (11 x 4 B); FP32 coefficients."""
     nfp = mesh.n_int_faces + int(np.count_nonzero(mesh.periodic_pair >= 0)) // 2
     if mixed:
-        return 52 * mesh.n_cells + 12 * nfp, nfp
+        return 44 * mesh.n_cells + 12 * nfp, nfp
     per_cell = 88 if variant == 1 else 40
     return per_cell * mesh.n_cells + 16 * nfp, nfp
 

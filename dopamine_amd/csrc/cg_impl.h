@@ -454,23 +454,29 @@ using Fused = FusedT<0>;
 // ---- mixed precision: FP64 iterative refinement around FP32 single-reduction CG sweeps.
 // The outer loop keeps x and the true residual r = b' - A x in FP64 (FP64 operator); each sweep
 // solves A e = r in FP32 (FP32 operator and state, FP64 dot-product accumulation) to a relative
-// reduction tau, adding alpha p straight into the FP64 x.
+// reduction tau, accumulating the correction e in FP32; the next sweep's start folds it into the
+// FP64 x (x += e), and the residual pass reads x + e meanwhile.
 
-// r = b' - A x in FP64; partials (0, r.r) in the layout Init produces
+// r = b' - A (x + e) in FP64; partials (0, r.r) in the layout Init produces
 struct IrResid {
   EllOp A;
   const double *bw, *x;
+  const float* e;
   double* r;
   CgScalars* sc;
   using Scal = NoScal;
-  DM_HD bool setup(const Red<1>&, bool, Scal&) const { return !sc->done; }
+  DM_HD bool setup(const Red<1>&, bool leader, Scal&) const {
+    if (sc->done) return false;
+    if (leader) sc->its = sc->its + sc->inner_its;
+    return true;
+  }
   DM_HD Red<2> operator()(int c, const Scal&) const {
     const int n = A.n;
-    double ax = A.diag[c] * x[c];
+    double ax = A.diag[c] * (x[c] + double(e[c]));
     for (int k = 0; k < A.w; ++k) {
       const int j = A.nbr[size_t(k) * n + c];
       if (j < 0) continue;
-      ax = ax - A.coef[size_t(k) * n + c] * x[j];
+      ax = ax - A.coef[size_t(k) * n + c] * (x[j] + double(e[j]));
     }
     const double rc = bw[c] - ax;
     r[c] = rc;
@@ -481,12 +487,24 @@ struct IrResid {
   }
 };
 
+// x += e, e = 0 (the FP32 correction of a finished sweep)
+struct IrFold {
+  double* x;
+  float* e;
+  DM_HD void operator()(int c) const {
+    x[c] = x[c] + double(e[c]);
+    e[c] = 0.0f;
+  }
+};
+
 // Start of a sweep: the outer test ||r|| <= ttol on the FP64 residual, then the FP32 packed state
 // {r, 0, w, 0} with w = A D^-1 r, partials as W0. The first sweep also fixes ttol from ||b'||.
 struct IrStart {
   JacobiScaled J;
   const double* r64;
   St4f* st;
+  double* x;
+  float* e;
   CgScalars* sc;
   int sweep, max_sweeps;
   double tau;
@@ -519,6 +537,8 @@ struct IrStart {
       wc = wc - J.chat32[q] * float(r64[j]);
     }
     st[c] = St4f{rc, 0.0f, wc, 0.0f};
+    x[c] = x[c] + double(e[c]);
+    e[c] = 0.0f;
     Red<6> o;
     o.v[0] = double(rc) * double(rc);
     o.v[1] = double(rc) * double(uc);
@@ -530,14 +550,14 @@ struct IrStart {
   }
 };
 
-// One FP32 single-reduction iteration of a sweep (as FusedT), x accumulated in FP64.
+// One FP32 single-reduction iteration of a sweep (as FusedT); the correction e accumulates in FP32.
 template <int WC>
 struct Fused32T {
   static constexpr int kSlots = WC;
   JacobiScaled J;
   const St4f* st0;
   St4f* st1;
-  double* x;
+  float* e;  // the sweep's correction
   CgScalars* sc;
   int k;
   struct Scal {
@@ -588,7 +608,7 @@ struct Fused32T {
     const float uc = o.r * di - um;
     const float pc = uc + beta * o.p;
     const float sc_ = o.w + beta * o.s;
-    x[c] = x[c] + s.alpha * double(pc);
+    e[c] = e[c] + alpha * pc;
     const float rc = o.r - alpha * sc_;
     float wc = rc;
     for (int kk = 0; kk < J.w; ++kk) {
@@ -609,14 +629,14 @@ struct Fused32T {
       int cc[CG_U];
       St4f o[CG_U];
       float di[CG_U], ch[CG_U][WC], g[CG_U][WC];
-      double xo[CG_U];
+      float eo[CG_U];
       int jj[CG_U][WC];
 #pragma unroll
       for (int u = 0; u < CG_U; ++u) {
         cc[u] = (c0 + u * nt < hi) ? c0 + u * nt : c0;
         o[u] = st0[cc[u]];
         di[u] = J.dinv32[cc[u]];
-        xo[u] = x[cc[u]];
+        eo[u] = e[cc[u]];
 #pragma unroll
         for (int kk = 0; kk < WC; ++kk) {
           const bool live = kk < w;
@@ -644,7 +664,7 @@ struct Fused32T {
 #pragma unroll
         for (int kk = 0; kk < WC; ++kk) wc = wc - ch[u][kk] * g[u][kk];
         if (c0 + u * nt < hi) {
-          x[cc[u]] = xo[u] + s.alpha * double(pc);
+          e[cc[u]] = eo[u] + alpha * pc;
           st1[cc[u]] = St4f{rc, sc_, wc, pc};
           const Red<6> v = sums(rc, rc * di[u], wc);
 #pragma unroll
@@ -710,8 +730,10 @@ struct FcgStart {
 };
 
 // z -= mean(z) on a singular operator; beta; p = z + beta p_old; q = A p (neighbours' p formed on
-// the fly); partial p.q
-struct FcgDir {
+// the fly); partial p.q. The device form (WC > 0) reads WC operator slots branch-free, as FusedT.
+template <int WC>
+struct FcgDirT {
+  static constexpr int kSlots = WC;
   EllOp A;
   const double *z, *pold;
   double *pnew, *q;
@@ -747,7 +769,37 @@ struct FcgDir {
     o.v[0] = pc * qc;
     return o;
   }
+  DM_HD Red<1> run(int lo, int hi, int t, int nt, const Scal& s) const {
+    const int n = A.n, w = A.w;
+    const bool first = k == 0;
+    Red<1> acc = red_identity<1, false>();
+    for (int c = lo + t; c < hi; c += nt) {
+      int jj[WC > 0 ? WC : 1];
+      double cf[WC > 0 ? WC : 1], pj[WC > 0 ? WC : 1];
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) {
+        const bool live = kk < w;
+        const size_t q_ = size_t(live ? kk : 0) * n + c;
+        const int j = A.nbr[q_];
+        const double h = A.coef[q_];
+        jj[kk] = (live && j >= 0) ? j : c;
+        cf[kk] = (live && j >= 0) ? h : 0.0;
+      }
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk)
+        pj[kk] = first ? z[jj[kk]] - s.zm : (z[jj[kk]] - s.zm) + s.beta * pold[jj[kk]];
+      const double pc = first ? z[c] - s.zm : (z[c] - s.zm) + s.beta * pold[c];
+      double qc = A.diag[c] * pc;
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) qc = qc - cf[kk] * pj[kk];
+      pnew[c] = pc;
+      q[c] = qc;
+      acc.v[0] = acc.v[0] + pc * qc;
+    }
+    return acc;
+  }
 };
+using FcgDir = FcgDirT<0>;
 
 // alpha = (z.r) / (p.q); x += alpha p; r_old = r; r -= alpha q; partial r.r
 struct FcgUpd {
@@ -821,10 +873,11 @@ class CgT {
     if (!x_) return;
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
-                    (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_})
+                    (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_, (void*)e32_})
       if (p) x_->release(p);
     st_[0] = st_[1] = nullptr;
     st32_[0] = st32_[1] = nullptr;
+    e32_ = nullptr;
     chat_ = dinv_ = nullptr;
     chat32_ = dinv32_ = nullptr;
     ro_ = fpart_ = nullptr;
@@ -871,7 +924,9 @@ class CgT {
         for (int i = 0; i < batch; ++i, ++k) {
           fcg_->apply(k, r_, ro_, z_, sc_, RR, g, fpart_);  // times its hidden layer as K_CG_DIR
           xx.tbegin(K_CG_UPD);
-          xx.template pass<5, 1>(n, FcgDir{A, z_, pold, pnew, q_, sc_, k}, fpart_, gt, D);
+          by_width(A.w, [&](auto wc) {
+            xx.template pass<5, 1>(n, FcgDirT<decltype(wc)::value>{A, z_, pold, pnew, q_, sc_, k}, fpart_, gt, D);
+          });
           xx.template pass<1, 1>(n, FcgUpd{x, r_, ro_, pnew, q_, sc_, k}, D, g, RR);
           xx.tend(K_CG_UPD);
           std::swap(pold, pnew);
@@ -883,36 +938,40 @@ class CgT {
     } else if (mixed_) {
       double* F[2] = {U[0], U[0] + 6 * size_t(g)};
       const double* in = P1;
-      int total = 0, ibatch = std::max(4, predict_in_);
+      int total = 0;
+      xx.zero(e32_, size_t(n));
       for (int sweep = 0;; ++sweep) {
-        xx.template pass<2, 6>(n, IrStart{J, r_, st32_[0], sc_, sweep, MAX_SWEEPS, tau_}, in, g, F[0]);
+        xx.template pass<2, 6>(n, IrStart{J, r_, st32_[0], x, e32_, sc_, sweep, MAX_SWEEPS, tau_}, in, g, F[0]);
         int kin = 0;
+        // sized from the same sweep of the previous solve; later batches a quarter of the
+        // iterations so far (a pass after the stop costs a few microseconds, a host poll more)
+        int ibatch = std::max(8, std::min(maxit + 1, predict_sw_[std::min(sweep, MAX_SWEEPS)]));
         for (;;) {
           for (int i = 0; i < ibatch; ++i, ++kin) {
             const int a = kin & 1, b2 = a ^ 1;
             xx.tbegin(K_CG_DIR);
             by_width(J.w, [&](auto wc) {
-              xx.template pass<6, 6>(n, Fused32T<decltype(wc)::value>{J, st32_[a], st32_[b2], x, sc_, kin}, F[a],
+              xx.template pass<6, 6>(n, Fused32T<decltype(wc)::value>{J, st32_[a], st32_[b2], e32_, sc_, kin}, F[a],
                                      g, F[b2]);
             });
             xx.tend(K_CG_DIR);
           }
           xx.download(&h, sc_, 1);
           if (h.done || h.inner_done || kin > maxit + 1) break;
-          ibatch = std::max(2, std::min(32, (h.inner_its + 7) / 8));
+          ibatch = std::max(8, std::min(64, (h.inner_its + 3) / 4));
         }
         xx.tcommit(K_CG_DIR, h.inner_its);
+        predict_sw_[std::min(sweep, MAX_SWEEPS)] = std::max(1, h.inner_its + 1);
         if (h.done) break;
         total += h.inner_its;
-        predict_in_ = h.inner_its + 1;
-        ibatch = predict_in_;
         last_sweeps_ = sweep + 1;
         h.its = total;
-        xx.upload(&sc_->its, &total, 1);
-        xx.template pass<1, 2>(n, IrResid{A, bw_, x, r_, sc_}, nullptr, 0, P1);
+        // IrResid adds the sweep's inner iterations to the device-side count
+        xx.template pass<1, 2>(n, IrResid{A, bw_, x, e32_, r_, sc_}, nullptr, 0, P1);
         in = P1;
         if (sweep + 1 > MAX_SWEEPS) break;
       }
+      xx.for_each(n, IrFold{x, e32_});  // the last sweep's correction, when no sweep start followed
       h.its = total;
     } else if (single_) {
       double* F[2] = {U[0], U[0] + 6 * size_t(g)};  // 6 x g each, over the U blocks
@@ -1006,6 +1065,7 @@ class CgT {
     if (mixed_ && !st32_[0]) {
       st32_[0] = xx.template alloc<St4f>(n_);
       st32_[1] = xx.template alloc<St4f>(n_);
+      e32_ = xx.template alloc<float>(n_);
     }
     JacobiScaled J;
     J.n = A.n;
@@ -1019,6 +1079,7 @@ class CgT {
   }
   St4* st_[2] = {nullptr, nullptr};
   St4f* st32_[2] = {nullptr, nullptr};
+  float* e32_ = nullptr;  // mixed precision: the sweep's FP32 correction
   double *chat_ = nullptr, *dinv_ = nullptr;
   float *chat32_ = nullptr, *dinv32_ = nullptr;
   const double* scaled_of_ = nullptr;
@@ -1039,7 +1100,8 @@ class CgT {
   double *ro_ = nullptr, *fpart_ = nullptr;
   int fpart_g_ = 0;
   double tau_ = 1e-3;
-  int predict_in_ = 24, last_sweeps_ = 0;
+  int predict_sw_[MAX_SWEEPS + 1] = {};
+  int last_sweeps_ = 0;
   double* part_ = nullptr;
   CgScalars* sc_ = nullptr;
   std::vector<double> hpart_;

@@ -1644,4 +1644,36 @@ struct FluxDivergence {
   }
 };
 
+// cell_velocity_divergence (src/solver/mod_fv_solver.f90:3582-3607): div u from the
+// gc-interpolated cell velocities (boundary faces: U_bnd), owner +, neighbour -, per cell in
+// face order; |div| or |div| V.
+struct CellDivergence {
+  DevMesh M;
+  const double *U, *Ub;
+  int weighted;
+  DM_HD Red<1> operator()(int c) const {
+    const int nc = M.nc;
+    double d = 0.0;
+    const int cnt = M.nA[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
+      double uf[3];
+      if (oth >= 0) {
+        const int o = side ? oth : c, nb = side ? c : oth;
+        const double gc = M.gc[f];
+        for (int i = 0; i < 3; ++i) uf[i] = gc * U[3 * size_t(o) + i] + (1.0 - gc) * U[3 * size_t(nb) + i];
+      } else {
+        for (int i = 0; i < 3; ++i) uf[i] = Ub[3 * size_t(-1 - oth) + i];
+      }
+      const double* sf = &M.sf[3 * size_t(f)];
+      const double flux = uf[0] * sf[0] + uf[1] * sf[1] + uf[2] * sf[2];
+      d = side ? d - flux * M.iv[c] : d + flux * M.iv[c];
+    }
+    Red<1> r;
+    r.v[0] = weighted ? fabs(d) * M.vol[c] : fabs(d);
+    return r;
+  }
+};
+
 }  // namespace dm

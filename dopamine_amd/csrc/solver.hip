@@ -197,6 +197,22 @@ int dopamine_solver_kernel_time(void* h, int kernel, int64_t* launches, double* 
   });
 }
 
+int dopamine_solver_cell_divergence(void* h, double out[2]) {
+  return guard([&] {
+    static_cast<Solver*>(h)->cell_divergence(out);
+    return 0;
+  });
+}
+
+int dopamine_solver_stats_csv(void* h, const char* path, int step, double dt, double* E0, int append) {
+  return guard([&] {
+    if (!path || !E0) throw std::runtime_error("null path or E0");
+    if (static_cast<Solver*>(h)->stats_csv(path, step, dt, E0, append) != 0)
+      throw std::runtime_error(std::string("cannot open ") + path);
+    return 0;
+  });
+}
+
 int dopamine_solver_learned_info(void* h, double out[8]) {
   return guard([&] {
     auto* lp = dynamic_cast<dm::mp::LearnedPolicy*>(static_cast<Solver*>(h)->policy.get());

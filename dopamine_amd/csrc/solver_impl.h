@@ -13,6 +13,7 @@
 // CG convergence).
 #pragma once
 
+#include <cstdio>
 #include <cstdlib>
 
 #include <algorithm>
@@ -1079,6 +1080,49 @@ class SolverT {
     out[5] = h.st[6] / V;
     out[6] = h.st[10];
     out[7] = h.st[11] / V;
+  }
+
+  // The cell-centred divergence of solver_divergence_metrics (mod_fv_solver.f90:3641-3713): the
+  // velocity boundary values refreshed without gradients first, as there. out = {max |div|,
+  // volume-averaged |div|}.
+  void cell_divergence(double out[2]) {
+    bc_velocity(0, nullptr);
+    x.template reduce<1, true>(nc, CellDivergence{M, U, U_bnd, 0}, sk::StatsFinDiv{sc, 10});
+    x.template reduce<1, false>(nc, CellDivergence{M, U, U_bnd, 1}, sk::StatsFinDiv{sc, 11});
+    x.template reduce<7, false>(nc, StatsCells{M, U, grad, nu_sgs, prm.nu}, sk::StatsFin7{sc});
+    SolverScalars h;
+    x.download(&h, sc, 1);
+    out[0] = h.st[10];
+    out[1] = h.st[11] / h.st[1];
+  }
+
+  // One row of {prefix}_stats.csv (stats_sample, src/stats/mod_stats.f90:161-252): step, time, E,
+  // eps, E/E0, eps_ref (= eps), Umax, face-flux and cell divergence (max, volume mean), Ub,
+  // tauw_mean, Cf_bulk, nu_sgs_mean, nu_sgs/nu, dt; E0 <= 0 on entry takes this row's E. The
+  // header is written when the file is new (append = 0 or the file is absent).
+  int stats_csv(const char* path, int step, double dt, double* E0, int append) {
+    double s8[8], cd[2];
+    stats(s8);
+    cell_divergence(cd);
+    std::FILE* fp = append ? std::fopen(path, "r") : nullptr;
+    const bool fresh = fp == nullptr;
+    if (fp) std::fclose(fp);
+    fp = std::fopen(path, fresh ? "w" : "a");
+    if (!fp) return -1;
+    if (fresh)
+      std::fprintf(fp, "step,time,E,eps,E_over_E0,eps_ref,Umax,max_abs_divU,volavg_abs_divU,max_abs_divU_cell,"
+                       "volavg_abs_divU_cell,Ub,tauw_mean,Cf_bulk,nu_sgs_mean,nu_sgs_over_nu,dt\n");
+    if (!(*E0 > SMALL)) *E0 = s8[0];
+    const double E_norm = *E0 > SMALL ? s8[0] / *E0 : 1.0;
+    const double cf = s8[4] / std::fmax(0.5 * s8[3] * s8[3], SMALL);
+    const double sgs = s8[5] / std::fmax(prm.nu, SMALL);
+    const double v[16] = {time, s8[0], s8[1], E_norm, s8[1], s8[2], s8[6], s8[7], cd[0], cd[1],
+                          s8[3], s8[4], cf, s8[5], sgs, dt};
+    std::fprintf(fp, "%d", step);
+    for (double q : v) std::fprintf(fp, ",%18.9E", q);  // Fortran ES18.9
+    std::fprintf(fp, "\n");
+    std::fclose(fp);
+    return 0;
   }
 };
 

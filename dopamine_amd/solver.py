@@ -94,6 +94,8 @@ def _bind(L):
     L.dopamine_solver_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _P(_i64), _P(_d)]
     L.dopamine_solver_learned_info.argtypes = [ctypes.c_void_p, _d * 8]
+    L.dopamine_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
+    L.dopamine_solver_stats_csv.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _d, _P(_d), ctypes.c_int]
     L.dopamine_solver_learned_loss.argtypes = [ctypes.c_void_p, _P(_d), ctypes.c_int]
     return L
 
@@ -196,6 +198,18 @@ class FVSolver:
     def kernel_timing(self, kernel, enable=True):
         """Start (clearing the totals) or stop HIP-event timing of a kernel class."""
         self._check(self.L.dopamine_solver_kernel_timing(self.h, KERNELS[kernel], int(enable)), "kernel_timing")
+
+    def cell_divergence(self):
+        out = (_d * 2)()
+        self._check(self.L.dopamine_solver_cell_divergence(self.h, out), "cell_divergence")
+        return {"max_abs_divU_cell": out[0], "volavg_abs_divU_cell": out[1]}
+
+    def stats_csv(self, path, step, dt, E0=0.0, append=True):
+        """Append a row of the reference's stats CSV; returns the reference energy E0 used."""
+        e0 = _d(E0)
+        self._check(self.L.dopamine_solver_stats_csv(self.h, str(path).encode(), step, dt, ctypes.byref(e0),
+                                                     int(append)), "stats_csv")
+        return e0.value
 
     LEARNED_INFO = ("phase", "collected", "samples", "learned_solves", "learned_iterations", "fallbacks",
                     "jacobi_iterations_mean", "train_ms")

@@ -108,6 +108,14 @@ double dopamine_solver_compute_dt(void *solver);
  * max_abs_divU, volavg_abs_divU} (src/stats/mod_stats.f90:271-336). */
 int dopamine_solver_stats(void *solver, double out[8]);
 double dopamine_solver_time(void *solver);
+/* Cell-centred velocity divergence of solver_divergence_metrics (mod_fv_solver.f90:3641-3713):
+ * {max |div u|, volume-averaged |div u|}, after a gradient-free velocity boundary refresh. */
+int dopamine_solver_cell_divergence(void *solver, double out[2]);
+/* Append one row to the stats CSV at path in the format of src/stats/mod_stats.f90:68-70, 241-249
+ * (header when the file is new or append == 0). dt is the requested step (main.f90:452 passes it
+ * before solver_advance's CFL clamp). *E0 <= 0 takes this row's E as the reference energy and
+ * returns it. */
+int dopamine_solver_stats_csv(void *solver, const char *path, int step, double dt, double *E0, int append);
 /* {pressure solves, total CG iterations, last CG iteration count, refinement sweeps}. */
 int dopamine_solver_counters(void *solver, int64_t out[4]);
 /* {skip_nonortho, skip_grad_p, n_nonortho_corr}. */

@@ -58,6 +58,7 @@ def lib():
         L.om_project_once.argtypes = [ctypes.c_void_p, _d]
         L.om_advance.argtypes = [ctypes.c_void_p, _d, _P(_d)]
         L.om_stats.argtypes = [ctypes.c_void_p, _d * 8]
+        L.om_divergence_cell.argtypes = [ctypes.c_void_p, _d * 2]
         L.om_int_array.restype = _P(_i32)
         L.om_int_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _P(_i64)]
         L.om_time.restype = _d
@@ -185,6 +186,11 @@ class Solver:
         out = (_d * 8)()
         lib().om_stats(self.h, out)
         return dict(zip(STATS_COLUMNS, list(out)))
+
+    def cell_divergence(self):
+        out = (_d * 2)()
+        lib().om_divergence_cell(self.h, out)
+        return {"max_abs_divU_cell": out[0], "volavg_abs_divU_cell": out[1]}
 
     def counters(self):
         out = (_i64 * 3)()

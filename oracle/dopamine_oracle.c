@@ -2111,6 +2111,37 @@ void om_stats(om_solver *s, double out[8]) {
   out[7] = as / V;
 }
 
+/* cell_velocity_divergence inside solver_divergence_metrics (src/solver/mod_fv_solver.f90:3582-3607,
+ * 3641-3713): velocity boundary values refreshed without gradients, then per face the
+ * gc-interpolated (interior) or boundary velocity flux, owner +, neighbour -. */
+void om_divergence_cell(om_solver *s, double out[2]) {
+  apply_velocity_bc(s, 0, NULL);
+  double *div = s->div_diag;
+  memset(div, 0, sizeof(double) * s->nc);
+  for (int f = 0; f < s->nf; ++f) {
+    const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
+    double uf[3];
+    if (nb >= 0) {
+      const double gc = s->m.face_gc[f];
+      for (int k = 0; k < 3; ++k) uf[k] = gc * s->U[3 * o + k] + (1.0 - gc) * s->U[3 * nb + k];
+    } else {
+      for (int k = 0; k < 3; ++k) uf[k] = s->U_bnd[3 * (f - s->nint) + k];
+    }
+    const double *sf = &s->m.face_sf[3 * f];
+    const double flux = uf[0] * sf[0] + uf[1] * sf[1] + uf[2] * sf[2];
+    div[o] = div[o] + flux * s->inv_vol[o];
+    if (nb >= 0) div[nb] = div[nb] - flux * s->inv_vol[nb];
+  }
+  double mx = 0.0, as = 0.0, V = 0.0;
+  for (int c = 0; c < s->nc; ++c) {
+    mx = fmax(mx, fabs(div[c]));
+    as = as + fabs(div[c]) * s->m.cell_volume[c];
+    V = V + s->m.cell_volume[c];
+  }
+  out[0] = mx;
+  out[1] = as / V;
+}
+
 /* ---------------------------------------------------------------- set-up */
 
 /* invert_sym3 (src/solver/mod_fv_solver.f90:3466-3491), row-major output */

@@ -88,6 +88,8 @@ int om_advance(om_solver *s, double dt_req, double *dt_used);
  * max_abs_divU, volavg_abs_divU}. */
 void om_stats(om_solver *s, double out[8]);
 double om_time(const om_solver *s);
+/* Cell-centred divergence metrics {max, volume mean} (src/solver/mod_fv_solver.f90:3641-3713). */
+void om_divergence_cell(om_solver *s, double out[2]);
 /* Boundary refresh, then the closures a RK3W stage starts with (velocity gradients into "grad_U",
  * SGS viscosity into "nu_sgs" with stability cap 0.4 V^(2/3) / dt_stab, wall traction into
  * "tau_w_bnd"), on the current U, without advancing. */

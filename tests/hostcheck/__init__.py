@@ -34,6 +34,8 @@ def lib():
         L.hc_solver_time.argtypes = [ctypes.c_void_p]
         L.hc_solver_counters.argtypes = [ctypes.c_void_p, _i64 * 4]
         L.hc_solver_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int32 * 3]
+        L.hc_solver_stats_csv.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _d, _P(_d), ctypes.c_int]
+        L.hc_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
         _lib = L
     return _lib
 
@@ -75,6 +77,17 @@ class HostSolver:
     def set_temperature(self, T):
         self.array("T")[:] = np.asarray(T, dtype=np.float64).reshape(-1)
         lib().hc_solver_apply_all_bc(self.h)
+
+    def cell_divergence(self):
+        out = (_d * 2)()
+        lib().hc_solver_cell_divergence(self.h, out)
+        return {"max_abs_divU_cell": out[0], "volavg_abs_divU_cell": out[1]}
+
+    def stats_csv(self, path, step, dt, E0=0.0, append=True):
+        e0 = _d(E0)
+        if lib().hc_solver_stats_csv(self.h, str(path).encode(), step, dt, ctypes.byref(e0), int(append)) != 0:
+            raise OSError(f"cannot write {path}")
+        return e0.value
 
     def project_once(self, dt):
         return lib().hc_solver_project_once(self.h, dt)

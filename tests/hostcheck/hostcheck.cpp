@@ -46,6 +46,10 @@ void* hc_solver_create(const dopamine_mesh_desc* m, const dopamine_params* p) {
   }
 }
 void hc_solver_destroy(void* h) { delete static_cast<Solver*>(h); }
+int hc_solver_stats_csv(void* h, const char* path, int step, double dt, double* E0, int append) {
+  return static_cast<Solver*>(h)->stats_csv(path, step, dt, E0, append);
+}
+void hc_solver_cell_divergence(void* h, double out[2]) { static_cast<Solver*>(h)->cell_divergence(out); }
 double* hc_solver_field(void* h, const char* name, int64_t* n) { return field(static_cast<Solver*>(h), name, n); }
 void hc_solver_apply_all_bc(void* h) { static_cast<Solver*>(h)->apply_all_bc(); }
 int hc_solver_project_once(void* h, double dt) { return static_cast<Solver*>(h)->project_once(dt); }

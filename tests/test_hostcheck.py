@@ -10,7 +10,11 @@ from dopamine_amd import cases
 from helpers import compare_stats, load_reference_csv, rel_l2, run_case
 from hostcheck import HostSolver
 
-REG = cases.regression_cases()
+
+
+def fresh(name):
+    """A new copy of a regression case (tests change its parameters)."""
+    return cases.regression_cases()[name]
 
 
 def _pair(case, mesh, nsteps):
@@ -23,7 +27,7 @@ def _pair(case, mesh, nsteps):
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("name", ["tgv_skew", "tgv_skew_chk", "hydrowall", "hydro", "pdrop"])
 def test_host_kernels_match_oracle_fields(name, variant):
-    case = REG[name]
+    case = fresh(name)
     case.params.p_tol = 1e-12
     case.params.cg_variant = variant
     mesh = case.make_mesh()
@@ -35,7 +39,7 @@ def test_host_kernels_match_oracle_fields(name, variant):
 
 
 def test_host_kernels_match_reference_csv():
-    case = REG["tgv_skew"]
+    case = fresh("tgv_skew")
     mesh = case.make_mesh()
     _, fails = compare_stats(load_reference_csv("tgv_skew"), run_case(HostSolver(mesh, case.params), case, mesh))
     assert not fails, fails[:5]
@@ -81,7 +85,7 @@ def test_host_tgv_sgs_matches_oracle(sgs):
 
 
 def test_host_rotational_off_and_cds_match_oracle():
-    case = REG["pdrop"]
+    case = fresh("pdrop")
     case.params.p_tol = 1e-12
     case.params.use_rotational_projection = False
     mesh = case.make_mesh()
@@ -91,7 +95,7 @@ def test_host_rotational_off_and_cds_match_oracle():
 
 @pytest.mark.parametrize("name", ["tgv_skew", "hydrowall"])
 def test_host_mixed_precision_matches_oracle_to_solver_tolerance(name):
-    case = REG[name]
+    case = fresh(name)
     case.params.p_tol = 1e-8
     mesh = case.make_mesh()
     h, o = HostSolver(mesh, case.params, precision=1), oracle.Solver(mesh, case.params)
@@ -105,7 +109,7 @@ def test_host_mixed_precision_matches_oracle_to_solver_tolerance(name):
 @pytest.mark.parametrize("scheme", ["QUICK", "VANLEER_TVD", "KOREN_TVD"])
 @pytest.mark.parametrize("name", ["tgv_skew", "pdrop"])
 def test_host_reconstruction_schemes_match_oracle(scheme, name):
-    case = REG[name]
+    case = fresh(name)
     case.params.p_tol = 1e-12
     case.params.advection_scheme = scheme
     mesh = case.make_mesh()

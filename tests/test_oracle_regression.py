@@ -9,12 +9,16 @@ import oracle
 from dopamine_amd import cases
 from helpers import compare_stats, load_reference_csv, run_case
 
-REG = cases.regression_cases()
+
+
+def fresh(name):
+    """A new copy of a regression case (tests change its parameters)."""
+    return cases.regression_cases()[name]
 
 
 @pytest.mark.parametrize("name", ["tgv", "tgv_skew", "tgv_skew_chk", "hydrowall", "hydro", "pdrop"])
 def test_oracle_matches_reference_csv(name):
-    case = REG[name]
+    case = fresh(name)
     mesh = case.make_mesh()
     rows = run_case(oracle.Solver(mesh, case.params), case, mesh)
     ref = load_reference_csv(name)
@@ -27,7 +31,7 @@ def test_oracle_matches_reference_csv(name):
 
 def test_planted_sign_error_is_caught():
     """A wrong body-force sign (a plausible mistake) must fail the hydrowall pin."""
-    case = REG["hydrowall"]
+    case = fresh("hydrowall")
     case.params.f_const = tuple(-f for f in case.params.f_const)
     mesh = case.make_mesh()
     rows = run_case(oracle.Solver(mesh, case.params), case, mesh)
