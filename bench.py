@@ -4,10 +4,14 @@
 Workload: turbulent channel 2pi x 2 x pi, 128 x 96 x 64 = 786,432 hex cells, tanh wall-normal
 stretching (factor 2), periodic in x and z, Re_tau 395 (nu = 1/395, f_x = 1), KEP advection,
 Vreman SGS, log-law wall model (kappa 0.41, B 5.2), laminar parabolic profile with a seeded 10%
-uniform perturbation, pressure by Jacobi CG to relative residual 1e-8, fixed dt = 1e-3.
+uniform perturbation, pressure to relative residual 1e-8, fixed dt = 1e-3.
 
 A step is one solver_advance (three RK3W stages with their pressure solves) with every field
-resident in HBM. `value` = cells x timed steps / wall time of the timed steps. The roofline entry
+resident in HBM. By default the measured path is the mixed-precision pressure solve (FP32 CG
+sweeps inside FP64 iterative refinement, same FP64 residual test) and the plain FP64 Jacobi-CG
+path is timed the same way beside it (`plain_fp64`, `speedup_vs_plain_fp64`); `--precond
+learned` measures flexible CG with the learned preconditioner instead. `value` = cells x timed
+steps / wall time of the timed steps. The roofline entry
 is for the dominant kernel (the CG iteration pass), from HIP events that its dispatches stamp on
 the solver's stream over a further run of the same number of steps; the CPU baseline is the FP64
 C restatement of the same algorithm run single-threaded on one step of the same workload.
@@ -30,7 +34,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mesh", type=int, nargs=3, default=(128, 96, 64), metavar=("NX", "NY", "NZ"))
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed",
+                    help="mixed: FP32 CG sweeps inside FP64 iterative refinement (the default measured path); "
+                         "fp64: the plain method")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 run that the line reports beside a mixed/learned measurement")
     ap.add_argument("--cg-variant", type=int, choices=(0, 1), default=1,
                     help="pressure CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction")
     ap.add_argument("--precond", choices=("jacobi", "learned"), default="jacobi",
@@ -112,26 +120,39 @@ def main():
     t0 = time.perf_counter()
     mesh = case.make_mesh()
     log(f"mesh {nx}x{ny}x{nz}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
-    solver = FVSolver(mesh, case.params, precision=1 if args.precision == "mixed" else 0)
-    P.initialize(solver, case, mesh)
-    solver.sync()
-    log("initial projections done")
-    for i in range(args.warmup):
-        solver.advance()
-        log(f"warmup step {i + 1}/{args.warmup}: {solver.counters()['last']} CG iterations in the last solve")
-    solver.sync()
-    if learned:
-        log(f"learned preconditioner after warm-up: {solver.learned_info()}")
 
-    it0 = solver.counters()["iterations"]
-    solver.sync()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        solver.advance()
-    solver.sync()
-    elapsed = time.perf_counter() - t0
-    its = solver.counters()["iterations"] - it0
-    log(f"timed {args.steps} steps in {elapsed:.3f} s, {its} CG iterations")
+    def timed_run(params, precision, tag):
+        solver = FVSolver(mesh, params, precision=precision)
+        P.initialize(solver, case, mesh)
+        solver.sync()
+        log(f"[{tag}] initial projections done")
+        for i in range(args.warmup):
+            solver.advance()
+            log(f"[{tag}] warmup step {i + 1}/{args.warmup}: {solver.counters()['last']} CG iterations in the last solve")
+        solver.sync()
+        if params.precond.upper() == "LEARNED":
+            log(f"[{tag}] learned preconditioner after warm-up: {solver.learned_info()}")
+        it0 = solver.counters()["iterations"]
+        solver.sync()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            solver.advance()
+        solver.sync()
+        elapsed = time.perf_counter() - t0
+        its = solver.counters()["iterations"] - it0
+        log(f"[{tag}] timed {args.steps} steps in {elapsed:.3f} s, {its} CG iterations")
+        return solver, elapsed, its
+
+    mixed_run = args.precision == "mixed" and not learned  # the learned FCG runs in FP64
+    plain = None
+    if (mixed_run or learned) and not args.no_plain:
+        import dataclasses
+        ps, pe, pi = timed_run(dataclasses.replace(case.params, precond="JACOBI"), 0, "plain FP64")
+        plain = {"value": mesh.n_cells * args.steps / pe, "ms_per_step": 1e3 * pe / args.steps,
+                 "cg_iterations_per_step": pi / args.steps}
+        ps.close()
+    solver, elapsed, its = timed_run(case.params, 1 if mixed_run else 0,
+                                     "learned" if learned else ("mixed" if mixed_run else "plain FP64"))
 
     # Kernel durations: the same number of further steps with a dispatch-stamped event pair on
     # every launch of the timed kernel classes (the stamps add a few microseconds between
@@ -153,7 +174,7 @@ def main():
         log(f"kernel-timing pass: {ktimed_ms:.1f} ms per step")
 
     nc = mesh.n_cells
-    mixed = args.precision == "mixed"
+    mixed = mixed_run
     bytes_dir, nfp = cg_pass_bytes(mesh, args.cg_variant, mixed)
     info = solver.learned_info() if learned else None
     timed_learned = learned and info is not None and info["phase"] != "off"
@@ -181,7 +202,7 @@ def main():
                                + ("FP32 CG sweeps in FP64 iterative refinement" if mixed else "plain FP64")
                                + (", flexible CG with the learned message-passing preconditioner (Jacobi CG "
                                   "fallback)" if learned else ""),
-                   "precision": args.precision, "cg_variant": args.cg_variant, "precond": args.precond,
+                   "precision": "mixed" if mixed else "fp64", "cg_variant": args.cg_variant, "precond": args.precond,
                    "dt": case.params.dt},
         "roofline": {"bound": "hbm",
                      "kernel": ("learned preconditioner hidden layer k_tile<1,mp::L1<6>> (MFMA bf16)" if timed_learned else
@@ -189,8 +210,8 @@ def main():
                                 "CG iteration pass k_pass<6,6,cgk::Fused>" if args.cg_variant == 1
                                 else "CG direction/operator pass k_pass<4,1,cgk::Dir>"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic("FusedT<6>", f"channel {nx}x{ny}x{nz}")
-                                 if (not mixed and not timed_learned and args.cg_variant == 1) else None),
+                     "traffic": (measured_traffic("Fused32T<6>" if mixed else "FusedT<6>", f"channel {nx}x{ny}x{nz}")
+                                 if (not timed_learned and args.cg_variant == 1) else None),
                      "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
                      "launches": n_dir},
         "kernels_us": {k: (v[1] * 1e3 / v[0] if v[0] else None) for k, v in ktime.items()},
@@ -200,6 +221,9 @@ def main():
         "state": {"E": st["E"], "Ub": st["Ub"], "max_abs_divU": st["max_abs_divU"]},
         "cpu_baseline": None,
     }
+    if plain is not None:
+        out["plain_fp64"] = plain
+        out["speedup_vs_plain_fp64"] = out["value"] / plain["value"]
     if learned:
         info = solver.learned_info()
         loss = solver.learned_loss()
