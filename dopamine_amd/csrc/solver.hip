@@ -197,6 +197,20 @@ int dopamine_solver_kernel_time(void* h, int kernel, int64_t* launches, double* 
   });
 }
 
+int dopamine_solver_write_checkpoint(void* h, const char* dir, int step, int rank) {
+  return guard([&] {
+    if (!dir) throw std::runtime_error("null directory");
+    return static_cast<Solver*>(h)->write_checkpoint(dir, step, rank);
+  });
+}
+
+int dopamine_solver_read_checkpoint(void* h, const char* dir, int step, int rank) {
+  return guard([&] {
+    if (!dir) throw std::runtime_error("null directory");
+    return static_cast<Solver*>(h)->read_checkpoint(dir, step, rank);
+  });
+}
+
 int dopamine_solver_cell_divergence(void* h, double out[2]) {
   return guard([&] {
     static_cast<Solver*>(h)->cell_divergence(out);

@@ -17,6 +17,7 @@
 #include <cstdlib>
 
 #include <algorithm>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -1080,6 +1081,136 @@ class SolverT {
     out[5] = h.st[6] / V;
     out[6] = h.st[10];
     out[7] = h.st[11] / V;
+  }
+
+  // ---------------------------------------------------------------- checkpoint / restart
+  // write_checkpoint / read_checkpoint (src/io/mod_checkpoint_io.f90:15-283): per-rank file
+  // dir/processorR/checkpoint_XXXXXXXX.bin of Fortran unformatted sequential records (a 4-byte
+  // length before and after each payload): header(8) int32 {nc, n_scalars = 0, history = 0, step,
+  // has_C = 0, 0, has U_face, wall mask (1 tau_w, 2 q_w, 4 L)}, time, global ids (int64, 1..nc),
+  // U (3, nc), p, T (T_ref when no temperature is solved), nu_sgs, then U_face and the wall-model
+  // records the mask names. Rank 0 also writes dir/checkpoint_latest with the step.
+  static void put_record(std::FILE* f, const void* p, size_t bytes) {
+    if (bytes > 0x7fffffffull) throw std::runtime_error("checkpoint record above 2 GiB");
+    const int32_t m = int32_t(bytes);
+    if (std::fwrite(&m, 4, 1, f) != 1 || (bytes && std::fwrite(p, 1, bytes, f) != bytes) || std::fwrite(&m, 4, 1, f) != 1)
+      throw std::runtime_error("checkpoint write failed");
+  }
+  static void get_record(std::FILE* f, void* p, size_t bytes) {
+    int32_t m0 = 0, m1 = 0;
+    if (std::fread(&m0, 4, 1, f) != 1) throw std::runtime_error("checkpoint: truncated file");
+    if (size_t(m0) != bytes) throw std::runtime_error("checkpoint: record length does not match the mesh");
+    if ((bytes && std::fread(p, 1, bytes, f) != bytes) || std::fread(&m1, 4, 1, f) != 1 || m1 != m0)
+      throw std::runtime_error("checkpoint: corrupt record");
+  }
+  static std::string ckpt_name(const char* dir, int step, int rank) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "/processor%d/checkpoint_%08d.bin", rank, step);
+    return std::string(dir) + b;
+  }
+  int write_checkpoint(const char* dir, int step, int rank) {
+    const std::string pdir = std::string(dir) + "/processor" + std::to_string(rank);
+    if (std::system(("mkdir -p '" + pdir + "'").c_str()) != 0) throw std::runtime_error("cannot create " + pdir);
+    std::FILE* f = std::fopen(ckpt_name(dir, step, rank).c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot open " + ckpt_name(dir, step, rank));
+    const int nbb = std::max(1, nb);
+    int32_t header[8] = {nc, 0, 0, step, 0, 0, 1, 1 + (q_w ? 2 + 4 : 0)};
+    std::vector<double> h(3 * size_t(std::max(nc, nf)));
+    std::vector<int64_t> gid(nc);
+    for (int c = 0; c < nc; ++c) gid[c] = c + 1;
+    try {
+      put_record(f, header, sizeof(header));
+      put_record(f, &time, 8);
+      put_record(f, gid.data(), 8 * size_t(nc));
+      x.download(h.data(), U, 3 * size_t(nc));
+      put_record(f, h.data(), 24 * size_t(nc));
+      x.download(h.data(), P, nc);
+      put_record(f, h.data(), 8 * size_t(nc));
+      if (bous) {
+        x.download(h.data(), T, nc);
+      } else {
+        std::fill(h.begin(), h.begin() + nc, prm.T_ref);
+      }
+      put_record(f, h.data(), 8 * size_t(nc));
+      x.download(h.data(), nu_sgs, nc);
+      put_record(f, h.data(), 8 * size_t(nc));
+      x.download(h.data(), UF, nf);
+      put_record(f, h.data(), 8 * size_t(nf));
+      std::vector<double> hb(3 * size_t(nbb));
+      x.download(hb.data(), tau, 3 * size_t(nbb));
+      put_record(f, hb.data(), 24 * size_t(nb));
+      if (q_w) {
+        x.download(hb.data(), q_w, nbb);
+        put_record(f, hb.data(), 8 * size_t(nb));
+        x.download(hb.data(), L_bnd, nbb);
+        put_record(f, hb.data(), 8 * size_t(nb));
+      }
+    } catch (...) {
+      std::fclose(f);
+      throw;
+    }
+    std::fclose(f);
+    if (rank == 0) {
+      std::FILE* m = std::fopen((std::string(dir) + "/checkpoint_latest").c_str(), "w");
+      if (!m) throw std::runtime_error("cannot write checkpoint_latest");
+      std::fprintf(m, "%d\n", step);
+      std::fclose(m);
+    }
+    return 0;
+  }
+  // Returns the checkpoint's step. Restores the fields, then rebuilds boundary values and
+  // gradients as solver_refresh_after_restart does (mod_fv_solver.f90:601-609).
+  int read_checkpoint(const char* dir, int step, int rank) {
+    std::FILE* f = std::fopen(ckpt_name(dir, step, rank).c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open " + ckpt_name(dir, step, rank));
+    int32_t header[8];
+    const int nbb = std::max(1, nb);
+    std::vector<double> h(3 * size_t(std::max(nc, nf)));
+    try {
+      get_record(f, header, sizeof(header));
+      if (header[0] != nc) throw std::runtime_error("checkpoint: n_cells does not match the mesh");
+      if (header[1] != 0) throw std::runtime_error("checkpoint: passive scalars are not supported");
+      get_record(f, &time, 8);
+      std::vector<int64_t> gid(nc);
+      get_record(f, gid.data(), 8 * size_t(nc));
+      for (int c = 0; c < nc; ++c)
+        if (gid[c] != c + 1) throw std::runtime_error("checkpoint: global_id mismatch (different partition)");
+      get_record(f, h.data(), 24 * size_t(nc));
+      x.upload(U, h.data(), 3 * size_t(nc));
+      get_record(f, h.data(), 8 * size_t(nc));
+      x.upload(P, h.data(), nc);
+      get_record(f, h.data(), 8 * size_t(nc));
+      if (bous) x.upload(T, h.data(), nc);
+      get_record(f, h.data(), 8 * size_t(nc));
+      x.upload(nu_sgs, h.data(), nc);
+      if (header[6] == 1) {
+        get_record(f, h.data(), 8 * size_t(nf));
+        x.upload(UF, h.data(), nf);
+      }
+      std::vector<double> hb(3 * size_t(nbb));
+      if (header[7] & 1) {
+        get_record(f, hb.data(), 24 * size_t(nb));
+        x.upload(tau, hb.data(), 3 * size_t(nb));
+      }
+      for (int bit : {2, 4}) {
+        if (!(header[7] & bit)) continue;
+        get_record(f, hb.data(), 8 * size_t(nb));
+        double* dst = bit == 2 ? q_w : L_bnd;
+        if (dst) x.upload(dst, hb.data(), nb);
+      }
+      if (header[7] & (8 | 16)) throw std::runtime_error("checkpoint: wall-filter / inflow-eddy records are not supported");
+    } catch (...) {
+      std::fclose(f);
+      throw;
+    }
+    std::fclose(f);
+    refresh_bc();
+    gradients(false, 0.0);
+    pressure_gradient();
+    refresh_bc();
+    gradients(false, 0.0);
+    x.sync();
+    return header[3];
   }
 
   // The cell-centred divergence of solver_divergence_metrics (mod_fv_solver.f90:3641-3713): the

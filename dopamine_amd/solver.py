@@ -95,6 +95,8 @@ def _bind(L):
     L.dopamine_solver_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _P(_i64), _P(_d)]
     L.dopamine_solver_learned_info.argtypes = [ctypes.c_void_p, _d * 8]
     L.dopamine_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
+    L.dopamine_solver_write_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    L.dopamine_solver_read_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_stats_csv.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _d, _P(_d), ctypes.c_int]
     L.dopamine_solver_learned_loss.argtypes = [ctypes.c_void_p, _P(_d), ctypes.c_int]
     return L
@@ -198,6 +200,18 @@ class FVSolver:
     def kernel_timing(self, kernel, enable=True):
         """Start (clearing the totals) or stop HIP-event timing of a kernel class."""
         self._check(self.L.dopamine_solver_kernel_timing(self.h, KERNELS[kernel], int(enable)), "kernel_timing")
+
+    def write_checkpoint(self, directory, step, rank=0):
+        """The reference's per-rank checkpoint (dir/processor<rank>/checkpoint_<step>.bin)."""
+        self._check(self.L.dopamine_solver_write_checkpoint(self.h, str(directory).encode(), step, rank),
+                    "write_checkpoint")
+
+    def read_checkpoint(self, directory, step, rank=0):
+        """Restore from a checkpoint (then the restart refresh); returns its step."""
+        rc = self.L.dopamine_solver_read_checkpoint(self.h, str(directory).encode(), step, rank)
+        if rc < 0:
+            raise RuntimeError(f"read_checkpoint failed ({rc}): {self.L.dopamine_last_error().decode()}")
+        return rc
 
     def cell_divergence(self):
         out = (_d * 2)()

@@ -116,6 +116,14 @@ int dopamine_solver_cell_divergence(void *solver, double out[2]);
  * before solver_advance's CFL clamp). *E0 <= 0 takes this row's E as the reference energy and
  * returns it. */
 int dopamine_solver_stats_csv(void *solver, const char *path, int step, double dt, double *E0, int append);
+/* Checkpoint in the reference's per-rank format (write_checkpoint, src/io/mod_checkpoint_io.f90:15-150):
+ * dir/processor<rank>/checkpoint_<step, 8 digits>.bin, Fortran unformatted sequential records
+ * (header, time, global ids, U, p, T, nu_sgs, U_face, wall traction [, q_w, L for MOST]); rank 0
+ * also writes dir/checkpoint_latest. */
+int dopamine_solver_write_checkpoint(void *solver, const char *dir, int step, int rank);
+/* read_checkpoint (:153-283) followed by solver_refresh_after_restart (mod_fv_solver.f90:601-609).
+ * Returns the checkpoint's step (>= 0) or a negative error (dimension or partition mismatch). */
+int dopamine_solver_read_checkpoint(void *solver, const char *dir, int step, int rank);
 /* {pressure solves, total CG iterations, last CG iteration count, refinement sweeps}. */
 int dopamine_solver_counters(void *solver, int64_t out[4]);
 /* {skip_nonortho, skip_grad_p, n_nonortho_corr}. */

@@ -36,6 +36,8 @@ def lib():
         L.hc_solver_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int32 * 3]
         L.hc_solver_stats_csv.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _d, _P(_d), ctypes.c_int]
         L.hc_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
+        L.hc_solver_write_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        L.hc_solver_read_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -88,6 +90,16 @@ class HostSolver:
         if lib().hc_solver_stats_csv(self.h, str(path).encode(), step, dt, ctypes.byref(e0), int(append)) != 0:
             raise OSError(f"cannot write {path}")
         return e0.value
+
+    def write_checkpoint(self, directory, step, rank=0):
+        if lib().hc_solver_write_checkpoint(self.h, str(directory).encode(), step, rank) != 0:
+            raise OSError(lib().hc_last_error().decode())
+
+    def read_checkpoint(self, directory, step, rank=0):
+        rc = lib().hc_solver_read_checkpoint(self.h, str(directory).encode(), step, rank)
+        if rc < 0:
+            raise OSError(lib().hc_last_error().decode())
+        return rc
 
     def project_once(self, dt):
         return lib().hc_solver_project_once(self.h, dt)

@@ -50,6 +50,22 @@ int hc_solver_stats_csv(void* h, const char* path, int step, double dt, double* 
   return static_cast<Solver*>(h)->stats_csv(path, step, dt, E0, append);
 }
 void hc_solver_cell_divergence(void* h, double out[2]) { static_cast<Solver*>(h)->cell_divergence(out); }
+int hc_solver_write_checkpoint(void* h, const char* dir, int step, int rank) {
+  try {
+    return static_cast<Solver*>(h)->write_checkpoint(dir, step, rank);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -100;
+  }
+}
+int hc_solver_read_checkpoint(void* h, const char* dir, int step, int rank) {
+  try {
+    return static_cast<Solver*>(h)->read_checkpoint(dir, step, rank);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -100;
+  }
+}
 double* hc_solver_field(void* h, const char* name, int64_t* n) { return field(static_cast<Solver*>(h), name, n); }
 void hc_solver_apply_all_bc(void* h) { static_cast<Solver*>(h)->apply_all_bc(); }
 int hc_solver_project_once(void* h, double dt) { return static_cast<Solver*>(h)->project_once(dt); }

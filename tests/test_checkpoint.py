@@ -1,0 +1,96 @@
+"""Checkpoint in the reference's format (src/io/mod_checkpoint_io.f90): the file parsed record by
+record as Fortran unformatted sequential data (4-byte length markers around each payload), and a
+restart that continues the run as a straight run does."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from dopamine_amd import cases, params as P
+from helpers import rel_l2
+from hostcheck import HostSolver
+
+
+def fortran_records(path):
+    """Payloads of a Fortran unformatted sequential file (gfortran record markers)."""
+    data = open(path, "rb").read()
+    out, i = [], 0
+    while i < len(data):
+        (n,) = struct.unpack_from("<i", data, i)
+        payload = data[i + 4:i + 4 + n]
+        (m,) = struct.unpack_from("<i", data, i + 4 + n)
+        assert m == n and len(payload) == n
+        out.append(payload)
+        i += 8 + n
+    return out
+
+
+def _run(case, mesh, steps):
+    s = HostSolver(mesh, case.params)
+    P.initialize(s, case, mesh)
+    for _ in range(steps):
+        s.advance()
+    return s
+
+
+def test_checkpoint_records_follow_the_reference_layout(tmp_path):
+    case = cases.small_channel(12, 10, 6)
+    mesh = case.make_mesh()
+    s = _run(case, mesh, 2)
+    s.write_checkpoint(tmp_path, 2)
+    path = tmp_path / "processor0" / "checkpoint_00000002.bin"
+    assert path.exists()
+    assert open(tmp_path / "checkpoint_latest").read().strip() == "2"
+    rec = fortran_records(path)
+    nc, nf, nb = mesh.n_cells, mesh.n_faces, mesh.n_bnd_faces
+    header = struct.unpack("<8i", rec[0])
+    assert header == (nc, 0, 0, 2, 0, 0, 1, 1)  # U_face record present, wall mask: tau_w
+    assert len(rec) == 9
+    (t,) = struct.unpack("<d", rec[1])
+    assert t == s.time
+    assert np.array_equal(np.frombuffer(rec[2], dtype="<i8"), np.arange(1, nc + 1))
+    assert np.array_equal(np.frombuffer(rec[3], dtype="<f8"), s.array("U"))  # U(3, nc), component fastest
+    assert np.array_equal(np.frombuffer(rec[4], dtype="<f8"), s.array("p"))
+    assert np.all(np.frombuffer(rec[5], dtype="<f8") == case.params.T_ref)  # T without a temperature solve
+    assert np.array_equal(np.frombuffer(rec[6], dtype="<f8"), s.array("nu_sgs"))
+    assert np.array_equal(np.frombuffer(rec[7], dtype="<f8"), s.array("U_face"))
+    assert np.array_equal(np.frombuffer(rec[8], dtype="<f8"), s.array("tau_w_bnd")[:3 * nb])
+
+
+def test_restart_continues_like_a_straight_run(tmp_path):
+    case = cases.small_channel(12, 10, 6)
+    mesh = case.make_mesh()
+    a = _run(case, mesh, 3)
+    a.write_checkpoint(tmp_path, 3)
+    for _ in range(2):
+        a.advance()
+    b = HostSolver(mesh, case.params)
+    P.initialize(b, case, mesh)  # then overwritten by the checkpoint
+    assert b.read_checkpoint(tmp_path, 3) == 3
+    for _ in range(2):
+        b.advance()
+    assert b.time == a.time
+    assert rel_l2(a.U, b.U) < 1e-12 and rel_l2(a.p, b.p) < 1e-10
+
+
+def test_checkpoint_with_most_temperature_records(tmp_path):
+    case = cases.most_channel(12, 10, 6, dT=1.0)
+    mesh = case.make_mesh()
+    s = _run(case, mesh, 2)
+    s.write_checkpoint(tmp_path, 7, rank=0)
+    rec = fortran_records(tmp_path / "processor0" / "checkpoint_00000007.bin")
+    header = struct.unpack("<8i", rec[0])
+    assert header[7] == 1 + 2 + 4 and len(rec) == 11
+    assert np.array_equal(np.frombuffer(rec[5], dtype="<f8"), s.array("T"))
+    assert np.array_equal(np.frombuffer(rec[9], dtype="<f8"), s.array("q_w_bnd")[:mesh.n_bnd_faces])
+    assert np.array_equal(np.frombuffer(rec[10], dtype="<f8"), s.array("L_bnd")[:mesh.n_bnd_faces])
+
+
+def test_checkpoint_for_another_mesh_is_refused(tmp_path):
+    c1, c2 = cases.small_channel(12, 10, 6), cases.small_channel(12, 10, 8)
+    m1, m2 = c1.make_mesh(), c2.make_mesh()
+    _run(c1, m1, 1).write_checkpoint(tmp_path, 1)
+    s = HostSolver(m2, c2.params)
+    with pytest.raises(OSError, match="n_cells"):
+        s.read_checkpoint(tmp_path, 1)

@@ -205,3 +205,25 @@ def test_gpu_stats_csv_matches_reference_file(name, tmp_path):
             # the projected face-flux divergence is solver round-off: the reference rule bounds it by 1e-9
             tol = 1e-9 if col in ("max_abs_divU", "volavg_abs_divU") else 1e-15
             assert abs(a - row[col]) <= 1e-6 * abs(a) + tol, (f[0], col, a, row[col])
+
+
+def test_gpu_checkpoint_restart_continues_the_run(tmp_path):
+    """Write a checkpoint on the GPU, restore it into a fresh solver, continue both: same fields."""
+    case = cases.small_channel(16, 12, 8)
+    mesh = case.make_mesh()
+    print("[gpu] checkpoint restart channel 16x12x8", flush=True)
+    a = FVSolver(mesh, case.params)
+    P.initialize(a, case, mesh)
+    for _ in range(3):
+        a.advance()
+    a.write_checkpoint(tmp_path, 3)
+    for _ in range(2):
+        a.advance()
+    b = FVSolver(mesh, case.params)
+    P.initialize(b, case, mesh)
+    assert b.read_checkpoint(tmp_path, 3) == 3
+    for _ in range(2):
+        b.advance()
+    du, dp = rel_l2(a.U, b.U), rel_l2(a.p, b.p)
+    print(f"[gpu] restart: rel-L2 U {du:.2e} p {dp:.2e}", flush=True)
+    assert du < 1e-12 and dp < 1e-10
