@@ -2,9 +2,10 @@
 
 The reference reads Gmsh v4 / OpenFOAM meshes and builds its face list, geometry, periodic pairs
 and least-squares stencil once at start-up (src/mesh/mod_gmsh_reader.f90:45-116,
-src/mesh/mod_mesh.f90:215-758, src/bc/mod_bc_setup.f90:134-330). Here the same element lists are
-produced by generators (structured hex boxes, with the node/element/boundary ordering that
-tests/regression/mk_box_mesh.py writes) and handed to the native builder in libdopamine_hip.so.
+src/mesh/mod_mesh.f90:215-758, src/bc/mod_bc_setup.f90:134-330). Here the same element lists come
+from a Gmsh v4 ASCII file (read_gmsh: parsed by the native reader) or from generators (structured
+hex boxes, with the node/element/boundary ordering that tests/regression/mk_box_mesh.py writes),
+and are handed to the native builder in libdopamine_hip.so.
 """
 import ctypes
 import math
@@ -127,8 +128,10 @@ class Mesh:
             self.vertex_coords = _copy(v.vertex_coords, 3 * v.n_vertices, (v.n_vertices, 3))
         finally:
             L.dm_mesh_free(h)
-        self.patch_names = list(elements.get("group_names", []))
-        # group index of each patch (patches are the non-empty groups in group order)
+        # patches are the groups that hold boundary faces, in group order (an empty group gets none)
+        names = list(elements.get("group_names", []))
+        used = np.bincount(keep["bnd_group"], minlength=len(keep["group_bc"])) > 0
+        self.patch_names = [n for n, u in zip(names, used) if u] if names else []
         self.bounds_min = self.vertex_coords.min(axis=0)
         self.bounds_max = self.vertex_coords.max(axis=0)
 
@@ -197,6 +200,11 @@ def _round16g(a):
 
 def skewed_box(n, amp, group_types=("periodic",) * 6, group_names=("x0", "xL", "y0", "yL", "z0", "zL")):
     """Curvilinear unit box of tests/regression/mk_box_mesh.py (N^3 hexes, node displacement amp)."""
+    return Mesh(skewed_box_elements(n, amp, group_types, group_names))
+
+
+def skewed_box_elements(n, amp, group_types=("periodic",) * 6, group_names=("x0", "xL", "y0", "yL", "z0", "zL")):
+    """Element lists of skewed_box."""
     def xyz(I, J, K):
         out = np.empty((3, len(I)))
         for q, (i, j, k) in enumerate(zip(I.tolist(), J.tolist(), K.tolist())):
@@ -205,7 +213,7 @@ def skewed_box(n, amp, group_types=("periodic",) * 6, group_names=("x0", "xL", "
             out[1, q] = Y + amp * math.sin(math.pi * Y) * math.sin(2 * math.pi * Z) * math.sin(math.pi * X)
             out[2, q] = Z + amp * math.sin(math.pi * Z) * math.sin(2 * math.pi * X) * math.sin(math.pi * Y)
         return _round16g(out[0]), _round16g(out[1]), _round16g(out[2])
-    return Mesh(box_elements(n, n, n, xyz, group_names, group_types))
+    return box_elements(n, n, n, xyz, group_names, group_types)
 
 
 def uniform_box(nx, ny, nz, lx, ly, lz, group_types=("periodic",) * 6,
@@ -253,6 +261,12 @@ _KUHN_PATHS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
 
 def tet_box(n, lx=1.0, ly=1.0, lz=1.0, jitter=0.15, seed=3200, shuffle_seed=None,
             group_types=("noSlip",) * 6, group_names=("x0", "xL", "y0", "yL", "z0", "zL")):
+    """Mesh of tet_box_elements."""
+    return Mesh(tet_box_elements(n, lx, ly, lz, jitter, seed, shuffle_seed, group_types, group_names))
+
+
+def tet_box_elements(n, lx=1.0, ly=1.0, lz=1.0, jitter=0.15, seed=3200, shuffle_seed=None,
+                     group_types=("noSlip",) * 6, group_names=("x0", "xL", "y0", "yL", "z0", "zL")):
     """n^3 hexes split into 6 tets each (Kuhn split, conforming across hexes), interior nodes
     jittered by a seeded uniform +-jitter of the local spacing per coordinate, tets optionally put
     in a seeded random order (cell ids shuffled). Boundary quads split along the same diagonals."""
@@ -306,7 +320,7 @@ def tet_box(n, lx=1.0, ly=1.0, lz=1.0, jitter=0.15, seed=3200, shuffle_seed=None
     bnd = np.concatenate(tris).astype(np.int32)
     bnd_group = np.concatenate([np.full(len(t), g, dtype=np.int32) for g, t in enumerate(tris)])
     nv, nt = len(tets), len(bnd)
-    return Mesh({
+    return {
         "coords": coords,
         "vol_type": np.full(nv, GMSH_TET4, dtype=np.int32),
         "vol_offset": np.arange(nv + 1, dtype=np.int64) * 4,
@@ -317,4 +331,117 @@ def tet_box(n, lx=1.0, ly=1.0, lz=1.0, jitter=0.15, seed=3200, shuffle_seed=None
         "bnd_group": bnd_group,
         "group_bc": np.array([BC_NAMES[t.lower()] for t in group_types], dtype=np.int32),
         "group_names": list(group_names),
-    })
+    }
+
+
+class _GmshView(ctypes.Structure):
+    _fields_ = [("n_nodes", _i64), ("coords", _P(_d)),
+                ("n_vol", _i64), ("vol_type", _P(_i32)), ("vol_offset", _P(_i64)), ("vol_nodes", _P(_i32)),
+                ("n_bnd", _i64), ("bnd_type", _P(_i32)), ("bnd_offset", _P(_i64)), ("bnd_nodes", _P(_i32)),
+                ("bnd_group", _P(_i32)), ("n_groups", _i32)]
+
+
+def read_gmsh_elements(path, patch_types=None, warn=print):
+    """Element lists of a Gmsh v4 ASCII file (src/mesh/mod_gmsh_reader.f90:45-236), parsed natively.
+
+    ``patch_types`` maps a 2-D physical group name to a patch_type name (BC_NAMES), as the
+    reference's &dopamine_bc entries do. A group with no entry becomes zeroGradient with a warning;
+    an entry naming no group of the file is an error (src/bc/mod_bc_setup.f90:42-106).
+    """
+    L = lib()
+    L.dm_gmsh_read.restype = ctypes.c_void_p
+    L.dm_gmsh_read.argtypes = [ctypes.c_char_p]
+    L.dm_gmsh_get_view.argtypes = [ctypes.c_void_p, ctypes.POINTER(_GmshView)]
+    L.dm_gmsh_group_name.restype = ctypes.c_char_p
+    L.dm_gmsh_group_name.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dm_gmsh_free.argtypes = [ctypes.c_void_p]
+    L.dm_gmsh_last_error.restype = ctypes.c_char_p
+    h = L.dm_gmsh_read(str(path).encode())
+    if not h:
+        raise ValueError(f"{path}: " + L.dm_gmsh_last_error().decode())
+    try:
+        v = _GmshView()
+        L.dm_gmsh_get_view(h, ctypes.byref(v))
+        names = [L.dm_gmsh_group_name(h, g).decode() for g in range(v.n_groups)]
+        e = {
+            "coords": _copy(v.coords, 3 * v.n_nodes),
+            "vol_type": _copy(v.vol_type, v.n_vol),
+            "vol_offset": _copy(v.vol_offset, v.n_vol + 1),
+            "bnd_type": _copy(v.bnd_type, v.n_bnd),
+            "bnd_offset": _copy(v.bnd_offset, v.n_bnd + 1),
+            "bnd_group": _copy(v.bnd_group, v.n_bnd),
+        }
+        e["vol_nodes"] = _copy(v.vol_nodes, int(e["vol_offset"][-1]))
+        e["bnd_nodes"] = _copy(v.bnd_nodes, int(e["bnd_offset"][-1]))
+    finally:
+        L.dm_gmsh_free(h)
+    types = {k: v for k, v in (patch_types or {}).items()}
+    missing = [k for k in types if k not in names]
+    if missing:
+        raise ValueError("input BC patch names not in the mesh: " + ", ".join(missing))
+    bc = []
+    for n in names:
+        if n in types:
+            bc.append(BC_NAMES[types[n].lower()])
+        else:
+            if warn:
+                warn(f'mesh patch "{n}" has no patch type; using zeroGradient')
+            bc.append(BC_ZEROGRADIENT)
+    e["group_bc"] = np.array(bc, dtype=np.int32)
+    e["group_names"] = names
+    return e
+
+
+def read_gmsh(path, patch_types=None, warn=print):
+    """Mesh of a Gmsh v4 ASCII file; see read_gmsh_elements."""
+    return Mesh(read_gmsh_elements(path, patch_types, warn))
+
+
+def write_gmsh(elements, path):
+    """Write element lists as a Gmsh 4.1 ASCII file that read_gmsh_elements reads back unchanged.
+
+    One entity per dimension: volume elements on volume 1 (physical "fluid"), boundary group g on
+    surface g+1 with physical tag g+1. Node tags are 1-based positions; coordinates are written
+    with '%.17g' so they read back bit for bit.
+    """
+    e = elements
+    names = list(e["group_names"])
+    ng = len(names)
+    coords = np.asarray(e["coords"], dtype=np.float64).reshape(-1, 3)
+    nn = len(coords)
+    lines = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$PhysicalNames", str(ng + 1)]
+    lines += [f'2 {g + 1} "{n}"' for g, n in enumerate(names)]
+    lines += [f'3 {ng + 1} "fluid"', "$EndPhysicalNames", "$Entities", f"0 0 {ng} 1"]
+    lo, hi = coords.min(axis=0), coords.max(axis=0)
+    box = " ".join(f"{x:.17g}" for x in (*lo, *hi))
+    lines += [f"{g + 1} {box} 1 {g + 1} 0" for g in range(ng)]
+    lines += [f"1 {box} 1 {ng + 1} 0", "$EndEntities", "$Nodes", f"1 {nn} 1 {nn}", f"3 1 0 {nn}"]
+    lines += [str(i + 1) for i in range(nn)]
+    lines += [f"{x:.17g} {y:.17g} {z:.17g}" for x, y, z in coords]
+    lines += ["$EndNodes"]
+    vt, vo, vn = e["vol_type"], e["vol_offset"], e["vol_nodes"]
+    bt, bo, bn, bg = e["bnd_type"], e["bnd_offset"], e["bnd_nodes"], e["bnd_group"]
+    # blocks: one per (entity, element type) run, in element order
+    blocks = []
+
+    def runs(types, key):
+        start = 0
+        for i in range(1, len(types) + 1):
+            if i == len(types) or types[i] != types[start] or key(i) != key(start):
+                blocks.append((start, i, types, key(start)))
+                start = i
+    runs(bt, lambda i: int(bg[i]) if i < len(bg) else -1)
+    nb_blocks = len(blocks)
+    runs(vt, lambda i: -1)
+    tag = 1
+    body = []
+    for b, (s0, s1, types, grp) in enumerate(blocks):
+        is_bnd = b < nb_blocks
+        off, nod = (bo, bn) if is_bnd else (vo, vn)
+        body.append(f"{2 if is_bnd else 3} {grp + 1 if is_bnd else 1} {int(types[s0])} {s1 - s0}")
+        for i in range(s0, s1):
+            body.append(f"{tag} " + " ".join(str(int(x) + 1) for x in nod[off[i]:off[i + 1]]))
+            tag += 1
+    lines += ["$Elements", f"{len(blocks)} {tag - 1} 1 {tag - 1}"] + body + ["$EndElements"]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

@@ -106,6 +106,34 @@ void dm_mesh_free(void* mesh);
 /* Last error message of this library (thread-local); empty when none. */
 const char* dm_last_error(void);
 
+/* Gmsh v4 ASCII reader (gmsh_read, src/mesh/mod_gmsh_reader.f90:45-236): nodes in file order,
+ * 3-D elements as volume elements in file order, 2-D elements as boundary elements tagged by
+ * their entity's first physical group, numbered among the 2-D physical groups in $PhysicalNames
+ * order. The view's arrays are the builder's dm_mesh_input fields (group_bc is the caller's:
+ * patch types come from the case configuration, by group name). */
+typedef struct dm_gmsh_view {
+  int64_t n_nodes;
+  const double* coords;
+  int64_t n_vol;
+  const int32_t* vol_type;
+  const int64_t* vol_offset;
+  const int32_t* vol_nodes;
+  int64_t n_bnd;
+  const int32_t* bnd_type;
+  const int64_t* bnd_offset;
+  const int32_t* bnd_nodes;
+  const int32_t* bnd_group;
+  int32_t n_groups;
+} dm_gmsh_view;
+/* NULL on error (dm_gmsh_last_error()): unreadable file, not v4 ASCII, unsupported element type,
+ * a boundary element outside every 2-D physical group. */
+void* dm_gmsh_read(const char* path);
+int dm_gmsh_get_view(const void* file, dm_gmsh_view* view);
+/* Name of 2-D physical group g, or NULL. */
+const char* dm_gmsh_group_name(const void* file, int group);
+void dm_gmsh_free(void* file);
+const char* dm_gmsh_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif
