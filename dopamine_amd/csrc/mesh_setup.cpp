@@ -1,4 +1,5 @@
-// Mesh setup: Gmsh-style element lists -> the cell-centred FV mesh every hot-path kernel reads.
+// Mesh setup: Gmsh-style element lists, or face-based (OpenFOAM polyMesh) input -> the
+// cell-centred FV mesh every hot-path kernel reads.
 //
 // Host-side setup code (runs once per mesh). Each step names the reference routine whose result
 // it reproduces, so a mesh built here carries the same face numbering, geometry, periodic pairs and
@@ -48,6 +49,8 @@ struct Mesh {
   std::vector<double> vertex_coords;
   std::vector<int64_t> cell_vertex_offset;
   std::vector<int32_t> cell_vertices;
+  std::vector<int64_t> face_node_offset;  // per face, right-hand normal along face_normal
+  std::vector<int32_t> face_nodes;
 
   V3 centroid(int c) const { return {cell_centroid[3 * c], cell_centroid[3 * c + 1], cell_centroid[3 * c + 2]}; }
   V3 fcentre(int f) const { return {face_centre[3 * f], face_centre[3 * f + 1], face_centre[3 * f + 2]}; }
@@ -148,6 +151,24 @@ void face_geometry(const int* fnodes, int nfn, const double* coords, V3& centre,
   normal = {nacc.x / (area + SMALL), nacc.y / (area + SMALL), nacc.z / (area + SMALL)};
 }
 
+// Right-hand-rule area vector of a point list (fan about its first point).
+V3 fan_area(const int* nodes, int n, const double* coords) {
+  auto P = [&](int i) { return V3{coords[3 * nodes[i]], coords[3 * nodes[i] + 1], coords[3 * nodes[i] + 2]}; };
+  V3 acc = {0.0, 0.0, 0.0};
+  for (int i = 1; i + 1 < n; ++i) acc = add(acc, cross(sub(P(i), P(0)), sub(P(i + 1), P(0))));
+  return acc;
+}
+
+// Appends a face's points so that the right-hand normal follows `normal`.
+void push_face_nodes(std::vector<int32_t>& out, const int* nodes, int n, const double* coords, V3 normal) {
+  if (dot(fan_area(nodes, n, coords), normal) < 0.0) {
+    out.push_back(nodes[0]);
+    for (int i = n - 1; i >= 1; --i) out.push_back(nodes[i]);
+  } else {
+    out.insert(out.end(), nodes, nodes + n);
+  }
+}
+
 using FaceKey = std::array<int32_t, 4>;
 struct FaceKeyHash {
   size_t operator()(const FaceKey& k) const {
@@ -220,7 +241,8 @@ void build_faces(Mesh& m, const dm_mesh_input& in) {
   };
   std::unordered_map<FaceKey, Slot, FaceKeyHash> table;
   table.reserve(size_t(nc) * 4);
-  std::vector<int32_t> fown, fnb, fpatch;
+  std::vector<int32_t> fown, fnb, fpatch, fnodes;
+  std::vector<int64_t> foff{0};
   std::vector<double> fcen, fnrm, farea;
   fown.reserve(size_t(nc) * 3);
   for (int c = 0; c < nc; ++c) {
@@ -238,6 +260,8 @@ void build_faces(Mesh& m, const dm_mesh_input& in) {
         face_geometry(fn[fi], fcnt[fi], in.coords, cen, nrm, area);
         const V3 d = sub(m.centroid(c), m.centroid(owner));
         if (dot(nrm, d) < 0.0) nrm = scl(-1.0, nrm);  // orient_normal
+        push_face_nodes(fnodes, fn[fi], fcnt[fi], in.coords, nrm);
+        foff.push_back(int64_t(fnodes.size()));
         fown.push_back(owner);
         fnb.push_back(c);
         fpatch.push_back(-1);
@@ -252,7 +276,7 @@ void build_faces(Mesh& m, const dm_mesh_input& in) {
 
   // Boundary faces from the 2-D elements, owner found through the same table (:470-504).
   const int nb = int(in.n_bnd);
-  std::vector<int32_t> bown(nb), bpatch(nb);
+  std::vector<int32_t> bown(nb), bpatch(nb), bnodes(4 * size_t(nb));
   std::vector<double> bcen(3 * size_t(nb)), bnrm(3 * size_t(nb)), barea(nb);
   for (int e = 0; e < nb; ++e) {
     const int64_t o = in.bnd_offset[e];
@@ -268,6 +292,11 @@ void build_faces(Mesh& m, const dm_mesh_input& in) {
     double area;
     face_geometry(nodes, cnt, in.coords, cen, nrm, area);
     if (dot(nrm, sub(cen, m.centroid(bown[e]))) < 0.0) nrm = scl(-1.0, nrm);  // orient_outward_normal
+    {
+      std::vector<int32_t> o;
+      push_face_nodes(o, nodes, cnt, in.coords, nrm);
+      for (int i = 0; i < 4; ++i) bnodes[4 * size_t(e) + i] = i < cnt ? o[i] : -1;
+    }
     bcen[3 * e] = cen.x; bcen[3 * e + 1] = cen.y; bcen[3 * e + 2] = cen.z;
     bnrm[3 * e] = nrm.x; bnrm[3 * e + 1] = nrm.y; bnrm[3 * e + 2] = nrm.z;
     barea[e] = area;
@@ -304,14 +333,187 @@ void build_faces(Mesh& m, const dm_mesh_input& in) {
   m.face_centre = fcen;
   m.face_normal = fnrm;
   m.face_area = farea;
+  m.face_nodes = fnodes;
+  m.face_node_offset = foff;
   for (int i = 0; i < nb; ++i) {
     const int e = order[i];
+    for (int k = 0; k < 4 && bnodes[4 * size_t(e) + k] >= 0; ++k) m.face_nodes.push_back(bnodes[4 * size_t(e) + k]);
+    m.face_node_offset.push_back(int64_t(m.face_nodes.size()));
     m.face_owner.push_back(bown[e]);
     m.face_neighbour.push_back(-1);
     m.face_patch.push_back(pmap[bpatch[e]]);
     m.face_centre.insert(m.face_centre.end(), {bcen[3 * e], bcen[3 * e + 1], bcen[3 * e + 2]});
     m.face_normal.insert(m.face_normal.end(), {bnrm[3 * e], bnrm[3 * e + 1], bnrm[3 * e + 2]});
     m.face_area.push_back(barea[e]);
+  }
+}
+
+// Face-based input (OpenFOAM polyMesh layout): faces in input order, geometry by the reference's
+// polyhedral formulas (mod_mesh.f90:1336-1461), cell vertex lists in first-appearance order over
+// the cell's owner faces then its neighbour faces (build_cell_topology_metadata,
+// mod_foam_reader.f90:417-579), patches as given, empty ones included (:346-415).
+void build_faces_poly(Mesh& m, const dm_poly_input& in) {
+  const int64_t nf = in.n_faces, nint = in.n_int_faces, npts = in.n_points;
+  if (nf <= 0 || nint < 0 || nint > nf) throw std::runtime_error("bad face counts");
+  if (nf > INT32_MAX || npts > INT32_MAX) throw std::runtime_error("mesh too large for 32-bit indices");
+  if (in.face_offset[0] != 0) throw std::runtime_error("face offsets must start at 0");
+  for (int64_t f = 0; f < nf; ++f)
+    if (in.face_offset[f + 1] < in.face_offset[f]) throw std::runtime_error("face offsets are not monotonic");
+  for (int64_t k = 0; k < in.face_offset[nf]; ++k)
+    if (in.face_nodes[k] < 0 || in.face_nodes[k] >= npts) throw std::runtime_error("face point label out of range");
+  int64_t nc = 0;
+  for (int64_t f = 0; f < nf; ++f) {
+    if (in.owner[f] < 0) throw std::runtime_error("negative cell index in owner/neighbour");
+    nc = std::max<int64_t>(nc, int64_t(in.owner[f]) + 1);
+  }
+  for (int64_t f = 0; f < nint; ++f) {
+    if (in.neighbour[f] < 0) throw std::runtime_error("negative cell index in owner/neighbour");
+    nc = std::max<int64_t>(nc, int64_t(in.neighbour[f]) + 1);
+    if (in.owner[f] >= in.neighbour[f])
+      throw std::runtime_error("face " + std::to_string(f) + " violates owner < neighbour");
+  }
+  m.nc = int(nc);
+  m.nf = int(nf);
+  m.nint = int(nint);
+  m.nbnd = int(nf - nint);
+  m.nvert = int(npts);
+  m.vertex_coords.assign(in.points, in.points + 3 * npts);
+  m.face_node_offset.assign(in.face_offset, in.face_offset + nf + 1);
+  m.face_nodes.assign(in.face_nodes, in.face_nodes + in.face_offset[nf]);
+  m.face_owner.assign(in.owner, in.owner + nf);
+  m.face_neighbour.assign(nf, -1);
+  for (int64_t f = 0; f < nint; ++f) m.face_neighbour[f] = in.neighbour[f];
+
+  // patches: ranges inside the boundary faces, covering each boundary face once
+  m.npatch = in.n_patches;
+  m.face_patch.assign(nf, -1);
+  for (int p = 0; p < in.n_patches; ++p) {
+    const int64_t s = in.patch_start[p], n = in.patch_nfaces[p];
+    if (n < 0 || s < nint || s + n > nf)
+      throw std::runtime_error("patch " + std::to_string(p) + ": face range outside the boundary faces");
+    for (int64_t f = s; f < s + n; ++f) {
+      if (m.face_patch[f] >= 0) throw std::runtime_error("boundary face in two patches");
+      m.face_patch[f] = p;
+    }
+    m.patch_bc.push_back(in.patch_bc[p]);
+    m.patch_start.push_back(int32_t(s - nint));
+    m.patch_nfaces.push_back(int32_t(n));
+  }
+  for (int64_t f = nint; f < nf; ++f)
+    if (m.face_patch[f] < 0) throw std::runtime_error("boundary faces not mapped to a patch");
+
+  auto P = [&](int64_t k) {
+    const int v = in.face_nodes[k];
+    return V3{in.points[3 * size_t(v)], in.points[3 * size_t(v) + 1], in.points[3 * size_t(v) + 2]};
+  };
+  // compute_polygon_face_geometry (mod_mesh.f90:1415-1461)
+  m.face_centre.assign(3 * size_t(nf), 0.0);
+  m.face_normal.assign(3 * size_t(nf), 0.0);
+  m.face_area.assign(nf, 0.0);
+  for (int64_t f = 0; f < nf; ++f) {
+    const int64_t b = in.face_offset[f], e = in.face_offset[f + 1];
+    if (e - b < 3) {
+      m.face_area[f] = SMALL;
+      continue;
+    }
+    const V3 pref = P(b);
+    V3 nacc = {0.0, 0.0, 0.0};
+    for (int64_t i = b + 1; i < e - 1; ++i) nacc = add(nacc, scl(0.5, cross(sub(P(i), pref), sub(P(i + 1), pref))));
+    const double area = std::sqrt(nacc.x * nacc.x + nacc.y * nacc.y + nacc.z * nacc.z) + SMALL;
+    const V3 nrm = {nacc.x / area, nacc.y / area, nacc.z / area};
+    V3 ctr = {0.0, 0.0, 0.0};
+    double asum = 0.0;
+    for (int64_t i = b + 1; i < e - 1; ++i) {
+      const V3 pi = P(i), pj = P(i + 1);
+      const double ta = dot(scl(0.5, cross(sub(pi, pref), sub(pj, pref))), nrm);
+      const V3 t = add(add(pref, pi), pj);
+      ctr = add(ctr, V3{ta * t.x / 3.0, ta * t.y / 3.0, ta * t.z / 3.0});
+      asum += ta;
+    }
+    if (std::fabs(asum) > SMALL) ctr = {ctr.x / asum, ctr.y / asum, ctr.z / asum};
+    else ctr = pref;
+    m.face_area[f] = area;
+    m.face_normal[3 * f] = nrm.x;
+    m.face_normal[3 * f + 1] = nrm.y;
+    m.face_normal[3 * f + 2] = nrm.z;
+    m.face_centre[3 * f] = ctr.x;
+    m.face_centre[3 * f + 1] = ctr.y;
+    m.face_centre[3 * f + 2] = ctr.z;
+  }
+
+  // compute_polyhedron_cell_moments (mod_mesh.f90:1336-1412): signed tets about a per-cell
+  // reference point, the first point of the first face (of >= 3 points) that touches the cell
+  std::vector<V3> ref(nc), mom(nc, V3{0.0, 0.0, 0.0});
+  std::vector<char> ref_set(nc, 0);
+  std::vector<double> vol(nc, 0.0);
+  for (int64_t f = 0; f < nf; ++f) {
+    if (in.face_offset[f + 1] - in.face_offset[f] < 3) continue;
+    for (int side = 0; side < 2; ++side) {
+      const int c = side == 0 ? in.owner[f] : (f < nint ? in.neighbour[f] : -1);
+      if (c < 0 || ref_set[c]) continue;
+      ref[c] = P(in.face_offset[f]);
+      ref_set[c] = 1;
+    }
+  }
+  for (int64_t f = 0; f < nf; ++f) {
+    const int64_t b = in.face_offset[f], e = in.face_offset[f + 1];
+    if (e - b < 3) continue;
+    for (int side = 0; side < 2; ++side) {
+      const int c = side == 0 ? in.owner[f] : (f < nint ? in.neighbour[f] : -1);
+      if (c < 0) break;
+      const V3 r = ref[c], v0 = sub(P(b), r);
+      for (int64_t i = b + 1; i < e - 1; ++i) {
+        const V3 v1 = sub(P(i), r), v2 = sub(P(i + 1), r);
+        const double sv = dot(v0, cross(v1, v2)) / 6.0;
+        const V3 t = add(add(v0, v1), v2);
+        const V3 sm = {sv * t.x / 4.0, sv * t.y / 4.0, sv * t.z / 4.0};
+        if (side == 0) {
+          vol[c] += sv;
+          mom[c] = add(mom[c], sm);
+        } else {
+          vol[c] -= sv;
+          mom[c] = sub(mom[c], sm);
+        }
+      }
+    }
+  }
+  m.cell_centroid.assign(3 * size_t(nc), 0.0);
+  m.cell_volume.assign(nc, 0.0);
+  for (int64_t c = 0; c < nc; ++c) {
+    if (std::fabs(vol[c]) > SMALL) {
+      m.cell_centroid[3 * c] = ref[c].x + mom[c].x / vol[c];
+      m.cell_centroid[3 * c + 1] = ref[c].y + mom[c].y / vol[c];
+      m.cell_centroid[3 * c + 2] = ref[c].z + mom[c].z / vol[c];
+    }
+    m.cell_volume[c] = std::fabs(vol[c]);
+  }
+
+  // cell vertex lists: owner faces in face order, then neighbour faces, first appearance
+  std::vector<int64_t> cfo(nc + 1, 0);
+  for (int64_t f = 0; f < nf; ++f) cfo[in.owner[f] + 1]++;
+  for (int64_t f = 0; f < nint; ++f) cfo[in.neighbour[f] + 1]++;
+  for (int64_t c = 0; c < nc; ++c) cfo[c + 1] += cfo[c];
+  std::vector<int32_t> cf(cfo[nc]);
+  {
+    std::vector<int64_t> fill(cfo.begin(), cfo.end() - 1);
+    for (int64_t f = 0; f < nf; ++f) cf[fill[in.owner[f]]++] = int32_t(f);
+    for (int64_t f = 0; f < nint; ++f) cf[fill[in.neighbour[f]]++] = int32_t(f);
+  }
+  std::vector<int64_t> marker(npts, -1);
+  m.cell_vertex_offset.assign(1, 0);
+  m.cell_vertices.clear();
+  for (int64_t c = 0; c < nc; ++c) {
+    for (int64_t k = cfo[c]; k < cfo[c + 1]; ++k) {
+      const int f = cf[k];
+      for (int64_t q = in.face_offset[f]; q < in.face_offset[f + 1]; ++q) {
+        const int v = in.face_nodes[q];
+        if (marker[v] != c) {
+          marker[v] = c;
+          m.cell_vertices.push_back(v);
+        }
+      }
+    }
+    m.cell_vertex_offset.push_back(int64_t(m.cell_vertices.size()));
   }
 }
 
@@ -598,6 +800,28 @@ void* dm_mesh_build(const dm_mesh_input* in) {
   }
 }
 
+void* dm_mesh_build_poly(const dm_poly_input* in) {
+  g_err.clear();
+  try {
+    if (!in || in->n_faces <= 0) throw std::runtime_error("empty mesh input");
+    auto* m = new Mesh();
+    try {
+      build_faces_poly(*m, *in);
+      build_connectivity(*m);
+      compute_geometry(*m);
+      build_periodic_pairs(*m);
+      build_ls_stencil(*m);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    return m;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
 int dm_mesh_get_view(const void* mesh, dm_mesh_view* v) {
   const Mesh& m = *static_cast<const Mesh*>(mesh);
   std::memset(v, 0, sizeof(*v));
@@ -638,6 +862,8 @@ int dm_mesh_get_view(const void* mesh, dm_mesh_view* v) {
   v->vertex_coords = m.vertex_coords.data();
   v->cell_vertex_offset = m.cell_vertex_offset.data();
   v->cell_vertices = m.cell_vertices.data();
+  v->face_node_offset = m.face_node_offset.data();
+  v->face_nodes = m.face_nodes.data();
   return 0;
 }
 

@@ -54,7 +54,15 @@ class _View(ctypes.Structure):
                 ("patch_bc", _P(_i32)), ("patch_start", _P(_i32)), ("patch_nfaces", _P(_i32)),
                 ("periodic_pair", _P(_i32)), ("periodic_pair_dcf", _P(_d)), ("periodic_gc", _P(_d)),
                 ("ls_offset", _P(_i32)), ("ls_cell", _P(_i32)), ("ls_face", _P(_i32)), ("ls_weight", _P(_d)),
-                ("vertex_coords", _P(_d)), ("cell_vertex_offset", _P(_i64)), ("cell_vertices", _P(_i32))]
+                ("vertex_coords", _P(_d)), ("cell_vertex_offset", _P(_i64)), ("cell_vertices", _P(_i32)),
+                ("face_node_offset", _P(_i64)), ("face_nodes", _P(_i32))]
+
+
+class _PolyInput(ctypes.Structure):
+    _fields_ = [("n_points", _i64), ("points", _P(_d)), ("n_faces", _i64), ("face_offset", _P(_i64)),
+                ("face_nodes", _P(_i32)), ("n_int_faces", _i64), ("owner", _P(_i32)), ("neighbour", _P(_i32)),
+                ("n_patches", _i32), ("patch_start", _P(_i64)), ("patch_nfaces", _P(_i64)),
+                ("patch_bc", _P(_i32))]
 
 
 def _ptr(a, ct):
@@ -82,23 +90,49 @@ class Mesh:
               "ls_weight")
 
     def __init__(self, elements):
+        """``elements``: Gmsh-style element lists (box_elements, read_gmsh_elements) or face-based
+        input (read_foam_input: keys points, face_offset, face_nodes, owner, neighbour,
+        patch_start, patch_nfaces, patch_bc, patch_names)."""
         L = lib()
         L.dm_mesh_build.restype = ctypes.c_void_p
         L.dm_mesh_build.argtypes = [ctypes.POINTER(_Input)]
+        L.dm_mesh_build_poly.restype = ctypes.c_void_p
+        L.dm_mesh_build_poly.argtypes = [ctypes.POINTER(_PolyInput)]
         L.dm_mesh_get_view.argtypes = [ctypes.c_void_p, ctypes.POINTER(_View)]
         L.dm_mesh_free.argtypes = [ctypes.c_void_p]
         L.dm_last_error.restype = ctypes.c_char_p
         e = elements
-        keep = {k: np.ascontiguousarray(v) for k, v in e.items() if isinstance(v, np.ndarray)}
-        inp = _Input(
-            n_nodes=len(keep["coords"]) // 3, coords=_ptr(keep["coords"], _d),
-            n_vol=len(keep["vol_type"]), vol_type=_ptr(keep["vol_type"], _i32),
-            vol_offset=_ptr(keep["vol_offset"], _i64), vol_nodes=_ptr(keep["vol_nodes"], _i32),
-            n_bnd=len(keep["bnd_type"]), bnd_type=_ptr(keep["bnd_type"], _i32),
-            bnd_offset=_ptr(keep["bnd_offset"], _i64), bnd_nodes=_ptr(keep["bnd_nodes"], _i32),
-            bnd_group=_ptr(keep["bnd_group"], _i32), n_groups=len(keep["group_bc"]),
-            group_bc=_ptr(keep["group_bc"], _i32))
-        h = L.dm_mesh_build(ctypes.byref(inp))
+        poly = "face_offset" in e
+        if poly:
+            keep = {
+                "points": np.ascontiguousarray(e["points"], dtype=np.float64).reshape(-1),
+                "face_offset": np.ascontiguousarray(e["face_offset"], dtype=np.int64),
+                "face_nodes": np.ascontiguousarray(e["face_nodes"], dtype=np.int32),
+                "owner": np.ascontiguousarray(e["owner"], dtype=np.int32),
+                "neighbour": np.ascontiguousarray(e["neighbour"], dtype=np.int32),
+                "patch_start": np.ascontiguousarray(e["patch_start"], dtype=np.int64),
+                "patch_nfaces": np.ascontiguousarray(e["patch_nfaces"], dtype=np.int64),
+                "patch_bc": np.ascontiguousarray(e["patch_bc"], dtype=np.int32),
+            }
+            inp = _PolyInput(
+                n_points=len(keep["points"]) // 3, points=_ptr(keep["points"], _d),
+                n_faces=len(keep["face_offset"]) - 1, face_offset=_ptr(keep["face_offset"], _i64),
+                face_nodes=_ptr(keep["face_nodes"], _i32), n_int_faces=len(keep["neighbour"]),
+                owner=_ptr(keep["owner"], _i32), neighbour=_ptr(keep["neighbour"], _i32),
+                n_patches=len(keep["patch_bc"]), patch_start=_ptr(keep["patch_start"], _i64),
+                patch_nfaces=_ptr(keep["patch_nfaces"], _i64), patch_bc=_ptr(keep["patch_bc"], _i32))
+            h = L.dm_mesh_build_poly(ctypes.byref(inp))
+        else:
+            keep = {k: np.ascontiguousarray(v) for k, v in e.items() if isinstance(v, np.ndarray)}
+            inp = _Input(
+                n_nodes=len(keep["coords"]) // 3, coords=_ptr(keep["coords"], _d),
+                n_vol=len(keep["vol_type"]), vol_type=_ptr(keep["vol_type"], _i32),
+                vol_offset=_ptr(keep["vol_offset"], _i64), vol_nodes=_ptr(keep["vol_nodes"], _i32),
+                n_bnd=len(keep["bnd_type"]), bnd_type=_ptr(keep["bnd_type"], _i32),
+                bnd_offset=_ptr(keep["bnd_offset"], _i64), bnd_nodes=_ptr(keep["bnd_nodes"], _i32),
+                bnd_group=_ptr(keep["bnd_group"], _i32), n_groups=len(keep["group_bc"]),
+                group_bc=_ptr(keep["group_bc"], _i32))
+            h = L.dm_mesh_build(ctypes.byref(inp))
         if not h:
             raise RuntimeError("mesh build failed: " + L.dm_last_error().decode())
         try:
@@ -126,12 +160,20 @@ class Mesh:
             self.ls_face = _copy(v.ls_face, v.n_ls)
             self.ls_weight = _copy(v.ls_weight, 3 * v.n_ls, (v.n_ls, 3))
             self.vertex_coords = _copy(v.vertex_coords, 3 * v.n_vertices, (v.n_vertices, 3))
+            self.cell_vertex_offset = _copy(v.cell_vertex_offset, nc + 1)
+            self.cell_vertices = _copy(v.cell_vertices, int(self.cell_vertex_offset[-1]))
+            self.face_node_offset = _copy(v.face_node_offset, nf + 1)
+            self.face_nodes = _copy(v.face_nodes, int(self.face_node_offset[-1]))
         finally:
             L.dm_mesh_free(h)
-        # patches are the groups that hold boundary faces, in group order (an empty group gets none)
-        names = list(elements.get("group_names", []))
-        used = np.bincount(keep["bnd_group"], minlength=len(keep["group_bc"])) > 0
-        self.patch_names = [n for n, u in zip(names, used) if u] if names else []
+        if poly:
+            # every patch of the input, empty ones included
+            self.patch_names = list(elements.get("patch_names", []))
+        else:
+            # patches are the groups that hold boundary faces, in group order (an empty group gets none)
+            names = list(elements.get("group_names", []))
+            used = np.bincount(keep["bnd_group"], minlength=len(keep["group_bc"])) > 0
+            self.patch_names = [n for n, u in zip(names, used) if u] if names else []
         self.bounds_min = self.vertex_coords.min(axis=0)
         self.bounds_max = self.vertex_coords.max(axis=0)
 
@@ -445,3 +487,139 @@ def write_gmsh(elements, path):
     lines += ["$Elements", f"{len(blocks)} {tag - 1} 1 {tag - 1}"] + body + ["$EndElements"]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def _resolve_patch_types(names, own_bc, patch_types, warn):
+    """bc id per patch: the case configuration's patch type by name, else the mesh's own
+    (src/bc/mod_bc_setup.f90:42-106); an input name missing from the mesh is an error."""
+    types = dict(patch_types or {})
+    missing = [k for k in types if k not in names]
+    if missing:
+        raise ValueError("input BC patch names not in the mesh: " + ", ".join(missing))
+    out = []
+    for n, b in zip(names, own_bc):
+        if n in types:
+            out.append(BC_NAMES[types[n].lower()])
+        else:
+            if warn:
+                warn(f'mesh patch "{n}" is not listed in the patch types; using its mesh type (bc id {b})')
+            out.append(b)
+    return np.array(out, dtype=np.int32)
+
+
+def read_foam_input(polymesh_dir, patch_types=None, warn=print):
+    """Face-based input of an OpenFOAM constant/polyMesh directory, parsed natively
+    (src/mesh/mod_foam_reader.f90:16-97). A patch's bc id comes from ``patch_types`` by name when
+    listed, else from its OpenFOAM ``type`` (bc_type_to_id, :1144-1167)."""
+    L = lib()
+    L.dm_foam_read.restype = ctypes.c_void_p
+    L.dm_foam_read.argtypes = [ctypes.c_char_p]
+    L.dm_foam_get_view.argtypes = [ctypes.c_void_p, ctypes.POINTER(_PolyInput)]
+    L.dm_foam_patch_name.restype = ctypes.c_char_p
+    L.dm_foam_patch_name.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dm_foam_patch_type.restype = ctypes.c_char_p
+    L.dm_foam_patch_type.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dm_foam_free.argtypes = [ctypes.c_void_p]
+    L.dm_foam_last_error.restype = ctypes.c_char_p
+    h = L.dm_foam_read(str(polymesh_dir).encode())
+    if not h:
+        raise ValueError(f"{polymesh_dir}: " + L.dm_foam_last_error().decode())
+    try:
+        v = _PolyInput()
+        L.dm_foam_get_view(h, ctypes.byref(v))
+        nf, npch = v.n_faces, v.n_patches
+        e = {
+            "points": _copy(v.points, 3 * v.n_points, (v.n_points, 3)),
+            "face_offset": _copy(v.face_offset, nf + 1),
+            "owner": _copy(v.owner, nf),
+            "neighbour": _copy(v.neighbour, v.n_int_faces),
+            "patch_start": _copy(v.patch_start, npch),
+            "patch_nfaces": _copy(v.patch_nfaces, npch),
+            "patch_names": [L.dm_foam_patch_name(h, p).decode() for p in range(npch)],
+            "patch_types": [L.dm_foam_patch_type(h, p).decode() for p in range(npch)],
+        }
+        e["face_nodes"] = _copy(v.face_nodes, int(e["face_offset"][-1]))
+        own_bc = _copy(v.patch_bc, npch)
+    finally:
+        L.dm_foam_free(h)
+    e["patch_bc"] = _resolve_patch_types(e["patch_names"], own_bc, patch_types, warn)
+    return e
+
+
+def read_foam(polymesh_dir, patch_types=None, warn=print):
+    """Mesh of an OpenFOAM polyMesh directory; see read_foam_input."""
+    return Mesh(read_foam_input(polymesh_dir, patch_types, warn))
+
+
+# OpenFOAM patch type written for each bc id: the inverse of bc_type_to_id where it has one
+# (src/mesh/mod_foam_reader.f90:1144-1167); other ids are written as "patch", whose bc id then has
+# to come from the case's patch types.
+_FOAM_TYPE = {BC_INLET: "inlet", BC_OUTLET: "outlet", BC_ZEROGRADIENT: "zeroGradient", BC_WALL_MODEL: "wall",
+              BC_FIXEDTEMP: "fixedTemperature", BC_WALL_VELOCITY: "movingWall", BC_SYMMETRY: "symmetryPlane",
+              BC_SLIP: "empty", BC_PERIODIC: "cyclic"}
+
+
+def write_foam(mesh, polymesh_dir, binary=False, label_bytes=4, scalar_bytes=8, msb=False, compact=True):
+    """Write a built Mesh as an OpenFOAM polyMesh directory (points, faces, owner, neighbour,
+    boundary) that read_foam reads back with the same faces, cells and patches.
+
+    Faces keep the mesh's order (internal faces first, each with owner < neighbour; boundary faces
+    by patch) and their points run so that the right-hand normal points from owner to neighbour
+    or out of the domain. ASCII writes coordinates with '%.17g'; binary writes raw labels and
+    scalars in the requested widths and byte order, faces as a compact offset/label pair
+    (``compact``) or as (n, labels) records.
+    """
+    import os
+    os.makedirs(polymesh_dir, exist_ok=True)
+    order = ">" if msb else "<"
+    lt, st = np.dtype(f"{order}i{label_bytes}"), np.dtype(f"{order}f{scalar_bytes}")
+    arch = f'"{"MSB" if msb else "LSB"};label={8 * label_bytes};scalar={8 * scalar_bytes}"'
+
+    def header(cls, obj, fmt):
+        return ("FoamFile\n{\n    version     2.0;\n    format      " + fmt + ";\n    arch        " + arch +
+                ";\n    class       " + cls + ";\n    location    \"constant/polyMesh\";\n    object      " +
+                obj + ";\n}\n\n").encode()
+
+    def write(name, cls, lists, ascii_body=None):
+        with open(os.path.join(polymesh_dir, name), "wb") as f:
+            f.write(header(cls, name, "binary" if binary else "ascii"))
+            if binary:
+                for a in lists:  # a list of vectors is counted in vectors
+                    f.write(f"{len(a)}\n(".encode())
+                    f.write(np.ascontiguousarray(a).tobytes())
+                    f.write(b")\n")
+            else:
+                f.write(ascii_body.encode())
+
+    nf, nint = mesh.n_faces, mesh.n_int_faces
+    off, nodes = mesh.face_node_offset, mesh.face_nodes
+    pts = mesh.vertex_coords
+    write("points", "vectorField", [pts.astype(st)],
+          f"{len(pts)}\n(\n" + "".join(f"({x:.17g} {y:.17g} {z:.17g})\n" for x, y, z in pts) + ")\n")
+    if binary and compact:
+        write("faces", "faceCompactList", [off.astype(lt), nodes.astype(lt)])
+    elif binary:
+        rec = np.concatenate([np.concatenate([[off[f + 1] - off[f]], nodes[off[f]:off[f + 1]]]) for f in range(nf)])
+        with open(os.path.join(polymesh_dir, "faces"), "wb") as fh:
+            fh.write(header("faceList", "faces", "binary"))
+            fh.write(f"{nf}\n(".encode())
+            fh.write(rec.astype(lt).tobytes())
+            fh.write(b")\n")
+    else:
+        write("faces", "faceList", None, f"{nf}\n(\n" + "".join(
+            f"{off[f + 1] - off[f]}(" + " ".join(str(int(x)) for x in nodes[off[f]:off[f + 1]]) + ")\n"
+            for f in range(nf)) + ")\n")
+    write("owner", "labelList", [mesh.face_owner.astype(lt)],
+          f"{nf}\n(\n" + "".join(f"{int(x)}\n" for x in mesh.face_owner) + ")\n")
+    nb = mesh.face_neighbour[:nint]
+    write("neighbour", "labelList", [nb.astype(lt)], f"{nint}\n(\n" + "".join(f"{int(x)}\n" for x in nb) + ")\n")
+    names = mesh.patch_names or [f"patch{p}" for p in range(mesh.n_patches)]
+    body = f"{mesh.n_patches}\n(\n"
+    for p in range(mesh.n_patches):
+        body += (f"    {names[p]}\n    {{\n        type            {_FOAM_TYPE.get(int(mesh.patch_bc[p]), 'patch')};\n"
+                 f"        nFaces          {int(mesh.patch_nfaces[p])};\n"
+                 f"        startFace       {nint + int(mesh.patch_start[p])};\n    }}\n")
+    body += ")\n"
+    with open(os.path.join(polymesh_dir, "boundary"), "wb") as f:
+        f.write(header("polyBoundaryMesh", "boundary", "ascii"))
+        f.write(body.encode())

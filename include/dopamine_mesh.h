@@ -96,6 +96,9 @@ typedef struct dm_mesh_view {
   const double* vertex_coords;      /* 3*n_vertices */
   const int64_t* cell_vertex_offset;
   const int32_t* cell_vertices;
+  /* face point lists, ordered so that the right-hand normal points along face_normal */
+  const int64_t* face_node_offset;  /* nf+1 */
+  const int32_t* face_nodes;
 } dm_mesh_view;
 
 /* Build a mesh. Returns an opaque handle, or NULL with a message available from dm_last_error(). */
@@ -105,6 +108,47 @@ int dm_mesh_get_view(const void* mesh, dm_mesh_view* view);
 void dm_mesh_free(void* mesh);
 /* Last error message of this library (thread-local); empty when none. */
 const char* dm_last_error(void);
+
+/* Face-based (polyhedral) input, the OpenFOAM polyMesh layout: internal faces first, each with
+ * owner < neighbour, then the boundary faces, every one inside exactly one patch range.
+ * Geometry follows the reference's polyhedral path (src/mesh/mod_mesh.f90:1336-1461: fan
+ * triangulated face area, normal and area-weighted centre; cell volume and centroid from signed
+ * tets about a per-cell reference vertex), then the same adjacency, weights, periodic pairing and
+ * least-squares stencil as dm_mesh_build. Faces keep their input order. */
+typedef struct dm_poly_input {
+  int64_t n_points;
+  const double* points;        /* 3*n_points */
+  int64_t n_faces;
+  const int64_t* face_offset;  /* n_faces+1 offsets into face_nodes */
+  const int32_t* face_nodes;   /* 0-based point labels, right-hand normal owner -> neighbour / outward */
+  int64_t n_int_faces;
+  const int32_t* owner;        /* n_faces */
+  const int32_t* neighbour;    /* n_int_faces */
+  int32_t n_patches;
+  const int64_t* patch_start;  /* first (global) face of each patch */
+  const int64_t* patch_nfaces;
+  const int32_t* patch_bc;     /* bc id per patch (patch type after name matching) */
+} dm_poly_input;
+/* NULL with dm_last_error() on a negative cell index, owner >= neighbour, a point label out of
+ * range, a boundary face in no patch or in two, or a patch range outside the boundary faces. */
+void* dm_mesh_build_poly(const dm_poly_input* in);
+
+/* OpenFOAM polyMesh reader (foam_read, src/mesh/mod_foam_reader.f90:16-97): points, faces,
+ * owner, neighbour, boundary of a constant/polyMesh directory, ASCII or binary (label 32/64,
+ * scalar 32/64, LSB/MSB, compact or plain face lists). The view is a dm_poly_input whose patch_bc
+ * holds the bc id of each patch's `type` entry (bc_type_to_id, :1144-1167); the case
+ * configuration may override it by patch name. NULL with dm_foam_last_error() on an unreadable
+ * or malformed file and on the reference's topology checks (negative cell index, owner >=
+ * neighbour, unmapped boundary face, patch range outside the boundary faces, odd number of
+ * cyclic patches). */
+void* dm_foam_read(const char* polymesh_dir);
+int dm_foam_get_view(const void* file, dm_poly_input* view);
+const char* dm_foam_patch_name(const void* file, int patch);
+const char* dm_foam_patch_type(const void* file, int patch);
+/* bc id of an OpenFOAM patch type (bc_type_to_id; unknown types give zeroGradient). */
+int32_t dm_foam_bc_of_type(const char* type);
+void dm_foam_free(void* file);
+const char* dm_foam_last_error(void);
 
 /* Gmsh v4 ASCII reader (gmsh_read, src/mesh/mod_gmsh_reader.f90:45-236): nodes in file order,
  * 3-D elements as volume elements in file order, 2-D elements as boundary elements tagged by
