@@ -4,6 +4,7 @@ and the benchmark configurations.
 Regression cases follow tests/regression/cases/<name>/case.input with meshes from
 tests/regression/mk_box_mesh.py (its node, hex and quad order are what mesh.box_elements emits).
 """
+import dataclasses
 import math
 from dataclasses import dataclass, field
 from typing import Callable
@@ -78,6 +79,13 @@ def regression_cases():
                      p_max_iter=1000, n_nonortho_corr=4, rhie_chow_dual=False, cell_projection="LS",
                      pressure_min_cos_theta=0.0, patch_p={"z0": 1.0, "zL": 0.0}),
         ic="uniform", ic_uref=0.0, nsteps=20, stats_every=10)
+    # tests/regression/cases/{tgv_skew,hydro}_exact/case.input: the same cases with the exact
+    # V-orthogonal cell projection
+    for base in ("tgv_skew", "hydro"):
+        c = cases[base]
+        cases[base + "_exact"] = Case(base + "_exact", c.make_mesh,
+                                      dataclasses.replace(c.params, cell_projection="EXACT"), c.ic, c.ic_uref,
+                                      c.nsteps, c.stats_every)
     return cases
 
 

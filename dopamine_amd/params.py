@@ -44,7 +44,11 @@ class SolverParams:
     pressure_min_cos_theta: float = 0.0
     rhie_chow_dual: bool = True
     nonortho_check: bool = False
+    # cell velocity correction: "LS" (correct_velocity_psolver_dual) or "EXACT" (V-orthogonal
+    # projection by flexible PCG with the pressure solve as preconditioner); the reference's
+    # parser default is EXACT (src/io/mod_input_parser.f90:200)
     cell_projection: str = "EXACT"
+    cell_projection_tol: float = 1.0e-1
     use_rotational_projection: bool = True
     p_tol: float = 1.0e-10
     p_max_iter: int = 1000
@@ -76,9 +80,10 @@ class SolverParams:
     patch_U: dict = field(default_factory=dict)
 
     def validate(self):
-        if self.cell_projection.upper() != "LS":
-            raise ValueError("cell_projection must be 'LS' (the EXACT V-orthogonal projection is not part of this "
-                             "hot path)")
+        if self.cell_projection.upper() not in ("LS", "EXACT"):
+            raise ValueError(f"cell_projection {self.cell_projection!r} not supported (LS, EXACT)")
+        if not self.cell_projection_tol > 0.0:
+            raise ValueError("cell_projection_tol must be positive")
         if self.advection_scheme.upper() not in SCHEME_IDS:
             raise ValueError(f"advection_scheme {self.advection_scheme!r} not supported "
                              "(KOREN_TVD, VANLEER_TVD, QUICK, KEP, CDS)")
@@ -95,6 +100,9 @@ class SolverParams:
             raise ValueError(f"precond {self.precond!r} not supported (JACOBI, LEARNED)")
         if self.adapt_dt:
             raise ValueError("adapt_dt is not supported; the fixed-dt CFL clamp of solver_advance applies")
+
+    def projection_id(self):
+        return 1 if self.cell_projection.upper() == "EXACT" else 0
 
     def scheme_id(self):
         return SCHEME_IDS[self.advection_scheme.upper()]

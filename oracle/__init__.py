@@ -37,7 +37,8 @@ class _Params(ctypes.Structure):
                 ("dt", _d), ("patch_p_val", _P(_d)), ("patch_U_val", _P(_d)), ("c_amd", _d),
                 ("amd_wall_gradient", ctypes.c_int), ("boussinesq", ctypes.c_int), ("alpha_T", _d), ("beta", _d),
                 ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d), ("t_scheme", ctypes.c_int), ("patch_T_val", _P(_d)),
-                ("most_z0_m", _d), ("most_z0_h", _d), ("T_surface", _d), ("L_obukhov", _d), ("compute_L", ctypes.c_int)]
+                ("most_z0_m", _d), ("most_z0_h", _d), ("T_surface", _d), ("L_obukhov", _d), ("compute_L", ctypes.c_int),
+                ("cell_projection", ctypes.c_int), ("cell_projection_tol", _d)]
 
 
 _lib = None
@@ -69,6 +70,10 @@ def lib():
         L.om_amd_point.restype = _d
         L.om_amd_point.argtypes = [_d * 9, _d * 9, _d]
         L.om_counters.argtypes = [ctypes.c_void_p, _i64 * 3]
+        L.om_proj_counters.argtypes = [ctypes.c_void_p, _i64 * 2]
+        L.om_proj_divergence.argtypes = [ctypes.c_void_p, _P(_d), _P(_d)]
+        L.om_proj_adjoint.argtypes = [ctypes.c_void_p, _P(_d), _P(_d)]
+        L.om_proj_project.argtypes = [ctypes.c_void_p, _P(_d), _P(_d), ctypes.c_int, _d, ctypes.c_int]
         L.om_most_iterate.argtypes = [_d, _d, _d, _d, _d, _d, _d, _d * 3]
         L.om_psi_m.restype = _d
         L.om_psi_m.argtypes = [_d]
@@ -123,8 +128,10 @@ class Solver:
                            params.dt, arr(pv, _d), arr(uv.reshape(-1), _d), params.c_amd,
                            int(params.amd_wall_gradient), int(params.boussinesq), params.alpha_T, params.beta,
                            params.T_ref, (_d * 3)(*params.g), params.Pr_t, params.t_scheme_id(), arr(tv, _d),
-                           params.z0_m, params.z0_h, params.T_surface, params.L_obukhov, int(params.compute_L))
+                           params.z0_m, params.z0_h, params.T_surface, params.L_obukhov, int(params.compute_L),
+                           params.projection_id(), params.cell_projection_tol)
         self.params = params
+        self.mesh_nc = mesh.n_cells
         self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))
         if not self.h:
             raise ValueError("oracle om_create refused the set-up (MOST: first cell inside the roughness layer)")
@@ -196,6 +203,32 @@ class Solver:
         out = (_i64 * 3)()
         lib().om_counters(self.h, out)
         return {"solves": out[0], "iterations": out[1], "last": out[2]}
+
+    def proj_counters(self):
+        out = (_i64 * 2)()
+        lib().om_proj_counters(self.h, out)
+        return {"projections": out[0], "iterations": out[1]}
+
+    def proj_divergence(self, u):
+        """B u on the solver's projection faces (cp_divergence)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        out = np.zeros(self.mesh_nc)
+        lib().om_proj_divergence(self.h, _p(u, _d), _p(out, _d))
+        return out
+
+    def proj_adjoint(self, chi):
+        """V^-1 B^T chi (cp_adjoint_gradient), shape (nc, 3)."""
+        chi = np.ascontiguousarray(chi, dtype=np.float64)
+        v = np.zeros(3 * self.mesh_nc)
+        lib().om_proj_adjoint(self.h, _p(chi, _d), _p(v, _d))
+        return v.reshape(-1, 3)
+
+    def proj_project(self, u, bconst=None, precond="pressure", tol=1e-1, max_it=100):
+        """cp_project on a copy of u; returns (projected u, iterations)."""
+        u = np.array(u, dtype=np.float64).reshape(-1)
+        bc = np.zeros(self.mesh_nc) if bconst is None else np.ascontiguousarray(bconst, dtype=np.float64)
+        its = lib().om_proj_project(self.h, _p(u, _d), _p(bc, _d), 0 if precond == "identity" else 1, tol, max_it)
+        return u.reshape(-1, 3), its
 
     def nonortho_info(self):
         out = (ctypes.c_int * 3)()

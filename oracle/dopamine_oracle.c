@@ -7,6 +7,7 @@
  */
 #include "dopamine_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -62,6 +63,11 @@ struct om_solver {
   double *T, *T_bnd, *grad_T, *gT_prev, *H_T, *H_T0, *conv_T, *visc_T;
   /* MOST: per-face heat flux and Obukhov length, the configuration's L */
   double *q_w, *L_bnd, L_cfg, most_z0h;
+  /* exact cell projection: pressure faces that are not skipped, outlet boundary faces */
+  int cp_n, cp_nout;
+  int *cp_own, *cp_nbr, *cp_bcell;
+  double *cp_sf, *cp_gc, *cp_bsf;
+  int64_t n_proj, n_proj_its;
 };
 
 static double *dz(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
@@ -1594,6 +1600,15 @@ static void face_fluxes_and_rhs(om_solver *s, double dt, int incr) {
     else
       flux = s->U_bnd[3 * b] * s->m.face_sf[3 * f] + s->U_bnd[3 * b + 1] * s->m.face_sf[3 * f + 1] +
              s->U_bnd[3 * b + 2] * s->m.face_sf[3 * f + 2];
+    if (bc == BC_OUTLET && s->p.cell_projection == OM_PROJ_EXACT) {
+      /* EXACT: the compact old-pressure term of the outlet face (mod_fv_solver.f90:2629-2636) */
+      const double *n = &s->m.face_normal[3 * f];
+      const double d1 = s->m.face_centre[3 * f] - s->m.cell_centroid[3 * o];
+      const double d2 = s->m.face_centre[3 * f + 1] - s->m.cell_centroid[3 * o + 1];
+      const double d3 = s->m.face_centre[3 * f + 2] - s->m.cell_centroid[3 * o + 2];
+      const double c_b = s->m.face_area[f] / fmax(fabs(d1 * n[0] + d2 * n[1] + d3 * n[2]), SMALL);
+      flux = flux - dt * c_b * (s->outlet_pb[b] - s->phi_prev[o]);
+    }
     ff[f] = flux;
     s->built[f] = 1;
     rhs[o] = rhs[o] - flux;
@@ -1839,6 +1854,186 @@ static int pressure_solve(om_solver *s, double atol, int ptot) {
   return rc;
 }
 
+/* ---------------------------------------------------------------- exact cell projection */
+
+/* build_cell_projection (src/solver/mod_fv_solver.f90:3164-3221): the pressure faces that are not
+ * skipped, with the mesh face's area vector and the interpolation weight (face gc, periodic gc on
+ * periodic edges), then the outlet faces among the non-periodic boundary faces. */
+static void build_cell_projection(om_solver *s) {
+  s->cp_own = iz(s->npf); s->cp_nbr = iz(s->npf);
+  s->cp_sf = dz(3 * (size_t)s->npf); s->cp_gc = dz(s->npf);
+  int n = 0;
+  for (int idx = 0; idx < s->npf; ++idx) {
+    if (s->ps_skip[idx]) continue;
+    const int fm = s->ps_fmesh[idx];
+    s->cp_own[n] = s->ps_owner[idx];
+    s->cp_nbr[n] = s->ps_nb[idx];
+    for (int k = 0; k < 3; ++k) s->cp_sf[3 * n + k] = s->m.face_sf[3 * fm + k];
+    s->cp_gc[n] = (fm >= s->nint) ? s->m.periodic_gc[fm - s->nint] : s->m.face_gc[fm];
+    n++;
+  }
+  s->cp_n = n;
+  int nout = 0;
+  for (int ib = 0; ib < s->n_nper_bf; ++ib)
+    if (bc_of_face(s, s->nint + s->nper_bf[ib]) == BC_OUTLET) nout++;
+  s->cp_bcell = iz(nout);
+  s->cp_bsf = dz(3 * (size_t)nout);
+  nout = 0;
+  for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+    const int f = s->nint + s->nper_bf[ib];
+    if (bc_of_face(s, f) != BC_OUTLET) continue;
+    s->cp_bcell[nout] = s->m.face_owner[f];
+    for (int k = 0; k < 3; ++k) s->cp_bsf[3 * nout + k] = s->m.face_sf[3 * f + k];
+    nout++;
+  }
+  s->cp_nout = nout;
+}
+
+/* cp_divergence (mod_cell_projection.f90:89-130): outflow sum of the gc-interpolated flux
+ * (gc u_own + (1-gc) u_nbr).Sf, plus u.Sf on outlet faces. */
+void om_proj_divergence(om_solver *s, const double *u, double *out) {
+  for (int c = 0; c < s->nc; ++c) out[c] = 0.0;
+  for (int f = 0; f < s->cp_n; ++f) {
+    const int o = s->cp_own[f], n = s->cp_nbr[f];
+    const double w = s->cp_gc[f], *sf = &s->cp_sf[3 * f];
+    const double fl = (w * u[3 * o] + (1.0 - w) * u[3 * n]) * sf[0] + (w * u[3 * o + 1] + (1.0 - w) * u[3 * n + 1]) * sf[1] +
+                      (w * u[3 * o + 2] + (1.0 - w) * u[3 * n + 2]) * sf[2];
+    out[o] = out[o] + fl;
+    out[n] = out[n] - fl;
+  }
+  for (int f = 0; f < s->cp_nout; ++f) {
+    const int o = s->cp_bcell[f];
+    const double *bsf = &s->cp_bsf[3 * f];
+    const double fl = u[3 * o] * bsf[0] + u[3 * o + 1] * bsf[1] + u[3 * o + 2] * bsf[2];
+    out[o] = out[o] + fl;
+  }
+}
+
+/* cp_adjoint_gradient (:133-188): v = V^-1 B^T chi, zero Dirichlet chi on outlet faces. */
+void om_proj_adjoint(om_solver *s, const double *chi, double *v) {
+  memset(v, 0, sizeof(double) * 3 * (size_t)s->nc);
+  for (int f = 0; f < s->cp_n; ++f) {
+    const int o = s->cp_own[f], n = s->cp_nbr[f];
+    const double w = s->cp_gc[f], *sf = &s->cp_sf[3 * f];
+    const double d = chi[o] - chi[n];
+    for (int k = 0; k < 3; ++k) v[3 * o + k] = v[3 * o + k] + w * d * sf[k];
+    for (int k = 0; k < 3; ++k) v[3 * n + k] = v[3 * n + k] + (1.0 - w) * d * sf[k];
+  }
+  for (int f = 0; f < s->cp_nout; ++f) {
+    const int o = s->cp_bcell[f];
+    for (int k = 0; k < 3; ++k) v[3 * o + k] = v[3 * o + k] + chi[o] * s->cp_bsf[3 * f + k];
+  }
+  for (int c = 0; c < s->nc; ++c) {
+    const double iv = 1.0 / s->m.cell_volume[c];
+    for (int k = 0; k < 3; ++k) v[3 * c + k] = v[3 * c + k] * iv;
+  }
+}
+
+/* cproj_precond of project_cells_exact (mod_fv_solver.f90:3276-3281): z = 0, then the pressure
+ * solve of r at rtol 0.05, 50 iterations, atol off (pressure_solver_solve's 1e-50 default). */
+static void cp_precond(om_solver *s, int kind, const double *r, double *z) {
+  const int nc = s->nc;
+  if (kind == 0) {
+    memcpy(z, r, sizeof(double) * nc);
+    return;
+  }
+  memset(z, 0, sizeof(double) * nc);
+  int its = 0;
+  double res = 0.0;
+  om_cg_jacobi(nc, s->npf, s->ps_owner, s->ps_nb, s->ps_coeff, s->ps_diag, r, z, 0.05, 1.0e-50, 50, s->n_dir == 0,
+               &its, &res);
+}
+
+/* cp_project (:192-303): flexible PCG on (B V^-1 B^T) chi = B u + bconst; stops when max|r|
+ * falls to 1e-12 max|b| or, from the second iteration, when the last step's energy alpha^2 p.q is
+ * below tol^2 of the accumulated energy; then u -= V^-1 B^T chi. */
+int om_proj_project(om_solver *s, double *u, const double *bconst, int precond, double tol, int max_it) {
+  const int nc = s->nc;
+  double *b = dz(nc), *x = dz(nc), *r = dz(nc), *z = dz(nc), *p = dz(nc), *q = dz(nc), *rold = dz(nc), *chi = dz(nc);
+  double *v = dz(3 * (size_t)nc);
+  om_proj_divergence(s, u, b);
+  double smax = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    b[c] = b[c] + bconst[c];
+    r[c] = b[c];
+    smax = fmax(smax, fabs(r[c]));
+  }
+  int n_it = 0;
+  const double bn = smax;
+  if (bn > DBL_MIN) {
+    cp_precond(s, precond, r, z);
+    double s1 = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      p[c] = z[c];
+      s1 = s1 + r[c] * z[c];
+    }
+    double rz = s1, en_tot = 0.0;
+    for (int it = 1; it <= max_it; ++it) {
+      n_it = it;
+      memcpy(chi, p, sizeof(double) * nc);
+      om_proj_adjoint(s, chi, v);
+      om_proj_divergence(s, v, q);
+      s1 = 0.0;
+      for (int c = 0; c < nc; ++c) s1 = s1 + p[c] * q[c];
+      const double pq = s1;
+      if (fabs(pq) <= DBL_MIN) break;
+      const double alpha = rz / pq;
+      const double en_new = alpha * alpha * pq;
+      en_tot = en_tot + en_new;
+      smax = 0.0;
+      for (int c = 0; c < nc; ++c) {
+        rold[c] = r[c];
+        x[c] = x[c] + alpha * p[c];
+        r[c] = r[c] - alpha * q[c];
+        smax = fmax(smax, fabs(r[c]));
+      }
+      const double rn = smax;
+      if (rn <= 1.0e-12 * bn || (it >= 2 && en_new <= tol * tol * en_tot)) break;
+      cp_precond(s, precond, r, z);
+      double t1 = 0.0, t2 = 0.0;
+      for (int c = 0; c < nc; ++c) {
+        t1 = t1 + (r[c] - rold[c]) * z[c];
+        t2 = t2 + r[c] * z[c];
+      }
+      const double beta = fmax(0.0, t1 / rz);
+      rz = t2;
+      for (int c = 0; c < nc; ++c) p[c] = z[c] + beta * p[c];
+    }
+    memcpy(chi, x, sizeof(double) * nc);
+    om_proj_adjoint(s, chi, v);
+    for (int c = 0; c < nc; ++c)
+      for (int k = 0; k < 3; ++k) u[3 * c + k] = u[3 * c + k] - v[3 * c + k];
+  }
+  free(b); free(x); free(r); free(z); free(p); free(q); free(rold); free(chi); free(v);
+  return n_it;
+}
+
+/* project_cells_exact (mod_fv_solver.f90:3224-3283): fixed boundary outflow of inlet-type faces into
+ * bconst, the outlet pressure as the boundary-face force -dt p_b Sf / V on the outlet cell, then
+ * the projection with the pressure solve as preconditioner (100 iterations at most). */
+static void project_cells_exact(om_solver *s, double dt) {
+  const int nc = s->nc;
+  double *bconst = dz(nc);
+  for (int ib = 0; ib < s->n_nper_bf; ++ib) {
+    const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
+    const int bc = bc_of_face(s, f);
+    const double *sf = &s->m.face_sf[3 * f];
+    if (bc == BC_WALL_NOSLIP || bc == BC_WALL_MODEL || bc == BC_WALL_VELOCITY || bc == BC_SLIP || bc == BC_SYMMETRY)
+      continue;
+    if (bc == BC_OUTLET) {
+      const double fl = dt * s->outlet_pb[b] / s->m.cell_volume[o];
+      for (int k = 0; k < 3; ++k) s->U[3 * o + k] = s->U[3 * o + k] - fl * sf[k];
+    } else {
+      const double *ub = &s->U_bnd[3 * b];
+      bconst[o] = bconst[o] + (ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2]);
+    }
+  }
+  const int its = om_proj_project(s, s->U, bconst, 1, s->p.cell_projection_tol, 100);
+  s->n_proj++;
+  s->n_proj_its += its;
+  free(bconst);
+}
+
 /* ---------------------------------------------------------------- time step */
 
 /* solver_cfl_rates + solver_compute_dt (src/solver/mod_fv_solver.f90:3718-3899) */
@@ -1909,7 +2104,10 @@ int om_project_once(om_solver *s, double dt) {
   memcpy(s->U_face, s->face_flux, sizeof(double) * s->nf);
   s->u_face_ready = 1;
   apply_pressure_bc(s, s->P, s->p_bnd, 0, NULL, 1.0, s->outlet_pb);
-  correct_velocity(s, dt, 1, 1);
+  if (s->p.cell_projection == OM_PROJ_EXACT)
+    project_cells_exact(s, dt);
+  else
+    correct_velocity(s, dt, 1, 1);
   apply_velocity_bc(s, 0, NULL);
   if (s->p.rotational) {
     for (int c = 0; c < nc; ++c) s->P[c] = s->P[c] + (s->p.nu + s->nu_sgs[c]) * s->rhs_p[c] * dt * s->inv_vol[c];
@@ -1974,7 +2172,10 @@ static int advance_rk3w(om_solver *s, double dt) {
     }
     velocity_bc_both(s);
     memcpy(s->p_old, s->P, sizeof(double) * nc);
-    correct_velocity(s, gdt, 0, 1);
+    if (s->p.cell_projection == OM_PROJ_EXACT)
+      memset(s->gp_cell, 0, sizeof(double) * 3 * (size_t)nc); /* the projection takes the whole correction */
+    else
+      correct_velocity(s, gdt, 0, 1);
     memcpy(s->gp_rc, s->gp_cell, sizeof(double) * 3 * (size_t)nc);
     velocity_bc_both(s);
     compute_all_gradients(s);
@@ -2019,7 +2220,10 @@ static int advance_rk3w(om_solver *s, double dt) {
     correct_face_fluxes(s, gdt, 0);
     memcpy(s->U_face, s->face_flux, sizeof(double) * s->nf);
     s->u_face_ready = 1;
-    correct_velocity(s, gdt, 1, 0);
+    if (s->p.cell_projection == OM_PROJ_EXACT)
+      project_cells_exact(s, gdt);
+    else
+      correct_velocity(s, gdt, 1, 0);
     velocity_bc_both(s);
     if (s->p.rotational) {
       for (int c = 0; c < nc; ++c) s->P[c] = s->P[c] + (s->p.nu + s->nu_sgs[c]) * s->rhs_p[c] * gdt * s->inv_vol[c];
@@ -2403,6 +2607,7 @@ om_solver *om_create(const om_mesh *m, const om_params *p) {
         return NULL;
       }
   }
+  build_cell_projection(s);
   om_apply_all_bc(s);
   return s;
 }
@@ -2429,6 +2634,7 @@ void om_free(om_solver *s) {
   free(s->built);
   free(s->act_int);
   free(s->act_bnd);
+  free(s->cp_own); free(s->cp_nbr); free(s->cp_bcell); free(s->cp_sf); free(s->cp_gc); free(s->cp_bsf);
   free(s);
 }
 
@@ -2462,6 +2668,11 @@ int32_t *om_int_array(om_solver *s, const char *name, int64_t *len) {
 }
 
 double om_time(const om_solver *s) { return s->time; }
+
+void om_proj_counters(const om_solver *s, int64_t out[2]) {
+  out[0] = s->n_proj;
+  out[1] = s->n_proj_its;
+}
 
 void om_counters(const om_solver *s, int64_t out[3]) {
   out[0] = s->n_solves;

@@ -69,7 +69,13 @@ typedef struct om_params {
    * surface temperature, initial Obukhov length (0 = neutral), domain-mean L update */
   double most_z0_m, most_z0_h, T_surface, L_obukhov;
   int compute_L;
+  /* cell_projection: OM_PROJ_LS (correct_velocity_psolver_dual) or OM_PROJ_EXACT (V-orthogonal
+   * projection by flexible PCG, src/numerics/mod_cell_projection.f90), its energy tolerance */
+  int cell_projection;
+  double cell_projection_tol;
 } om_params;
+
+enum { OM_PROJ_LS = 0, OM_PROJ_EXACT = 1 };
 
 typedef struct om_solver om_solver;
 
@@ -105,6 +111,16 @@ double om_amd_point(const double alpha[9], const double G[9], double c_amd);
  * starting Obukhov length L_init. out = {u_star, T_star, L}. */
 void om_most_iterate(double u_par, double T_o, double T_surface, double y_w, double z0_m, double z0_h,
                      double L_init, double out[3]);
+/* Exact cell projection operators on the solver's pressure faces (cp_divergence,
+ * cp_adjoint_gradient, cp_project of src/numerics/mod_cell_projection.f90:89-303): out = B u
+ * (nc), v = V^-1 B^T chi (3*nc), and u <- u - V^-1 B^T chi with (B V^-1 B^T) chi = B u + bconst by
+ * flexible PCG, preconditioned by the identity (precond 0) or by the pressure solve at rtol 0.05,
+ * 50 iterations (precond 1, project_cells_exact). Returns the iteration count. */
+void om_proj_divergence(om_solver *s, const double *u, double *out);
+void om_proj_adjoint(om_solver *s, const double *chi, double *v);
+int om_proj_project(om_solver *s, double *u, const double *bconst, int precond, double tol, int max_it);
+/* {projections, total projection iterations} of the time steps so far. */
+void om_proj_counters(const om_solver *s, int64_t out[2]);
 /* Businger-Dyer stability corrections psi_m, psi_h (src/abl/mod_abl_profiles.f90:51-85). */
 double om_psi_m(double zeta);
 double om_psi_h(double zeta);

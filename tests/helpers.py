@@ -11,9 +11,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 COMPARED = ("time", "E", "Umax", "Ub", "tauw_mean", "nu_sgs_mean")
 
 
-def load_reference_csv(case_name):
+def load_reference_csv(case_name, np_ranks=1):
     rows = {}
-    with open(os.path.join(GOLDEN, "regression", f"{case_name}_np1.csv"), newline="") as fh:
+    with open(os.path.join(GOLDEN, "regression", f"{case_name}_np{np_ranks}.csv"), newline="") as fh:
         for r in csv.DictReader(fh):
             rows[int(r["step"])] = {k: float(v) for k, v in r.items() if k != "step"}
     return rows
@@ -48,6 +48,20 @@ def compare_stats(ref, test, rtol=1e-6, atol=1e-12, div_max=1e-9):
         if test[step]["max_abs_divU"] > div_max:
             fails.append(f"step {step} max_abs_divU={test[step]['max_abs_divU']:.3e}")
     return worst, fails
+
+
+def rank_spread(case_name):
+    """Per-column relative difference between the reference's one- and two-rank runs of a case,
+    with compare_stats' normalisation: how much the column moves when only the pressure
+    preconditioner's parallel path changes."""
+    a, b = load_reference_csv(case_name, 1), load_reference_csv(case_name, 2)
+    worst = {}
+    for step, r in a.items():
+        if step not in b:
+            continue
+        for col in COMPARED:
+            worst[col] = max(worst.get(col, 0.0), abs(r[col] - b[step][col]) / max(abs(r[col]), 1e-12 / 1e-6))
+    return worst
 
 
 def rel_l2(a, b):

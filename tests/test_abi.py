@@ -64,8 +64,11 @@ def test_missing_library_fails_loudly(monkeypatch):
 
 
 def test_params_refuse_out_of_scope_options():
+    SolverParams(cell_projection="EXACT").validate()
     with pytest.raises(ValueError):
-        SolverParams(cell_projection="EXACT").validate()
+        SolverParams(cell_projection="MIXED").validate()
+    with pytest.raises(ValueError):
+        SolverParams(cell_projection="EXACT", cell_projection_tol=0.0).validate()
     with pytest.raises(ValueError):
         SolverParams(cell_projection="LS", advection_scheme="UPWIND").validate()
     with pytest.raises(ValueError):
