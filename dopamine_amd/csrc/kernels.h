@@ -1313,6 +1313,8 @@ struct FluxRhs {
   double dt;
   int nonortho;  // deferred cross terms on (mesh not orthogonal)
   int rck;       // Rhie-Chow subtracts the k-vector (non-orthogonal mesh, rhie_chow_dual off)
+  const double* pob;  // cell_projection EXACT: outlet faces carry -dt c_b (p_b - phi_o) (:2629-2636)
+  int cproj;
   DM_HD Red<1> operator()(int c) const {
     const int nc = M.nc;
     double acc = 0.0;
@@ -1358,7 +1360,8 @@ struct FluxRhs {
         const int b = -1 - oth;
         const int bc = M.bbc[b];
         const double* ub = &Ub[3 * size_t(b)];
-        const double flux = is_solid(bc) ? 0.0 : ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2];
+        double flux = is_solid(bc) ? 0.0 : ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2];
+        if (cproj && bc == BC_OUTLET) flux = flux - dt * M.cb[b] * (pob[b] - phi[c]);
         ff[fm] = flux;
         acc = acc - flux;
       }
@@ -1386,6 +1389,197 @@ struct FluxRhs {
     Red<1> out;
     out.v[0] = fabs(r);
     return out;
+  }
+};
+
+// ------------------------------------------------------------------ exact cell projection
+// cell_projection = 'EXACT' (src/numerics/mod_cell_projection.f90:89-303 on the operators of
+// build_cell_projection, src/solver/mod_fv_solver.f90:3164-3221): the projection faces are the
+// pressure-operator faces of the cell's list (interior faces and canonical periodic edges, with the
+// face gc or the periodic gc), then its outlet faces. Every sum runs over the cell's list in that
+// order, which is the order in which the reference's sequential face loops reach the cell.
+
+// (B u)_c: outflow sum of (gc u_own + (1-gc) u_nbr).Sf, plus u_c.Sf on outlet faces (cp_divergence).
+DM_HD inline double proj_div(const DevMesh& M, const double* u, int c) {
+  const int nc = M.nc, cnt = M.nB[c];
+  double out = 0.0;
+  for (int k = 0; k < cnt; ++k) {
+    const size_t s = size_t(k) * nc + c;
+    const int idx = M.bidx[s];
+    if (idx < 0) continue;
+    const int e = M.bface[s], fm = e >> 1, side = e & 1, oth = M.both[s];
+    const size_t o = size_t(side ? oth : c), n = size_t(side ? c : oth);
+    const double w = M.xgc[idx];
+    const double* sf = &M.sf[3 * size_t(fm)];
+    const double fl = (w * u[3 * o] + (1.0 - w) * u[3 * n]) * sf[0] + (w * u[3 * o + 1] + (1.0 - w) * u[3 * n + 1]) * sf[1] +
+                      (w * u[3 * o + 2] + (1.0 - w) * u[3 * n + 2]) * sf[2];
+    out = side ? out - fl : out + fl;
+  }
+  for (int k = 0; k < cnt; ++k) {
+    const size_t s = size_t(k) * nc + c;
+    if (M.bidx[s] >= 0) continue;
+    const int b = -1 - M.both[s];
+    if (M.bbc[b] != BC_OUTLET) continue;
+    const double* sf = &M.sf[3 * size_t(M.nint + b)];
+    const double* uc = &u[3 * size_t(c)];
+    out = out + (uc[0] * sf[0] + uc[1] * sf[1] + uc[2] * sf[2]);
+  }
+  return out;
+}
+
+// V_c (V^-1 B^T chi)_c before the 1/V scaling (cp_adjoint_gradient): own side gc d Sf, neighbour
+// side (1-gc) d Sf with d = chi_own - chi_nbr, chi_c Sf on outlet faces.
+DM_HD inline void proj_adj(const DevMesh& M, const double* chi, int c, double a[3]) {
+  const int nc = M.nc, cnt = M.nB[c];
+  a[0] = a[1] = a[2] = 0.0;
+  for (int k = 0; k < cnt; ++k) {
+    const size_t s = size_t(k) * nc + c;
+    const int idx = M.bidx[s];
+    if (idx < 0) continue;
+    const int e = M.bface[s], fm = e >> 1, side = e & 1, oth = M.both[s];
+    const int o = side ? oth : c, n = side ? c : oth;
+    const double w = side ? 1.0 - M.xgc[idx] : M.xgc[idx];
+    const double d = chi[o] - chi[n];
+    const double* sf = &M.sf[3 * size_t(fm)];
+    for (int i = 0; i < 3; ++i) a[i] = a[i] + w * d * sf[i];
+  }
+  for (int k = 0; k < cnt; ++k) {
+    const size_t s = size_t(k) * nc + c;
+    if (M.bidx[s] >= 0) continue;
+    const int b = -1 - M.both[s];
+    if (M.bbc[b] != BC_OUTLET) continue;
+    const double* sf = &M.sf[3 * size_t(M.nint + b)];
+    for (int i = 0; i < 3; ++i) a[i] = a[i] + chi[c] * sf[i];
+  }
+}
+
+// project_cells_exact's boundary pass (src/solver/mod_fv_solver.f90:3240-3270) for one cell: the
+// fixed outflow of inlet-type faces into bconst, the outlet pressure as the force -dt p_b Sf / V.
+struct ProjBoundary {
+  DevMesh M;
+  const double *Ub, *pob;
+  double *U, *bconst;
+  double dt;
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc, cnt = M.nB[c];
+    double bcn = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      if (M.bidx[s] >= 0) continue;
+      const int b = -1 - M.both[s], bc = M.bbc[b];
+      if (is_solid(bc)) continue;
+      const double* sf = &M.sf[3 * size_t(M.nint + b)];
+      if (bc == BC_OUTLET) {
+        const double fl = dt * pob[b] / M.vol[c];
+        for (int i = 0; i < 3; ++i) U[3 * size_t(c) + i] = U[3 * size_t(c) + i] - fl * sf[i];
+      } else {
+        const double* ub = &Ub[3 * size_t(b)];
+        bcn = bcn + (ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2]);
+      }
+    }
+    bconst[c] = bcn;
+  }
+};
+
+// b = B u + bconst, r = b; returns |r| for max|r|.
+struct ProjInit {
+  DevMesh M;
+  const double *U, *bconst;
+  double *b, *r;
+  DM_HD Red<1> operator()(int c) const {
+    const double v = proj_div(M, U, c) + bconst[c];
+    b[c] = v;
+    r[c] = v;
+    Red<1> o;
+    o.v[0] = fabs(v);
+    return o;
+  }
+};
+
+// p = z; returns r z.
+struct ProjFirstDir {
+  const double *r, *z;
+  double* p;
+  DM_HD Red<1> operator()(int c) const {
+    p[c] = z[c];
+    Red<1> o;
+    o.v[0] = r[c] * z[c];
+    return o;
+  }
+};
+
+// v = V^-1 B^T p (chi = p).
+struct ProjAdjoint {
+  DevMesh M;
+  const double* chi;
+  double* v;
+  DM_HD void operator()(int c) const {
+    double a[3];
+    proj_adj(M, chi, c, a);
+    const double iv = 1.0 / M.vol[c];
+    for (int i = 0; i < 3; ++i) v[3 * size_t(c) + i] = a[i] * iv;
+  }
+};
+
+// q = B v; returns p q.
+struct ProjApply {
+  DevMesh M;
+  const double *v, *p;
+  double* q;
+  DM_HD Red<1> operator()(int c) const {
+    const double qc = proj_div(M, v, c);
+    q[c] = qc;
+    Red<1> o;
+    o.v[0] = p[c] * qc;
+    return o;
+  }
+};
+
+// r_old = r; x += alpha p; r -= alpha q; returns |r| for max|r|.
+struct ProjUpdate {
+  const double *p, *q;
+  double *x, *r, *rold;
+  double alpha;
+  DM_HD Red<1> operator()(int c) const {
+    rold[c] = r[c];
+    x[c] = x[c] + alpha * p[c];
+    const double rc = r[c] - alpha * q[c];
+    r[c] = rc;
+    Red<1> o;
+    o.v[0] = fabs(rc);
+    return o;
+  }
+};
+
+// returns ((r - r_old) z, r z) for the flexible beta.
+struct ProjBeta {
+  const double *r, *rold, *z;
+  DM_HD Red<2> operator()(int c) const {
+    Red<2> o;
+    o.v[0] = (r[c] - rold[c]) * z[c];
+    o.v[1] = r[c] * z[c];
+    return o;
+  }
+};
+
+// p = z + beta p.
+struct ProjDir {
+  const double* z;
+  double* p;
+  double beta;
+  DM_HD void operator()(int c) const { p[c] = z[c] + beta * p[c]; }
+};
+
+// u -= V^-1 B^T x.
+struct ProjCorrect {
+  DevMesh M;
+  const double* chi;
+  double* U;
+  DM_HD void operator()(int c) const {
+    double a[3];
+    proj_adj(M, chi, c, a);
+    const double iv = 1.0 / M.vol[c];
+    for (int i = 0; i < 3; ++i) U[3 * size_t(c) + i] = U[3 * size_t(c) + i] - a[i] * iv;
   }
 };
 

@@ -76,6 +76,10 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (p->boussinesq && (p->t_scheme < 1 || p->t_scheme > 5 || p->t_scheme == 4))
       throw std::runtime_error("temperature scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3) or QUICK (5)");
     if (p->precond != 0 && p->precond != 1) throw std::runtime_error("precond must be 0 (Jacobi) or 1 (learned)");
+    if (p->cell_projection != 0 && p->cell_projection != 1)
+      throw std::runtime_error("cell_projection must be 0 (LS) or 1 (EXACT)");
+    if (p->cell_projection == 1 && !(p->cell_projection_tol > 0.0))
+      throw std::runtime_error("cell_projection_tol must be positive");
     if (p->precond == 1 && (p->learn_collect < 1 || p->learn_capacity < 1 || p->learn_iters < 0 ||
                             p->learn_batch < 1 || !(p->learn_lr > 0.0)))
       throw std::runtime_error("learned preconditioner: learn_collect, learn_capacity, learn_batch >= 1, "
@@ -235,6 +239,15 @@ int dopamine_solver_learned_info(void* h, double out[8]) {
                          double(lp->learned_solves), double(lp->learned_its), double(lp->fallbacks),
                          lp->ref_its,                lp->train_ms};
     for (int i = 0; i < 8; ++i) out[i] = v[i];
+    return 0;
+  });
+}
+
+int dopamine_solver_projection_info(void* h, int64_t out[2]) {
+  return guard([&] {
+    auto* s = static_cast<Solver*>(h);
+    out[0] = s->n_proj;
+    out[1] = s->n_proj_its;
     return 0;
   });
 }

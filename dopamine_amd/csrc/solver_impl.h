@@ -17,6 +17,7 @@
 #include <cstdlib>
 
 #include <algorithm>
+#include <cfloat>
 #include <string>
 #include <memory>
 #include <vector>
@@ -31,12 +32,21 @@ struct SolverScalars {
   double divmax, ub1, ub2, atol, pmean, shift, cfl, dif;
   double st[12];
   double vmin, vtot, gdt, pad;
+  double pj[4];  // exact cell projection: reductions of the current iteration
 };
 
 namespace sk {
 struct DivFin {
   SolverScalars* s;
   DM_HD void operator()(const Red<1>& t) const { s->divmax = t.v[0]; }
+};
+template <int K>
+struct ProjFin {  // exact cell projection: store K reduction results from pj[at]
+  SolverScalars* s;
+  int at;
+  DM_HD void operator()(const Red<K>& t) const {
+    for (int i = 0; i < K; ++i) s->pj[at + i] = t.v[i];
+  }
 };
 struct AtolFin {
   SolverScalars* s;
@@ -127,6 +137,11 @@ class SolverT {
   double time = 0.0;
   int u_face_ready = 0;
   int64_t n_solves = 0, n_iters = 0, last_iters = 0, n_sweeps = 0;
+  // exact cell projection (cell_projection = 1): work vectors, projections and iterations so far
+  int cproj = 0;
+  double *pj_b = nullptr, *pj_x = nullptr, *pj_r = nullptr, *pj_z = nullptr, *pj_p = nullptr, *pj_q = nullptr,
+         *pj_ro = nullptr, *pj_bc = nullptr, *pj_v = nullptr;
+  int64_t n_proj = 0, n_proj_its = 0;
 
   // fields (execution space)
   double *U = nullptr, *U2 = nullptr, *P = nullptr, *p_old = nullptr, *phi_prev = nullptr, *p_bnd = nullptr;
@@ -631,6 +646,12 @@ class SolverT {
       x.upload(T_bnd, tref_b.data(), tref_b.size());
       patchT = up(pt);
     }
+    cproj = p.cell_projection == 1;
+    if (cproj) {
+      pj_b = fresh<double>(nc); pj_x = fresh<double>(nc); pj_r = fresh<double>(nc); pj_z = fresh<double>(nc);
+      pj_p = fresh<double>(nc); pj_q = fresh<double>(nc); pj_ro = fresh<double>(nc); pj_bc = fresh<double>(nc);
+      pj_v = fresh<double>(3 * size_t(nc));
+    }
     cg.init(&x, nc);
     cg.set_single_reduction(p.cg_variant == 1);
     if (p.precision == 1) cg.set_mixed(A32, mixed_tau());
@@ -836,7 +857,8 @@ class SolverT {
         scalar_gradient(P, nullptr, p_bnd, nullptr);
     }
     const int rck = (!skip_nonortho && !prm.rhie_chow_dual) ? 1 : 0;
-    x.template reduce<1, true>(nc, FluxRhs{M, U, U_bnd, grad, gp_rc, phi_prev, gp, ff, rhs, dt, skip_nonortho ? 0 : 1, rck},
+    x.template reduce<1, true>(nc, FluxRhs{M, U, U_bnd, grad, gp_rc, phi_prev, gp, ff, rhs, dt, skip_nonortho ? 0 : 1, rck,
+                                           outlet_pb, cproj},
                                sk::DivFin{sc});
     x.for_each(nb, PeriodicAntisym{M, ff});
   }
@@ -874,6 +896,58 @@ class SolverT {
     return rc;
   }
 
+  // project_cells_exact (src/solver/mod_fv_solver.f90:3224-3283) with cp_project
+  // (src/numerics/mod_cell_projection.f90:192-303): flexible PCG on (B V^-1 B^T) chi = B u + bconst,
+  // preconditioned by the pressure solve (z = 0, rtol 0.05, 50 iterations, no absolute floor),
+  // at most 100 iterations; then u -= V^-1 B^T chi.
+  double pj_get(int i) {
+    SolverScalars h;
+    x.download(&h, sc, 1);
+    return h.pj[i];
+  }
+  void pj_precond() {
+    x.zero(pj_z, nc);
+    int its = 0;
+    double res = 0.0;
+    cg.solve(A, pj_r, pj_z, 0.05, 1.0e-50, nullptr, 50, n_dir == 0, &its, &res);
+  }
+  void project_exact(double dt) {
+    x.for_each(nc, ProjBoundary{M, U_bnd, outlet_pb, U, pj_bc, dt});
+    x.template reduce<1, true>(nc, ProjInit{M, U, pj_bc, pj_b, pj_r}, sk::ProjFin<1>{sc, 0});
+    const double bn = pj_get(0);
+    int n_it = 0;
+    if (bn > DBL_MIN) {
+      x.zero(pj_x, nc);
+      pj_precond();
+      x.template reduce<1, false>(nc, ProjFirstDir{pj_r, pj_z, pj_p}, sk::ProjFin<1>{sc, 0});
+      double rz = pj_get(0), en_tot = 0.0;
+      const double tol = prm.cell_projection_tol;
+      for (int it = 1; it <= 100; ++it) {
+        n_it = it;
+        x.for_each(nc, ProjAdjoint{M, pj_p, pj_v});
+        x.template reduce<1, false>(nc, ProjApply{M, pj_v, pj_p, pj_q}, sk::ProjFin<1>{sc, 0});
+        const double pq = pj_get(0);
+        if (std::fabs(pq) <= DBL_MIN) break;
+        const double alpha = rz / pq;
+        const double en_new = alpha * alpha * pq;
+        en_tot = en_tot + en_new;
+        x.template reduce<1, true>(nc, ProjUpdate{pj_p, pj_q, pj_x, pj_r, pj_ro, alpha}, sk::ProjFin<1>{sc, 0});
+        const double rn = pj_get(0);
+        if (rn <= 1.0e-12 * bn || (it >= 2 && en_new <= tol * tol * en_tot)) break;
+        pj_precond();
+        x.template reduce<2, false>(nc, ProjBeta{pj_r, pj_ro, pj_z}, sk::ProjFin<2>{sc, 0});
+        SolverScalars h;
+        x.download(&h, sc, 1);
+        const double beta = std::fmax(0.0, h.pj[0] / rz);
+        rz = h.pj[1];
+        x.for_each(nc, ProjDir{pj_z, pj_p, beta});
+      }
+      x.for_each(nc, ProjCorrect{M, pj_x, U});
+    }
+    n_proj++;
+    n_proj_its += n_it;
+  }
+
   void anchor_pressure() {
     // mean removal is folded into RotAccumulate's reduction; this variant serves project_once
     x.template reduce<1, false>(nc, RotAccumulate{M, P, zeros(), nu_sgs, rhs, 0.0, 0.0, 0}, sk::AnchorFin{sc, n_outlet == 0});
@@ -909,7 +983,10 @@ class SolverT {
     std::swap(UF, ff);
     u_face_ready = 1;
     bc_pressure(0, nullptr, 1.0, outlet_pb);
-    correct_velocity(dt, true, true, nullptr);
+    if (cproj)
+      project_exact(dt);
+    else
+      correct_velocity(dt, true, true, nullptr);
     bc_velocity(0, nullptr);
     if (prm.rotational) {
       x.template reduce<1, false>(nc, RotAccumulate{M, P, zeros(), nu_sgs, rhs, prm.nu, dt, 1}, sk::AnchorFin{sc, 0});
@@ -993,7 +1070,10 @@ class SolverT {
       if (bous) std::swap(T, T2);
       velocity_bc_both();
       x.copy(p_old, P, nc);
-      correct_velocity(gdt, false, true, gp_rc);
+      if (cproj)
+        x.zero(gp_rc, 3 * size_t(nc));  // the exact projection takes the whole correction
+      else
+        correct_velocity(gdt, false, true, gp_rc);
       velocity_bc_both();
       gradients(false, 0.0);
       x.copy(phi_prev, P, nc);
@@ -1030,7 +1110,10 @@ class SolverT {
       correct_face_fluxes(gdt, false);
       std::swap(UF, ff);
       u_face_ready = 1;
-      correct_velocity(gdt, true, false, nullptr);
+      if (cproj)
+        project_exact(gdt);
+      else
+        correct_velocity(gdt, true, false, nullptr);
       velocity_bc_both();
       // rotational correction + accumulation + mean removal (the intermediate increment BC is
       // overwritten before anything reads it)

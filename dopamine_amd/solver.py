@@ -40,7 +40,7 @@ class Params(ctypes.Structure):
                 ("t_scheme", _i32), ("patch_T_val", _P(_d)), ("precond", _i32), ("learn_collect", _i32),
                 ("learn_capacity", _i32), ("learn_iters", _i32), ("learn_batch", _i32), ("learn_seed", _i32),
                 ("learn_lr", _d), ("most_z0_m", _d), ("most_z0_h", _d), ("T_surface", _d), ("L_obukhov", _d),
-                ("compute_L", _i32)]
+                ("compute_L", _i32), ("cell_projection", _i32), ("cell_projection_tol", _d)]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -70,7 +70,7 @@ def build_abi(mesh, params, precision=0):
                 params.t_scheme_id(), arr(tv, _d), 1 if params.precond.upper() == "LEARNED" else 0,
                 params.learn_collect, params.learn_capacity, params.learn_iters, params.learn_batch,
                 params.learn_seed, params.learn_lr, params.z0_m, params.z0_h, params.T_surface, params.L_obukhov,
-                int(params.compute_L))
+                int(params.compute_L), params.projection_id(), params.cell_projection_tol)
     return md, pr, keep
 
 
@@ -94,6 +94,7 @@ def _bind(L):
     L.dopamine_solver_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _P(_i64), _P(_d)]
     L.dopamine_solver_learned_info.argtypes = [ctypes.c_void_p, _d * 8]
+    L.dopamine_solver_projection_info.argtypes = [ctypes.c_void_p, _i64 * 2]
     L.dopamine_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
     L.dopamine_solver_write_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_read_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
@@ -236,6 +237,12 @@ class FVSolver:
         d = dict(zip(self.LEARNED_INFO, list(out)))
         d["phase"] = ("collecting", "in use", "off")[int(d["phase"])]
         return d
+
+    def proj_counters(self):
+        """Exact cell projection: projections and flexible-PCG iterations so far."""
+        out = (_i64 * 2)()
+        self._check(self.L.dopamine_solver_projection_info(self.h, out), "projection_info")
+        return {"projections": out[0], "iterations": out[1]}
 
     def learned_loss(self, max_steps=100000):
         out = np.zeros(max_steps)

@@ -81,6 +81,11 @@ typedef struct dopamine_params {
    * else T_ref; the heat flux enters the temperature equation. */
   double most_z0_m, most_z0_h, T_surface, L_obukhov;
   int32_t compute_L;
+  /* cell_projection (src/solver/mod_fv_solver.f90:91-92): 0 'LS' (correct_velocity_psolver_dual),
+   * 1 'EXACT' (V-orthogonal projection by flexible PCG with the pressure solve as preconditioner,
+   * src/numerics/mod_cell_projection.f90); cell_projection_tol its relative energy tolerance. */
+  int32_t cell_projection;
+  double cell_projection_tol;
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
@@ -132,6 +137,8 @@ int dopamine_solver_nonortho_info(void *solver, int32_t out[3]);
  * collected, residual samples kept, learned solves, their CG iterations, fallbacks to Jacobi CG,
  * mean Jacobi CG iterations per solve, training time in ms}. -1 when precond != 1. */
 int dopamine_solver_learned_info(void *solver, double out[8]);
+/* Exact cell projection counters: out = {projections, flexible-PCG iterations} so far. */
+int dopamine_solver_projection_info(void *solver, int64_t out[2]);
 /* Mean training loss of each Adam step (up to max values); returns the count. */
 int dopamine_solver_learned_loss(void *solver, double *out, int max);
 /* Wait for all work queued by the solver. */

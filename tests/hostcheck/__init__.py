@@ -34,6 +34,7 @@ def lib():
         L.hc_solver_time.argtypes = [ctypes.c_void_p]
         L.hc_solver_counters.argtypes = [ctypes.c_void_p, _i64 * 4]
         L.hc_solver_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int32 * 3]
+        L.hc_solver_proj_counters.argtypes = [ctypes.c_void_p, _i64 * 2]
         L.hc_solver_stats_csv.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _d, _P(_d), ctypes.c_int]
         L.hc_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
         L.hc_solver_write_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
@@ -124,6 +125,11 @@ class HostSolver:
         out = (ctypes.c_int32 * 3)()
         lib().hc_solver_nonortho_info(self.h, out)
         return {"skip_nonortho": out[0], "skip_grad_p": out[1], "n_nonortho_corr": out[2]}
+
+    def proj_counters(self):
+        out = (_i64 * 2)()
+        lib().hc_solver_proj_counters(self.h, out)
+        return {"projections": out[0], "iterations": out[1]}
 
     def counters(self):
         out = (_i64 * 4)()

@@ -76,6 +76,10 @@ void hc_solver_counters(void* h, int64_t out[4]) {
   auto* s = static_cast<Solver*>(h);
   out[0] = s->n_solves; out[1] = s->n_iters; out[2] = s->last_iters; out[3] = s->n_sweeps;
 }
+void hc_solver_proj_counters(void* h, int64_t out[2]) {
+  out[0] = static_cast<Solver*>(h)->n_proj;
+  out[1] = static_cast<Solver*>(h)->n_proj_its;
+}
 void hc_solver_nonortho_info(void* h, int32_t out[3]) {
   auto* s = static_cast<Solver*>(h);
   out[0] = s->skip_nonortho; out[1] = s->skip_grad_p; out[2] = s->n_corr;

@@ -144,3 +144,33 @@ def test_host_most_channel_matches_oracle(dT):
         assert rel_l2(a, h.get(name)[: a.size]) < 1e-10, name
     qw = o.array("q_w_bnd")
     assert np.count_nonzero(qw) > 0 and np.all(np.sign(qw[qw != 0]) == -np.sign(dT))
+
+
+def _duct_tets_exact():
+    """Jittered shuffled tets with a velocity inlet (fixed outflow bconst), an outlet (pressure
+    force and compact outlet flux term) and walls, under the exact cell projection."""
+    from dopamine_amd import mesh as M
+    from dopamine_amd.params import SolverParams
+    mesh = M.tet_box(4, lx=2.0, jitter=0.15, shuffle_seed=2,
+                     group_types=("inlet", "outlet", "noSlip", "noSlip", "slip", "slip"))
+    prm = SolverParams(nu=1e-2, advection_scheme="CDS", use_sgs=False, dt=5e-3, cfl_max=0.5, p_tol=1e-12,
+                       p_max_iter=1000, rhie_chow_dual=False, cell_projection="EXACT", pressure_min_cos_theta=0.0,
+                       patch_U={"x0": (1.0, 0.0, 0.0)}, patch_p={"xL": 0.0})
+    return cases.Case("duct_tets_exact", lambda: mesh, prm, ic="uniform", ic_uref=1.0), mesh
+
+
+@pytest.mark.parametrize("name", ["tgv_skew_exact", "hydro_exact", "duct_tets_exact"])
+def test_host_exact_projection_matches_oracle(name):
+    """cell_projection = 'EXACT' (project_exact, ProjBoundary ... ProjCorrect) against the oracle:
+    fields within 1e-10 and the same number of projections and flexible-PCG iterations."""
+    if name == "duct_tets_exact":
+        case, mesh = _duct_tets_exact()
+    else:
+        case = fresh(name)
+        mesh = case.make_mesh()
+    case.params.p_tol = 1e-12
+    h, o = _pair(case, mesh, 4)
+    assert rel_l2(o.U, h.U) < 1e-10
+    assert rel_l2(o.p, h.p) < 1e-10
+    assert h.proj_counters() == o.proj_counters()
+    assert h.proj_counters()["iterations"] > h.proj_counters()["projections"]
