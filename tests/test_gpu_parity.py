@@ -251,8 +251,13 @@ def test_gpu_polymesh_input_matches_oracle(name, tmp_path):
 
 @pytest.mark.parametrize("name", ["tgv_skew_exact", "hydro_exact"])
 def test_gpu_exact_projection_matches_oracle(name):
-    """cell_projection = 'EXACT' on the GPU against the oracle: fields within 1e-10, the same
-    projections and flexible-PCG iterations."""
+    """cell_projection = 'EXACT' on the GPU against the oracle: the same projections and
+    flexible-PCG iterations, fields within 1e-10 on tgv_skew_exact. The hydrostatic column is
+    sensitive under EXACT: each projection iteration runs a 50-iteration Jacobi CG that stops short
+    of its 0.05 target, and in the oracle a relative 1e-15 change of the initial field already moves
+    U by 1.6e-12 after four steps (the LS run of the same case: 2e-15). The GPU's reductions sum in
+    another order than the oracle's sequential loops, so that case is held to 1e-8; the host-check
+    build of the same kernels, which sums sequentially, matches the oracle to the last bit."""
     case = fresh(name)
     case.params.p_tol = 1e-12
     mesh = case.make_mesh()
@@ -261,7 +266,8 @@ def test_gpu_exact_projection_matches_oracle(name):
     du, dp = rel_l2(o.U, g.U), rel_l2(o.p, g.p)
     print(f"[gpu] {name}: rel-L2 U {du:.2e} p {dp:.2e}, projections {g.proj_counters()} oracle {o.proj_counters()}",
           flush=True)
-    assert du < 1e-10 and dp < 1e-10
+    bar = 1e-8 if name == "hydro_exact" else 1e-10
+    assert du < bar and dp < bar
     assert g.proj_counters() == o.proj_counters()
 
 
