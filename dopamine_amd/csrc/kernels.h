@@ -1134,6 +1134,8 @@ struct ExplicitRhs {
   const double* phiF;  // reconstruction schemes: face values from ReconFace
   const double* T;     // Boussinesq buoyancy -bb (T - T_ref) g (bb = 0: off)
   double bb, Tref, g0, g1, g2;
+  int rot;             // Coriolis -2 om x U (src/solver/mod_fv_solver.f90:1694-1698)
+  double om0, om1, om2;
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -1276,6 +1278,11 @@ struct ExplicitRhs {
     if (SCHEME == 4)
       for (int q = 0; q < 3; ++q) cv[q] = cv[q] + 0.5 * Uc[q] * dw * iv;
     double h[3] = {cv[0] + vs[0] + f0, cv[1] + vs[1] + f1, cv[2] + vs[2] + f2};
+    if (rot) {
+      h[0] = h[0] - 2.0 * (om1 * Uc[2] - om2 * Uc[1]);
+      h[1] = h[1] - 2.0 * (om2 * Uc[0] - om0 * Uc[2]);
+      h[2] = h[2] - 2.0 * (om0 * Uc[1] - om1 * Uc[0]);
+    }
     if (T) {
       const double dT = T[c] - Tref;
       h[0] = h[0] - bb * dT * g0;
@@ -1687,6 +1694,18 @@ struct CorrectVelocity {
 // ------------------------------------------------------------------ reductions
 
 // Volume-weighted mean speed for the divergence tolerance (src/solver/mod_fv_solver.f90:1424-1437).
+// Momentum sum U V for the constant mass flux forcing (update_mass_flux_forcing,
+// src/solver/mod_fv_solver.f90:960-983).
+struct Momentum {
+  DevMesh M;
+  const double* U;
+  DM_HD Red<3> operator()(int c) const {
+    Red<3> r;
+    for (int k = 0; k < 3; ++k) r.v[k] = U[3 * size_t(c) + k] * M.vol[c];
+    return r;
+  }
+};
+
 struct BulkSpeed {
   DevMesh M;
   const double* U;

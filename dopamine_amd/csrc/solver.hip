@@ -76,6 +76,8 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (p->boussinesq && (p->t_scheme < 1 || p->t_scheme > 5 || p->t_scheme == 4))
       throw std::runtime_error("temperature scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3) or QUICK (5)");
     if (p->precond != 0 && p->precond != 1) throw std::runtime_error("precond must be 0 (Jacobi) or 1 (learned)");
+    if (p->mass_flux && (p->f_const[0] != 0.0 || p->f_const[1] != 0.0 || p->f_const[2] != 0.0))
+      throw std::runtime_error("f_const must be 0 with constant mass flux forcing");
     if (p->cell_projection != 0 && p->cell_projection != 1)
       throw std::runtime_error("cell_projection must be 0 (LS) or 1 (EXACT)");
     if (p->cell_projection == 1 && !(p->cell_projection_tol > 0.0))

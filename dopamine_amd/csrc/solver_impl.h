@@ -781,9 +781,27 @@ class SolverT {
   // compute_all_gradients (velocity), with the SGS viscosity of the stage fused in when sgs_dt > 0
   template <int SCHEME>
   void explicit_rhs(double dt, int k) {
-    x.for_each(nc, ExplicitRhs<SCHEME>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, prm.f_const[0],
-                                       prm.f_const[1], prm.f_const[2], dt, RK3_A[k], RK3_B[k], n_wm > 0, phiF,
-                                       bous ? T : nullptr, prm.beta, prm.T_ref, prm.g[0], prm.g[1], prm.g[2]});
+    double fb[3];
+    body_force_now(fb);
+    x.for_each(nc, ExplicitRhs<SCHEME>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, fb[0], fb[1], fb[2], dt,
+                                       RK3_A[k], RK3_B[k], n_wm > 0, phiF, bous ? T : nullptr, prm.beta, prm.T_ref,
+                                       prm.g[0], prm.g[1], prm.g[2], prm.use_rotation ? 1 : 0, prm.omega_rot[0],
+                                       prm.omega_rot[1], prm.omega_rot[2]});
+  }
+  // body_force_now (src/solver/mod_fv_solver.f90:2904-2923): constant or mass-flux force plus the
+  // harmonic forcing at the current (stage) time
+  double mff[3] = {0.0, 0.0, 0.0};
+  void body_force_now(double f[3]) const {
+    for (int k = 0; k < 3; ++k) f[k] = prm.mass_flux ? mff[k] : prm.f_const[k];
+    if (prm.use_harmonic) {
+      const double* d = prm.forcing_dir;
+      const double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      double hd[3] = {1.0, 0.0, 0.0};
+      if (dn > SMALL)
+        for (int k = 0; k < 3; ++k) hd[k] = d[k] / dn;
+      const double hs = prm.forcing_amp * std::cos(prm.forcing_omega * time + prm.forcing_phase);
+      for (int k = 0; k < 3; ++k) f[k] = f[k] + hs * hd[k];
+    }
   }
   // temperature face values (reconstruction schemes), then H_T and the T predictor into T2
   void explicit_rhs_T(double dt, int k) {
@@ -870,7 +888,9 @@ class SolverT {
 
   void correct_velocity(double dt, bool with_cross, bool ptot, double* gpc_out) {
     const int do_cross = (with_cross && !skip_nonortho) ? 1 : 0;
-    x.for_each(nc, CorrectVelocity{M, P, gp, outlet_pb, U, gpc_out, dt, prm.f_const[0], prm.f_const[1], prm.f_const[2],
+    double fb[3];
+    body_force_now(fb);
+    x.for_each(nc, CorrectVelocity{M, P, gp, outlet_pb, U, gpc_out, dt, fb[0], fb[1], fb[2],
                                    do_cross, ptot ? 1 : 0, bous ? T : nullptr, prm.beta, prm.T_ref, prm.g[0],
                                    prm.g[1], prm.g[2]});
   }
@@ -1023,6 +1043,12 @@ class SolverT {
     const double safe = compute_dt();
     if (safe < 0.0) return -3;
     if (dt > safe) dt = safe;
+    if (prm.mass_flux) {
+      x.template reduce<3, false>(nc, Momentum{M, U}, sk::ProjFin<3>{sc, 0});
+      SolverScalars h;
+      x.download(&h, sc, 1);
+      for (int k = 0; k < 3; ++k) mff[k] = (prm.U_bulk_target[k] - h.pj[k] / std::fmax(h.vtot, SMALL)) / dt;
+    }
     const int rc = advance_rk3w(dt);
     time = time + dt;
     if (dt_used) *dt_used = dt;

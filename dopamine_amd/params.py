@@ -75,6 +75,18 @@ class SolverParams:
     learn_batch: int = 4
     learn_lr: float = 3.0e-3
     learn_seed: int = 1
+    # momentum forcing (&dopamine_physics, src/io/mod_input_parser.f90:392-438): Coriolis -2 Omega x U,
+    # harmonic forcing amp cos(omega t + phase) along forcing_dir, constant mass flux (the uniform
+    # force (U_bulk_target - U_bulk) / dt of each step replaces f_const, which must then be 0)
+    use_rotation: bool = False
+    omega_rot: tuple = (0.0, 0.0, 0.0)
+    use_harmonic_forcing: bool = False
+    forcing_amp: float = 0.0
+    forcing_omega: float = 0.0
+    forcing_phase: float = 0.0
+    forcing_dir: tuple = (1.0, 0.0, 0.0)
+    constant_mass_flux_forcing: bool = False
+    U_bulk_target: tuple = (0.0, 0.0, 0.0)
     # per-patch values, by patch name: p_val (outlet), U_val (inlet / moving wall)
     patch_p: dict = field(default_factory=dict)
     patch_U: dict = field(default_factory=dict)
@@ -98,8 +110,18 @@ class SolverParams:
             raise ValueError(f"wall_model {self.wall_model!r} not supported (loglaw, most, none)")
         if self.precond.upper() not in ("JACOBI", "LEARNED"):
             raise ValueError(f"precond {self.precond!r} not supported (JACOBI, LEARNED)")
+        if self.constant_mass_flux_forcing and any(f != 0.0 for f in self.f_const):
+            raise ValueError("f_const must be 0 with constant_mass_flux_forcing")
         if self.adapt_dt:
             raise ValueError("adapt_dt is not supported; the fixed-dt CFL clamp of solver_advance applies")
+
+    def forcing_args(self):
+        """The forcing fields in the C ABI's order (after cell_projection_tol)."""
+        import ctypes
+        d3 = ctypes.c_double * 3
+        return (int(self.use_rotation), d3(*self.omega_rot), int(self.use_harmonic_forcing), self.forcing_amp,
+                self.forcing_omega, self.forcing_phase, d3(*self.forcing_dir), int(self.constant_mass_flux_forcing),
+                d3(*self.U_bulk_target))
 
     def projection_id(self):
         return 1 if self.cell_projection.upper() == "EXACT" else 0

@@ -40,7 +40,10 @@ class Params(ctypes.Structure):
                 ("t_scheme", _i32), ("patch_T_val", _P(_d)), ("precond", _i32), ("learn_collect", _i32),
                 ("learn_capacity", _i32), ("learn_iters", _i32), ("learn_batch", _i32), ("learn_seed", _i32),
                 ("learn_lr", _d), ("most_z0_m", _d), ("most_z0_h", _d), ("T_surface", _d), ("L_obukhov", _d),
-                ("compute_L", _i32), ("cell_projection", _i32), ("cell_projection_tol", _d)]
+                ("compute_L", _i32), ("cell_projection", _i32), ("cell_projection_tol", _d),
+                ("use_rotation", _i32), ("omega_rot", _d * 3), ("use_harmonic", _i32), ("forcing_amp", _d),
+                ("forcing_omega", _d), ("forcing_phase", _d), ("forcing_dir", _d * 3), ("mass_flux", _i32),
+                ("U_bulk_target", _d * 3)]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -70,7 +73,7 @@ def build_abi(mesh, params, precision=0):
                 params.t_scheme_id(), arr(tv, _d), 1 if params.precond.upper() == "LEARNED" else 0,
                 params.learn_collect, params.learn_capacity, params.learn_iters, params.learn_batch,
                 params.learn_seed, params.learn_lr, params.z0_m, params.z0_h, params.T_surface, params.L_obukhov,
-                int(params.compute_L), params.projection_id(), params.cell_projection_tol)
+                int(params.compute_L), params.projection_id(), params.cell_projection_tol, *params.forcing_args())
     return md, pr, keep
 
 

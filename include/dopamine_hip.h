@@ -86,6 +86,17 @@ typedef struct dopamine_params {
    * src/numerics/mod_cell_projection.f90); cell_projection_tol its relative energy tolerance. */
   int32_t cell_projection;
   double cell_projection_tol;
+  /* momentum forcing (compute_explicit_rhs and body_force_now, src/solver/mod_fv_solver.f90:
+   * 1661-1700, 2904-2923): Coriolis -2 omega_rot x U when use_rotation; harmonic forcing
+   * forcing_amp cos(forcing_omega t + forcing_phase) along forcing_dir when use_harmonic; with
+   * mass_flux the uniform force (U_bulk_target - U_bulk) / dt, set at each step start
+   * (update_mass_flux_forcing, :960-983), replaces f_const (which must then be 0). */
+  int32_t use_rotation;
+  double omega_rot[3];
+  int32_t use_harmonic;
+  double forcing_amp, forcing_omega, forcing_phase, forcing_dir[3];
+  int32_t mass_flux;
+  double U_bulk_target[3];
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */

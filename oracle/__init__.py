@@ -38,7 +38,10 @@ class _Params(ctypes.Structure):
                 ("amd_wall_gradient", ctypes.c_int), ("boussinesq", ctypes.c_int), ("alpha_T", _d), ("beta", _d),
                 ("T_ref", _d), ("g", _d * 3), ("Pr_t", _d), ("t_scheme", ctypes.c_int), ("patch_T_val", _P(_d)),
                 ("most_z0_m", _d), ("most_z0_h", _d), ("T_surface", _d), ("L_obukhov", _d), ("compute_L", ctypes.c_int),
-                ("cell_projection", ctypes.c_int), ("cell_projection_tol", _d)]
+                ("cell_projection", ctypes.c_int), ("cell_projection_tol", _d),
+                ("use_rotation", ctypes.c_int), ("omega_rot", _d * 3), ("use_harmonic", ctypes.c_int),
+                ("forcing_amp", _d), ("forcing_omega", _d), ("forcing_phase", _d), ("forcing_dir", _d * 3),
+                ("mass_flux", ctypes.c_int), ("U_bulk_target", _d * 3)]
 
 
 _lib = None
@@ -129,7 +132,7 @@ class Solver:
                            int(params.amd_wall_gradient), int(params.boussinesq), params.alpha_T, params.beta,
                            params.T_ref, (_d * 3)(*params.g), params.Pr_t, params.t_scheme_id(), arr(tv, _d),
                            params.z0_m, params.z0_h, params.T_surface, params.L_obukhov, int(params.compute_L),
-                           params.projection_id(), params.cell_projection_tol)
+                           params.projection_id(), params.cell_projection_tol, *params.forcing_args())
         self.params = params
         self.mesh_nc = mesh.n_cells
         self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))

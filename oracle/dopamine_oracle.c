@@ -68,7 +68,23 @@ struct om_solver {
   int *cp_own, *cp_nbr, *cp_bcell;
   double *cp_sf, *cp_gc, *cp_bsf;
   int64_t n_proj, n_proj_its;
+  double mff[3]; /* constant mass flux forcing of the current step */
 };
+
+/* body_force_now (src/solver/mod_fv_solver.f90:2904-2923): constant or mass-flux force plus the
+ * harmonic forcing at the solver's current (stage) time. */
+static void body_force_now(const om_solver *s, double f[3]) {
+  for (int k = 0; k < 3; ++k) f[k] = s->p.mass_flux ? s->mff[k] : s->p.f_const[k];
+  if (s->p.use_harmonic) {
+    const double *d = s->p.forcing_dir;
+    const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double hd[3] = {1.0, 0.0, 0.0};
+    if (dn > 1.0e-30)
+      for (int k = 0; k < 3; ++k) hd[k] = d[k] / dn;
+    const double hs = s->p.forcing_amp * cos(s->p.forcing_omega * s->time + s->p.forcing_phase);
+    for (int k = 0; k < 3; ++k) f[k] = f[k] + hs * hd[k];
+  }
+}
 
 static double *dz(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 static int *iz(size_t n) { return (int *)calloc(n ? n : 1, sizeof(int)); }
@@ -1489,8 +1505,17 @@ static void explicit_rhs(om_solver *s) {
       advection_scalar_recon(s, s->p.t_scheme, s->T, s->T_bnd, s->grad_T, s->conv_T);
     viscous_scalar(s, s->T, s->T_bnd, s->grad_T, s->p.alpha_T, s->visc_T);
   }
+  double fb[3];
+  body_force_now(s, fb);
+  const double *om = s->p.omega_rot;
   for (int c = 0; c < s->nc; ++c) {
-    for (int k = 0; k < 3; ++k) s->H[3 * c + k] = s->conv[3 * c + k] + s->visc[3 * c + k] + s->p.f_const[k];
+    for (int k = 0; k < 3; ++k) s->H[3 * c + k] = s->conv[3 * c + k] + s->visc[3 * c + k] + fb[k];
+    if (s->p.use_rotation) {
+      const double *u = &s->U[3 * c];
+      s->H[3 * c] = s->H[3 * c] - 2.0 * (om[1] * u[2] - om[2] * u[1]);
+      s->H[3 * c + 1] = s->H[3 * c + 1] - 2.0 * (om[2] * u[0] - om[0] * u[2]);
+      s->H[3 * c + 2] = s->H[3 * c + 2] - 2.0 * (om[0] * u[1] - om[1] * u[0]);
+    }
     if (bous)
       for (int k = 0; k < 3; ++k) s->H[3 * c + k] = s->H[3 * c + k] - s->p.beta * (s->T[c] - s->p.T_ref) * s->p.g[k];
     if (bous) s->H_T[c] = s->conv_T[c] + s->visc_T[c];
@@ -1714,7 +1739,8 @@ static void correct_velocity(om_solver *s, double dt, int with_cross, int ptot) 
     for (int i = 0; i < 3; ++i) gpc[3 * o + i] = gpc[3 * o + i] + g * n[i];
   }
   if (!do_cross) {
-    const double *fb = s->p.f_const;
+    double fb[3];
+    body_force_now(s, fb);
     const double bb = s->p.boussinesq ? s->p.beta : 0.0;
     for (int ib = 0; ib < s->n_nper_bf; ++ib) {
       const int b = s->nper_bf[ib], f = s->nint + b, o = s->m.face_owner[f];
@@ -2247,6 +2273,13 @@ int om_advance(om_solver *s, double dt_req, double *dt_used) {
   const double safe = compute_dt(s);
   if (safe < 0.0) return -3;
   if (dt > safe) dt = safe;
+  if (s->p.mass_flux) {
+    /* update_mass_flux_forcing (src/solver/mod_fv_solver.f90:960-983) */
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < s->nc; ++c)
+      for (int k = 0; k < 3; ++k) m[k] = m[k] + s->U[3 * c + k] * s->m.cell_volume[c];
+    for (int k = 0; k < 3; ++k) s->mff[k] = (s->p.U_bulk_target[k] - m[k] / fmax(s->V_total, SMALL)) / dt;
+  }
   const int rc = advance_rk3w(s, dt);
   /* apply_scalar_bc: no temperature or scalars here */
   s->time = s->time + dt;

@@ -73,6 +73,15 @@ typedef struct om_params {
    * projection by flexible PCG, src/numerics/mod_cell_projection.f90), its energy tolerance */
   int cell_projection;
   double cell_projection_tol;
+  /* momentum forcing (compute_explicit_rhs, src/solver/mod_fv_solver.f90:1661-1700): Coriolis
+   * -2 Omega x U; harmonic forcing amp cos(omega t + phase) along dir; constant mass flux: the
+   * uniform force (U_bulk_target - U_bulk) / dt set at each step start in place of f_const */
+  int use_rotation;
+  double omega_rot[3];
+  int use_harmonic;
+  double forcing_amp, forcing_omega, forcing_phase, forcing_dir[3];
+  int mass_flux;
+  double U_bulk_target[3];
 } om_params;
 
 enum { OM_PROJ_LS = 0, OM_PROJ_EXACT = 1 };

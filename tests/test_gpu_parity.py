@@ -286,3 +286,16 @@ def test_gpu_exact_within_the_reference_rank_spread(name):
     for col in ("time", "E", "Umax", "Ub"):
         assert worst[col] <= 10.0 * spread[col] + 1e-8, col
     assert not [f for f in fails if "max_abs_divU" in f]
+
+
+def test_gpu_forcing_matches_oracle():
+    """Coriolis, harmonic and constant-mass-flux forcing on a wall-bounded channel against the
+    oracle (fields within 1e-10 after four steps)."""
+    from test_forcing import forced_channel
+    case = forced_channel()
+    mesh = case.make_mesh()
+    print("[gpu] forced channel", flush=True)
+    g, o = _pair(case, mesh, 4)
+    du, dp = rel_l2(o.U, g.U), rel_l2(o.p, g.p)
+    print(f"[gpu] forced channel: rel-L2 U {du:.2e} p {dp:.2e}", flush=True)
+    assert du < 1e-10 and dp < 1e-10
