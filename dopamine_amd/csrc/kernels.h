@@ -885,6 +885,43 @@ struct BcTemperature {
   }
 };
 
+// apply_passive_scalar_bc (src/bc/mod_bc.f90:582-679) for one boundary face and scalar k:
+// periodic faces interpolate with the partner cell; otherwise the patch's mode: Dirichlet value,
+// Neumann flux q (times cos(omega t + phase) when omega != 0) as C_o + q d_cf / (D + nu_t / Pr_t),
+// else zero gradient. Per-patch arrays are indexed [4 patch + k].
+struct BcScalar {
+  DevMesh M;
+  const double *C, *nus;
+  double* Cb;
+  const int* mode;
+  const double *val, *flux, *om, *ph;
+  double D, prt, time;
+  int k, filter;
+  DM_HD void operator()(int b) const {
+    const int f = M.nint + b, bc = M.bbc[b];
+    if (filter == 2 && bc != BC_PERIODIC) return;
+    if (filter == 1 && bc == BC_PERIODIC) return;
+    const int o = M.bown[b];
+    if (bc == BC_PERIODIC) {
+      const int nb = M.pnb[b];
+      Cb[b] = (nb >= 0) ? M.pgc[b] * C[o] + (1.0 - M.pgc[b]) * C[nb] : C[o];
+      return;
+    }
+    const int q = 4 * M.bpatch[b] + k;
+    if (mode[q] == 1) {
+      Cb[b] = val[q];
+    } else if (mode[q] == 2) {
+      double fl = flux[q];
+      if (fabs(om[q]) > SMALL) fl = fl * cos(om[q] * time + ph[q]);
+      const double af = D + nus[o] / prt;
+      const double dcf = fmax(M.dcf[f], SMALL);
+      Cb[b] = (af > SMALL) ? C[o] + fl * dcf / af : C[o];
+    } else {
+      Cb[b] = C[o];
+    }
+  }
+};
+
 // Face values of a scalar by QUICK / van Leer / Koren (advection_rhs_scalar_recon_soa,
 // src/numerics/mod_advection.f90:488-675): the context and far-upwind search of ReconFace, the
 // single-field QUICK fit (soa_quick_reconstruction_ctx_dev, :2552-2629).

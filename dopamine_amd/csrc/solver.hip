@@ -47,6 +47,12 @@ FieldRef field(Solver* s, const std::string& name) {
     if (name == "T_bnd") return {s->T_bnd, nb};
     if (name == "grad_T") return {s->gT, 3 * nc};
   }
+  for (int k = 0; k < s->ns; ++k) {
+    const std::string c = "C" + std::to_string(k);
+    if (name == c) return {s->Cs[k], nc};
+    if (name == c + "_bnd") return {s->Cb[k], nb};
+    if (name == "grad_" + c) return {s->gC[k], 3 * nc};
+  }
   throw std::runtime_error("unknown field " + name);
 }
 }  // namespace
@@ -78,6 +84,10 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (p->precond != 0 && p->precond != 1) throw std::runtime_error("precond must be 0 (Jacobi) or 1 (learned)");
     if (p->mass_flux && (p->f_const[0] != 0.0 || p->f_const[1] != 0.0 || p->f_const[2] != 0.0))
       throw std::runtime_error("f_const must be 0 with constant mass flux forcing");
+    if (p->n_scalars < 0 || p->n_scalars > 4) throw std::runtime_error("n_scalars must be 0..4");
+    if (p->n_scalars && p->scalar_scheme != dm::SCHEME_CDS && p->scalar_scheme != dm::SCHEME_QUICK &&
+        p->scalar_scheme != dm::SCHEME_VANLEER && p->scalar_scheme != dm::SCHEME_KOREN)
+      throw std::runtime_error("scalar_scheme must be CDS, QUICK, VANLEER_TVD or KOREN_TVD");
     if (p->cell_projection != 0 && p->cell_projection != 1)
       throw std::runtime_error("cell_projection must be 0 (LS) or 1 (EXACT)");
     if (p->cell_projection == 1 && !(p->cell_projection_tol > 0.0))
@@ -126,7 +136,8 @@ int dopamine_solver_set_field(void* h, const char* name, const double* host) {
     auto* s = static_cast<Solver*>(h);
     FieldRef f = field(s, name);
     s->x.upload(f.p, host, size_t(f.n));
-    if (std::string(name) == "U" || std::string(name) == "T") s->apply_all_bc();
+    const std::string n(name);
+    if (n == "U" || n == "T" || (n.size() == 2 && n[0] == 'C')) s->apply_all_bc();
     s->x.sync();
     return 0;
   });

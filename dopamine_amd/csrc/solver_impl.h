@@ -157,6 +157,13 @@ class SolverT {
   int bous = 0;
   double *T = nullptr, *T2 = nullptr, *T_bnd = nullptr, *gT = nullptr, *gT_prev = nullptr, *HT = nullptr,
          *HT0 = nullptr, *phiT = nullptr, *patchT = nullptr;
+  // passive scalars: per scalar field, predictor target, boundary values, gradient, H and H_prev;
+  // shared face values and gradient scratch; per-patch BC arrays
+  int ns = 0;
+  double *Cs[4] = {}, *Cs2[4] = {}, *Cb[4] = {}, *gC[4] = {}, *HC[4] = {}, *HC0[4] = {};
+  double *phiC = nullptr, *gCprev = nullptr;
+  int* s_mode = nullptr;
+  double *s_val = nullptr, *s_flux = nullptr, *s_om = nullptr, *s_ph = nullptr;
   // MOST wall model: heat flux and Obukhov length per boundary face, the configuration's L
   double *q_w = nullptr, *L_bnd = nullptr, *L_cfg = nullptr;
   double most_z0m = 0.0, most_z0h = 0.0;
@@ -646,6 +653,26 @@ class SolverT {
       x.upload(T_bnd, tref_b.data(), tref_b.size());
       patchT = up(pt);
     }
+    ns = p.n_scalars;
+    if (ns) {
+      for (int k = 0; k < ns; ++k) {
+        Cs[k] = fresh<double>(nc); Cs2[k] = fresh<double>(nc); Cb[k] = fresh<double>(std::max(1, nb));
+        gC[k] = fresh<double>(3 * size_t(nc)); HC[k] = fresh<double>(nc); HC0[k] = fresh<double>(nc);
+      }
+      gCprev = fresh<double>(3 * size_t(nc));
+      if (p.scalar_scheme != SCHEME_CDS) phiC = fresh<double>(nf);
+      const size_t m = 4 * size_t(std::max(1, npatch));
+      std::vector<int> hm(m, 3);
+      std::vector<double> hv(m, 0.0), hf(m, 0.0), ho(m, 0.0), hp(m, 0.0);
+      for (int q = 0; q < 4 * npatch; ++q) {
+        if (p.scalar_bc_mode) hm[q] = p.scalar_bc_mode[q];
+        if (p.scalar_bc_value) hv[q] = p.scalar_bc_value[q];
+        if (p.scalar_bc_flux) hf[q] = p.scalar_bc_flux[q];
+        if (p.scalar_bc_omega) ho[q] = p.scalar_bc_omega[q];
+        if (p.scalar_bc_phase) hp[q] = p.scalar_bc_phase[q];
+      }
+      s_mode = up(hm); s_val = up(hv); s_flux = up(hf); s_om = up(ho); s_ph = up(hp);
+    }
     cproj = p.cell_projection == 1;
     if (cproj) {
       pj_b = fresh<double>(nc); pj_x = fresh<double>(nc); pj_r = fresh<double>(nc); pj_z = fresh<double>(nc);
@@ -758,10 +785,16 @@ class SolverT {
   void bc_temperature(int filter, const double* g) {
     if (bous) x.for_each(nb, BcTemperature{M, T, g, patchT, T_bnd, filter});
   }
+  void bc_scalars(int filter) {
+    for (int k = 0; k < ns; ++k)
+      x.for_each(nb, BcScalar{M, Cs[k], nu_sgs, Cb[k], s_mode, s_val, s_flux, s_om, s_ph, prm.scalar_diffusivity[k],
+                              prm.Pr_t, time, k, filter});
+  }
   void apply_all_bc() {
     bc_velocity(0, grad);
     bc_pressure(0, skip_grad_p ? nullptr : gp, 1.0, outlet_pb);
     bc_temperature(0, gT);
+    bc_scalars(0);
   }
   // refresh_halos_and_bc (single rank: non-periodic pass, then periodic pass)
   void refresh_bc() {
@@ -769,6 +802,7 @@ class SolverT {
       bc_velocity(pass, grad);
       bc_pressure(pass, skip_grad_p ? nullptr : gp, 1.0, outlet_pb);
       bc_temperature(pass, gT);
+      bc_scalars(pass);
     }
   }
   void velocity_bc_both() {
@@ -819,6 +853,24 @@ class SolverT {
                                 (prm.wall_model == 2 && n_wm) ? q_w : nullptr});
   }
 
+  // H_C and the predictor of each passive scalar into Cs2 (compute_explicit_rhs, :1719-1743)
+  void explicit_rhs_C(double dt, int k) {
+    const ReconFace<SCHEME_QUICK> rq{M, U, U_bnd, UF, grad, nullptr};
+    const ReconFace<SCHEME_VANLEER> rv{M, U, U_bnd, UF, grad, nullptr};
+    const ReconFace<SCHEME_KOREN> rk{M, U, U_bnd, UF, grad, nullptr};
+    for (int q = 0; q < ns; ++q) {
+      switch (prm.scalar_scheme) {
+        case SCHEME_QUICK: x.for_each(nf, ReconFaceScalar<SCHEME_QUICK>{rq, Cs[q], Cb[q], gC[q], phiC}); break;
+        case SCHEME_VANLEER: x.for_each(nf, ReconFaceScalar<SCHEME_VANLEER>{rv, Cs[q], Cb[q], gC[q], phiC}); break;
+        case SCHEME_KOREN: x.for_each(nf, ReconFaceScalar<SCHEME_KOREN>{rk, Cs[q], Cb[q], gC[q], phiC}); break;
+        default: break;
+      }
+      x.for_each(nc, ExplicitRhsT{M, Cs[q], Cb[q], gC[q], UF, nu_sgs, HC0[q], phiC, HC[q], Cs2[q],
+                                  prm.scalar_diffusivity[q], prm.scalar_pr_t[q], dt, RK3_A[k], RK3_B[k],
+                                  prm.scalar_scheme == SCHEME_CDS ? 1 : 0, nullptr});
+    }
+  }
+
   void gradients(bool with_sgs, double sgs_dt) {
     const int sgs = with_sgs ? prm.sgs : -1;
     if (bous) {  // compute_all_gradients: the temperature gradient alongside the velocity's
@@ -829,6 +881,17 @@ class SolverT {
         for (int it = 2; it <= prm.gg_iterations; ++it) {
           x.copy(gT_prev, gT, 3 * size_t(nc));
           x.for_each(nc, GgGradSca{M, T, nullptr, T_bnd, nullptr, gT_prev, gT});
+        }
+      }
+    }
+    for (int k = 0; k < ns; ++k) {  // the scalars' gradients (LS, or GG with its sweeps)
+      if (prm.gradient == 0) {
+        x.for_each(nc, LsGradSca{M, Cs[k], nullptr, Cb[k], nullptr, gC[k]});
+      } else {
+        x.for_each(nc, GgGradSca{M, Cs[k], nullptr, Cb[k], nullptr, nullptr, gC[k]});
+        for (int it = 2; it <= prm.gg_iterations; ++it) {
+          x.copy(gCprev, gC[k], 3 * size_t(nc));
+          x.for_each(nc, GgGradSca{M, Cs[k], nullptr, Cb[k], nullptr, gCprev, gC[k]});
         }
       }
     }
@@ -1062,6 +1125,10 @@ class SolverT {
       x.zero(HT, nc);
       x.zero(HT0, nc);
     }
+    for (int q = 0; q < ns; ++q) {
+      x.zero(HC[q], nc);
+      x.zero(HC0[q], nc);
+    }
     x.zero(rhs, nc);
     const double time_n = time;
     int rc = 0;
@@ -1092,8 +1159,10 @@ class SolverT {
       }
       x.tend(K_EXPLICIT);
       if (bous) explicit_rhs_T(dt, k);
+      if (ns) explicit_rhs_C(dt, k);
       std::swap(U, U2);
       if (bous) std::swap(T, T2);
+      for (int q = 0; q < ns; ++q) std::swap(Cs[q], Cs2[q]);
       velocity_bc_both();
       x.copy(p_old, P, nc);
       if (cproj)
@@ -1151,6 +1220,7 @@ class SolverT {
       pressure_gradient();
       std::swap(H, H0);
       if (bous) std::swap(HT, HT0);
+      for (int q = 0; q < ns; ++q) std::swap(HC[q], HC0[q]);
     }
     time = time_n;
     return rc;

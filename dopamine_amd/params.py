@@ -13,6 +13,9 @@ SCHEME_IDS = {"VANLEER_TVD": 1, "VANLEER": 1, "KOREN_TVD": 2, "KOREN": 2, "CDS":
 SGS_IDS = {"NONE": 0, "VREMAN": 1, "SMAGORINSKY": 2, "AMD": 3}
 WALL_MODEL_IDS = {"loglaw": 0, "none": 1, "most": 2}
 
+MAX_SCALARS = 4
+SCALAR_DIRICHLET, SCALAR_NEUMANN, SCALAR_ZERO_GRADIENT = 1, 2, 3  # src/mod_scalar.f90:13-15
+
 # Wray RK3 coefficients (src/mod_parameters.f90:64-71)
 RK3W_GAMMA1 = 8.0 / 15.0
 
@@ -87,6 +90,15 @@ class SolverParams:
     forcing_dir: tuple = (1.0, 0.0, 0.0)
     constant_mass_flux_forcing: bool = False
     U_bulk_target: tuple = (0.0, 0.0, 0.0)
+    # passive scalars (&dopamine_scalars; src/solver/mod_fv_solver.f90:56-78): count (at most 4),
+    # advection scheme, molecular diffusivity and turbulent Schmidt number per scalar; per patch
+    # name a list with one entry per scalar: {"mode": "dirichlet"|"neumann"|"zero_gradient",
+    # "value", "flux", "omega", "phase"} (patch_scalar_*, src/io/mod_input_parser.f90:465-546)
+    n_scalars: int = 0
+    scalar_scheme: str = "VANLEER_TVD"
+    scalar_diffusivity: tuple = (0.0, 0.0, 0.0, 0.0)
+    scalar_pr_t: tuple = (1.0, 1.0, 1.0, 1.0)
+    patch_scalar: dict = field(default_factory=dict)
     # per-patch values, by patch name: p_val (outlet), U_val (inlet / moving wall)
     patch_p: dict = field(default_factory=dict)
     patch_U: dict = field(default_factory=dict)
@@ -112,8 +124,29 @@ class SolverParams:
             raise ValueError(f"precond {self.precond!r} not supported (JACOBI, LEARNED)")
         if self.constant_mass_flux_forcing and any(f != 0.0 for f in self.f_const):
             raise ValueError("f_const must be 0 with constant_mass_flux_forcing")
+        if not 0 <= self.n_scalars <= MAX_SCALARS:
+            raise ValueError(f"n_scalars must be 0..{MAX_SCALARS}")
+        if self.n_scalars and self.scalar_scheme.upper() not in ("CDS", "CENTRAL", "QUICK", "VANLEER_TVD", "VANLEER",
+                                                                 "KOREN_TVD", "KOREN"):
+            raise ValueError(f"scalar_scheme {self.scalar_scheme!r} not supported")
         if self.adapt_dt:
             raise ValueError("adapt_dt is not supported; the fixed-dt CFL clamp of solver_advance applies")
+
+    def scalar_arrays(self, mesh):
+        """Per-patch scalar BC arrays [MAX_SCALARS * patch + scalar]: mode (int32), value, flux,
+        omega, phase. Modes by name as scalar_bc_mode_from_name (src/mod_scalar.f90:63-73)."""
+        n = max(1, mesh.n_patches) * MAX_SCALARS
+        mode = np.full(n, SCALAR_ZERO_GRADIENT, dtype=np.int32)
+        vals = {k: np.zeros(n) for k in ("value", "flux", "omega", "phase")}
+        for p, name in enumerate(mesh.patch_names[:mesh.n_patches]):
+            for k, spec in enumerate(self.patch_scalar.get(name, [])[:MAX_SCALARS]):
+                q = MAX_SCALARS * p + k
+                m = str(spec.get("mode", "zero_gradient")).strip().lower()
+                mode[q] = {"dirichlet": SCALAR_DIRICHLET, "fixed": SCALAR_DIRICHLET, "fixedvalue": SCALAR_DIRICHLET,
+                           "neumann": SCALAR_NEUMANN, "flux": SCALAR_NEUMANN}.get(m, SCALAR_ZERO_GRADIENT)
+                for key in vals:
+                    vals[key][q] = float(spec.get(key, 0.0))
+        return mode, vals["value"], vals["flux"], vals["omega"], vals["phase"]
 
     def forcing_args(self):
         """The forcing fields in the C ABI's order (after cell_projection_tol)."""
@@ -125,6 +158,9 @@ class SolverParams:
 
     def projection_id(self):
         return 1 if self.cell_projection.upper() == "EXACT" else 0
+
+    def scalar_scheme_id(self):
+        return SCHEME_IDS[self.scalar_scheme.upper()]
 
     def scheme_id(self):
         return SCHEME_IDS[self.advection_scheme.upper()]

@@ -43,7 +43,10 @@ class Params(ctypes.Structure):
                 ("compute_L", _i32), ("cell_projection", _i32), ("cell_projection_tol", _d),
                 ("use_rotation", _i32), ("omega_rot", _d * 3), ("use_harmonic", _i32), ("forcing_amp", _d),
                 ("forcing_omega", _d), ("forcing_phase", _d), ("forcing_dir", _d * 3), ("mass_flux", _i32),
-                ("U_bulk_target", _d * 3)]
+                ("U_bulk_target", _d * 3), ("n_scalars", _i32), ("scalar_scheme", _i32),
+                ("scalar_diffusivity", _d * 4), ("scalar_pr_t", _d * 4), ("scalar_bc_mode", _P(_i32)),
+                ("scalar_bc_value", _P(_d)), ("scalar_bc_flux", _P(_d)), ("scalar_bc_omega", _P(_d)),
+                ("scalar_bc_phase", _P(_d))]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -62,6 +65,7 @@ def build_abi(mesh, params, precision=0):
     md = MeshDesc(m.n_cells, m.n_faces, m.n_int_faces, m.n_bnd_faces, m.n_patches,
                   *[arr(getattr(m, n), t) for n, t in MESH_FIELDS])
     pv, uv, tv = params.patch_arrays(mesh, with_T=True)
+    sm, sv, sfl, so, sph = params.scalar_arrays(mesh)
     pr = Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
                 0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
                 params.sgs_id(), params.c_vreman, params.cs_smagorinsky, WALL_MODEL_IDS[params.wall_model],
@@ -73,7 +77,9 @@ def build_abi(mesh, params, precision=0):
                 params.t_scheme_id(), arr(tv, _d), 1 if params.precond.upper() == "LEARNED" else 0,
                 params.learn_collect, params.learn_capacity, params.learn_iters, params.learn_batch,
                 params.learn_seed, params.learn_lr, params.z0_m, params.z0_h, params.T_surface, params.L_obukhov,
-                int(params.compute_L), params.projection_id(), params.cell_projection_tol, *params.forcing_args())
+                int(params.compute_L), params.projection_id(), params.cell_projection_tol, *params.forcing_args(),
+                params.n_scalars, params.scalar_scheme_id(), (_d * 4)(*params.scalar_diffusivity),
+                (_d * 4)(*params.scalar_pr_t), arr(sm, _i32), arr(sv, _d), arr(sfl, _d), arr(so, _d), arr(sph, _d))
     return md, pr, keep
 
 
@@ -166,6 +172,13 @@ class FVSolver:
 
     def set_temperature(self, T):
         self.set("T", T)
+
+    def set_scalar(self, k, C):
+        """Passive scalar k (cell values); its boundary values are refreshed."""
+        self.set(f"C{k}", C)
+
+    def scalar(self, k):
+        return self.get(f"C{k}")
 
     def project_once(self, dt):
         self._check(self.L.dopamine_solver_project_once(self.h, dt), "project_once")

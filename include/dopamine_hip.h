@@ -97,6 +97,15 @@ typedef struct dopamine_params {
   double forcing_amp, forcing_omega, forcing_phase, forcing_dir[3];
   int32_t mass_flux;
   double U_bulk_target[3];
+  /* passive scalars (n_scalars <= 4; compute_explicit_rhs :1719-1743, apply_passive_scalar_bc,
+   * src/bc/mod_bc.f90:582-679): advection scheme (as t_scheme), molecular diffusivity and
+   * turbulent Schmidt number per scalar; per patch and scalar ([4 patch + s], 4 n_patches entries)
+   * the BC mode (1 Dirichlet, 2 Neumann, 3 zero gradient), value, flux, flux frequency, phase.
+   * Fields "C<s>", "C<s>_bnd", "grad_C<s>". */
+  int32_t n_scalars, scalar_scheme;
+  double scalar_diffusivity[4], scalar_pr_t[4];
+  const int32_t *scalar_bc_mode;
+  const double *scalar_bc_value, *scalar_bc_flux, *scalar_bc_omega, *scalar_bc_phase;
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */
@@ -109,8 +118,9 @@ const char *dopamine_last_error(void);
 /* Copy a host field in / out. Names: "U" (3*nc, component-fastest), "p" (nc), "nu_sgs", "U_bnd",
  * "p_bnd", "grad_U" (9*nc, grad(i,j) = du_j/dx_i at [9c+3j+i]), "U_face" (nf), "tau_w_bnd",
  * "rhs_p", "H", with boussinesq "T" (nc), "T_bnd", "grad_T" (3*nc), with wall_model 2 "q_w_bnd"
- * and "L_bnd" (one per boundary face). set_field("U") and
- * set_field("T") re-apply the boundary conditions as solver_init does. */
+ * and "L_bnd" (one per boundary face), with passive scalars "C<s>" (nc), "C<s>_bnd", "grad_C<s>".
+ * set_field("U"), set_field("T") and set_field("C<s>") re-apply the boundary conditions as
+ * solver_init does. */
 int dopamine_solver_set_field(void *solver, const char *name, const double *host);
 int dopamine_solver_get_field(void *solver, const char *name, double *host, int64_t len);
 

@@ -41,7 +41,10 @@ class _Params(ctypes.Structure):
                 ("cell_projection", ctypes.c_int), ("cell_projection_tol", _d),
                 ("use_rotation", ctypes.c_int), ("omega_rot", _d * 3), ("use_harmonic", ctypes.c_int),
                 ("forcing_amp", _d), ("forcing_omega", _d), ("forcing_phase", _d), ("forcing_dir", _d * 3),
-                ("mass_flux", ctypes.c_int), ("U_bulk_target", _d * 3)]
+                ("mass_flux", ctypes.c_int), ("U_bulk_target", _d * 3),
+                ("n_scalars", ctypes.c_int), ("scalar_scheme", ctypes.c_int), ("scalar_diffusivity", _d * 4),
+                ("scalar_pr_t", _d * 4), ("scalar_bc_mode", _P(ctypes.c_int)), ("scalar_bc_value", _P(_d)),
+                ("scalar_bc_flux", _P(_d)), ("scalar_bc_omega", _P(_d)), ("scalar_bc_phase", _P(_d))]
 
 
 _lib = None
@@ -121,6 +124,8 @@ class Solver:
                         arr(m.periodic_gc, _d), arr(m.ls_offset, _i32), arr(m.ls_cell, _i32), arr(m.ls_face, _i32),
                         arr(m.ls_weight, _d))
         pv, uv, tv = params.patch_arrays(mesh, with_T=True)
+        sm, sv, sfl, so, sph = params.scalar_arrays(mesh)
+        scalar_ptrs = (arr(sm, ctypes.c_int), arr(sv, _d), arr(sfl, _d), arr(so, _d), arr(sph, _d))
         from dopamine_amd.params import WALL_MODEL_IDS
         self._pr = _Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
                            0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
@@ -132,7 +137,9 @@ class Solver:
                            int(params.amd_wall_gradient), int(params.boussinesq), params.alpha_T, params.beta,
                            params.T_ref, (_d * 3)(*params.g), params.Pr_t, params.t_scheme_id(), arr(tv, _d),
                            params.z0_m, params.z0_h, params.T_surface, params.L_obukhov, int(params.compute_L),
-                           params.projection_id(), params.cell_projection_tol, *params.forcing_args())
+                           params.projection_id(), params.cell_projection_tol, *params.forcing_args(),
+                           params.n_scalars, params.scalar_scheme_id(), (_d * 4)(*params.scalar_diffusivity),
+                           (_d * 4)(*params.scalar_pr_t), *scalar_ptrs)
         self.params = params
         self.mesh_nc = mesh.n_cells
         self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))
@@ -169,6 +176,13 @@ class Solver:
     def set_temperature(self, T):
         self.array("T")[:] = np.asarray(T, dtype=np.float64).reshape(-1)
         lib().om_apply_all_bc(self.h)
+
+    def set_scalar(self, k, C):
+        self.array(f"C{k}")[:] = np.asarray(C, dtype=np.float64).reshape(-1)
+        lib().om_apply_all_bc(self.h)
+
+    def scalar(self, k):
+        return self.array(f"C{k}").copy()
 
     def set_velocity(self, U):
         self.array("U")[:] = np.asarray(U, dtype=np.float64).reshape(-1)

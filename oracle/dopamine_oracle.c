@@ -9,6 +9,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -69,6 +70,9 @@ struct om_solver {
   double *cp_sf, *cp_gc, *cp_bsf;
   int64_t n_proj, n_proj_its;
   double mff[3]; /* constant mass flux forcing of the current step */
+  /* passive scalars */
+  double *C[OM_MAX_SCALARS], *C_bnd[OM_MAX_SCALARS], *grad_C[OM_MAX_SCALARS], *H_C[OM_MAX_SCALARS],
+      *H_C0[OM_MAX_SCALARS], *gC_prev;
 };
 
 /* body_force_now (src/solver/mod_fv_solver.f90:2904-2923): constant or mass-flux force plus the
@@ -249,6 +253,39 @@ static void apply_pressure_bc(om_solver *s, const double *p, double *pb, int fil
 }
 
 /* apply_temperature_bc (src/bc/mod_bc.f90:489-580). filter: 0 all, 1 non-periodic, 2 periodic. */
+/* apply_passive_scalar_bc (src/bc/mod_bc.f90:582-679) for scalar k: periodic faces interpolate
+ * with the partner cell, other faces by the patch's mode: Dirichlet value, Neumann flux q (times
+ * cos(omega t + phase) when omega != 0) as C_o + q d_cf / (D + nu_t / Pr_t) with the temperature's
+ * Pr_t, else zero gradient. */
+static void apply_scalar_bc(om_solver *s, int k, int filter) {
+  double *C = s->C[k], *Cb = s->C_bnd[k];
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b;
+    const int bc = bc_of_face(s, f);
+    if (filter == 2 && bc != BC_PERIODIC) continue;
+    if (filter == 1 && bc == BC_PERIODIC) continue;
+    const int o = s->m.face_owner[f], pid = s->m.face_patch[f];
+    if (bc == BC_PERIODIC) {
+      const int nb = s->per_nb[b];
+      Cb[b] = (nb >= 0) ? s->m.periodic_gc[b] * C[o] + (1.0 - s->m.periodic_gc[b]) * C[nb] : C[o];
+      continue;
+    }
+    const int q = OM_MAX_SCALARS * pid + k;
+    const int mode = s->p.scalar_bc_mode ? s->p.scalar_bc_mode[q] : OM_SCALAR_ZERO_GRADIENT;
+    if (mode == OM_SCALAR_DIRICHLET) {
+      Cb[b] = s->p.scalar_bc_value[q];
+    } else if (mode == OM_SCALAR_NEUMANN) {
+      double fl = s->p.scalar_bc_flux[q];
+      if (fabs(s->p.scalar_bc_omega[q]) > SMALL) fl = fl * cos(s->p.scalar_bc_omega[q] * s->time + s->p.scalar_bc_phase[q]);
+      const double af = s->p.scalar_diffusivity[k] + s->nu_sgs[o] / s->p.Pr_t;
+      const double dcf = fmax(s->m.face_dcf[f], SMALL);
+      Cb[b] = (af > SMALL) ? C[o] + fl * dcf / af : C[o];
+    } else {
+      Cb[b] = C[o];
+    }
+  }
+}
+
 static void apply_temperature_bc(om_solver *s, int filter, const double *gT) {
   if (!s->p.boussinesq) return;
   for (int b = 0; b < s->nb; ++b) {
@@ -402,10 +439,11 @@ static void gg_gradient_scalar(const om_solver *s, const double *phi, const doub
 
 /* compute_all_gradients (src/solver/mod_fv_solver.f90:1003-1077), velocity only */
 static void compute_all_gradients(om_solver *s) {
-  const int temp = s->p.boussinesq;
+  const int temp = s->p.boussinesq, ns = s->p.n_scalars;
   if (s->p.gradient == OM_GRAD_GG) {
     gg_gradient_vector(s, s->U, s->U_bnd, NULL, s->grad_U);
     if (temp) gg_gradient_scalar(s, s->T, s->T_bnd, NULL, s->grad_T);
+    for (int k = 0; k < ns; ++k) gg_gradient_scalar(s, s->C[k], s->C_bnd[k], NULL, s->grad_C[k]);
     for (int it = 2; it <= s->p.gg_iterations; ++it) {
       memcpy(s->gU_prev, s->grad_U, sizeof(double) * 9 * (size_t)s->nc);
       gg_gradient_vector(s, s->U, s->U_bnd, s->gU_prev, s->grad_U);
@@ -413,10 +451,15 @@ static void compute_all_gradients(om_solver *s) {
         memcpy(s->gT_prev, s->grad_T, sizeof(double) * 3 * (size_t)s->nc);
         gg_gradient_scalar(s, s->T, s->T_bnd, s->gT_prev, s->grad_T);
       }
+      for (int k = 0; k < ns; ++k) {
+        memcpy(s->gC_prev, s->grad_C[k], sizeof(double) * 3 * (size_t)s->nc);
+        gg_gradient_scalar(s, s->C[k], s->C_bnd[k], s->gC_prev, s->grad_C[k]);
+      }
     }
   } else {
     ls_gradient_vector(s, s->U, s->U_bnd, s->grad_U);
     if (temp) ls_gradient_scalar(s, s->T, s->T_bnd, s->grad_T);
+    for (int k = 0; k < ns; ++k) ls_gradient_scalar(s, s->C[k], s->C_bnd[k], s->grad_C[k]);
   }
 }
 
@@ -445,6 +488,7 @@ static void refresh_bc(om_solver *s) {
     apply_velocity_bc(s, pass, s->grad_U);
     apply_pressure_bc(s, s->P, s->p_bnd, pass, s->skip_grad_p ? NULL : s->gp, 1.0, s->outlet_pb);
     apply_temperature_bc(s, pass, s->grad_T);
+    for (int k = 0; k < s->p.n_scalars; ++k) apply_scalar_bc(s, k, pass);
   }
 }
 
@@ -1430,8 +1474,7 @@ static void advection_scalar_recon(om_solver *s, int scheme, const double *phi, 
 
 /* viscous_rhs_scalar (src/numerics/mod_viscous.f90:286-513) */
 static void viscous_scalar(om_solver *s, const double *phi, const double *pb, const double *gphi, double alpha,
-                           double *rhs) {
-  const double prt = s->p.Pr_t;
+                           double prt, double *rhs) {
   memset(rhs, 0, sizeof(double) * s->nc);
   for (int f = 0; f < s->nf; ++f) {
     const int o = s->m.face_owner[f], nb = s->m.face_neighbour[f];
@@ -1503,7 +1546,7 @@ static void explicit_rhs(om_solver *s) {
       advection_scalar_cds(s, s->T, s->T_bnd, s->grad_T, s->conv_T);
     else
       advection_scalar_recon(s, s->p.t_scheme, s->T, s->T_bnd, s->grad_T, s->conv_T);
-    viscous_scalar(s, s->T, s->T_bnd, s->grad_T, s->p.alpha_T, s->visc_T);
+    viscous_scalar(s, s->T, s->T_bnd, s->grad_T, s->p.alpha_T, s->p.Pr_t, s->visc_T);
   }
   double fb[3];
   body_force_now(s, fb);
@@ -1537,6 +1580,16 @@ static void explicit_rhs(om_solver *s) {
       const int o = s->m.face_owner[f];
       s->H_T[o] = s->H_T[o] + s->q_w[b] * s->m.face_area[f] / s->m.cell_volume[o];
     }
+  /* passive scalars, one at a time (:1719-1743): advection by scalar_scheme, diffusion with the
+   * scalar's diffusivity and turbulent Schmidt number */
+  for (int k = 0; k < s->p.n_scalars; ++k) {
+    if (s->p.scalar_scheme == OM_SCHEME_CDS)
+      advection_scalar_cds(s, s->C[k], s->C_bnd[k], s->grad_C[k], s->conv_T);
+    else
+      advection_scalar_recon(s, s->p.scalar_scheme, s->C[k], s->C_bnd[k], s->grad_C[k], s->conv_T);
+    viscous_scalar(s, s->C[k], s->C_bnd[k], s->grad_C[k], s->p.scalar_diffusivity[k], s->p.scalar_pr_t[k], s->visc_T);
+    for (int c = 0; c < s->nc; ++c) s->H_C[k][c] = s->conv_T[c] + s->visc_T[c];
+  }
 }
 
 /* ---------------------------------------------------------------- projection */
@@ -2182,6 +2235,10 @@ static int advance_rk3w(om_solver *s, double dt) {
   memset(s->H0, 0, sizeof(double) * 3 * (size_t)nc);
   memset(s->H_T, 0, sizeof(double) * nc);
   memset(s->H_T0, 0, sizeof(double) * nc);
+  for (int q = 0; q < s->p.n_scalars; ++q) {
+    memset(s->H_C[q], 0, sizeof(double) * nc);
+    memset(s->H_C0[q], 0, sizeof(double) * nc);
+  }
   memset(s->rhs_p, 0, sizeof(double) * nc);
   const double time_n = s->time;
   int rc = 0;
@@ -2196,6 +2253,8 @@ static int advance_rk3w(om_solver *s, double dt) {
       for (int q = 0; q < 3; ++q) s->U[3 * c + q] = s->U[3 * c + q] + dt * (a * s->H[3 * c + q] + bb * s->H0[3 * c + q]);
       if (s->p.boussinesq) s->T[c] = s->T[c] + dt * (a * s->H_T[c] + bb * s->H_T0[c]);
     }
+    for (int q = 0; q < s->p.n_scalars; ++q)
+      for (int c = 0; c < nc; ++c) s->C[q][c] = s->C[q][c] + dt * (a * s->H_C[q][c] + bb * s->H_C0[q][c]);
     velocity_bc_both(s);
     memcpy(s->p_old, s->P, sizeof(double) * nc);
     if (s->p.cell_projection == OM_PROJ_EXACT)
@@ -2263,6 +2322,7 @@ static int advance_rk3w(om_solver *s, double dt) {
     compute_pressure_gradient(s);
     memcpy(s->H0, s->H, sizeof(double) * 3 * (size_t)nc);
     if (s->p.boussinesq) memcpy(s->H_T0, s->H_T, sizeof(double) * nc);
+    for (int q = 0; q < s->p.n_scalars; ++q) memcpy(s->H_C0[q], s->H_C[q], sizeof(double) * nc);
   }
   s->time = time_n;
   return rc;
@@ -2418,6 +2478,15 @@ om_solver *om_create(const om_mesh *m, const om_params *p) {
   s->H = dz(3 * (size_t)nc); s->H0 = dz(3 * (size_t)nc); s->conv = dz(3 * (size_t)nc); s->visc = dz(3 * (size_t)nc);
   s->T = dz(nc); s->T_bnd = dz(nb); s->grad_T = dz(3 * (size_t)nc); s->gT_prev = dz(3 * (size_t)nc);
   s->H_T = dz(nc); s->H_T0 = dz(nc); s->conv_T = dz(nc); s->visc_T = dz(nc);
+  if (p->n_scalars < 0 || p->n_scalars > OM_MAX_SCALARS) {
+    om_free(s);
+    return NULL;
+  }
+  for (int q = 0; q < p->n_scalars; ++q) {
+    s->C[q] = dz(nc); s->C_bnd[q] = dz(nb); s->grad_C[q] = dz(3 * (size_t)nc);
+    s->H_C[q] = dz(nc); s->H_C0[q] = dz(nc);
+  }
+  s->gC_prev = dz(3 * (size_t)nc);
   for (int c = 0; c < nc; ++c) s->T[c] = p->T_ref;
   for (int b = 0; b < nb; ++b) s->T_bnd[b] = p->T_ref;
   s->rhs_p = dz(nc); s->div_w = dz(nc); s->phi_it = dz(nc); s->div_diag = dz(nc);
@@ -2649,6 +2718,7 @@ void om_apply_all_bc(om_solver *s) {
   apply_velocity_bc(s, 0, s->grad_U);
   apply_pressure_bc(s, s->P, s->p_bnd, 0, s->skip_grad_p ? NULL : s->gp, 1.0, s->outlet_pb);
   apply_temperature_bc(s, 0, s->grad_T);
+  for (int k = 0; k < s->p.n_scalars; ++k) apply_scalar_bc(s, k, 0);
 }
 
 void om_free(om_solver *s) {
@@ -2668,6 +2738,10 @@ void om_free(om_solver *s) {
   free(s->act_int);
   free(s->act_bnd);
   free(s->cp_own); free(s->cp_nbr); free(s->cp_bcell); free(s->cp_sf); free(s->cp_gc); free(s->cp_bsf);
+  for (int q = 0; q < OM_MAX_SCALARS; ++q) {
+    free(s->C[q]); free(s->C_bnd[q]); free(s->grad_C[q]); free(s->H_C[q]); free(s->H_C0[q]);
+  }
+  free(s->gC_prev);
   free(s);
 }
 
@@ -2688,6 +2762,15 @@ double *om_array(om_solver *s, const char *name, int64_t *len) {
       if (len) *len = t[i].l;
       return t[i].p;
     }
+  for (int k = 0; k < s->p.n_scalars; ++k) {
+    char a[32], b[32], g[32];
+    snprintf(a, sizeof a, "C%d", k);
+    snprintf(b, sizeof b, "C%d_bnd", k);
+    snprintf(g, sizeof g, "grad_C%d", k);
+    if (strcmp(name, a) == 0) { if (len) *len = nc; return s->C[k]; }
+    if (strcmp(name, b) == 0) { if (len) *len = nb; return s->C_bnd[k]; }
+    if (strcmp(name, g) == 0) { if (len) *len = 3 * nc; return s->grad_C[k]; }
+  }
   if (len) *len = 0;
   return NULL;
 }

@@ -82,7 +82,17 @@ typedef struct om_params {
   double forcing_amp, forcing_omega, forcing_phase, forcing_dir[3];
   int mass_flux;
   double U_bulk_target[3];
+  /* passive scalars (n_scalars <= OM_MAX_SCALARS): advection scheme, molecular diffusivity and
+   * turbulent Schmidt number per scalar; per patch and scalar ([OM_MAX_SCALARS p + s]) the BC mode
+   * (OM_SCALAR_DIRICHLET, _NEUMANN, _ZERO_GRADIENT), value, flux, flux frequency and phase */
+  int n_scalars;
+  int scalar_scheme;
+  double scalar_diffusivity[4], scalar_pr_t[4];
+  const int *scalar_bc_mode;
+  const double *scalar_bc_value, *scalar_bc_flux, *scalar_bc_omega, *scalar_bc_phase;
 } om_params;
+
+enum { OM_MAX_SCALARS = 4, OM_SCALAR_DIRICHLET = 1, OM_SCALAR_NEUMANN = 2, OM_SCALAR_ZERO_GRADIENT = 3 };
 
 enum { OM_PROJ_LS = 0, OM_PROJ_EXACT = 1 };
 
@@ -91,8 +101,8 @@ typedef struct om_solver om_solver;
 om_solver *om_create(const om_mesh *m, const om_params *p);
 void om_free(om_solver *s);
 /* Pointer to an internal array by name ("U","p","p_bnd","U_bnd","nu_sgs","grad_U","U_face",
- * "face_flux","tau_w_bnd","w_gp","rhs_p","T","T_bnd","grad_T","q_w_bnd","L_bnd"), with its
- * length in doubles. */
+ * "face_flux","tau_w_bnd","w_gp","rhs_p","T","T_bnd","grad_T","q_w_bnd","L_bnd", and per
+ * passive scalar s "C<s>", "C<s>_bnd", "grad_C<s>"), with its length in doubles. */
 double *om_array(om_solver *s, const char *name, int64_t *len);
 /* After U (and p) were written from outside: refresh boundary values as solver_init does. */
 void om_apply_all_bc(om_solver *s);

@@ -81,6 +81,13 @@ class HostSolver:
         self.array("T")[:] = np.asarray(T, dtype=np.float64).reshape(-1)
         lib().hc_solver_apply_all_bc(self.h)
 
+    def set_scalar(self, k, C):
+        self.array(f"C{k}")[:] = np.asarray(C, dtype=np.float64).reshape(-1)
+        lib().hc_solver_apply_all_bc(self.h)
+
+    def scalar(self, k):
+        return self.array(f"C{k}").copy()
+
     def cell_divergence(self):
         out = (_d * 2)()
         lib().hc_solver_cell_divergence(self.h, out)

@@ -27,6 +27,11 @@ double* field(Solver* s, const std::string& name, int64_t* n) {
   if (s->bous && name == "T") { *n = nc; return s->T; }
   if (s->bous && name == "T_bnd") { *n = nb; return s->T_bnd; }
   if (s->bous && name == "grad_T") { *n = 3 * nc; return s->gT; }
+  for (int k = 0; k < s->ns; ++k) {
+    const std::string c = "C" + std::to_string(k);
+    if (name == c) { *n = nc; return s->Cs[k]; }
+    if (name == c + "_bnd") { *n = nb; return s->Cb[k]; }
+  }
   *n = 0;
   return nullptr;
 }
