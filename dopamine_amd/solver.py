@@ -134,6 +134,9 @@ class FVSolver:
                        "rhs_p": mesh.n_cells, "H": 3 * mesh.n_cells, "w_gp": 3 * mesh.n_cells,
                        "T": mesh.n_cells, "T_bnd": max(1, mesh.n_bnd_faces), "grad_T": 3 * mesh.n_cells,
                        "q_w_bnd": max(1, mesh.n_bnd_faces), "L_bnd": max(1, mesh.n_bnd_faces)}
+        for k in range(params.n_scalars):
+            self._sizes.update({f"C{k}": mesh.n_cells, f"C{k}_bnd": max(1, mesh.n_bnd_faces),
+                                f"grad_C{k}": 3 * mesh.n_cells})
 
     def _check(self, rc, what):
         if rc != 0:
