@@ -172,3 +172,32 @@ def most_channel(nx=16, ny=12, nz=8, dT=1.0, z0_m=1.0e-3):
     p.temperature_scheme = "VANLEER_TVD"
     c.extra["t_noise"] = 1.0e-2
     return c
+
+
+def full_stack(nx=250, ny=210, nz=160, n_obstacles=40, seed=2024, lx=1000.0, ly=1000.0, lz=200.0):
+    """The full-stack workload: a 1000 x 1000 x 200 m box, periodic sides, slip top, with seeded
+    cuboid obstacles (heights 20-120 m) whose cells are removed; MOST rough walls (z0 = 0.1 m) on
+    the ground and the obstacles over a surface 2 K warmer than the air (unstable; the prescribed
+    surface heat flux of the workload description is not a reference feature, the surface
+    temperature drives the flux instead; with much stronger heating at this resolution the
+    reference's 20-step relaxed MOST Newton iteration stops unconverged on some faces and its
+    result turns on rounding), AMD SGS, Boussinesq temperature and two passive scalars
+    by van Leer TVD, Coriolis f = 1e-4 (Omega_z = 5e-5), a constant pressure gradient along x.
+    Scalar 0 is emitted by the ground (flux 1e-3) and vanishes at the top; scalar 1 is held at 1
+    on the obstacles."""
+    obs = M.seeded_obstacles(n_obstacles, lx, ly, 20.0, 120.0, seed)
+    nu = 1.5e-5
+    params = SolverParams(
+        nu=nu, f_const=(2.0e-3, 0.0, 0.0), advection_scheme="KEP", use_sgs=True, sgs_model="AMD",
+        wall_model="most", z0_m=0.1, T_surface=302.0, boussinesq=True, alpha_T=nu / 0.7, beta=1.0 / 300.0,
+        T_ref=300.0, g=(0.0, 0.0, -9.81), temperature_scheme="VANLEER_TVD", use_rotation=True,
+        omega_rot=(0.0, 0.0, 5.0e-5), n_scalars=2, scalar_scheme="VANLEER_TVD",
+        scalar_diffusivity=(nu / 0.7, nu / 0.7, 0.0, 0.0), scalar_pr_t=(0.85, 0.85, 1.0, 1.0),
+        patch_scalar={"z0": [{"mode": "neumann", "flux": 1.0e-3}],
+                      "zL": [{"mode": "dirichlet", "value": 0.0}],
+                      "obstacles": [{}, {"mode": "dirichlet", "value": 1.0}]},
+        dt=0.2, cfl_max=0.5, p_tol=1.0e-8, p_max_iter=1000, n_nonortho_corr=0, rhie_chow_dual=False,
+        cell_projection="LS", pressure_min_cos_theta=0.0)
+    return Case(f"full_stack{nx}x{ny}x{nz}",
+                lambda: M.Mesh(M.obstacle_box_elements(nx, ny, nz, lx, ly, lz, obs)),
+                params, ic="uniform", ic_uref=5.0, nsteps=300, extra={"t_noise": 0.1, "seed": 1700})

@@ -623,3 +623,86 @@ def write_foam(mesh, polymesh_dir, binary=False, label_bytes=4, scalar_bytes=8, 
     with open(os.path.join(polymesh_dir, "boundary"), "wb") as f:
         f.write(header("polyBoundaryMesh", "boundary", "ascii"))
         f.write(body.encode())
+
+
+def obstacle_box_elements(nx, ny, nz, lx, ly, lz, obstacles, group_names=("x0", "xL", "y0", "yL", "z0", "zL"),
+                          group_types=("periodic", "periodic", "periodic", "periodic", "wallModel", "slip"),
+                          obstacle_name="obstacles", obstacle_type="wallModel"):
+    """Uniform nx*ny*nz hex box with cuboid obstacles: every cell whose centre lies inside an
+    obstacle ((x0, x1, y0, y1, z0, z1) in coordinates) is removed, and the faces between kept and
+    removed cells form one more patch, `obstacle_name`. Node numbering and the order of the kept
+    hexes and boundary quads follow box_elements; boundary quads of removed cells are dropped."""
+    def xyz(I, J, K):
+        return I * (lx / nx), J * (ly / ny), K * (lz / nz)
+    e = box_elements(nx, ny, nz, xyz, group_names, group_types)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.reshape(-1), j.reshape(-1), k.reshape(-1)
+    cx, cy, cz = (i + 0.5) * lx / nx, (j + 0.5) * ly / ny, (k + 0.5) * lz / nz
+    removed = np.zeros(nx * ny * nz, dtype=bool)
+    for (x0, x1, y0, y1, z0, z1) in obstacles:
+        removed |= (cx > x0) & (cx < x1) & (cy > y0) & (cy < y1) & (cz > z0) & (cz < z1)
+    if removed.all():
+        raise ValueError("the obstacles remove every cell")
+    keep = ~removed
+    hexes = e["vol_nodes"].reshape(-1, 8)
+
+    def cid(ii, jj, kk):
+        return ii + nx * (jj + ny * kk)
+
+    # outer quads in box_elements' order, each tagged with its cell
+    cells = [
+        (np.zeros(ny * nz, int), *[a.reshape(-1) for a in np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")]),
+        (np.full(ny * nz, nx - 1), *[a.reshape(-1) for a in np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")]),
+    ]
+    a, b = [q.reshape(-1) for q in np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")]
+    cells += [(a, np.zeros_like(a), b), (a, np.full_like(a, ny - 1), b)]
+    a, b = [q.reshape(-1) for q in np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")]
+    cells += [(a, b, np.zeros_like(a)), (a, b, np.full_like(a, nz - 1))]
+    owner = np.concatenate([cid(ci, cj, ck) for ci, cj, ck in cells])
+    quads = e["bnd_nodes"].reshape(-1, 4)
+    ok = keep[owner]
+    bnd = [quads[ok]]
+    grp = [e["bnd_group"][ok]]
+    # faces between a kept cell and a removed neighbour: direction by direction (-x +x -y +y -z
+    # +z), kept cells ascending within each
+    corner = {(-1, 0, 0): (0, 3, 7, 4), (1, 0, 0): (1, 2, 6, 5), (0, -1, 0): (0, 1, 5, 4), (0, 1, 0): (3, 2, 6, 7),
+              (0, 0, -1): (0, 1, 2, 3), (0, 0, 1): (4, 5, 6, 7)}
+    g_obst = len(group_names)
+    for (di, dj, dk), loc in corner.items():
+        ni, nj, nk = i + di, j + dj, k + dk
+        inb = (ni >= 0) & (ni < nx) & (nj >= 0) & (nj < ny) & (nk >= 0) & (nk < nz)
+        sel = np.zeros_like(keep)
+        sel[inb] = keep[inb] & removed[cid(ni[inb], nj[inb], nk[inb])]
+        if sel.any():
+            bnd.append(hexes[sel][:, list(loc)].astype(np.int32))
+            grp.append(np.full(int(sel.sum()), g_obst, dtype=np.int32))
+    bnd = np.concatenate(bnd).astype(np.int32)
+    grp = np.concatenate(grp).astype(np.int32)
+    kept = hexes[keep]
+    nv, nq = len(kept), len(bnd)
+    return {
+        "coords": e["coords"],
+        "vol_type": np.full(nv, GMSH_HEX8, dtype=np.int32),
+        "vol_offset": np.arange(nv + 1, dtype=np.int64) * 8,
+        "vol_nodes": kept.reshape(-1),
+        "bnd_type": np.full(nq, GMSH_QUAD4, dtype=np.int32),
+        "bnd_offset": np.arange(nq + 1, dtype=np.int64) * 4,
+        "bnd_nodes": bnd.reshape(-1),
+        "bnd_group": grp,
+        "group_bc": np.array([BC_NAMES[t.lower()] for t in (*group_types, obstacle_type)], dtype=np.int32),
+        "group_names": [*group_names, obstacle_name],
+    }
+
+
+def seeded_obstacles(n, lx, ly, hmin, hmax, seed, wmin=0.04, wmax=0.1, margin=0.05):
+    """n cuboids on the ground: footprint sides uniform in [wmin, wmax] of the box's horizontal
+    size, heights uniform in [hmin, hmax], positions uniform inside the box at least `margin` of
+    its size from the sides, so that periodic side patches keep matching faces (seeded)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        wx, wy = rng.uniform(wmin, wmax) * lx, rng.uniform(wmin, wmax) * ly
+        x0 = rng.uniform(margin * lx, (1.0 - margin) * lx - wx)
+        y0 = rng.uniform(margin * ly, (1.0 - margin) * ly - wy)
+        out.append((x0, x0 + wx, y0, y0 + wy, -1.0, rng.uniform(hmin, hmax)))
+    return out

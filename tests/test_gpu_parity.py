@@ -319,3 +319,16 @@ def test_gpu_passive_scalars_match_oracle():
     d = [rel_l2(o.scalar(k), g.scalar(k)) for k in range(2)]
     print(f"[gpu] passive scalars rel-L2 {d}, U {rel_l2(o.U, g.U):.2e}", flush=True)
     assert max(d) < 1e-10 and rel_l2(o.U, g.U) < 1e-10
+
+
+def test_gpu_full_stack_matches_oracle():
+    """The full-stack workload at 16 x 16 x 8 with three obstacles (MOST, AMD, temperature, two
+    passive scalars, Coriolis) against the oracle after four steps."""
+    case = cases.full_stack(16, 16, 8, n_obstacles=3, seed=5)
+    case.params.p_tol = 1e-12
+    mesh = case.make_mesh()
+    print(f"[gpu] full stack {mesh.n_cells} cells", flush=True)
+    g, o = _pair(case, mesh, 4)
+    d = [rel_l2(o.U, g.U), rel_l2(o.p, g.p)] + [rel_l2(o.scalar(k), g.scalar(k)) for k in range(2)]
+    print(f"[gpu] full stack rel-L2 U, p, C0, C1: {d}", flush=True)
+    assert max(d) < 1e-10

@@ -114,3 +114,36 @@ def test_channel_box_stretching_and_wall_patches():
     names = m.patch_names
     assert names[2] == "bottom_wall" and names[3] == "top_wall"
     assert math.isclose(m.cell_volume.sum(), 2 * math.pi * 2.0 * math.pi, rel_tol=1e-12)
+
+
+def test_obstacle_box_removes_cells_and_walls_them():
+    """Cells whose centres lie inside the cuboids are removed; the faces between kept and removed
+    cells form the obstacle patch; every cell stays closed and the volume is the kept cells'."""
+    nx, ny, nz, L, H = 16, 16, 8, 1000.0, 200.0
+    obs = M.seeded_obstacles(3, L, L, 20.0, 120.0, seed=5, wmin=0.1, wmax=0.2)
+    m = M.Mesh(M.obstacle_box_elements(nx, ny, nz, L, L, H, obs))
+    h = np.array([L / nx, L / ny, H / nz])
+    ctr = (np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3) + 0.5) * h
+    inside = np.zeros(len(ctr), bool)
+    for (x0, x1, y0, y1, z0, z1) in obs:
+        inside |= ((ctr[:, 0] > x0) & (ctr[:, 0] < x1) & (ctr[:, 1] > y0) & (ctr[:, 1] < y1) & (ctr[:, 2] > z0)
+                   & (ctr[:, 2] < z1))
+    assert m.n_cells == nx * ny * nz - inside.sum() and inside.sum() > 0
+    np.testing.assert_allclose(m.cell_volume, np.prod(h), rtol=1e-13)
+    assert np.abs(_cell_sums(m)).max() < 1e-9
+    assert m.patch_names[-1] == "obstacles" and m.patch_bc[-1] == M.BC_WALL_MODEL
+    # ground faces lost under the obstacles' footprints
+    p = m.patch_names.index("z0")
+    footprint = inside.reshape(nx, ny, nz)[:, :, 0].sum()
+    assert m.patch_nfaces[p] == nx * ny - footprint
+    # obstacle face area equals the exposed surface of the removed block set
+    rem = inside.reshape(nx, ny, nz)
+    exposed = 0.0
+    for ax, area in ((0, h[1] * h[2]), (1, h[0] * h[2]), (2, h[0] * h[1])):
+        d = np.diff(rem.astype(int), axis=ax)
+        exposed += np.abs(d).sum() * area
+    # faces on the domain boundary of removed cells are not obstacle faces (top of the domain
+    # never reached; the ground under an obstacle is removed with it)
+    q = m.patch_names.index("obstacles")
+    s = slice(m.n_int_faces + m.patch_start[q], m.n_int_faces + m.patch_start[q] + m.patch_nfaces[q])
+    assert m.face_area[s].sum() == pytest.approx(exposed, rel=1e-12)

@@ -125,8 +125,8 @@ def test_scalar_boundary_values_follow_the_modes():
 
 
 def scalar_channel():
-    """Small channel with two passive scalars: a Dirichlet/Neumann pair on the walls (one
-    oscillating) under KEP + Vreman + log-law walls; the second scalar with QUICK."""
+    """Small channel with two passive scalars by van Leer TVD: Dirichlet walls for the first,
+    Neumann walls (one flux oscillating) for the second, under KEP + Vreman + log-law walls."""
     c = cases.small_channel(12, 10, 8)
     p = c.params
     p.p_tol = 1e-12
@@ -154,3 +154,16 @@ def test_host_scalars_match_oracle():
     for k in range(2):
         assert rel_l2(o.scalar(k), h.scalar(k)) < 1e-10, k
     assert rel_l2(o.U, h.U) < 1e-10
+
+
+def test_host_full_stack_matches_oracle():
+    """The full-stack workload at 16 x 16 x 8 with three obstacles: MOST walls on the ground and
+    the obstacles, AMD, temperature, two scalars, Coriolis."""
+    case = cases.full_stack(16, 16, 8, n_obstacles=3, seed=5)
+    case.params.p_tol = 1e-12
+    mesh = case.make_mesh()
+    h, o = HostSolver(mesh, case.params), oracle.Solver(mesh, case.params)
+    for s in (h, o):
+        run_case(s, case, mesh, nsteps=4, stats_every=4)
+    assert rel_l2(o.U, h.U) < 1e-10 and rel_l2(o.p, h.p) < 1e-10
+    assert max(rel_l2(o.scalar(k), h.scalar(k)) for k in range(2)) < 1e-10
