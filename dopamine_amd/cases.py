@@ -123,16 +123,21 @@ def small_channel(nx=16, ny=12, nz=8):
     return c
 
 
-def cavity_tets(n=55, jitter=0.15, seed=3200, shuffle_seed=1):
+def cavity_tets(n=55, jitter=0.15, seed=3200, shuffle_seed=1, rcm=False):
     """Lid-driven cavity at Re 3200 on 6 n^3 jittered, shuffled tetrahedra (unit box, lid z = 1
     moving at u = 1, other walls no-slip), Koren TVD advection, AMD SGS (c = 1/3), automatic
-    non-orthogonal correctors."""
+    non-orthogonal correctors; with rcm the shuffled cells are renumbered in reverse Cuthill-McKee
+    order before the final build."""
     names = ("x0", "xL", "y0", "yL", "z0", "lid")
     types = ("noSlip",) * 5 + ("movingWall",)
+
+    def make():
+        e = M.tet_box_elements(n, jitter=jitter, seed=seed, shuffle_seed=shuffle_seed, group_types=types,
+                               group_names=names)
+        return M.Mesh(M.rcm_elements(e) if rcm else e)
     return Case(
-        f"cavity_tets{n}",
-        lambda: M.tet_box(n, jitter=jitter, seed=seed, shuffle_seed=shuffle_seed, group_types=types,
-                          group_names=names),
+        f"cavity_tets{n}{'_rcm' if rcm else ''}",
+        make,
         SolverParams(nu=1.0 / 3200.0, advection_scheme="KOREN_TVD", use_sgs=True, sgs_model="AMD",
                      wall_model="none", dt=2.0e-3, cfl_max=0.5, p_tol=1.0e-8, p_max_iter=1000,
                      n_nonortho_corr=0, rhie_chow_dual=True, cell_projection="LS",

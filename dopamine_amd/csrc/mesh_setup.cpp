@@ -822,6 +822,74 @@ void* dm_mesh_build_poly(const dm_poly_input* in) {
   }
 }
 
+// Reverse Cuthill-McKee order of the cells over the face graph (interior faces and periodic
+// pairs): breadth-first from a pseudo-peripheral cell (the last cell reached from the lowest-
+// degree one), neighbours taken by increasing degree then index, components one after another,
+// the whole order reversed. order[new] = old.
+int dm_mesh_rcm_order(const void* mesh, int32_t* order) {
+  g_err.clear();
+  try {
+    const Mesh& m = *static_cast<const Mesh*>(mesh);
+    const int nc = m.nc;
+    std::vector<std::vector<int>> adj(nc);
+    for (int f = 0; f < m.nint; ++f) {
+      adj[m.face_owner[f]].push_back(m.face_neighbour[f]);
+      adj[m.face_neighbour[f]].push_back(m.face_owner[f]);
+    }
+    for (int b = 0; b < m.nbnd; ++b) {
+      const int q = m.periodic_pair[b];
+      if (q > b) {
+        const int a = m.face_owner[m.nint + b], c = m.face_owner[m.nint + q];
+        if (a != c) {
+          adj[a].push_back(c);
+          adj[c].push_back(a);
+        }
+      }
+    }
+    for (auto& l : adj) {
+      std::sort(l.begin(), l.end());
+      l.erase(std::unique(l.begin(), l.end()), l.end());
+    }
+    for (auto& l : adj)
+      std::stable_sort(l.begin(), l.end(), [&](int a, int c) { return adj[a].size() < adj[c].size(); });
+    std::vector<char> seen(nc, 0);
+    std::vector<int> out;
+    out.reserve(nc);
+    auto bfs = [&](int start, std::vector<int>& visit, std::vector<char>& mark) {
+      size_t head = visit.size();
+      visit.push_back(start);
+      mark[start] = 1;
+      while (head < visit.size()) {
+        const int c = visit[head++];
+        for (int nb : adj[c])
+          if (!mark[nb]) {
+            mark[nb] = 1;
+            visit.push_back(nb);
+          }
+      }
+    };
+    for (int c0 = 0; c0 < nc; ++c0) {
+      if (seen[c0]) continue;
+      // lowest-degree cell of this component, then the last cell a search from it reaches
+      std::vector<int> comp;
+      std::vector<char> mark(seen);
+      bfs(c0, comp, mark);
+      int lo = comp[0];
+      for (int c : comp)
+        if (adj[c].size() < adj[lo].size()) lo = c;
+      std::vector<int> probe;
+      std::vector<char> mark2(seen);
+      bfs(lo, probe, mark2);
+      bfs(probe.back(), out, seen);
+    }
+    for (int i = 0; i < nc; ++i) order[i] = out[nc - 1 - i];
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 int dm_mesh_get_view(const void* mesh, dm_mesh_view* v) {
   const Mesh& m = *static_cast<const Mesh*>(mesh);
   std::memset(v, 0, sizeof(*v));

@@ -706,3 +706,42 @@ def seeded_obstacles(n, lx, ly, hmin, hmax, seed, wmin=0.04, wmax=0.1, margin=0.
         y0 = rng.uniform(margin * ly, (1.0 - margin) * ly - wy)
         out.append((x0, x0 + wx, y0, y0 + wy, -1.0, rng.uniform(hmin, hmax)))
     return out
+
+
+def rcm_elements(elements):
+    """The same element lists with the volume elements in reverse Cuthill-McKee order of the
+    cells' face graph (dm_mesh_rcm_order), for meshes whose cell numbering scatters neighbours
+    (shuffled tets): the rebuilt mesh keeps its geometry and faces, with neighbours close in
+    memory. Boundary elements are unchanged."""
+    L = lib()
+    L.dm_mesh_rcm_order.argtypes = [ctypes.c_void_p, _P(_i32)]
+    L.dm_mesh_build.restype = ctypes.c_void_p
+    L.dm_mesh_build.argtypes = [ctypes.POINTER(_Input)]
+    L.dm_mesh_free.argtypes = [ctypes.c_void_p]
+    L.dm_last_error.restype = ctypes.c_char_p
+    e = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in elements.items()}
+    inp = _Input(
+        n_nodes=len(e["coords"]) // 3, coords=_ptr(e["coords"], _d),
+        n_vol=len(e["vol_type"]), vol_type=_ptr(e["vol_type"], _i32),
+        vol_offset=_ptr(e["vol_offset"], _i64), vol_nodes=_ptr(e["vol_nodes"], _i32),
+        n_bnd=len(e["bnd_type"]), bnd_type=_ptr(e["bnd_type"], _i32),
+        bnd_offset=_ptr(e["bnd_offset"], _i64), bnd_nodes=_ptr(e["bnd_nodes"], _i32),
+        bnd_group=_ptr(e["bnd_group"], _i32), n_groups=len(e["group_bc"]), group_bc=_ptr(e["group_bc"], _i32))
+    h = L.dm_mesh_build(ctypes.byref(inp))
+    if not h:
+        raise RuntimeError("mesh build failed: " + L.dm_last_error().decode())
+    try:
+        order = np.zeros(len(e["vol_type"]), dtype=np.int32)
+        if L.dm_mesh_rcm_order(h, _ptr(order, _i32)) != 0:
+            raise RuntimeError("RCM ordering failed: " + L.dm_last_error().decode())
+    finally:
+        L.dm_mesh_free(h)
+    off = e["vol_offset"]
+    cnt = np.diff(off)[order]
+    new_off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    idx = np.concatenate([np.arange(off[c], off[c + 1]) for c in order]) if len(order) else np.zeros(0, np.int64)
+    out = dict(elements)
+    out["vol_type"] = e["vol_type"][order]
+    out["vol_offset"] = new_off
+    out["vol_nodes"] = e["vol_nodes"][idx]
+    return out

@@ -147,3 +147,19 @@ def test_obstacle_box_removes_cells_and_walls_them():
     q = m.patch_names.index("obstacles")
     s = slice(m.n_int_faces + m.patch_start[q], m.n_int_faces + m.patch_start[q] + m.patch_nfaces[q])
     assert m.face_area[s].sum() == pytest.approx(exposed, rel=1e-12)
+
+
+def test_rcm_renumbering_is_a_permutation_with_a_narrow_band():
+    e = M.tet_box_elements(6, jitter=0.15, shuffle_seed=1)
+    a, b = M.Mesh(e), M.Mesh(M.rcm_elements(e))
+    assert a.n_cells == b.n_cells and a.n_faces == b.n_faces
+    # same cells: match by centroid
+    ka = np.lexsort(a.cell_centroid.T)
+    kb = np.lexsort(b.cell_centroid.T)
+    np.testing.assert_array_equal(a.cell_centroid[ka], b.cell_centroid[kb])
+    np.testing.assert_array_equal(a.cell_volume[ka], b.cell_volume[kb])
+
+    def band(m):
+        return np.abs(m.face_neighbour[:m.n_int_faces].astype(np.int64) - m.face_owner[:m.n_int_faces])
+    assert band(b).max() * 10 < band(a).max()
+    assert b.patch_names == a.patch_names and (b.patch_nfaces == a.patch_nfaces).all()
