@@ -103,8 +103,9 @@ typedef struct dm_mesh_view {
 
 /* Reverse Cuthill-McKee order of a built mesh's cells over its face graph (interior faces and
  * periodic pairs), order[new] = old, for renumbering the cells of shuffled meshes before the
- * final build (no counterpart in the reference's single-rank path; its partitioner orders cells
- * for locality, src/mesh/mod_partitioner.f90). Returns 0, or -1 with dm_last_error(). */
+ * final build. No counterpart in the reference, whose multi-rank partitioner orders cells along a
+ * Morton curve instead (build_morton_partition, src/mesh/mod_partitioner.f90:39). Returns 0, or
+ * -1 with dm_last_error(). */
 int dm_mesh_rcm_order(const void* mesh, int32_t* order);
 
 /* Build a mesh. Returns an opaque handle, or NULL with a message available from dm_last_error(). */
