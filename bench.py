@@ -15,6 +15,9 @@ steps / wall time of the timed steps. The roofline entry
 is for the dominant kernel (the CG iteration pass), from HIP events that its dispatches stamp on
 the solver's stream over a further run of the same number of steps; the CPU baseline is the FP64
 C restatement of the same algorithm run single-threaded on one step of the same workload.
+`--workload tets` measures the unstructured-gather workload instead: the lid-driven cavity on
+998,250 jittered, shuffled tets renumbered by reverse Cuthill-McKee (Koren TVD, AMD SGS,
+non-orthogonal correction).
 """
 import argparse
 import json
@@ -33,6 +36,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("channel", "tets"), default="channel",
+                    help="channel: the turbulent channel (the line's default workload); tets: the lid-driven "
+                         "cavity on 6 x 55^3 jittered, shuffled tets renumbered by reverse Cuthill-McKee")
     ap.add_argument("--mesh", type=int, nargs=3, default=(128, 96, 64), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed",
                     help="mixed: FP32 CG sweeps inside FP64 iterative refinement (the default measured path); "
@@ -95,7 +101,7 @@ def cpu_baseline(mesh, case, U, p, steps):
     dt = time.perf_counter() - t0
     its = o.counters()["iterations"]
     return {"value": mesh.n_cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} RK3 step(s) of the same {mesh.n_cells}-cell channel from the GPU state after the "
+            "sample": f"{steps} RK3 step(s) of the same {mesh.n_cells}-cell workload from the GPU state after the "
                       f"timed steps; FP64 C restatement, one thread; {its} CG iterations; {dt:.1f} s"}
 
 
@@ -110,7 +116,8 @@ def main():
     from dopamine_amd.solver import FVSolver
 
     nx, ny, nz = args.mesh
-    case = cases.channel(nx, ny, nz)
+    tets = args.workload == "tets"
+    case = cases.cavity_tets(55, rcm=True) if tets else cases.channel(nx, ny, nz)
     case.params.cg_variant = args.cg_variant
     learned = args.precond == "learned"
     if learned:
@@ -119,7 +126,9 @@ def main():
         case.params.learn_iters = args.learn_iters
     t0 = time.perf_counter()
     mesh = case.make_mesh()
-    log(f"mesh {nx}x{ny}x{nz}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
+    log(f"mesh {case.name}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
+    width = 4 if mesh.max_cell_faces <= 4 else 6 if mesh.max_cell_faces <= 6 else 8 if mesh.max_cell_faces <= 8 else 0
+    wl_key = "tets 6x55^3 rcm" if tets else f"channel {nx}x{ny}x{nz}"
 
     def timed_run(params, precision, tag):
         solver = FVSolver(mesh, params, precision=precision)
@@ -196,9 +205,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if mixed else ("f64+bf16" if learned else "f64"),
-        "data": "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh",
-        "config": {"workload": f"turbulent channel {nx}x{ny}x{nz} ({nc} hex cells, {nfp} pressure faces), "
-                               "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, "
+        "data": ("synthetic: fluid at rest, moving lid, on a generated seeded jittered mesh" if tets else
+                 "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh"),
+        "config": {"workload": (f"lid-driven cavity Re 3200 on {nc} jittered shuffled tets renumbered by reverse "
+                                f"Cuthill-McKee ({nfp} pressure faces), Koren TVD + AMD, Jasak non-orthogonal "
+                                "correction, Jacobi CG to 1e-8, " if tets else
+                                f"turbulent channel {nx}x{ny}x{nz} ({nc} hex cells, {nfp} pressure faces), "
+                                "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, ")
                                + ("FP32 CG sweeps in FP64 iterative refinement" if mixed else "plain FP64")
                                + (", flexible CG with the learned message-passing preconditioner (Jacobi CG "
                                   "fallback)" if learned else ""),
@@ -206,11 +219,11 @@ def main():
                    "dt": case.params.dt},
         "roofline": {"bound": "hbm",
                      "kernel": ("learned preconditioner hidden layer k_tile<1,mp::L1<6>> (MFMA bf16)" if timed_learned else
-                                "CG FP32 sweep iteration k_pass<6,6,cgk::Fused32>" if mixed else
-                                "CG iteration pass k_pass<6,6,cgk::Fused>" if args.cg_variant == 1
+                                f"CG FP32 sweep iteration k_pass<6,6,cgk::Fused32T<{width}>>" if mixed else
+                                f"CG iteration pass k_pass<6,6,cgk::FusedT<{width}>>" if args.cg_variant == 1
                                 else "CG direction/operator pass k_pass<4,1,cgk::Dir>"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic("Fused32T<6>" if mixed else "FusedT<6>", f"channel {nx}x{ny}x{nz}")
+                     "traffic": (measured_traffic(f"Fused32T<{width}>" if mixed else f"FusedT<{width}>", wl_key)
                                  if (not timed_learned and args.cg_variant == 1) else None),
                      "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
                      "launches": n_dir},
