@@ -2783,6 +2783,10 @@ int32_t *om_int_array(om_solver *s, const char *name, int64_t *len) {
   return NULL;
 }
 
+void om_kfactor_limited(const double sf[3], const double d_on[3], double min_cos, double k[3]) {
+  kfactor_limited(sf, d_on, min_cos, k);
+}
+
 double om_time(const om_solver *s) { return s->time; }
 
 void om_proj_counters(const om_solver *s, int64_t out[2]) {

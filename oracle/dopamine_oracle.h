@@ -140,6 +140,9 @@ void om_proj_adjoint(om_solver *s, const double *chi, double *v);
 int om_proj_project(om_solver *s, double *u, const double *bconst, int precond, double tol, int max_it);
 /* {projections, total projection iterations} of the time steps so far. */
 void om_proj_counters(const om_solver *s, int64_t out[2]);
+/* Over-relaxed non-orthogonal k-vector with the cos(theta) floor min_cos (nonortho_kfactor_limited,
+ * src/numerics/mod_nonortho.f90:65-98): k = Sf - (|Sf|^2 / max(Sf.d, min_cos |Sf| |d|)) d. */
+void om_kfactor_limited(const double sf[3], const double d_on[3], double min_cos, double k[3]);
 /* Businger-Dyer stability corrections psi_m, psi_h (src/abl/mod_abl_profiles.f90:51-85). */
 double om_psi_m(double zeta);
 double om_psi_h(double zeta);

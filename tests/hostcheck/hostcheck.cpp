@@ -81,6 +81,9 @@ void hc_solver_counters(void* h, int64_t out[4]) {
   auto* s = static_cast<Solver*>(h);
   out[0] = s->n_solves; out[1] = s->n_iters; out[2] = s->last_iters; out[3] = s->n_sweeps;
 }
+void hc_kfactor_limited(const double sf[3], const double d[3], double min_cos, double k[3]) {
+  dm::kfactor_limited(sf, d, min_cos, k);
+}
 void hc_solver_proj_counters(void* h, int64_t out[2]) {
   out[0] = static_cast<Solver*>(h)->n_proj;
   out[1] = static_cast<Solver*>(h)->n_proj_its;
