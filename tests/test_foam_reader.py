@@ -352,3 +352,32 @@ def test_owner_not_below_neighbour_is_rejected(tmp_path):
 def test_missing_directory_is_an_error(tmp_path):
     with pytest.raises(ValueError, match="cannot open"):
         M.read_foam_input(str(tmp_path / "none"))
+
+
+def _l_prism(start=0, h=0.5):
+    """One polyhedral cell: the L-shaped hexagon of the reference's mesh-geometry test extruded by
+    h; the bottom face's point list starts at vertex `start`."""
+    base = [(0, 0), (2, 0), (2, 1), (1, 1), (1, 2), (0, 2)]
+    pts = np.array([(x, y, 0.0) for x, y in base] + [(x, y, h) for x, y in base], dtype=float)
+    bottom = [(start - k) % 6 for k in range(6)]          # clockwise from below: normal -z
+    top = [6 + k for k in range(6)]                        # normal +z
+    sides = [[k, (k + 1) % 6, 6 + (k + 1) % 6, 6 + k] for k in range(6)]
+    faces = [bottom, top] + sides
+    off = np.concatenate([[0], np.cumsum([len(f) for f in faces])])
+    return {"points": pts, "face_offset": off, "face_nodes": np.concatenate(faces), "owner": np.zeros(len(faces), int),
+            "neighbour": np.zeros(0, int), "patch_start": np.array([0]), "patch_nfaces": np.array([len(faces)]),
+            "patch_bc": np.array([M.BC_WALL_NOSLIP]), "patch_names": ["walls"]}
+
+
+@pytest.mark.parametrize("start", range(6))
+def test_concave_face_geometry_known_answers(start):
+    """The reference's mesh-geometry test (tests/unit/test_mesh_geometry.f90:22-41): the L-shaped
+    face has area 3 and centroid (5/6, 5/6) whatever vertex the fan starts from; the extruded cell
+    closes with volume 3 h and centroid (5/6, 5/6, h/2)."""
+    h = 0.5
+    m = M.Mesh(_l_prism(start, h))
+    assert m.face_area[0] == pytest.approx(3.0, abs=1e-12)
+    np.testing.assert_allclose(m.face_centre[0], [5 / 6, 5 / 6, 0.0], atol=1e-12)
+    np.testing.assert_allclose(m.face_normal[0], [0.0, 0.0, -1.0], atol=1e-12)
+    assert m.cell_volume[0] == pytest.approx(3.0 * h, rel=1e-12)
+    np.testing.assert_allclose(m.cell_centroid[0], [5 / 6, 5 / 6, h / 2], atol=1e-12)
