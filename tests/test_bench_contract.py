@@ -1,0 +1,33 @@
+"""bench.py's command-line contract on a machine without a GPU: more than one GPU is refused with
+a JSON message, and the algorithmic byte model and traffic lookup read what the line reports."""
+import json
+import os
+import subprocess
+import sys
+
+import bench
+from dopamine_amd import mesh as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_byte_model_and_committed_traffic():
+    m = M.channel_box(8, 6, 4)
+    nc = m.n_cells
+    b_mixed, nfp = bench.cg_pass_bytes(m, 1, True)
+    assert b_mixed == 44 * nc + 12 * nfp
+    assert bench.cg_pass_bytes(m, 1, False)[0] == 88 * nc + 16 * nfp
+    assert bench.cg_pass_bytes(m, 0, False)[0] == 40 * nc + 16 * nfp
+    # interior faces plus one per periodic pair
+    assert nfp == m.n_int_faces + int((m.periodic_pair >= 0).sum()) // 2
+    # the measured traffic of the benchmark kernels is committed under profiles/
+    assert bench.measured_traffic("Fused32T<6>", "channel 128x96x64") > 62_000_000
+    assert bench.measured_traffic("FusedT<6>", "channel 128x96x64") > 106_000_000
+    assert bench.measured_traffic("FusedT<6>", "no such workload") is None
