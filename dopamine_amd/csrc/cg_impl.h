@@ -828,6 +828,32 @@ struct FcgUpd {
 };
 }  // namespace cgk
 
+// The reference's handling of a pressure solve that stops at its iteration limit
+// (src/solver/pressure_backend_petsc.c:878-937): the solve is repeated once from x = 0 with four
+// times the limit (the reference switches its preconditioner to block Jacobi for that retry; here
+// the retry is the Jacobi-preconditioned CG); if the retry stops at its limit too, the result is
+// accepted when the true residual norm is below 1e4 rtol or 1e3 atol (absolute comparisons, as
+// the reference makes them), otherwise the solve fails with -(1000 + KSP_DIVERGED_ITS).
+// first(its, resid) and retry(maxit, its, resid) return 0 when converged; zero_x() sets x = 0;
+// atol() is the absolute tolerance the solves used. *its is the last solve's iteration count (as
+// the reference reports it), *total_its the count over both solves.
+constexpr int KSP_DIVERGED_ITS = -3;
+template <class First, class Retry, class ZeroX, class Atol>
+int solve_with_retry(int maxit, double rtol, First&& first, Retry&& retry, ZeroX&& zero_x, Atol&& atol, int* its,
+                     double* resid, int* total_its, int* retried) {
+  *retried = 0;
+  const int rc1 = first(its, resid);
+  *total_its = *its;
+  if (rc1 == 0) return 0;
+  *retried = 1;
+  zero_x();
+  const int rc2 = retry(4 * maxit, its, resid);
+  *total_its += *its;
+  if (rc2 == 0) return 0;
+  if (*resid < rtol * 1.0e4 || *resid < atol() * 1.0e3) return 0;
+  return -(1000 + KSP_DIVERGED_ITS);
+}
+
 template <class X>
 class CgT {
  public:
@@ -1021,6 +1047,7 @@ class CgT {
     }
     last_its_ = h.its;
     last_ttol_ = h.ttol;
+    last_atol_ = h.atol;
     (fcg_ ? predict_fcg_ : predict_) = std::max(2, h.its + 1);
     if (its) *its = h.its;
     if (resid) *resid = std::sqrt(rr);
@@ -1088,6 +1115,7 @@ class CgT {
  public:
   int last_its() const { return last_its_; }
   double last_ttol() const { return last_ttol_; }
+  double last_atol() const { return last_atol_; }
 
  private:
   static constexpr int PARTS = 17;  // partial quantities per block: P0 1, P1 2, D 1, R 1, U 3 x 4
@@ -1106,7 +1134,7 @@ class CgT {
   CgScalars* sc_ = nullptr;
   std::vector<double> hpart_;
   int last_its_ = 0, predict_ = 24;
-  double last_ttol_ = 0.0;
+  double last_ttol_ = 0.0, last_atol_ = 0.0;
   int predict_fcg_ = 24;
 };
 

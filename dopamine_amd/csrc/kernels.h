@@ -1743,6 +1743,17 @@ struct Momentum {
   }
 };
 
+// sum of U1 + U2 + U3 + p over the cells; a NaN or Inf anywhere makes it non-finite
+// (solver_state_is_finite, src/solver/mod_fv_solver.f90:3925-3945)
+struct FiniteSum {
+  const double *U, *p;
+  DM_HD Red<1> operator()(int c) const {
+    Red<1> r;
+    r.v[0] = U[3 * size_t(c)] + U[3 * size_t(c) + 1] + U[3 * size_t(c) + 2] + p[c];
+    return r;
+  }
+};
+
 struct BulkSpeed {
   DevMesh M;
   const double* U;

@@ -43,6 +43,14 @@ struct Ctx {
 };
 std::unique_ptr<Ctx> g_ctx;
 
+// Selectors: the reference's names ("boomeramg", its default, "gamg", "gmres") and "jacobi" run the
+// Jacobi-preconditioned CG; "mixed" runs FP32 sweeps inside FP64 refinement; "fcg-learned" the
+// learned preconditioner in flexible CG. Any other name fails the solve with -11 instead of
+// silently running something else.
+bool known_selector(const std::string& s) {
+  return s == "boomeramg" || s == "gamg" || s == "gmres" || s == "jacobi" || s == "mixed" || s == "fcg-learned";
+}
+
 int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner, const int* nb_local,
               const int* nb_global, const int* skip, const double* diagonal, const double* coeff, int ref_row) {
   if (n_global != n) return -46;  // one rank: every row is local
@@ -184,6 +192,7 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
   (void)dirichlet_values;
   try {
     if (n_cells_local <= 0) return -10;
+    if (!known_selector(g_solver)) return -11;
     int ref_row = reference_cell - 1;
     if (ref_row < 0 || ref_row >= n_cells_global) ref_row = 0;
     if (!g_ctx || g_ctx->n_local != n_cells_local || g_ctx->n_global != n_cells_global || g_ctx->n_faces != n_faces ||
@@ -207,28 +216,33 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
     int maxit = max_iterations > 2000 ? max_iterations : 2000;
     bool single = false;
     for (const auto& o : g_options) {
-      if (o.rfind("-ksp_max_it", 0) == 0) {
+      if (o.rfind("-ksp_max_it", 0) == 0) {  // a later "-ksp_max_it 0" restores the default
         const int v = std::atoi(o.c_str() + 11);
-        if (v > 0) maxit = v;
+        maxit = v > 0 ? v : (max_iterations > 2000 ? max_iterations : 2000);
       }
       // PETSc's single-synchronisation CG types select the one-pass recurrence
       if (o.rfind("-ksp_type", 0) == 0)
         single = o.find("pipecg") != std::string::npos || o.find("groppcg") != std::string::npos;
     }
     c.cg.set_single_reduction(single);
-    int its = 0;
+    const bool ns = n_dirichlet_rows <= 0;
+    int its = 0, total = 0, retried = 0;
     double res = 0.0;
-    const int rc = c.learned
-                       ? c.learned->solve(c.A, b, x, rtol, atol, nullptr, maxit, n_dirichlet_rows <= 0, &its, &res)
-                       : c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, n_dirichlet_rows <= 0, &its, &res);
+    // a solve stopped at its limit: once more from x = 0 with 4x the limit, then the soft accept
+    // or -(1000 + KSP_DIVERGED_ITS) (pressure_backend_petsc.c:878-937)
+    const int rc = dm::solve_with_retry(
+        maxit, rtol,
+        [&](int* it, double* r) {
+          return c.learned ? c.learned->solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r)
+                           : c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r);
+        },
+        [&](int mi, int* it, double* r) { return c.cg.solve(c.A, b, x, rtol, atol, nullptr, mi, ns, it, r); },
+        [&] { c.x.zero(x, size_t(n)); }, [&] { return atol; }, &its, &res, &total, &retried);
     if (!g_io_on_device) c.x.download(solution, c.sol, n);
     c.x.sync();
     *iteration_count = its;
     *residual_norm = res;
-    if (rc == 0) return 0;
-    // iteration limit (KSP_DIVERGED_ITS = -3): soft accept near tolerance (pressure_backend_petsc.c:917-920)
-    if (res < rtol * 1.0e4 || res < atol * 1.0e3) return 0;
-    return -(1000 + (-3));
+    return rc;
   } catch (const std::exception&) {
     return -33;
   }

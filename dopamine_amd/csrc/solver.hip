@@ -82,6 +82,9 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (p->boussinesq && (p->t_scheme < 1 || p->t_scheme > 5 || p->t_scheme == 4))
       throw std::runtime_error("temperature scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3) or QUICK (5)");
     if (p->precond != 0 && p->precond != 1) throw std::runtime_error("precond must be 0 (Jacobi) or 1 (learned)");
+    if (p->precond == 1 && p->precision == 1)
+      throw std::runtime_error("the learned preconditioner runs in the FP64 flexible CG only: precision 1 with "
+                               "precond 1 is not supported");
     if (p->mass_flux && (p->f_const[0] != 0.0 || p->f_const[1] != 0.0 || p->f_const[2] != 0.0))
       throw std::runtime_error("f_const must be 0 with constant mass flux forcing");
     if (p->n_scalars < 0 || p->n_scalars > 4) throw std::runtime_error("n_scalars must be 0..4");
@@ -154,11 +157,30 @@ int dopamine_solver_get_field(void* h, const char* name, double* host, int64_t l
 }
 
 int dopamine_solver_project_once(void* h, double dt) {
-  return guard([&] { return static_cast<Solver*>(h)->project_once(dt); });
+  return guard([&] {
+    auto* s = static_cast<Solver*>(h);
+    const int rc = s->project_once(dt);
+    if (rc != 0) g_error = s->err.empty() ? "project_once failed (" + std::to_string(rc) + ")" : s->err;
+    return rc;
+  });
 }
 
 int dopamine_solver_advance(void* h, double dt_req, double* dt_used) {
-  return guard([&] { return static_cast<Solver*>(h)->advance(dt_req, dt_used); });
+  return guard([&] {
+    auto* s = static_cast<Solver*>(h);
+    const int rc = s->advance(dt_req, dt_used);
+    if (rc != 0) g_error = s->err.empty() ? "advance failed (" + std::to_string(rc) + ")" : s->err;
+    return rc;
+  });
+}
+
+int dopamine_solver_state_is_finite(void* h) {
+  int ok = 0;
+  const int rc = guard([&] {
+    ok = static_cast<Solver*>(h)->state_is_finite() ? 1 : 0;
+    return 0;
+  });
+  return rc != 0 ? rc : ok;
 }
 
 double dopamine_solver_compute_dt(void* h) {
@@ -187,6 +209,8 @@ int dopamine_solver_counters(void* h, int64_t out[4]) {
   out[3] = s->n_sweeps;
   return 0;
 }
+
+int64_t dopamine_solver_retries(void* h) { return static_cast<Solver*>(h)->n_retries; }
 
 int dopamine_solver_nonortho_info(void* h, int32_t out[3]) {
   auto* s = static_cast<Solver*>(h);

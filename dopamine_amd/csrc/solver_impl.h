@@ -136,7 +136,8 @@ class SolverT {
   int skip_nonortho = 0, skip_grad_p = 0, n_corr = 1, diag_interface = 0, n_outlet = 0, n_wm = 0, n_dir = 0;
   double time = 0.0;
   int u_face_ready = 0;
-  int64_t n_solves = 0, n_iters = 0, last_iters = 0, n_sweeps = 0;
+  int64_t n_solves = 0, n_iters = 0, last_iters = 0, n_sweeps = 0, n_retries = 0;
+  std::string err;  // text of the last failed step (empty after a successful one)
   // exact cell projection (cell_projection = 1): work vectors, projections and iterations so far
   int cproj = 0;
   double *pj_b = nullptr, *pj_x = nullptr, *pj_r = nullptr, *pj_z = nullptr, *pj_p = nullptr, *pj_q = nullptr,
@@ -968,14 +969,35 @@ class SolverT {
       x.upload(cg_b, hb.data(), nc);
     }
     const int maxit = prm.p_max_iter > 2000 ? prm.p_max_iter : 2000;
-    int its = 0;
+    int its = 0, total = 0, retried = 0;
     double res = 0.0;
-    const int rc = policy ? policy->solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, n_dir == 0, &its, &res)
-                          : cg.solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, n_dir == 0, &its, &res);
+    const bool ns = n_dir == 0;
+    int sweeps = 0;
+    const int rc = solve_with_retry(
+        maxit, prm.p_tol,
+        [&](int* it, double* r) {
+          const int q = policy ? policy->solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, ns, it, r)
+                               : cg.solve(A, cg_b, P, prm.p_tol, atol, atol_dev, maxit, ns, it, r);
+          sweeps += cg.last_sweeps();
+          return q;
+        },
+        [&](int mi, int* it, double* r) {
+          const int q = cg.solve(A, cg_b, P, prm.p_tol, atol, atol_dev, mi, ns, it, r);
+          sweeps += cg.last_sweeps();
+          return q;
+        },
+        [&] { x.zero(P, nc); }, [&] { return cg.last_atol(); }, &its, &res, &total, &retried);
     n_solves++;
-    n_iters += its;
+    n_iters += total;
     last_iters = its;
-    n_sweeps += cg.last_sweeps();
+    n_sweeps += sweeps;
+    n_retries += retried;
+    if (rc != 0) {
+      err = "pressure solve failed: " + std::to_string(its) + " iterations of the retry from x = 0 (limit " +
+            std::to_string(4 * maxit) + ") left the true residual at " + std::to_string(res) + " (rtol " +
+            std::to_string(prm.p_tol) + ", atol " + std::to_string(cg.last_atol()) + "); returns " +
+            std::to_string(rc);
+    }
     return rc;
   }
 
@@ -1049,6 +1071,7 @@ class SolverT {
   // ---------------------------------------------------------------- public steps
 
   int project_once(double dt) {
+    err.clear();
     if (dt <= 0.0) dt = prm.dt;
     x.zero(rhs, nc);
     x.zero(phi_prev, nc);
@@ -1084,6 +1107,15 @@ class SolverT {
     return rc;
   }
 
+  // solver_state_is_finite (src/solver/mod_fv_solver.f90:3925-3945): U and p hold no NaN or Inf
+  bool state_is_finite() {
+    x.template reduce<1, false>(nc, FiniteSum{U, P}, sk::ProjFin<1>{sc, 0});
+    SolverScalars h;
+    x.download(&h, sc, 1);
+    const double v = h.pj[0];
+    return std::fabs(v) <= DBL_MAX;
+  }
+
   double compute_dt() {
     x.template reduce<2, true>(nc, CflRates{M, U, UF, nu_sgs, prm.nu, u_face_ready}, sk::CflFin{sc});
     SolverScalars h;
@@ -1102,9 +1134,13 @@ class SolverT {
   }
 
   int advance(double dt_req, double* dt_used) {
+    err.clear();
     double dt = dt_req;
     const double safe = compute_dt();
-    if (safe < 0.0) return -3;
+    if (safe < 0.0) {
+      err = "time step below 1e-6 dt by the CFL/diffusion limit";
+      return -3;
+    }
     if (dt > safe) dt = safe;
     if (prm.mass_flux) {
       x.template reduce<3, false>(nc, Momentum{M, U}, sk::ProjFin<3>{sc, 0});
@@ -1265,9 +1301,10 @@ class SolverT {
   // ---------------------------------------------------------------- checkpoint / restart
   // write_checkpoint / read_checkpoint (src/io/mod_checkpoint_io.f90:15-283): per-rank file
   // dir/processorR/checkpoint_XXXXXXXX.bin of Fortran unformatted sequential records (a 4-byte
-  // length before and after each payload): header(8) int32 {nc, n_scalars = 0, history = 0, step,
-  // has_C = 0, 0, has U_face, wall mask (1 tau_w, 2 q_w, 4 L)}, time, global ids (int64, 1..nc),
-  // U (3, nc), p, T (T_ref when no temperature is solved), nu_sgs, then U_face and the wall-model
+  // length before and after each payload): header(8) int32 {nc, n_scalars, history = 0, step,
+  // has_C, 0, has U_face, wall mask (1 tau_w, 2 q_w, 4 L)}, time, global ids (int64, 1..nc),
+  // U (3, nc), p, T (T_ref when no temperature is solved), nu_sgs, C (n_scalars, nc) when there are
+  // passive scalars, then U_face (only once the face flux is set: u_face_ready) and the wall-model
   // records the mask names. Rank 0 also writes dir/checkpoint_latest with the step.
   static void put_record(std::FILE* f, const void* p, size_t bytes) {
     if (bytes > 0x7fffffffull) throw std::runtime_error("checkpoint record above 2 GiB");
@@ -1293,7 +1330,7 @@ class SolverT {
     std::FILE* f = std::fopen(ckpt_name(dir, step, rank).c_str(), "wb");
     if (!f) throw std::runtime_error("cannot open " + ckpt_name(dir, step, rank));
     const int nbb = std::max(1, nb);
-    int32_t header[8] = {nc, 0, 0, step, 0, 0, 1, 1 + (q_w ? 2 + 4 : 0)};
+    int32_t header[8] = {nc, ns, 0, step, ns > 0 ? 1 : 0, 0, u_face_ready ? 1 : 0, 1 + (q_w ? 2 + 4 : 0)};
     std::vector<double> h(3 * size_t(std::max(nc, nf)));
     std::vector<int64_t> gid(nc);
     for (int c = 0; c < nc; ++c) gid[c] = c + 1;
@@ -1313,8 +1350,18 @@ class SolverT {
       put_record(f, h.data(), 8 * size_t(nc));
       x.download(h.data(), nu_sgs, nc);
       put_record(f, h.data(), 8 * size_t(nc));
-      x.download(h.data(), UF, nf);
-      put_record(f, h.data(), 8 * size_t(nf));
+      if (ns > 0) {  // C(1:ns, 1:nc): the scalar index fastest
+        std::vector<double> hc(size_t(ns) * nc), one(nc);
+        for (int q = 0; q < ns; ++q) {
+          x.download(one.data(), Cs[q], nc);
+          for (int c = 0; c < nc; ++c) hc[size_t(c) * ns + q] = one[c];
+        }
+        put_record(f, hc.data(), 8 * size_t(ns) * nc);
+      }
+      if (u_face_ready) {
+        x.download(h.data(), UF, nf);
+        put_record(f, h.data(), 8 * size_t(nf));
+      }
       std::vector<double> hb(3 * size_t(nbb));
       x.download(hb.data(), tau, 3 * size_t(nbb));
       put_record(f, hb.data(), 24 * size_t(nb));
@@ -1348,7 +1395,9 @@ class SolverT {
     try {
       get_record(f, header, sizeof(header));
       if (header[0] != nc) throw std::runtime_error("checkpoint: n_cells does not match the mesh");
-      if (header[1] != 0) throw std::runtime_error("checkpoint: passive scalars are not supported");
+      if (header[1] != ns)
+        throw std::runtime_error("checkpoint: n_scalars " + std::to_string(header[1]) + " does not match the solver's " +
+                                 std::to_string(ns));
       get_record(f, &time, 8);
       std::vector<int64_t> gid(nc);
       get_record(f, gid.data(), 8 * size_t(nc));
@@ -1362,9 +1411,18 @@ class SolverT {
       if (bous) x.upload(T, h.data(), nc);
       get_record(f, h.data(), 8 * size_t(nc));
       x.upload(nu_sgs, h.data(), nc);
+      if (header[1] > 0) {
+        std::vector<double> hc(size_t(ns) * nc), one(nc);
+        get_record(f, hc.data(), 8 * size_t(ns) * nc);
+        for (int q = 0; q < ns; ++q) {
+          for (int c = 0; c < nc; ++c) one[c] = hc[size_t(c) * ns + q];
+          x.upload(Cs[q], one.data(), nc);
+        }
+      }
       if (header[6] == 1) {
         get_record(f, h.data(), 8 * size_t(nf));
         x.upload(UF, h.data(), nf);
+        u_face_ready = 1;  // mod_checkpoint_io.f90:226-228
       }
       std::vector<double> hb(3 * size_t(nbb));
       if (header[7] & 1) {

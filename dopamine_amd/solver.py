@@ -109,6 +109,9 @@ def _bind(L):
     L.dopamine_solver_read_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_stats_csv.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _d, _P(_d), ctypes.c_int]
     L.dopamine_solver_learned_loss.argtypes = [ctypes.c_void_p, _P(_d), ctypes.c_int]
+    L.dopamine_solver_state_is_finite.argtypes = [ctypes.c_void_p]
+    L.dopamine_solver_retries.restype = _i64
+    L.dopamine_solver_retries.argtypes = [ctypes.c_void_p]
     return L
 
 
@@ -116,11 +119,21 @@ def _bind(L):
 KERNELS = {"cg_dir": 0, "cg_upd": 1, "explicit": 2, "ls_grad": 3}
 
 
+def check_precision(params, precision):
+    """The pressure paths that exist: precision 0 (FP64) with any preconditioner, precision 1 (FP32
+    sweeps in FP64 refinement) with the Jacobi CG."""
+    if precision not in (0, 1):
+        raise ValueError(f"precision must be 0 (FP64) or 1 (mixed), got {precision}")
+    if precision == 1 and params.precond.upper() == "LEARNED":
+        raise ValueError("the learned preconditioner runs in the FP64 flexible CG only (precision 0)")
+
+
 class FVSolver:
     """One RK3W solver on the current HIP device."""
 
     def __init__(self, mesh, params, precision=0):
         params.validate()
+        check_precision(params, precision)
         self.L = _bind(lib())
         self.mesh = mesh
         self.params = params
@@ -194,6 +207,26 @@ class FVSolver:
 
     def compute_dt(self):
         return self.L.dopamine_solver_compute_dt(self.h)
+
+    def state_is_finite(self):
+        """solver_state_is_finite (src/solver/mod_fv_solver.f90:3925-3945): U and p hold no NaN/Inf."""
+        rc = self.L.dopamine_solver_state_is_finite(self.h)
+        if rc < 0:
+            raise RuntimeError(f"state_is_finite failed ({rc}): {self.L.dopamine_last_error().decode()}")
+        return rc == 1
+
+    def run(self, n_steps, dt=None, check_every=10):
+        """n_steps of advance(); like the reference's time loop (src/main.f90:434-442) the state is
+        checked for NaN/Inf every `check_every` steps and a diverged run stops with an error."""
+        for step in range(1, n_steps + 1):
+            self.advance(dt)
+            if check_every and step % check_every == 0 and not self.state_is_finite():
+                raise FloatingPointError(f"non-finite velocity/pressure detected at step {step}, t = {self.time:.4e}")
+
+    @property
+    def retries(self):
+        """Pressure solves that were repeated from p = 0 with four times the iteration limit."""
+        return self.L.dopamine_solver_retries(self.h)
 
     @property
     def time(self):

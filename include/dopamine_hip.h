@@ -124,10 +124,21 @@ const char *dopamine_last_error(void);
 int dopamine_solver_set_field(void *solver, const char *name, const double *host);
 int dopamine_solver_get_field(void *solver, const char *name, double *host, int64_t len);
 
-/* solver_project_once (src/solver/mod_fv_solver.f90:837-917). */
+/* solver_project_once (src/solver/mod_fv_solver.f90:837-917). A pressure solve that stops at its
+ * iteration limit is repeated from p = 0 with four times the limit, then soft-accepted or failed
+ * with -997 as the reference backend does (src/solver/pressure_backend_petsc.c:878-937); every
+ * non-zero return leaves its reason in dopamine_last_error(). */
 int dopamine_solver_project_once(void *solver, double dt);
-/* solver_advance (src/solver/mod_fv_solver.f90:920-957): CFL clamp, one RK3W step. */
+/* solver_advance (src/solver/mod_fv_solver.f90:920-957): CFL clamp, one RK3W step. Returns -2
+ * when a pressure solve failed (after the retry above), -3 when the CFL/diffusion limit is below
+ * 1e-6 dt; the reason is in dopamine_last_error(). */
 int dopamine_solver_advance(void *solver, double dt_req, double *dt_used);
+/* solver_state_is_finite (src/solver/mod_fv_solver.f90:3925-3945): 1 when U and p hold no NaN or
+ * Inf (the sum of U and p over the cells, reduced on the device, is finite), 0 otherwise, negative
+ * on error. The reference's time loop checks it every 10 steps (src/main.f90:434-442). */
+int dopamine_solver_state_is_finite(void *solver);
+/* Pressure solves that needed the retry from p = 0 (see dopamine_solver_project_once). */
+int64_t dopamine_solver_retries(void *solver);
 /* solver_compute_dt (src/solver/mod_fv_solver.f90:3864-3899). */
 double dopamine_solver_compute_dt(void *solver);
 /* compute_stats_sums + solver_divergence_metrics: {E, eps, Umax, Ub, tauw_mean, nu_sgs_mean,

@@ -21,13 +21,14 @@ int fvm_petsc_backend_available(void);
 int fvm_petsc_backend_initialize(void);
 /* Releases the cached solve context; 0 on success (pressure_backend_petsc.c:618-646). */
 int fvm_petsc_backend_finalize(void);
-/* Preconditioner selector (pressure_backend_petsc.c:534-539). "boomeramg", "gamg" and "gmres" are
- * accepted for input compatibility and map to the Jacobi CG of this backend; "mixed" selects
+/* Preconditioner selector (pressure_backend_petsc.c:534-539). "boomeramg", "gamg", "gmres" and
+ * "jacobi" run the Jacobi CG of this backend; "mixed" selects
  * FP32-operator Jacobi-CG sweeps inside FP64 iterative refinement (same stopping test on the FP64
  * residual); "fcg-learned" selects the learned preconditioner (dopamine_mpnn.h): the first solves
  * run Jacobi CG and keep residual snapshots, the network is then trained on them and later solves
  * run flexible CG with it, falling back to Jacobi CG when a solve fails or needs more iterations.
- * A change of selector rebuilds the solve context. */
+ * Any other name makes fvm_petsc_solve_pressure return -11. A change of selector rebuilds the
+ * solve context. */
 void fvm_petsc_set_pressure_solver(const char *solver_name);
 /* PETSc-style option string (pressure_backend_petsc.c:543-551); "-ksp_max_it N" and
  * "-ksp_type pipecg|groppcg" (the one-pass single-reduction CG recurrence; default "cg", two
@@ -43,9 +44,11 @@ int fvm_petsc_device_io_supported(void);
 /* When on != 0, rhs and solution of the next solves are device pointers (pressure_backend_petsc.c:22-24). */
 void fvm_petsc_set_io_on_device(int on);
 /* Solve A x = rhs (pressure_backend_petsc.c:648-1034). solution is the initial guess in and the
- * answer out; *residual_norm is the true ||rhs - A x||_2. Returns 0 when converged (or soft-accepted:
- * iteration limit with residual < 1e4*rtol or < 1e3*atol), -(1000+reason) when not, other negative
- * values for set-up and copy errors. */
+ * answer out; *residual_norm is the true ||rhs - A x||_2. A solve that stops at its iteration limit
+ * is repeated from x = 0 with four times the limit (Jacobi CG; the reference retries with block
+ * Jacobi, :878-916). Returns 0 when converged (or soft-accepted after the retry: residual
+ * < 1e4*rtol or < 1e3*atol), -(1000+reason) when not (-997, KSP_DIVERGED_ITS), -11 for an unknown
+ * selector, other negative values for set-up and copy errors. */
 int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces, const int *local_to_global,
                              const int *face_owner, const int *face_neighbor_local, const int *face_neighbor_global,
                              const int *skip_face, const double *diagonal, const double *rhs,

@@ -39,6 +39,14 @@ def lib():
         L.hc_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
         L.hc_solver_write_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
         L.hc_solver_read_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        L.hc_solver_state_is_finite.argtypes = [ctypes.c_void_p]
+        L.hc_solver_retries.restype = _i64
+        L.hc_solver_retries.argtypes = [ctypes.c_void_p]
+        L.hc_solver_error.restype = ctypes.c_char_p
+        L.hc_solver_error.argtypes = [ctypes.c_void_p]
+        _pi = _P(ctypes.c_int)
+        L.hc_cg_solve_retry.argtypes = [ctypes.c_int, ctypes.c_int, _pi, _P(_d), _P(_d), _P(_d), _P(_d), _d, _d,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _pi, _P(_d), _pi, _pi]
         _lib = L
     return _lib
 
@@ -116,8 +124,15 @@ class HostSolver:
         used = _d(0.0)
         rc = lib().hc_solver_advance(self.h, self.params.dt if dt is None else dt, ctypes.byref(used))
         if rc != 0:
-            raise RuntimeError(f"advance failed {rc}")
+            raise RuntimeError(f"advance failed {rc}: {lib().hc_solver_error(self.h).decode()}")
         return used.value
+
+    def state_is_finite(self):
+        return lib().hc_solver_state_is_finite(self.h) == 1
+
+    @property
+    def retries(self):
+        return lib().hc_solver_retries(self.h)
 
     @property
     def time(self):
@@ -142,3 +157,22 @@ class HostSolver:
         out = (_i64 * 4)()
         lib().hc_solver_counters(self.h, out)
         return {"solves": out[0], "iterations": out[1], "last": out[2], "sweeps": out[3]}
+
+
+def cg_solve_retry(nbr, coef, diag, b, x0, rtol, atol, maxit, nullspace=False, single=False, mixed=False):
+    """The product CG with the reference's failure handling (solve_with_retry) on an ELL operator
+    (nbr/coef as [slot][row] arrays); returns (rc, x, its, resid, total_its, retried)."""
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    w, n = nbr.shape
+    its, total, retried = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    resid = _d(0.0)
+    pi, pd = _P(ctypes.c_int), _P(_d)
+    rc = lib().hc_cg_solve_retry(n, w, nbr.ctypes.data_as(pi), coef.ctypes.data_as(pd), diag.ctypes.data_as(pd),
+                                 b.ctypes.data_as(pd), x.ctypes.data_as(pd), rtol, atol, maxit, int(nullspace),
+                                 int(single), int(mixed), ctypes.byref(its), ctypes.byref(resid),
+                                 ctypes.byref(total), ctypes.byref(retried))
+    return rc, x, its.value, resid.value, total.value, retried.value

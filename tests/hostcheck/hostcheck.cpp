@@ -2,6 +2,7 @@
 // libdopamine_hip.so (dopamine_amd/csrc/solver_impl.h), executed by HostExec on the CPU, so the
 // CPU test suite can compare the product's indexing and operation order with the oracle without
 // a GPU. Test infrastructure only; the product library never contains HostExec.
+#include <algorithm>
 #include <string>
 
 #include "solver_impl.h"
@@ -87,6 +88,34 @@ void hc_kfactor_limited(const double sf[3], const double d[3], double min_cos, d
 void hc_solver_proj_counters(void* h, int64_t out[2]) {
   out[0] = static_cast<Solver*>(h)->n_proj;
   out[1] = static_cast<Solver*>(h)->n_proj_its;
+}
+int hc_solver_state_is_finite(void* h) { return static_cast<Solver*>(h)->state_is_finite() ? 1 : 0; }
+int64_t hc_solver_retries(void* h) { return static_cast<Solver*>(h)->n_retries; }
+const char* hc_solver_error(void* h) { return static_cast<Solver*>(h)->err.c_str(); }
+// The product's CG with the reference's retry (solve_with_retry) on an ELL operator, with any
+// iteration limit (the solver path clamps it to >= 2000), so the retry, soft-accept and failure
+// branches can be driven on small systems.
+int hc_cg_solve_retry(int n, int w, const int* nbr, const double* coef, const double* diag, const double* b,
+                      double* xv, double rtol, double atol, int maxit, int nullspace, int single, int mixed, int* its,
+                      double* resid, int* total, int* retried) {
+  dm::HostExec ex;
+  dm::CgT<dm::HostExec> cg;
+  cg.init(&ex, n);
+  cg.set_single_reduction(single != 0);
+  if (mixed) cg.set_mixed(dm::EllOp32{}, 1.0e-4);
+  dm::EllOp A;
+  A.n = n;
+  A.w = w;
+  A.nbr = nbr;
+  A.coef = coef;
+  A.diag = diag;
+  const bool ns = nullspace != 0;
+  auto once = [&](int mi, int* it, double* r) { return cg.solve(A, b, xv, rtol, atol, nullptr, mi, ns, it, r); };
+  const int rc = dm::solve_with_retry(
+      maxit, rtol, [&](int* it, double* r) { return once(maxit, it, r); }, once,
+      [&] { std::fill(xv, xv + n, 0.0); }, [&] { return cg.last_atol(); }, its, resid, total, retried);
+  cg.destroy();
+  return rc;
 }
 void hc_solver_nonortho_info(void* h, int32_t out[3]) {
   auto* s = static_cast<Solver*>(h);

@@ -94,3 +94,65 @@ def test_checkpoint_for_another_mesh_is_refused(tmp_path):
     s = HostSolver(m2, c2.params)
     with pytest.raises(OSError, match="n_cells"):
         s.read_checkpoint(tmp_path, 1)
+
+
+def test_checkpoint_keeps_passive_scalars(tmp_path):
+    """header(2) = n_scalars, header(5) = has_C and the C(1:ns, 1:nc) record after nu_sgs
+    (src/io/mod_checkpoint_io.f90:41-50, 109-111); a restart continues the scalars too."""
+    case = cases.full_stack(16, 16, 8, n_obstacles=3, seed=5)
+    mesh = case.make_mesh()
+    a = _run(case, mesh, 2)
+    a.write_checkpoint(tmp_path, 2)
+    rec = fortran_records(tmp_path / "processor0" / "checkpoint_00000002.bin")
+    header = struct.unpack("<8i", rec[0])
+    nc = mesh.n_cells
+    assert header == (nc, 2, 0, 2, 1, 0, 1, 1 + 2 + 4)
+    C = np.frombuffer(rec[7], dtype="<f8").reshape(nc, 2)  # scalar index fastest
+    assert np.array_equal(C[:, 0], a.scalar(0)) and np.array_equal(C[:, 1], a.scalar(1))
+    assert np.array_equal(np.frombuffer(rec[8], dtype="<f8"), a.array("U_face"))
+    a.advance()
+    b = HostSolver(mesh, case.params)
+    P.initialize(b, case, mesh)
+    for k in range(2):
+        b.set_scalar(k, np.zeros(nc))  # whatever the fresh solver holds is overwritten
+    assert b.read_checkpoint(tmp_path, 2) == 2
+    b.advance()
+    # the checkpoint omits boundary values and gradients; the restart rebuilds them from the
+    # restored fields (solver_refresh_after_restart, mod_fv_solver.f90:601-609), where the straight
+    # run still holds the last stage's lagged gradients that the zero-gradient and wall-model
+    # boundary extrapolations read: the continuation agrees to that lag, not to the last bit
+    for k in range(2):
+        assert rel_l2(a.scalar(k), b.scalar(k)) < 1e-7
+    assert rel_l2(a.U, b.U) < 1e-7
+
+
+def test_checkpoint_with_a_different_scalar_count_is_refused(tmp_path):
+    case = cases.full_stack(16, 16, 8, n_obstacles=3, seed=5)
+    mesh = case.make_mesh()
+    _run(case, mesh, 1).write_checkpoint(tmp_path, 1)
+    case.params.n_scalars = 1
+    s = HostSolver(mesh, case.params)
+    with pytest.raises(OSError, match="n_scalars"):
+        s.read_checkpoint(tmp_path, 1)
+
+
+def test_face_flux_record_only_once_it_is_set(tmp_path):
+    """header(7) flags U_face only when u_face_ready (mod_checkpoint_io.f90:46-47); a restart that
+    reads it takes the CFL limit from the face fluxes again (:226-228), so a CFL-limited step after
+    the restart matches the straight run."""
+    case = cases.small_channel(12, 10, 6)
+    mesh = case.make_mesh()
+    fresh = HostSolver(mesh, case.params)
+    fresh.write_checkpoint(tmp_path / "fresh", 0)
+    rec = fortran_records(tmp_path / "fresh" / "processor0" / "checkpoint_00000000.bin")
+    assert struct.unpack("<8i", rec[0])[6] == 0 and len(rec) == 8  # no U_face record
+    a = _run(case, mesh, 2)
+    a.write_checkpoint(tmp_path, 2)
+    dt_big = 50.0 * case.params.dt
+    used_a = a.advance(dt_big)
+    b = HostSolver(mesh, case.params)
+    assert b.read_checkpoint(tmp_path, 2) == 2
+    used_b = b.advance(dt_big)
+    assert used_a < dt_big  # the CFL limit is active
+    assert used_b == used_a
+    assert rel_l2(a.U, b.U) < 1e-12

@@ -73,11 +73,11 @@ def test_forward_matches_oracle(small):
     err = np.linalg.norm(corr - corro) / np.linalg.norm(corro)
     print(f"[gpu] mpnn forward: correction rel-L2 {err:.2e}", flush=True)
     assert np.linalg.norm(corro) > 1e-3 * np.linalg.norm(r / g.diag)
-    assert err < 1e-2
+    assert err < 1e-3  # measured 1.2e-4
     for l in range(3):
         same = np.mean(hs[l] == hso[l])
         print(f"[gpu] mpnn h{l + 1}: {same:.4f} of the bfloat16 activations identical", flush=True)
-        assert same > 0.95
+        assert same > 0.99  # measured >= 0.997
         assert np.max(np.abs(hs[l] - hso[l])) <= 2.0 ** -5
 
 
@@ -101,12 +101,12 @@ def test_weight_gradients_match_oracle(small):
     loss, grad, hs = net.loss_grad(r, activations=True)
     L, grads = om.backward_from(w, g, r, hs)
     print(f"[gpu] mpnn loss {loss:.6e} oracle {L:.6e}", flush=True)
-    assert abs(loss - L) <= 1e-4 * L
+    assert abs(loss - L) <= 1e-5 * L
     gd = om.unflatten(grad)
     for k in om.PARAM_ORDER:
         e = np.linalg.norm(gd[k] - grads[k]) / np.linalg.norm(grads[k])
         print(f"[gpu] mpnn grad {k}: rel-L2 {e:.2e}", flush=True)
-        assert e < 5e-3, k
+        assert e < 1e-6, k  # measured <= 3e-8
 
 
 def test_adam_step_matches_published_update(small):

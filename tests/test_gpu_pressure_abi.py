@@ -116,3 +116,40 @@ def test_mixed_precision_solve_meets_the_fp64_residual(name):
         assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e3 * rtol
     L.fvm_petsc_set_pressure_solver(b"boomeramg")
     assert L.fvm_petsc_backend_finalize() == 0
+
+
+def test_solve_stopped_at_its_limit_is_retried_from_zero_then_fails_with_minus_997():
+    """The reference's failure handling (pressure_backend_petsc.c:878-937) through the ABI: with
+    the iteration limit (-ksp_max_it) at half of what the solve needs, the retry from x = 0 with
+    four times the limit converges and reports the plain solve's iterations; with a limit of 2
+    the retry stops too and the solve fails with -(1000 + KSP_DIVERGED_ITS) = -997. An unknown
+    selector fails with -11."""
+    mesh, own, nb, skip, coeff, diag = _operator("hydrowall")
+    n = mesh.n_cells
+    L = lib()
+    assert L.fvm_petsc_backend_initialize() == 0
+    L.fvm_petsc_insert_option.argtypes = [ctypes.c_char_p]
+    L.fvm_petsc_set_pressure_solver.argtypes = [ctypes.c_char_p]
+    L.fvm_petsc_insert_option(b"-ksp_type cg")
+    b = np.random.default_rng(21).standard_normal(n)
+    b -= b.mean()
+    rc, x, need, res = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, 0)
+    assert rc == 0
+    try:
+        L.fvm_petsc_insert_option(f"-ksp_max_it {need // 2}".encode())
+        rc, x2, its, res2 = _call(n, own, nb, skip, diag, b, coeff, np.full(n, 1.0e6), 1e-12, 1e-300, 0)
+        print(f"[gpu] retry: rc {rc} its {its} (plain {need}) resid {res2:.3e}", flush=True)
+        assert rc == 0 and its == need and res2 <= 1e-12 * np.linalg.norm(b) * 1.001
+        L.fvm_petsc_insert_option(b"-ksp_max_it 2")
+        rc, _, its, res3 = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, 0)
+        print(f"[gpu] failed retry: rc {rc} its {its} resid {res3:.3e}", flush=True)
+        assert rc == -997 and its == 8
+        L.fvm_petsc_set_pressure_solver(b"no-such-solver")
+        rc, _, _, _ = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, 0)
+        assert rc == -11
+    finally:
+        L.fvm_petsc_insert_option(b"-ksp_max_it 0")
+        L.fvm_petsc_set_pressure_solver(b"boomeramg")
+    rc, _, its, _ = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, 0)
+    assert rc == 0 and its == need
+    assert L.fvm_petsc_backend_finalize() == 0
