@@ -1,0 +1,620 @@
+// Aggregation multigrid preconditioner (FP32) for the pressure Poisson operator.
+//
+// The reference preconditions its pressure CG with hypre BoomerAMG or PETSc GAMG
+// (src/solver/pressure_backend_petsc.c:375-465), 21-46 CG iterations per solve on its TGV
+// benchmark (docs/usage.md:193), where Jacobi CG needs hundreds to thousands. This is the
+// iteration-cutting preconditioner of this build, written for the GPU:
+//
+// - set-up on the host, once per operator: greedy aggregation by strong connections (Vanek,
+//   Mandel & Brezina, Computing 56 (1996) 179-196: phase 1 takes a cell and its strongly connected
+//   neighbours when none of them is aggregated yet, phase 2 attaches the rest to the aggregate of
+//   their strongest aggregated neighbour, phase 3 groups what is left), piecewise-constant
+//   prolongation P, Galerkin coarse operators A_c = P^T A P (for constant P: sums of the fine
+//   coefficients between and inside aggregates, so every coarse operator is again a symmetric
+//   M-matrix in the same ELL form), down to at most COARSE_MAX cells, whose inverse (with a
+//   rank-one shift when the operator is singular) is formed densely;
+// - one application is a V(1,1) cycle: per level a damped-Jacobi pre-smoothing step from zero
+//   fused with the residual, a gather restriction over each aggregate's children, the dense solve
+//   at the bottom, and the prolongation (scaled by alpha, the over-correction that unsmoothed
+//   aggregation needs) fused with a damped-Jacobi post-smoothing step. Pre- and post-smoothing are
+//   the same operator, so the cycle is a fixed symmetric linear map;
+// - everything is FP32 (operator values, vectors); the flexible CG around it keeps its dot
+//   products in FP64 and, in mixed precision, sits inside FP64 iterative refinement.
+//
+// Stored coefficients: cs_ij = coef_ij / diag_j (the pre-smoothing residual
+// r = b - A (w D^-1 b) = (1 - w) b + w sum_j cs_ij b_j needs no neighbour diagonals) and
+// cr_ij = coef_ij / diag_i (the post-smoothing step y = x + w (D^-1 b - x + sum_j cr_ij x_j)).
+#pragma once
+
+#include "cg_impl.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+namespace dm {
+
+// ---------------------------------------------------------------- host set-up
+
+// One level's operator, FP64, ELL slot-major: A_ii = diag_i, A_ij = -coef for each listed j.
+struct AmgOp {
+  int n = 0, w = 0;
+  std::vector<int> nbr;
+  std::vector<double> coef, diag;
+};
+
+// Greedy aggregation; returns the aggregate count and agg[i] for every cell. j is strongly
+// connected to i when coef_ij >= theta sqrt(diag_i diag_j). Deterministic (cell order).
+inline int amg_aggregate(const AmgOp& A, double theta, std::vector<int>& agg) {
+  const int n = A.n, w = A.w;
+  agg.assign(n, -1);
+  auto strong = [&](int i, int k, int* j) {
+    const size_t q = size_t(k) * n + i;
+    *j = A.nbr[q];
+    return *j >= 0 && *j != i && A.coef[q] >= theta * std::sqrt(A.diag[i] * A.diag[*j]);
+  };
+  int na = 0;
+  // phase 1: a cell whose strong neighbourhood is entirely free roots a new aggregate
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    bool free_nbhd = true;
+    for (int k = 0; k < w && free_nbhd; ++k) {
+      int j;
+      if (strong(i, k, &j) && agg[j] >= 0) free_nbhd = false;
+    }
+    if (!free_nbhd) continue;
+    agg[i] = na;
+    for (int k = 0; k < w; ++k) {
+      int j;
+      if (strong(i, k, &j)) agg[j] = na;
+    }
+    ++na;
+  }
+  // phase 2: join the aggregate of the strongest aggregated strong neighbour (decided on the
+  // phase-1 state, applied afterwards)
+  std::vector<int> join(n, -1);
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    double best = -1.0;
+    for (int k = 0; k < w; ++k) {
+      int j;
+      if (!strong(i, k, &j) || agg[j] < 0) continue;
+      const double c = A.coef[size_t(k) * n + i];
+      if (c > best) {
+        best = c;
+        join[i] = agg[j];
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (agg[i] < 0 && join[i] >= 0) agg[i] = join[i];
+  // phase 3: what is left groups with its free neighbours
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    agg[i] = na;
+    for (int k = 0; k < w; ++k) {
+      const int j = A.nbr[size_t(k) * n + i];
+      if (j >= 0 && agg[j] < 0) agg[j] = na;
+    }
+    ++na;
+  }
+  return na;
+}
+
+// Children of each aggregate as a CSR list (cells in increasing order).
+inline void amg_children(const std::vector<int>& agg, int na, std::vector<int>& ptr, std::vector<int>& idx) {
+  ptr.assign(size_t(na) + 1, 0);
+  for (int a : agg) ptr[size_t(a) + 1]++;
+  for (int I = 0; I < na; ++I) ptr[size_t(I) + 1] += ptr[I];
+  idx.assign(agg.size(), 0);
+  std::vector<int> at(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < int(agg.size()); ++i) idx[at[agg[i]]++] = i;
+}
+
+// A_c = P^T A P for piecewise-constant P: diag_I = sum_{i in I} diag_i - sum of coef_ij with both
+// ends in I (each inside pair is listed from both rows, as the double sum counts it);
+// coef_IJ = sum of coef_ij with i in I, j in J.
+inline AmgOp amg_galerkin(const AmgOp& F, const std::vector<int>& agg, int na, const std::vector<int>& ptr,
+                          const std::vector<int>& idx) {
+  AmgOp C;
+  C.n = na;
+  C.diag.assign(na, 0.0);
+  std::vector<std::vector<std::pair<int, double>>> rows(na);
+  std::vector<std::pair<int, double>> tmp;
+  for (int I = 0; I < na; ++I) {
+    tmp.clear();
+    double d = 0.0;
+    for (int q = ptr[I]; q < ptr[I + 1]; ++q) {
+      const int i = idx[q];
+      d = d + F.diag[i];
+      for (int k = 0; k < F.w; ++k) {
+        const size_t s = size_t(k) * F.n + i;
+        const int j = F.nbr[s];
+        if (j < 0) continue;
+        const int J = agg[j];
+        if (J == I)
+          d = d - F.coef[s];
+        else
+          tmp.push_back({J, F.coef[s]});
+      }
+    }
+    std::stable_sort(tmp.begin(), tmp.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    auto& row = rows[I];
+    for (const auto& e : tmp) {
+      if (!row.empty() && row.back().first == e.first)
+        row.back().second = row.back().second + e.second;
+      else
+        row.push_back(e);
+    }
+    C.diag[I] = d;
+  }
+  C.w = 0;
+  for (const auto& r : rows) C.w = std::max<int>(C.w, int(r.size()));
+  const int w = std::max(1, C.w);
+  C.w = w;
+  C.nbr.assign(size_t(w) * na, -1);
+  C.coef.assign(size_t(w) * na, 0.0);
+  for (int I = 0; I < na; ++I)
+    for (size_t k = 0; k < rows[I].size(); ++k) {
+      C.nbr[k * na + I] = rows[I][k].first;
+      C.coef[k * na + I] = rows[I][k].second;
+    }
+  return C;
+}
+
+// Dense inverse of the coarsest operator (column-major, FP32 out). Each connected component whose
+// rows all sum to zero (no Dirichlet row: its constant is in the null space) is shifted by
+// s 1 1^T over its cells, s = mean diagonal / component size; for a right-hand side with zero sum
+// over the component the shifted inverse returns the pseudo-inverse solution. Cholesky in FP64.
+inline void amg_dense_inverse(const AmgOp& C, std::vector<float>& inv, bool* singular) {
+  const int n = C.n;
+  std::vector<double> a(size_t(n) * n, 0.0), rs(n);
+  double dsum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    a[size_t(i) * n + i] = C.diag[i];
+    rs[i] = C.diag[i];
+    for (int k = 0; k < C.w; ++k) {
+      const int j = C.nbr[size_t(k) * n + i];
+      if (j < 0) continue;
+      a[size_t(i) * n + j] -= C.coef[size_t(k) * n + i];
+      rs[i] -= C.coef[size_t(k) * n + i];
+    }
+    dsum += std::fabs(C.diag[i]);
+  }
+  const double dmean = dsum > 0.0 ? dsum / n : 1.0;
+  // connected components (breadth first over the ELL lists)
+  std::vector<int> comp(n, -1), queue;
+  int ncomp = 0;
+  *singular = false;
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (comp[s0] >= 0) continue;
+    queue.assign(1, s0);
+    comp[s0] = ncomp;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int i = queue[h];
+      for (int k = 0; k < C.w; ++k) {
+        const int j = C.nbr[size_t(k) * n + i];
+        if (j >= 0 && comp[j] < 0) {
+          comp[j] = ncomp;
+          queue.push_back(j);
+        }
+      }
+    }
+    bool sing = true;
+    for (int i : queue) sing = sing && std::fabs(rs[i]) <= 1.0e-10 * std::fabs(C.diag[i]);
+    if (sing) {
+      *singular = true;
+      const double sh = dmean / double(queue.size());
+      for (int i : queue)
+        for (int j : queue) a[size_t(i) * n + j] += sh;
+    }
+    ++ncomp;
+  }
+  inv.assign(size_t(n) * n, 0.0f);
+  // Cholesky a = L L^T (lower, in place)
+  for (int j = 0; j < n; ++j) {
+    double d = a[size_t(j) * n + j];
+    for (int k = 0; k < j; ++k) d -= a[size_t(j) * n + k] * a[size_t(j) * n + k];
+    if (!(d > 0.0)) throw std::runtime_error("multigrid: coarsest operator is not positive definite");
+    const double l = std::sqrt(d);
+    a[size_t(j) * n + j] = l;
+    for (int i = j + 1; i < n; ++i) {
+      double v = a[size_t(i) * n + j];
+      for (int k = 0; k < j; ++k) v -= a[size_t(i) * n + k] * a[size_t(j) * n + k];
+      a[size_t(i) * n + j] = v / l;
+    }
+  }
+  // columns of the inverse: solve L L^T x = e_c
+  std::vector<double> y(n);
+  for (int c = 0; c < n; ++c) {
+    for (int i = 0; i < n; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) v -= a[size_t(i) * n + k] * y[k];
+      y[i] = v / a[size_t(i) * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double v = y[i];
+      for (int k = i + 1; k < n; ++k) v -= a[size_t(k) * n + i] * y[k];
+      y[i] = v / a[size_t(i) * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[size_t(c) * n + i] = float(y[i]);  // column c
+  }
+}
+
+struct AmgCfg {
+  double theta = 0.08;       // strength threshold
+  double omega = 2.0 / 3.0;  // Jacobi damping
+  double alpha = 1.5;        // coarse-correction scaling
+  int coarse_max = 400;      // cells of the densely solved level
+  int max_levels = 16;
+};
+
+// ---------------------------------------------------------------- device kernels
+
+// One level in the execution space.
+struct AmgLevelDev {
+  int n = 0, w = 0;
+  const int* nbr = nullptr;
+  const float *cs = nullptr, *cr = nullptr, *dinv = nullptr;
+  const int* agg = nullptr;                       // to the next coarser level (n entries)
+  const int *cptr = nullptr, *cidx = nullptr;     // children of the next coarser level's cells
+  float *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
+};
+
+namespace amgk {
+
+// How the first and last kernels of a cycle take part in the flexible CG around it:
+// OUTER: the FP64 flexible CG's convergence test ||r_k|| <= ttol, k >= maxit (sets done);
+// INNER: an FP32 sweep of mixed precision, stopping at rr <= inner_tol2 or the iteration limit
+// (sets inner_done).
+enum Mode { OUTER = 0, INNER = 1 };
+
+DM_INLINE bool gated(const CgScalars* sc, int mode) { return sc && (sc->done || (mode == INNER && sc->inner_done)); }
+
+DM_INLINE bool cycle_test(CgScalars* sc, int mode, int k, double rr, bool leader) {
+  if (gated(sc, mode)) return false;
+  if (mode == OUTER) {
+    const int fin = (sqrt(rr) <= sc->ttol) ? 1 : (k >= sc->maxit ? 2 : 0);
+    if (leader) {
+      sc->its = k;
+      sc->rr = rr;
+      if (fin) sc->done = fin;
+    }
+    return fin == 0;
+  }
+  if (k > 0 && (rr <= sc->inner_tol2 || sc->its + k >= sc->maxit)) {
+    if (leader) {
+      sc->inner_done = 1;
+      sc->inner_its = k;
+    }
+    return false;
+  }
+  if (leader) sc->inner_its = k;
+  return true;
+}
+
+// pre-smoothing from zero and its residual: x = w D^-1 b, r = (1 - w) b + w sum_j cs_ij b_j
+template <class T>
+DM_INLINE void down_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
+  const int n = L.n;
+  const float bc = float(b[c]);
+  float s = 0.0f;
+  for (int k = 0; k < L.w; ++k) {
+    const size_t q = size_t(k) * n + c;
+    const int j = L.nbr[q];
+    if (j < 0) continue;
+    s = s + L.cs[q] * float(b[j]);
+  }
+  L.x[c] = omega * L.dinv[c] * bc;
+  L.r[c] = (1.0f - omega) * bc + omega * s;
+}
+
+// prolongation with over-correction and post-smoothing: xt = x + alpha xc[agg],
+// y = xt + w (D^-1 b - xt + sum_j cr_ij xt_j)
+template <class T>
+DM_INLINE float up_cell(const AmgLevelDev& L, const T* b, const float* xc, float omega, float alpha, int c) {
+  const int n = L.n;
+  const float xt = L.x[c] + alpha * xc[L.agg[c]];
+  float s = 0.0f;
+  for (int k = 0; k < L.w; ++k) {
+    const size_t q = size_t(k) * n + c;
+    const int j = L.nbr[q];
+    if (j < 0) continue;
+    s = s + L.cr[q] * (L.x[j] + alpha * xc[L.agg[j]]);
+  }
+  return xt + omega * (L.dinv[c] * float(b[c]) - xt + s);
+}
+
+// level 0 pre-smoothing as a chained pass: its setup is the convergence test of iteration k
+template <class T>
+struct Down0 {
+  AmgLevelDev L;
+  const T* b;
+  CgScalars* sc;
+  int mode, k;
+  float omega;
+  using Scal = cgk::NoScal;
+  DM_HD bool setup(const Red<1>& t, bool leader, Scal&) const { return cycle_test(sc, mode, k, t.v[0], leader); }
+  DM_HD Red<1> operator()(int c, const Scal&) const {
+    down_cell(L, b, omega, c);
+    Red<1> o;
+    o.v[0] = 0.0;
+    return o;
+  }
+};
+
+// the same two level-0 steps outside a solve (one V-cycle on its own)
+template <class T>
+struct Down0F {
+  AmgLevelDev L;
+  const T* b;
+  float omega;
+  DM_HD void operator()(int c) const { down_cell(L, b, omega, c); }
+};
+template <class T>
+struct Up0F {
+  AmgLevelDev L;
+  const T* r;
+  T* z;
+  const float* xc;
+  float omega, alpha;
+  DM_HD void operator()(int c) const { z[c] = T(up_cell(L, r, xc, omega, alpha, c)); }
+};
+
+struct DownL {
+  AmgLevelDev L;
+  const CgScalars* sc;
+  int mode;
+  float omega;
+  DM_HD void operator()(int c) const {
+    if (gated(sc, mode)) return;
+    down_cell(L, static_cast<const float*>(L.b), omega, c);
+  }
+};
+
+// b_coarse[I] = sum of the fine residual over aggregate I's cells
+struct Restrict {
+  const int *cptr, *cidx;
+  const float* rf;
+  float* bc;
+  const CgScalars* sc;
+  int mode;
+  DM_HD void operator()(int I) const {
+    if (gated(sc, mode)) return;
+    float s = 0.0f;
+    for (int q = cptr[I]; q < cptr[I + 1]; ++q) s = s + rf[cidx[q]];
+    bc[I] = s;
+  }
+};
+
+// y = inv b on the coarsest level (inv column-major: consecutive rows read consecutive words)
+struct Coarsest {
+  int n;
+  const float* inv;
+  const float* b;
+  float* y;
+  const CgScalars* sc;
+  int mode;
+  DM_HD void operator()(int i) const {
+    if (gated(sc, mode)) return;
+    float s = 0.0f;
+    for (int j = 0; j < n; ++j) s = s + inv[size_t(j) * n + i] * b[j];
+    y[i] = s;
+  }
+};
+
+struct UpL {
+  AmgLevelDev L;
+  const float* xc;
+  const CgScalars* sc;
+  int mode;
+  float omega, alpha;
+  DM_HD void operator()(int c) const {
+    if (gated(sc, mode)) return;
+    L.y[c] = up_cell(L, static_cast<const float*>(L.b), xc, omega, alpha, c);
+  }
+};
+
+// level 0 post-smoothing as a chained pass writing z and the flexible-CG partials
+// z.r, z.r_old, sum z, sum r, sum r_old
+template <class T>
+struct Up0 {
+  AmgLevelDev L;
+  const T *r, *rold;
+  T* z;
+  const float* xc;
+  const CgScalars* sc;
+  int mode;
+  float omega, alpha;
+  using Scal = cgk::NoScal;
+  DM_HD bool setup(const Red<1>&, bool, Scal&) const { return !gated(sc, mode); }
+  DM_HD Red<5> operator()(int c, const Scal&) const {
+    const float y = up_cell(L, r, xc, omega, alpha, c);
+    const T zc = T(y);
+    z[c] = zc;
+    const double rc = double(r[c]), ro = double(rold[c]), zd = double(zc);
+    Red<5> o;
+    o.v[0] = zd * rc;
+    o.v[1] = zd * ro;
+    o.v[2] = zd;
+    o.v[3] = rc;
+    o.v[4] = ro;
+    return o;
+  }
+};
+
+}  // namespace amgk
+
+// The multigrid hierarchy in the execution space and its V-cycle, usable as the FP64 flexible
+// CG's preconditioner (FcgPc) and as the FP32 sweeps' (FcgPc32).
+template <class X>
+class AmgPc : public FcgPc, public FcgPc32 {
+ public:
+  // A: the operator on the host (FP64); nbr_dev: its neighbour list already in the execution
+  // space (shared by level 0).
+  void build(X* x, const AmgOp& A, const int* nbr_dev, const AmgCfg& cfg) {
+    x_ = x;
+    cfg_ = cfg;
+    AmgOp cur = A;
+    const int* cur_nbr_dev = nbr_dev;
+    for (int lev = 0;; ++lev) {
+      Level L;
+      L.d.n = cur.n;
+      L.d.w = cur.w;
+      L.d.nbr = cur_nbr_dev ? cur_nbr_dev : up(cur.nbr);
+      {
+        // cs_ij = coef_ij / diag_j, cr_ij = coef_ij / diag_i, dinv_i = 1 / diag_i; a cell whose
+        // diagonal vanishes (an aggregate that is a whole connected component of a singular
+        // operator) is left out of the smoothing (dinv = 0, no couplings)
+        double dmax = 0.0;
+        for (double d : cur.diag) dmax = std::max(dmax, std::fabs(d));
+        const double dmin = 1.0e-12 * dmax;
+        const size_t m = size_t(std::max(1, cur.w)) * cur.n;
+        std::vector<float> cs(m, 0.0f), cr(m, 0.0f), di(cur.n, 0.0f);
+        for (int c = 0; c < cur.n; ++c) {
+          if (!(cur.diag[c] > dmin)) continue;
+          di[c] = float(1.0 / cur.diag[c]);
+          for (int k = 0; k < cur.w; ++k) {
+            const size_t q = size_t(k) * cur.n + c;
+            const int j = cur.nbr[q];
+            if (j < 0 || !(cur.diag[j] > dmin)) continue;
+            cs[q] = float(cur.coef[q] / cur.diag[j]);
+            cr[q] = float(cur.coef[q] / cur.diag[c]);
+          }
+        }
+        L.d.cs = up(cs);
+        L.d.cr = up(cr);
+        L.d.dinv = up(di);
+      }
+      L.d.b = lev == 0 ? nullptr : fresh<float>(cur.n);
+      L.d.x = fresh<float>(cur.n);
+      L.d.y = lev == 0 ? nullptr : fresh<float>(cur.n);
+      L.d.r = fresh<float>(cur.n);
+      const bool last = (cur.n <= cfg.coarse_max && lev > 0) || lev + 1 >= cfg.max_levels;
+      if (last) {
+        if (cur.n > 4096) throw std::runtime_error("multigrid: coarsest level too large for the dense solve");
+        std::vector<float> inv;
+        bool sing = false;
+        amg_dense_inverse(cur, inv, &sing);
+        inv_ = up(inv);
+        n_coarse_ = cur.n;
+        singular_ = sing;
+        levels_.push_back(L);
+        break;
+      }
+      std::vector<int> agg;
+      int na = amg_aggregate(cur, cfg.theta, agg);
+      if (na * 3 > cur.n * 2 && cur.n > 16)  // stalled coarsening: every connection counts as strong
+        na = amg_aggregate(cur, 0.0, agg);
+      std::vector<int> ptr, idx;
+      amg_children(agg, na, ptr, idx);
+      AmgOp next = amg_galerkin(cur, agg, na, ptr, idx);
+      L.d.agg = up(agg);
+      L.d.cptr = up(ptr);
+      L.d.cidx = up(idx);
+      levels_.push_back(L);
+      sizes_.push_back(cur.n);
+      widths_.push_back(cur.w);
+      cur = std::move(next);
+      cur_nbr_dev = nullptr;
+    }
+    sizes_.push_back(n_coarse_);
+    widths_.push_back(cur.w);
+    zpart_ = fresh<double>(size_t(x_->pass_grid(A.n)));
+  }
+  void destroy() {
+    if (!x_) return;
+    for (void* p : owned_) x_->release(p);
+    owned_.clear();
+    levels_.clear();
+    x_ = nullptr;
+  }
+  ~AmgPc() override { destroy(); }
+
+  int levels() const { return int(levels_.size()); }
+  const std::vector<int>& sizes() const { return sizes_; }
+  const std::vector<int>& widths() const { return widths_; }
+  bool singular() const { return singular_; }
+  const AmgCfg& cfg() const { return cfg_; }
+
+  // ---- FcgPc (FP64 flexible CG)
+  int out_grid(int n) const override { return x_->pass_grid(n); }
+  void apply(int k, const double* r, const double* rold, double* z, CgScalars* sc, const double* rr_part, int rr_g,
+             double* out_part) override {
+    cycle<double>(k, r, rold, z, sc, rr_part, rr_g, out_part, amgk::OUTER);
+  }
+  // ---- FcgPc32 (FP32 sweeps)
+  int out_grid32(int n) const override { return x_->pass_grid(n); }
+  void apply32(int k, const float* r, const float* rold, float* z, CgScalars* sc, const double* rr_part, int rr_g,
+               double* out_part) override {
+    cycle<float>(k, r, rold, z, sc, rr_part, rr_g, out_part, amgk::INNER);
+  }
+
+  // One V-cycle z = M r outside a solve (tests): no gate, no convergence test.
+  template <class T>
+  void vcycle(const T* r, T* z) {
+    cycle<T>(0, r, r, z, nullptr, nullptr, 0, nullptr, amgk::OUTER);
+  }
+
+ private:
+  struct Level {
+    AmgLevelDev d;
+  };
+  template <class T>
+  T* up(const std::vector<T>& v) {
+    T* p = x_->template alloc<T>(std::max<size_t>(1, v.size()));
+    x_->upload(p, v.data(), v.size());
+    owned_.push_back(p);
+    return p;
+  }
+  template <class T>
+  T* fresh(size_t n) {
+    T* p = x_->template alloc<T>(std::max<size_t>(1, n));
+    owned_.push_back(p);
+    return p;
+  }
+
+  template <class T>
+  void cycle(int k, const T* r, const T* rold, T* z, CgScalars* sc, const double* rr_part, int rr_g,
+             double* out_part, int mode) {
+    X& xx = *x_;
+    const float om = float(cfg_.omega), al = float(cfg_.alpha);
+    const int nl = int(levels_.size());
+    const AmgLevelDev& L0 = levels_[0].d;
+    if (sc)
+      xx.template pass<1, 1>(L0.n, amgk::Down0<T>{L0, r, sc, mode, k, om}, rr_part, rr_g, zpart_);
+    else
+      xx.for_each(L0.n, amgk::Down0F<T>{L0, r, om});
+    for (int l = 0; l + 1 < nl; ++l) {
+      const AmgLevelDev& F = levels_[l].d;
+      const AmgLevelDev& C = levels_[l + 1].d;
+      if (l > 0) xx.for_each(F.n, amgk::DownL{F, sc, mode, om});
+      xx.for_each(C.n, amgk::Restrict{F.cptr, F.cidx, F.r, C.b, sc, mode});
+    }
+    const AmgLevelDev& Lc = levels_[nl - 1].d;
+    xx.for_each(n_coarse_, amgk::Coarsest{n_coarse_, inv_, Lc.b, Lc.y, sc, mode});
+    for (int l = nl - 2; l >= 1; --l) {
+      const AmgLevelDev& F = levels_[l].d;
+      xx.for_each(F.n, amgk::UpL{F, levels_[l + 1].d.y, sc, mode, om, al});
+    }
+    const float* xc = levels_[1].d.y;
+    if (sc)
+      xx.template pass<1, 5>(L0.n, amgk::Up0<T>{L0, r, rold, z, xc, sc, mode, om, al}, nullptr, 0, out_part);
+    else
+      xx.for_each(L0.n, amgk::Up0F<T>{L0, r, z, xc, om, al});
+  }
+
+  X* x_ = nullptr;
+  AmgCfg cfg_;
+  std::vector<Level> levels_;
+  std::vector<int> sizes_, widths_;
+  std::vector<void*> owned_;
+  float* inv_ = nullptr;
+  int n_coarse_ = 0;
+  bool singular_ = false;
+  double* zpart_ = nullptr;
+};
+
+}  // namespace dm

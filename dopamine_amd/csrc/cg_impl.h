@@ -81,6 +81,18 @@ struct Scale {
 };
 }  // namespace cgk
 
+namespace cgk {
+struct To32 {
+  int n, w;
+  const double *coef, *diag;
+  float *coef32, *diag32;
+  DM_HD void operator()(int c) const {
+    diag32[c] = float(diag[c]);
+    for (int k = 0; k < w; ++k) coef32[size_t(k) * n + c] = float(coef[size_t(k) * n + c]);
+  }
+};
+}  // namespace cgk
+
 // FP32 copy of the operator for the mixed-precision inner solves (same rows and slots).
 struct EllOp32 {
   int n = 0, w = 0;
@@ -695,6 +707,16 @@ struct FcgPc {
   virtual int out_grid(int n) const = 0;
 };
 
+// Preconditioner of the FP32 sweeps of mixed precision: apply32(k, r, r_old, z, ...) with the
+// contract of FcgPc on FP32 vectors, the sweep's stopping test (rr <= inner_tol2 or the iteration
+// limit, setting inner_done) in its first pass.
+struct FcgPc32 {
+  virtual ~FcgPc32() = default;
+  virtual void apply32(int k, const float* r, const float* rold, float* z, CgScalars* sc, const double* rr_part,
+                       int rr_g, double* out_part) = 0;
+  virtual int out_grid32(int n) const = 0;
+};
+
 // Chooses, solve by solve, how a pressure system is solved (the learned preconditioner's
 // collection, training and fallback policy); same contract as CgT::solve.
 struct PressurePolicy {
@@ -826,6 +848,115 @@ struct FcgUpd {
     return o;
   }
 };
+
+// ---- the FP32 flexible CG of a mixed-precision sweep (vectors FP32, dot products FP64)
+
+// Sweep start: the outer FP64 test on r64 (as IrStart), then r = float(r64), r_old = r, x += e,
+// e = 0; partial r.r of the FP32 residual.
+struct FcgStart32 {
+  const double* r64;
+  float *r, *rold;
+  double* x;
+  float* e;
+  CgScalars* sc;
+  int sweep, max_sweeps;
+  double tau;
+  using Scal = NoScal;
+  DM_HD bool setup(const Red<2>& t, bool leader, Scal&) const {
+    if (sc->done) return false;
+    const double ttol = sweep == 0 ? fmax(sc->rtol * sqrt(t.v[0]), sc->atol) : sc->ttol;
+    const double rr = t.v[1];
+    const int fin = (sqrt(rr) <= ttol) ? 1 : ((sweep >= max_sweeps || sc->its >= sc->maxit) ? 2 : 0);
+    if (leader) {
+      if (sweep == 0) sc->ttol = ttol;
+      sc->rr = rr;
+      sc->inner_done = 0;
+      sc->inner_its = 0;
+      sc->inner_tol2 = fmax(tau * tau * rr, 0.25 * ttol * ttol);
+      if (fin) sc->done = fin;
+    }
+    return fin == 0;
+  }
+  DM_HD Red<1> operator()(int c, const Scal&) const {
+    const float rc = float(r64[c]);
+    r[c] = rc;
+    rold[c] = rc;
+    x[c] = x[c] + double(e[c]);
+    e[c] = 0.0f;
+    Red<1> o;
+    o.v[0] = double(rc) * double(rc);
+    return o;
+  }
+};
+
+// z -= mean(z) on a singular operator; beta = max(0, z.(r - r_old) / (z.r)_old);
+// p = z + beta p_old; q = A p (FP32 operator); partial p.q
+struct FcgDir32 {
+  EllOp32 A;
+  const float *z, *pold;
+  float *pnew, *q;
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double zm, beta;
+  };
+  DM_HD bool setup(const Red<5>& t, bool leader, Scal& s) const {
+    if (sc->done || sc->inner_done) return false;
+    s.zm = sc->nullspace ? t.v[2] / double(A.n) : 0.0;
+    const double zr = t.v[0] - s.zm * t.v[3];
+    const double zro = t.v[1] - s.zm * t.v[4];
+    s.beta = k == 0 ? 0.0 : fmax(0.0, (zr - zro) / sc->rz_hist[(k + 1) & 1]);
+    if (leader) sc->rz_hist[k & 1] = zr;
+    return true;
+  }
+  DM_HD Red<1> operator()(int c, const Scal& s) const {
+    const int n = A.n;
+    const float zm = float(s.zm), beta = float(s.beta);
+    const bool first = k == 0;
+    const float pc = first ? z[c] - zm : (z[c] - zm) + beta * pold[c];
+    float qc = A.diag[c] * pc;
+    for (int kk = 0; kk < A.w; ++kk) {
+      const size_t q_ = size_t(kk) * n + c;
+      const int j = A.nbr[q_];
+      if (j < 0) continue;
+      const float pj = first ? z[j] - zm : (z[j] - zm) + beta * pold[j];
+      qc = qc - A.coef[q_] * pj;
+    }
+    pnew[c] = pc;
+    q[c] = qc;
+    Red<1> o;
+    o.v[0] = double(pc) * double(qc);
+    return o;
+  }
+};
+
+// alpha = (z.r) / (p.q); e += alpha p; r_old = r; r -= alpha q; partial r.r
+struct FcgUpd32 {
+  float *e, *r, *rold;
+  const float *p, *q;
+  CgScalars* sc;
+  int k;
+  struct Scal {
+    double alpha;
+  };
+  DM_HD bool setup(const Red<1>& t, bool, Scal& s) const {
+    if (sc->done || sc->inner_done) return false;
+    s.alpha = sc->rz_hist[k & 1] / t.v[0];
+    return true;
+  }
+  DM_HD Red<1> operator()(int c, const Scal& s) const {
+    const float a = float(s.alpha);
+    e[c] = e[c] + a * p[c];
+    const float rc = r[c];
+    rold[c] = rc;
+    const float rn = rc - a * q[c];
+    r[c] = rn;
+    Red<1> o;
+    o.v[0] = double(rn) * double(rn);
+    return o;
+  }
+};
+
 }  // namespace cgk
 
 // The reference's handling of a pressure solve that stops at its iteration limit
@@ -885,6 +1016,22 @@ class CgT {
     }
   }
   bool fcg() const { return fcg_ != nullptr; }
+  // Mixed precision with a preconditioner: each FP32 sweep is a flexible CG with pc (null: the
+  // Jacobi single-reduction sweeps).
+  void set_pc32(FcgPc32* pc) {
+    pc32_ = pc;
+    if (!pc) return;
+    for (float** v : {&r32_, &ro32_, &z32_, &pa32_, &pb32_, &q32_})
+      if (!*v) *v = x_->template alloc<float>(n_);
+    if (fpart32_g_ < pc->out_grid32(n_)) {
+      if (fpart32_) x_->release(fpart32_);
+      fpart32_g_ = pc->out_grid32(n_);
+      fpart32_ = x_->template alloc<double>(size_t(5) * fpart32_g_);
+    }
+  }
+  // The next solves run the plain Jacobi passes, whatever preconditioner is set (the retry of a
+  // failed solve).
+  void set_plain(bool on) { plain_ = on; }
   // Residual snapshots of the Jacobi passes at their batch boundaries (null: none).
   void set_capture(CgCapture* c) { capture_ = c; }
   // Mixed precision: FP64 iterative refinement around FP32 sweeps on A32 (same rows as the FP64
@@ -899,8 +1046,14 @@ class CgT {
     if (!x_) return;
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
-                    (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_, (void*)e32_})
+                    (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_, (void*)e32_, (void*)r32_,
+                    (void*)ro32_, (void*)z32_, (void*)pa32_, (void*)pb32_, (void*)q32_, (void*)fpart32_,
+                    (void*)coef32_, (void*)diag32_})
       if (p) x_->release(p);
+    r32_ = ro32_ = z32_ = pa32_ = pb32_ = q32_ = coef32_ = diag32_ = nullptr;
+    fpart32_ = nullptr;
+    fpart32_g_ = 0;
+    pc32_ = nullptr;
     st_[0] = st_[1] = nullptr;
     st32_[0] = st32_[1] = nullptr;
     e32_ = nullptr;
@@ -939,16 +1092,18 @@ class CgT {
     int batch = std::max(4, std::min(maxit + 1, predict_));
     if (capture_) batch = std::min(batch, 16);  // residual snapshots along the whole solve
     last_sweeps_ = 0;
-    if (fcg_) {
+    FcgPc* fcg = plain_ ? nullptr : fcg_;
+    FcgPc32* pc32 = plain_ ? nullptr : pc32_;
+    if (fcg) {
       double* RR = U[0];
-      const int gt = fcg_->out_grid(n);
+      const int gt = fcg->out_grid(n);
       xx.template pass<2, 1>(n, FcgStart{r_, ro_, sc_}, P1, g, RR);
       double* pold = pa_;
       double* pnew = pb_;
       batch = std::max(4, std::min(maxit + 1, predict_fcg_));
       for (;;) {
         for (int i = 0; i < batch; ++i, ++k) {
-          fcg_->apply(k, r_, ro_, z_, sc_, RR, g, fpart_);  // times its hidden layer as K_CG_DIR
+          fcg->apply(k, r_, ro_, z_, sc_, RR, g, fpart_);  // times its hidden layer as K_CG_DIR
           xx.tbegin(K_CG_UPD);
           by_width(A.w, [&](auto wc) {
             xx.template pass<5, 1>(n, FcgDirT<decltype(wc)::value>{A, z_, pold, pnew, q_, sc_, k}, fpart_, gt, D);
@@ -961,6 +1116,44 @@ class CgT {
         if (h.done || k > maxit + 1) break;
         batch = std::max(2, std::min(16, (h.its + 7) / 8));
       }
+    } else if (mixed_ && pc32) {
+      // FP64 refinement around FP32 flexible-CG sweeps preconditioned by pc32
+      const EllOp32 A32 = op32(A);
+      double* RR = U[0];
+      const int gt = pc32->out_grid32(n);
+      const double* in = P1;
+      int total = 0;
+      xx.zero(e32_, size_t(n));
+      for (int sweep = 0;; ++sweep) {
+        xx.template pass<2, 1>(n, FcgStart32{r_, r32_, ro32_, x, e32_, sc_, sweep, MAX_SWEEPS, tau_}, in, g, RR);
+        float* pold = pa32_;
+        float* pnew = pb32_;
+        int kin = 0;
+        int ibatch = std::max(4, std::min(maxit + 1, predict_sw_[std::min(sweep, MAX_SWEEPS)]));
+        for (;;) {
+          for (int i = 0; i < ibatch; ++i, ++kin) {
+            pc32->apply32(kin, r32_, ro32_, z32_, sc_, RR, g, fpart32_);
+            xx.tbegin(K_CG_UPD);
+            xx.template pass<5, 1>(n, FcgDir32{A32, z32_, pold, pnew, q32_, sc_, kin}, fpart32_, gt, D);
+            xx.tend(K_CG_UPD);
+            xx.template pass<1, 1>(n, FcgUpd32{e32_, r32_, ro32_, pnew, q32_, sc_, kin}, D, g, RR);
+            std::swap(pold, pnew);
+          }
+          xx.download(&h, sc_, 1);
+          if (h.done || h.inner_done || kin > maxit + 1) break;
+          ibatch = std::max(2, std::min(16, (h.inner_its + 3) / 4));
+        }
+        predict_sw_[std::min(sweep, MAX_SWEEPS)] = std::max(1, h.inner_its + 1);
+        if (h.done) break;
+        total += h.inner_its;
+        last_sweeps_ = sweep + 1;
+        h.its = total;
+        xx.template pass<1, 2>(n, IrResid{A, bw_, x, e32_, r_, sc_}, nullptr, 0, P1);
+        in = P1;
+        if (sweep + 1 > MAX_SWEEPS) break;
+      }
+      xx.for_each(n, IrFold{x, e32_});
+      h.its = total;
     } else if (mixed_) {
       double* F[2] = {U[0], U[0] + 6 * size_t(g)};
       const double* in = P1;
@@ -1041,14 +1234,14 @@ class CgT {
     xx.download(&h, sc_, 1);
     double rr = 0.0;
     for (int q = 0; q < g; ++q) rr = rr + hpart_[q];
-    if (!mixed_) {
+    if (!mixed_ || pc32) {
       xx.tcommit(K_CG_DIR, h.its);
       xx.tcommit(K_CG_UPD, h.its);
     }
     last_its_ = h.its;
     last_ttol_ = h.ttol;
     last_atol_ = h.atol;
-    (fcg_ ? predict_fcg_ : predict_) = std::max(2, h.its + 1);
+    (fcg ? predict_fcg_ : predict_) = std::max(2, h.its + 1);
     if (its) *its = h.its;
     if (resid) *resid = std::sqrt(rr);
     return h.done == 1 ? 0 : -1;
@@ -1104,6 +1297,30 @@ class CgT {
     J.dinv32 = dinv32_;
     return J;
   }
+  // FP32 copy of the operator (coefficients and diagonal) for the preconditioned FP32 sweeps
+  EllOp32 op32(const EllOp& A) {
+    X& xx = *x_;
+    const size_t m = size_t(std::max(1, A.w)) * A.n;
+    if (!coef32_ || op32_of_ != A.coef || op32_w_ != A.w) {
+      if (coef32_) xx.release(coef32_);
+      if (diag32_) xx.release(diag32_);
+      coef32_ = xx.template alloc<float>(m);
+      diag32_ = xx.template alloc<float>(A.n);
+      xx.for_each(A.n, cgk::To32{A.n, A.w, A.coef, A.diag, coef32_, diag32_});
+      op32_of_ = A.coef;
+      op32_w_ = A.w;
+    }
+    EllOp32 o;
+    o.n = A.n;
+    o.w = A.w;
+    o.nbr = A.nbr;
+    o.coef = coef32_;
+    o.diag = diag32_;
+    return o;
+  }
+  float *coef32_ = nullptr, *diag32_ = nullptr;
+  const double* op32_of_ = nullptr;
+  int op32_w_ = -1;
   St4* st_[2] = {nullptr, nullptr};
   St4f* st32_[2] = {nullptr, nullptr};
   float* e32_ = nullptr;  // mixed precision: the sweep's FP32 correction
@@ -1124,6 +1341,11 @@ class CgT {
   double *bw_ = nullptr, *r_ = nullptr, *z_ = nullptr, *pa_ = nullptr, *pb_ = nullptr, *q_ = nullptr;
   bool single_ = false, mixed_ = false;
   FcgPc* fcg_ = nullptr;
+  FcgPc32* pc32_ = nullptr;
+  bool plain_ = false;
+  float *r32_ = nullptr, *ro32_ = nullptr, *z32_ = nullptr, *pa32_ = nullptr, *pb32_ = nullptr, *q32_ = nullptr;
+  double* fpart32_ = nullptr;
+  int fpart32_g_ = 0;
   CgCapture* capture_ = nullptr;
   double *ro_ = nullptr, *fpart_ = nullptr;
   int fpart_g_ = 0;

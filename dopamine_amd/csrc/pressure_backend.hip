@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "amg_impl.h"
 #include "cg_impl.h"
 #include "dm_device.h"
 #include "fvm_pressure_backend.h"
@@ -32,9 +33,12 @@ struct Ctx {
   double *coef = nullptr, *diag = nullptr, *b = nullptr, *sol = nullptr;
   float *coef32 = nullptr, *diag32 = nullptr;
   bool mixed = false;
+  std::string selector;
   std::unique_ptr<dm::mp::LearnedPolicy> learned;  // selector "fcg-learned"
+  std::unique_ptr<dm::AmgPc<dm::DeviceExec>> amg;  // multigrid selectors
   ~Ctx() {
     learned.reset();
+    amg.reset();
     cg.destroy();
     for (void* p : {(void*)nbr, (void*)coef, (void*)diag, (void*)b, (void*)sol, (void*)coef32, (void*)diag32})
       x.release(p);
@@ -43,13 +47,18 @@ struct Ctx {
 };
 std::unique_ptr<Ctx> g_ctx;
 
-// Selectors: the reference's names ("boomeramg", its default, "gamg", "gmres") and "jacobi" run the
-// Jacobi-preconditioned CG; "mixed" runs FP32 sweeps inside FP64 refinement; "fcg-learned" the
-// learned preconditioner in flexible CG. Any other name fails the solve with -11 instead of
-// silently running something else.
+// Selectors: the reference's algebraic-multigrid names ("boomeramg", its default, and "gamg") run
+// the flexible CG with this backend's aggregation multigrid; "gmres" and "jacobi" the
+// Jacobi-preconditioned CG; "mixed" FP32 Jacobi sweeps inside FP64 refinement, "mixed-multigrid"
+// FP32 multigrid-preconditioned sweeps inside FP64 refinement; "fcg-learned" the learned
+// preconditioner in flexible CG. Any other name fails the solve with -11 instead of silently
+// running something else.
 bool known_selector(const std::string& s) {
-  return s == "boomeramg" || s == "gamg" || s == "gmres" || s == "jacobi" || s == "mixed" || s == "fcg-learned";
+  return s == "boomeramg" || s == "gamg" || s == "gmres" || s == "jacobi" || s == "mixed" || s == "fcg-learned" ||
+         s == "mixed-multigrid";
 }
+bool multigrid_selector(const std::string& s) { return s == "boomeramg" || s == "gamg" || s == "mixed-multigrid"; }
+bool mixed_selector(const std::string& s) { return s == "mixed" || s == "mixed-multigrid"; }
 
 int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner, const int* nb_local,
               const int* nb_global, const int* skip, const double* diagonal, const double* coeff, int ref_row) {
@@ -101,7 +110,7 @@ int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner
   ctx->A.coef = ctx->coef;
   ctx->A.diag = ctx->diag;
   ctx->cg.init(&ctx->x, n);
-  if (g_solver == "mixed") {  // FP32 operator copy for the mixed-precision sweeps
+  if (mixed_selector(g_solver)) {  // FP32 operator copy for the mixed-precision sweeps
     std::vector<float> c32(cf.begin(), cf.end()), d32(dg.begin(), dg.end());
     ctx->coef32 = ctx->x.alloc<float>(c32.size());
     ctx->diag32 = ctx->x.alloc<float>(n);
@@ -114,6 +123,20 @@ int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner
     ctx->A32.diag = ctx->diag32;
     ctx->cg.set_mixed(ctx->A32, 1.0e-4);
     ctx->mixed = true;
+  }
+  if (multigrid_selector(g_solver)) {
+    dm::AmgOp op;
+    op.n = n;
+    op.w = std::max(1, w);
+    op.nbr = nbr;
+    op.coef = cf;
+    op.diag = dg;
+    ctx->amg.reset(new dm::AmgPc<dm::DeviceExec>());
+    ctx->amg->build(&ctx->x, op, ctx->nbr, dm::AmgCfg{});
+    if (ctx->mixed)
+      ctx->cg.set_pc32(ctx->amg.get());
+    else
+      ctx->cg.set_fcg(ctx->amg.get());
   }
   if (g_solver == "fcg-learned") {
     // options: -learn_collect N, -learn_capacity N, -learn_iters N, -learn_batch N, -learn_lr X,
@@ -133,6 +156,7 @@ int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner
     }
     ctx->learned.reset(new dm::mp::LearnedPolicy(&ctx->x, &ctx->cg, ctx->A, cfg));
   }
+  ctx->selector = g_solver;
   ctx->n_local = n;
   ctx->n_global = n_global;
   ctx->n_faces = n_faces;
@@ -196,8 +220,7 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
     int ref_row = reference_cell - 1;
     if (ref_row < 0 || ref_row >= n_cells_global) ref_row = 0;
     if (!g_ctx || g_ctx->n_local != n_cells_local || g_ctx->n_global != n_cells_global || g_ctx->n_faces != n_faces ||
-        g_ctx->ref_row != ref_row || g_ctx->mixed != (g_solver == "mixed") ||
-        bool(g_ctx->learned) != (g_solver == "fcg-learned")) {
+        g_ctx->ref_row != ref_row || g_ctx->selector != g_solver) {
       g_ctx.reset();
       const int rc = build_ctx(n_cells_local, n_cells_global, n_faces, local_to_global, face_owner,
                                face_neighbor_local, face_neighbor_global, skip_face, diagonal, face_coeff, ref_row);
@@ -236,7 +259,12 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
           return c.learned ? c.learned->solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r)
                            : c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r);
         },
-        [&](int mi, int* it, double* r) { return c.cg.solve(c.A, b, x, rtol, atol, nullptr, mi, ns, it, r); },
+        [&](int mi, int* it, double* r) {
+          c.cg.set_plain(true);  // the retry runs the Jacobi passes
+          const int q = c.cg.solve(c.A, b, x, rtol, atol, nullptr, mi, ns, it, r);
+          c.cg.set_plain(false);
+          return q;
+        },
         [&] { c.x.zero(x, size_t(n)); }, [&] { return atol; }, &its, &res, &total, &retried);
     if (!g_io_on_device) c.x.download(solution, c.sol, n);
     c.x.sync();

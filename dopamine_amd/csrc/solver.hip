@@ -81,7 +81,8 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
     if (p->cg_variant != 0 && p->cg_variant != 1) throw std::runtime_error("cg_variant must be 0 or 1");
     if (p->boussinesq && (p->t_scheme < 1 || p->t_scheme > 5 || p->t_scheme == 4))
       throw std::runtime_error("temperature scheme must be VANLEER_TVD (1), KOREN_TVD (2), CDS (3) or QUICK (5)");
-    if (p->precond != 0 && p->precond != 1) throw std::runtime_error("precond must be 0 (Jacobi) or 1 (learned)");
+    if (p->precond < 0 || p->precond > 2)
+      throw std::runtime_error("precond must be 0 (Jacobi), 1 (learned) or 2 (aggregation multigrid)");
     if (p->precond == 1 && p->precision == 1)
       throw std::runtime_error("the learned preconditioner runs in the FP64 flexible CG only: precision 1 with "
                                "precond 1 is not supported");

@@ -22,6 +22,7 @@
 #include <memory>
 #include <vector>
 
+#include "amg_impl.h"
 #include "cg_impl.h"
 #include "dopamine_hip.h"
 #include "kernels.h"
@@ -178,6 +179,8 @@ class SolverT {
   }
   CgT<X> cg;
   std::unique_ptr<PressurePolicy> policy;  // learned preconditioner (device only); null: CG passes
+  std::unique_ptr<AmgPc<X>> amg;           // aggregation multigrid (precond 2); null: none
+  static constexpr int PRECOND_MULTIGRID = 2;
   std::vector<void*> owned;
   std::vector<int> dir_rows_h;
   std::vector<double> dir_vals_h;
@@ -683,6 +686,20 @@ class SolverT {
     cg.init(&x, nc);
     cg.set_single_reduction(p.cg_variant == 1);
     if (p.precision == 1) cg.set_mixed(A32, mixed_tau());
+    if (p.precond == PRECOND_MULTIGRID) {  // aggregation multigrid in the flexible CG
+      AmgOp op;
+      op.n = nc;
+      op.w = std::max(1, wP);
+      op.nbr = anbr;
+      op.coef = acoef;
+      op.diag = diag;
+      amg.reset(new AmgPc<X>());
+      amg->build(&x, op, A.nbr, AmgCfg{});
+      if (p.precision == 1)
+        cg.set_pc32(amg.get());
+      else
+        cg.set_fcg(amg.get());
+    }
     apply_all_bc();
     x.sync();
   }
@@ -982,7 +999,9 @@ class SolverT {
           return q;
         },
         [&](int mi, int* it, double* r) {
+          cg.set_plain(true);  // the retry runs the Jacobi passes, whatever preconditioner is set
           const int q = cg.solve(A, cg_b, P, prm.p_tol, atol, atol_dev, mi, ns, it, r);
+          cg.set_plain(false);
           sweeps += cg.last_sweeps();
           return q;
         },

@@ -11,6 +11,7 @@ import numpy as np
 SCHEME_IDS = {"VANLEER_TVD": 1, "VANLEER": 1, "KOREN_TVD": 2, "KOREN": 2, "CDS": 3, "CENTRAL": 3, "KEP": 4,
               "SKEW": 4, "QUICK": 5}
 SGS_IDS = {"NONE": 0, "VREMAN": 1, "SMAGORINSKY": 2, "AMD": 3}
+PRECOND_IDS = {"JACOBI": 0, "LEARNED": 1, "MULTIGRID": 2}
 WALL_MODEL_IDS = {"loglaw": 0, "none": 1, "most": 2}
 
 MAX_SCALARS = 4
@@ -69,8 +70,10 @@ class SolverParams:
     Pr_t: float = 0.85
     temperature_scheme: str = "VANLEER_TVD"
     patch_T: dict = field(default_factory=dict)
-    # pressure preconditioner: "JACOBI", or "LEARNED" (flexible CG with the message-passing network,
-    # trained on residuals of the first learn_collect Jacobi solves; Jacobi CG is the fallback)
+    # pressure preconditioner: "JACOBI", "LEARNED" (flexible CG with the message-passing network,
+    # trained on residuals of the first learn_collect Jacobi solves; Jacobi CG is the fallback) or
+    # "MULTIGRID" (flexible CG with the FP32 aggregation-multigrid V-cycle; in mixed precision the
+    # FP32 sweeps are that flexible CG)
     precond: str = "JACOBI"
     learn_collect: int = 600
     learn_capacity: int = 128
@@ -120,8 +123,8 @@ class SolverParams:
             raise ValueError(f"temperature_scheme {self.temperature_scheme!r} not supported")
         if self.wall_model not in WALL_MODEL_IDS:
             raise ValueError(f"wall_model {self.wall_model!r} not supported (loglaw, most, none)")
-        if self.precond.upper() not in ("JACOBI", "LEARNED"):
-            raise ValueError(f"precond {self.precond!r} not supported (JACOBI, LEARNED)")
+        if self.precond.upper() not in PRECOND_IDS:
+            raise ValueError(f"precond {self.precond!r} not supported (JACOBI, LEARNED, MULTIGRID)")
         if self.constant_mass_flux_forcing and any(f != 0.0 for f in self.f_const):
             raise ValueError("f_const must be 0 with constant_mass_flux_forcing")
         if not 0 <= self.n_scalars <= MAX_SCALARS:

@@ -10,6 +10,7 @@ import ctypes
 import numpy as np
 
 from ._lib import lib
+from . import params as P
 from .params import WALL_MODEL_IDS
 
 _P = ctypes.POINTER
@@ -74,7 +75,7 @@ def build_abi(mesh, params, precision=0):
                 params.p_tol, params.p_max_iter, params.cfl_max, params.dt, arr(pv, _d), arr(uv.reshape(-1), _d),
                 int(precision), int(params.cg_variant), params.c_amd, int(params.amd_wall_gradient),
                 int(params.boussinesq), params.alpha_T, params.beta, params.T_ref, (_d * 3)(*params.g), params.Pr_t,
-                params.t_scheme_id(), arr(tv, _d), 1 if params.precond.upper() == "LEARNED" else 0,
+                params.t_scheme_id(), arr(tv, _d), P.PRECOND_IDS[params.precond.upper()],
                 params.learn_collect, params.learn_capacity, params.learn_iters, params.learn_batch,
                 params.learn_seed, params.learn_lr, params.z0_m, params.z0_h, params.T_surface, params.L_obukhov,
                 int(params.compute_L), params.projection_id(), params.cell_projection_tol, *params.forcing_args(),

@@ -21,8 +21,10 @@ int fvm_petsc_backend_available(void);
 int fvm_petsc_backend_initialize(void);
 /* Releases the cached solve context; 0 on success (pressure_backend_petsc.c:618-646). */
 int fvm_petsc_backend_finalize(void);
-/* Preconditioner selector (pressure_backend_petsc.c:534-539). "boomeramg", "gamg", "gmres" and
- * "jacobi" run the Jacobi CG of this backend; "mixed" selects
+/* Preconditioner selector (pressure_backend_petsc.c:534-539). The reference's algebraic-multigrid
+ * names "boomeramg" (its default) and "gamg" run the flexible CG preconditioned by this backend's
+ * FP32 aggregation-multigrid V-cycle; "gmres" and "jacobi" the Jacobi CG; "mixed-multigrid" FP32
+ * multigrid-preconditioned flexible-CG sweeps inside FP64 iterative refinement; "mixed" selects
  * FP32-operator Jacobi-CG sweeps inside FP64 iterative refinement (same stopping test on the FP64
  * residual); "fcg-learned" selects the learned preconditioner (dopamine_mpnn.h): the first solves
  * run Jacobi CG and keep residual snapshots, the network is then trained on them and later solves

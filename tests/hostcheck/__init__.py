@@ -45,6 +45,10 @@ def lib():
         L.hc_solver_error.restype = ctypes.c_char_p
         L.hc_solver_error.argtypes = [ctypes.c_void_p]
         _pi = _P(ctypes.c_int)
+        L.hc_amg_solve.argtypes = [ctypes.c_int, ctypes.c_int, _pi, _P(_d), _P(_d), _P(_d), _P(_d), _d, _d,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(_d), _pi, _P(_d), _pi]
+        L.hc_amg_vcycle.argtypes = [ctypes.c_int, ctypes.c_int, _pi, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d), _pi,
+                                    _pi]
         L.hc_cg_solve_retry.argtypes = [ctypes.c_int, ctypes.c_int, _pi, _P(_d), _P(_d), _P(_d), _P(_d), _d, _d,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _pi, _P(_d), _pi, _pi]
         _lib = L
@@ -176,3 +180,47 @@ def cg_solve_retry(nbr, coef, diag, b, x0, rtol, atol, maxit, nullspace=False, s
                                  int(single), int(mixed), ctypes.byref(its), ctypes.byref(resid),
                                  ctypes.byref(total), ctypes.byref(retried))
     return rc, x, its.value, resid.value, total.value, retried.value
+
+
+AMG_DEFAULTS = (0.08, 2.0 / 3.0, 1.5, 400)
+
+
+def amg_solve(nbr, coef, diag, b, rtol, atol=1e-50, maxit=2000, nullspace=False, mixed=False, cfg=AMG_DEFAULTS):
+    """The product's flexible CG preconditioned by the aggregation multigrid (host-check build);
+    returns (rc, x, its, resid, level sizes, refinement sweeps)."""
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    w, n = nbr.shape
+    its = ctypes.c_int(0)
+    resid = _d(0.0)
+    info = (ctypes.c_int * 16)()
+    c = (_d * 4)(*cfg)
+    pi, pd = _P(ctypes.c_int), _P(_d)
+    rc = lib().hc_amg_solve(n, w, nbr.ctypes.data_as(pi), coef.ctypes.data_as(pd), diag.ctypes.data_as(pd),
+                            b.ctypes.data_as(pd), x.ctypes.data_as(pd), rtol, atol, maxit, int(nullspace), int(mixed),
+                            c, ctypes.byref(its), ctypes.byref(resid), info)
+    if rc == -100:
+        raise RuntimeError(lib().hc_last_error().decode())
+    return rc, x, its.value, resid.value, list(info[1:1 + info[0]]), info[15]
+
+
+def amg_vcycle(nbr, coef, diag, r, cfg=AMG_DEFAULTS):
+    """One V-cycle z = M r (host-check build); returns (z, level-0 aggregates, level sizes)."""
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    w, n = nbr.shape
+    agg = np.zeros(n, dtype=np.int32)
+    info = (ctypes.c_int * 16)()
+    c = (_d * 4)(*cfg)
+    pi, pd = _P(ctypes.c_int), _P(_d)
+    rc = lib().hc_amg_vcycle(n, w, nbr.ctypes.data_as(pi), coef.ctypes.data_as(pd), diag.ctypes.data_as(pd),
+                             r.ctypes.data_as(pd), z.ctypes.data_as(pd), c, agg.ctypes.data_as(pi), info)
+    if rc != 0:
+        raise RuntimeError(lib().hc_last_error().decode())
+    return z, agg, list(info[1:1 + info[0]])

@@ -117,6 +117,83 @@ int hc_cg_solve_retry(int n, int w, const int* nbr, const double* coef, const do
   cg.destroy();
   return rc;
 }
+// The product's flexible CG with the aggregation multigrid (AmgPc) on an ELL operator: FP64
+// flexible CG (mixed = 0) or FP32 sweeps in FP64 refinement (mixed = 1). cfg = {theta, omega,
+// alpha, coarse_max}; info receives {levels, sizes of up to 14 levels...}.
+int hc_amg_solve(int n, int w, const int* nbr, const double* coef, const double* diag, const double* b, double* xv,
+                 double rtol, double atol, int maxit, int nullspace, int mixed, const double cfg[4], int* its,
+                 double* resid, int* info) {
+  try {
+    dm::HostExec ex;
+    dm::AmgOp op;
+    op.n = n;
+    op.w = w;
+    op.nbr.assign(nbr, nbr + size_t(w) * n);
+    op.coef.assign(coef, coef + size_t(w) * n);
+    op.diag.assign(diag, diag + n);
+    dm::AmgCfg c;
+    c.theta = cfg[0];
+    c.omega = cfg[1];
+    c.alpha = cfg[2];
+    c.coarse_max = int(cfg[3]);
+    dm::AmgPc<dm::HostExec> amg;
+    amg.build(&ex, op, nbr, c);
+    info[0] = amg.levels();
+    for (int l = 0; l < amg.levels() && l < 14; ++l) info[1 + l] = amg.sizes()[l];
+    dm::CgT<dm::HostExec> cg;
+    cg.init(&ex, n);
+    if (mixed) {
+      cg.set_mixed(dm::EllOp32{}, 1.0e-4);
+      cg.set_pc32(&amg);
+    } else {
+      cg.set_fcg(&amg);
+    }
+    dm::EllOp A;
+    A.n = n;
+    A.w = w;
+    A.nbr = nbr;
+    A.coef = coef;
+    A.diag = diag;
+    const int rc = cg.solve(A, b, xv, rtol, atol, nullptr, maxit, nullspace != 0, its, resid);
+    info[15] = cg.last_sweeps();
+    cg.destroy();
+    return rc;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -100;
+  }
+}
+// One V-cycle z = M r of the aggregation multigrid (FP32 inside, FP64 in and out) and its
+// hierarchy: agg_out receives the level-0 aggregate of every cell (n entries).
+int hc_amg_vcycle(int n, int w, const int* nbr, const double* coef, const double* diag, const double* r, double* z,
+                  const double cfg[4], int* agg_out, int* info) {
+  try {
+    dm::HostExec ex;
+    dm::AmgOp op;
+    op.n = n;
+    op.w = w;
+    op.nbr.assign(nbr, nbr + size_t(w) * n);
+    op.coef.assign(coef, coef + size_t(w) * n);
+    op.diag.assign(diag, diag + n);
+    dm::AmgCfg c;
+    c.theta = cfg[0];
+    c.omega = cfg[1];
+    c.alpha = cfg[2];
+    c.coarse_max = int(cfg[3]);
+    std::vector<int> agg;
+    dm::amg_aggregate(op, c.theta, agg);
+    std::copy(agg.begin(), agg.end(), agg_out);
+    dm::AmgPc<dm::HostExec> amg;
+    amg.build(&ex, op, nbr, c);
+    info[0] = amg.levels();
+    for (int l = 0; l < amg.levels() && l < 14; ++l) info[1 + l] = amg.sizes()[l];
+    amg.vcycle(r, z);
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -100;
+  }
+}
 void hc_solver_nonortho_info(void* h, int32_t out[3]) {
   auto* s = static_cast<Solver*>(h);
   out[0] = s->skip_nonortho; out[1] = s->skip_grad_p; out[2] = s->n_corr;

@@ -1,0 +1,179 @@
+"""The aggregation-multigrid preconditioner on the CPU: the NumPy statement (oracle/amg.py) pinned
+by what the method fixes, and the product's implementation (host-check build of amg_impl.h, the
+same code the GPU runs) against it."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle.amg as om
+from dopamine_amd import cases
+from hostcheck import AMG_DEFAULTS, amg_solve, amg_vcycle
+
+
+def chain(n, c=1.0, dirichlet=0.0):
+    """1-D chain Laplacian as ELL: slot 0 the left neighbour, slot 1 the right one."""
+    nbr = -np.ones((2, n), dtype=np.int64)
+    coef = np.zeros((2, n))
+    nbr[0, 1:] = np.arange(n - 1)
+    nbr[1, :-1] = np.arange(1, n)
+    coef[nbr >= 0] = c
+    diag = (nbr >= 0).sum(axis=0) * c
+    diag[0] += dirichlet
+    return nbr, coef, diag.astype(float)
+
+
+def operator(case):
+    """The pressure operator of a case as ELL arrays (as the solver assembles it)."""
+    import oracle
+    mesh = case.make_mesh()
+    o = oracle.Solver(mesh, case.params)
+    own, nb, skip = o.int_array("ps_owner").copy(), o.int_array("ps_nb").copy(), o.int_array("ps_skip").copy()
+    coeff, diag = o.array("ps_coeff").copy(), o.array("ps_diag").copy()
+    n = mesh.n_cells
+    keep = (skip == 0) & (nb >= 0)
+    own, nb, coeff = own[keep], nb[keep], coeff[keep]
+    rows, cols, vals = np.concatenate([own, nb]), np.concatenate([nb, own]), np.concatenate([coeff, coeff])
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    cnt = np.bincount(rows, minlength=n)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    slot = np.arange(rows.size) - start[rows]
+    nbr = -np.ones((cnt.max(), n), dtype=np.int64)
+    cf = np.zeros((cnt.max(), n))
+    nbr[slot, rows] = cols
+    cf[slot, rows] = vals
+    ns = bool(np.all(np.abs(diag - np.bincount(rows, vals, n)) <= 1e-10 * diag))
+    return nbr, cf, diag, ns
+
+
+OPERATORS = {
+    "tgv_skew": lambda: cases.regression_cases()["tgv_skew"],   # periodic skewed hexes, singular
+    "hydro": lambda: cases.regression_cases()["hydro"],         # outlet rows: non-singular
+    "channel": lambda: cases.small_channel(16, 12, 8),          # stretched walls, singular
+    "tets": lambda: cases.cavity_tets(6, rcm=True),             # jittered tets, singular
+}
+
+
+# ---- the statement pinned by what the method fixes
+
+def test_aggregation_of_a_chain_known_answer():
+    """Equal couplings, cells visited in order. Six cells: cell 0 roots {0, 1}; cell 2 is no root
+    (its neighbour 1 is taken); cell 3 roots {2, 3, 4}; cell 5 (neighbour 4 taken) joins
+    aggregate 1 in phase 2. Seven cells: the same, then cell 6 roots {5, 6}."""
+    nbr, coef, diag = chain(6)
+    agg, na = om.aggregate(nbr, coef, diag, 0.08)
+    assert na == 2 and list(agg) == [0, 0, 1, 1, 1, 1]
+    nbr2, coef2, diag2 = chain(7)
+    agg2, na2 = om.aggregate(nbr2, coef2, diag2, 0.08)
+    assert na2 == 3 and list(agg2) == [0, 0, 1, 1, 1, 2, 2]
+
+
+def test_weak_couplings_are_not_aggregated_together():
+    """A chain whose middle coupling is 1e-3 of the others: with theta = 0.08 no aggregate spans it."""
+    nbr, coef, diag = chain(8)
+    coef[1, 3] = coef[0, 4] = 1e-3
+    diag = np.zeros(8)
+    for k in range(2):
+        diag += np.where(nbr[k] >= 0, coef[k], 0.0)
+    agg, _ = om.aggregate(nbr, coef, diag, 0.08)
+    assert set(agg[:4]).isdisjoint(set(agg[4:]))
+
+
+@pytest.mark.parametrize("name", ["tgv_skew", "hydro"])
+def test_statement_invariants(name):
+    nbr, coef, diag, ns = operator(OPERATORS[name]())
+    H = om.Hierarchy(nbr, coef, diag)
+    A = H.A[0]
+    n = A.shape[0]
+    # every cell in one aggregate, every aggregate connected
+    agg = H.aggs[0]
+    assert agg.min() == 0 and agg.max() == H.sizes[1] - 1
+    Afine = A.tolil()
+    for a in range(min(H.sizes[1], 200)):
+        cells = np.flatnonzero(agg == a)
+        sub = A[cells][:, cells]
+        ncomp, _ = __import__("scipy.sparse.csgraph", fromlist=["x"]).connected_components(sub != 0, directed=False)
+        assert ncomp == 1, a
+    # Galerkin: the coarse operator is P^T A P and keeps the row-sum (null space) structure
+    P = H.P[0]
+    Ac = (P.T @ A @ P).toarray()
+    assert np.allclose(H.A[1].toarray(), Ac, rtol=0, atol=1e-12 * np.abs(Ac).max())
+    assert np.allclose(Ac, Ac.T)
+    if ns:
+        assert np.abs(Ac.sum(axis=1)).max() <= 1e-10 * np.abs(np.diag(Ac)).max()
+    # the cycle is a symmetric positive (semi-)definite linear map
+    rng = np.random.default_rng(1)
+    u, v = rng.standard_normal(n), rng.standard_normal(n)
+    if ns:
+        u -= u.mean()
+        v -= v.mean()
+    Mu, Mv = H.vcycle(u), H.vcycle(v)
+    assert abs(Mu @ v - u @ Mv) <= 1e-10 * abs(Mu @ v)
+    assert u @ Mu > 0 and v @ Mv > 0
+
+
+def test_two_level_without_smoothing_is_the_galerkin_projection():
+    """omega = 0 leaves M = alpha P A_c^-1 P^T: M A is alpha times an A-orthogonal projection,
+    (M A)^2 = alpha M A."""
+    nbr, coef, diag = chain(40, dirichlet=1.0)
+    H = om.Hierarchy(nbr, coef, diag, omega=0.0, alpha=1.0, coarse_max=1000)
+    assert len(H.A) == 2
+    A = H.A[0].toarray()
+    M = np.column_stack([H.vcycle(e) for e in np.eye(40)])
+    MA = M @ A
+    assert np.allclose(MA @ MA, MA, atol=1e-10)
+    P = H.P[0].toarray()
+    assert np.allclose(M, P @ np.linalg.inv(P.T @ A @ P) @ P.T, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["tgv_skew", "hydro"])
+def test_statement_fcg_solves_the_system(name):
+    nbr, coef, diag, ns = operator(OPERATORS[name]())
+    H = om.Hierarchy(nbr, coef, diag)
+    A = H.A[0]
+    b = np.random.default_rng(2).standard_normal(diag.size)
+    if ns:
+        b -= b.mean()
+    x, its = om.fcg(A, b, H.vcycle, rtol=1e-10, nullspace=ns)
+    assert its < 40
+    if ns:
+        import scipy.sparse as sp
+        xd = spla.spsolve((A + sp.coo_matrix(([1.0], ([0], [0])), shape=A.shape)).tocsc(), b)
+        xd -= xd.mean()
+        x -= x.mean()
+    else:
+        xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+# ---- the product (host-check build of the same code the GPU runs) against the statement
+
+@pytest.mark.parametrize("name", list(OPERATORS))
+def test_product_hierarchy_and_cycle_match_the_statement(name):
+    nbr, coef, diag, ns = operator(OPERATORS[name]())
+    H = om.Hierarchy(nbr, coef, diag, *AMG_DEFAULTS)
+    r = np.random.default_rng(3).standard_normal(diag.size)
+    z, agg, sizes = amg_vcycle(nbr, coef, diag, r)
+    assert sizes == H.sizes
+    assert np.array_equal(agg, H.aggs[0])
+    zo = H.vcycle(r)
+    err = np.linalg.norm(z - zo) / np.linalg.norm(zo)
+    assert err < 2e-5, err  # FP32 cycle against the float64 statement
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("name", list(OPERATORS))
+def test_product_multigrid_cg_reaches_the_fp64_tolerance(name, mixed):
+    nbr, coef, diag, ns = operator(OPERATORS[name]())
+    A = om.ell_to_csr(nbr, coef, diag)
+    b = np.random.default_rng(4).standard_normal(diag.size)
+    if ns:
+        b -= b.mean()
+    rc, x, its, res, sizes, sweeps = amg_solve(nbr, coef, diag, b, 1e-8, nullspace=ns, mixed=mixed)
+    H = om.Hierarchy(nbr, coef, diag, *AMG_DEFAULTS)
+    _, its_o = om.fcg(A, b, H.vcycle, rtol=1e-8, nullspace=ns)
+    assert rc == 0
+    assert res == pytest.approx(np.linalg.norm(b - A @ x), rel=1e-6)
+    assert res <= 1e-8 * np.linalg.norm(b)
+    assert its <= its_o + 3 + (4 if mixed else 0)  # FP32 sweeps restart the Krylov space once
+    assert (sweeps > 0) == mixed

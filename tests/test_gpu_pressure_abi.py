@@ -60,6 +60,8 @@ def test_fvm_petsc_solve_pressure_matches_oracle_and_direct_solve(name, ksp_type
     assert L.fvm_petsc_backend_initialize() == 0
     L.fvm_petsc_insert_option.argtypes = [ctypes.c_char_p]
     L.fvm_petsc_insert_option(f"-ksp_type {ksp_type}".encode())
+    L.fvm_petsc_set_pressure_solver.argtypes = [ctypes.c_char_p]
+    L.fvm_petsc_set_pressure_solver(b"jacobi")  # the oracle's recurrence
     rng = np.random.default_rng(7)
     b = rng.standard_normal(n)
     n_dir = int(np.sum(np.abs(diag - np.bincount(own[skip == 0], coeff[skip == 0], n)
@@ -87,7 +89,71 @@ def test_fvm_petsc_solve_pressure_matches_oracle_and_direct_solve(name, ksp_type
         xd = spla.spsolve(A.tocsc(), b)
         xg = x
     assert np.linalg.norm(xg - xd) / np.linalg.norm(xd) < 1e-8
+    L.fvm_petsc_set_pressure_solver(b"boomeramg")
     assert L.fvm_petsc_backend_finalize() == 0
+
+
+@pytest.mark.parametrize("selector", ["boomeramg", "gamg", "mixed-multigrid"])
+@pytest.mark.parametrize("name", ["tgv_skew", "hydrowall", "hydro"])
+def test_multigrid_selectors_solve_with_far_fewer_iterations(name, selector):
+    """The reference's AMG selectors (its default "boomeramg") run the flexible CG with the FP32
+    aggregation multigrid: the true FP64 residual meets rtol, the solution matches a direct sparse
+    solve, and the iteration count is that of the NumPy statement of the same method (oracle/amg.py,
+    within 3 + FP32 sweep restarts) and far below Jacobi CG's."""
+    import oracle.amg as om
+    mesh, own, nb, skip, coeff, diag = _operator(name)
+    n = mesh.n_cells
+    L = lib()
+    assert L.fvm_petsc_backend_initialize() == 0
+    L.fvm_petsc_set_pressure_solver.argtypes = [ctypes.c_char_p]
+    L.fvm_petsc_set_pressure_solver(selector.encode())
+    b = np.random.default_rng(13).standard_normal(n)
+    n_dir = 0 if name != "hydro" else 1
+    if n_dir == 0:
+        b -= b.mean()
+    rtol = 1e-10
+    rc, x, its, res = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), rtol, 1e-300, n_dir)
+    keep = skip == 0
+    A = sp.coo_matrix((np.concatenate([diag, -coeff[keep], -coeff[keep]]),
+                       (np.concatenate([np.arange(n), own[keep], nb[keep]]),
+                        np.concatenate([np.arange(n), nb[keep], own[keep]]))), shape=(n, n)).tocsr()
+    _, its_j, _, _ = oracle.cg_jacobi(own[keep], nb[keep], coeff[keep], diag, b, np.zeros(n), rtol, 1e-300, 5000,
+                                      n_dir == 0)
+    ell = _ell(n, own[keep], nb[keep], coeff[keep])
+    H = om.Hierarchy(ell[0], ell[1], diag)
+    _, its_o = om.fcg(A, b, H.vcycle, rtol=rtol, nullspace=n_dir == 0)
+    print(f"[gpu] {selector} {name}: rc {rc} its {its} (statement {its_o}, Jacobi {its_j}) resid {res:.3e}",
+          flush=True)
+    assert rc == 0
+    assert res <= rtol * np.linalg.norm(b) * 1.001
+    assert its <= its_o + 3 + (8 if selector.startswith("mixed") else 0)
+    assert 3 * its < its_j
+    if n_dir == 0:
+        xd = spla.spsolve((A + sp.coo_matrix((np.ones(1), ([0], [0])), shape=(n, n))).tocsc(), b)
+        xd, xg = xd - xd.mean(), x - x.mean()
+    else:
+        xd, xg = spla.spsolve(A.tocsc(), b), x
+    assert np.linalg.norm(xg - xd) / np.linalg.norm(xd) < 1e-7
+    L.fvm_petsc_set_pressure_solver(b"boomeramg")
+    assert L.fvm_petsc_backend_finalize() == 0
+
+
+def _ell(n, own, nb, coeff):
+    """ELL arrays of the operator's face list in the backend's slot order (faces in order, owner
+    side then neighbour side)."""
+    rows = [[] for _ in range(n)]
+    for o, j, c in zip(own, nb, coeff):
+        if not np.isfinite(c) or c == 0.0:
+            continue
+        rows[o].append((j, c))
+        rows[j].append((o, c))
+    w = max(len(r) for r in rows)
+    nbr = -np.ones((w, n), dtype=np.int64)
+    cf = np.zeros((w, n))
+    for i, r in enumerate(rows):
+        for k, (j, c) in enumerate(r):
+            nbr[k, i], cf[k, i] = j, c
+    return nbr, cf
 
 
 @pytest.mark.parametrize("name", ["tgv_skew", "hydrowall", "hydro"])
