@@ -1,25 +1,30 @@
 #!/usr/bin/env python3
 """Throughput of the RK3W fractional step on one MI355X: cell-updates/s per RK3 step.
 
-Workload: turbulent channel 2pi x 2 x pi, 128 x 96 x 64 = 786,432 hex cells, tanh wall-normal
-stretching (factor 2), periodic in x and z, Re_tau 395 (nu = 1/395, f_x = 1), KEP advection,
-Vreman SGS, log-law wall model (kappa 0.41, B 5.2), laminar parabolic profile with a seeded 10%
-uniform perturbation, pressure to relative residual 1e-8, fixed dt = 1e-3.
+Default workload (the full stack): a 1000 x 1000 x 200 m box meshed 250 x 210 x 160 with 40 seeded
+cuboid obstacles (heights 20-120 m) whose cells are removed (7,943,987 hex cells), periodic sides,
+slip top, MOST rough walls (z0 = 0.1 m) over a surface 2 K warmer than the air, AMD SGS, Boussinesq
+temperature and two passive scalars by van Leer TVD, KEP momentum, Coriolis f = 1e-4, a constant
+pressure gradient, uniform 5 m/s start with seeded temperature noise, dt = 0.2 s, pressure to
+relative residual 1e-8.
 
 A step is one solver_advance (three RK3W stages with their pressure solves) with every field
-resident in HBM. By default the measured path is the mixed-precision pressure solve (FP32 CG
-sweeps inside FP64 iterative refinement, same FP64 residual test) and the plain FP64 Jacobi-CG
-path is timed the same way beside it (`plain_fp64`, `speedup_vs_plain_fp64`); `--precond
-learned` measures flexible CG with the learned preconditioner instead. `value` = cells x timed
-steps / wall time of the timed steps. The roofline entry
-is for the dominant kernel (the CG iteration pass), from HIP events that its dispatches stamp on
-the solver's stream over a further run of the same number of steps; the CPU baseline is the FP64
-C restatement of the same algorithm run single-threaded on one step of the same workload.
-`--workload tets` measures the unstructured-gather workload instead: the lid-driven cavity on
-998,250 jittered, shuffled tets renumbered by reverse Cuthill-McKee (Koren TVD, AMD SGS,
-non-orthogonal correction).
+resident in HBM. The measured path is mixed precision with the aggregation-multigrid
+preconditioner: FP32 flexible-CG sweeps preconditioned by the FP32 multigrid V-cycle inside FP64
+iterative refinement, the same FP64 residual test. The plain FP64 path (the reference's plain
+method: Jacobi-preconditioned CG, FP64 throughout) is timed the same way in the same invocation
+(`plain_fp64`, `speedup_vs_plain_fp64`). `value` = cells x timed steps / wall time of the timed
+steps. The roofline entry is for the dominant kernel of the measured path, from HIP events that its
+dispatches stamp on the solver's stream over a further run of the same number of steps; the CPU
+baseline is the FP64 C restatement of the same algorithm (Jacobi CG on OpenMP threads) on a
+bounded sample of the workload.
+
+`--workload channel` measures the turbulent channel 128 x 96 x 64 (Re_tau 395, KEP + Vreman +
+log-law wall model); `--workload tets` the lid-driven cavity on 998,250 jittered, shuffled tets
+renumbered by reverse Cuthill-McKee (Koren TVD, AMD SGS, non-orthogonal correction).
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -29,29 +34,32 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s per RK3 step at p-residual 1e-8: learned/mixed vs plain FP64"
+WORKLOAD_KEYS = {"full": "full stack 250x210x160", "channel": "channel 128x96x64", "tets": "tets 6x55^3 rcm"}
 
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("channel", "tets"), default="channel",
-                    help="channel: the turbulent channel (the line's default workload); tets: the lid-driven "
-                         "cavity on 6 x 55^3 jittered, shuffled tets renumbered by reverse Cuthill-McKee")
-    ap.add_argument("--mesh", type=int, nargs=3, default=(128, 96, 64), metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("full", "channel", "tets"), default="full",
+                    help="full: the full-stack obstacle box (the line's workload); channel: the turbulent channel; "
+                         "tets: the lid-driven cavity on 6 x 55^3 jittered, shuffled, RCM-renumbered tets")
+    ap.add_argument("--mesh", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
+                    help="mesh of the full-stack or channel workload (defaults 250 210 160 and 128 96 64)")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed",
-                    help="mixed: FP32 CG sweeps inside FP64 iterative refinement (the default measured path); "
-                         "fp64: the plain method")
+                    help="mixed: FP32 CG sweeps inside FP64 iterative refinement (the measured path); fp64: FP64 CG")
+    ap.add_argument("--precond", choices=("multigrid", "jacobi", "learned"), default="multigrid",
+                    help="multigrid: FP32 aggregation-multigrid V-cycle in flexible CG (the measured path); jacobi: "
+                         "Jacobi CG; learned: flexible CG with the learned message-passing preconditioner (FP64 "
+                         "only; the warm-up steps' Jacobi solves train it)")
     ap.add_argument("--no-plain", action="store_true",
-                    help="skip the plain FP64 run that the line reports beside a mixed/learned measurement")
+                    help="skip the plain FP64 Jacobi-CG run that the line reports beside the measured path")
     ap.add_argument("--cg-variant", type=int, choices=(0, 1), default=1,
-                    help="pressure CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction")
-    ap.add_argument("--precond", choices=("jacobi", "learned"), default="jacobi",
-                    help="learned: the warm-up steps' Jacobi solves train the message-passing network, the "
-                         "timed steps run flexible CG with it")
+                    help="Jacobi CG recurrence: 0 two passes per iteration (KSPCG's), 1 single reduction")
     ap.add_argument("--learn-iters", type=int, default=300, help="Adam steps of the learned preconditioner")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="RK3 steps of the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline (0: the host's cores, at most 16; -1: skip it)")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without the per-launch event pairs (no roofline entry)")
     return ap.parse_args()
@@ -61,19 +69,44 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def n_op_faces(mesh):
+    """Faces of the pressure operator: interior faces plus one per periodic pair."""
+    return mesh.n_int_faces + int(np.count_nonzero(mesh.periodic_pair >= 0)) // 2
+
+
 def cg_pass_bytes(mesh, variant, mixed):
-    """Algorithmic HBM bytes of the CG's first pass per iteration: one coefficient and two int32
-    indices per operator face, and per cell:
+    """Algorithmic HBM bytes of the Jacobi CG's first pass per iteration: one coefficient and two
+    int32 indices per operator face, and per cell:
     FP64 two-pass recurrence: diag, z, p_old read, p, q written (5 x 8 B);
     FP64 single-reduction recurrence (the whole iteration): diag, p, s, x, r, w read and p, s, x,
     r, w written (11 x 8 B);
     mixed precision (FP32 single-reduction sweep): 1/diag, p, s, r, w and the correction e read
     and p, s, r, w, e written, all FP32 (11 x 4 B); FP32 coefficients."""
-    nfp = mesh.n_int_faces + int(np.count_nonzero(mesh.periodic_pair >= 0)) // 2
+    nfp = n_op_faces(mesh)
     if mixed:
         return 44 * mesh.n_cells + 12 * nfp, nfp
     per_cell = 88 if variant == 1 else 40
     return per_cell * mesh.n_cells + 16 * nfp, nfp
+
+
+def multigrid_bytes(mesh, mixed):
+    """Algorithmic HBM bytes per launch of the multigrid path's finest-level kernels (FP32 operator
+    values and vectors when mixed, FP64 vectors otherwise), with nf = operator faces:
+    cg_dir, the flexible CG's direction pass FcgDir32T: one FP32 coefficient and two int32 indices
+      per face (12 nf); per cell the diagonal, z and the previous direction read, the direction
+      and q = A p written (5 x 4 B); FP64 (FcgDirT): 16 nf + 5 x 8 B per cell;
+    cg_upd, FcgUpd32: e, r read and written, r_old, p, q (7 x 4 B per cell; 7 x 8 B in FP64);
+    mg_down, the pre-smoothing pass Down0: the column-scaled coefficient of each face side and two
+      indices (16 nf); per cell b, 1/diag read, x, r written (4 x 4 B; b FP64 in FP64 mode);
+    mg_up, the post-smoothing pass Up0: the row-scaled coefficient of each face side and two
+      indices (16 nf); per cell x, 1/diag, r, r_old read, z written (5 x 4 B; r, r_old, z FP64 in
+      FP64 mode)."""
+    nf, nc = n_op_faces(mesh), mesh.n_cells
+    v = 4 if mixed else 8
+    return {"cg_dir": 12 * nf + 20 * nc if mixed else 16 * nf + 40 * nc,
+            "cg_upd": 28 * nc if mixed else 56 * nc,
+            "mg_down": 16 * nf + (v + 12) * nc,
+            "mg_up": 16 * nf + (8 + 3 * v) * nc}
 
 
 def measured_traffic(kernel, workload):
@@ -90,19 +123,45 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def cpu_baseline(mesh, case, U, p, steps):
+def host_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count(), model
+
+
+def cpu_baseline(case_fn, threads, sample_mesh, full_mesh_cells):
+    """The oracle (FP64 C restatement of the reference algorithm, Jacobi CG to 1e-8 on OpenMP
+    threads, the rest of the step sequential) on a bounded sample of the workload: the same case
+    generated at a reduced mesh, the initial projections, one warm-up step and one timed step."""
     import oracle
-    o = oracle.Solver(mesh, case.params)
-    o.array("p")[:] = p
-    o.set_velocity(U)
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    from dopamine_amd import params as P
+    case = case_fn(*sample_mesh)
+    mesh = case.make_mesh()
+    oracle.set_threads(threads)
+    try:
+        o = oracle.Solver(mesh, case.params)
+        P.initialize(o, case, mesh)
         o.advance()
-    dt = time.perf_counter() - t0
-    its = o.counters()["iterations"]
-    return {"value": mesh.n_cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} RK3 step(s) of the same {mesh.n_cells}-cell workload from the GPU state after the "
-                      f"timed steps; FP64 C restatement, one thread; {its} CG iterations; {dt:.1f} s"}
+        c0 = o.counters()["iterations"]
+        t0 = time.perf_counter()
+        o.advance()
+        dt = time.perf_counter() - t0
+        its = o.counters()["iterations"] - c0
+    finally:
+        oracle.set_threads(1)
+    ncpu, model = host_info()
+    return {"value": mesh.n_cells / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"the same workload generated at {sample_mesh[0]}x{sample_mesh[1]}x{sample_mesh[2]} "
+                      f"({mesh.n_cells} cells, against {full_mesh_cells} on the GPU): initial projections and one "
+                      f"warm-up step, then one timed RK3 step of the FP64 C restatement with its Jacobi CG on "
+                      f"{threads} OpenMP threads ({its} CG iterations in the step; {dt:.1f} s)",
+            "cg_iterations_per_step": its, "host_cpus": ncpu, "host_model": model}
 
 
 def main():
@@ -115,20 +174,35 @@ def main():
     from dopamine_amd import cases, params as P
     from dopamine_amd.solver import FVSolver
 
-    nx, ny, nz = args.mesh
-    tets = args.workload == "tets"
-    case = cases.cavity_tets(55, rcm=True) if tets else cases.channel(nx, ny, nz)
+    wl = args.workload
+    if wl == "full":
+        dims = tuple(args.mesh) if args.mesh else (250, 210, 160)
+        case_fn = cases.full_stack
+        case = case_fn(*dims)
+    elif wl == "channel":
+        dims = tuple(args.mesh) if args.mesh else (128, 96, 64)
+        case_fn = cases.channel
+        case = case_fn(*dims)
+    else:
+        dims = None
+        case_fn = None
+        case = cases.cavity_tets(55, rcm=True)
     case.params.cg_variant = args.cg_variant
     learned = args.precond == "learned"
+    multigrid = args.precond == "multigrid"
     if learned:
         case.params.precond = "LEARNED"
         case.params.learn_collect = max(1, 3 * args.warmup)
         case.params.learn_iters = args.learn_iters
+    elif multigrid:
+        case.params.precond = "MULTIGRID"
+    mixed = args.precision == "mixed" and not learned  # the learned flexible CG runs in FP64
     t0 = time.perf_counter()
     mesh = case.make_mesh()
-    log(f"mesh {case.name}: {mesh.n_cells} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
+    nc = mesh.n_cells
+    log(f"mesh {case.name}: {nc} cells, {mesh.n_faces} faces ({time.perf_counter() - t0:.1f} s)")
     width = 4 if mesh.max_cell_faces <= 4 else 6 if mesh.max_cell_faces <= 6 else 8 if mesh.max_cell_faces <= 8 else 0
-    wl_key = "tets 6x55^3 rcm" if tets else f"channel {nx}x{ny}x{nz}"
+    wl_key = WORKLOAD_KEYS[wl] if (wl == "tets" or args.mesh is None) else f"{wl} {dims[0]}x{dims[1]}x{dims[2]}"
 
     def timed_run(params, precision, tag):
         solver = FVSolver(mesh, params, precision=precision)
@@ -141,32 +215,35 @@ def main():
         solver.sync()
         if params.precond.upper() == "LEARNED":
             log(f"[{tag}] learned preconditioner after warm-up: {solver.learned_info()}")
-        it0 = solver.counters()["iterations"]
+        c0 = solver.counters()
         solver.sync()
         t0 = time.perf_counter()
         for i in range(args.steps):
             solver.advance()
         solver.sync()
         elapsed = time.perf_counter() - t0
-        its = solver.counters()["iterations"] - it0
-        log(f"[{tag}] timed {args.steps} steps in {elapsed:.3f} s, {its} CG iterations")
-        return solver, elapsed, its
+        c1 = solver.counters()
+        its, solves = c1["iterations"] - c0["iterations"], c1["solves"] - c0["solves"]
+        if not solver.state_is_finite():
+            raise FloatingPointError(f"[{tag}] non-finite velocity or pressure after the timed steps")
+        log(f"[{tag}] timed {args.steps} steps in {elapsed:.3f} s, {its} CG iterations in {solves} solves")
+        return solver, elapsed, its, solves
 
-    mixed_run = args.precision == "mixed" and not learned  # the learned FCG runs in FP64
     plain = None
-    if (mixed_run or learned) and not args.no_plain:
-        import dataclasses
-        ps, pe, pi = timed_run(dataclasses.replace(case.params, precond="JACOBI"), 0, "plain FP64")
-        plain = {"value": mesh.n_cells * args.steps / pe, "ms_per_step": 1e3 * pe / args.steps,
-                 "cg_iterations_per_step": pi / args.steps}
+    if (mixed or learned or multigrid) and not args.no_plain:
+        ps, pe, pi, pn = timed_run(dataclasses.replace(case.params, precond="JACOBI"), 0, "plain FP64")
+        st = ps.stats()
+        plain = {"value": nc * args.steps / pe, "ms_per_step": 1e3 * pe / args.steps,
+                 "cg_iterations_per_step": pi / args.steps, "cg_iterations_per_solve": pi / max(pn, 1),
+                 "max_abs_divU": st["max_abs_divU"], "max_abs_cell_net_flux": ps.cell_net_flux_max()}
         ps.close()
-    solver, elapsed, its = timed_run(case.params, 1 if mixed_run else 0,
-                                     "learned" if learned else ("mixed" if mixed_run else "plain FP64"))
+    tag = ("mixed" if mixed else "FP64") + (" + multigrid" if multigrid else " + learned" if learned else " Jacobi")
+    solver, elapsed, its, solves = timed_run(case.params, 1 if mixed else 0, tag)
 
     # Kernel durations: the same number of further steps with a dispatch-stamped event pair on
     # every launch of the timed kernel classes (the stamps add a few microseconds between
     # launches, so these steps are not part of the throughput figure).
-    kernels = ("cg_dir", "cg_upd", "explicit", "ls_grad")
+    kernels = ("cg_dir", "cg_upd", "explicit", "ls_grad", "mg_down", "mg_up")
     ktime = {k: (0, 0.0) for k in kernels}
     ktimed_ms = None
     if not args.no_kernel_timing:
@@ -182,17 +259,55 @@ def main():
             solver.kernel_timing(k, False)
         log(f"kernel-timing pass: {ktimed_ms:.1f} ms per step")
 
-    nc = mesh.n_cells
-    mixed = mixed_run
-    bytes_dir, nfp = cg_pass_bytes(mesh, args.cg_variant, mixed)
+    # the dominant kernel of the pressure path and its algorithmic bytes per launch
     info = solver.learned_info() if learned else None
     timed_learned = learned and info is not None and info["phase"] != "off"
-    if timed_learned:  # hidden layer: h read and written (32 bfloat16 per cell), FP32 edge weight + index per face side
-        bytes_dir = 128 * nc + 16 * nfp
-    n_dir, ms_dir = ktime["cg_dir"]
-    avg_s = ms_dir / max(n_dir, 1) / 1e3
-    achieved = bytes_dir / avg_s / 1e9 if avg_s > 0 else 0.0
+    if multigrid:
+        model = multigrid_bytes(mesh, mixed)
+        vt = "float" if mixed else "double"
+        names = {"cg_dir": f"flexible-CG direction pass k_pass<5,1,cgk::FcgDir32T<{width}>>" if mixed else
+                 f"flexible-CG direction pass k_pass<5,1,cgk::FcgDirT<{width}>>",
+                 "cg_upd": "flexible-CG update pass k_pass<1,1,cgk::FcgUpd32>" if mixed else
+                 "flexible-CG update pass k_pass<1,1,cgk::FcgUpd>",
+                 "mg_down": f"multigrid pre-smoothing k_pass<1,1,amgk::Down0<{vt},{width}>>",
+                 "mg_up": f"multigrid post-smoothing k_pass<1,5,amgk::Up0<{vt},{width}>>"}
+        traffic_names = {"cg_dir": f"FcgDir32T<{width}>", "cg_upd": "FcgUpd32", "mg_down": f"Down0<float, {width}>",
+                         "mg_up": f"Up0<float, {width}>"}
+        dom = max(model, key=lambda k: ktime[k][1])
+        bytes_dom, kname, tname = model[dom], names[dom], traffic_names[dom] if mixed else None
+    else:
+        bytes_dom, _ = cg_pass_bytes(mesh, args.cg_variant, mixed)
+        if timed_learned:  # hidden layer: h read and written (32 bfloat16 per cell), FP32 weight + index per face side
+            bytes_dom = 128 * nc + 16 * n_op_faces(mesh)
+        dom = "cg_dir"
+        kname = ("learned preconditioner hidden layer k_tile<1,mp::L1<6>> (MFMA bf16)" if timed_learned else
+                 f"CG FP32 sweep iteration k_pass<6,6,cgk::Fused32T<{width}>>" if mixed else
+                 f"CG iteration pass k_pass<6,6,cgk::FusedT<{width}>>" if args.cg_variant == 1
+                 else "CG direction/operator pass k_pass<4,1,cgk::Dir>")
+        tname = (f"Fused32T<{width}>" if mixed else f"FusedT<{width}>") if (not timed_learned and
+                                                                             args.cg_variant == 1) else None
+    n_dom, ms_dom = ktime[dom]
+    avg_s = ms_dom / max(n_dom, 1) / 1e3
+    achieved = bytes_dom / avg_s / 1e9 if avg_s > 0 else 0.0
     st = solver.stats()
+    if wl == "full":
+        workload_text = (f"full stack: 1000 x 1000 x 200 m box {dims[0]}x{dims[1]}x{dims[2]} with 40 seeded cuboid "
+                         f"obstacles removed ({nc} hex cells), MOST walls, AMD SGS, temperature + 2 passive scalars "
+                         "(van Leer TVD), Coriolis, pressure to 1e-8")
+    elif wl == "channel":
+        workload_text = (f"turbulent channel {dims[0]}x{dims[1]}x{dims[2]} ({nc} hex cells), Re_tau 395, KEP + "
+                         "Vreman + log-law wall model, pressure to 1e-8")
+    else:
+        workload_text = (f"lid-driven cavity Re 3200 on {nc} jittered shuffled tets renumbered by reverse "
+                         "Cuthill-McKee, Koren TVD + AMD, Jasak non-orthogonal correction, pressure to 1e-8")
+    if multigrid:
+        path_text = ("FP32 flexible-CG sweeps preconditioned by the FP32 aggregation-multigrid V-cycle in FP64 "
+                     "iterative refinement" if mixed else
+                     "FP64 flexible CG preconditioned by the FP32 aggregation-multigrid V-cycle")
+    elif learned:
+        path_text = "flexible CG with the learned message-passing preconditioner (Jacobi CG fallback)"
+    else:
+        path_text = "FP32 Jacobi-CG sweeps in FP64 iterative refinement" if mixed else "plain FP64 Jacobi CG"
     out = {
         "metric": METRIC,
         "value": nc * args.steps / elapsed,
@@ -205,42 +320,30 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if mixed else ("f64+bf16" if learned else "f64"),
-        "data": ("synthetic: fluid at rest, moving lid, on a generated seeded jittered mesh" if tets else
+        "data": ("synthetic: seeded obstacles and temperature noise on a generated mesh" if wl == "full" else
+                 "synthetic: fluid at rest, moving lid, on a generated seeded jittered mesh" if wl == "tets" else
                  "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh"),
-        "config": {"workload": (f"lid-driven cavity Re 3200 on {nc} jittered shuffled tets renumbered by reverse "
-                                f"Cuthill-McKee ({nfp} pressure faces), Koren TVD + AMD, Jasak non-orthogonal "
-                                "correction, Jacobi CG to 1e-8, " if tets else
-                                f"turbulent channel {nx}x{ny}x{nz} ({nc} hex cells, {nfp} pressure faces), "
-                                "Re_tau 395, KEP + Vreman + log-law wall model, Jacobi CG to 1e-8, ")
-                               + ("FP32 CG sweeps in FP64 iterative refinement" if mixed else "plain FP64")
-                               + (", flexible CG with the learned message-passing preconditioner (Jacobi CG "
-                                  "fallback)" if learned else ""),
-                   "precision": "mixed" if mixed else "fp64", "cg_variant": args.cg_variant, "precond": args.precond,
-                   "dt": case.params.dt},
-        "roofline": {"bound": "hbm",
-                     "kernel": ("learned preconditioner hidden layer k_tile<1,mp::L1<6>> (MFMA bf16)" if timed_learned else
-                                f"CG FP32 sweep iteration k_pass<6,6,cgk::Fused32T<{width}>>" if mixed else
-                                f"CG iteration pass k_pass<6,6,cgk::FusedT<{width}>>" if args.cg_variant == 1
-                                else "CG direction/operator pass k_pass<4,1,cgk::Dir>"),
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic(f"Fused32T<{width}>" if mixed else f"FusedT<{width}>", wl_key)
-                                 if (not timed_learned and args.cg_variant == 1) else None),
-                     "bytes_per_launch": bytes_dir, "avg_launch_us": avg_s * 1e6,
-                     "launches": n_dir},
+        "config": {"workload": workload_text + "; " + path_text, "precision": "mixed" if mixed else "fp64",
+                   "precond": args.precond, "cg_variant": args.cg_variant, "dt": case.params.dt},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(tname, wl_key) if tname else None,
+                     "bytes_per_launch": bytes_dom, "avg_launch_us": avg_s * 1e6, "launches": n_dom},
         "kernels_us": {k: (v[1] * 1e3 / v[0] if v[0] else None) for k, v in ktime.items()},
+        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in ktime.items()},
         "kernel_timing_pass_ms_per_step": ktimed_ms,
         "cg_iterations_per_step": its / args.steps,
-
-        "state": {"E": st["E"], "Ub": st["Ub"], "max_abs_divU": st["max_abs_divU"]},
+        "cg_iterations_per_solve": its / max(solves, 1),
+        "state": {"E": st["E"], "Ub": st["Ub"], "max_abs_divU": st["max_abs_divU"],
+                  "max_abs_cell_net_flux": solver.cell_net_flux_max()},
         "cpu_baseline": None,
     }
     if plain is not None:
         out["plain_fp64"] = plain
         out["speedup_vs_plain_fp64"] = out["value"] / plain["value"]
     if learned:
-        info = solver.learned_info()
         loss = solver.learned_loss()
-        out["learned"] = {**info, "loss_first": float(loss[:10].mean()) if loss.size else None,
+        out["learned"] = {**solver.learned_info(), "loss_first": float(loss[:10].mean()) if loss.size else None,
                           "loss_last": float(loss[-10:].mean()) if loss.size else None}
     # plain copy kernel of known size: achievable HBM bandwidth, and the calibration of the traffic
     # counters (FETCH_SIZE / WRITE_SIZE) for the same process
@@ -253,12 +356,13 @@ def main():
     if L.dopamine_hbm_copy_probe(ncopy, 20, ctypes.byref(cms)) == 0 and cms.value > 0:
         out["hbm_copy"] = {"bytes_per_launch": 16 * ncopy, "avg_launch_us": cms.value * 1e3,
                            "GBs": 16 * ncopy / (cms.value * 1e-3) / 1e9}
-    if args.cpu_steps > 0:
-        U, p = solver.U, solver.p
-        log("CPU baseline (oracle restatement, one thread) ...")
-        out["cpu_baseline"] = cpu_baseline(mesh, case, U, p, args.cpu_steps)
-        log(f"CPU baseline: {out['cpu_baseline']['sample']}")
     solver.close()
+    if args.cpu_threads >= 0 and case_fn is not None:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = (100, 84, 64) if wl == "full" else (64, 48, 32)
+        log(f"CPU baseline (oracle restatement, {threads} threads) on {sample} ...")
+        out["cpu_baseline"] = cpu_baseline(case_fn, threads, sample, nc)
+        log(f"CPU baseline: {out['cpu_baseline']['sample']}")
     print(json.dumps(out), flush=True)
 
 

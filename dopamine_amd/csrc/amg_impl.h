@@ -253,14 +253,17 @@ struct AmgCfg {
 
 // ---------------------------------------------------------------- device kernels
 
-// One level in the execution space.
+// One level in the execution space. Level 0 keeps the operator's ELL lists (one thread per cell,
+// the solver's own neighbour array); coarser levels, whose Galerkin rows are short on average but
+// occasionally very long, are CSR rows worked by groups of AMG_G lanes.
+constexpr int AMG_G = 8;
 struct AmgLevelDev {
-  int n = 0, w = 0;
-  const int* nbr = nullptr;
+  int n = 0, w = 0;                              // cells; ELL width (level 0 only)
+  const int* nbr = nullptr;                      // level 0: ELL neighbours [slot][cell]
+  const int *rptr = nullptr, *col = nullptr;     // coarse levels: CSR rows
   const float *cs = nullptr, *cr = nullptr, *dinv = nullptr;
-  const int* agg = nullptr;                       // to the next coarser level (n entries)
-  const int *cptr = nullptr, *cidx = nullptr;     // children of the next coarser level's cells
-  float *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
+  const int *cptr = nullptr, *cidx = nullptr;    // children (cells of this level) of the next coarser level's cells
+  float *b = nullptr, *x = nullptr, *r = nullptr;
 };
 
 namespace amgk {
@@ -295,7 +298,33 @@ DM_INLINE bool cycle_test(CgScalars* sc, int mode, int k, double rr, bool leader
   return true;
 }
 
-// pre-smoothing from zero and its residual: x = w D^-1 b, r = (1 - w) b + w sum_j cs_ij b_j
+// Sum over the G lanes of an aligned lane group (every lane gets the total). On the host the
+// group's first lane does the whole row, so the value passes through.
+template <int G>
+DM_INLINE float group_sum(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+#endif
+  return v;
+}
+// Lane of the group and its stride through a row: all G lanes on the device, the first lane alone
+// on the host (the others return before touching memory).
+template <int G>
+DM_INLINE bool group_lane(int t, int* lane, int* stride) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *lane = t % G;
+  *stride = G;
+  return true;
+#else
+  *lane = 0;
+  *stride = 1;
+  return t % G == 0;
+#endif
+}
+
+// pre-smoothing from zero and its residual, level 0 (ELL): x = w D^-1 b,
+// r = (1 - w) b + w sum_j cs_ij b_j
 template <class T>
 DM_INLINE void down_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
   const int n = L.n;
@@ -311,25 +340,41 @@ DM_INLINE void down_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
   L.r[c] = (1.0f - omega) * bc + omega * s;
 }
 
-// prolongation with over-correction and post-smoothing: xt = x + alpha xc[agg],
-// y = xt + w (D^-1 b - xt + sum_j cr_ij xt_j)
+// post-smoothing, level 0 (ELL), x already holding the prolongated coarse correction:
+// y = x + w (D^-1 b - x + sum_j cr_ij x_j)
 template <class T>
-DM_INLINE float up_cell(const AmgLevelDev& L, const T* b, const float* xc, float omega, float alpha, int c) {
+DM_INLINE float up_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
   const int n = L.n;
-  const float xt = L.x[c] + alpha * xc[L.agg[c]];
+  const float xc = L.x[c];
   float s = 0.0f;
   for (int k = 0; k < L.w; ++k) {
     const size_t q = size_t(k) * n + c;
     const int j = L.nbr[q];
     if (j < 0) continue;
-    s = s + L.cr[q] * (L.x[j] + alpha * xc[L.agg[j]]);
+    s = s + L.cr[q] * L.x[j];
   }
-  return xt + omega * (L.dinv[c] * float(b[c]) - xt + s);
+  return xc + omega * (L.dinv[c] * float(b[c]) - xc + s);
+}
+
+// Level-0 slot loads of the batched (branch-free) kernels: WC slots of cell c, an empty slot
+// pointing at the cell itself with a zero coefficient.
+template <int WC>
+DM_INLINE void slots0(const AmgLevelDev& L, const float* cf, int c, int* jj, float* h) {
+#pragma unroll
+  for (int kk = 0; kk < WC; ++kk) {
+    const bool live = kk < L.w;
+    const size_t q = size_t(live ? kk : 0) * L.n + c;
+    const int j = L.nbr[q];
+    const float v = cf[q];
+    jj[kk] = (live && j >= 0) ? j : c;
+    h[kk] = (live && j >= 0) ? v : 0.0f;
+  }
 }
 
 // level 0 pre-smoothing as a chained pass: its setup is the convergence test of iteration k
-template <class T>
+template <class T, int WC = 0>
 struct Down0 {
+  static constexpr int kSlots = WC;
   AmgLevelDev L;
   const T* b;
   CgScalars* sc;
@@ -343,9 +388,25 @@ struct Down0 {
     o.v[0] = 0.0;
     return o;
   }
+  DM_HD Red<1> run(int lo, int hi, int t, int nt, const Scal&) const {
+    for (int c = lo + t; c < hi; c += nt) {
+      int jj[WC > 0 ? WC : 1];
+      float h[WC > 0 ? WC : 1], bj[WC > 0 ? WC : 1];
+      slots0<WC>(L, L.cs, c, jj, h);
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) bj[kk] = float(b[jj[kk]]);
+      const float bc = float(b[c]);
+      float s = 0.0f;
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) s = s + h[kk] * bj[kk];
+      L.x[c] = omega * L.dinv[c] * bc;
+      L.r[c] = (1.0f - omega) * bc + omega * s;
+    }
+    return red_identity<1, false>();
+  }
 };
 
-// the same two level-0 steps outside a solve (one V-cycle on its own)
+// the same level-0 steps outside a solve (one V-cycle on its own)
 template <class T>
 struct Down0F {
   AmgLevelDev L;
@@ -358,82 +419,119 @@ struct Up0F {
   AmgLevelDev L;
   const T* r;
   T* z;
-  const float* xc;
-  float omega, alpha;
-  DM_HD void operator()(int c) const { z[c] = T(up_cell(L, r, xc, omega, alpha, c)); }
-};
-
-struct DownL {
-  AmgLevelDev L;
-  const CgScalars* sc;
-  int mode;
   float omega;
-  DM_HD void operator()(int c) const {
-    if (gated(sc, mode)) return;
-    down_cell(L, static_cast<const float*>(L.b), omega, c);
-  }
+  DM_HD void operator()(int c) const { z[c] = T(up_cell(L, r, omega, c)); }
 };
 
-// b_coarse[I] = sum of the fine residual over aggregate I's cells
+// b_coarse[I] = sum of the fine residual over aggregate I's cells (a lane group per aggregate)
 struct Restrict {
   const int *cptr, *cidx;
   const float* rf;
   float* bc;
   const CgScalars* sc;
   int mode;
-  DM_HD void operator()(int I) const {
+  DM_HD void operator()(int t) const {
+    int lane, st;
+    if (!group_lane<AMG_G>(t, &lane, &st)) return;
     if (gated(sc, mode)) return;
+    const int I = t / AMG_G;
     float s = 0.0f;
-    for (int q = cptr[I]; q < cptr[I + 1]; ++q) s = s + rf[cidx[q]];
-    bc[I] = s;
+    for (int q = cptr[I] + lane; q < cptr[I + 1]; q += st) s = s + rf[cidx[q]];
+    s = group_sum<AMG_G>(s);
+    if (lane == 0) bc[I] = s;
   }
 };
 
-// y = inv b on the coarsest level (inv column-major: consecutive rows read consecutive words)
+// coarse-level pre-smoothing and residual (CSR, a lane group per row)
+struct DownC {
+  AmgLevelDev L;
+  const CgScalars* sc;
+  int mode;
+  float omega;
+  DM_HD void operator()(int t) const {
+    int lane, st;
+    if (!group_lane<AMG_G>(t, &lane, &st)) return;
+    if (gated(sc, mode)) return;
+    const int c = t / AMG_G;
+    float s = 0.0f;
+    for (int q = L.rptr[c] + lane; q < L.rptr[c + 1]; q += st) s = s + L.cs[q] * L.b[L.col[q]];
+    s = group_sum<AMG_G>(s);
+    if (lane == 0) {
+      const float bc = L.b[c];
+      L.x[c] = omega * L.dinv[c] * bc;
+      L.r[c] = (1.0f - omega) * bc + omega * s;
+    }
+  }
+};
+
+// Coarse-level post-smoothing (CSR, a lane group per row), then the prolongation with
+// over-correction into the next finer level: x_fine[i] += alpha y for the cell's children i
+// (each fine cell has one parent, so the writes never collide).
+struct UpC {
+  AmgLevelDev L;
+  const int *fptr, *fidx;  // this level's cells' children on the finer level
+  float* xf;
+  const CgScalars* sc;
+  int mode;
+  float omega, alpha;
+  DM_HD void operator()(int t) const {
+    int lane, st;
+    if (!group_lane<AMG_G>(t, &lane, &st)) return;
+    if (gated(sc, mode)) return;
+    const int c = t / AMG_G;
+    float s = 0.0f;
+    for (int q = L.rptr[c] + lane; q < L.rptr[c + 1]; q += st) s = s + L.cr[q] * L.x[L.col[q]];
+    s = group_sum<AMG_G>(s);
+    const float xc = L.x[c];
+    const float y = xc + omega * (L.dinv[c] * L.b[c] - xc + s);
+    const float a = alpha * y;
+    for (int q = fptr[c] + lane; q < fptr[c + 1]; q += st) xf[fidx[q]] = xf[fidx[q]] + a;
+  }
+};
+
+// Coarsest level: y = inv b (dense, symmetric, a wavefront per row), then the prolongation into
+// the next finer level as in UpC.
+constexpr int AMG_GC = 64;
 struct Coarsest {
   int n;
   const float* inv;
   const float* b;
-  float* y;
+  const int *fptr, *fidx;
+  float* xf;
   const CgScalars* sc;
   int mode;
-  DM_HD void operator()(int i) const {
+  float alpha;
+  DM_HD void operator()(int t) const {
+    int lane, st;
+    if (!group_lane<AMG_GC>(t, &lane, &st)) return;
     if (gated(sc, mode)) return;
+    const int i = t / AMG_GC;
     float s = 0.0f;
-    for (int j = 0; j < n; ++j) s = s + inv[size_t(j) * n + i] * b[j];
-    y[i] = s;
-  }
-};
-
-struct UpL {
-  AmgLevelDev L;
-  const float* xc;
-  const CgScalars* sc;
-  int mode;
-  float omega, alpha;
-  DM_HD void operator()(int c) const {
-    if (gated(sc, mode)) return;
-    L.y[c] = up_cell(L, static_cast<const float*>(L.b), xc, omega, alpha, c);
+    for (int j = lane; j < n; j += st) s = s + inv[size_t(i) * n + j] * b[j];
+    s = group_sum<AMG_GC>(s);
+    const float a = alpha * s;
+    for (int q = fptr[i] + lane; q < fptr[i + 1]; q += st) xf[fidx[q]] = xf[fidx[q]] + a;
   }
 };
 
 // level 0 post-smoothing as a chained pass writing z and the flexible-CG partials
 // z.r, z.r_old, sum z, sum r, sum r_old
-template <class T>
+template <class T, int WC = 0>
 struct Up0 {
+  static constexpr int kSlots = WC;
   AmgLevelDev L;
   const T *r, *rold;
-  T* z;
-  const float* xc;
+  T* z;  // z[c * zs]
+  int zs;
   const CgScalars* sc;
   int mode;
-  float omega, alpha;
+  float omega;
   using Scal = cgk::NoScal;
   DM_HD bool setup(const Red<1>&, bool, Scal&) const { return !gated(sc, mode); }
   DM_HD Red<5> operator()(int c, const Scal&) const {
-    const float y = up_cell(L, r, xc, omega, alpha, c);
+    const float y = up_cell(L, r, omega, c);
     const T zc = T(y);
-    z[c] = zc;
+    z[size_t(c) * zs] = zc;
     const double rc = double(r[c]), ro = double(rold[c]), zd = double(zc);
     Red<5> o;
     o.v[0] = zd * rc;
@@ -442,6 +540,30 @@ struct Up0 {
     o.v[3] = rc;
     o.v[4] = ro;
     return o;
+  }
+  DM_HD Red<5> run(int lo, int hi, int t, int nt, const Scal&) const {
+    Red<5> acc = red_identity<5, false>();
+    for (int c = lo + t; c < hi; c += nt) {
+      int jj[WC > 0 ? WC : 1];
+      float h[WC > 0 ? WC : 1], xj[WC > 0 ? WC : 1];
+      slots0<WC>(L, L.cr, c, jj, h);
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) xj[kk] = L.x[jj[kk]];
+      const float xc = L.x[c];
+      const float bc = float(r[c]);
+      float s = 0.0f;
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) s = s + h[kk] * xj[kk];
+      const T zc = T(xc + omega * (L.dinv[c] * bc - xc + s));
+      z[size_t(c) * zs] = zc;
+      const double rc = double(r[c]), ro = double(rold[c]), zd = double(zc);
+      acc.v[0] = acc.v[0] + zd * rc;
+      acc.v[1] = acc.v[1] + zd * ro;
+      acc.v[2] = acc.v[2] + zd;
+      acc.v[3] = acc.v[3] + rc;
+      acc.v[4] = acc.v[4] + ro;
+    }
+    return acc;
   }
 };
 
@@ -458,40 +580,61 @@ class AmgPc : public FcgPc, public FcgPc32 {
     x_ = x;
     cfg_ = cfg;
     AmgOp cur = A;
-    const int* cur_nbr_dev = nbr_dev;
     for (int lev = 0;; ++lev) {
-      Level L;
-      L.d.n = cur.n;
-      L.d.w = cur.w;
-      L.d.nbr = cur_nbr_dev ? cur_nbr_dev : up(cur.nbr);
-      {
-        // cs_ij = coef_ij / diag_j, cr_ij = coef_ij / diag_i, dinv_i = 1 / diag_i; a cell whose
-        // diagonal vanishes (an aggregate that is a whole connected component of a singular
-        // operator) is left out of the smoothing (dinv = 0, no couplings)
-        double dmax = 0.0;
-        for (double d : cur.diag) dmax = std::max(dmax, std::fabs(d));
-        const double dmin = 1.0e-12 * dmax;
+      AmgLevelDev L;
+      L.n = cur.n;
+      // cs_ij = coef_ij / diag_j, cr_ij = coef_ij / diag_i, dinv_i = 1 / diag_i; a cell whose
+      // diagonal vanishes (an aggregate that is a whole connected component of a singular
+      // operator) is left out of the smoothing (dinv = 0, no couplings)
+      double dmax = 0.0;
+      for (double d : cur.diag) dmax = std::max(dmax, std::fabs(d));
+      const double dmin = 1.0e-12 * dmax;
+      std::vector<float> di(cur.n, 0.0f);
+      for (int c = 0; c < cur.n; ++c)
+        if (cur.diag[c] > dmin) di[c] = float(1.0 / cur.diag[c]);
+      auto scaled = [&](int c, size_t q, float* cs, float* cr) {
+        const int j = cur.nbr[q];
+        if (j < 0 || !(cur.diag[c] > dmin) || !(cur.diag[j] > dmin)) return false;
+        *cs = float(cur.coef[q] / cur.diag[j]);
+        *cr = float(cur.coef[q] / cur.diag[c]);
+        return true;
+      };
+      if (lev == 0) {  // ELL, the solver's neighbour array
+        L.w = cur.w;
+        L.nbr = nbr_dev;
         const size_t m = size_t(std::max(1, cur.w)) * cur.n;
-        std::vector<float> cs(m, 0.0f), cr(m, 0.0f), di(cur.n, 0.0f);
-        for (int c = 0; c < cur.n; ++c) {
-          if (!(cur.diag[c] > dmin)) continue;
-          di[c] = float(1.0 / cur.diag[c]);
+        std::vector<float> cs(m, 0.0f), cr(m, 0.0f);
+        for (int c = 0; c < cur.n; ++c)
           for (int k = 0; k < cur.w; ++k) {
             const size_t q = size_t(k) * cur.n + c;
-            const int j = cur.nbr[q];
-            if (j < 0 || !(cur.diag[j] > dmin)) continue;
-            cs[q] = float(cur.coef[q] / cur.diag[j]);
-            cr[q] = float(cur.coef[q] / cur.diag[c]);
+            scaled(c, q, &cs[q], &cr[q]);
           }
+        L.cs = up(cs);
+        L.cr = up(cr);
+      } else {  // CSR of the non-empty entries
+        std::vector<int> rptr(size_t(cur.n) + 1, 0), col;
+        std::vector<float> cs, cr;
+        for (int c = 0; c < cur.n; ++c) {
+          for (int k = 0; k < cur.w; ++k) {
+            float a, b;
+            if (!scaled(c, size_t(k) * cur.n + c, &a, &b)) continue;
+            col.push_back(cur.nbr[size_t(k) * cur.n + c]);
+            cs.push_back(a);
+            cr.push_back(b);
+          }
+          rptr[size_t(c) + 1] = int(col.size());
         }
-        L.d.cs = up(cs);
-        L.d.cr = up(cr);
-        L.d.dinv = up(di);
+        L.rptr = up(rptr);
+        L.col = up(col);
+        L.cs = up(cs);
+        L.cr = up(cr);
+        nnz_.push_back(int64_t(col.size()));
       }
-      L.d.b = lev == 0 ? nullptr : fresh<float>(cur.n);
-      L.d.x = fresh<float>(cur.n);
-      L.d.y = lev == 0 ? nullptr : fresh<float>(cur.n);
-      L.d.r = fresh<float>(cur.n);
+      L.dinv = up(di);
+      L.b = lev == 0 ? nullptr : fresh<float>(cur.n);
+      L.x = fresh<float>(cur.n);
+      L.r = fresh<float>(cur.n);
+      sizes_.push_back(cur.n);
       const bool last = (cur.n <= cfg.coarse_max && lev > 0) || lev + 1 >= cfg.max_levels;
       if (last) {
         if (cur.n > 4096) throw std::runtime_error("multigrid: coarsest level too large for the dense solve");
@@ -511,17 +654,11 @@ class AmgPc : public FcgPc, public FcgPc32 {
       std::vector<int> ptr, idx;
       amg_children(agg, na, ptr, idx);
       AmgOp next = amg_galerkin(cur, agg, na, ptr, idx);
-      L.d.agg = up(agg);
-      L.d.cptr = up(ptr);
-      L.d.cidx = up(idx);
+      L.cptr = up(ptr);
+      L.cidx = up(idx);
       levels_.push_back(L);
-      sizes_.push_back(cur.n);
-      widths_.push_back(cur.w);
       cur = std::move(next);
-      cur_nbr_dev = nullptr;
     }
-    sizes_.push_back(n_coarse_);
-    widths_.push_back(cur.w);
     zpart_ = fresh<double>(size_t(x_->pass_grid(A.n)));
   }
   void destroy() {
@@ -535,7 +672,7 @@ class AmgPc : public FcgPc, public FcgPc32 {
 
   int levels() const { return int(levels_.size()); }
   const std::vector<int>& sizes() const { return sizes_; }
-  const std::vector<int>& widths() const { return widths_; }
+  const std::vector<int64_t>& nnz() const { return nnz_; }
   bool singular() const { return singular_; }
   const AmgCfg& cfg() const { return cfg_; }
 
@@ -543,25 +680,22 @@ class AmgPc : public FcgPc, public FcgPc32 {
   int out_grid(int n) const override { return x_->pass_grid(n); }
   void apply(int k, const double* r, const double* rold, double* z, CgScalars* sc, const double* rr_part, int rr_g,
              double* out_part) override {
-    cycle<double>(k, r, rold, z, sc, rr_part, rr_g, out_part, amgk::OUTER);
+    cycle<double>(k, r, rold, z, 1, sc, rr_part, rr_g, out_part, amgk::OUTER);
   }
   // ---- FcgPc32 (FP32 sweeps)
   int out_grid32(int n) const override { return x_->pass_grid(n); }
-  void apply32(int k, const float* r, const float* rold, float* z, CgScalars* sc, const double* rr_part, int rr_g,
-               double* out_part) override {
-    cycle<float>(k, r, rold, z, sc, rr_part, rr_g, out_part, amgk::INNER);
+  void apply32(int k, const float* r, const float* rold, float* z, int zstride, CgScalars* sc, const double* rr_part,
+               int rr_g, double* out_part) override {
+    cycle<float>(k, r, rold, z, zstride, sc, rr_part, rr_g, out_part, amgk::INNER);
   }
 
   // One V-cycle z = M r outside a solve (tests): no gate, no convergence test.
   template <class T>
   void vcycle(const T* r, T* z) {
-    cycle<T>(0, r, r, z, nullptr, nullptr, 0, nullptr, amgk::OUTER);
+    cycle<T>(0, r, r, z, 1, nullptr, nullptr, 0, nullptr, amgk::OUTER);
   }
 
  private:
-  struct Level {
-    AmgLevelDev d;
-  };
   template <class T>
   T* up(const std::vector<T>& v) {
     T* p = x_->template alloc<T>(std::max<size_t>(1, v.size()));
@@ -577,39 +711,54 @@ class AmgPc : public FcgPc, public FcgPc32 {
   }
 
   template <class T>
-  void cycle(int k, const T* r, const T* rold, T* z, CgScalars* sc, const double* rr_part, int rr_g,
+  void cycle(int k, const T* r, const T* rold, T* z, int zs, CgScalars* sc, const double* rr_part, int rr_g,
              double* out_part, int mode) {
     X& xx = *x_;
     const float om = float(cfg_.omega), al = float(cfg_.alpha);
     const int nl = int(levels_.size());
-    const AmgLevelDev& L0 = levels_[0].d;
+    const AmgLevelDev& L0 = levels_[0];
     if (sc)
-      xx.template pass<1, 1>(L0.n, amgk::Down0<T>{L0, r, sc, mode, k, om}, rr_part, rr_g, zpart_);
+      with_width(L0.w, [&](auto wc) {
+        xx.tbegin(K_MG_DOWN);
+        xx.template pass<1, 1>(L0.n, amgk::Down0<T, decltype(wc)::value>{L0, r, sc, mode, k, om}, rr_part, rr_g,
+                               zpart_);
+        xx.tend(K_MG_DOWN);
+      });
     else
       xx.for_each(L0.n, amgk::Down0F<T>{L0, r, om});
-    for (int l = 0; l + 1 < nl; ++l) {
-      const AmgLevelDev& F = levels_[l].d;
-      const AmgLevelDev& C = levels_[l + 1].d;
-      if (l > 0) xx.for_each(F.n, amgk::DownL{F, sc, mode, om});
-      xx.for_each(C.n, amgk::Restrict{F.cptr, F.cidx, F.r, C.b, sc, mode});
+    for (int l = 1; l < nl; ++l) {
+      const AmgLevelDev& F = levels_[l - 1];
+      const AmgLevelDev& C = levels_[l];
+      xx.for_each(C.n * amgk_g(), amgk::Restrict{F.cptr, F.cidx, F.r, C.b, sc, mode});
+      if (l < nl - 1) xx.for_each(C.n * amgk_g(), amgk::DownC{C, sc, mode, om});
     }
-    const AmgLevelDev& Lc = levels_[nl - 1].d;
-    xx.for_each(n_coarse_, amgk::Coarsest{n_coarse_, inv_, Lc.b, Lc.y, sc, mode});
+    {
+      const AmgLevelDev& F = levels_[nl - 2];
+      xx.for_each(n_coarse_ * amgk::AMG_GC,
+                  amgk::Coarsest{n_coarse_, inv_, levels_[nl - 1].b, F.cptr, F.cidx, F.x, sc, mode, al});
+    }
     for (int l = nl - 2; l >= 1; --l) {
-      const AmgLevelDev& F = levels_[l].d;
-      xx.for_each(F.n, amgk::UpL{F, levels_[l + 1].d.y, sc, mode, om, al});
+      const AmgLevelDev& C = levels_[l];
+      const AmgLevelDev& F = levels_[l - 1];
+      xx.for_each(C.n * amgk_g(), amgk::UpC{C, F.cptr, F.cidx, F.x, sc, mode, om, al});
     }
-    const float* xc = levels_[1].d.y;
     if (sc)
-      xx.template pass<1, 5>(L0.n, amgk::Up0<T>{L0, r, rold, z, xc, sc, mode, om, al}, nullptr, 0, out_part);
+      with_width(L0.w, [&](auto wc) {
+        xx.tbegin(K_MG_UP);
+        xx.template pass<1, 5>(L0.n, amgk::Up0<T, decltype(wc)::value>{L0, r, rold, z, zs, sc, mode, om}, nullptr,
+                               0, out_part);
+        xx.tend(K_MG_UP);
+      });
     else
-      xx.for_each(L0.n, amgk::Up0F<T>{L0, r, z, xc, om, al});
+      xx.for_each(L0.n, amgk::Up0F<T>{L0, r, z, om});
   }
+  static constexpr int amgk_g() { return AMG_G; }
 
   X* x_ = nullptr;
   AmgCfg cfg_;
-  std::vector<Level> levels_;
-  std::vector<int> sizes_, widths_;
+  std::vector<AmgLevelDev> levels_;
+  std::vector<int> sizes_;
+  std::vector<int64_t> nnz_;
   std::vector<void*> owned_;
   float* inv_ = nullptr;
   int n_coarse_ = 0;

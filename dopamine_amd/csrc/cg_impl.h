@@ -38,6 +38,12 @@ struct EllOp {
   const double* diag = nullptr;  // n (already repaired: non-finite or tiny -> 1)
 };
 
+// Preconditioned residual and search direction of the FP32 flexible CG, packed so one gather
+// fetches both (z of this iteration in x, the previous direction in y).
+struct alignas(8) ZP {
+  float x, y;
+};
+
 // Packed per-cell state of the single-reduction recurrence: one 32-byte (16-byte in FP32) load
 // fetches everything a neighbour contributes.
 struct alignas(32) St4 {
@@ -80,6 +86,20 @@ struct Scale {
   }
 };
 }  // namespace cgk
+
+// Calls fn with the slot count of the batched (branch-free) passes for an operator of width w
+// (0: the generic per-slot loop).
+template <class Fn>
+inline void with_width(int w, Fn&& fn) {
+  if (w <= 4)
+    fn(std::integral_constant<int, 4>{});
+  else if (w <= 6)
+    fn(std::integral_constant<int, 6>{});
+  else if (w <= 8)
+    fn(std::integral_constant<int, 8>{});
+  else
+    fn(std::integral_constant<int, 0>{});
+}
 
 namespace cgk {
 struct To32 {
@@ -707,13 +727,13 @@ struct FcgPc {
   virtual int out_grid(int n) const = 0;
 };
 
-// Preconditioner of the FP32 sweeps of mixed precision: apply32(k, r, r_old, z, ...) with the
+// Preconditioner of the FP32 sweeps of mixed precision: apply32(k, r, r_old, z, zstride, ...) (z[c * zstride]) with the
 // contract of FcgPc on FP32 vectors, the sweep's stopping test (rr <= inner_tol2 or the iteration
 // limit, setting inner_done) in its first pass.
 struct FcgPc32 {
   virtual ~FcgPc32() = default;
-  virtual void apply32(int k, const float* r, const float* rold, float* z, CgScalars* sc, const double* rr_part,
-                       int rr_g, double* out_part) = 0;
+  virtual void apply32(int k, const float* r, const float* rold, float* z, int zstride, CgScalars* sc,
+                       const double* rr_part, int rr_g, double* out_part) = 0;
   virtual int out_grid32(int n) const = 0;
 };
 
@@ -891,10 +911,13 @@ struct FcgStart32 {
 
 // z -= mean(z) on a singular operator; beta = max(0, z.(r - r_old) / (z.r)_old);
 // p = z + beta p_old; q = A p (FP32 operator); partial p.q
-struct FcgDir32 {
+template <int WC>
+struct FcgDir32T {
+  static constexpr int kSlots = WC;
   EllOp32 A;
-  const float *z, *pold;
-  float *pnew, *q;
+  const ZP* cur;  // z (x) and the previous direction (y)
+  ZP* nxt;        // the new direction goes to nxt[c].y
+  float* q;
   CgScalars* sc;
   int k;
   struct Scal {
@@ -909,31 +932,67 @@ struct FcgDir32 {
     if (leader) sc->rz_hist[k & 1] = zr;
     return true;
   }
+  DM_HD static float dir(const ZP& v, float zm, float beta, bool first) {
+    return first ? v.x - zm : (v.x - zm) + beta * v.y;
+  }
   DM_HD Red<1> operator()(int c, const Scal& s) const {
     const int n = A.n;
     const float zm = float(s.zm), beta = float(s.beta);
     const bool first = k == 0;
-    const float pc = first ? z[c] - zm : (z[c] - zm) + beta * pold[c];
+    const float pc = dir(cur[c], zm, beta, first);
     float qc = A.diag[c] * pc;
     for (int kk = 0; kk < A.w; ++kk) {
       const size_t q_ = size_t(kk) * n + c;
       const int j = A.nbr[q_];
       if (j < 0) continue;
-      const float pj = first ? z[j] - zm : (z[j] - zm) + beta * pold[j];
-      qc = qc - A.coef[q_] * pj;
+      qc = qc - A.coef[q_] * dir(cur[j], zm, beta, first);
     }
-    pnew[c] = pc;
+    nxt[c].y = pc;
     q[c] = qc;
     Red<1> o;
     o.v[0] = double(pc) * double(qc);
     return o;
   }
+  // batched form: all WC slot loads, then all neighbour gathers, branch-free (an empty slot reads
+  // the cell itself with a zero coefficient)
+  DM_HD Red<1> run(int lo, int hi, int t, int nt, const Scal& s) const {
+    const int n = A.n, w = A.w;
+    const float zm = float(s.zm), beta = float(s.beta);
+    const bool first = k == 0;
+    Red<1> acc = red_identity<1, false>();
+    for (int c = lo + t; c < hi; c += nt) {
+      int jj[WC > 0 ? WC : 1];
+      float cf[WC > 0 ? WC : 1];
+      ZP vj[WC > 0 ? WC : 1];
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) {
+        const bool live = kk < w;
+        const size_t q_ = size_t(live ? kk : 0) * n + c;
+        const int j = A.nbr[q_];
+        const float h = A.coef[q_];
+        jj[kk] = (live && j >= 0) ? j : c;
+        cf[kk] = (live && j >= 0) ? h : 0.0f;
+      }
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) vj[kk] = cur[jj[kk]];
+      const float pc = dir(cur[c], zm, beta, first);
+      float qc = A.diag[c] * pc;
+#pragma unroll
+      for (int kk = 0; kk < WC; ++kk) qc = qc - cf[kk] * dir(vj[kk], zm, beta, first);
+      nxt[c].y = pc;
+      q[c] = qc;
+      acc.v[0] = acc.v[0] + double(pc) * double(qc);
+    }
+    return acc;
+  }
 };
+using FcgDir32 = FcgDir32T<0>;
 
 // alpha = (z.r) / (p.q); e += alpha p; r_old = r; r -= alpha q; partial r.r
 struct FcgUpd32 {
   float *e, *r, *rold;
-  const float *p, *q;
+  const ZP* p;  // the direction in y
+  const float* q;
   CgScalars* sc;
   int k;
   struct Scal {
@@ -946,7 +1005,7 @@ struct FcgUpd32 {
   }
   DM_HD Red<1> operator()(int c, const Scal& s) const {
     const float a = float(s.alpha);
-    e[c] = e[c] + a * p[c];
+    e[c] = e[c] + a * p[c].y;
     const float rc = r[c];
     rold[c] = rc;
     const float rn = rc - a * q[c];
@@ -1021,8 +1080,10 @@ class CgT {
   void set_pc32(FcgPc32* pc) {
     pc32_ = pc;
     if (!pc) return;
-    for (float** v : {&r32_, &ro32_, &z32_, &pa32_, &pb32_, &q32_})
+    for (float** v : {&r32_, &ro32_, &q32_})
       if (!*v) *v = x_->template alloc<float>(n_);
+    for (ZP** v : {&zp32_[0], &zp32_[1]})
+      if (!*v) *v = x_->template alloc<ZP>(n_);
     if (fpart32_g_ < pc->out_grid32(n_)) {
       if (fpart32_) x_->release(fpart32_);
       fpart32_g_ = pc->out_grid32(n_);
@@ -1047,10 +1108,11 @@ class CgT {
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
                     (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_, (void*)e32_, (void*)r32_,
-                    (void*)ro32_, (void*)z32_, (void*)pa32_, (void*)pb32_, (void*)q32_, (void*)fpart32_,
+                    (void*)ro32_, (void*)zp32_[0], (void*)zp32_[1], (void*)q32_, (void*)fpart32_,
                     (void*)coef32_, (void*)diag32_})
       if (p) x_->release(p);
-    r32_ = ro32_ = z32_ = pa32_ = pb32_ = q32_ = coef32_ = diag32_ = nullptr;
+    r32_ = ro32_ = q32_ = coef32_ = diag32_ = nullptr;
+    zp32_[0] = zp32_[1] = nullptr;
     fpart32_ = nullptr;
     fpart32_g_ = 0;
     pc32_ = nullptr;
@@ -1126,23 +1188,28 @@ class CgT {
       xx.zero(e32_, size_t(n));
       for (int sweep = 0;; ++sweep) {
         xx.template pass<2, 1>(n, FcgStart32{r_, r32_, ro32_, x, e32_, sc_, sweep, MAX_SWEEPS, tau_}, in, g, RR);
-        float* pold = pa32_;
-        float* pnew = pb32_;
         int kin = 0;
         int ibatch = std::max(4, std::min(maxit + 1, predict_sw_[std::min(sweep, MAX_SWEEPS)]));
         for (;;) {
           for (int i = 0; i < ibatch; ++i, ++kin) {
-            pc32->apply32(kin, r32_, ro32_, z32_, sc_, RR, g, fpart32_);
+            ZP* cur = zp32_[kin & 1];
+            ZP* nxt = zp32_[(kin + 1) & 1];
+            pc32->apply32(kin, r32_, ro32_, &cur->x, 2, sc_, RR, g, fpart32_);
+            xx.tbegin(K_CG_DIR);
+            by_width(A32.w, [&](auto wc) {
+              xx.template pass<5, 1>(n, FcgDir32T<decltype(wc)::value>{A32, cur, nxt, q32_, sc_, kin}, fpart32_, gt,
+                                     D);
+            });
+            xx.tend(K_CG_DIR);
             xx.tbegin(K_CG_UPD);
-            xx.template pass<5, 1>(n, FcgDir32{A32, z32_, pold, pnew, q32_, sc_, kin}, fpart32_, gt, D);
+            xx.template pass<1, 1>(n, FcgUpd32{e32_, r32_, ro32_, nxt, q32_, sc_, kin}, D, g, RR);
             xx.tend(K_CG_UPD);
-            xx.template pass<1, 1>(n, FcgUpd32{e32_, r32_, ro32_, pnew, q32_, sc_, kin}, D, g, RR);
-            std::swap(pold, pnew);
           }
           xx.download(&h, sc_, 1);
           if (h.done || h.inner_done || kin > maxit + 1) break;
           ibatch = std::max(2, std::min(16, (h.inner_its + 3) / 4));
         }
+        for (int kc : {int(K_CG_DIR), int(K_CG_UPD), int(K_MG_DOWN), int(K_MG_UP)}) xx.tcommit(kc, h.inner_its);
         predict_sw_[std::min(sweep, MAX_SWEEPS)] = std::max(1, h.inner_its + 1);
         if (h.done) break;
         total += h.inner_its;
@@ -1234,10 +1301,8 @@ class CgT {
     xx.download(&h, sc_, 1);
     double rr = 0.0;
     for (int q = 0; q < g; ++q) rr = rr + hpart_[q];
-    if (!mixed_ || pc32) {
-      xx.tcommit(K_CG_DIR, h.its);
-      xx.tcommit(K_CG_UPD, h.its);
-    }
+    if (!mixed_ || pc32)
+      for (int kc : {int(K_CG_DIR), int(K_CG_UPD), int(K_MG_DOWN), int(K_MG_UP)}) xx.tcommit(kc, h.its);
     last_its_ = h.its;
     last_ttol_ = h.ttol;
     last_atol_ = h.atol;
@@ -1252,14 +1317,7 @@ class CgT {
   // Calls fn with the slot count of the batched passes for an operator of width w (0: generic).
   template <class Fn>
   static void by_width(int w, Fn&& fn) {
-    if (w <= 4)
-      fn(std::integral_constant<int, 4>{});
-    else if (w <= 6)
-      fn(std::integral_constant<int, 6>{});
-    else if (w <= 8)
-      fn(std::integral_constant<int, 8>{});
-    else
-      fn(std::integral_constant<int, 0>{});
+    with_width(w, fn);
   }
 
   // chat = coef / diag(column), dinv = 1 / diag (and FP32 copies when mixed), built on the device
@@ -1343,7 +1401,8 @@ class CgT {
   FcgPc* fcg_ = nullptr;
   FcgPc32* pc32_ = nullptr;
   bool plain_ = false;
-  float *r32_ = nullptr, *ro32_ = nullptr, *z32_ = nullptr, *pa32_ = nullptr, *pb32_ = nullptr, *q32_ = nullptr;
+  float *r32_ = nullptr, *ro32_ = nullptr, *q32_ = nullptr;
+  ZP* zp32_[2] = {nullptr, nullptr};
   double* fpart32_ = nullptr;
   int fpart32_g_ = 0;
   CgCapture* capture_ = nullptr;

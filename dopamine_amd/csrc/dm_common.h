@@ -122,6 +122,8 @@ struct HostExec {
 };
 
 // Kernel classes with device timing (dopamine_solver_kernel_timing).
-enum KernelClass { K_CG_DIR = 0, K_CG_UPD = 1, K_EXPLICIT = 2, K_LS_GRAD = 3, K_NCLASS = 4 };
+enum KernelClass {
+  K_CG_DIR = 0, K_CG_UPD = 1, K_EXPLICIT = 2, K_LS_GRAD = 3, K_MG_DOWN = 4, K_MG_UP = 5, K_NCLASS = 6
+};
 
 }  // namespace dm

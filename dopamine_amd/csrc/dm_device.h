@@ -169,7 +169,7 @@ struct KernelTimer {
   bool created = false;
   // Classes whose launches are kept only when committed: the CG's passes after convergence are
   // gated no-ops, so a solve commits just its first `iterations` launches.
-  bool deferred[K_NCLASS] = {true, true, false, false};
+  bool deferred[K_NCLASS] = {true, true, false, false, true, true};
   std::vector<float> pending[K_NCLASS];
 
   void create() {

@@ -20,6 +20,7 @@ std::vector<std::string> g_options;
 int g_local_rank = -1;
 int g_io_on_device = 0;
 int g_initialized = 0;
+int g_last_info[4] = {0, 0, 0, 0};  // first solve's rc and iterations, retried, total iterations
 
 // Cached solve context: rebuilt when (n_local, n_global, n_faces, reference row) change
 // (pressure_backend_petsc.c:687-715).
@@ -256,8 +257,11 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
     const int rc = dm::solve_with_retry(
         maxit, rtol,
         [&](int* it, double* r) {
-          return c.learned ? c.learned->solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r)
-                           : c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r);
+          const int q = c.learned ? c.learned->solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r)
+                                  : c.cg.solve(c.A, b, x, rtol, atol, nullptr, maxit, ns, it, r);
+          g_last_info[0] = q;
+          g_last_info[1] = *it;
+          return q;
         },
         [&](int mi, int* it, double* r) {
           c.cg.set_plain(true);  // the retry runs the Jacobi passes
@@ -266,6 +270,8 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
           return q;
         },
         [&] { c.x.zero(x, size_t(n)); }, [&] { return atol; }, &its, &res, &total, &retried);
+    g_last_info[2] = retried;
+    g_last_info[3] = total;
     if (!g_io_on_device) c.x.download(solution, c.sol, n);
     c.x.sync();
     *iteration_count = its;
@@ -274,6 +280,10 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
   } catch (const std::exception&) {
     return -33;
   }
+}
+
+void dopamine_backend_last_solve(int out[4]) {
+  for (int i = 0; i < 4; ++i) out[i] = g_last_info[i];
 }
 
 int dopamine_mpi_gpu_aware(void) {

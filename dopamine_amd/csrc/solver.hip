@@ -175,6 +175,13 @@ int dopamine_solver_advance(void* h, double dt_req, double* dt_used) {
   });
 }
 
+int dopamine_solver_cell_net_flux(void* h, double* out) {
+  return guard([&] {
+    *out = static_cast<Solver*>(h)->cell_net_flux_max();
+    return 0;
+  });
+}
+
 int dopamine_solver_state_is_finite(void* h) {
   int ok = 0;
   const int rc = guard([&] {

@@ -172,6 +172,15 @@ class SolverT {
   int n_amdw = 0;
   int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
+  // Multigrid set-up: AmgCfg's defaults; DOPAMINE_AMG_THETA / _ALPHA / _COARSE override the
+  // strength threshold, the coarse-correction scaling and the densely solved size.
+  static AmgCfg amg_cfg() {
+    AmgCfg c;
+    if (const char* e = std::getenv("DOPAMINE_AMG_THETA")) c.theta = std::atof(e);
+    if (const char* e = std::getenv("DOPAMINE_AMG_ALPHA")) c.alpha = std::atof(e);
+    if (const char* e = std::getenv("DOPAMINE_AMG_COARSE")) c.coarse_max = std::max(1, std::atoi(e));
+    return c;
+  }
   static double mixed_tau() {
     const char* e = std::getenv("DOPAMINE_MIXED_TAU");
     const double t = e ? std::atof(e) : 1.0e-4;
@@ -694,7 +703,7 @@ class SolverT {
       op.coef = acoef;
       op.diag = diag;
       amg.reset(new AmgPc<X>());
-      amg->build(&x, op, A.nbr, AmgCfg{});
+      amg->build(&x, op, A.nbr, amg_cfg());
       if (p.precision == 1)
         cg.set_pc32(amg.get());
       else
@@ -1124,6 +1133,15 @@ class SolverT {
     pressure_gradient();
     x.sync();
     return rc;
+  }
+
+  // max over the cells of |sum of the projected face fluxes| (m^3/s): the discrete divergence
+  // integrated over the cell, the per-cell measure of the projection's mass error
+  double cell_net_flux_max() {
+    x.template reduce<1, true>(nc, FluxDivergence{M, UF, 1}, sk::ProjFin<1>{sc, 0});
+    SolverScalars h;
+    x.download(&h, sc, 1);
+    return h.pj[0];
   }
 
   // solver_state_is_finite (src/solver/mod_fv_solver.f90:3925-3945): U and p hold no NaN or Inf

@@ -111,13 +111,14 @@ def _bind(L):
     L.dopamine_solver_stats_csv.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _d, _P(_d), ctypes.c_int]
     L.dopamine_solver_learned_loss.argtypes = [ctypes.c_void_p, _P(_d), ctypes.c_int]
     L.dopamine_solver_state_is_finite.argtypes = [ctypes.c_void_p]
+    L.dopamine_solver_cell_net_flux.argtypes = [ctypes.c_void_p, _P(_d)]
     L.dopamine_solver_retries.restype = _i64
     L.dopamine_solver_retries.argtypes = [ctypes.c_void_p]
     return L
 
 
 # Kernel classes with device timing (include/dopamine_hip.h).
-KERNELS = {"cg_dir": 0, "cg_upd": 1, "explicit": 2, "ls_grad": 3}
+KERNELS = {"cg_dir": 0, "cg_upd": 1, "explicit": 2, "ls_grad": 3, "mg_down": 4, "mg_up": 5}
 
 
 def check_precision(params, precision):
@@ -223,6 +224,12 @@ class FVSolver:
             self.advance(dt)
             if check_every and step % check_every == 0 and not self.state_is_finite():
                 raise FloatingPointError(f"non-finite velocity/pressure detected at step {step}, t = {self.time:.4e}")
+
+    def cell_net_flux_max(self):
+        """max over the cells of |sum of the projected face fluxes| (m^3/s)."""
+        out = _d(0.0)
+        self._check(self.L.dopamine_solver_cell_net_flux(self.h, ctypes.byref(out)), "cell_net_flux")
+        return out.value
 
     @property
     def retries(self):

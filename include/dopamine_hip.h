@@ -137,6 +137,10 @@ int dopamine_solver_advance(void *solver, double dt_req, double *dt_used);
  * Inf (the sum of U and p over the cells, reduced on the device, is finite), 0 otherwise, negative
  * on error. The reference's time loop checks it every 10 steps (src/main.f90:434-442). */
 int dopamine_solver_state_is_finite(void *solver);
+/* max over the cells of |sum of the cell's projected face fluxes| (m^3/s, the discrete divergence
+ * integrated over the cell; solver_divergence_metrics, mod_fv_solver.f90:3641-3713, reports the
+ * same sum divided by the cell volume). No counterpart of its own in the reference. */
+int dopamine_solver_cell_net_flux(void *solver, double *out);
 /* Pressure solves that needed the retry from p = 0 (see dopamine_solver_project_once). */
 int64_t dopamine_solver_retries(void *solver);
 /* solver_compute_dt (src/solver/mod_fv_solver.f90:3864-3899). */
@@ -179,10 +183,13 @@ int dopamine_solver_sync(void *solver);
 /* Kernel classes with device timing. */
 enum {
   DOPAMINE_K_CG_DIR = 0,   /* pressure CG, first pass of an iteration: p update + operator apply
-                              + p.q (cg_variant 0), or the whole iteration (cg_variant 1) */
+                              + p.q (cg_variant 0 and the flexible CG), or the whole iteration
+                              (cg_variant 1) */
   DOPAMINE_K_CG_UPD = 1,   /* pressure CG, second pass (cg_variant 0): x/r update, Jacobi, dots */
   DOPAMINE_K_EXPLICIT = 2, /* advection + viscous + forcing + RK3 predictor */
-  DOPAMINE_K_LS_GRAD = 3   /* least-squares velocity gradient with the fused SGS viscosity */
+  DOPAMINE_K_LS_GRAD = 3,  /* least-squares velocity gradient with the fused SGS viscosity */
+  DOPAMINE_K_MG_DOWN = 4,  /* multigrid cycle, finest level: pre-smoothing and residual */
+  DOPAMINE_K_MG_UP = 5     /* multigrid cycle, finest level: post-smoothing and the flexible-CG partials */
 };
 /* Device timing of one kernel class: an event pair on the solver's stream that the dispatch of
  * each launch stamps at the kernel's start and end (hipExtLaunchKernelGGL).

@@ -58,6 +58,8 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.om_create.restype = ctypes.c_void_p
         L.om_create.argtypes = [_P(_Mesh), _P(_Params)]
+        L.om_set_threads.argtypes = [ctypes.c_int]
+        L.om_get_threads.restype = ctypes.c_int
         L.om_free.argtypes = [ctypes.c_void_p]
         L.om_array.restype = _P(_d)
         L.om_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _P(_i64)]
@@ -278,6 +280,12 @@ def psi_m(zeta):
 
 def psi_h(zeta):
     return lib().om_psi_h(zeta)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's Jacobi CG (1: the sequential face loops; more: the same CG as
+    row-parallel loops, used for the CPU baseline of the measurement)."""
+    lib().om_set_threads(int(n))
 
 
 def cg_jacobi(owner, neighbour, coeff, diag, b, x0, rtol, atol, max_it, nullspace):

@@ -1853,9 +1853,24 @@ static void remove_mean(int n, double *v) {
   for (int i = 0; i < n; ++i) v[i] = v[i] - mean;
 }
 
+/* Threads of the CG below (om_set_threads; 1 by default). With one thread the CG runs the
+ * sequential face loops above; with more, the same Jacobi CG runs as OpenMP loops over rows of a
+ * gathered (CSR) copy of the operator -- the same recurrence in another summation order. Only the
+ * CPU baseline of the measurement asks for more than one thread. */
+static int g_threads = 1;
+void om_set_threads(int n) { g_threads = n > 1 ? n : 1; }
+int om_get_threads(void) { return g_threads; }
+
+static int cg_jacobi_omp(int n, int nfaces, const int32_t *owner, const int32_t *neighbour, const double *coeff,
+                         const double *diag_in, const double *b_in, double *x, double rtol, double atol, int max_it,
+                         int nullspace, int *its, double *resid);
+
 int om_cg_jacobi(int n, int nfaces, const int32_t *owner, const int32_t *neighbour, const double *coeff,
                  const double *diag_in, const double *b_in, double *x, double rtol, double atol, int max_it,
                  int nullspace, int *its, double *resid) {
+  if (g_threads > 1)
+    return cg_jacobi_omp(n, nfaces, owner, neighbour, coeff, diag_in, b_in, x, rtol, atol, max_it, nullspace, its,
+                         resid);
   double *diag = dz(n), *b = dz(n), *r = dz(n), *z = dz(n), *p = dz(n), *q = dz(n);
   for (int i = 0; i < n; ++i) {
     double d = diag_in[i];
@@ -1912,6 +1927,109 @@ int om_cg_jacobi(int n, int nfaces, const int32_t *owner, const int32_t *neighbo
   *resid = sqrt(rr);
   *its = it;
   free(diag); free(b); free(r); free(z); free(p); free(q);
+  return conv ? 0 : -1;
+}
+
+/* The Jacobi CG of om_cg_jacobi on g_threads OpenMP threads: rows gathered into CSR once per call. */
+static int cg_jacobi_omp(int n, int nfaces, const int32_t *owner, const int32_t *neighbour, const double *coeff,
+                         const double *diag_in, const double *b_in, double *x, double rtol, double atol, int max_it,
+                         int nullspace, int *its, double *resid) {
+  const int nt = g_threads;
+  int *ptr = (int *)calloc((size_t)n + 1, sizeof(int));
+  for (int f = 0; f < nfaces; ++f) {
+    if (owner[f] < 0 || neighbour[f] < 0 || !isfinite(coeff[f]) || coeff[f] == 0.0) continue;
+    ptr[owner[f] + 1]++;
+    ptr[neighbour[f] + 1]++;
+  }
+  for (int i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+  int *col = (int *)malloc(sizeof(int) * (size_t)(ptr[n] > 0 ? ptr[n] : 1));
+  double *val = (double *)malloc(sizeof(double) * (size_t)(ptr[n] > 0 ? ptr[n] : 1));
+  int *at = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) at[i] = ptr[i];
+  for (int f = 0; f < nfaces; ++f) {
+    if (owner[f] < 0 || neighbour[f] < 0 || !isfinite(coeff[f]) || coeff[f] == 0.0) continue;
+    col[at[owner[f]]] = neighbour[f];
+    val[at[owner[f]]++] = coeff[f];
+    col[at[neighbour[f]]] = owner[f];
+    val[at[neighbour[f]]++] = coeff[f];
+  }
+  double *diag = dz(n), *b = dz(n), *r = dz(n), *z = dz(n), *p = dz(n), *q = dz(n);
+  double sb = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : sb)
+  for (int i = 0; i < n; ++i) {
+    double d = diag_in[i];
+    if (!isfinite(d) || fabs(d) < 1.0e-20) d = 1.0;
+    diag[i] = d;
+    b[i] = isfinite(b_in[i]) ? b_in[i] : 0.0;
+    if (!isfinite(x[i])) x[i] = 0.0;
+    sb += b[i];
+  }
+  if (nullspace) {
+    const double mb = sb / (double)n;
+#pragma omp parallel for num_threads(nt)
+    for (int i = 0; i < n; ++i) b[i] -= mb;
+  }
+#define OM_SPMV(xv, yv)                                                              \
+  _Pragma("omp parallel for num_threads(nt)") for (int i = 0; i < n; ++i) {          \
+    double y = diag[i] * (xv)[i];                                                    \
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) y -= val[k] * (xv)[col[k]];            \
+    (yv)[i] = y;                                                                     \
+  }
+  double bb = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : bb)
+  for (int i = 0; i < n; ++i) bb += b[i] * b[i];
+  const double ttol = fmax(rtol * sqrt(bb), atol);
+  OM_SPMV(x, q)
+  double rr = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : rr)
+  for (int i = 0; i < n; ++i) {
+    r[i] = b[i] - q[i];
+    rr += r[i] * r[i];
+  }
+  int it = 0, conv = sqrt(rr) <= ttol;
+  double beta = 0.0, betaold = 1.0;
+  while (!conv && it < max_it) {
+    double rz = 0.0, sz = 0.0, sr = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : rz, sz, sr)
+    for (int i = 0; i < n; ++i) {
+      z[i] = r[i] / diag[i];
+      rz += r[i] * z[i];
+      sz += z[i];
+      sr += r[i];
+    }
+    const double zm = nullspace ? sz / (double)n : 0.0;
+    betaold = beta;
+    beta = rz - zm * sr; /* r.(z - mean z) */
+    const double bq = it == 0 ? 0.0 : beta / betaold;
+#pragma omp parallel for num_threads(nt)
+    for (int i = 0; i < n; ++i) p[i] = (z[i] - zm) + bq * p[i];
+    double pq = 0.0;
+    OM_SPMV(p, q)
+#pragma omp parallel for num_threads(nt) reduction(+ : pq)
+    for (int i = 0; i < n; ++i) pq += p[i] * q[i];
+    const double a = beta / pq;
+    rr = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : rr)
+    for (int i = 0; i < n; ++i) {
+      x[i] += a * p[i];
+      r[i] -= a * q[i];
+      rr += r[i] * r[i];
+    }
+    it++;
+    conv = sqrt(rr) <= ttol;
+  }
+  OM_SPMV(x, q)
+  double tr = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : tr)
+  for (int i = 0; i < n; ++i) {
+    const double bi = isfinite(b_in[i]) ? b_in[i] : 0.0;
+    tr += (bi - q[i]) * (bi - q[i]);
+  }
+#undef OM_SPMV
+  *resid = sqrt(tr);
+  *its = it;
+  free(diag); free(b); free(r); free(z); free(p); free(q);
+  free(ptr); free(col); free(val); free(at);
   return conv ? 0 : -1;
 }
 

@@ -122,8 +122,11 @@ def test_multigrid_selectors_solve_with_far_fewer_iterations(name, selector):
     ell = _ell(n, own[keep], nb[keep], coeff[keep])
     H = om.Hierarchy(ell[0], ell[1], diag)
     _, its_o = om.fcg(A, b, H.vcycle, rtol=rtol, nullspace=n_dir == 0)
-    print(f"[gpu] {selector} {name}: rc {rc} its {its} (statement {its_o}, Jacobi {its_j}) resid {res:.3e}",
-          flush=True)
+    info = (ctypes.c_int * 4)()
+    L.dopamine_backend_last_solve(info)
+    print(f"[gpu] {selector} {name}: rc {rc} its {its} (statement {its_o}, Jacobi {its_j}) resid {res:.3e}; "
+          f"first solve rc {info[0]} its {info[1]}, retried {info[2]}", flush=True)
+    assert info[2] == 0
     assert rc == 0
     assert res <= rtol * np.linalg.norm(b) * 1.001
     assert its <= its_o + 3 + (8 if selector.startswith("mixed") else 0)
@@ -197,6 +200,7 @@ def test_solve_stopped_at_its_limit_is_retried_from_zero_then_fails_with_minus_9
     L.fvm_petsc_insert_option.argtypes = [ctypes.c_char_p]
     L.fvm_petsc_set_pressure_solver.argtypes = [ctypes.c_char_p]
     L.fvm_petsc_insert_option(b"-ksp_type cg")
+    L.fvm_petsc_set_pressure_solver(b"jacobi")
     b = np.random.default_rng(21).standard_normal(n)
     b -= b.mean()
     rc, x, need, res = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, 0)
@@ -213,9 +217,10 @@ def test_solve_stopped_at_its_limit_is_retried_from_zero_then_fails_with_minus_9
         L.fvm_petsc_set_pressure_solver(b"no-such-solver")
         rc, _, _, _ = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, 0)
         assert rc == -11
+        L.fvm_petsc_set_pressure_solver(b"jacobi")
     finally:
         L.fvm_petsc_insert_option(b"-ksp_max_it 0")
-        L.fvm_petsc_set_pressure_solver(b"boomeramg")
     rc, _, its, _ = _call(n, own, nb, skip, diag, b, coeff, np.zeros(n), 1e-12, 1e-300, 0)
     assert rc == 0 and its == need
+    L.fvm_petsc_set_pressure_solver(b"boomeramg")
     assert L.fvm_petsc_backend_finalize() == 0

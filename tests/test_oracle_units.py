@@ -240,3 +240,27 @@ def test_amd_on_mesh_axisymmetric_compression():
     # stretching along x (a > 0) is backscatter for AMD: clipped to zero
     mesh, o = _closures(_params(sgs_model="AMD", c_amd=C_AMD, wall_model="none"), np.diag([-a, a / 2, a / 2]))
     assert np.abs(o.array("nu_sgs")[_inner(mesh, 3)]).max() == 0.0
+
+
+def test_threaded_cg_is_the_same_jacobi_cg():
+    """The oracle's OpenMP CG (the CPU baseline's) against its sequential CG on a regression
+    operator: the same iteration count to within one and the same solution to the tolerance."""
+    import oracle
+    case = cases.regression_cases()["hydrowall"]
+    mesh = case.make_mesh()
+    o = oracle.Solver(mesh, case.params)
+    own, nb, skip = o.int_array("ps_owner").copy(), o.int_array("ps_nb").copy(), o.int_array("ps_skip").copy()
+    coeff, diag = o.array("ps_coeff").copy(), o.array("ps_diag").copy()
+    keep = skip == 0
+    b = np.random.default_rng(9).standard_normal(mesh.n_cells)
+    b -= b.mean()
+    x1, it1, r1, rc1 = oracle.cg_jacobi(own[keep], nb[keep], coeff[keep], diag, b, np.zeros_like(b), 1e-10, 1e-300,
+                                        2000, True)
+    try:
+        oracle.set_threads(4)
+        x4, it4, r4, rc4 = oracle.cg_jacobi(own[keep], nb[keep], coeff[keep], diag, b, np.zeros_like(b), 1e-10,
+                                            1e-300, 2000, True)
+    finally:
+        oracle.set_threads(1)
+    assert rc1 == rc4 == 0 and abs(it1 - it4) <= 1
+    assert np.linalg.norm(x4 - x1) <= 1e-8 * np.linalg.norm(x1)
