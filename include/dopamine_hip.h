@@ -199,6 +199,10 @@ int dopamine_solver_kernel_timing(void *solver, int kernel, int enable);
 /* Launches timed since enabling and their summed device duration in milliseconds. CG passes that
  * run after convergence (gated no-ops) are not counted. */
 int dopamine_solver_kernel_time(void *solver, int kernel, int64_t *launches, double *total_ms);
+/* Outcome of the last fvm_petsc_solve_pressure call of the pressure backend (no counterpart in the
+ * reference): {return code of the first solve, its iterations, 1 if it was repeated from x = 0,
+ * iterations of both solves}. */
+void dopamine_backend_last_solve(int out[4]);
 /* Measurement utility (no counterpart in the reference): copies n doubles between two fresh HBM
  * buffers reps times with a plain grid-stride kernel and returns the mean kernel duration; the
  * known 16 n bytes per launch calibrate the HBM traffic counters and the achievable bandwidth. */

@@ -60,10 +60,6 @@ int fvm_petsc_solve_pressure(int n_cells_local, int n_cells_global, int n_faces,
 /* Whether MPI may receive device pointers: DOPAMINE_GPU_AWARE_MPI=0/1 override, else 0
  * (src/mpi_gpu_support.c:13-21). */
 int dopamine_mpi_gpu_aware(void);
-/* Outcome of the last fvm_petsc_solve_pressure call (no counterpart in the reference):
- * {return code of the first solve, its iterations, 1 if it was repeated from x = 0, iterations of
- * both solves}. */
-void dopamine_backend_last_solve(int out[4]);
 /* Profiler range markers (src/io/nvtx_shim.c:7-8), emitted as ROCTx ranges. */
 void dopamine_nvtx_push(const char *name);
 void dopamine_nvtx_pop(void);
