@@ -1060,6 +1060,38 @@ class CgT {
     sc_ = x->template alloc<CgScalars>(1);
     part_ = x->template alloc<double>(size_t(PARTS) * g_);
     hpart_.resize(g_);
+    hp_ = hpart_.data();
+  }
+  // Bytes of caller memory that init_external carves the work vectors and partials from (a
+  // multiple of 256; each piece 256-aligned).
+  static size_t external_bytes(int n, int g) {
+    const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return 6 * al(sizeof(double) * size_t(n)) + al(sizeof(CgScalars)) + al(sizeof(double) * PARTS * size_t(g));
+  }
+  // The two-pass Jacobi CG on memory the caller owns: the work vectors and partials are carved
+  // from mem (external_bytes(n, pass_grid(n)) bytes in the execution space), the host copy of the
+  // residual partials goes to host_part (pass_grid(n) doubles). Nothing is allocated, here or in
+  // solve(); destroy() releases nothing. Only the plain Jacobi passes run on such a workspace.
+  void init_external(X* x, int n, void* mem, double* host_part) {
+    x_ = x;
+    n_ = n;
+    g_ = x->pass_grid(n);
+    external_ = true;
+    char* m = static_cast<char*>(mem);
+    const auto take = [&](size_t b) {
+      char* p = m;
+      m += (b + 255) / 256 * 256;
+      return p;
+    };
+    bw_ = reinterpret_cast<double*>(take(sizeof(double) * size_t(n)));
+    r_ = reinterpret_cast<double*>(take(sizeof(double) * size_t(n)));
+    z_ = reinterpret_cast<double*>(take(sizeof(double) * size_t(n)));
+    pa_ = reinterpret_cast<double*>(take(sizeof(double) * size_t(n)));
+    pb_ = reinterpret_cast<double*>(take(sizeof(double) * size_t(n)));
+    q_ = reinterpret_cast<double*>(take(sizeof(double) * size_t(n)));
+    sc_ = reinterpret_cast<CgScalars*>(take(sizeof(CgScalars)));
+    part_ = reinterpret_cast<double*>(take(sizeof(double) * PARTS * size_t(g_)));
+    hp_ = host_part;
   }
   // Hestenes-Stiefel two passes per iteration (PETSc KSPCG's recurrence) or the single-reduction
   // variant, one pass per iteration.
@@ -1105,6 +1137,10 @@ class CgT {
   int last_sweeps() const { return last_sweeps_; }
   void destroy() {
     if (!x_) return;
+    if (external_) {
+      x_ = nullptr;
+      return;
+    }
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
                     (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_, (void*)e32_, (void*)r32_,
@@ -1146,6 +1182,8 @@ class CgT {
     h.maxit = maxit;
     h.nullspace = nullspace ? 1 : 0;
     xx.upload(sc_, &h, 1);
+    if (external_ && (single_ || mixed_ || fcg_ || pc32_))
+      throw std::runtime_error("an external CG workspace runs the two-pass Jacobi CG only");
     JacobiScaled J;
     if (single_ || mixed_) J = scaled(A);
     xx.template pass<1, 1>(n, SumB{b, bw_, x}, nullptr, 0, P0);
@@ -1297,10 +1335,10 @@ class CgT {
       }
     }
     xx.template pass<1, 1>(n, Resid{A, b, x}, nullptr, 0, R);
-    xx.download(hpart_.data(), R, size_t(g));
+    xx.download(hp_, R, size_t(g));
     xx.download(&h, sc_, 1);
     double rr = 0.0;
-    for (int q = 0; q < g; ++q) rr = rr + hpart_[q];
+    for (int q = 0; q < g; ++q) rr = rr + hp_[q];
     if (!mixed_ || pc32)
       for (int kc : {int(K_CG_DIR), int(K_CG_UPD), int(K_MG_DOWN), int(K_MG_UP)}) xx.tcommit(kc, h.its);
     last_its_ = h.its;
@@ -1414,6 +1452,8 @@ class CgT {
   double* part_ = nullptr;
   CgScalars* sc_ = nullptr;
   std::vector<double> hpart_;
+  double* hp_ = nullptr;   // host copy of the residual partials (hpart_ or the caller's array)
+  bool external_ = false;  // work vectors carved from caller memory (init_external)
   int last_its_ = 0, predict_ = 24;
   double last_ttol_ = 0.0, last_atol_ = 0.0;
   int predict_fcg_ = 24;

@@ -2,6 +2,7 @@
 // src/solver/pressure_backend_petsc.c, served by the device-resident Jacobi CG on gfx950.
 #include <rocprofiler-sdk-roctx/roctx.h>
 
+#include <cstdio>
 #include <cstdlib>
 #include <memory>
 #include <string>
@@ -166,9 +167,167 @@ int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner
   return 0;
 }
 
+// ---- the allocation-free variant (fvm_petsc_solve_pressure_noalloc)
+//
+// Its device memory is this file-scope array, one copy per GPU, sized for operators of at most
+// NOALLOC_MAX_CELLS rows and NOALLOC_MAX_WIDTH off-diagonal entries per row: the ELL operator
+// (neighbours, coefficients, diagonal), then the CG's work vectors and partials. Host code never
+// uses the array's name as an address: scratch_ptr() looks the device address up once per device
+// (hipGetSymbolAddress) and everything else uses that pointer. The ELL rows are assembled in
+// host arrays of static storage. Every call on a GPU reuses the same memory, so calls that overlap
+// (on one GPU, or from several host threads) race; each call writes every word it reads.
+constexpr int NOALLOC_MAX_CELLS = 131072;
+constexpr int NOALLOC_MAX_WIDTH = 8;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+constexpr size_t noalloc_al(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NOALLOC_NBR_BYTES = noalloc_al(sizeof(int) * NOALLOC_MAX_WIDTH * size_t(NOALLOC_MAX_CELLS));
+constexpr size_t NOALLOC_COEF_BYTES = noalloc_al(sizeof(double) * NOALLOC_MAX_WIDTH * size_t(NOALLOC_MAX_CELLS));
+constexpr size_t NOALLOC_DIAG_BYTES = noalloc_al(sizeof(double) * size_t(NOALLOC_MAX_CELLS));
+constexpr size_t NOALLOC_OP_BYTES = NOALLOC_NBR_BYTES + NOALLOC_COEF_BYTES + NOALLOC_DIAG_BYTES;
+// the CG's work vectors and partials for the largest operator (the pass grid never exceeds PGRID_MAX)
+constexpr size_t NOALLOC_CG_BYTES = 6 * noalloc_al(sizeof(double) * size_t(NOALLOC_MAX_CELLS)) +
+                                    noalloc_al(sizeof(dm::CgScalars)) +
+                                    noalloc_al(sizeof(double) * 17 * size_t(dm::PGRID_MAX));
+constexpr size_t NOALLOC_BYTES = NOALLOC_OP_BYTES + NOALLOC_CG_BYTES;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+unsigned char* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+int h_cnt[NOALLOC_MAX_CELLS];
+int h_nbr[NOALLOC_MAX_WIDTH * NOALLOC_MAX_CELLS];
+double h_coef[NOALLOC_MAX_WIDTH * NOALLOC_MAX_CELLS];
+double h_diag[NOALLOC_MAX_CELLS];
+double h_part[dm::PGRID_MAX];
+char g_noalloc_error[256] = "";
+
+// Device address of this GPU's scratch array (looked up on the first call on each device).
+unsigned char* scratch_ptr() {
+  int dev = 0;
+  DM_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) throw std::runtime_error("device index beyond the scratch table");
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    DM_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
 }  // namespace
 
 extern "C" {
+
+const char* dopamine_backend_last_error(void) { return g_noalloc_error; }
+
+int fvm_petsc_solve_pressure_noalloc(int n_cells_local, int n_cells_global, int n_faces, const int* local_to_global,
+                                     const int* face_owner, const int* face_neighbor_local,
+                                     const int* face_neighbor_global, const int* skip_face, const double* diagonal,
+                                     const double* rhs, const double* face_coeff, int reference_cell,
+                                     int max_iterations, double rtol, double atol, const int* dirichlet_rows,
+                                     const double* dirichlet_values, int n_dirichlet_rows, double* solution,
+                                     int* iteration_count, double* residual_norm) {
+  (void)dirichlet_rows;
+  (void)dirichlet_values;
+  (void)reference_cell;
+  g_noalloc_error[0] = '\0';
+  const auto fail = [&](int code, const char* what) {
+    std::snprintf(g_noalloc_error, sizeof(g_noalloc_error), "%s", what);
+    return code;
+  };
+  try {
+    const int n = n_cells_local;
+    if (n <= 0) return fail(-10, "no cells");
+    if (n > NOALLOC_MAX_CELLS)
+      return fail(-61, "fvm_petsc_solve_pressure_noalloc: more cells than its scratch holds (131072)");
+    if (n_cells_global != n) return fail(-46, "one rank: every row is local");
+    for (int i = 0; i < n; ++i)
+      if (local_to_global[i] != i + 1) return fail(-45, "compact ids on one rank are the identity");
+    // ELL rows in face order, owner side then neighbour side (the slot order of the cached path)
+    const auto keep = [&](int f) {
+      const double c = face_coeff[f];
+      return skip_face[f] == 0 && face_neighbor_global[f] > 0 && face_owner[f] >= 1 && face_owner[f] <= n &&
+             std::isfinite(c) && c != 0.0;
+    };
+    for (int i = 0; i < n; ++i) h_cnt[i] = 0;
+    for (int f = 0; f < n_faces; ++f) {
+      if (!keep(f)) continue;
+      const int o = face_owner[f] - 1, g = face_neighbor_global[f] - 1;
+      if (g < 0 || g >= n) return fail(-16, "face neighbour out of range");
+      h_cnt[o]++;
+      if (face_neighbor_local[f] > 0) {
+        const int nl = face_neighbor_local[f] - 1;
+        if (nl < 0 || nl >= n) return fail(-17, "face neighbour out of range");
+        h_cnt[nl]++;
+      }
+    }
+    int w = 1;
+    for (int i = 0; i < n; ++i) w = std::max(w, h_cnt[i]);
+    if (w > NOALLOC_MAX_WIDTH)
+      return fail(-62, "fvm_petsc_solve_pressure_noalloc: a row with more than 8 neighbours");
+    for (int i = 0; i < w * n; ++i) {
+      h_nbr[i] = -1;
+      h_coef[i] = 0.0;
+    }
+    for (int i = 0; i < n; ++i) h_cnt[i] = 0;
+    for (int f = 0; f < n_faces; ++f) {
+      if (!keep(f)) continue;
+      const int o = face_owner[f] - 1, g = face_neighbor_global[f] - 1;
+      const double c = face_coeff[f];
+      h_nbr[size_t(h_cnt[o]) * n + o] = g;
+      h_coef[size_t(h_cnt[o]++) * n + o] = c;
+      if (face_neighbor_local[f] > 0) {
+        const int nl = face_neighbor_local[f] - 1;
+        h_nbr[size_t(h_cnt[nl]) * n + nl] = o;
+        h_coef[size_t(h_cnt[nl]++) * n + nl] = c;
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      double d = diagonal[i];
+      if (!std::isfinite(d) || std::fabs(d) < 1.0e-20) d = 1.0;
+      h_diag[i] = d;
+    }
+    unsigned char* base = scratch_ptr();
+    dm::DeviceExec ex;  // the null stream; no partials buffer, no timer: nothing allocated
+    ex.stream = nullptr;
+    int* d_nbr = reinterpret_cast<int*>(base);
+    double* d_coef = reinterpret_cast<double*>(base + NOALLOC_NBR_BYTES);
+    double* d_diag = reinterpret_cast<double*>(base + NOALLOC_NBR_BYTES + NOALLOC_COEF_BYTES);
+    ex.upload(d_nbr, h_nbr, size_t(w) * n);
+    ex.upload(d_coef, h_coef, size_t(w) * n);
+    ex.upload(d_diag, h_diag, size_t(n));
+    dm::CgT<dm::DeviceExec> cg;
+    if (dm::CgT<dm::DeviceExec>::external_bytes(n, ex.pass_grid(n)) > NOALLOC_CG_BYTES)
+      return fail(-63, "scratch too small for the CG workspace");
+    cg.init_external(&ex, n, base + NOALLOC_OP_BYTES, h_part);
+    dm::EllOp A;
+    A.n = n;
+    A.w = w;
+    A.nbr = d_nbr;
+    A.coef = d_coef;
+    A.diag = d_diag;
+    int maxit = max_iterations > 2000 ? max_iterations : 2000;
+    for (const auto& o : g_options)
+      if (o.rfind("-ksp_max_it", 0) == 0) {
+        const int v = std::atoi(o.c_str() + 11);
+        maxit = v > 0 ? v : (max_iterations > 2000 ? max_iterations : 2000);
+      }
+    const bool ns = n_dirichlet_rows <= 0;
+    int its = 0, total = 0, retried = 0;
+    double res = 0.0;
+    auto once = [&](int mi, int* it, double* r) { return cg.solve(A, rhs, solution, rtol, atol, nullptr, mi, ns, it, r); };
+    const int rc = dm::solve_with_retry(
+        maxit, rtol, [&](int* it, double* r) { return once(maxit, it, r); }, once,
+        [&] { ex.zero(solution, size_t(n)); }, [&] { return atol; }, &its, &res, &total, &retried);
+    ex.sync();
+    *iteration_count = its;
+    *residual_norm = res;
+    return rc;
+  } catch (const std::exception& e) {  // a HIP error reported by the runtime
+    std::snprintf(g_noalloc_error, sizeof(g_noalloc_error), "%s", e.what());
+    return -33;
+  }
+}
+
 
 int fvm_petsc_backend_available(void) {
   int n = 0;

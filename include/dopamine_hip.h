@@ -199,6 +199,25 @@ int dopamine_solver_kernel_timing(void *solver, int kernel, int enable);
 /* Launches timed since enabling and their summed device duration in milliseconds. CG passes that
  * run after convergence (gated no-ops) are not counted. */
 int dopamine_solver_kernel_time(void *solver, int kernel, int64_t *launches, double *total_ms);
+/* fvm_petsc_solve_pressure without any allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and return codes as fvm_petsc_solve_pressure
+ * (fvm_pressure_backend.h), with rhs and solution always device pointers; the operator arrays are
+ * host arrays as there. It runs the Jacobi CG (KSPCG's two-pass recurrence, with the reference's
+ * retry and soft accept) whatever the selector, on the null stream. Its device memory is a
+ * file-scope array of the library, one copy per GPU, about 20 MB: operators of at most 131072
+ * rows and 8 off-diagonal entries per row; larger ones return -61 and -62 with the reason in
+ * dopamine_backend_last_error(). Calls share that memory and the host assembly arrays: calls that
+ * overlap (on one GPU or from several host threads) race. */
+int fvm_petsc_solve_pressure_noalloc(int n_cells_local, int n_cells_global, int n_faces, const int *local_to_global,
+                                     const int *face_owner, const int *face_neighbor_local,
+                                     const int *face_neighbor_global, const int *skip_face, const double *diagonal,
+                                     const double *rhs, const double *face_coeff, int reference_cell,
+                                     int max_iterations, double rtol, double atol, const int *dirichlet_rows,
+                                     const double *dirichlet_values, int n_dirichlet_rows, double *solution,
+                                     int *iteration_count, double *residual_norm);
+/* Reason of the last non-zero return of fvm_petsc_solve_pressure_noalloc ("" after success). */
+const char *dopamine_backend_last_error(void);
 /* Outcome of the last fvm_petsc_solve_pressure call of the pressure backend (no counterpart in the
  * reference): {return code of the first solve, its iterations, 1 if it was repeated from x = 0,
  * iterations of both solves}. */
