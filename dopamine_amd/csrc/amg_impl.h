@@ -13,6 +13,10 @@
 //   coefficients between and inside aggregates, so every coarse operator is again a symmetric
 //   M-matrix in the same ELL form), down to at most COARSE_MAX cells, whose inverse (with a
 //   rank-one shift when the operator is singular) is formed densely;
+// - when the strength threshold leaves the first level poorly coarsened (under 3.5 cells per
+//   aggregate: strongly anisotropic cells give line aggregates), the first aggregation is redone
+//   with a small threshold (every coupling strong: compact 3-D aggregates of about 9 cells) and
+//   the cycle uses the larger over-correction alpha = 2 (AmgCfg::theta0, min_ratio0, alpha_aggr);
 // - one application is a V(1,1) cycle: per level a damped-Jacobi pre-smoothing step from zero
 //   fused with the residual, a gather restriction over each aggregate's children, the dense solve
 //   at the bottom, and the prolongation (scaled by alpha, the over-correction that unsmoothed
@@ -247,6 +251,14 @@ struct AmgCfg {
   double theta = 0.08;       // strength threshold
   double omega = 2.0 / 3.0;  // Jacobi damping
   double alpha = 1.5;        // coarse-correction scaling
+  // Aggressive first coarsening: when theta leaves the first level with fewer than min_ratio0
+  // fine cells per aggregate (strongly anisotropic cells whose weak couplings all fall below
+  // theta, so the aggregates are thin lines), the first level is aggregated again with theta0
+  // and the whole cycle uses alpha_aggr, the larger over-correction that bigger unsmoothed
+  // aggregates need. theta0 < 0 turns this off.
+  double theta0 = 0.02;
+  double min_ratio0 = 3.5;
+  double alpha_aggr = 2.0;
   int coarse_max = 400;      // cells of the densely solved level
   int max_levels = 16;
 };
@@ -579,6 +591,8 @@ class AmgPc : public FcgPc, public FcgPc32 {
   void build(X* x, const AmgOp& A, const int* nbr_dev, const AmgCfg& cfg) {
     x_ = x;
     cfg_ = cfg;
+    alpha_ = cfg.alpha;
+    aggressive_ = false;
     AmgOp cur = A;
     for (int lev = 0;; ++lev) {
       AmgLevelDev L;
@@ -649,6 +663,11 @@ class AmgPc : public FcgPc, public FcgPc32 {
       }
       std::vector<int> agg;
       int na = amg_aggregate(cur, cfg.theta, agg);
+      if (lev == 0 && cfg.theta0 >= 0.0 && double(cur.n) < cfg.min_ratio0 * double(na)) {
+        na = amg_aggregate(cur, cfg.theta0, agg);
+        alpha_ = cfg.alpha_aggr;
+        aggressive_ = true;
+      }
       if (na * 3 > cur.n * 2 && cur.n > 16)  // stalled coarsening: every connection counts as strong
         na = amg_aggregate(cur, 0.0, agg);
       std::vector<int> ptr, idx;
@@ -674,6 +693,8 @@ class AmgPc : public FcgPc, public FcgPc32 {
   const std::vector<int>& sizes() const { return sizes_; }
   const std::vector<int64_t>& nnz() const { return nnz_; }
   bool singular() const { return singular_; }
+  double alpha() const { return alpha_; }
+  bool aggressive() const { return aggressive_; }
   const AmgCfg& cfg() const { return cfg_; }
 
   // ---- FcgPc (FP64 flexible CG)
@@ -714,7 +735,7 @@ class AmgPc : public FcgPc, public FcgPc32 {
   void cycle(int k, const T* r, const T* rold, T* z, int zs, CgScalars* sc, const double* rr_part, int rr_g,
              double* out_part, int mode) {
     X& xx = *x_;
-    const float om = float(cfg_.omega), al = float(cfg_.alpha);
+    const float om = float(cfg_.omega), al = float(alpha_);
     const int nl = int(levels_.size());
     const AmgLevelDev& L0 = levels_[0];
     if (sc)
@@ -761,6 +782,8 @@ class AmgPc : public FcgPc, public FcgPc32 {
   std::vector<int64_t> nnz_;
   std::vector<void*> owned_;
   float* inv_ = nullptr;
+  double alpha_ = 1.5;
+  bool aggressive_ = false;
   int n_coarse_ = 0;
   bool singular_ = false;
   double* zpart_ = nullptr;

@@ -172,11 +172,13 @@ class SolverT {
   int n_amdw = 0;
   int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
-  // Multigrid set-up: AmgCfg's defaults; DOPAMINE_AMG_THETA / _ALPHA / _COARSE override the
-  // strength threshold, the coarse-correction scaling and the densely solved size.
+  // Multigrid set-up: AmgCfg's defaults; DOPAMINE_AMG_THETA / _THETA0 / _ALPHA / _COARSE override
+  // the strength thresholds (all levels / the first), the coarse-correction scaling and the
+  // densely solved size.
   static AmgCfg amg_cfg() {
     AmgCfg c;
     if (const char* e = std::getenv("DOPAMINE_AMG_THETA")) c.theta = std::atof(e);
+    if (const char* e = std::getenv("DOPAMINE_AMG_THETA0")) c.theta0 = std::atof(e);
     if (const char* e = std::getenv("DOPAMINE_AMG_ALPHA")) c.alpha = std::atof(e);
     if (const char* e = std::getenv("DOPAMINE_AMG_COARSE")) c.coarse_max = std::max(1, std::atoi(e));
     return c;

@@ -11,6 +11,9 @@ checked against an independent one:
   aggregate at a cell whose strong neighbourhood is entirely unaggregated, phase 2 attaches the
   remaining cells to the aggregate of their strongest aggregated strong neighbour, phase 3 groups
   what is left with its unaggregated neighbours; cells are visited in index order;
+- when the first aggregation leaves fewer than min_ratio0 cells per aggregate (line aggregates of
+  strongly anisotropic cells), it is redone with the small threshold theta0 and the cycle uses
+  the over-correction alpha_aggr;
 - piecewise-constant prolongation P, Galerkin coarse operators A_c = P^T A P;
 - the coarsest operator inverted densely, each singular connected component shifted by
   (mean |diagonal| / component size) 1 1^T;
@@ -124,8 +127,10 @@ class Hierarchy:
     """The multigrid levels of an ELL operator (theta, omega, alpha, coarse_max as the product's
     AmgCfg)."""
 
-    def __init__(self, nbr, coef, diag, theta=0.08, omega=2.0 / 3.0, alpha=1.5, coarse_max=400, max_levels=16):
+    def __init__(self, nbr, coef, diag, theta=0.08, omega=2.0 / 3.0, alpha=1.5, coarse_max=400, theta0=0.02,
+                 min_ratio0=3.5, alpha_aggr=2.0, max_levels=16):
         self.omega, self.alpha = omega, alpha
+        self.aggressive = False
         self.A, self.P, self.aggs = [], [], []
         nbr, coef, diag = np.asarray(nbr), np.asarray(coef, dtype=float), np.asarray(diag, dtype=float)
         A = ell_to_csr(nbr, coef, diag)
@@ -137,6 +142,10 @@ class Hierarchy:
                 self.inv = dense_inverse(A)
                 break
             agg, na = aggregate(nbr, coef, diag, theta)
+            if lev == 0 and theta0 >= 0 and n < min_ratio0 * na:
+                # aggressive first coarsening (line aggregates of anisotropic cells)
+                agg, na = aggregate(nbr, coef, diag, theta0)
+                self.alpha, self.aggressive = alpha_aggr, True
             if na * 3 > n * 2 and n > 16:
                 agg, na = aggregate(nbr, coef, diag, 0.0)
             P = prolongation(agg, na)

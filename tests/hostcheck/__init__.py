@@ -182,7 +182,7 @@ def cg_solve_retry(nbr, coef, diag, b, x0, rtol, atol, maxit, nullspace=False, s
     return rc, x, its.value, resid.value, total.value, retried.value
 
 
-AMG_DEFAULTS = (0.08, 2.0 / 3.0, 1.5, 400)
+AMG_DEFAULTS = (0.08, 2.0 / 3.0, 1.5, 400, -1.0)  # theta, omega, alpha, coarse_max, theta0 (< 0: off)
 
 
 def amg_solve(nbr, coef, diag, b, rtol, atol=1e-50, maxit=2000, nullspace=False, mixed=False, cfg=AMG_DEFAULTS):
@@ -197,7 +197,7 @@ def amg_solve(nbr, coef, diag, b, rtol, atol=1e-50, maxit=2000, nullspace=False,
     its = ctypes.c_int(0)
     resid = _d(0.0)
     info = (ctypes.c_int * 16)()
-    c = (_d * 4)(*cfg)
+    c = (_d * 5)(*(tuple(cfg) + (-1.0,) * (5 - len(cfg))))
     pi, pd = _P(ctypes.c_int), _P(_d)
     rc = lib().hc_amg_solve(n, w, nbr.ctypes.data_as(pi), coef.ctypes.data_as(pd), diag.ctypes.data_as(pd),
                             b.ctypes.data_as(pd), x.ctypes.data_as(pd), rtol, atol, maxit, int(nullspace), int(mixed),
@@ -217,7 +217,7 @@ def amg_vcycle(nbr, coef, diag, r, cfg=AMG_DEFAULTS):
     w, n = nbr.shape
     agg = np.zeros(n, dtype=np.int32)
     info = (ctypes.c_int * 16)()
-    c = (_d * 4)(*cfg)
+    c = (_d * 5)(*(tuple(cfg) + (-1.0,) * (5 - len(cfg))))
     pi, pd = _P(ctypes.c_int), _P(_d)
     rc = lib().hc_amg_vcycle(n, w, nbr.ctypes.data_as(pi), coef.ctypes.data_as(pd), diag.ctypes.data_as(pd),
                              r.ctypes.data_as(pd), z.ctypes.data_as(pd), c, agg.ctypes.data_as(pi), info)

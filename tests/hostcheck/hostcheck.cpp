@@ -121,7 +121,7 @@ int hc_cg_solve_retry(int n, int w, const int* nbr, const double* coef, const do
 // flexible CG (mixed = 0) or FP32 sweeps in FP64 refinement (mixed = 1). cfg = {theta, omega,
 // alpha, coarse_max}; info receives {levels, sizes of up to 14 levels...}.
 int hc_amg_solve(int n, int w, const int* nbr, const double* coef, const double* diag, const double* b, double* xv,
-                 double rtol, double atol, int maxit, int nullspace, int mixed, const double cfg[4], int* its,
+                 double rtol, double atol, int maxit, int nullspace, int mixed, const double cfg[5], int* its,
                  double* resid, int* info) {
   try {
     dm::HostExec ex;
@@ -136,6 +136,7 @@ int hc_amg_solve(int n, int w, const int* nbr, const double* coef, const double*
     c.omega = cfg[1];
     c.alpha = cfg[2];
     c.coarse_max = int(cfg[3]);
+    c.theta0 = cfg[4];  // < 0: no aggressive first coarsening
     dm::AmgPc<dm::HostExec> amg;
     amg.build(&ex, op, nbr, c);
     info[0] = amg.levels();
@@ -166,7 +167,7 @@ int hc_amg_solve(int n, int w, const int* nbr, const double* coef, const double*
 // One V-cycle z = M r of the aggregation multigrid (FP32 inside, FP64 in and out) and its
 // hierarchy: agg_out receives the level-0 aggregate of every cell (n entries).
 int hc_amg_vcycle(int n, int w, const int* nbr, const double* coef, const double* diag, const double* r, double* z,
-                  const double cfg[4], int* agg_out, int* info) {
+                  const double cfg[5], int* agg_out, int* info) {
   try {
     dm::HostExec ex;
     dm::AmgOp op;
@@ -180,6 +181,7 @@ int hc_amg_vcycle(int n, int w, const int* nbr, const double* coef, const double
     c.omega = cfg[1];
     c.alpha = cfg[2];
     c.coarse_max = int(cfg[3]);
+    c.theta0 = cfg[4];
     std::vector<int> agg;
     dm::amg_aggregate(op, c.theta, agg);
     std::copy(agg.begin(), agg.end(), agg_out);

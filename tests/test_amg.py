@@ -51,7 +51,10 @@ OPERATORS = {
     "hydro": lambda: cases.regression_cases()["hydro"],         # outlet rows: non-singular
     "channel": lambda: cases.small_channel(16, 12, 8),          # stretched walls, singular
     "tets": lambda: cases.cavity_tets(6, rcm=True),             # jittered tets, singular
+    "full_stack": lambda: cases.full_stack(16, 16, 8, n_obstacles=3, seed=5),  # flat cells, obstacles
 }
+# the product's defaults, with the aggressive first coarsening on
+AMG_PRODUCT = AMG_DEFAULTS[:4] + (0.02,)
 
 
 # ---- the statement pinned by what the method fixes
@@ -116,7 +119,7 @@ def test_two_level_without_smoothing_is_the_galerkin_projection():
     """omega = 0 leaves M = alpha P A_c^-1 P^T: M A is alpha times an A-orthogonal projection,
     (M A)^2 = alpha M A."""
     nbr, coef, diag = chain(40, dirichlet=1.0)
-    H = om.Hierarchy(nbr, coef, diag, omega=0.0, alpha=1.0, coarse_max=1000)
+    H = om.Hierarchy(nbr, coef, diag, omega=0.0, alpha=1.0, coarse_max=1000, theta0=-1.0)
     assert len(H.A) == 2
     A = H.A[0].toarray()
     M = np.column_stack([H.vcycle(e) for e in np.eye(40)])
@@ -177,3 +180,30 @@ def test_product_multigrid_cg_reaches_the_fp64_tolerance(name, mixed):
     assert res <= 1e-8 * np.linalg.norm(b)
     assert its <= its_o + 3 + (4 if mixed else 0)  # FP32 sweeps restart the Krylov space once
     assert (sweeps > 0) == mixed
+
+
+def test_aggressive_first_coarsening_on_flat_cells():
+    """The full-stack cells are 2.5 times wider than high: theta = 0.08 keeps only the vertical
+    couplings strong and the first aggregates are short vertical lines (under 3.5 cells each), so
+    the first level is aggregated again with theta0 = 0.02 (compact aggregates) and the cycle
+    takes alpha = 2. Product and statement agree on that branch, and the flexible CG converges."""
+    nbr, coef, diag, ns = operator(OPERATORS["full_stack"]())
+    line, na_line = om.aggregate(nbr, coef, diag, 0.08)
+    assert diag.size < 3.5 * na_line
+    H = om.Hierarchy(nbr, coef, diag, *AMG_PRODUCT)
+    assert H.aggressive and H.alpha == 2.0 and H.sizes[1] < diag.size / 3.5
+    r = np.random.default_rng(5).standard_normal(diag.size)
+    z, agg, sizes = amg_vcycle(nbr, coef, diag, r, cfg=AMG_PRODUCT)
+    assert sizes == H.sizes and np.array_equal(agg, line)  # agg_out is the theta = 0.08 aggregation
+    zo = H.vcycle(r)
+    assert np.linalg.norm(z - zo) <= 2e-5 * np.linalg.norm(zo)
+    b = r - r.mean()
+    rc, x, its, res, sizes2, _ = amg_solve(nbr, coef, diag, b, 1e-8, nullspace=ns, mixed=True, cfg=AMG_PRODUCT)
+    assert rc == 0 and res <= 1e-8 * np.linalg.norm(b) and sizes2 == H.sizes
+
+
+def test_channel_keeps_the_strength_threshold():
+    """The stretched channel's first aggregation already coarsens by more than 3.5: no second pass."""
+    nbr, coef, diag, ns = operator(cases.small_channel(32, 24, 16))
+    H = om.Hierarchy(nbr, coef, diag, *AMG_PRODUCT)
+    assert not H.aggressive and H.alpha == 1.5
