@@ -207,3 +207,20 @@ def full_stack(nx=250, ny=210, nz=160, n_obstacles=40, seed=2024, lx=1000.0, ly=
     return Case(f"full_stack{nx}x{ny}x{nz}",
                 lambda: M.Mesh(M.obstacle_box_elements(nx, ny, nz, lx, ly, lz, obs)),
                 params, ic="uniform", ic_uref=5.0, nsteps=300, extra={"t_noise": 0.1, "seed": 1700})
+
+
+def advection_bar(n, scheme, kind="scalar_gauss", cfl=0.2):
+    """The reference's scalar-advection study (examples/convergence_test/run_advection.py with
+    inputs/advection_base.input and meshes/advection_1d.geo): a periodic bar [0, 1] x [0, 4/n]^2 of
+    n x 4 x 4 cubic cells, uniform velocity 1 along x, the temperature solved as a passive scalar
+    (Boussinesq with beta = 0, no diffusion, no SGS) by `scheme`, CENTRAL (= CDS) momentum, RK3W with
+    dt = cfl / n for one revolution (5 n steps at cfl 0.2), pressure to 1e-10. kind: 'scalar_gauss'
+    (a Gaussian of width 0.05 at x = 0.25) or 'scalar_pulse' (the Gaussian plus a unit square pulse
+    on [0.55, 0.75)), mod_initconditions.f90:336-360."""
+    h = 1.0 / n
+    names = ("periodic_x0", "periodic_xL", "periodic_y0", "periodic_yL", "periodic_z0", "periodic_zL")
+    params = SolverParams(nu=0.0, advection_scheme="CDS", use_sgs=False, boussinesq=True, alpha_T=0.0, beta=0.0,
+                          T_ref=0.0, temperature_scheme=scheme, dt=cfl * h, cfl_max=100.0, p_tol=1.0e-10,
+                          p_max_iter=2000, rhie_chow_dual=False, cell_projection="LS")
+    return Case(f"advection_bar{n}_{scheme}", lambda: M.uniform_box(n, 4, 4, 1.0, 4 * h, 4 * h, group_names=names),
+                params, ic=kind, ic_uref=1.0, nsteps=int(round(n / cfl)))

@@ -233,11 +233,29 @@ def initial_velocity(case, mesh):
     return U, noise > 0.0
 
 
+def scalar_advection_ic(mesh, kind):
+    """initialize_scalar_advection_ic (src/init/mod_initconditions.f90:336-360): with s the cell's
+    x position as a fraction of the domain length (periodic), a Gaussian exp(-d^2 / (2 0.05^2)) at
+    s = 0.25 (d the periodic distance), plus 1 on 0.55 <= s < 0.75 for 'scalar_pulse'. The domain
+    bounds are the mesh vertices' (compute_domain_bounds, :252-274)."""
+    xmin, xmax = mesh.bounds_min[0], mesh.bounds_max[0]
+    lx = max(xmax - xmin, 1e-30)
+    s = np.mod(mesh.cell_centroid[:, 0] - xmin, lx) / lx
+    d = s - 0.25
+    d = d - np.where(d >= 0.0, np.floor(d + 0.5), np.ceil(d - 0.5))  # Fortran anint: halves away from zero
+    f = np.exp(-0.5 * (d / 0.05) ** 2)
+    if kind == "scalar_pulse":
+        f = f + ((s >= 0.55) & (s < 0.75))
+    return f
+
+
 def initialize(solver, case, mesh):
     """apply_initial_conditions (src/init/mod_initconditions.f90:39-105): set U, then one projection
     with dt = gamma_1 dt (two when noise was added)."""
     U, twice = initial_velocity(case, mesh)
-    if case.params.boussinesq:
+    if case.ic in ("scalar_gauss", "scalar_pulse"):
+        solver.set_temperature(case.params.T_ref + scalar_advection_ic(mesh, case.ic))
+    elif case.params.boussinesq:
         # temperature: T_ref plus a seeded Gaussian perturbation of amplitude t_noise
         T = np.full(mesh.n_cells, case.params.T_ref)
         tn = case.extra.get("t_noise", 0.0)
