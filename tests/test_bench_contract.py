@@ -27,10 +27,10 @@ def test_byte_model_and_committed_traffic():
     assert bench.cg_pass_bytes(m, 0, False)[0] == 40 * nc + 16 * nfp
     # interior faces plus one per periodic pair
     assert nfp == m.n_int_faces + int((m.periodic_pair >= 0).sum()) // 2
-    # the measured traffic of the benchmark kernels is committed under profiles/
-    assert bench.measured_traffic("Fused32T<6>", "channel 128x96x64") > 62_000_000
-    assert bench.measured_traffic("FusedT<6>", "channel 128x96x64") > 106_000_000
-    assert bench.measured_traffic("FusedT<6>", "no such workload") is None
+    # the measured traffic of the benchmark's finest-level passes is committed under profiles/
+    for k in ("FcgDir32T<6>", "Up0<float, 6>", "Down0<float, 6>", "FcgUpd32"):
+        assert bench.measured_traffic(k, "full stack 250x210x160") > 200_000_000
+    assert bench.measured_traffic("FcgDir32T<6>", "no such workload") is None
 
 
 def test_multigrid_byte_model():
