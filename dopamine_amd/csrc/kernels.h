@@ -1042,6 +1042,7 @@ struct ExplicitRhsT {
   double alpha, prt, dt, a, bcoef;
   int cds;
   const double* qw;  // MOST wall heat flux per boundary face, or null
+  int part = 0;      // 0: H_T and the predictor; 1: only the advection term into HT; 2: only the diffusion term
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -1140,6 +1141,10 @@ struct ExplicitRhsT {
         vs = vs + flux * iv;
       }
     }
+    if (part) {  // advection_rhs_scalar / viscous_rhs_scalar on their own
+      HT[c] = part == 1 ? cv : vs;
+      return;
+    }
     double h = cv + vs;
     // add_wall_heat_flux_to_rhs (src/solver/mod_fv_solver.f90:2279-2313): q_w A / V
     if (qw)
@@ -1173,6 +1178,7 @@ struct ExplicitRhs {
   double bb, Tref, g0, g1, g2;
   int rot;             // Coriolis -2 om x U (src/solver/mod_fv_solver.f90:1694-1698)
   double om0, om1, om2;
+  int part = 0;        // 0: H and the predictor; 1: only the advection term into H; 2: only the viscous term
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -1314,6 +1320,10 @@ struct ExplicitRhs {
     }
     if (SCHEME == 4)
       for (int q = 0; q < 3; ++q) cv[q] = cv[q] + 0.5 * Uc[q] * dw * iv;
+    if (part) {  // advection_rhs_vector / viscous_rhs_vector on their own
+      for (int q = 0; q < 3; ++q) H[3 * size_t(c) + q] = part == 1 ? cv[q] : vs[q];
+      return;
+    }
     double h[3] = {cv[0] + vs[0] + f0, cv[1] + vs[1] + f1, cv[2] + vs[2] + f2};
     if (rot) {
       h[0] = h[0] - 2.0 * (om1 * Uc[2] - om2 * Uc[1]);

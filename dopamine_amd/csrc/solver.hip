@@ -355,3 +355,247 @@ extern "C" int dopamine_hbm_copy_probe(int64_t n, int reps, double* avg_ms) {
     return 0;
   });
 }
+
+// ---------------------------------------------------------------- operators on their own
+// The numerics and SGS routines of the reference's Fortran modules as single calls on the
+// solver's mesh (the Fortran drop-in modules bind them): host arrays in and out in the
+// reference's layout, copied to scratch device buffers, one kernel launch per routine.
+namespace {
+struct OpBufs {
+  dm::DeviceExec& x;
+  std::vector<void*> ptrs;
+  explicit OpBufs(dm::DeviceExec& e) : x(e) {}
+  ~OpBufs() {
+    try {
+      x.sync();
+    } catch (...) {
+    }
+    for (void* p : ptrs) x.release(p);
+  }
+  template <class T>
+  T* in(const T* h, size_t n) {
+    T* d = x.alloc<T>(n);
+    ptrs.push_back(d);
+    if (h) x.upload(d, h, n);
+    return d;
+  }
+  double* zeros(size_t n) { return in<double>(nullptr, n); }
+  template <class T>
+  void out(T* h, const T* d, size_t n) {
+    x.download(h, d, n);
+  }
+};
+struct Sizes {
+  size_t nc, nf, nb;
+};
+Sizes sizes(Solver* s) { return {size_t(s->nc), size_t(s->nf), size_t(std::max(1, s->nb))}; }
+Solver* solver_of(void* h) {
+  if (!h) throw std::runtime_error("null solver handle");
+  return static_cast<Solver*>(h);
+}
+}  // namespace
+
+extern "C" {
+
+int dopamine_op_sizes(void* h, int64_t out[3]) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    out[0] = s->nc;
+    out[1] = s->nf;
+    out[2] = s->nb;
+    return 0;
+  });
+}
+
+int dopamine_op_ls_gradient_vector(void* h, const double* U, const double* U_bnd, double* grad_U) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    double* g = b.zeros(9 * z.nc);
+    s->x.for_each(s->nc, dm::LsGradVec{s->M, b.in(U, 3 * z.nc), b.in(U_bnd, 3 * z.nb), g, nullptr, -1, 0.0, 0.0, 0.0,
+                                       0.0});
+    b.out(grad_U, g, 9 * z.nc);
+    return 0;
+  });
+}
+
+int dopamine_op_ls_gradient_scalar(void* h, const double* phi, const double* phi_bnd, double* grad) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    double* g = b.zeros(3 * z.nc);
+    s->x.for_each(s->nc, dm::LsGradSca{s->M, b.in(phi, z.nc), nullptr, b.in(phi_bnd, z.nb), nullptr, g});
+    b.out(grad, g, 3 * z.nc);
+    return 0;
+  });
+}
+
+int dopamine_op_gg_gradient_vector(void* h, const double* U, const double* U_bnd, const double* grad_prev,
+                                   double* grad_U) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    double* g = b.zeros(9 * z.nc);
+    s->x.for_each(s->nc, dm::GgGradVec{s->M, b.in(U, 3 * z.nc), b.in(U_bnd, 3 * z.nb),
+                                       grad_prev ? b.in(grad_prev, 9 * z.nc) : nullptr, g});
+    b.out(grad_U, g, 9 * z.nc);
+    return 0;
+  });
+}
+
+int dopamine_op_gg_gradient_scalar(void* h, const double* phi, const double* phi_bnd, const double* grad_prev,
+                                   double* grad) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    double* g = b.zeros(3 * z.nc);
+    s->x.for_each(s->nc, dm::GgGradSca{s->M, b.in(phi, z.nc), nullptr, b.in(phi_bnd, z.nb), nullptr,
+                                       grad_prev ? b.in(grad_prev, 3 * z.nc) : nullptr, g});
+    b.out(grad, g, 3 * z.nc);
+    return 0;
+  });
+}
+
+int dopamine_op_sgs_viscosity(void* h, int model, const double* U, const double* grad_U, double coef, double nu,
+                              int wall_gradient, double dt_stab, double* nu_sgs) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    if (model < 1 || model > 3) throw std::runtime_error("sgs model must be 1 (Vreman), 2 (Smagorinsky) or 3 (AMD)");
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    double* nus = b.zeros(z.nc);
+    const double* g = b.in(grad_U, 9 * z.nc);
+    s->x.for_each(s->nc, dm::SgsCell{s->M, g, nus, model, model == 1 ? coef : 0.0, model == 2 ? coef : 0.0,
+                                     model == 3 ? coef : 0.0, dt_stab});
+    if (model == 3 && wall_gradient && s->n_amdw > 0)
+      s->x.for_each(s->n_amdw, dm::AmdWall{s->M, s->amdw_cell, s->amdw_off, s->amdw_face, b.in(U, 3 * z.nc), g, nus,
+                                           nu, coef, dt_stab});
+    b.out(nu_sgs, nus, z.nc);
+    return 0;
+  });
+}
+
+int dopamine_op_wall_model(void* h, const double* U, double nu, double* tau_w_bnd) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    double* t = b.zeros(3 * z.nb);
+    if (s->nb > 0) s->x.for_each(s->nb, dm::WallModel{s->M, b.in(U, 3 * z.nc), t, nu});
+    b.out(tau_w_bnd, t, 3 * z.nb);
+    return 0;
+  });
+}
+
+// part 1: advection_rhs_vector, part 2: viscous_rhs_vector
+static int op_rhs_vector(void* h, int part, int scheme, double nu, const double* U, const double* U_bnd,
+                  const double* face_flux, const double* nu_sgs, const double* grad_U, double* rhs) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    if (scheme < 1 || scheme > 5) throw std::runtime_error("unknown advection scheme");
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    const double* u = b.in(U, 3 * z.nc);
+    const double* ub = b.in(U_bnd, 3 * z.nb);
+    const double* uf = face_flux ? b.in(face_flux, z.nf) : b.zeros(z.nf);
+    const double* nus = nu_sgs ? b.in(nu_sgs, z.nc) : b.zeros(z.nc);
+    const double* g = grad_U ? b.in(grad_U, 9 * z.nc) : b.zeros(9 * z.nc);
+    double* H = b.zeros(3 * z.nc);
+    double* Un = b.zeros(3 * z.nc);
+    double* phiF = (part == 1 && scheme != dm::SCHEME_CDS && scheme != dm::SCHEME_KEP) ? b.zeros(3 * z.nf) : nullptr;
+    const auto run = [&](auto sch) {
+      constexpr int S = decltype(sch)::value;
+      if (phiF) s->x.for_each(s->nf, dm::ReconFace<S>{s->M, u, ub, uf, g, phiF});
+      dm::ExplicitRhs<S> f{s->M, u, ub, uf, g, nus, nullptr, H, H, Un, part == 2 ? nu : 0.0, 0.0, 0.0, 0.0, 0.0,
+                           0.0, 0.0, 0, phiF, nullptr, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, 0.0, part};
+      s->x.for_each(s->nc, f);
+    };
+    switch (part == 2 ? int(dm::SCHEME_CDS) : scheme) {
+      case dm::SCHEME_VANLEER: run(std::integral_constant<int, dm::SCHEME_VANLEER>{}); break;
+      case dm::SCHEME_KOREN: run(std::integral_constant<int, dm::SCHEME_KOREN>{}); break;
+      case dm::SCHEME_CDS: run(std::integral_constant<int, dm::SCHEME_CDS>{}); break;
+      case dm::SCHEME_KEP: run(std::integral_constant<int, dm::SCHEME_KEP>{}); break;
+      default: run(std::integral_constant<int, dm::SCHEME_QUICK>{}); break;
+    }
+    std::vector<double> hv(3 * z.nc);
+    b.out(hv.data(), H, 3 * z.nc);
+    if (part == 1)
+      for (size_t i = 0; i < 3 * z.nc; ++i) rhs[i] = hv[i];  // advection overwrites (mod_advection.f90:724)
+    else
+      for (size_t i = 0; i < 3 * z.nc; ++i) rhs[i] = rhs[i] + hv[i];  // viscous adds (mod_viscous.f90:37-283)
+    return 0;
+  });
+}
+
+int dopamine_op_advection_rhs_vector(void* h, int scheme, const double* U, const double* U_bnd,
+                                     const double* face_flux, const double* grad_U, double* rhs) {
+  return op_rhs_vector(h, 1, scheme, 0.0, U, U_bnd, face_flux, nullptr, grad_U, rhs);
+}
+
+int dopamine_op_viscous_rhs_vector(void* h, double nu, const double* U, const double* nu_sgs, const double* grad_U,
+                                   const double* U_bnd, double* rhs) {
+  return op_rhs_vector(h, 2, dm::SCHEME_CDS, nu, U, U_bnd, nullptr, nu_sgs, grad_U, rhs);
+}
+
+// part 1: advection_rhs_scalar, part 2: viscous_rhs_scalar
+static int op_rhs_scalar(void* h, int part, int scheme, const double* phi, const double* U, const double* phi_bnd,
+                  const double* U_bnd, const double* face_flux, const double* grad_phi, double alpha,
+                  const double* nu_sgs, double pr_t, double* rhs) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    if (scheme < 1 || scheme > 5 || scheme == dm::SCHEME_KEP)
+      throw std::runtime_error("scalar advection scheme must be VANLEER_TVD, KOREN_TVD, CDS or QUICK");
+    if (part == 2 && !(pr_t > 0.0)) throw std::runtime_error("Pr_t must be positive");
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    const double* t = b.in(phi, z.nc);
+    const double* tb = b.in(phi_bnd, z.nb);
+    const double* u = U ? b.in(U, 3 * z.nc) : b.zeros(3 * z.nc);
+    const double* ub = U_bnd ? b.in(U_bnd, 3 * z.nb) : b.zeros(3 * z.nb);
+    const double* uf = face_flux ? b.in(face_flux, z.nf) : b.zeros(z.nf);
+    const double* gt = grad_phi ? b.in(grad_phi, 3 * z.nc) : b.zeros(3 * z.nc);
+    const double* nus = nu_sgs ? b.in(nu_sgs, z.nc) : b.zeros(z.nc);
+    double* gv = b.zeros(9 * z.nc);  // the velocity gradient the reconstruction context does not use here
+    double* H = b.zeros(z.nc);
+    double* Tn = b.zeros(z.nc);
+    const bool cds = part == 2 || scheme == dm::SCHEME_CDS;
+    double* phiT = cds ? nullptr : b.zeros(z.nf);
+    if (!cds) {
+      const dm::ReconFace<dm::SCHEME_QUICK> rq{s->M, u, ub, uf, gv, nullptr};
+      const dm::ReconFace<dm::SCHEME_VANLEER> rv{s->M, u, ub, uf, gv, nullptr};
+      const dm::ReconFace<dm::SCHEME_KOREN> rk{s->M, u, ub, uf, gv, nullptr};
+      if (scheme == dm::SCHEME_QUICK) s->x.for_each(s->nf, dm::ReconFaceScalar<dm::SCHEME_QUICK>{rq, t, tb, gt, phiT});
+      if (scheme == dm::SCHEME_VANLEER)
+        s->x.for_each(s->nf, dm::ReconFaceScalar<dm::SCHEME_VANLEER>{rv, t, tb, gt, phiT});
+      if (scheme == dm::SCHEME_KOREN) s->x.for_each(s->nf, dm::ReconFaceScalar<dm::SCHEME_KOREN>{rk, t, tb, gt, phiT});
+    }
+    dm::ExplicitRhsT f{s->M, t, tb, gt, uf, nus, H, phiT, H, Tn, part == 2 ? alpha : 0.0, part == 2 ? pr_t : 1.0, 0.0,
+                       0.0, 0.0, cds ? 1 : 0, nullptr, part};
+    s->x.for_each(s->nc, f);
+    std::vector<double> hv(z.nc);
+    b.out(hv.data(), H, z.nc);
+    if (part == 1)
+      for (size_t i = 0; i < z.nc; ++i) rhs[i] = hv[i];
+    else
+      for (size_t i = 0; i < z.nc; ++i) rhs[i] = rhs[i] + hv[i];
+    return 0;
+  });
+}
+
+int dopamine_op_advection_rhs_scalar(void* h, int scheme, const double* phi, const double* U, const double* phi_bnd,
+                                     const double* U_bnd, const double* face_flux, const double* grad_phi,
+                                     double* rhs) {
+  return op_rhs_scalar(h, 1, scheme, phi, U, phi_bnd, U_bnd, face_flux, grad_phi, 0.0, nullptr, 1.0, rhs);
+}
+
+int dopamine_op_viscous_rhs_scalar(void* h, const double* phi, double alpha_mol, const double* nu_sgs, double pr_t,
+                                   const double* grad_phi, const double* phi_bnd, double* rhs) {
+  return op_rhs_scalar(h, 2, dm::SCHEME_CDS, phi, nullptr, phi_bnd, nullptr, nullptr, grad_phi, alpha_mol, nu_sgs,
+                       pr_t, rhs);
+}
+
+}  // extern "C"

@@ -180,6 +180,45 @@ int dopamine_solver_learned_loss(void *solver, double *out, int max);
 /* Wait for all work queued by the solver. */
 int dopamine_solver_sync(void *solver);
 
+/* ---- Single routines of the reference's numerics and SGS modules on the solver's mesh, for the
+ * Fortran drop-in modules (dopamine_amd/fortran/dopamine_modules.f90). Host arrays in the
+ * reference's layout (component fastest: U(3, n_cells), grad_U(3, 3, n_cells) with
+ * grad(i, j) = du_j/dx_i, U_bnd(3, n_bnd_faces)); each call copies its inputs to the GPU, runs the
+ * routine's kernel on the solver's device mesh and copies the result back. 0 or a negative error. */
+/* {n_cells, n_faces, n_bnd_faces} of the solver's mesh. */
+int dopamine_op_sizes(void *solver, int64_t out[3]);
+/* ls_gradient_vector / ls_gradient_scalar (src/numerics/mod_gradient.f90:425, 307). */
+int dopamine_op_ls_gradient_vector(void *solver, const double *U, const double *U_bnd, double *grad_U);
+int dopamine_op_ls_gradient_scalar(void *solver, const double *phi, const double *phi_bnd, double *grad_phi);
+/* green_gauss_gradient_vector / _scalar (mod_gradient.f90:172, 38): one sweep; grad_prev (or NULL)
+ * is the previous sweep's gradient for the skewness correction. */
+int dopamine_op_gg_gradient_vector(void *solver, const double *U, const double *U_bnd, const double *grad_prev,
+                                   double *grad_U);
+int dopamine_op_gg_gradient_scalar(void *solver, const double *phi, const double *phi_bnd, const double *grad_prev,
+                                   double *grad_phi);
+/* nu_t from grad_U: model 1 vreman_sgs_viscosity (coef = C_v; src/sgs/mod_vreman.f90:50), 2
+ * smagorinsky_sgs (coef = C_s; :295), 3 amd_sgs_viscosity (coef = c_amd, with the log-law
+ * wall-cell gradient when wall_gradient != 0, nu the molecular viscosity; src/sgs/mod_amd.f90:69);
+ * dt_stab > 0 caps nu_t at 0.4 V^(2/3) / dt_stab. */
+int dopamine_op_sgs_viscosity(void *solver, int model, const double *U, const double *grad_U, double coef, double nu,
+                              int wall_gradient, double dt_stab, double *nu_sgs);
+/* apply_wall_model_bc without time filter (src/sgs/mod_wall_model.f90:176): tau_w_bnd(3, n_bnd). */
+int dopamine_op_wall_model(void *solver, const double *U, double nu, double *tau_w_bnd);
+/* advection_rhs_vector (src/numerics/mod_advection.f90:679): rhs = the advective term (overwritten);
+ * scheme VANLEER_TVD 1, KOREN_TVD 2, CDS 3, KEP 4, QUICK 5; face_flux(n_faces) the projected flux. */
+int dopamine_op_advection_rhs_vector(void *solver, int scheme, const double *U, const double *U_bnd,
+                                     const double *face_flux, const double *grad_U, double *rhs);
+/* viscous_rhs_vector (src/numerics/mod_viscous.f90:37): adds the viscous term to rhs. */
+int dopamine_op_viscous_rhs_vector(void *solver, double nu, const double *U, const double *nu_sgs,
+                                   const double *grad_U, const double *U_bnd, double *rhs);
+/* advection_rhs_scalar (mod_advection.f90:56): rhs(n_cells) overwritten; scheme 1, 2, 3 or 5. */
+int dopamine_op_advection_rhs_scalar(void *solver, int scheme, const double *phi, const double *U,
+                                     const double *phi_bnd, const double *U_bnd, const double *face_flux,
+                                     const double *grad_phi, double *rhs);
+/* viscous_rhs_scalar (mod_viscous.f90:286): adds (alpha + nu_t / Pr_t) diffusion to rhs. */
+int dopamine_op_viscous_rhs_scalar(void *solver, const double *phi, double alpha_mol, const double *nu_sgs,
+                                   double pr_t, const double *grad_phi, const double *phi_bnd, double *rhs);
+
 /* Kernel classes with device timing. */
 enum {
   DOPAMINE_K_CG_DIR = 0,   /* pressure CG, first pass of an iteration: p update + operator apply

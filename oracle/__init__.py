@@ -88,6 +88,15 @@ def lib():
         L.om_psi_h.restype = _d
         L.om_psi_h.argtypes = [_d]
         L.om_nonortho_info.argtypes = [ctypes.c_void_p, ctypes.c_int * 3]
+        vp, ci = ctypes.c_void_p, ctypes.c_int
+        L.om_op_gradient_vector.argtypes = [vp, ci, ci, _P(_d)]
+        L.om_op_gradient_scalar.argtypes = [vp, ci, ci, _P(_d), _P(_d), _P(_d)]
+        L.om_op_advection_vector.argtypes = [vp, ci, _P(_d)]
+        L.om_op_viscous_vector.argtypes = [vp, _P(_d)]
+        L.om_op_advection_scalar.argtypes = [vp, ci, _P(_d), _P(_d), _P(_d), _P(_d)]
+        L.om_op_viscous_scalar.argtypes = [vp, _P(_d), _P(_d), _P(_d), _d, _d, _P(_d)]
+        L.om_op_sgs.argtypes = [vp, ci, _d, ci, _d, _P(_d)]
+        L.om_op_wall_model.argtypes = [vp, _P(_d)]
         L.om_cg_jacobi.argtypes = [ctypes.c_int, ctypes.c_int, _P(_i32), _P(_i32), _P(_d), _P(_d), _P(_d), _P(_d),
                                    _d, _d, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(_d)]
         _lib = L
@@ -248,6 +257,53 @@ class Solver:
         bc = np.zeros(self.mesh_nc) if bconst is None else np.ascontiguousarray(bconst, dtype=np.float64)
         its = lib().om_proj_project(self.h, _p(u, _d), _p(bc, _d), 0 if precond == "identity" else 1, tol, max_it)
         return u.reshape(-1, 3), its
+
+    # single operator routines (src/numerics/mod_gradient.f90, mod_advection.f90, mod_viscous.f90,
+    # src/sgs/mod_vreman.f90, mod_amd.f90, mod_wall_model.f90); the momentum ones read the solver's
+    # U, U_bnd, U_face, grad_U and nu_sgs
+    def op_gradient_vector(self, gg=False, iterations=1):
+        out = np.zeros(9 * self.mesh_nc)
+        lib().om_op_gradient_vector(self.h, int(gg), iterations, _p(out, _d))
+        return out
+
+    def op_gradient_scalar(self, phi, phi_bnd, gg=False, iterations=1):
+        phi, pb = (np.ascontiguousarray(a, dtype=np.float64) for a in (phi, phi_bnd))
+        out = np.zeros(3 * self.mesh_nc)
+        lib().om_op_gradient_scalar(self.h, int(gg), iterations, _p(phi, _d), _p(pb, _d), _p(out, _d))
+        return out
+
+    def op_advection_vector(self, scheme):
+        out = np.zeros(3 * self.mesh_nc)
+        lib().om_op_advection_vector(self.h, scheme, _p(out, _d))
+        return out
+
+    def op_viscous_vector(self):
+        out = np.zeros(3 * self.mesh_nc)
+        lib().om_op_viscous_vector(self.h, _p(out, _d))
+        return out
+
+    def op_advection_scalar(self, scheme, phi, phi_bnd, grad_phi):
+        phi, pb, g = (np.ascontiguousarray(a, dtype=np.float64) for a in (phi, phi_bnd, grad_phi))
+        out = np.zeros(self.mesh_nc)
+        lib().om_op_advection_scalar(self.h, scheme, _p(phi, _d), _p(pb, _d), _p(g, _d), _p(out, _d))
+        return out
+
+    def op_viscous_scalar(self, phi, phi_bnd, grad_phi, alpha, pr_t):
+        phi, pb, g = (np.ascontiguousarray(a, dtype=np.float64) for a in (phi, phi_bnd, grad_phi))
+        out = np.zeros(self.mesh_nc)
+        lib().om_op_viscous_scalar(self.h, _p(phi, _d), _p(pb, _d), _p(g, _d), alpha, pr_t, _p(out, _d))
+        return out
+
+    def op_sgs(self, model, coef, wall_gradient, dt_stab):
+        """model 1 Vreman, 2 Smagorinsky, 3 AMD on the solver's grad_U."""
+        out = np.zeros(self.mesh_nc)
+        lib().om_op_sgs(self.h, model, coef, int(wall_gradient), dt_stab, _p(out, _d))
+        return out
+
+    def op_wall_model(self):
+        out = np.zeros(self.array("tau_w_bnd").size)
+        lib().om_op_wall_model(self.h, _p(out, _d))
+        return out
 
     def nonortho_info(self):
         out = (ctypes.c_int * 3)()

@@ -2345,6 +2345,93 @@ int om_eval_closures(om_solver *s, double dt_stab) {
   return 0;
 }
 
+/* Single routines of src/numerics/mod_gradient.f90, mod_advection.f90, mod_viscous.f90 and
+ * src/sgs/mod_vreman.f90 / mod_amd.f90 / mod_wall_model.f90, one call each, for the operator-level
+ * tests. The momentum routines read the solver's U, U_bnd, U_face, grad_U and nu_sgs (set them
+ * through om_array first); the scalar routines take their field, boundary values and gradient. */
+void om_op_gradient_vector(om_solver *s, int gg, int iterations, double *out) {
+  if (!gg) {
+    ls_gradient_vector(s, s->U, s->U_bnd, out);
+    return;
+  }
+  double *prev = dz(9 * (size_t)s->nc);
+  gg_gradient_vector(s, s->U, s->U_bnd, NULL, out);
+  for (int it = 2; it <= iterations; ++it) {
+    memcpy(prev, out, sizeof(double) * 9 * (size_t)s->nc);
+    gg_gradient_vector(s, s->U, s->U_bnd, prev, out);
+  }
+  free(prev);
+}
+
+void om_op_gradient_scalar(om_solver *s, int gg, int iterations, const double *phi, const double *pb, double *out) {
+  if (!gg) {
+    ls_gradient_scalar(s, phi, pb, out);
+    return;
+  }
+  double *prev = dz(3 * (size_t)s->nc);
+  gg_gradient_scalar(s, phi, pb, NULL, out);
+  for (int it = 2; it <= iterations; ++it) {
+    memcpy(prev, out, sizeof(double) * 3 * (size_t)s->nc);
+    gg_gradient_scalar(s, phi, pb, prev, out);
+  }
+  free(prev);
+}
+
+void om_op_advection_vector(om_solver *s, int scheme, double *out) {
+  const int keep = s->p.scheme;
+  s->p.scheme = scheme;
+  if (scheme == OM_SCHEME_KEP)
+    advection_kep(s, out);
+  else if (scheme == OM_SCHEME_CDS)
+    advection_cds(s, out);
+  else
+    advection_recon(s, out);
+  s->p.scheme = keep;
+}
+
+void om_op_viscous_vector(om_solver *s, double *out) { viscous_vector(s, out); }
+
+void om_op_advection_scalar(om_solver *s, int scheme, const double *phi, const double *pb, const double *g,
+                            double *out) {
+  if (scheme == OM_SCHEME_CDS)
+    advection_scalar_cds(s, phi, pb, g, out);
+  else
+    advection_scalar_recon(s, scheme, phi, pb, g, out);
+}
+
+void om_op_viscous_scalar(om_solver *s, const double *phi, const double *pb, const double *g, double alpha,
+                          double prt, double *out) {
+  viscous_scalar(s, phi, pb, g, alpha, prt, out);
+}
+
+/* model 1 Vreman (coef C_v), 2 Smagorinsky (C_s), 3 AMD (C, wall_gradient) on the solver's grad_U */
+void om_op_sgs(om_solver *s, int model, double coef, int wall_gradient, double dt_stab, double *out) {
+  const om_params keep = s->p;
+  double *nus = s->nu_sgs;
+  s->nu_sgs = out;
+  s->p.amd_wall_gradient = wall_gradient;
+  if (model == OM_SGS_VREMAN) {
+    s->p.c_vreman = coef;
+    vreman(s, dt_stab);
+  } else if (model == OM_SGS_SMAGORINSKY) {
+    s->p.cs_smag = coef;
+    smagorinsky(s, dt_stab);
+  } else {
+    s->p.c_amd = coef;
+    amd(s, dt_stab);
+  }
+  s->nu_sgs = nus;
+  s->p = keep;
+}
+
+/* the log-law wall traction of every wall-model face from the solver's U */
+void om_op_wall_model(om_solver *s, double *out) {
+  double *t = s->tau_w;
+  s->tau_w = out;
+  wall_model(s);
+  s->tau_w = t;
+}
+
 double om_wall_tau(double u_par, double y_w, double nu) { return wall_tau(u_par, y_w, nu); }
 
 static int advance_rk3w(om_solver *s, double dt) {

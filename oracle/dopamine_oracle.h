@@ -119,6 +119,17 @@ void om_divergence_cell(om_solver *s, double out[2]);
  * SGS viscosity into "nu_sgs" with stability cap 0.4 V^(2/3) / dt_stab, wall traction into
  * "tau_w_bnd"), on the current U, without advancing. */
 int om_eval_closures(om_solver *s, double dt_stab);
+/* single operator routines (gradient, advection, viscous, SGS, wall model) on caller-set fields */
+void om_op_gradient_vector(om_solver *s, int gg, int iterations, double *out);
+void om_op_gradient_scalar(om_solver *s, int gg, int iterations, const double *phi, const double *pb, double *out);
+void om_op_advection_vector(om_solver *s, int scheme, double *out);
+void om_op_viscous_vector(om_solver *s, double *out);
+void om_op_advection_scalar(om_solver *s, int scheme, const double *phi, const double *pb, const double *g,
+                            double *out);
+void om_op_viscous_scalar(om_solver *s, const double *phi, const double *pb, const double *g, double alpha,
+                          double prt, double *out);
+void om_op_sgs(om_solver *s, int model, double coef, int wall_gradient, double dt_stab, double *out);
+void om_op_wall_model(om_solver *s, double *out);
 /* Wall shear stress tau_w = u_tau^2 of the Reichardt-law Newton solve for a wall-parallel speed
  * u_par sampled at wall distance y_w (src/sgs/mod_wall_model.f90:50-115). */
 double om_wall_tau(double u_par, double y_w, double nu);
