@@ -109,6 +109,19 @@ def multigrid_bytes(mesh, mixed):
             "mg_up": 16 * nf + (8 + 3 * v) * nc}
 
 
+def hbm_used_gb():
+    """Device memory in use on the current GPU (hipMemGetInfo: total - free), GB."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return None
+    return (total.value - free.value) / 1e9
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the
     same kernel and workload (profiles/), or None."""
@@ -338,6 +351,9 @@ def main():
                   "max_abs_cell_net_flux": solver.cell_net_flux_max()},
         "cpu_baseline": None,
     }
+    if multigrid:
+        out["multigrid"] = solver.multigrid_info()
+    out["hbm_used_gb"] = hbm_used_gb()
     if plain is not None:
         out["plain_fp64"] = plain
         out["speedup_vs_plain_fp64"] = out["value"] / plain["value"]

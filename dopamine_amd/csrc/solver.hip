@@ -288,6 +288,22 @@ int dopamine_solver_learned_info(void* h, double out[8]) {
   });
 }
 
+int dopamine_solver_multigrid_info(void* h, int64_t* out, int max) {
+  return guard([&] {
+    auto* s = static_cast<Solver*>(h);
+    if (!s->amg) throw std::runtime_error("no multigrid hierarchy (precond is not multigrid)");
+    const auto& sz = s->amg->sizes();
+    const auto& nz = s->amg->nnz();
+    const int nl = int(sz.size());
+    if (max < 2 + 2 * nl) throw std::runtime_error("multigrid_info: output holds fewer than 2 + 2 levels entries");
+    out[0] = nl;
+    out[1] = s->amg->aggressive() ? 1 : 0;
+    for (int l = 0; l < nl; ++l) out[2 + l] = sz[size_t(l)];
+    for (int l = 0; l < nl; ++l) out[2 + nl + l] = l == 0 ? 0 : (size_t(l - 1) < nz.size() ? nz[size_t(l - 1)] : 0);
+    return 0;
+  });
+}
+
 int dopamine_solver_projection_info(void* h, int64_t out[2]) {
   return guard([&] {
     auto* s = static_cast<Solver*>(h);

@@ -105,6 +105,7 @@ def _bind(L):
     L.dopamine_solver_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _P(_i64), _P(_d)]
     L.dopamine_solver_learned_info.argtypes = [ctypes.c_void_p, _d * 8]
     L.dopamine_solver_projection_info.argtypes = [ctypes.c_void_p, _i64 * 2]
+    L.dopamine_solver_multigrid_info.argtypes = [ctypes.c_void_p, _P(_i64), ctypes.c_int]
     L.dopamine_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
     L.dopamine_solver_write_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_read_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
@@ -303,6 +304,16 @@ class FVSolver:
         out = (_i64 * 2)()
         self._check(self.L.dopamine_solver_projection_info(self.h, out), "projection_info")
         return {"projections": out[0], "iterations": out[1]}
+
+    def multigrid_info(self):
+        """Hierarchy of the multigrid pressure preconditioner (None before its first solve or when
+        the preconditioner is not multigrid): cells and stored off-diagonals per level."""
+        out = (_i64 * 68)()
+        if self.L.dopamine_solver_multigrid_info(self.h, out, 68) != 0:
+            return None
+        nl = int(out[0])
+        return {"levels": nl, "aggressive": bool(out[1]), "cells": [int(out[2 + i]) for i in range(nl)],
+                "nnz": [int(out[2 + nl + i]) for i in range(nl)]}
 
     def learned_loss(self, max_steps=100000):
         out = np.zeros(max_steps)
