@@ -35,6 +35,9 @@ struct DevMesh {
   const double *Tinv, *Gf;
   // patch values
   const double *patchU;
+  // per patch: convective outlet (1), backflow-traction coefficient, MOST roughness lengths
+  const int* pconv = nullptr;
+  const double *pbeta = nullptr, *pz0m = nullptr, *pz0h = nullptr;
 };
 
 DM_INLINE double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -67,6 +70,7 @@ struct BcVelocity {
   const double *U, *gU;
   double* Ub;
   int filter;
+  int conv_ready = 0;  // a convective outlet's boundary velocity is a state once a stage advanced it
   DM_HD void operator()(int b) const {
     const int f = M.nint + b, bc = M.bbc[b];
     if (filter == 2 && bc != BC_PERIODIC) return;
@@ -83,6 +87,9 @@ struct BcVelocity {
       }
       case BC_OUTLET:
       case BC_ZEROGRADIENT: {
+        // a convective outlet face keeps its RK state; only its start-up value is extrapolated
+        // (src/bc/mod_bc.f90:304-311)
+        if (bc == BC_OUTLET && conv_ready && M.pconv[M.bpatch[b]]) break;
         double u[3] = {uo[0], uo[1], uo[2]};
         if (gU) {
           const double d[3] = {M.fctr[3 * size_t(f)] - M.ctr[3 * size_t(c)], M.fctr[3 * size_t(f) + 1] - M.ctr[3 * size_t(c) + 1],
@@ -474,23 +481,135 @@ struct AmdWall {
   }
 };
 
-// apply_wall_model_bc without time filter (src/sgs/mod_wall_model.f90:176-313), one boundary face.
+// apply_wall_model_bc (src/sgs/mod_wall_model.f90:176-313), one boundary face: the wall-parallel
+// velocity of the sampling cell, optionally time-filtered (fmode 1: u_f += w (u - u_f) with
+// w = dt / (T + dt); fmode 2: eps = min(1, dt kappa u_tau(|u_f|) / (f y)), eps = 0 when dt = 0;
+// seed: the first filtered call copies u), then the Reichardt law.
 struct WallModel {
   DevMesh M;
   const double* U;
   double* tau;
   double nu;
+  double* uf = nullptr;  // filtered velocity, 3 per boundary face
+  int fmode = 0, seed = 0;
+  double w = 0.0, dt_use = 0.0, tf = 0.0;
   DM_HD void operator()(int b) const {
     if (M.bbc[b] != BC_WALL_MODEL) return;
     const int f = M.nint + b;
     const double* n = &M.nrm[3 * size_t(f)];
     const double* u = &U[3 * size_t(M.wsc[b])];
+    const double y = M.wsy[b];
     const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
-    const double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    if (fmode) {
+      double* q = &uf[3 * size_t(b)];
+      if (seed) {
+        for (int k = 0; k < 3; ++k) q[k] = up[k];
+      } else {
+        double eps = w;
+        if (fmode == 2) {
+          eps = 0.0;
+          if (dt_use > 0.0) {
+            const double prev = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+            const double ut = wall_utau(prev, y, nu);
+            eps = 1.0;
+            if (ut > SMALL) eps = fmin(1.0, dt_use * 0.41 * ut / (tf * y));  // kappa = 0.41
+          }
+        }
+        for (int k = 0; k < 3; ++k) q[k] = q[k] + eps * (up[k] - q[k]);
+      }
+      for (int k = 0; k < 3; ++k) up[k] = q[k];
+    }
     const double mag = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
-    const double t = wall_tau_mag(mag, M.wsy[b], nu);
+    const double t = wall_tau_mag(mag, y, nu);
     const double scl = -t / (mag + SMALL);
     for (int k = 0; k < 3; ++k) tau[3 * size_t(b) + k] = scl * up[k];
+  }
+};
+
+// ------------------------------------------------------------------ convective outlet
+// outlet_advance_boundary / update_outlet_pressure (src/bc/mod_bc.f90:681-818).
+
+// area sums of u.n, |u| and the area over one outlet patch's faces (boundary faces start..)
+struct OutletStat {
+  DevMesh M;
+  const double* U;
+  int start;
+  DM_HD Red<3> operator()(int i) const {
+    const int b = start + i, f = M.nint + b, o = M.bown[b];
+    const double* u = &U[3 * size_t(o)];
+    const double a = M.area[f];
+    Red<3> r;
+    r.v[0] = a * dot3(u, &M.nrm[3 * size_t(f)]);
+    r.v[1] = a * sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    r.v[2] = a;
+    return r;
+  }
+};
+// sp[2q] = max(mean u.n, 0) (the convection speed), sp[2q+1] = mean |u| (the reference speed)
+struct OutletStatFin {
+  double* sp;
+  int q;
+  DM_HD void operator()(const Red<3>& s) const {
+    if (s.v[2] <= 0.0) return;
+    sp[2 * q + 1] = s.v[1] / s.v[2];
+    sp[2 * q] = fmax(s.v[0] / s.v[2], 0.0);
+  }
+};
+
+// outlet_dn_u (src/bc/mod_bc.f90:775-786)
+DM_INLINE double outlet_dn_u(double ub, double uo, double gdd, double gt, double len_d, double nd) {
+  return nd * (2.0 * (ub - uo) / len_d - gdd) + gt;
+}
+
+// one RK3W stage of du_b/dt = -U_c n.grad u on a convective outlet face
+struct OutletAdvance {
+  DevMesh M;
+  const double *U, *grad, *sp;
+  double *Ub, *gprev;
+  double dt, a, bk;
+  DM_HD void operator()(int b) const {
+    if (M.bbc[b] != BC_OUTLET) return;
+    const int q = M.bpatch[b];
+    if (!M.pconv[q]) return;
+    const int f = M.nint + b, o = M.bown[b];
+    double dl[3];
+    for (int i = 0; i < 3; ++i) dl[i] = M.fctr[3 * size_t(f) + i] - M.ctr[3 * size_t(o) + i];
+    const double len_d = fmax(sqrt(dot3(dl, dl)), SMALL);
+    for (int i = 0; i < 3; ++i) dl[i] = dl[i] / len_d;
+    const double* n = &M.nrm[3 * size_t(f)];
+    const double nd = dot3(n, dl);
+    double tn[3];
+    for (int i = 0; i < 3; ++i) tn[i] = n[i] - nd * dl[i];
+    const double uc = sp[2 * q];
+    for (int k = 0; k < 3; ++k) {
+      const double gdd = grad[DM_G(o, 0, k)] * dl[0] + grad[DM_G(o, 1, k)] * dl[1] + grad[DM_G(o, 2, k)] * dl[2];
+      const double gt = grad[DM_G(o, 0, k)] * tn[0] + grad[DM_G(o, 1, k)] * tn[1] + grad[DM_G(o, 2, k)] * tn[2];
+      const size_t j = 3 * size_t(b) + k;
+      const double g = -uc * outlet_dn_u(Ub[j], U[3 * size_t(o) + k], gdd, gt, len_d, nd);
+      Ub[j] = Ub[j] + dt * (a * g + bk * gprev[j]);
+      gprev[j] = g;
+    }
+  }
+};
+
+// backflow traction p_b = p_val - beta |u|^2 S0 / 2, S0 = (1 - tanh(u.n / (U_ref 0.05))) / 2
+struct OutletPressure {
+  DevMesh M;
+  const double *U, *sp, *patchP;
+  double* pob;
+  DM_HD void operator()(int b) const {
+    if (M.bbc[b] != BC_OUTLET) return;
+    const int q = M.bpatch[b];
+    const double beta = M.pbeta[q];
+    if (beta <= 0.0) return;
+    const int f = M.nint + b, o = M.bown[b];
+    const double* u = &U[3 * size_t(o)];
+    const double un = dot3(u, &M.nrm[3 * size_t(f)]);
+    const double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double ur = fmax(sp[2 * q + 1], SMALL);
+    const double s0 = 0.5 * (1.0 - tanh(un / (ur * 0.05)));
+    pob[b] = patchP[q] - beta * 0.5 * u2 * s0;
   }
 };
 
@@ -566,7 +685,7 @@ DM_INLINE void most_iterate(double u_par, double T_o, double T_s, double y, doub
 struct MostWall {
   DevMesh M;
   const double *U, *T;
-  double T_const, z0m, z0h, Ts;
+  double T_const, Ts;
   const double* Lcfg;
   double *Lb, *tau, *qw;
   DM_HD Red<2> operator()(int b) const {
@@ -584,7 +703,8 @@ struct MostWall {
     double Lg = Lb[b];
     if (fabs(Lg) < SMALL) Lg = *Lcfg;
     double us, ts, L;
-    most_iterate(mag, T_o, Ts, M.wsy[b], z0m, z0h, Lg, &us, &ts, &L);
+    const int q = M.bpatch[b];
+    most_iterate(mag, T_o, Ts, M.wsy[b], M.pz0m[q], M.pz0h[q], Lg, &us, &ts, &L);
     Lb[b] = L;
     for (int k = 0; k < 3; ++k) tau[3 * size_t(b) + k] = -us * us * up[k];
     qw[b] = -us * ts;
@@ -1415,6 +1535,18 @@ struct FluxRhs {
         const int bc = M.bbc[b];
         const double* ub = &Ub[3 * size_t(b)];
         double flux = is_solid(bc) ? 0.0 : ub[0] * sf[0] + ub[1] * sf[1] + ub[2] * sf[2];
+        if (bc == BC_OUTLET && M.pconv[M.bpatch[b]]) {
+          // continuity sets the outlet flux: the extrapolated cell velocity, not the convective
+          // boundary state (src/solver/mod_fv_solver.f90:2619-2627)
+          const int f = M.nint + b;
+          const double d[3] = {M.fctr[3 * size_t(f)] - M.ctr[3 * size_t(c)],
+                               M.fctr[3 * size_t(f) + 1] - M.ctr[3 * size_t(c) + 1],
+                               M.fctr[3 * size_t(f) + 2] - M.ctr[3 * size_t(c) + 2]};
+          flux = 0.0;
+          for (int k = 0; k < 3; ++k)
+            flux = flux + (U[3 * size_t(c) + k] + grad[DM_G(c, 0, k)] * d[0] + grad[DM_G(c, 1, k)] * d[1] +
+                           grad[DM_G(c, 2, k)] * d[2]) * sf[k];
+        }
         if (cproj && bc == BC_OUTLET) flux = flux - dt * M.cb[b] * (pob[b] - phi[c]);
         ff[fm] = flux;
         acc = acc - flux;

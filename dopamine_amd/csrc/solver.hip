@@ -38,6 +38,8 @@ FieldRef field(Solver* s, const std::string& name) {
   if (name == "rhs_p") return {s->rhs, nc};
   if (name == "H") return {s->H0, 3 * nc};  // after a step the last stage's H sits in H0
   if (name == "w_gp") return {s->gp, 3 * nc};
+  if (name == "outlet_pb") return {s->outlet_pb, nb};
+  if (name == "wm_filt") return {s->wm_filt, 3 * nb};
   if (s->q_w) {
     if (name == "q_w_bnd") return {s->q_w, nb};
     if (name == "L_bnd") return {s->L_bnd, nb};
@@ -72,7 +74,6 @@ void* dopamine_solver_create(const dopamine_mesh_desc* m, const dopamine_params*
   const int rc = guard([&] {
     if (!m || !p) throw std::runtime_error("null mesh or params");
     if (dopamine_device_count() <= 0) throw std::runtime_error("no HIP device available");
-    if (p->wall_model_height > 0.0) throw std::runtime_error("wall_model_height > 0 (sampling walk) is not supported");
     if (p->wall_model < 0 || p->wall_model > 2)
       throw std::runtime_error("wall_model must be 0 (log-law), 1 (none) or 2 (MOST)");
     if (p->scheme < 1 || p->scheme > 5)

@@ -168,7 +168,15 @@ class SolverT {
   double *s_val = nullptr, *s_flux = nullptr, *s_om = nullptr, *s_ph = nullptr;
   // MOST wall model: heat flux and Obukhov length per boundary face, the configuration's L
   double *q_w = nullptr, *L_bnd = nullptr, *L_cfg = nullptr;
-  double most_z0m = 0.0, most_z0h = 0.0;
+  // convective outlet / backflow traction: per patch (U_c, U_ref) of the last stage, per boundary
+  // face the previous stage's tendency; the outlet patches that need the stage statistics
+  double *out_sp = nullptr, *out_gprev = nullptr;
+  int out_ready = 0;
+  std::vector<int> out_q, out_start, out_n;
+  // log-law wall model time filter: filtered velocity per boundary face, mode (0 off, 1 fixed,
+  // 2 adaptive), whether it is seeded
+  double* wm_filt = nullptr;
+  int wm_fmode = 0, wm_filt_ready = 0;
   int n_amdw = 0;
   int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
@@ -265,15 +273,38 @@ class SolverT {
     }
     for (int q = 0; q < npatch; ++q)
       if (d.patch_bc[q] == BC_WALL_MODEL) n_wm++;
-    if (p.wall_model == 2) {
-      // MOST roughness (patch_roughness, src/abl/mod_abl_wall.f90:243-258, one value for every
-      // wall-model patch); the first cell must lie above the roughness layer (mod_fv_solver.f90:767-777)
-      most_z0m = std::fmax(p.most_z0_m, SMALL);
-      most_z0h = std::fmax(p.most_z0_h > 0.0 ? p.most_z0_h : most_z0m * std::exp(-2.0), SMALL);
+    // the log-law wall model samples wall_model_height above the wall (find_sample_cell,
+    // src/sgs/mod_wall_model.f90:126-173); MOST keeps the wall-adjacent cell
+    if (p.wall_model == 0 && p.wall_model_height > 0.0)
       for (int b = 0; b < nb; ++b)
-        if (bbc[b] == BC_WALL_MODEL && wsy[b] / most_z0m <= 1.0)
+        if (bbc[b] == BC_WALL_MODEL) sample_walk(d, b, p.wall_model_height, &wsc[b], &wsy[b]);
+    // per-patch roughness (patch_roughness, src/abl/mod_abl_wall.f90:249-262): a positive patch
+    // z0_m and a non-zero patch z0_h override the global values; a non-positive z0_h is z0_m e^-2
+    std::vector<double> z0m_h(std::max(1, npatch)), z0h_h(std::max(1, npatch)), beta_h(std::max(1, npatch), 0.0);
+    std::vector<int> conv_h(std::max(1, npatch), 0);
+    for (int q = 0; q < npatch; ++q) {
+      double zm = (p.patch_z0_m && p.patch_z0_m[q] > 0.0) ? p.patch_z0_m[q] : p.most_z0_m;
+      if (zm <= 0.0) zm = p.most_z0_m;
+      zm = std::fmax(zm, SMALL);
+      double zh = (p.patch_z0_h && p.patch_z0_h[q] != 0.0) ? p.patch_z0_h[q] : p.most_z0_h;
+      if (zh <= 0.0) zh = zm * std::exp(-2.0);
+      z0m_h[q] = zm;
+      z0h_h[q] = std::fmax(zh, SMALL);
+      conv_h[q] = p.patch_outlet_conv ? (p.patch_outlet_conv[q] != 0) : 0;
+      beta_h[q] = p.patch_backflow_beta ? p.patch_backflow_beta[q] : 0.0;
+      if (d.patch_bc[q] == BC_OUTLET && (conv_h[q] || beta_h[q] > 0.0) && d.patch_nfaces[q] > 0) {
+        out_q.push_back(q);
+        out_start.push_back(d.patch_start[q]);
+        out_n.push_back(d.patch_nfaces[q]);
+      }
+    }
+    if (p.wall_model == 2) {
+      // the first cell must lie above each patch's roughness layer (mod_fv_solver.f90:736-777)
+      for (int b = 0; b < nb; ++b)
+        if (bbc[b] == BC_WALL_MODEL && wsy[b] / z0m_h[bpatch[b]] <= 1.0)
           throw std::runtime_error("wall model: first cell inside the roughness layer (y_w / z0_m <= 1)");
     }
+    wm_fmode = p.wall_model_filter_tf_fac > SMALL ? 2 : (p.wall_model_filter_time > SMALL ? 1 : 0);
     // refinement-interface faces (build_interface_diagnostics, mod_fv_solver.f90:685-718): more
     // correctors and a looser stagnation test
     for (int f = 0; f < nint; ++f)
@@ -563,6 +594,10 @@ class SolverT {
     }
     M.bbc = up(bbc); M.bpatch = up(bpatch); M.bown = up(bown); M.pnb = up(pnb); M.pair = up(pair);
     M.pgc = up(pgc); M.pdcf = up(pdcf); M.cb = up(cb); M.kbnd = up(kbnd); M.wsy = up(wsy); M.wsc = up(wsc);
+    M.pconv = up(conv_h); M.pbeta = up(beta_h); M.pz0m = up(z0m_h); M.pz0h = up(z0h_h);
+    out_sp = up(std::vector<double>(2 * size_t(std::max(1, npatch)), 0.0));
+    out_gprev = up(std::vector<double>(3 * size_t(std::max(1, nb)), 0.0));
+    wm_filt = up(std::vector<double>(3 * size_t(std::max(1, nb)), 0.0));
     M.pcoef = up(pcoef.empty() ? std::vector<double>(1, 0.0) : pcoef);
     M.rcgc = up(rcgc); M.xgc = up(xgc); M.kint = up(kint);
     M.pown = up(pown.empty() ? std::vector<int>(1, -1) : pown);
@@ -807,7 +842,73 @@ class SolverT {
 
   // ---------------------------------------------------------------- boundary passes
 
-  void bc_velocity(int filter, const double* gU) { x.for_each(nb, BcVelocity{M, U, gU, U_bnd, filter}); }
+  void bc_velocity(int filter, const double* gU) { x.for_each(nb, BcVelocity{M, U, gU, U_bnd, filter, out_ready}); }
+
+  // find_sample_cell (src/sgs/mod_wall_model.f90:126-173): from the wall face's owner, step to
+  // the face neighbour farthest from the wall along the inward normal until the distance reaches
+  // h_wm (at most 20 steps, interior faces only)
+  static void sample_walk(const dopamine_mesh_desc& d, int b, double h_wm, int* cell, double* y) {
+    const int fg = d.n_int_faces + b;
+    const double* n = &d.face_normal[3 * size_t(fg)];
+    const double* xw = &d.face_centre[3 * size_t(fg)];
+    const double n_in[3] = {-n[0], -n[1], -n[2]};
+    int c = d.face_owner[fg];
+    auto inward = [&](int cc) {
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i) s = s + (d.cell_centroid[3 * size_t(cc) + i] - xw[i]) * n_in[i];
+      return s;
+    };
+    *cell = c;
+    *y = std::fabs(inward(c)) + SMALL;
+    for (int step = 1; step <= 20; ++step) {
+      if (*y >= h_wm) break;
+      int best = -1;
+      double best_dist = *y;
+      for (int q = d.cell_face_offset[c]; q < d.cell_face_offset[c + 1]; ++q) {
+        const int fj = d.cell_faces[q];
+        if (fj < 0 || fj >= d.n_faces || d.face_neighbour[fj] < 0) continue;
+        const int other = d.face_owner[fj] == c ? d.face_neighbour[fj] : d.face_owner[fj];
+        const double dist = inward(other);
+        if (dist > best_dist) {
+          best_dist = dist;
+          best = other;
+        }
+      }
+      if (best < 0) break;
+      c = best;
+      *cell = best;
+      *y = best_dist;
+    }
+  }
+
+  // outlet_advance_boundary (src/bc/mod_bc.f90:681-772): each active outlet patch's stage-start
+  // statistics, then one RK3W stage of the convective faces' boundary velocity
+  void outlet_advance(double dt, double a, double bk) {
+    if (out_q.empty()) return;
+    for (size_t i = 0; i < out_q.size(); ++i)
+      x.template reduce<3, false>(out_n[i], OutletStat{M, U, out_start[i]}, OutletStatFin{out_sp, out_q[i]});
+    x.for_each(nb, OutletAdvance{M, U, grad, out_sp, U_bnd, out_gprev, dt, a, bk});
+    out_ready = 1;
+  }
+  // apply_wall_model_bc with the time filter (src/sgs/mod_wall_model.f90:176-313); the first
+  // stage of a step hands the filter dt, later stages 0, so it advances once per step
+  void wall_model(double dt_use) {
+    WallModel w{M, U, tau, prm.nu};
+    if (wm_fmode) {
+      w.uf = wm_filt;
+      w.fmode = wm_fmode;
+      w.seed = wm_filt_ready ? 0 : 1;
+      w.w = wm_fmode == 1 ? dt_use / (prm.wall_model_filter_time + dt_use) : 0.0;
+      w.dt_use = dt_use;
+      w.tf = prm.wall_model_filter_tf_fac;
+    }
+    x.for_each(nb, w);
+    if (wm_fmode) wm_filt_ready = 1;
+  }
+  // update_outlet_pressure (src/bc/mod_bc.f90:789-818) from the predicted velocity
+  void outlet_pressure() {
+    if (!out_q.empty()) x.for_each(nb, OutletPressure{M, U, out_sp, patchP, outlet_pb});
+  }
   void bc_pressure(int filter, const double* g, double scale, const double* pob) {
     x.for_each(nb, BcPressure{M, P, nullptr, g, pob, patchP, p_bnd, scale, filter});
   }
@@ -1216,11 +1317,11 @@ class SolverT {
       if (prm.wall_model == 1)
         x.zero(tau, 3 * size_t(std::max(1, nb)));
       else if (n_wm && prm.wall_model == 2)
-        x.template reduce<2, false>(nb, MostWall{M, U, bous ? T : nullptr, prm.T_ref, most_z0m, most_z0h, prm.T_surface,
+        x.template reduce<2, false>(nb, MostWall{M, U, bous ? T : nullptr, prm.T_ref, prm.T_surface,
                                                 L_cfg, L_bnd, tau, q_w},
                                     MostLFin{L_cfg, (prm.compute_L && k == 0) ? 1 : 0});
       else if (n_wm)
-        x.for_each(nb, WallModel{M, U, tau, prm.nu});
+        wall_model(k == 0 ? dt : 0.0);
       if (scheme == SCHEME_QUICK) x.for_each(nf, ReconFace<SCHEME_QUICK>{M, U, U_bnd, UF, grad, phiF});
       if (scheme == SCHEME_VANLEER) x.for_each(nf, ReconFace<SCHEME_VANLEER>{M, U, U_bnd, UF, grad, phiF});
       if (scheme == SCHEME_KOREN) x.for_each(nf, ReconFace<SCHEME_KOREN>{M, U, U_bnd, UF, grad, phiF});
@@ -1235,9 +1336,11 @@ class SolverT {
       x.tend(K_EXPLICIT);
       if (bous) explicit_rhs_T(dt, k);
       if (ns) explicit_rhs_C(dt, k);
+      outlet_advance(dt, RK3_A[k], RK3_B[k]);  // from the stage-start velocity, before the predictor lands
       std::swap(U, U2);
       if (bous) std::swap(T, T2);
       for (int q = 0; q < ns; ++q) std::swap(Cs[q], Cs2[q]);
+      outlet_pressure();
       velocity_bc_both();
       x.copy(p_old, P, nc);
       if (cproj)
@@ -1369,7 +1472,10 @@ class SolverT {
     std::FILE* f = std::fopen(ckpt_name(dir, step, rank).c_str(), "wb");
     if (!f) throw std::runtime_error("cannot open " + ckpt_name(dir, step, rank));
     const int nbb = std::max(1, nb);
-    int32_t header[8] = {nc, ns, 0, step, ns > 0 ? 1 : 0, 0, u_face_ready ? 1 : 0, 1 + (q_w ? 2 + 4 : 0)};
+    // header[7]: trailing wall records, 1 tau_w, 2 q_w, 4 L, 8 the wall-stress filter state
+    const bool filt = wm_fmode && wm_filt_ready;
+    int32_t header[8] = {nc, ns, 0, step, ns > 0 ? 1 : 0, 0, u_face_ready ? 1 : 0,
+                         1 + (q_w ? 2 + 4 : 0) + (filt ? 8 : 0)};
     std::vector<double> h(3 * size_t(std::max(nc, nf)));
     std::vector<int64_t> gid(nc);
     for (int c = 0; c < nc; ++c) gid[c] = c + 1;
@@ -1409,6 +1515,10 @@ class SolverT {
         put_record(f, hb.data(), 8 * size_t(nb));
         x.download(hb.data(), L_bnd, nbb);
         put_record(f, hb.data(), 8 * size_t(nb));
+      }
+      if (filt) {
+        x.download(hb.data(), wm_filt, 3 * size_t(nbb));
+        put_record(f, hb.data(), 24 * size_t(nb));
       }
     } catch (...) {
       std::fclose(f);
@@ -1474,7 +1584,12 @@ class SolverT {
         double* dst = bit == 2 ? q_w : L_bnd;
         if (dst) x.upload(dst, hb.data(), nb);
       }
-      if (header[7] & (8 | 16)) throw std::runtime_error("checkpoint: wall-filter / inflow-eddy records are not supported");
+      if (header[7] & 8) {  // the filter state is adopted as seeded (wall_model_set_filter_state)
+        get_record(f, hb.data(), 24 * size_t(nb));
+        x.upload(wm_filt, hb.data(), 3 * size_t(nb));
+        wm_filt_ready = 1;
+      }
+      if (header[7] & 16) throw std::runtime_error("checkpoint: inflow-eddy records are not supported");
     } catch (...) {
       std::fclose(f);
       throw;

@@ -105,6 +105,18 @@ class SolverParams:
     # per-patch values, by patch name: p_val (outlet), U_val (inlet / moving wall)
     patch_p: dict = field(default_factory=dict)
     patch_U: dict = field(default_factory=dict)
+    # outlet patches (src/io/mod_input_parser.f90:481-521): 'extrapolate' (default) or 'convective'
+    # (du_b/dt = -U_c n.grad u, src/bc/mod_bc.f90:681-786); the backflow-traction coefficient
+    # (< 0: 1 for a convective outlet, else 0; src/bc/mod_bc.f90:789-818)
+    patch_outlet_mode: dict = field(default_factory=dict)
+    patch_backflow_beta: dict = field(default_factory=dict)
+    # MOST roughness per wall patch (0 inherits z0_m / z0_h; a negative z0_h derives it from z0_m)
+    patch_z0_m: dict = field(default_factory=dict)
+    patch_z0_h: dict = field(default_factory=dict)
+    # log-law wall model time filter (src/sgs/mod_wall_model.f90:176-313): the exponential filter's
+    # time constant (<= 0 off) and the adaptive factor f of T_f = f y / (kappa u_tau) (<= 0 off)
+    wall_model_filter_time: float = 0.0
+    wall_model_filter_tf_fac: float = 0.0
 
     def validate(self):
         if self.cell_projection.upper() not in ("LS", "EXACT"):
@@ -134,6 +146,23 @@ class SolverParams:
             raise ValueError(f"scalar_scheme {self.scalar_scheme!r} not supported")
         if self.adapt_dt:
             raise ValueError("adapt_dt is not supported; the fixed-dt CFL clamp of solver_advance applies")
+        for name, mode in self.patch_outlet_mode.items():
+            if str(mode).strip().lower() not in ("extrapolate", "convective"):
+                raise ValueError(f"unrecognised patch_outlet_mode {mode!r} for patch {name!r}")
+
+    def bc_ext_arrays(self, mesh):
+        """Per-patch convective-outlet flag (int32), backflow-traction coefficient and MOST roughness
+        lengths in the mesh's patch order, as read_bc resolves them (mod_input_parser.f90:504-529)."""
+        n = max(1, mesh.n_patches)
+        conv = np.zeros(n, dtype=np.int32)
+        beta, z0m, z0h = np.zeros(n), np.zeros(n), np.zeros(n)
+        for i, name in enumerate(mesh.patch_names[:mesh.n_patches]):
+            conv[i] = 1 if str(self.patch_outlet_mode.get(name, "extrapolate")).strip().lower() == "convective" else 0
+            b = float(self.patch_backflow_beta.get(name, -1.0))
+            beta[i] = (1.0 if conv[i] else 0.0) if b < 0.0 else b
+            z0m[i] = float(self.patch_z0_m.get(name, 0.0))
+            z0h[i] = float(self.patch_z0_h.get(name, 0.0))
+        return conv, beta, z0m, z0h
 
     def scalar_arrays(self, mesh):
         """Per-patch scalar BC arrays [MAX_SCALARS * patch + scalar]: mode (int32), value, flux,

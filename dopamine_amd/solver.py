@@ -47,7 +47,9 @@ class Params(ctypes.Structure):
                 ("U_bulk_target", _d * 3), ("n_scalars", _i32), ("scalar_scheme", _i32),
                 ("scalar_diffusivity", _d * 4), ("scalar_pr_t", _d * 4), ("scalar_bc_mode", _P(_i32)),
                 ("scalar_bc_value", _P(_d)), ("scalar_bc_flux", _P(_d)), ("scalar_bc_omega", _P(_d)),
-                ("scalar_bc_phase", _P(_d))]
+                ("scalar_bc_phase", _P(_d)), ("patch_outlet_conv", _P(_i32)), ("patch_backflow_beta", _P(_d)),
+                ("patch_z0_m", _P(_d)), ("patch_z0_h", _P(_d)), ("wall_model_filter_time", _d),
+                ("wall_model_filter_tf_fac", _d)]
 
 
 STATS_COLUMNS = ("E", "eps", "Umax", "Ub", "tauw_mean", "nu_sgs_mean", "max_abs_divU", "volavg_abs_divU")
@@ -67,6 +69,7 @@ def build_abi(mesh, params, precision=0):
                   *[arr(getattr(m, n), t) for n, t in MESH_FIELDS])
     pv, uv, tv = params.patch_arrays(mesh, with_T=True)
     sm, sv, sfl, so, sph = params.scalar_arrays(mesh)
+    oc, ob, zm, zh = params.bc_ext_arrays(mesh)
     pr = Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
                 0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
                 params.sgs_id(), params.c_vreman, params.cs_smagorinsky, WALL_MODEL_IDS[params.wall_model],
@@ -80,7 +83,9 @@ def build_abi(mesh, params, precision=0):
                 params.learn_seed, params.learn_lr, params.z0_m, params.z0_h, params.T_surface, params.L_obukhov,
                 int(params.compute_L), params.projection_id(), params.cell_projection_tol, *params.forcing_args(),
                 params.n_scalars, params.scalar_scheme_id(), (_d * 4)(*params.scalar_diffusivity),
-                (_d * 4)(*params.scalar_pr_t), arr(sm, _i32), arr(sv, _d), arr(sfl, _d), arr(so, _d), arr(sph, _d))
+                (_d * 4)(*params.scalar_pr_t), arr(sm, _i32), arr(sv, _d), arr(sfl, _d), arr(so, _d), arr(sph, _d),
+                arr(oc, _i32), arr(ob, _d), arr(zm, _d), arr(zh, _d), params.wall_model_filter_time,
+                params.wall_model_filter_tf_fac)
     return md, pr, keep
 
 
@@ -149,7 +154,8 @@ class FVSolver:
                        "grad_U": 9 * mesh.n_cells, "U_face": mesh.n_faces, "tau_w_bnd": 3 * max(1, mesh.n_bnd_faces),
                        "rhs_p": mesh.n_cells, "H": 3 * mesh.n_cells, "w_gp": 3 * mesh.n_cells,
                        "T": mesh.n_cells, "T_bnd": max(1, mesh.n_bnd_faces), "grad_T": 3 * mesh.n_cells,
-                       "q_w_bnd": max(1, mesh.n_bnd_faces), "L_bnd": max(1, mesh.n_bnd_faces)}
+                       "q_w_bnd": max(1, mesh.n_bnd_faces), "L_bnd": max(1, mesh.n_bnd_faces),
+                       "outlet_pb": max(1, mesh.n_bnd_faces), "wm_filt": 3 * max(1, mesh.n_bnd_faces)}
         for k in range(params.n_scalars):
             self._sizes.update({f"C{k}": mesh.n_cells, f"C{k}_bnd": max(1, mesh.n_bnd_faces),
                                 f"grad_C{k}": 3 * mesh.n_cells})

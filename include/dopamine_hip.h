@@ -45,7 +45,7 @@ typedef struct dopamine_params {
   double c_vreman;           /* reference C_VREMAN = 0.025 */
   double cs_smag;            /* 0.1 */
   int32_t wall_model;        /* 0 log-law (Reichardt), 1 none, 2 MOST (Monin-Obukhov, rough wall) */
-  double wall_model_height;  /* <= 0: wall-adjacent cell */
+  double wall_model_height;  /* log-law sampling height; <= 0: wall-adjacent cell */
   int32_t n_nonortho_corr;   /* 0 = auto */
   double min_cos_theta;
   int32_t rhie_chow_dual;
@@ -106,6 +106,19 @@ typedef struct dopamine_params {
   double scalar_diffusivity[4], scalar_pr_t[4];
   const int32_t *scalar_bc_mode;
   const double *scalar_bc_value, *scalar_bc_flux, *scalar_bc_omega, *scalar_bc_phase;
+  /* boundary extensions, per patch (n_patches entries; null: all off): patch_outlet_conv 1 = the
+   * outlet's boundary velocity is a convected RK state (patch_outlet_mode 'convective',
+   * src/bc/mod_bc.f90:681-786); patch_backflow_beta > 0 the backflow-traction coefficient of the
+   * outlet pressure (:789-818); patch_z0_m / patch_z0_h the MOST roughness of a wall patch (0
+   * inherits most_z0_m / most_z0_h; a negative z0_h derives it from z0_m,
+   * src/abl/mod_abl_wall.f90:249-262). The log-law wall model (wall_model 0) samples the velocity
+   * wall_model_height above the wall (> 0: the cell-to-cell walk of find_sample_cell) and filters
+   * it in time: wall_model_filter_time > 0 an exponential filter of that time constant,
+   * wall_model_filter_tf_fac > 0 the adaptive T_f = f y / (kappa u_tau), advanced once per step
+   * (src/sgs/mod_wall_model.f90:126-313). */
+  const int32_t *patch_outlet_conv;
+  const double *patch_backflow_beta, *patch_z0_m, *patch_z0_h;
+  double wall_model_filter_time, wall_model_filter_tf_fac;
 } dopamine_params;
 
 /* Create a solver on the current HIP device (mesh + params copied to HBM). NULL on error. */

@@ -44,7 +44,9 @@ class _Params(ctypes.Structure):
                 ("mass_flux", ctypes.c_int), ("U_bulk_target", _d * 3),
                 ("n_scalars", ctypes.c_int), ("scalar_scheme", ctypes.c_int), ("scalar_diffusivity", _d * 4),
                 ("scalar_pr_t", _d * 4), ("scalar_bc_mode", _P(ctypes.c_int)), ("scalar_bc_value", _P(_d)),
-                ("scalar_bc_flux", _P(_d)), ("scalar_bc_omega", _P(_d)), ("scalar_bc_phase", _P(_d))]
+                ("scalar_bc_flux", _P(_d)), ("scalar_bc_omega", _P(_d)), ("scalar_bc_phase", _P(_d)),
+                ("patch_outlet_conv", _P(ctypes.c_int)), ("patch_backflow_beta", _P(_d)), ("patch_z0_m", _P(_d)),
+                ("patch_z0_h", _P(_d)), ("wall_model_filter_time", _d), ("wall_model_filter_tf_fac", _d)]
 
 
 _lib = None
@@ -97,6 +99,13 @@ def lib():
         L.om_op_viscous_scalar.argtypes = [vp, _P(_d), _P(_d), _P(_d), _d, _d, _P(_d)]
         L.om_op_sgs.argtypes = [vp, ci, _d, ci, _d, _P(_d)]
         L.om_op_wall_model.argtypes = [vp, _P(_d)]
+        L.om_outlet_dn_u.restype = _d
+        L.om_outlet_dn_u.argtypes = [_d] * 6
+        L.om_wall_sample.restype = ci
+        L.om_wall_sample.argtypes = [vp, ci, _P(_d)]
+        L.om_wall_filter_state.restype = ci
+        L.om_wall_filter_state.argtypes = [vp, _P(_d)]
+        L.om_outlet_speeds.argtypes = [vp, _P(_d)]
         L.om_cg_jacobi.argtypes = [ctypes.c_int, ctypes.c_int, _P(_i32), _P(_i32), _P(_d), _P(_d), _P(_d), _P(_d),
                                    _d, _d, ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(_d)]
         _lib = L
@@ -137,6 +146,9 @@ class Solver:
         pv, uv, tv = params.patch_arrays(mesh, with_T=True)
         sm, sv, sfl, so, sph = params.scalar_arrays(mesh)
         scalar_ptrs = (arr(sm, ctypes.c_int), arr(sv, _d), arr(sfl, _d), arr(so, _d), arr(sph, _d))
+        oc, ob, zm, zh = params.bc_ext_arrays(mesh)
+        ext = (arr(oc, ctypes.c_int), arr(ob, _d), arr(zm, _d), arr(zh, _d), params.wall_model_filter_time,
+               params.wall_model_filter_tf_fac)
         from dopamine_amd.params import WALL_MODEL_IDS
         self._pr = _Params(params.nu, (_d * 3)(*params.f_const), params.scheme_id(),
                            0 if params.gradient_scheme.upper() == "LEAST_SQUARES" else 1, params.gg_iterations,
@@ -150,7 +162,7 @@ class Solver:
                            params.z0_m, params.z0_h, params.T_surface, params.L_obukhov, int(params.compute_L),
                            params.projection_id(), params.cell_projection_tol, *params.forcing_args(),
                            params.n_scalars, params.scalar_scheme_id(), (_d * 4)(*params.scalar_diffusivity),
-                           (_d * 4)(*params.scalar_pr_t), *scalar_ptrs)
+                           (_d * 4)(*params.scalar_pr_t), *scalar_ptrs, *ext)
         self.params = params
         self.mesh_nc = mesh.n_cells
         self.h = L.om_create(ctypes.byref(self._m), ctypes.byref(self._pr))
@@ -305,10 +317,33 @@ class Solver:
         lib().om_op_wall_model(self.h, _p(out, _d))
         return out
 
+    def wall_sample(self, b):
+        """Sampling cell and wall distance of boundary face b (the log-law wall model)."""
+        y = _d(0.0)
+        c = lib().om_wall_sample(self.h, b, ctypes.byref(y))
+        return c, y.value
+
+    def wall_filter_state(self):
+        """(seeded, filtered wall-parallel velocity per boundary face (nb, 3))."""
+        out = np.zeros(self.array("tau_w_bnd").size)
+        ready = lib().om_wall_filter_state(self.h, _p(out, _d))
+        return bool(ready), out.reshape(-1, 3)
+
+    def outlet_speeds(self):
+        """Per patch (convection speed U_c, reference speed U_ref) of the last stage."""
+        out = np.zeros(2 * max(1, self.mesh.n_patches))
+        lib().om_outlet_speeds(self.h, _p(out, _d))
+        return out.reshape(-1, 2)
+
     def nonortho_info(self):
         out = (ctypes.c_int * 3)()
         lib().om_nonortho_info(self.h, out)
         return {"skip_nonortho": out[0], "skip_grad_p": out[1], "n_nonortho_corr": out[2]}
+
+
+def outlet_dn_u(u_b, u_o, gdd, gt, len_d, nd):
+    """outlet_dn_u (src/bc/mod_bc.f90:775-786) of the oracle."""
+    return lib().om_outlet_dn_u(u_b, u_o, gdd, gt, len_d, nd)
 
 
 def wall_tau(u_par, y_w, nu):

@@ -63,7 +63,7 @@ struct om_solver {
   /* temperature */
   double *T, *T_bnd, *grad_T, *gT_prev, *H_T, *H_T0, *conv_T, *visc_T;
   /* MOST: per-face heat flux and Obukhov length, the configuration's L */
-  double *q_w, *L_bnd, L_cfg, most_z0h;
+  double *q_w, *L_bnd, L_cfg;
   /* exact cell projection: pressure faces that are not skipped, outlet boundary faces */
   int cp_n, cp_nout;
   int *cp_own, *cp_nbr, *cp_bcell;
@@ -73,6 +73,18 @@ struct om_solver {
   /* passive scalars */
   double *C[OM_MAX_SCALARS], *C_bnd[OM_MAX_SCALARS], *grad_C[OM_MAX_SCALARS], *H_C[OM_MAX_SCALARS],
       *H_C0[OM_MAX_SCALARS], *gC_prev;
+  /* convective outlet and backflow traction (src/bc/mod_bc.f90:61-63): per patch the mode, the
+   * traction coefficient, the convection speed and the reference speed; per boundary face the
+   * previous stage's boundary tendency; whether a stage has advanced the boundary state */
+  int out_active, out_ready;
+  int *out_conv;
+  double *out_beta, *out_uc, *out_uref, *out_gprev;
+  /* log-law wall model: filtered wall-parallel velocity per boundary face, whether it is seeded,
+   * the time step the first stage of the current step hands the filter */
+  double *wm_filt, wm_dt;
+  int wm_filt_ready;
+  /* MOST roughness lengths per patch */
+  double *z0m_p, *z0h_p;
 };
 
 /* body_force_now (src/solver/mod_fv_solver.f90:2904-2923): constant or mass-flux force plus the
@@ -172,6 +184,9 @@ static void apply_velocity_bc(om_solver *s, int filter, const double *gU) {
           for (int k = 0; k < 3; ++k)
             u[k] = u[k] + gU[GI(o, 0, k)] * d[0] + gU[GI(o, 1, k)] * d[1] + gU[GI(o, 2, k)] * d[2];
         }
+        /* a convective outlet face is an RK state advanced by outlet_advance_boundary; only its
+         * start-up value is extrapolated (src/bc/mod_bc.f90:304-311) */
+        if (bc == BC_OUTLET && s->out_ready && pid >= 0 && pid < s->npatch && s->out_conv[pid]) break;
         for (int k = 0; k < 3; ++k) ub[k] = u[k];
         break;
       }
@@ -700,20 +715,189 @@ static double wall_tau(double u_par, double y_w, double nu) {
   return ut * ut;
 }
 
-/* apply_wall_model_bc without time filtering (src/sgs/mod_wall_model.f90:176-313) */
-static void wall_model(om_solver *s) {
+/* apply_wall_model_bc (src/sgs/mod_wall_model.f90:176-313): the wall-parallel velocity of the
+ * sampling cell, optionally time-filtered (exponential with w = dt / (T + dt), or adaptive with
+ * eps = min(1, dt kappa u_tau / (f y)) from the previous filtered velocity; stages after the
+ * first pass dt = 0, so the filter advances once per step; the first filtered call seeds it),
+ * then the Reichardt law. dt_use is the step's dt at the first stage, else 0. */
+static void wall_model_dt(om_solver *s, double dt_use) {
+  const double t_use = s->p.wall_model_filter_time, tf_use = s->p.wall_model_filter_tf_fac;
+  const int adaptive = tf_use > SMALL;
+  const int do_filter = t_use > SMALL || adaptive;
+  const int seed = !s->wm_filt_ready;
+  const double w_filt = (do_filter && !adaptive) ? dt_use / (t_use + dt_use) : 0.0;
   for (int b = 0; b < s->nb; ++b) {
     const int f = s->nint + b;
     if (bc_of_face(s, f) != BC_WALL_MODEL) continue;
     const double *n = &s->m.face_normal[3 * f];
     const int c = s->ws_cell[b];
+    const double y_w = s->ws_y[b];
     const double *u = &s->U[3 * c];
     const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
-    const double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    double up[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+    if (do_filter) {
+      double *uf = &s->wm_filt[3 * b];
+      if (seed) {
+        for (int k = 0; k < 3; ++k) uf[k] = up[k];
+      } else {
+        double eps = w_filt;
+        if (adaptive) {
+          eps = 0.0;
+          if (dt_use > 0.0) {
+            const double prev = sqrt(uf[0] * uf[0] + uf[1] * uf[1] + uf[2] * uf[2]);
+            const double ut_prev = wall_utau(prev, y_w, s->p.nu);
+            eps = 1.0;
+            if (ut_prev > SMALL) eps = fmin(1.0, dt_use * KAPPA * ut_prev / (tf_use * y_w));
+          }
+        }
+        for (int k = 0; k < 3; ++k) uf[k] = uf[k] + eps * (up[k] - uf[k]);
+      }
+      for (int k = 0; k < 3; ++k) up[k] = uf[k];
+    }
     const double mag = sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
-    const double tau = wall_tau(mag, s->ws_y[b], s->p.nu);
+    const double tau = wall_tau(mag, y_w, s->p.nu);
     const double sc = -tau / (mag + SMALL);
     for (int k = 0; k < 3; ++k) s->tau_w[3 * b + k] = sc * up[k];
+  }
+  if (do_filter) s->wm_filt_ready = 1;
+}
+
+static void wall_model(om_solver *s) { wall_model_dt(s, 0.0); }
+
+/* find_sample_cell (src/sgs/mod_wall_model.f90:126-173): from the wall face's owner, step to the
+ * face neighbour farthest from the wall along the inward normal until the distance reaches h_wm
+ * (at most 20 steps; interior faces only) */
+static void find_sample_cell(const om_solver *s, int b, double h_wm, int *cell_out, double *y_out) {
+  const om_mesh *m = &s->m;
+  const int fg = s->nint + b;
+  const double *n = &m->face_normal[3 * fg], *xw = &m->face_centre[3 * fg];
+  const double n_in[3] = {-n[0], -n[1], -n[2]};
+  int c = m->face_owner[fg];
+  *cell_out = c;
+  {
+    const double d[3] = {m->cell_centroid[3 * c] - xw[0], m->cell_centroid[3 * c + 1] - xw[1],
+                         m->cell_centroid[3 * c + 2] - xw[2]};
+    *y_out = fabs(dot3(d, n_in)) + SMALL;
+  }
+  if (h_wm <= 0.0) return;
+  for (int step = 1; step <= 20; ++step) {
+    if (*y_out >= h_wm) break;
+    int best = -1;
+    double best_dist = *y_out;
+    for (int q = m->cell_face_offset[c]; q < m->cell_face_offset[c + 1]; ++q) {
+      const int fj = m->cell_faces[q];
+      if (fj < 0 || fj >= s->nf) continue;
+      if (m->face_neighbour[fj] < 0) continue;
+      int other = m->face_owner[fj];
+      if (other == c) other = m->face_neighbour[fj];
+      const double d[3] = {m->cell_centroid[3 * other] - xw[0], m->cell_centroid[3 * other + 1] - xw[1],
+                           m->cell_centroid[3 * other + 2] - xw[2]};
+      const double dist = dot3(d, n_in);
+      if (dist > best_dist) {
+        best_dist = dist;
+        best = other;
+      }
+    }
+    if (best < 0) break;
+    c = best;
+    *cell_out = best;
+    *y_out = best_dist;
+  }
+}
+
+int om_wall_sample(const om_solver *s, int b, double *y) {
+  if (y) *y = s->ws_y[b];
+  return s->ws_cell[b];
+}
+
+int om_wall_filter_state(const om_solver *s, double *out) {
+  if (out) memcpy(out, s->wm_filt, sizeof(double) * 3 * (size_t)s->nb);
+  return s->wm_filt_ready;
+}
+
+/* ---------------------------------------------------------------- convective outlet */
+
+/* outlet_dn_u (src/bc/mod_bc.f90:775-786): the Taylor expansion along the owner-to-face offset
+ * removes the curvature term of the one-sided difference */
+static double outlet_dn_u(double u_b, double u_o, double gdd, double gt, double len_d, double nd) {
+  return nd * (2.0 * (u_b - u_o) / len_d - gdd) + gt;
+}
+
+double om_outlet_dn_u(double u_b, double u_o, double gdd, double gt, double len_d, double nd) {
+  return outlet_dn_u(u_b, u_o, gdd, gt, len_d, nd);
+}
+
+/* outlet_advance_boundary (src/bc/mod_bc.f90:681-772): per outlet patch with a convective mode or
+ * a backflow traction, the area means of u.n (clipped at 0: the convection speed) and of |u| (the
+ * reference speed) from the stage-start cell velocity; then the convective faces' boundary
+ * velocity advances one RK3W stage of du_b/dt = -U_c n.grad u. */
+static void outlet_advance_boundary(om_solver *s, double dt, double a_k, double b_k) {
+  if (!s->out_active) return;
+  const om_mesh *m = &s->m;
+  double *stat = dz(3 * (size_t)s->npatch);
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b, q = m->face_patch[f];
+    if (q < 0 || q >= s->npatch || m->patch_bc[q] != BC_OUTLET) continue;
+    if (!s->out_conv[q] && s->out_beta[q] <= 0.0) continue;
+    const int o = m->face_owner[f];
+    const double *u = &s->U[3 * o], *n = &m->face_normal[3 * f];
+    const double area = m->face_area[f];
+    stat[3 * q] = stat[3 * q] + area * dot3(u, n);
+    stat[3 * q + 1] = stat[3 * q + 1] + area * sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    stat[3 * q + 2] = stat[3 * q + 2] + area;
+  }
+  for (int q = 0; q < s->npatch; ++q) {
+    if (stat[3 * q + 2] <= 0.0) continue;
+    s->out_uref[q] = stat[3 * q + 1] / stat[3 * q + 2];
+    s->out_uc[q] = fmax(stat[3 * q] / stat[3 * q + 2], 0.0);
+  }
+  free(stat);
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b, q = m->face_patch[f];
+    if (q < 0 || q >= s->npatch || m->patch_bc[q] != BC_OUTLET || !s->out_conv[q]) continue;
+    const int o = m->face_owner[f];
+    double dl[3];
+    for (int i = 0; i < 3; ++i) dl[i] = m->face_centre[3 * f + i] - m->cell_centroid[3 * o + i];
+    const double len_d = fmax(sqrt(dot3(dl, dl)), SMALL);
+    for (int i = 0; i < 3; ++i) dl[i] = dl[i] / len_d;
+    const double *n = &m->face_normal[3 * f];
+    const double nd = dot3(n, dl);
+    double tn[3];
+    for (int i = 0; i < 3; ++i) tn[i] = n[i] - nd * dl[i];
+    const double uc = s->out_uc[q];
+    for (int k = 0; k < 3; ++k) {
+      const double gdd = s->grad_U[GI(o, 0, k)] * dl[0] + s->grad_U[GI(o, 1, k)] * dl[1] + s->grad_U[GI(o, 2, k)] * dl[2];
+      const double gt = s->grad_U[GI(o, 0, k)] * tn[0] + s->grad_U[GI(o, 1, k)] * tn[1] + s->grad_U[GI(o, 2, k)] * tn[2];
+      const double g = -uc * outlet_dn_u(s->U_bnd[3 * b + k], s->U[3 * o + k], gdd, gt, len_d, nd);
+      s->U_bnd[3 * b + k] = s->U_bnd[3 * b + k] + dt * (a_k * g + b_k * s->out_gprev[3 * b + k]);
+      s->out_gprev[3 * b + k] = g;
+    }
+  }
+  s->out_ready = 1;
+}
+
+/* update_outlet_pressure (src/bc/mod_bc.f90:789-818): p_b = p_val - beta |u|^2 S0 / 2 with the
+ * backflow switch S0 = (1 - tanh(u.n / (U_ref delta))) / 2, delta = 0.05 (mod_parameters.f90:76) */
+static void update_outlet_pressure(om_solver *s) {
+  if (!s->out_active) return;
+  const om_mesh *m = &s->m;
+  for (int b = 0; b < s->nb; ++b) {
+    const int f = s->nint + b, q = m->face_patch[f];
+    if (q < 0 || q >= s->npatch || m->patch_bc[q] != BC_OUTLET || s->out_beta[q] <= 0.0) continue;
+    const int o = m->face_owner[f];
+    const double *u = &s->U[3 * o];
+    const double un = dot3(u, &m->face_normal[3 * f]);
+    const double u2 = dot3(u, u);
+    const double ur = fmax(s->out_uref[q], SMALL);
+    const double s0 = 0.5 * (1.0 - tanh(un / (ur * 0.05)));
+    s->outlet_pb[b] = (s->p.patch_p_val ? s->p.patch_p_val[q] : 0.0) - s->out_beta[q] * 0.5 * u2 * s0;
+  }
+}
+
+void om_outlet_speeds(const om_solver *s, double *out) {
+  for (int q = 0; q < s->npatch; ++q) {
+    out[2 * q] = s->out_uc[q];
+    out[2 * q + 1] = s->out_uref[q];
   }
 }
 
@@ -817,7 +1001,8 @@ static void most_wall(om_solver *s, int update_L) {
     double L_guess = s->L_bnd[b];
     if (fabs(L_guess) < SMALL) L_guess = s->L_cfg;
     double r[3];
-    om_most_iterate(mag, T_o, s->p.T_surface, y_w, s->p.most_z0_m, s->most_z0h, L_guess, r);
+    const int q = s->m.face_patch[f];
+    om_most_iterate(mag, T_o, s->p.T_surface, y_w, s->z0m_p[q], s->z0h_p[q], L_guess, r);
     s->L_bnd[b] = r[2];
     L_sum = L_sum + r[2];
     nwf++;
@@ -1678,6 +1863,17 @@ static void face_fluxes_and_rhs(om_solver *s, double dt, int incr) {
     else
       flux = s->U_bnd[3 * b] * s->m.face_sf[3 * f] + s->U_bnd[3 * b + 1] * s->m.face_sf[3 * f + 1] +
              s->U_bnd[3 * b + 2] * s->m.face_sf[3 * f + 2];
+    if (bc == BC_OUTLET && s->out_conv[s->m.face_patch[f]]) {
+      /* outlet flux is set by continuity: the extrapolated cell velocity, not the convective
+       * boundary state (mod_fv_solver.f90:2619-2627) */
+      const double d[3] = {s->m.face_centre[3 * f] - s->m.cell_centroid[3 * o],
+                           s->m.face_centre[3 * f + 1] - s->m.cell_centroid[3 * o + 1],
+                           s->m.face_centre[3 * f + 2] - s->m.cell_centroid[3 * o + 2]};
+      flux = 0.0;
+      for (int k = 0; k < 3; ++k)
+        flux = flux + (s->U[3 * o + k] + s->grad_U[GI(o, 0, k)] * d[0] + s->grad_U[GI(o, 1, k)] * d[1] +
+                       s->grad_U[GI(o, 2, k)] * d[2]) * s->m.face_sf[3 * f + k];
+    }
     if (bc == BC_OUTLET && s->p.cell_projection == OM_PROJ_EXACT) {
       /* EXACT: the compact old-pressure term of the outlet face (mod_fv_solver.f90:2629-2636) */
       const double *n = &s->m.face_normal[3 * f];
@@ -2336,7 +2532,7 @@ static void stage_closures(om_solver *s, double dt_stab, int stage) {
   else if (s->p.wall_model == OM_WM_MOST)
     most_wall(s, stage == 0);
   else
-    wall_model(s);
+    wall_model_dt(s, stage == 0 ? s->wm_dt : 0.0);
 }
 
 int om_eval_closures(om_solver *s, double dt_stab) {
@@ -2447,6 +2643,7 @@ static int advance_rk3w(om_solver *s, double dt) {
   memset(s->rhs_p, 0, sizeof(double) * nc);
   const double time_n = s->time;
   int rc = 0;
+  s->wm_dt = dt;
   for (int k = 0; k < 3; ++k) {
     const double gdt = RK3_G[k] * dt;
     s->time = time_n + RK3_C[k] * dt;
@@ -2454,12 +2651,14 @@ static int advance_rk3w(om_solver *s, double dt) {
     stage_closures(s, gdt, k);
     explicit_rhs(s);
     const double a = RK3_A[k], bb = RK3_B[k];
+    outlet_advance_boundary(s, dt, a, bb);
     for (int c = 0; c < nc; ++c) {
       for (int q = 0; q < 3; ++q) s->U[3 * c + q] = s->U[3 * c + q] + dt * (a * s->H[3 * c + q] + bb * s->H0[3 * c + q]);
       if (s->p.boussinesq) s->T[c] = s->T[c] + dt * (a * s->H_T[c] + bb * s->H_T0[c]);
     }
     for (int q = 0; q < s->p.n_scalars; ++q)
       for (int c = 0; c < nc; ++c) s->C[q][c] = s->C[q][c] + dt * (a * s->H_C[q][c] + bb * s->H_C0[q][c]);
+    update_outlet_pressure(s);
     velocity_bc_both(s);
     memcpy(s->p_old, s->P, sizeof(double) * nc);
     if (s->p.cell_projection == OM_PROJ_EXACT)
@@ -2903,13 +3102,40 @@ om_solver *om_create(const om_mesh *m, const om_params *p) {
   s->q_w = dz(nb);
   s->L_bnd = dz(nb);
   s->L_cfg = p->L_obukhov;
-  if (p->wall_model == OM_WM_MOST) {
-    /* patch_roughness (src/abl/mod_abl_wall.f90:243-258) with one roughness for all wall patches;
-     * the first cell must lie above the roughness layer (mod_fv_solver.f90:767-777) */
-    s->p.most_z0_m = fmax(p->most_z0_m, SMALL);
-    s->most_z0h = fmax(p->most_z0_h > 0.0 ? p->most_z0_h : s->p.most_z0_m * exp(-KB_INV), SMALL);
+  /* the log-law wall model samples h_wm above the wall (find_sample_cell); MOST keeps the
+   * wall-adjacent cell (src/abl/mod_abl_wall.f90:101-103) */
+  if (p->wall_model == OM_WM_LOGLAW && p->wall_model_height > 0.0)
     for (int b = 0; b < nb; ++b)
-      if (bc_of_face(s, nint + b) == BC_WALL_MODEL && s->ws_y[b] / s->p.most_z0_m <= 1.0) {
+      if (bc_of_face(s, nint + b) == BC_WALL_MODEL) find_sample_cell(s, b, p->wall_model_height, &s->ws_cell[b], &s->ws_y[b]);
+  s->wm_filt = dz(3 * (size_t)nb);
+  /* convective outlet / backflow traction (ensure_bc_config_soa, src/bc/mod_bc.f90:119-148) */
+  s->out_conv = iz(s->npatch);
+  s->out_beta = dz(s->npatch);
+  s->out_uc = dz(s->npatch);
+  s->out_uref = dz(s->npatch);
+  s->out_gprev = dz(3 * (size_t)nb);
+  for (int q = 0; q < s->npatch; ++q) {
+    s->out_conv[q] = p->patch_outlet_conv ? (p->patch_outlet_conv[q] != 0) : 0;
+    s->out_beta[q] = p->patch_backflow_beta ? p->patch_backflow_beta[q] : 0.0;
+    if (m->patch_bc[q] == BC_OUTLET && (s->out_conv[q] || s->out_beta[q] > 0.0)) s->out_active = 1;
+  }
+  /* patch_roughness (src/abl/mod_abl_wall.f90:249-262, input parser :524-529): a positive patch
+   * z0_m and a non-zero patch z0_h override the global ones; a non-positive z0_h is z0_m e^-kB^-1 */
+  s->z0m_p = dz(s->npatch);
+  s->z0h_p = dz(s->npatch);
+  for (int q = 0; q < s->npatch; ++q) {
+    double zm = (p->patch_z0_m && p->patch_z0_m[q] > 0.0) ? p->patch_z0_m[q] : p->most_z0_m;
+    if (zm <= 0.0) zm = p->most_z0_m;
+    zm = fmax(zm, SMALL);
+    double zh = (p->patch_z0_h && p->patch_z0_h[q] != 0.0) ? p->patch_z0_h[q] : p->most_z0_h;
+    if (zh <= 0.0) zh = zm * exp(-KB_INV);
+    s->z0m_p[q] = zm;
+    s->z0h_p[q] = fmax(zh, SMALL);
+  }
+  if (p->wall_model == OM_WM_MOST) {
+    /* the first cell must lie above each patch's roughness layer (mod_fv_solver.f90:736-777) */
+    for (int b = 0; b < nb; ++b)
+      if (bc_of_face(s, nint + b) == BC_WALL_MODEL && s->ws_y[b] / s->z0m_p[m->face_patch[nint + b]] <= 1.0) {
         om_free(s);
         return NULL;
       }
@@ -2939,6 +3165,8 @@ void om_free(om_solver *s) {
   for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) free(dd[i]);
   int *ii[] = {s->per_nb, s->per_bf, s->nper_bf, s->ws_cell, s->ps_owner, s->ps_nb, s->ps_fmesh, s->ps_skip, s->dir_rows};
   for (size_t i = 0; i < sizeof(ii) / sizeof(ii[0]); ++i) free(ii[i]);
+  free(s->wm_filt); free(s->out_conv); free(s->out_beta); free(s->out_uc); free(s->out_uref); free(s->out_gprev);
+  free(s->z0m_p); free(s->z0h_p);
   free(s->built);
   free(s->act_int);
   free(s->act_bnd);
@@ -2961,7 +3189,8 @@ double *om_array(om_solver *s, const char *name, int64_t *len) {
            {"face_flux", s->face_flux, nf}, {"tau_w_bnd", s->tau_w, 3 * nb}, {"w_gp", s->gp, 3 * nc},
            {"rhs_p", s->rhs_p, nc}, {"H", s->H, 3 * nc}, {"Gf", s->Gf, 9 * nc}, {"Tinv", s->Tinv, 9 * nc},
            {"ps_coeff", s->ps_coeff, s->npf}, {"ps_diag", s->ps_diag, nc}, {"T", s->T, nc}, {"T_bnd", s->T_bnd, nb},
-           {"grad_T", s->grad_T, 3 * nc}, {"q_w_bnd", s->q_w, nb}, {"L_bnd", s->L_bnd, nb}};
+           {"grad_T", s->grad_T, 3 * nc}, {"q_w_bnd", s->q_w, nb}, {"L_bnd", s->L_bnd, nb},
+           {"outlet_pb", s->outlet_pb, nb}};
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i)
     if (strcmp(t[i].n, name) == 0) {
       if (len) *len = t[i].l;

@@ -90,6 +90,15 @@ typedef struct om_params {
   double scalar_diffusivity[4], scalar_pr_t[4];
   const int *scalar_bc_mode;
   const double *scalar_bc_value, *scalar_bc_flux, *scalar_bc_omega, *scalar_bc_phase;
+  /* boundary extensions, per patch (NULL: all off): convective outlet (patch_outlet_mode
+   * 'convective', src/bc/mod_bc.f90:681-786), backflow-traction coefficient (:789-818), MOST
+   * roughness lengths (0 inherits most_z0_m / most_z0_h; a negative z0_h derives it from z0_m,
+   * src/abl/mod_abl_wall.f90:249-262); the log-law wall model's exponential time filter
+   * (wall_model_filter_time > 0) and adaptive filter factor (wall_model_filter_tf_fac > 0:
+   * T_f = f y / (kappa u_tau)), src/sgs/mod_wall_model.f90:176-313 */
+  const int *patch_outlet_conv;
+  const double *patch_backflow_beta, *patch_z0_m, *patch_z0_h;
+  double wall_model_filter_time, wall_model_filter_tf_fac;
 } om_params;
 
 enum { OM_MAX_SCALARS = 4, OM_SCALAR_DIRICHLET = 1, OM_SCALAR_NEUMANN = 2, OM_SCALAR_ZERO_GRADIENT = 3 };
@@ -130,6 +139,15 @@ void om_op_viscous_scalar(om_solver *s, const double *phi, const double *pb, con
                           double prt, double *out);
 void om_op_sgs(om_solver *s, int model, double coef, int wall_gradient, double dt_stab, double *out);
 void om_op_wall_model(om_solver *s, double *out);
+/* outlet_dn_u (src/bc/mod_bc.f90:775-786): n.grad u at an outlet face from the face and owner
+ * values, the owner gradient along the offset (gdd) and along n - (n.d)d (gt), |d| and n.d */
+double om_outlet_dn_u(double u_b, double u_o, double gdd, double gt, double len_d, double nd);
+/* the wall-model sampling cell and distance of boundary face b, the filtered wall-parallel
+ * velocity (3 per boundary face) and whether the filter is seeded */
+int om_wall_sample(const om_solver *s, int b, double *y);
+int om_wall_filter_state(const om_solver *s, double *out);
+/* per-patch convective-outlet speed and reference speed of the last stage (2 per patch) */
+void om_outlet_speeds(const om_solver *s, double *out);
 /* Wall shear stress tau_w = u_tau^2 of the Reichardt-law Newton solve for a wall-parallel speed
  * u_par sampled at wall distance y_w (src/sgs/mod_wall_model.f90:50-115). */
 double om_wall_tau(double u_par, double y_w, double nu);

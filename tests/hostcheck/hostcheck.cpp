@@ -23,6 +23,8 @@ double* field(Solver* s, const std::string& name, int64_t* n) {
   if (name == "tau_w_bnd") { *n = 3 * nb; return s->tau; }
   if (name == "rhs_p") { *n = nc; return s->rhs; }
   if (name == "w_gp") { *n = 3 * nc; return s->gp; }
+  if (name == "outlet_pb") { *n = nb; return s->outlet_pb; }
+  if (name == "wm_filt") { *n = 3 * nb; return s->wm_filt; }
   if (s->q_w && name == "q_w_bnd") { *n = nb; return s->q_w; }
   if (s->q_w && name == "L_bnd") { *n = nb; return s->L_bnd; }
   if (s->bous && name == "T") { *n = nc; return s->T; }
