@@ -496,13 +496,29 @@ int dopamine_op_sgs_viscosity(void* h, int model, const double* U, const double*
   });
 }
 
-int dopamine_op_wall_model(void* h, const double* U, double nu, double* tau_w_bnd) {
+int dopamine_op_wall_model(void* h, const double* U, double nu, double h_wm, double t_filt, double dt,
+                           double tf_fac, double* tau_w_bnd) {
   return guard([&] {
     Solver* s = solver_of(h);
+    const double hs = s->prm.wall_model_height;
+    if ((h_wm > 0.0 || hs > 0.0) && h_wm != hs)
+      throw std::runtime_error("h_wm must equal the solver's wall_model_height (" + std::to_string(hs) +
+                               "): the sampling cells are found at create");
     const Sizes z = sizes(s);
     OpBufs b(s->x);
     double* t = b.zeros(3 * z.nb);
-    if (s->nb > 0) s->x.for_each(s->nb, dm::WallModel{s->M, b.in(U, 3 * z.nc), t, nu});
+    const int fmode = tf_fac > dm::SMALL ? 2 : (t_filt > dm::SMALL ? 1 : 0);
+    dm::WallModel w{s->M, b.in(U, 3 * z.nc), t, nu};
+    if (fmode) {
+      w.uf = s->wm_filt;
+      w.fmode = fmode;
+      w.seed = s->wm_filt_ready ? 0 : 1;
+      w.w = fmode == 1 ? dt / (t_filt + dt) : 0.0;
+      w.dt_use = dt;
+      w.tf = tf_fac;
+    }
+    if (s->nb > 0) s->x.for_each(s->nb, w);
+    if (fmode) s->wm_filt_ready = 1;
     b.out(tau_w_bnd, t, 3 * z.nb);
     return 0;
   });

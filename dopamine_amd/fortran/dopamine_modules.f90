@@ -11,7 +11,8 @@
 ! counts. Each routine copies its array arguments to the GPU, runs one kernel and copies the
 ! result back; the whole-step path (FVSolver%advance) keeps every field on the GPU across steps.
 ! Optional boundary-value arguments of the reference routines are required here (U_bnd,
-! phi_bnd): the routines stop with a message when they are absent.
+! phi_bnd): the routines stop with a message when they are absent. The wall model's sampling
+! height must be the one the solver was created with, and its filter state lives in the solver.
 
 module dopamine_capi
   use iso_c_binding
@@ -69,11 +70,12 @@ module dopamine_capi
       real(c_double), intent(out) :: nus(*)
       integer(c_int) :: rc
     end function
-    function dopamine_op_wall_model(h, u, nu, tau) bind(C, name='dopamine_op_wall_model') result(rc)
+    function dopamine_op_wall_model(h, u, nu, h_wm, t_filt, dt, tf_fac, tau) bind(C, name='dopamine_op_wall_model') &
+        result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h
       real(c_double), intent(in) :: u(*)
-      real(c_double), value :: nu
+      real(c_double), value :: nu, h_wm, t_filt, dt, tf_fac
       real(c_double), intent(out) :: tau(*)
       integer(c_int) :: rc
     end function
@@ -519,8 +521,8 @@ module mod_wall_model
 
 contains
 
-  ! src/sgs/mod_wall_model.f90:176 at the wall-adjacent cell without the time filter: h_wm > 0
-  ! and t_filt > 0 stop with a message
+  ! src/sgs/mod_wall_model.f90:176; h_wm must equal the solver's wall_model_height (the sampling
+  ! cells are found when the solver is created); the filter state persists in the solver
   subroutine apply_wall_model_bc(U, nu, msh, tau_w_bnd, h_wm, t_filt, dt, tf_fac)
     real(rp), intent(in) :: U(:,:)
     real(rp), intent(in) :: nu
@@ -530,14 +532,13 @@ contains
     real(rp), intent(in), optional :: t_filt
     real(rp), intent(in), optional :: dt
     real(rp), intent(in), optional :: tf_fac
-    if (present(h_wm)) then
-      if (h_wm > 0.0_rp) error stop 'apply_wall_model_bc: sampling height above the first cell is not supported'
-    end if
-    if (present(t_filt)) then
-      if (t_filt > 0.0_rp) error stop 'apply_wall_model_bc: the time filter is not supported'
-    end if
-    if (present(dt) .or. present(tf_fac)) continue
-    call check(dopamine_op_wall_model(msh%handle, U, nu, tau_w_bnd), 'apply_wall_model_bc')
+    real(rp) :: h_use, t_use, dt_use, tf_use
+    h_use = 0.0_rp; if (present(h_wm)) h_use = h_wm
+    t_use = 0.0_rp; if (present(t_filt)) t_use = t_filt
+    dt_use = 0.0_rp; if (present(dt)) dt_use = dt
+    tf_use = 0.0_rp; if (present(tf_fac)) tf_use = tf_fac
+    call check(dopamine_op_wall_model(msh%handle, U, nu, h_use, t_use, dt_use, tf_use, tau_w_bnd), &
+               'apply_wall_model_bc')
   end subroutine
 
 end module mod_wall_model

@@ -220,8 +220,12 @@ int dopamine_op_gg_gradient_scalar(void *solver, const double *phi, const double
  * dt_stab > 0 caps nu_t at 0.4 V^(2/3) / dt_stab. */
 int dopamine_op_sgs_viscosity(void *solver, int model, const double *U, const double *grad_U, double coef, double nu,
                               int wall_gradient, double dt_stab, double *nu_sgs);
-/* apply_wall_model_bc without time filter (src/sgs/mod_wall_model.f90:176): tau_w_bnd(3, n_bnd). */
-int dopamine_op_wall_model(void *solver, const double *U, double nu, double *tau_w_bnd);
+/* apply_wall_model_bc (src/sgs/mod_wall_model.f90:176): tau_w_bnd(3, n_bnd) of the log-law wall
+ * model. h_wm: the sampling height, which must equal the solver's wall_model_height (the sampling
+ * cells are found at create); t_filt > 0 / tf_fac > 0: the fixed / adaptive time filter, advanced
+ * by dt, its state (seeded by the first filtered call) kept in the solver between calls. */
+int dopamine_op_wall_model(void *solver, const double *U, double nu, double h_wm, double t_filt, double dt,
+                           double tf_fac, double *tau_w_bnd);
 /* advection_rhs_vector (src/numerics/mod_advection.f90:679): rhs = the advective term (overwritten);
  * scheme VANLEER_TVD 1, KOREN_TVD 2, CDS 3, KEP 4, QUICK 5; face_flux(n_faces) the projected flux. */
 int dopamine_op_advection_rhs_vector(void *solver, int scheme, const double *U, const double *U_bnd,

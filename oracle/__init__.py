@@ -98,7 +98,7 @@ def lib():
         L.om_op_advection_scalar.argtypes = [vp, ci, _P(_d), _P(_d), _P(_d), _P(_d)]
         L.om_op_viscous_scalar.argtypes = [vp, _P(_d), _P(_d), _P(_d), _d, _d, _P(_d)]
         L.om_op_sgs.argtypes = [vp, ci, _d, ci, _d, _P(_d)]
-        L.om_op_wall_model.argtypes = [vp, _P(_d)]
+        L.om_op_wall_model.argtypes = [vp, _d, _P(_d)]
         L.om_outlet_dn_u.restype = _d
         L.om_outlet_dn_u.argtypes = [_d] * 6
         L.om_wall_sample.restype = ci
@@ -312,9 +312,9 @@ class Solver:
         lib().om_op_sgs(self.h, model, coef, int(wall_gradient), dt_stab, _p(out, _d))
         return out
 
-    def op_wall_model(self):
+    def op_wall_model(self, dt=0.0):
         out = np.zeros(self.array("tau_w_bnd").size)
-        lib().om_op_wall_model(self.h, _p(out, _d))
+        lib().om_op_wall_model(self.h, dt, _p(out, _d))
         return out
 
     def wall_sample(self, b):

@@ -762,8 +762,6 @@ static void wall_model_dt(om_solver *s, double dt_use) {
   if (do_filter) s->wm_filt_ready = 1;
 }
 
-static void wall_model(om_solver *s) { wall_model_dt(s, 0.0); }
-
 /* find_sample_cell (src/sgs/mod_wall_model.f90:126-173): from the wall face's owner, step to the
  * face neighbour farthest from the wall along the inward normal until the distance reaches h_wm
  * (at most 20 steps; interior faces only) */
@@ -2620,11 +2618,12 @@ void om_op_sgs(om_solver *s, int model, double coef, int wall_gradient, double d
   s->p = keep;
 }
 
-/* the log-law wall traction of every wall-model face from the solver's U */
-void om_op_wall_model(om_solver *s, double *out) {
+/* the log-law wall traction of every wall-model face from the solver's U; dt advances the
+ * solver's time filter (when configured) as the first stage of a step does */
+void om_op_wall_model(om_solver *s, double dt, double *out) {
   double *t = s->tau_w;
   s->tau_w = out;
-  wall_model(s);
+  wall_model_dt(s, dt);
   s->tau_w = t;
 }
 

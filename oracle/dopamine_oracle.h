@@ -138,7 +138,7 @@ void om_op_advection_scalar(om_solver *s, int scheme, const double *phi, const d
 void om_op_viscous_scalar(om_solver *s, const double *phi, const double *pb, const double *g, double alpha,
                           double prt, double *out);
 void om_op_sgs(om_solver *s, int model, double coef, int wall_gradient, double dt_stab, double *out);
-void om_op_wall_model(om_solver *s, double *out);
+void om_op_wall_model(om_solver *s, double dt, double *out);
 /* outlet_dn_u (src/bc/mod_bc.f90:775-786): n.grad u at an outlet face from the face and owner
  * values, the owner gradient along the offset (gdd) and along n - (n.d)d (gt), |d| and n.d */
 double om_outlet_dn_u(double u_b, double u_o, double gdd, double gt, double len_d, double nd);

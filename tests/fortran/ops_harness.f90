@@ -179,16 +179,21 @@ contains
     end select
   end function
 
-  function ft_wall_model(h, pu, nu, ptau) bind(C, name='ft_wall_model') result(rc)
+  !> apply_wall_model_bc with all its optional arguments (a negative value leaves one absent)
+  function ft_wall_model(h, pu, nu, h_wm, t_filt, dt, tf_fac, ptau) bind(C, name='ft_wall_model') result(rc)
     type(c_ptr), value :: h, pu, ptau
-    real(c_double), value :: nu
+    real(c_double), value :: nu, h_wm, t_filt, dt, tf_fac
     integer(c_int) :: rc
     type(Mesh) :: msh
     real(rp), pointer :: U(:,:), tau(:,:)
     call msh%attach(h)
     call c_f_pointer(pu, U, [3, msh%n_cells])
     call c_f_pointer(ptau, tau, [3, max(1, msh%n_bnd_faces)])
-    call apply_wall_model_bc(U, nu, msh, tau)
+    if (h_wm < 0.0_rp) then
+      call apply_wall_model_bc(U, nu, msh, tau)
+    else
+      call apply_wall_model_bc(U, nu, msh, tau, h_wm=h_wm, t_filt=t_filt, dt=dt, tf_fac=tf_fac)
+    end if
     rc = 0
   end function
 

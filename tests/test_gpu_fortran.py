@@ -47,7 +47,7 @@ def _ftest():
         "ft_advection_scalar": [_vp, ci, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "ft_viscous_scalar": [_vp, _vp, cd, _vp, cd, _vp, _vp, _vp],
         "ft_sgs": [_vp, ci, _vp, _vp, _vp, cd, cd, ci, cd, _vp],
-        "ft_wall_model": [_vp, _vp, cd, _vp],
+        "ft_wall_model": [_vp, _vp, cd, cd, cd, cd, cd, _vp],
         "ft_solver_run": [_vp, ci, cd, _vp, _vp, _P(cd)],
     }.items():
         getattr(L, name).argtypes = args
@@ -225,10 +225,37 @@ def test_fortran_wall_model():
     print("[gpu] fortran apply_wall_model_bc channel", flush=True)
     F = _ftest()
     tau = np.zeros(3 * s.nb)
-    assert F.ft_wall_model(s.h, _a(s.U), s.case.params.nu, _a(tau)) == 0
+    assert F.ft_wall_model(s.h, _a(s.U), s.case.params.nu, -1.0, 0.0, 0.0, 0.0, _a(tau)) == 0
     ref = s.o.op_wall_model()
     assert np.abs(ref).max() > 0.0
     assert rel_l2(ref, tau) < TOL
+
+
+@pytest.mark.parametrize("t_filt,tf_fac", [(0.02, 0.0), (0.0, 2.0)])
+def test_fortran_wall_model_sampling_height_and_filter(t_filt, tf_fac):
+    """apply_wall_model_bc with h_wm, t_filt / tf_fac and dt: the first call seeds the filter, the
+    second moves it once towards the new sampled velocity; both against the oracle's routine."""
+    from helpers import wallmodel_case
+    case = wallmodel_case(0.12, t_filt, tf_fac)
+    mesh = case.make_mesh()
+    o = oracle.Solver(mesh, case.params)
+    P.initialize(o, case, mesh)
+    o.advance()
+    g = FVSolver(mesh, case.params)
+    print(f"[gpu] fortran apply_wall_model_bc h_wm 0.12 T {t_filt} f {tf_fac}", flush=True)
+    F = _ftest()
+    nb = o.array("tau_w_bnd").size // 3
+    dt, nu = case.params.dt, case.params.nu
+    U1 = o.array("U").copy()
+    U2 = U1 * (1.0 + 0.1 * np.random.default_rng(9).standard_normal(U1.size))
+    f = oracle.Solver(mesh, case.params)  # a fresh filter (the stepped oracle's is seeded)
+    for U in (U1, U2):
+        tau = np.zeros(3 * nb)
+        assert F.ft_wall_model(_vp(g.h), _a(U), nu, 0.12, t_filt, dt, tf_fac, _a(tau)) == 0
+        f.array("U")[:] = U
+        ref = f.op_wall_model(dt)
+        assert np.abs(ref).max() > 0.0
+        assert rel_l2(ref, tau) < TOL
 
 
 def test_fortran_fvsolver_advance_matches_python_driver():
