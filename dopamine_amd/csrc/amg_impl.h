@@ -266,15 +266,37 @@ struct AmgCfg {
 // ---------------------------------------------------------------- device kernels
 
 // One level in the execution space. Level 0 keeps the operator's ELL lists (one thread per cell,
-// the solver's own neighbour array); coarser levels, whose Galerkin rows are short on average but
-// occasionally very long, are CSR rows worked by groups of AMG_G lanes.
-constexpr int AMG_G = 8;
+// the solver's own neighbour array).
+// Coarse levels store their rows, and every level the children lists of the next coarser level's
+// cells, as sliced ELL: slices of AMG_SL rows (one wavefront), each as wide as its longest row,
+// entry k of row r at sptr[r / AMG_SL] + k AMG_SL + r % AMG_SL. One lane per row then reads a
+// contiguous 256-byte line per slot with no reduction across lanes. Row padding points at the row
+// itself with zero coefficients; children padding is -1.
+constexpr int AMG_SL = 64;
+constexpr int AMG_UN = 4;  // slice widths are multiples of this: each lane keeps 4 independent loads in flight
+
+// Slot of every CSR entry in the sliced-ELL layout; sptr gets the slice offsets. Returns the size.
+inline size_t amg_sell_layout(int n, const std::vector<int>& rptr, std::vector<int>& sptr, std::vector<int>& pos) {
+  const int nsl = (n + AMG_SL - 1) / AMG_SL;
+  sptr.assign(size_t(nsl) + 1, 0);
+  for (int sl = 0; sl < nsl; ++sl) {
+    int w = 0;
+    for (int r = sl * AMG_SL; r < std::min(n, (sl + 1) * AMG_SL); ++r) w = std::max(w, rptr[r + 1] - rptr[r]);
+    w = (w + AMG_UN - 1) / AMG_UN * AMG_UN;
+    sptr[size_t(sl) + 1] = sptr[sl] + w * AMG_SL;
+  }
+  pos.assign(size_t(rptr[n]), 0);
+  for (int r = 0; r < n; ++r)
+    for (int q = rptr[r]; q < rptr[r + 1]; ++q) pos[q] = sptr[r / AMG_SL] + (q - rptr[r]) * AMG_SL + r % AMG_SL;
+  return size_t(sptr[nsl]);
+}
+
 struct AmgLevelDev {
   int n = 0, w = 0;                              // cells; ELL width (level 0 only)
   const int* nbr = nullptr;                      // level 0: ELL neighbours [slot][cell]
-  const int *rptr = nullptr, *col = nullptr;     // coarse levels: CSR rows
+  const int *sptr = nullptr, *col = nullptr;     // coarse levels: sliced-ELL rows
   const float *cs = nullptr, *cr = nullptr, *dinv = nullptr;
-  const int *cptr = nullptr, *cidx = nullptr;    // children (cells of this level) of the next coarser level's cells
+  const int *kptr = nullptr, *kidx = nullptr;    // children (cells of this level) of the next coarser level's cells, sliced ELL
   float *b = nullptr, *x = nullptr, *r = nullptr;
 };
 
@@ -435,26 +457,68 @@ struct Up0F {
   DM_HD void operator()(int c) const { z[c] = T(up_cell(L, r, omega, c)); }
 };
 
-// b_coarse[I] = sum of the fine residual over aggregate I's cells (a lane group per aggregate)
+// Row slot helpers of the sliced-ELL layout: the first slot of row r and the slice's width.
+DM_INLINE int sell_base(const int* sptr, int r) { return sptr[r / AMG_SL] + r % AMG_SL; }
+DM_INLINE int sell_width(const int* sptr, int r) { return (sptr[r / AMG_SL + 1] - sptr[r / AMG_SL]) / AMG_SL; }
+// sum_k a_rk v[col_rk] over this lane's chunks of AMG_UN slots of row r (chunks lane, lane + st,
+// ...), AMG_UN independent loads at a time
+DM_INLINE float sell_row(const int* sptr, const int* col, const float* a, const float* v, int r, int lane, int st) {
+  const int q0 = sell_base(sptr, r), nch = sell_width(sptr, r) / AMG_UN;
+  float s = 0.0f;
+  for (int ch = lane; ch < nch; ch += st) {
+    const int k = ch * AMG_UN;
+    int j[AMG_UN];
+    float h[AMG_UN], x[AMG_UN];
+#pragma unroll
+    for (int u = 0; u < AMG_UN; ++u) {
+      j[u] = col[q0 + (k + u) * AMG_SL];
+      h[u] = a[q0 + (k + u) * AMG_SL];
+    }
+#pragma unroll
+    for (int u = 0; u < AMG_UN; ++u) x[u] = v[j[u]];
+#pragma unroll
+    for (int u = 0; u < AMG_UN; ++u) s = s + h[u] * x[u];
+  }
+  return s;
+}
+
+// Lanes per row: one on the large levels (every wavefront reads whole 256-byte slot lines), four
+// on levels of fewer rows, where the serial slot loop would leave the level latency-bound.
+constexpr int AMG_SMALL_ROWS = 1 << 18;
+
+// b_coarse[I] = sum of the fine residual over aggregate I's cells (G lanes per aggregate)
+template <int G>
 struct Restrict {
-  const int *cptr, *cidx;
+  const int *kptr, *kidx;
   const float* rf;
   float* bc;
   const CgScalars* sc;
   int mode;
   DM_HD void operator()(int t) const {
     int lane, st;
-    if (!group_lane<AMG_G>(t, &lane, &st)) return;
+    if (!group_lane<G>(t, &lane, &st)) return;
     if (gated(sc, mode)) return;
-    const int I = t / AMG_G;
+    const int I = t / G;
+    const int q0 = sell_base(kptr, I), nch = sell_width(kptr, I) / AMG_UN;
     float s = 0.0f;
-    for (int q = cptr[I] + lane; q < cptr[I + 1]; q += st) s = s + rf[cidx[q]];
-    s = group_sum<AMG_G>(s);
+    for (int ch = lane; ch < nch; ch += st) {
+      const int k = ch * AMG_UN;
+      int i[AMG_UN];
+      float v[AMG_UN];
+#pragma unroll
+      for (int u = 0; u < AMG_UN; ++u) i[u] = kidx[q0 + (k + u) * AMG_SL];
+#pragma unroll
+      for (int u = 0; u < AMG_UN; ++u) v[u] = i[u] >= 0 ? rf[i[u]] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < AMG_UN; ++u) s = s + v[u];
+    }
+    s = group_sum<G>(s);
     if (lane == 0) bc[I] = s;
   }
 };
 
-// coarse-level pre-smoothing and residual (CSR, a lane group per row)
+// coarse-level pre-smoothing and residual (G lanes per row)
+template <int G>
 struct DownC {
   AmgLevelDev L;
   const CgScalars* sc;
@@ -462,12 +526,10 @@ struct DownC {
   float omega;
   DM_HD void operator()(int t) const {
     int lane, st;
-    if (!group_lane<AMG_G>(t, &lane, &st)) return;
+    if (!group_lane<G>(t, &lane, &st)) return;
     if (gated(sc, mode)) return;
-    const int c = t / AMG_G;
-    float s = 0.0f;
-    for (int q = L.rptr[c] + lane; q < L.rptr[c + 1]; q += st) s = s + L.cs[q] * L.b[L.col[q]];
-    s = group_sum<AMG_G>(s);
+    const int c = t / G;
+    const float s = group_sum<G>(sell_row(L.sptr, L.col, L.cs, L.b, c, lane, st));
     if (lane == 0) {
       const float bc = L.b[c];
       L.x[c] = omega * L.dinv[c] * bc;
@@ -476,28 +538,39 @@ struct DownC {
   }
 };
 
-// Coarse-level post-smoothing (CSR, a lane group per row), then the prolongation with
-// over-correction into the next finer level: x_fine[i] += alpha y for the cell's children i
-// (each fine cell has one parent, so the writes never collide).
+// Coarse-level post-smoothing (G lanes per row), then the prolongation with over-correction into
+// the next finer level: x_fine[i] += alpha y for the cell's children i (each fine cell has one
+// parent, so the writes never collide).
+template <int G>
 struct UpC {
   AmgLevelDev L;
-  const int *fptr, *fidx;  // this level's cells' children on the finer level
+  const int *kptr, *kidx;  // this level's cells' children on the finer level
   float* xf;
   const CgScalars* sc;
   int mode;
   float omega, alpha;
   DM_HD void operator()(int t) const {
     int lane, st;
-    if (!group_lane<AMG_G>(t, &lane, &st)) return;
+    if (!group_lane<G>(t, &lane, &st)) return;
     if (gated(sc, mode)) return;
-    const int c = t / AMG_G;
-    float s = 0.0f;
-    for (int q = L.rptr[c] + lane; q < L.rptr[c + 1]; q += st) s = s + L.cr[q] * L.x[L.col[q]];
-    s = group_sum<AMG_G>(s);
+    const int c = t / G;
+    const float s = group_sum<G>(sell_row(L.sptr, L.col, L.cr, L.x, c, lane, st));
     const float xc = L.x[c];
     const float y = xc + omega * (L.dinv[c] * L.b[c] - xc + s);
     const float a = alpha * y;
-    for (int q = fptr[c] + lane; q < fptr[c + 1]; q += st) xf[fidx[q]] = xf[fidx[q]] + a;
+    const int p0 = sell_base(kptr, c), nch = sell_width(kptr, c) / AMG_UN;
+    for (int ch = lane; ch < nch; ch += st) {
+      const int k = ch * AMG_UN;
+      int i[AMG_UN];
+      float v[AMG_UN];
+#pragma unroll
+      for (int u = 0; u < AMG_UN; ++u) i[u] = kidx[p0 + (k + u) * AMG_SL];
+#pragma unroll
+      for (int u = 0; u < AMG_UN; ++u) v[u] = i[u] >= 0 ? xf[i[u]] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < AMG_UN; ++u)
+        if (i[u] >= 0) xf[i[u]] = v[u] + a;
+    }
   }
 };
 
@@ -508,7 +581,7 @@ struct Coarsest {
   int n;
   const float* inv;
   const float* b;
-  const int *fptr, *fidx;
+  const int *kptr, *kidx;
   float* xf;
   const CgScalars* sc;
   int mode;
@@ -522,7 +595,11 @@ struct Coarsest {
     for (int j = lane; j < n; j += st) s = s + inv[size_t(i) * n + j] * b[j];
     s = group_sum<AMG_GC>(s);
     const float a = alpha * s;
-    for (int q = fptr[i] + lane; q < fptr[i + 1]; q += st) xf[fidx[q]] = xf[fidx[q]] + a;
+    const int p0 = sell_base(kptr, i), pw = sell_width(kptr, i);
+    for (int k = lane; k < pw; k += st) {
+      const int c = kidx[p0 + k * AMG_SL];
+      if (c >= 0) xf[c] = xf[c] + a;
+    }
   }
 };
 
@@ -625,7 +702,7 @@ class AmgPc : public FcgPc, public FcgPc32 {
           }
         L.cs = up(cs);
         L.cr = up(cr);
-      } else {  // CSR of the non-empty entries
+      } else {  // the non-empty entries, sliced ELL
         std::vector<int> rptr(size_t(cur.n) + 1, 0), col;
         std::vector<float> cs, cr;
         for (int c = 0; c < cur.n; ++c) {
@@ -638,11 +715,25 @@ class AmgPc : public FcgPc, public FcgPc32 {
           }
           rptr[size_t(c) + 1] = int(col.size());
         }
-        L.rptr = up(rptr);
-        L.col = up(col);
-        L.cs = up(cs);
-        L.cr = up(cr);
+        std::vector<int> sptr, pos;
+        const size_t m = amg_sell_layout(cur.n, rptr, sptr, pos);
+        std::vector<int> scol(std::max<size_t>(1, m), 0);
+        std::vector<float> scs(std::max<size_t>(1, m), 0.0f), scr(std::max<size_t>(1, m), 0.0f);
+        for (int c = 0; c < cur.n; ++c) {  // padding: the row itself, zero coefficients
+          const int sl = c / AMG_SL, w = (sptr[sl + 1] - sptr[sl]) / AMG_SL;
+          for (int k = 0; k < w; ++k) scol[size_t(sptr[sl]) + size_t(k) * AMG_SL + c % AMG_SL] = c;
+        }
+        for (size_t q = 0; q < col.size(); ++q) {
+          scol[pos[q]] = col[q];
+          scs[pos[q]] = cs[q];
+          scr[pos[q]] = cr[q];
+        }
+        L.sptr = up(sptr);
+        L.col = up(scol);
+        L.cs = up(scs);
+        L.cr = up(scr);
         nnz_.push_back(int64_t(col.size()));
+        stored_.push_back(int64_t(m));
       }
       L.dinv = up(di);
       L.b = lev == 0 ? nullptr : fresh<float>(cur.n);
@@ -673,8 +764,14 @@ class AmgPc : public FcgPc, public FcgPc32 {
       std::vector<int> ptr, idx;
       amg_children(agg, na, ptr, idx);
       AmgOp next = amg_galerkin(cur, agg, na, ptr, idx);
-      L.cptr = up(ptr);
-      L.cidx = up(idx);
+      {
+        std::vector<int> kptr, pos;
+        const size_t m = amg_sell_layout(na, ptr, kptr, pos);
+        std::vector<int> kidx(std::max<size_t>(1, m), -1);
+        for (size_t q = 0; q < idx.size(); ++q) kidx[pos[q]] = idx[q];
+        L.kptr = up(kptr);
+        L.kidx = up(kidx);
+      }
       levels_.push_back(L);
       cur = std::move(next);
     }
@@ -692,6 +789,7 @@ class AmgPc : public FcgPc, public FcgPc32 {
   int levels() const { return int(levels_.size()); }
   const std::vector<int>& sizes() const { return sizes_; }
   const std::vector<int64_t>& nnz() const { return nnz_; }
+  const std::vector<int64_t>& stored() const { return stored_; }  // sliced-ELL slots, padding included
   bool singular() const { return singular_; }
   double alpha() const { return alpha_; }
   bool aggressive() const { return aggressive_; }
@@ -750,18 +848,26 @@ class AmgPc : public FcgPc, public FcgPc32 {
     for (int l = 1; l < nl; ++l) {
       const AmgLevelDev& F = levels_[l - 1];
       const AmgLevelDev& C = levels_[l];
-      xx.for_each(C.n * amgk_g(), amgk::Restrict{F.cptr, F.cidx, F.r, C.b, sc, mode});
-      if (l < nl - 1) xx.for_each(C.n * amgk_g(), amgk::DownC{C, sc, mode, om});
+      if (C.n >= amgk::AMG_SMALL_ROWS) {
+        xx.for_each(C.n, amgk::Restrict<1>{F.kptr, F.kidx, F.r, C.b, sc, mode});
+        if (l < nl - 1) xx.for_each(C.n, amgk::DownC<1>{C, sc, mode, om});
+      } else {
+        xx.for_each(C.n * 4, amgk::Restrict<4>{F.kptr, F.kidx, F.r, C.b, sc, mode});
+        if (l < nl - 1) xx.for_each(C.n * 4, amgk::DownC<4>{C, sc, mode, om});
+      }
     }
     {
       const AmgLevelDev& F = levels_[nl - 2];
       xx.for_each(n_coarse_ * amgk::AMG_GC,
-                  amgk::Coarsest{n_coarse_, inv_, levels_[nl - 1].b, F.cptr, F.cidx, F.x, sc, mode, al});
+                  amgk::Coarsest{n_coarse_, inv_, levels_[nl - 1].b, F.kptr, F.kidx, F.x, sc, mode, al});
     }
     for (int l = nl - 2; l >= 1; --l) {
       const AmgLevelDev& C = levels_[l];
       const AmgLevelDev& F = levels_[l - 1];
-      xx.for_each(C.n * amgk_g(), amgk::UpC{C, F.cptr, F.cidx, F.x, sc, mode, om, al});
+      if (C.n >= amgk::AMG_SMALL_ROWS)
+        xx.for_each(C.n, amgk::UpC<1>{C, F.kptr, F.kidx, F.x, sc, mode, om, al});
+      else
+        xx.for_each(C.n * 4, amgk::UpC<4>{C, F.kptr, F.kidx, F.x, sc, mode, om, al});
     }
     if (sc)
       with_width(L0.w, [&](auto wc) {
@@ -773,13 +879,12 @@ class AmgPc : public FcgPc, public FcgPc32 {
     else
       xx.for_each(L0.n, amgk::Up0F<T>{L0, r, z, om});
   }
-  static constexpr int amgk_g() { return AMG_G; }
 
   X* x_ = nullptr;
   AmgCfg cfg_;
   std::vector<AmgLevelDev> levels_;
   std::vector<int> sizes_;
-  std::vector<int64_t> nnz_;
+  std::vector<int64_t> nnz_, stored_;
   std::vector<void*> owned_;
   float* inv_ = nullptr;
   double alpha_ = 1.5;

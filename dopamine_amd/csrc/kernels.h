@@ -282,6 +282,60 @@ struct LsGradVec {
   }
 };
 
+// compute_all_gradients with least squares (src/solver/mod_fv_solver.f90:1003-1077): the velocity
+// gradient (+ SGS viscosity, as LsGradVec) and NSF scalar gradients (temperature, passive scalars,
+// as LsGradSca) in one pass over the stencil, so its indices and weights are read once. Each field
+// accumulates in the same order as its own kernel: the results are those kernels' bit for bit.
+constexpr int LS_MAX_SCALARS = 5;
+template <int NSF>
+struct LsGradAll {
+  DevMesh M;
+  const double *U, *Ub;
+  double *grad, *nu_sgs;
+  int sgs;
+  double c_v, cs, c_amd, dt_stab;
+  const double* sa[LS_MAX_SCALARS];  // scalar fields, their boundary values, their gradients
+  const double* sb[LS_MAX_SCALARS];
+  double* sg[LS_MAX_SCALARS];
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    const double u0 = U[3 * size_t(c)], u1 = U[3 * size_t(c) + 1], u2 = U[3 * size_t(c) + 2];
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double pc[NSF > 0 ? NSF : 1], gs[NSF > 0 ? NSF : 1][3];
+#pragma unroll
+    for (int i = 0; i < NSF; ++i) {
+      pc[i] = sa[i][c];
+      gs[i][0] = gs[i][1] = gs[i][2] = 0.0;
+    }
+    const int cnt = M.nL[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int j = M.lidx[s];
+      const double* un = (j >= 0) ? &U[3 * size_t(j)] : &Ub[3 * size_t(-1 - j)];
+      const double du[3] = {un[0] - u0, un[1] - u1, un[2] - u2};
+      const double w0 = M.lw[(3 * size_t(k)) * nc + c], w1 = M.lw[(3 * size_t(k) + 1) * nc + c],
+                   w2 = M.lw[(3 * size_t(k) + 2) * nc + c];
+      for (int q = 0; q < 3; ++q) {
+        g[3 * q + 0] = g[3 * q + 0] + w0 * du[q];
+        g[3 * q + 1] = g[3 * q + 1] + w1 * du[q];
+        g[3 * q + 2] = g[3 * q + 2] + w2 * du[q];
+      }
+#pragma unroll
+      for (int i = 0; i < NSF; ++i) {
+        const double d = ((j >= 0) ? sa[i][j] : sb[i][-1 - j]) - pc[i];
+        gs[i][0] = gs[i][0] + w0 * d;
+        gs[i][1] = gs[i][1] + w1 * d;
+        gs[i][2] = gs[i][2] + w2 * d;
+      }
+    }
+    for (int q = 0; q < 9; ++q) grad[9 * size_t(c) + q] = g[q];
+    if (sgs >= 0) nu_sgs[c] = sgs_nu(sgs, g, &M.Gf[9 * size_t(c)], c_v, cs, c_amd, M.vol[c], dt_stab);
+#pragma unroll
+    for (int i = 0; i < NSF; ++i)
+      for (int q = 0; q < 3; ++q) sg[i][3 * size_t(c) + q] = gs[i][q];
+  }
+};
+
 // ls_gradient_scalar (src/numerics/mod_gradient.f90:307-422) of phi = a (+ a2) with boundary
 // values pb (+ pb2).
 struct LsGradSca {
@@ -1148,6 +1202,190 @@ struct ReconFaceScalar {
     }
     if (!isfinite(v)) v = 0.0;
     phiF[f] = v;
+  }
+};
+
+// The face values of NF scalar fields that share a reconstruction scheme (temperature and the
+// passive scalars) in one pass: the upwind context and far-upwind cell depend on the face flux
+// alone, so they are found once and each field is reconstructed as ReconFaceScalar does it.
+constexpr int RHS_MAX_FIELDS = 5;
+template <int SCHEME, int NF>
+struct ReconFaceScalarN {
+  ReconFaceScalar<SCHEME> base;  // field 0; the others swap in their arrays
+  const double* phi[RHS_MAX_FIELDS];
+  const double* pb[RHS_MAX_FIELDS];
+  const double* gphi[RHS_MAX_FIELDS];
+  double* phiF[RHS_MAX_FIELDS];
+  DM_HD void operator()(int f) const {
+    const DevMesh& M = base.R.M;
+    const int o = M.fown[f], nb = M.fnb[f];
+    double Fn;
+    int b = 0;
+    if (nb >= 0) {
+      Fn = base.R.UF[f];
+    } else {
+      b = f - M.nint;
+      Fn = is_solid(M.bbc[b]) ? 0.0 : base.R.UF[f];
+    }
+    ReconCtx c;
+    const bool ok = base.R.context(f, Fn, c);
+    const int far_up = (ok && SCHEME != SCHEME_QUICK) ? base.R.far_upwind(c) : -1;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      ReconFaceScalar<SCHEME> g = base;
+      g.phi = phi[i];
+      g.pb = pb[i];
+      g.gphi = gphi[i];
+      double v;
+      if (ok) {
+        v = (SCHEME == SCHEME_QUICK) ? g.quick1(c) : g.tvd1(c, far_up);
+        if (!isfinite(v)) v = (Fn >= 0.0 || nb < 0) ? phi[i][o] : phi[i][nb];
+      } else {
+        v = pb[i][b];
+      }
+      if (!isfinite(v)) v = 0.0;
+      phiF[i][f] = v;
+    }
+  }
+};
+
+// Scalar transport of NF fields that share a scheme, in one pass over the cell's faces: each field
+// as ExplicitRhsT computes it (same expressions, same order), the face geometry read once.
+template <int NF>
+struct ExplicitRhsTN {
+  DevMesh M;
+  const double *UF, *nus;
+  double dt, a, bcoef;
+  int cds;
+  const double *T[RHS_MAX_FIELDS], *Tb[RHS_MAX_FIELDS], *gT[RHS_MAX_FIELDS], *HT0[RHS_MAX_FIELDS],
+      *phiF[RHS_MAX_FIELDS], *qw[RHS_MAX_FIELDS];
+  double *HT[RHS_MAX_FIELDS], *Tnew[RHS_MAX_FIELDS];
+  double alpha[RHS_MAX_FIELDS], prt[RHS_MAX_FIELDS];
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    const double iv = M.iv[c];
+    double cv[NF], vs[NF];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) cv[i] = vs[i] = 0.0;
+    const int cnt = M.nA[c];
+    for (int k = 0; k < cnt; ++k) {
+      const size_t s = size_t(k) * nc + c;
+      const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
+      const double* sf = &M.sf[3 * size_t(f)];
+      if (oth >= 0) {
+        const int o = side ? oth : c, nb = side ? c : oth;
+        const double gc = M.gc[f];
+        double d_on[3], dnp[3];
+        for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
+        const double dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+        const double sdd = dot3(sf, d_on) / dl;
+        for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+        const double ow = ortho_weight(sdd, M.area[f]);
+        const double nuf = gc * nus[o] + (1.0 - gc) * nus[nb];
+        const double Fn0 = UF[f];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          const double* Tq = T[q];
+          const double* gq = gT[q];
+          double Fn = Fn0;
+          double phif;
+          if (cds) {
+            const double pl = gc * Tq[o] + (1.0 - gc) * Tq[nb];
+            double gm[3];
+            for (int i = 0; i < 3; ++i) gm[i] = gc * gq[3 * size_t(o) + i] + (1.0 - gc) * gq[3 * size_t(nb) + i];
+            phif = pl + dot3(gm, &M.eps[3 * size_t(f)]);
+            const double w = M.wbl[f];
+            if (w < 1.0) phif = pl + w * (phif - pl);
+            if (!isfinite(phif)) phif = (Fn >= 0.0) ? Tq[o] : Tq[nb];
+            if (!isfinite(Fn)) Fn = 0.0;
+            if (!isfinite(phif)) phif = 0.0;
+          } else {
+            if (!isfinite(Fn)) Fn = 0.0;
+            phif = phiF[q][f];
+          }
+          cv[q] = side ? cv[q] + Fn * phif * iv : cv[q] - Fn * phif * iv;
+          const double ae = alpha[q] + nuf / prt[q];
+          double gf[3];
+          for (int i = 0; i < 3; ++i) gf[i] = gc * gq[3 * size_t(o) + i] + (1.0 - gc) * gq[3 * size_t(nb) + i];
+          const double flux = ae * ((Tq[nb] - Tq[o]) * ow / dl + dot3(gf, dnp));
+          vs[q] = side ? vs[q] - flux * iv : vs[q] + flux * iv;
+        }
+      } else {
+        const int b = -1 - oth, bc = M.bbc[b];
+        const int nbp = (bc == BC_PERIODIC) ? M.pnb[b] : -1;
+        const double Fn0 = is_solid(bc) ? 0.0 : UF[f];
+        double d_per = 0.0, gcp = 0.0;
+        double d_on[3], dnp[3], dl;
+        if (nbp >= 0) {
+          if (M.pair[b] >= 0) {
+            const int pf = M.nint + M.pair[b];
+            d_per = M.dcf[f] + M.dcf[pf];
+            gcp = M.dcf[pf] / (d_per + SMALL);
+          } else if (M.pdcf[b] > SMALL) {
+            d_per = M.dcf[f] + M.pdcf[b];
+            gcp = M.pdcf[b] / (d_per + SMALL);
+          } else {
+            d_per = 2.0 * M.dcf[f];
+            gcp = 0.5;
+          }
+          for (int i = 0; i < 3; ++i) d_on[i] = d_per * M.nrm[3 * size_t(f) + i];
+          dl = d_per + SMALL;
+        } else {
+          for (int i = 0; i < 3; ++i) d_on[i] = M.fctr[3 * size_t(f) + i] - M.ctr[3 * size_t(c) + i];
+          dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+        }
+        const double sdd = dot3(sf, d_on) / dl;
+        for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+        const double ow = ortho_weight(sdd, M.area[f]);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          const double* Tq = T[q];
+          const double* gq = gT[q];
+          double Fn = Fn0;
+          double phif;
+          if (cds) {
+            if (nbp >= 0) {
+              const double g = M.pgc[b];
+              phif = g * Tq[c] + (1.0 - g) * Tq[nbp];
+              if (!isfinite(phif)) phif = Tq[c];
+            } else {
+              phif = Tb[q][b];
+            }
+            if (!isfinite(Fn)) Fn = 0.0;
+            if (!isfinite(phif)) phif = 0.0;
+          } else {
+            if (!isfinite(Fn)) Fn = 0.0;
+            phif = phiF[q][f];
+          }
+          cv[q] = cv[q] - Fn * phif * iv;
+          double flux;
+          if (nbp >= 0) {
+            const double ae = alpha[q] + (gcp * nus[c] + (1.0 - gcp) * nus[nbp]) / prt[q];
+            double gf[3];
+            for (int i = 0; i < 3; ++i) gf[i] = gcp * gq[3 * size_t(c) + i] + (1.0 - gcp) * gq[3 * size_t(nbp) + i];
+            flux = ae * ((Tq[nbp] - Tq[c]) * ow / dl + dot3(gf, dnp));
+          } else {
+            const double ae = alpha[q] + nus[c] / prt[q];
+            flux = ae * ((Tb[q][b] - Tq[c]) * ow / dl + dot3(&gq[3 * size_t(c)], dnp));
+          }
+          vs[q] = vs[q] + flux * iv;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      double h = cv[q] + vs[q];
+      if (qw[q])
+        for (int k = 0; k < cnt; ++k) {
+          const size_t s = size_t(k) * nc + c;
+          const int oth = M.aoth[s];
+          if (oth >= 0 || M.bbc[-1 - oth] != BC_WALL_MODEL) continue;
+          const int f = M.aface[s] >> 1;
+          h = h + qw[q][-1 - oth] * M.area[f] / M.vol[c];
+        }
+      HT[q][c] = h;
+      Tnew[q][c] = T[q][c] + dt * (a * h + bcoef * HT0[q][c]);
+    }
   }
 };
 

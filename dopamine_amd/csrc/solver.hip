@@ -295,12 +295,14 @@ int dopamine_solver_multigrid_info(void* h, int64_t* out, int max) {
     if (!s->amg) throw std::runtime_error("no multigrid hierarchy (precond is not multigrid)");
     const auto& sz = s->amg->sizes();
     const auto& nz = s->amg->nnz();
+    const auto& st = s->amg->stored();
     const int nl = int(sz.size());
-    if (max < 2 + 2 * nl) throw std::runtime_error("multigrid_info: output holds fewer than 2 + 2 levels entries");
+    if (max < 2 + 3 * nl) throw std::runtime_error("multigrid_info: output holds fewer than 2 + 3 levels entries");
     out[0] = nl;
     out[1] = s->amg->aggressive() ? 1 : 0;
     for (int l = 0; l < nl; ++l) out[2 + l] = sz[size_t(l)];
     for (int l = 0; l < nl; ++l) out[2 + nl + l] = l == 0 ? 0 : (size_t(l - 1) < nz.size() ? nz[size_t(l - 1)] : 0);
+    for (int l = 0; l < nl; ++l) out[2 + 2 * nl + l] = l == 0 ? 0 : (size_t(l - 1) < st.size() ? st[size_t(l - 1)] : 0);
     return 0;
   });
 }

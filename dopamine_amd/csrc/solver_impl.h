@@ -164,6 +164,7 @@ class SolverT {
   int ns = 0;
   double *Cs[4] = {}, *Cs2[4] = {}, *Cb[4] = {}, *gC[4] = {}, *HC[4] = {}, *HC0[4] = {};
   double *phiC = nullptr, *gCprev = nullptr;
+  double* phiCk[4] = {};  // each scalar's face values, for the fused transport of T and the scalars
   int* s_mode = nullptr;
   double *s_val = nullptr, *s_flux = nullptr, *s_om = nullptr, *s_ph = nullptr;
   // MOST wall model: heat flux and Obukhov length per boundary face, the configuration's L
@@ -710,7 +711,10 @@ class SolverT {
         gC[k] = fresh<double>(3 * size_t(nc)); HC[k] = fresh<double>(nc); HC0[k] = fresh<double>(nc);
       }
       gCprev = fresh<double>(3 * size_t(nc));
-      if (p.scalar_scheme != SCHEME_CDS) phiC = fresh<double>(nf);
+      if (p.scalar_scheme != SCHEME_CDS) {
+        for (int k = 0; k < ns; ++k) phiCk[k] = fresh<double>(nf);
+        phiC = phiCk[0];
+      }
       const size_t m = 4 * size_t(std::max(1, npatch));
       std::vector<int> hm(m, 3);
       std::vector<double> hv(m, 0.0), hf(m, 0.0), ho(m, 0.0), hp(m, 0.0);
@@ -983,6 +987,62 @@ class SolverT {
                                 (prm.wall_model == 2 && n_wm) ? q_w : nullptr});
   }
 
+  // Temperature and passive scalars that share a scheme run as one multi-field pass (face values,
+  // then H and the predictor); the results are those of the per-field kernels.
+  bool fused_scalars() const {
+    return (bous ? 1 : 0) + ns >= 2 && (!bous || ns == 0 || prm.t_scheme == prm.scalar_scheme);
+  }
+  void explicit_rhs_TC(double dt, int k) {
+    const double *fv[RHS_MAX_FIELDS] = {}, *fb[RHS_MAX_FIELDS] = {}, *fg[RHS_MAX_FIELDS] = {},
+                 *f0[RHS_MAX_FIELDS] = {}, *fq[RHS_MAX_FIELDS] = {};
+    double *fphi[RHS_MAX_FIELDS] = {}, *fh[RHS_MAX_FIELDS] = {}, *fn[RHS_MAX_FIELDS] = {};
+    double fa[RHS_MAX_FIELDS] = {}, fp[RHS_MAX_FIELDS] = {};
+    int n = 0;
+    if (bous) {
+      fv[n] = T; fb[n] = T_bnd; fg[n] = gT; f0[n] = HT0; fphi[n] = phiT; fh[n] = HT; fn[n] = T2;
+      fq[n] = (prm.wall_model == 2 && n_wm) ? q_w : nullptr;
+      fa[n] = prm.alpha_T; fp[n] = prm.Pr_t;
+      ++n;
+    }
+    for (int q = 0; q < ns; ++q, ++n) {
+      fv[n] = Cs[q]; fb[n] = Cb[q]; fg[n] = gC[q]; f0[n] = HC0[q]; fphi[n] = phiCk[q]; fh[n] = HC[q]; fn[n] = Cs2[q];
+      fq[n] = nullptr;
+      fa[n] = prm.scalar_diffusivity[q]; fp[n] = prm.scalar_pr_t[q];
+    }
+    const int scheme = bous ? prm.t_scheme : prm.scalar_scheme;
+    const auto run = [&](auto nfc) {
+      constexpr int NF = decltype(nfc)::value;
+      const auto recon = [&](auto sch) {
+        constexpr int S = decltype(sch)::value;
+        ReconFaceScalarN<S, NF> r{ReconFaceScalar<S>{ReconFace<S>{M, U, U_bnd, UF, grad, nullptr}, fv[0], fb[0], fg[0],
+                                                     fphi[0]},
+                                  {}, {}, {}, {}};
+        for (int i = 0; i < RHS_MAX_FIELDS; ++i) {
+          r.phi[i] = fv[i];
+          r.pb[i] = fb[i];
+          r.gphi[i] = fg[i];
+          r.phiF[i] = fphi[i];
+        }
+        x.for_each(nf, r);
+      };
+      if (scheme == SCHEME_QUICK) recon(std::integral_constant<int, SCHEME_QUICK>{});
+      if (scheme == SCHEME_VANLEER) recon(std::integral_constant<int, SCHEME_VANLEER>{});
+      if (scheme == SCHEME_KOREN) recon(std::integral_constant<int, SCHEME_KOREN>{});
+      ExplicitRhsTN<NF> e{M, UF, nu_sgs, dt, RK3_A[k], RK3_B[k], scheme == SCHEME_CDS ? 1 : 0};
+      for (int i = 0; i < RHS_MAX_FIELDS; ++i) {
+        e.T[i] = fv[i]; e.Tb[i] = fb[i]; e.gT[i] = fg[i]; e.HT0[i] = f0[i]; e.phiF[i] = fphi[i]; e.qw[i] = fq[i];
+        e.HT[i] = fh[i]; e.Tnew[i] = fn[i]; e.alpha[i] = fa[i]; e.prt[i] = fp[i];
+      }
+      x.for_each(nc, e);
+    };
+    switch (n) {
+      case 2: run(std::integral_constant<int, 2>{}); break;
+      case 3: run(std::integral_constant<int, 3>{}); break;
+      case 4: run(std::integral_constant<int, 4>{}); break;
+      default: run(std::integral_constant<int, 5>{}); break;
+    }
+  }
+
   // H_C and the predictor of each passive scalar into Cs2 (compute_explicit_rhs, :1719-1743)
   void explicit_rhs_C(double dt, int k) {
     const ReconFace<SCHEME_QUICK> rq{M, U, U_bnd, UF, grad, nullptr};
@@ -1003,6 +1063,45 @@ class SolverT {
 
   void gradients(bool with_sgs, double sgs_dt) {
     const int sgs = with_sgs ? prm.sgs : -1;
+    if (prm.gradient == 0) {  // least squares: every field in one pass (LsGradAll)
+      const double* sa[LS_MAX_SCALARS] = {};
+      const double* sb[LS_MAX_SCALARS] = {};
+      double* sg[LS_MAX_SCALARS] = {};
+      int nsf = 0;
+      if (bous) {
+        sa[nsf] = T;
+        sb[nsf] = T_bnd;
+        sg[nsf++] = gT;
+      }
+      for (int k = 0; k < ns; ++k) {
+        sa[nsf] = Cs[k];
+        sb[nsf] = Cb[k];
+        sg[nsf++] = gC[k];
+      }
+      x.tbegin(K_LS_GRAD);
+      const auto run = [&](auto n) {
+        constexpr int N = decltype(n)::value;
+        LsGradAll<N> f{M, U, U_bnd, grad, nu_sgs, sgs, prm.c_vreman, prm.cs_smag, prm.c_amd, sgs_dt, {}, {}, {}};
+        for (int i = 0; i < LS_MAX_SCALARS; ++i) {
+          f.sa[i] = sa[i];
+          f.sb[i] = sb[i];
+          f.sg[i] = sg[i];
+        }
+        x.for_each(nc, f);
+      };
+      switch (nsf) {
+        case 0: run(std::integral_constant<int, 0>{}); break;
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, 2>{}); break;
+        case 3: run(std::integral_constant<int, 3>{}); break;
+        case 4: run(std::integral_constant<int, 4>{}); break;
+        default: run(std::integral_constant<int, 5>{}); break;
+      }
+      x.tend(K_LS_GRAD);
+      if (with_sgs && prm.sgs == 3 && prm.amd_wall_gradient && n_amdw > 0)
+        x.for_each(n_amdw, AmdWall{M, amdw_cell, amdw_off, amdw_face, U, grad, nu_sgs, prm.nu, prm.c_amd, sgs_dt});
+      return;
+    }
     if (bous) {  // compute_all_gradients: the temperature gradient alongside the velocity's
       if (prm.gradient == 0) {
         x.for_each(nc, LsGradSca{M, T, nullptr, T_bnd, nullptr, gT});
@@ -1334,8 +1433,12 @@ class SolverT {
         default: explicit_rhs<SCHEME_KOREN>(dt, k); break;
       }
       x.tend(K_EXPLICIT);
-      if (bous) explicit_rhs_T(dt, k);
-      if (ns) explicit_rhs_C(dt, k);
+      if (fused_scalars()) {
+        explicit_rhs_TC(dt, k);
+      } else {
+        if (bous) explicit_rhs_T(dt, k);
+        if (ns) explicit_rhs_C(dt, k);
+      }
       outlet_advance(dt, RK3_A[k], RK3_B[k]);  // from the stage-start velocity, before the predictor lands
       std::swap(U, U2);
       if (bous) std::swap(T, T2);

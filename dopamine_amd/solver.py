@@ -319,7 +319,8 @@ class FVSolver:
             return None
         nl = int(out[0])
         return {"levels": nl, "aggressive": bool(out[1]), "cells": [int(out[2 + i]) for i in range(nl)],
-                "nnz": [int(out[2 + nl + i]) for i in range(nl)]}
+                "nnz": [int(out[2 + nl + i]) for i in range(nl)],
+                "stored": [int(out[2 + 2 * nl + i]) for i in range(nl)]}
 
     def learned_loss(self, max_steps=100000):
         out = np.zeros(max_steps)

@@ -189,9 +189,10 @@ int dopamine_solver_learned_info(void *solver, double out[8]);
 /* Exact cell projection counters: out = {projections, flexible-PCG iterations} so far. */
 int dopamine_solver_projection_info(void *solver, int64_t out[2]);
 /* Multigrid hierarchy of the pressure solve (precond 2): out[0] levels L, out[1] 1 when the first
- * coarsening was aggressive, out[2..2+L) cells per level, out[2+L..2+2L) stored off-diagonal entries
- * per level (0 for level 0, which uses the solver's ELL lists); max >= 2 + 2L. No reference
- * counterpart (BoomerAMG's hierarchy lives inside PETSc). */
+ * coarsening was aggressive, out[2..2+L) cells per level, out[2+L..2+2L) off-diagonal entries per
+ * level, out[2+2L..2+3L) their sliced-ELL slots with padding (both 0 for level 0, which uses the
+ * solver's ELL lists); max >= 2 + 3L. No reference counterpart (BoomerAMG's hierarchy lives inside
+ * PETSc). */
 int dopamine_solver_multigrid_info(void *solver, int64_t *out, int max);
 /* Mean training loss of each Adam step (up to max values); returns the count. */
 int dopamine_solver_learned_loss(void *solver, double *out, int max);
