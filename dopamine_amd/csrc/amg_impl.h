@@ -298,6 +298,11 @@ struct AmgLevelDev {
   const float *cs = nullptr, *cr = nullptr, *dinv = nullptr;
   const int *kptr = nullptr, *kidx = nullptr;    // children (cells of this level) of the next coarser level's cells, sliced ELL
   float *b = nullptr, *x = nullptr, *r = nullptr;
+  // level 0 with three or more levels: its cells' aggregates and level 1's post-smoothed values,
+  // prolongated by a streaming pass (Prolong0)
+  const int* agg = nullptr;
+  const float* yc = nullptr;
+  float* y = nullptr;  // level 1: the post-smoothed values the finest level reads
 };
 
 namespace amgk {
@@ -377,7 +382,8 @@ DM_INLINE void down_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
 // post-smoothing, level 0 (ELL), x already holding the prolongated coarse correction:
 // y = x + w (D^-1 b - x + sum_j cr_ij x_j)
 template <class T>
-DM_INLINE float up_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
+DM_INLINE float up_cell(const AmgLevelDev& L, const T* b, float omega, float al, int c) {
+  (void)al;
   const int n = L.n;
   const float xc = L.x[c];
   float s = 0.0f;
@@ -389,6 +395,20 @@ DM_INLINE float up_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
   }
   return xc + omega * (L.dinv[c] * float(b[c]) - xc + s);
 }
+
+// the prolongation of level 1's post-smoothed values into level 0 as a streaming pass over the
+// fine cells: x += alpha y_c[agg] (coalesced reads of x and agg, y_c resident in L2), instead of
+// each coarse cell scattering into its children
+struct Prolong0 {
+  AmgLevelDev L;
+  const CgScalars* sc;
+  int mode;
+  float al;
+  DM_HD void operator()(int c) const {
+    if (gated(sc, mode)) return;
+    L.x[c] = L.x[c] + al * L.yc[L.agg[c]];
+  }
+};
 
 // Level-0 slot loads of the batched (branch-free) kernels: WC slots of cell c, an empty slot
 // pointing at the cell itself with a zero coefficient.
@@ -453,8 +473,8 @@ struct Up0F {
   AmgLevelDev L;
   const T* r;
   T* z;
-  float omega;
-  DM_HD void operator()(int c) const { z[c] = T(up_cell(L, r, omega, c)); }
+  float omega, al;
+  DM_HD void operator()(int c) const { z[c] = T(up_cell(L, r, omega, al, c)); }
 };
 
 // Row slot helpers of the sliced-ELL layout: the first slot of row r and the slice's width.
@@ -549,6 +569,7 @@ struct UpC {
   const CgScalars* sc;
   int mode;
   float omega, alpha;
+  float* yout = nullptr;  // level 1 under a level 0 that prolongates itself: store y, no scatter
   DM_HD void operator()(int t) const {
     int lane, st;
     if (!group_lane<G>(t, &lane, &st)) return;
@@ -557,6 +578,10 @@ struct UpC {
     const float s = group_sum<G>(sell_row(L.sptr, L.col, L.cr, L.x, c, lane, st));
     const float xc = L.x[c];
     const float y = xc + omega * (L.dinv[c] * L.b[c] - xc + s);
+    if (yout) {
+      if (lane == 0) yout[c] = y;
+      return;
+    }
     const float a = alpha * y;
     const int p0 = sell_base(kptr, c), nch = sell_width(kptr, c) / AMG_UN;
     for (int ch = lane; ch < nch; ch += st) {
@@ -614,11 +639,11 @@ struct Up0 {
   int zs;
   const CgScalars* sc;
   int mode;
-  float omega;
+  float omega, al;
   using Scal = cgk::NoScal;
   DM_HD bool setup(const Red<1>&, bool, Scal&) const { return !gated(sc, mode); }
   DM_HD Red<5> operator()(int c, const Scal&) const {
-    const float y = up_cell(L, r, omega, c);
+    const float y = up_cell(L, r, omega, al, c);
     const T zc = T(y);
     z[size_t(c) * zs] = zc;
     const double rc = double(r[c]), ro = double(rold[c]), zd = double(zc);
@@ -670,6 +695,7 @@ class AmgPc : public FcgPc, public FcgPc32 {
     cfg_ = cfg;
     alpha_ = cfg.alpha;
     aggressive_ = false;
+    const int* agg0 = nullptr;
     AmgOp cur = A;
     for (int lev = 0;; ++lev) {
       AmgLevelDev L;
@@ -761,6 +787,7 @@ class AmgPc : public FcgPc, public FcgPc32 {
       }
       if (na * 3 > cur.n * 2 && cur.n > 16)  // stalled coarsening: every connection counts as strong
         na = amg_aggregate(cur, 0.0, agg);
+      if (lev == 0) agg0 = up(agg);
       std::vector<int> ptr, idx;
       amg_children(agg, na, ptr, idx);
       AmgOp next = amg_galerkin(cur, agg, na, ptr, idx);
@@ -774,6 +801,11 @@ class AmgPc : public FcgPc, public FcgPc32 {
       }
       levels_.push_back(L);
       cur = std::move(next);
+    }
+    if (levels_.size() >= 3) {  // the finest post-smoothing adds level 1's correction itself
+      levels_[1].y = fresh<float>(size_t(levels_[1].n));
+      levels_[0].agg = agg0;
+      levels_[0].yc = levels_[1].y;
     }
     zpart_ = fresh<double>(size_t(x_->pass_grid(A.n)));
   }
@@ -864,20 +896,22 @@ class AmgPc : public FcgPc, public FcgPc32 {
     for (int l = nl - 2; l >= 1; --l) {
       const AmgLevelDev& C = levels_[l];
       const AmgLevelDev& F = levels_[l - 1];
+      float* yout = (l == 1 && L0.agg) ? C.y : nullptr;
       if (C.n >= amgk::AMG_SMALL_ROWS)
-        xx.for_each(C.n, amgk::UpC<1>{C, F.kptr, F.kidx, F.x, sc, mode, om, al});
+        xx.for_each(C.n, amgk::UpC<1>{C, F.kptr, F.kidx, F.x, sc, mode, om, al, yout});
       else
-        xx.for_each(C.n * 4, amgk::UpC<4>{C, F.kptr, F.kidx, F.x, sc, mode, om, al});
+        xx.for_each(C.n * 4, amgk::UpC<4>{C, F.kptr, F.kidx, F.x, sc, mode, om, al, yout});
     }
+    if (L0.agg) xx.for_each(L0.n, amgk::Prolong0{L0, sc, mode, al});
     if (sc)
       with_width(L0.w, [&](auto wc) {
         xx.tbegin(K_MG_UP);
-        xx.template pass<1, 5>(L0.n, amgk::Up0<T, decltype(wc)::value>{L0, r, rold, z, zs, sc, mode, om}, nullptr,
+        xx.template pass<1, 5>(L0.n, amgk::Up0<T, decltype(wc)::value>{L0, r, rold, z, zs, sc, mode, om, al}, nullptr,
                                0, out_part);
         xx.tend(K_MG_UP);
       });
     else
-      xx.for_each(L0.n, amgk::Up0F<T>{L0, r, z, om});
+      xx.for_each(L0.n, amgk::Up0F<T>{L0, r, z, om, al});
   }
 
   X* x_ = nullptr;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -99,8 +100,8 @@ __global__ void __launch_bounds__(BLOCK) k_final(int g, Fin fin, const double* p
 // Chained reduction passes (the CG's iterations): a pass first sums the previous pass's block
 // partials -- every block redundantly, in the same order, so all blocks hold bit-identical totals
 // without any cross-block synchronisation -- lets the functor derive its scalars (and stop the
-// whole grid uniformly), then streams a contiguous chunk of cells per block and stores its own
-// block partials for the next pass.
+// whole grid uniformly), then streams its cells (a contiguous chunk, or tiles interleaved with
+// the other blocks of its XCD) and stores its own block partials for the next pass.
 #ifndef DM_PBLOCK
 #define DM_PBLOCK 256
 #endif
@@ -124,7 +125,7 @@ struct PassSlots<F, std::void_t<decltype(F::kSlots)>> {
 
 template <int NIN, int NOUT, class F>
 __global__ void __launch_bounds__(PBLOCK) k_pass(int n, int chunk, F f, const double* in_part, int in_g,
-                                                 double* out_part) {
+                                                 double* out_part, int tile) {
   Red<NIN> tot = red_identity<NIN, false>();
   for (int q = threadIdx.x; q < in_g; q += PBLOCK)
 #pragma unroll
@@ -135,7 +136,28 @@ __global__ void __launch_bounds__(PBLOCK) k_pass(int n, int chunk, F f, const do
   if (!f.setup(tot, b == 0 && threadIdx.x == 0, sc)) return;  // uniform across the grid
   Red<NOUT> acc = red_identity<NOUT, false>();
   const int lo = b * chunk, hi = min(n, lo + chunk);
-  if constexpr (PassSlots<F>::value > 0) {
+  const int per = gridDim.x >> 3;
+  if (tile > 0 && per > 0 && b < (per << 3)) {
+    // the XCD's blocks sweep its run of cells together, tile by tile (block j takes tiles j,
+    // j + per, ...), so the neighbours' records are still in that XCD's L2 when they are read
+    const int x0 = (b / per) * per * chunk, x1 = min(n, x0 + per * chunk);
+    for (int t0 = x0 + (b % per) * tile; t0 < x1; t0 += per * tile) {
+      const int t1 = min(x1, t0 + tile);
+      Red<NOUT> v;
+      if constexpr (PassSlots<F>::value > 0) {
+        v = f.run(t0, t1, int(threadIdx.x), PBLOCK, sc);
+      } else {
+        v = red_identity<NOUT, false>();
+        for (int c = t0 + int(threadIdx.x); c < t1; c += PBLOCK) {
+          const Red<NOUT> w = f(c, sc);
+#pragma unroll
+          for (int k = 0; k < NOUT; ++k) v.v[k] = v.v[k] + w.v[k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
+    }
+  } else if constexpr (PassSlots<F>::value > 0) {
     acc = f.run(lo, hi, int(threadIdx.x), PBLOCK, sc);
   } else {
     for (int c = lo + int(threadIdx.x); c < hi; c += PBLOCK) {
@@ -211,9 +233,13 @@ struct DeviceExec {
   int part_blocks = 0;
   KernelTimer* kt = nullptr;
   int open_ = -1;            // class of the launch being timed
+  // chained passes: each XCD's blocks sweep its cells together in tiles of this many (k_pass);
+  // 0: one contiguous chunk per block. DOPAMINE_PASS_TILE overrides (for tuning).
+  int pass_tile = 256;
 
   void init(int max_n) {
     DM_CHECK(hipStreamCreate(&stream));  // default priority
+    if (const char* e = std::getenv("DOPAMINE_PASS_TILE")) pass_tile = std::max(0, std::atoi(e)) / 64 * 64;
     part_blocks = nblocks(max_n > 0 ? max_n : 1);
     DM_CHECK(hipMalloc(&part, sizeof(double) * 8 * size_t(part_blocks)));
     kt = new KernelTimer();
@@ -326,7 +352,7 @@ struct DeviceExec {
   void pass(int n, const F& f, const double* in_part, int in_g, double* out_part) {
     int g, c;
     pass_shape(n, &g, &c);
-    launch(k_pass<NIN, NOUT, F>, g, PBLOCK, n, c, f, in_part, in_g, out_part);
+    launch(k_pass<NIN, NOUT, F>, g, PBLOCK, n, c, f, in_part, in_g, out_part, pass_tile);
   }
   void sync() {
     DM_CHECK(hipStreamSynchronize(stream));
