@@ -465,6 +465,106 @@ struct SgsCell {
   }
 };
 
+// ------------------------------------------------------------------ dynamic Smagorinsky
+// dynamic_smagorinsky_sgs (src/sgs/mod_dynamic_smagorinsky.f90:39-186): Germano-Lilly with a
+// compact top-hat test filter, Cs^2 = <L_ij M_ij> / <M_ij M_ij> clipped at 0, ALPHA_SQ = 4.
+
+// u, the six u_i u_j and the six |S| S_ij of a cell, 15 values per cell
+struct DynPack {
+  const double *U, *grad;
+  double* fb;
+  DM_HD void operator()(int c) const {
+    const double* u = &U[3 * size_t(c)];
+    const double* g = &grad[9 * size_t(c)];
+    double* f = &fb[15 * size_t(c)];
+    const double S11 = g[0], S22 = g[4], S33 = g[8];
+    const double S12 = 0.5 * (g[3] + g[1]), S13 = 0.5 * (g[6] + g[2]), S23 = 0.5 * (g[7] + g[5]);
+    const double Sn = sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
+    f[0] = u[0]; f[1] = u[1]; f[2] = u[2];
+    f[3] = u[0] * u[0]; f[4] = u[0] * u[1]; f[5] = u[0] * u[2];
+    f[6] = u[1] * u[1]; f[7] = u[1] * u[2]; f[8] = u[2] * u[2];
+    f[9] = Sn * S11; f[10] = Sn * S12; f[11] = Sn * S13; f[12] = Sn * S22; f[13] = Sn * S23; f[14] = Sn * S33;
+  }
+};
+
+// test_filter (:238-304): the volume-weighted mean of a cell and its face neighbours (periodic
+// faces through their partner cell) over NC packed values; u3 (optional) gets values 0..2 as a
+// 3-per-cell vector
+template <int NC>
+struct DynFilter {
+  DevMesh M;
+  const double* in;
+  double *out, *u3;
+  DM_HD void operator()(int c) const {
+    const int nc = M.nc;
+    double acc[NC];
+    double wsum = M.vol[c];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) acc[k] = wsum * in[NC * size_t(c) + k];
+    const int cnt = M.nA[c];
+    for (int q = 0; q < cnt; ++q) {
+      const int oth = M.aoth[size_t(q) * nc + c];
+      const int nb = oth >= 0 ? oth : M.pnb[-1 - oth];
+      if (nb < 0 || nb == c) continue;
+      const double w = M.vol[nb];
+      wsum = wsum + w;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) acc[k] = acc[k] + w * in[NC * size_t(nb) + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) out[NC * size_t(c) + k] = acc[k] / fmax(wsum, SMALL);
+    if (u3)
+      for (int k = 0; k < 3; ++k) u3[3 * size_t(c) + k] = out[NC * size_t(c) + k];
+  }
+};
+
+// germano_lilly_terms (:203-236): L_ij M_ij and M_ij M_ij of a cell from the filtered values, the
+// gradient of the filtered velocity and Delta = det(G)^(1/6)
+struct DynLM {
+  DevMesh M;
+  const double *fh, *gh;
+  double* lm;
+  DM_HD void operator()(int c) const {
+    const double* h = &fh[15 * size_t(c)];
+    const double* g = &gh[9 * size_t(c)];
+    double Sh[9], ss = 0.0;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) Sh[3 * j + i] = 0.5 * (g[3 * j + i] + g[3 * i + j]);
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) ss = ss + Sh[3 * j + i] * Sh[3 * j + i];
+    const double Shn = sqrt(2.0 * ss);
+    const double L11 = h[3] - h[0] * h[0], L12 = h[4] - h[0] * h[1], L13 = h[5] - h[0] * h[2];
+    const double L22 = h[6] - h[1] * h[1], L23 = h[7] - h[1] * h[2], L33 = h[8] - h[2] * h[2];
+    const double d = fmax(pow(fmax(det3_colmajor(&M.Gf[9 * size_t(c)]), 0.0), 1.0 / 6.0), SMALL);
+    const double d2 = d * d;
+    const double M11 = d2 * (2.0 * (h[9] - 4.0 * Shn * Sh[0])), M12 = d2 * (2.0 * (h[10] - 4.0 * Shn * Sh[3]));
+    const double M13 = d2 * (2.0 * (h[11] - 4.0 * Shn * Sh[6])), M22 = d2 * (2.0 * (h[12] - 4.0 * Shn * Sh[4]));
+    const double M23 = d2 * (2.0 * (h[13] - 4.0 * Shn * Sh[7])), M33 = d2 * (2.0 * (h[14] - 4.0 * Shn * Sh[8]));
+    lm[2 * size_t(c)] = L11 * M11 + L22 * M22 + L33 * M33 + 2.0 * (L12 * M12 + L13 * M13 + L23 * M23);
+    lm[2 * size_t(c) + 1] = M11 * M11 + M22 * M22 + M33 * M33 + 2.0 * (M12 * M12 + M13 * M13 + M23 * M23);
+  }
+};
+
+// nu_t = max(0, <LM> / <MM>) Delta^2 |S|, capped by 0.4 Delta^2 / dt_stab
+struct DynNu {
+  DevMesh M;
+  const double *lmh, *grad;
+  double* nu_sgs;
+  double dt_stab;
+  DM_HD void operator()(int c) const {
+    const double* g = &grad[9 * size_t(c)];
+    double ss = 0.0;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) {
+        const double sij = 0.5 * (g[3 * j + i] + g[3 * i + j]);
+        ss = ss + sij * sij;
+      }
+    const double d = fmax(pow(fmax(det3_colmajor(&M.Gf[9 * size_t(c)]), 0.0), 1.0 / 6.0), SMALL);
+    const double nu = fmax(lmh[2 * size_t(c)] / fmax(lmh[2 * size_t(c) + 1], SMALL), 0.0) * (d * d) * sqrt(2.0 * ss);
+    nu_sgs[c] = fmin(nu, 0.4 * (d * d) / fmax(dt_stab, SMALL));
+  }
+};
+
 // ------------------------------------------------------------------ wall model
 
 // Reichardt law and Newton solve for u_tau (src/sgs/mod_wall_model.f90:50-115).

@@ -498,6 +498,29 @@ int dopamine_op_sgs_viscosity(void* h, int model, const double* U, const double*
   });
 }
 
+int dopamine_op_dynamic_smagorinsky(void* h, const double* U, const double* grad_U, const double* U_bnd,
+                                    double dt_stab, double* nu_sgs) {
+  return guard([&] {
+    Solver* s = solver_of(h);
+    if (!s->M.Gf) throw std::runtime_error("dynamic Smagorinsky needs the filter-width tensor: create the solver with an SGS model");
+    const Sizes z = sizes(s);
+    OpBufs b(s->x);
+    const double* u = b.in(U, 3 * z.nc);
+    const double* g = b.in(grad_U, 9 * z.nc);
+    const double* ub = b.in(U_bnd, 3 * z.nb);
+    double *fb = b.zeros(15 * z.nc), *fh = b.zeros(15 * z.nc), *uh = b.zeros(3 * z.nc), *gh = b.zeros(9 * z.nc);
+    double *lm = b.zeros(2 * z.nc), *lmh = b.zeros(2 * z.nc), *nus = b.zeros(z.nc);
+    s->x.for_each(s->nc, dm::DynPack{u, g, fb});
+    s->x.for_each(s->nc, dm::DynFilter<15>{s->M, fb, fh, uh});
+    s->x.for_each(s->nc, dm::GgGradVec{s->M, uh, ub, nullptr, gh});
+    s->x.for_each(s->nc, dm::DynLM{s->M, fh, gh, lm});
+    s->x.for_each(s->nc, dm::DynFilter<2>{s->M, lm, lmh, nullptr});
+    s->x.for_each(s->nc, dm::DynNu{s->M, lmh, g, nus, dt_stab});
+    b.out(nu_sgs, nus, z.nc);
+    return 0;
+  });
+}
+
 int dopamine_op_wall_model(void* h, const double* U, double nu, double h_wm, double t_filt, double dt,
                            double tf_fac, double* tau_w_bnd) {
   return guard([&] {

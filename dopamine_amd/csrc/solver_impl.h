@@ -178,6 +178,10 @@ class SolverT {
   // 2 adaptive), whether it is seeded
   double* wm_filt = nullptr;
   int wm_fmode = 0, wm_filt_ready = 0;
+  // dynamic Smagorinsky work arrays: packed and filtered values, filtered velocity and its
+  // gradient, L.M / M.M before and after the second filter
+  double *dyn_fb = nullptr, *dyn_fh = nullptr, *dyn_uh = nullptr, *dyn_gh = nullptr, *dyn_lm = nullptr,
+         *dyn_lmh = nullptr;
   int n_amdw = 0;
   int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
@@ -599,6 +603,14 @@ class SolverT {
     out_sp = up(std::vector<double>(2 * size_t(std::max(1, npatch)), 0.0));
     out_gprev = up(std::vector<double>(3 * size_t(std::max(1, nb)), 0.0));
     wm_filt = up(std::vector<double>(3 * size_t(std::max(1, nb)), 0.0));
+    if (p.sgs == 4) {
+      dyn_fb = fresh<double>(15 * size_t(nc));
+      dyn_fh = fresh<double>(15 * size_t(nc));
+      dyn_uh = fresh<double>(3 * size_t(nc));
+      dyn_gh = fresh<double>(9 * size_t(nc));
+      dyn_lm = fresh<double>(2 * size_t(nc));
+      dyn_lmh = fresh<double>(2 * size_t(nc));
+    }
     M.pcoef = up(pcoef.empty() ? std::vector<double>(1, 0.0) : pcoef);
     M.rcgc = up(rcgc); M.xgc = up(xgc); M.kint = up(kint);
     M.pown = up(pown.empty() ? std::vector<int>(1, -1) : pown);
@@ -1061,8 +1073,19 @@ class SolverT {
     }
   }
 
+  // dynamic Smagorinsky (sgs 4) on the current U, grad and U_bnd
+  void dynamic_sgs(double dt_stab) {
+    x.for_each(nc, DynPack{U, grad, dyn_fb});
+    x.for_each(nc, DynFilter<15>{M, dyn_fb, dyn_fh, dyn_uh});
+    x.for_each(nc, GgGradVec{M, dyn_uh, U_bnd, nullptr, dyn_gh});
+    x.for_each(nc, DynLM{M, dyn_fh, dyn_gh, dyn_lm});
+    x.for_each(nc, DynFilter<2>{M, dyn_lm, dyn_lmh, nullptr});
+    x.for_each(nc, DynNu{M, dyn_lmh, grad, nu_sgs, dt_stab});
+  }
+
   void gradients(bool with_sgs, double sgs_dt) {
-    const int sgs = with_sgs ? prm.sgs : -1;
+    const bool dyn = with_sgs && prm.sgs == 4;
+    const int sgs = (with_sgs && !dyn) ? prm.sgs : -1;
     if (prm.gradient == 0) {  // least squares: every field in one pass (LsGradAll)
       const double* sa[LS_MAX_SCALARS] = {};
       const double* sb[LS_MAX_SCALARS] = {};
@@ -1100,6 +1123,7 @@ class SolverT {
       x.tend(K_LS_GRAD);
       if (with_sgs && prm.sgs == 3 && prm.amd_wall_gradient && n_amdw > 0)
         x.for_each(n_amdw, AmdWall{M, amdw_cell, amdw_off, amdw_face, U, grad, nu_sgs, prm.nu, prm.c_amd, sgs_dt});
+      if (dyn) dynamic_sgs(sgs_dt);
       return;
     }
     if (bous) {  // compute_all_gradients: the temperature gradient alongside the velocity's
@@ -1134,7 +1158,9 @@ class SolverT {
         x.copy(gprev, grad, 9 * size_t(nc));
         x.for_each(nc, GgGradVec{M, U, U_bnd, gprev, grad});
       }
-      if (with_sgs)
+      if (dyn)
+        dynamic_sgs(sgs_dt);
+      else if (with_sgs)
         x.for_each(nc, SgsCell{M, grad, nu_sgs, prm.sgs, prm.c_vreman, prm.cs_smag, prm.c_amd, sgs_dt});
     }
     if (with_sgs && prm.sgs == 3 && prm.amd_wall_gradient && n_amdw > 0)

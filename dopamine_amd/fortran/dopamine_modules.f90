@@ -70,6 +70,15 @@ module dopamine_capi
       real(c_double), intent(out) :: nus(*)
       integer(c_int) :: rc
     end function
+    function dopamine_op_dynamic_smagorinsky(h, u, g, ub, dt_stab, nus) &
+        bind(C, name='dopamine_op_dynamic_smagorinsky') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: u(*), g(*), ub(*)
+      real(c_double), value :: dt_stab
+      real(c_double), intent(out) :: nus(*)
+      integer(c_int) :: rc
+    end function
     function dopamine_op_wall_model(h, u, nu, h_wm, t_filt, dt, tf_fac, tau) bind(C, name='dopamine_op_wall_model') &
         result(rc)
       import :: c_ptr, c_int, c_double
@@ -467,6 +476,34 @@ contains
   end subroutine
 
 end module mod_vreman
+
+
+module mod_dynamic_smagorinsky
+  use iso_c_binding
+  use dopamine_capi
+  use mod_mesh
+  implicit none
+  private
+  public :: dynamic_smagorinsky_sgs
+
+contains
+
+  ! src/sgs/mod_dynamic_smagorinsky.f90:39 (U_bnd required; without dt_stab no cap)
+  subroutine dynamic_smagorinsky_sgs(U, grad_U, msh, nu_sgs, U_bnd, dt_stab)
+    real(rp), intent(in) :: U(:,:)
+    real(rp), intent(in) :: grad_U(:,:,:)
+    type(Mesh), intent(inout) :: msh
+    real(rp), intent(inout) :: nu_sgs(:)
+    real(rp), intent(in), optional :: U_bnd(:,:)
+    real(rp), intent(in), optional :: dt_stab
+    real(rp) :: dts
+    if (.not. present(U_bnd)) error stop 'dynamic_smagorinsky_sgs: U_bnd is required'
+    dts = 0.0_rp
+    if (present(dt_stab)) dts = dt_stab
+    call check(dopamine_op_dynamic_smagorinsky(msh%handle, U, grad_U, U_bnd, dts, nu_sgs), 'dynamic_smagorinsky_sgs')
+  end subroutine
+
+end module mod_dynamic_smagorinsky
 
 
 module mod_amd

@@ -10,7 +10,7 @@ import numpy as np
 
 SCHEME_IDS = {"VANLEER_TVD": 1, "VANLEER": 1, "KOREN_TVD": 2, "KOREN": 2, "CDS": 3, "CENTRAL": 3, "KEP": 4,
               "SKEW": 4, "QUICK": 5}
-SGS_IDS = {"NONE": 0, "VREMAN": 1, "SMAGORINSKY": 2, "AMD": 3}
+SGS_IDS = {"NONE": 0, "VREMAN": 1, "SMAGORINSKY": 2, "AMD": 3, "DYNAMIC_SMAG": 4}
 PRECOND_IDS = {"JACOBI": 0, "LEARNED": 1, "MULTIGRID": 2}
 WALL_MODEL_IDS = {"loglaw": 0, "none": 1, "most": 2}
 
@@ -128,8 +128,8 @@ class SolverParams:
                              "(KOREN_TVD, VANLEER_TVD, QUICK, KEP, CDS)")
         if self.gradient_scheme.upper() not in ("LEAST_SQUARES", "GREEN_GAUSS"):
             raise ValueError("gradient_scheme must be LEAST_SQUARES or GREEN_GAUSS")
-        if self.use_sgs and self.sgs_model.upper() not in ("VREMAN", "SMAGORINSKY", "AMD"):
-            raise ValueError(f"sgs_model {self.sgs_model!r} not supported (VREMAN, SMAGORINSKY, AMD)")
+        if self.use_sgs and self.sgs_model.upper() not in SGS_IDS:
+            raise ValueError(f"sgs_model {self.sgs_model!r} not supported (VREMAN, SMAGORINSKY, DYNAMIC_SMAG, AMD)")
         if self.boussinesq and self.temperature_scheme.upper() not in ("CDS", "CENTRAL", "QUICK", "VANLEER_TVD",
                                                                          "VANLEER", "KOREN_TVD", "KOREN"):
             raise ValueError(f"temperature_scheme {self.temperature_scheme!r} not supported")

@@ -221,6 +221,11 @@ int dopamine_op_gg_gradient_scalar(void *solver, const double *phi, const double
  * dt_stab > 0 caps nu_t at 0.4 V^(2/3) / dt_stab. */
 int dopamine_op_sgs_viscosity(void *solver, int model, const double *U, const double *grad_U, double coef, double nu,
                               int wall_gradient, double dt_stab, double *nu_sgs);
+/* dynamic_smagorinsky_sgs (src/sgs/mod_dynamic_smagorinsky.f90:39): Germano-Lilly nu_sgs(n_cells) from
+ * U(3, n_cells), grad_U(3, 3, n_cells) and U_bnd(3, n_bnd); dt_stab > 0 caps it at 0.4 Delta^2 / dt_stab
+ * (the solver handle needs an SGS model other than none, so the filter-width tensor exists). */
+int dopamine_op_dynamic_smagorinsky(void *solver, const double *U, const double *grad_U, const double *U_bnd,
+                                    double dt_stab, double *nu_sgs);
 /* apply_wall_model_bc (src/sgs/mod_wall_model.f90:176): tau_w_bnd(3, n_bnd) of the log-law wall
  * model. h_wm: the sampling height, which must equal the solver's wall_model_height (the sampling
  * cells are found at create); t_filt > 0 / tf_fac > 0: the fixed / adaptive time filter, advanced

@@ -680,6 +680,97 @@ static void amd(om_solver *s, double dt_stab) {
   }
 }
 
+/* dynamic_smagorinsky_sgs (src/sgs/mod_dynamic_smagorinsky.f90:39-186): Germano-Lilly with the
+ * compact test filter (the volume-weighted mean of a cell and its face neighbours, periodic
+ * partners through the pair's owner, :238-304), Cs^2 = <L_ij M_ij> / <M_ij M_ij> clipped at 0,
+ * nu_t = Cs^2 Delta^2 |S|, Delta = det(G)^(1/6) of the Vreman filter tensor (mod_vreman.f90:367-381),
+ * capped by 0.4 Delta^2 / dt_stab. ALPHA_SQ = 4 (test-to-grid filter width ratio squared). */
+static void dyn_test_filter(const om_solver *s, int ncomp, const double *in, double *out) {
+  const int nc = s->nc;
+  for (int c = 0; c < nc; ++c) {
+    double wsum = s->m.cell_volume[c];
+    for (int k = 0; k < ncomp; ++k) out[(size_t)ncomp * c + k] = wsum * in[(size_t)ncomp * c + k];
+    for (int q = s->m.cell_face_offset[c]; q < s->m.cell_face_offset[c + 1]; ++q) {
+      const int fj = s->m.cell_faces[q];
+      if (fj < 0) continue;
+      int nb = s->m.face_neighbour[fj];
+      if (nb == c) nb = s->m.face_owner[fj];
+      if (nb < 0 && fj >= s->nint) {
+        const int pair = s->m.periodic_pair[fj - s->nint];
+        if (pair >= 0) nb = s->m.face_owner[s->nint + pair];
+      }
+      if (nb < 0 || nb >= nc || nb == c) continue;
+      const double w = s->m.cell_volume[nb];
+      wsum = wsum + w;
+      for (int k = 0; k < ncomp; ++k)
+        out[(size_t)ncomp * c + k] = out[(size_t)ncomp * c + k] + w * in[(size_t)ncomp * nb + k];
+    }
+    for (int k = 0; k < ncomp; ++k) out[(size_t)ncomp * c + k] = out[(size_t)ncomp * c + k] / fmax(wsum, SMALL);
+  }
+}
+
+/* |S| of a gradient g[3j+i] = d u_j / d x_i, summed over S(i,j) in column order (s_norm_of, :188-201) */
+static double dyn_s_norm(const double *g) {
+  double ss = 0.0;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      const double sij = 0.5 * (g[3 * j + i] + g[3 * i + j]);
+      ss = ss + sij * sij;
+    }
+  return sqrt(2.0 * ss);
+}
+
+static void dynamic_smag(om_solver *s, double dt_stab) {
+  const int nc = s->nc;
+  double *fb = dz(15 * (size_t)nc), *fh = dz(15 * (size_t)nc), *uh = dz(3 * (size_t)nc);
+  double *gh = dz(9 * (size_t)nc), *lm = dz(2 * (size_t)nc), *lmh = dz(2 * (size_t)nc);
+  for (int c = 0; c < nc; ++c) {
+    const double *u = &s->U[3 * (size_t)c], *g = &s->grad_U[9 * (size_t)c];
+    double *f = &fb[15 * (size_t)c];
+    /* S(i,j) from grad_U(i,j) = g[3j+i] */
+    const double S11 = g[0], S22 = g[4], S33 = g[8];
+    const double S12 = 0.5 * (g[3] + g[1]), S13 = 0.5 * (g[6] + g[2]), S23 = 0.5 * (g[7] + g[5]);
+    const double Sn = sqrt(2.0 * (S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23)));
+    f[0] = u[0]; f[1] = u[1]; f[2] = u[2];
+    f[3] = u[0] * u[0]; f[4] = u[0] * u[1]; f[5] = u[0] * u[2];
+    f[6] = u[1] * u[1]; f[7] = u[1] * u[2]; f[8] = u[2] * u[2];
+    f[9] = Sn * S11; f[10] = Sn * S12; f[11] = Sn * S13; f[12] = Sn * S22; f[13] = Sn * S23; f[14] = Sn * S33;
+  }
+  dyn_test_filter(s, 15, fb, fh);
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 3; ++k) uh[3 * (size_t)c + k] = fh[15 * (size_t)c + k];
+  gg_gradient_vector(s, uh, s->U_bnd, NULL, gh);
+  for (int c = 0; c < nc; ++c) {
+    /* germano_lilly_terms (:203-236) */
+    const double *h = &fh[15 * (size_t)c], *g = &gh[9 * (size_t)c];
+    double Sh[9], ss = 0.0;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) Sh[3 * j + i] = 0.5 * (g[3 * j + i] + g[3 * i + j]);
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) ss = ss + Sh[3 * j + i] * Sh[3 * j + i];
+    const double Shn = sqrt(2.0 * ss);
+    const double L11 = h[3] - h[0] * h[0], L12 = h[4] - h[0] * h[1], L13 = h[5] - h[0] * h[2];
+    const double L22 = h[6] - h[1] * h[1], L23 = h[7] - h[1] * h[2], L33 = h[8] - h[2] * h[2];
+    const double d = fmax(pow(fmax(det3(&s->Gf[9 * (size_t)c]), 0.0), 1.0 / 6.0), SMALL);
+    const double d2 = d * d;
+    /* Sh[3j+i] = Shat(i,j); symmetric */
+    const double M11 = d2 * (2.0 * (h[9] - 4.0 * Shn * Sh[0])), M12 = d2 * (2.0 * (h[10] - 4.0 * Shn * Sh[3]));
+    const double M13 = d2 * (2.0 * (h[11] - 4.0 * Shn * Sh[6])), M22 = d2 * (2.0 * (h[12] - 4.0 * Shn * Sh[4]));
+    const double M23 = d2 * (2.0 * (h[13] - 4.0 * Shn * Sh[7])), M33 = d2 * (2.0 * (h[14] - 4.0 * Shn * Sh[8]));
+    lm[2 * (size_t)c] = L11 * M11 + L22 * M22 + L33 * M33 + 2.0 * (L12 * M12 + L13 * M13 + L23 * M23);
+    lm[2 * (size_t)c + 1] = M11 * M11 + M22 * M22 + M33 * M33 + 2.0 * (M12 * M12 + M13 * M13 + M23 * M23);
+  }
+  dyn_test_filter(s, 2, lm, lmh);
+  for (int c = 0; c < nc; ++c) {
+    const double d = fmax(pow(fmax(det3(&s->Gf[9 * (size_t)c]), 0.0), 1.0 / 6.0), SMALL);
+    double nu = fmax(lmh[2 * (size_t)c] / fmax(lmh[2 * (size_t)c + 1], SMALL), 0.0) * (d * d) *
+                dyn_s_norm(&s->grad_U[9 * (size_t)c]);
+    const double cap = NU_SGS_STAB_FRAC * (d * d) / fmax(dt_stab, SMALL);
+    s->nu_sgs[c] = fmin(nu, cap);
+  }
+  free(fb); free(fh); free(uh); free(gh); free(lm); free(lmh);
+}
+
 /* reichardt_uplus / wall_stress_newton (src/sgs/mod_wall_model.f90:50-115) */
 static void reichardt(double yp, double cbuf, double *up, double *dup) {
   const double A1 = 11.0, A2 = 0.33, A1I = 1.0 / 11.0;
@@ -2523,6 +2614,8 @@ static void stage_closures(om_solver *s, double dt_stab, int stage) {
     smagorinsky(s, dt_stab);
   else if (s->p.sgs == OM_SGS_AMD)
     amd(s, dt_stab);
+  else if (s->p.sgs == OM_SGS_DYNAMIC)
+    dynamic_smag(s, dt_stab);
   else
     memset(s->nu_sgs, 0, sizeof(double) * s->nc);
   if (s->p.wall_model == OM_WM_NONE)
@@ -2610,9 +2703,11 @@ void om_op_sgs(om_solver *s, int model, double coef, int wall_gradient, double d
   } else if (model == OM_SGS_SMAGORINSKY) {
     s->p.cs_smag = coef;
     smagorinsky(s, dt_stab);
-  } else {
+  } else if (model == OM_SGS_AMD) {
     s->p.c_amd = coef;
     amd(s, dt_stab);
+  } else {
+    dynamic_smag(s, dt_stab);
   }
   s->nu_sgs = nus;
   s->p = keep;

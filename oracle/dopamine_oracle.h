@@ -33,7 +33,7 @@ typedef struct om_mesh {
 
 enum { OM_SCHEME_VANLEER = 1, OM_SCHEME_KOREN = 2, OM_SCHEME_CDS = 3, OM_SCHEME_KEP = 4, OM_SCHEME_QUICK = 5 };
 enum { OM_GRAD_LS = 0, OM_GRAD_GG = 1 };
-enum { OM_SGS_NONE = 0, OM_SGS_VREMAN = 1, OM_SGS_SMAGORINSKY = 2, OM_SGS_AMD = 3 };
+enum { OM_SGS_NONE = 0, OM_SGS_VREMAN = 1, OM_SGS_SMAGORINSKY = 2, OM_SGS_AMD = 3, OM_SGS_DYNAMIC = 4 };
 enum { OM_WM_LOGLAW = 0, OM_WM_NONE = 1, OM_WM_MOST = 2 };
 
 typedef struct om_params {

@@ -12,6 +12,7 @@ module ops_harness
   use mod_viscous
   use mod_vreman
   use mod_amd
+  use mod_dynamic_smagorinsky
   use mod_wall_model
   use mod_fv_solver
   implicit none
@@ -145,7 +146,8 @@ contains
   end function
 
   !> model 1: vreman_sgs_viscosity, 2: smagorinsky_sgs (cs_in = coef), 3: amd_sgs_viscosity
-  !> (c_amd = coef); a null grad_U takes the LS gradient of U first (Vreman and AMD)
+  !> (c_amd = coef), 4: dynamic_smagorinsky_sgs; a null grad_U takes the LS gradient of U first
+  !> (Vreman and AMD)
   function ft_sgs(h, model, pu, pub, pgu, coef, nu, wall_gradient, dt_stab, pnus) bind(C, name='ft_sgs') result(rc)
     type(c_ptr), value :: h, pu, pub, pgu, pnus
     integer(c_int), value :: model, wall_gradient
@@ -174,6 +176,8 @@ contains
       else
         call amd_sgs_viscosity(U, msh, nus, nu, coef, wall_gradient /= 0, Ub, dt_stab=dt_stab)
       end if
+    case (4)
+      call dynamic_smagorinsky_sgs(U, gU, msh, nus, Ub, dt_stab)
     case default
       rc = -1
     end select

@@ -22,6 +22,7 @@ MODULE_ROUTINES = {
     "mod_viscous": ["viscous_rhs_vector", "viscous_rhs_scalar"],
     "mod_vreman": ["vreman_sgs_viscosity", "smagorinsky_sgs"],
     "mod_amd": ["amd_sgs_viscosity"],
+    "mod_dynamic_smagorinsky": ["dynamic_smagorinsky_sgs"],
     "mod_wall_model": ["apply_wall_model_bc"],
     "mod_fv_solver": ["solver_advance", "solver_project_once", "solver_compute_dt", "solver_state_is_finite"],
 }

@@ -197,20 +197,20 @@ def test_fortran_scalar_transport(name, scheme):
     assert rel_l2(refv, rv) < TOL
 
 
-@pytest.mark.parametrize("model", [1, 2, 3])
+@pytest.mark.parametrize("model", [1, 2, 3, 4])
 @pytest.mark.parametrize("name", MESHES)
 def test_fortran_sgs(name, model):
     s = _state(name)
     print(f"[gpu] fortran sgs model {model} {name}", flush=True)
     F = _ftest()
-    coef = {1: 0.025, 2: 0.1, 3: 1.0 / 3.0}[model]
+    coef = {1: 0.025, 2: 0.1, 3: 1.0 / 3.0, 4: 0.0}[model]
     nu = s.case.params.nu
     for dt_stab in (0.0, 1.0e-3):
         nus = np.zeros(s.nc)
         assert F.ft_sgs(s.h, model, _a(s.U), _a(s.Ub), _a(s.gU), coef, nu, 1, dt_stab, _a(nus)) == 0
         ref = s.o.op_sgs(model, coef, True, dt_stab)
         assert rel_l2(ref, nus) < TOL, dt_stab
-    if model != 2:  # without grad_U_ext the routines take the LS gradient of U first
+    if model in (1, 3):  # without grad_U_ext the routines take the LS gradient of U first
         nus2 = np.zeros(s.nc)
         assert F.ft_sgs(s.h, model, _a(s.U), _a(s.Ub), None, coef, nu, 1, 0.0, _a(nus2)) == 0
         g_ls = s.o.op_gradient_vector()
