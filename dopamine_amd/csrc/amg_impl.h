@@ -34,7 +34,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <stdexcept>
 #include <vector>
 
@@ -291,11 +293,65 @@ inline size_t amg_sell_layout(int n, const std::vector<int>& rptr, std::vector<i
   return size_t(sptr[nsl]);
 }
 
+// The smoothers' scaled couplings cs = coef / diag_j, cr = coef / diag_i lie in [0, 1] and are
+// stored as IEEE binary16 (AmgCoef): half the bytes of the smoothing passes' coefficient stream.
+// Their 2^-11 relative rounding only perturbs the preconditioner, which the flexible CG and the
+// FP64 refinement around it absorb.
+using AmgCoef = uint16_t;
+DM_INLINE float coef_f(AmgCoef h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return float(__builtin_bit_cast(_Float16, h));
+#else
+  const uint32_t sign = uint32_t(h >> 15) << 31, e = (h >> 10) & 0x1f, m = h & 0x3ffu;
+  uint32_t bits;
+  if (e == 0) {
+    if (m == 0) {
+      bits = sign;
+    } else {  // subnormal: m 2^-24
+      const float v = float(m) * 5.9604644775390625e-8f;
+      std::memcpy(&bits, &v, 4);
+      bits |= sign;
+    }
+  } else if (e == 31) {
+    bits = sign | 0x7f800000u | (m << 13);
+  } else {
+    bits = sign | ((e + 112u) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+#endif
+}
+// round to nearest even (set-up, on the host)
+inline AmgCoef coef_h(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return AmgCoef(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+  if (ax >= 0x477ff000u) return AmgCoef(sign | 0x7c00u);  // rounds beyond the largest half: inf
+  if (ax < 0x38800000u) {  // below the smallest normal half: subnormal or zero
+    if (ax < 0x33000000u) return AmgCoef(sign);
+    const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126 - e;  // value = m 2^(e - 150); half subnormal unit 2^-24
+    const uint32_t q = m >> (shift - 1 + 1), rem = m & ((1u << shift) - 1), half = 1u << (shift - 1);
+    uint32_t r = q;
+    if (rem > half || (rem == half && (q & 1u))) ++r;
+    return AmgCoef(sign | r);
+  }
+  const uint32_t e = (ax >> 23) - 112, m = ax & 0x7fffffu;
+  uint32_t r = (e << 10) | (m >> 13);
+  const uint32_t rem = m & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) ++r;
+  return AmgCoef(sign | r);
+}
+
 struct AmgLevelDev {
   int n = 0, w = 0;                              // cells; ELL width (level 0 only)
   const int* nbr = nullptr;                      // level 0: ELL neighbours [slot][cell]
   const int *sptr = nullptr, *col = nullptr;     // coarse levels: sliced-ELL rows
-  const float *cs = nullptr, *cr = nullptr, *dinv = nullptr;
+  const AmgCoef *cs = nullptr, *cr = nullptr;
+  const float* dinv = nullptr;
   const int *kptr = nullptr, *kidx = nullptr;    // children (cells of this level) of the next coarser level's cells, sliced ELL
   float *b = nullptr, *x = nullptr, *r = nullptr;
   // level 0 with three or more levels: its cells' aggregates and level 1's post-smoothed values,
@@ -373,7 +429,7 @@ DM_INLINE void down_cell(const AmgLevelDev& L, const T* b, float omega, int c) {
     const size_t q = size_t(k) * n + c;
     const int j = L.nbr[q];
     if (j < 0) continue;
-    s = s + L.cs[q] * float(b[j]);
+    s = s + coef_f(L.cs[q]) * float(b[j]);
   }
   L.x[c] = omega * L.dinv[c] * bc;
   L.r[c] = (1.0f - omega) * bc + omega * s;
@@ -391,7 +447,7 @@ DM_INLINE float up_cell(const AmgLevelDev& L, const T* b, float omega, float al,
     const size_t q = size_t(k) * n + c;
     const int j = L.nbr[q];
     if (j < 0) continue;
-    s = s + L.cr[q] * L.x[j];
+    s = s + coef_f(L.cr[q]) * L.x[j];
   }
   return xc + omega * (L.dinv[c] * float(b[c]) - xc + s);
 }
@@ -413,13 +469,13 @@ struct Prolong0 {
 // Level-0 slot loads of the batched (branch-free) kernels: WC slots of cell c, an empty slot
 // pointing at the cell itself with a zero coefficient.
 template <int WC>
-DM_INLINE void slots0(const AmgLevelDev& L, const float* cf, int c, int* jj, float* h) {
+DM_INLINE void slots0(const AmgLevelDev& L, const AmgCoef* cf, int c, int* jj, float* h) {
 #pragma unroll
   for (int kk = 0; kk < WC; ++kk) {
     const bool live = kk < L.w;
     const size_t q = size_t(live ? kk : 0) * L.n + c;
     const int j = L.nbr[q];
-    const float v = cf[q];
+    const float v = coef_f(cf[q]);
     jj[kk] = (live && j >= 0) ? j : c;
     h[kk] = (live && j >= 0) ? v : 0.0f;
   }
@@ -482,7 +538,7 @@ DM_INLINE int sell_base(const int* sptr, int r) { return sptr[r / AMG_SL] + r % 
 DM_INLINE int sell_width(const int* sptr, int r) { return (sptr[r / AMG_SL + 1] - sptr[r / AMG_SL]) / AMG_SL; }
 // sum_k a_rk v[col_rk] over this lane's chunks of AMG_UN slots of row r (chunks lane, lane + st,
 // ...), AMG_UN independent loads at a time
-DM_INLINE float sell_row(const int* sptr, const int* col, const float* a, const float* v, int r, int lane, int st) {
+DM_INLINE float sell_row(const int* sptr, const int* col, const AmgCoef* a, const float* v, int r, int lane, int st) {
   const int q0 = sell_base(sptr, r), nch = sell_width(sptr, r) / AMG_UN;
   float s = 0.0f;
   for (int ch = lane; ch < nch; ch += st) {
@@ -492,7 +548,7 @@ DM_INLINE float sell_row(const int* sptr, const int* col, const float* a, const 
 #pragma unroll
     for (int u = 0; u < AMG_UN; ++u) {
       j[u] = col[q0 + (k + u) * AMG_SL];
-      h[u] = a[q0 + (k + u) * AMG_SL];
+      h[u] = coef_f(a[q0 + (k + u) * AMG_SL]);
     }
 #pragma unroll
     for (int u = 0; u < AMG_UN; ++u) x[u] = v[j[u]];
@@ -726,8 +782,8 @@ class AmgPc : public FcgPc, public FcgPc32 {
             const size_t q = size_t(k) * cur.n + c;
             scaled(c, q, &cs[q], &cr[q]);
           }
-        L.cs = up(cs);
-        L.cr = up(cr);
+        L.cs = up(to_coef(cs));
+        L.cr = up(to_coef(cr));
       } else {  // the non-empty entries, sliced ELL
         std::vector<int> rptr(size_t(cur.n) + 1, 0), col;
         std::vector<float> cs, cr;
@@ -756,8 +812,8 @@ class AmgPc : public FcgPc, public FcgPc32 {
         }
         L.sptr = up(sptr);
         L.col = up(scol);
-        L.cs = up(scs);
-        L.cr = up(scr);
+        L.cs = up(to_coef(scs));
+        L.cr = up(to_coef(scr));
         nnz_.push_back(int64_t(col.size()));
         stored_.push_back(int64_t(m));
       }
@@ -853,6 +909,11 @@ class AmgPc : public FcgPc, public FcgPc32 {
     x_->upload(p, v.data(), v.size());
     owned_.push_back(p);
     return p;
+  }
+  static std::vector<AmgCoef> to_coef(const std::vector<float>& v) {
+    std::vector<AmgCoef> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = coef_h(v[i]);
+    return h;
   }
   template <class T>
   T* fresh(size_t n) {

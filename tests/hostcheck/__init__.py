@@ -51,8 +51,19 @@ def lib():
                                     _pi]
         L.hc_cg_solve_retry.argtypes = [ctypes.c_int, ctypes.c_int, _pi, _P(_d), _P(_d), _P(_d), _P(_d), _d, _d,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _pi, _P(_d), _pi, _pi]
+        L.hc_coef_roundtrip.argtypes = [_P(ctypes.c_float), _P(ctypes.c_uint16), _P(ctypes.c_float), ctypes.c_int]
         _lib = L
     return _lib
+
+
+def coef_roundtrip(values):
+    """The multigrid's binary16 coupling storage: (half bits, value read back)."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    h = np.zeros(v.size, dtype=np.uint16)
+    out = np.zeros(v.size, dtype=np.float32)
+    lib().hc_coef_roundtrip(v.ctypes.data_as(_P(ctypes.c_float)), h.ctypes.data_as(_P(ctypes.c_uint16)),
+                            out.ctypes.data_as(_P(ctypes.c_float)), v.size)
+    return h, out
 
 
 class HostSolver:

@@ -74,6 +74,13 @@ int hc_solver_read_checkpoint(void* h, const char* dir, int step, int rank) {
     return -100;
   }
 }
+// binary16 storage of the multigrid couplings: round to nearest even and back
+void hc_coef_roundtrip(const float* in, uint16_t* h, float* out, int n) {
+  for (int i = 0; i < n; ++i) {
+    h[i] = dm::coef_h(in[i]);
+    out[i] = dm::coef_f(h[i]);
+  }
+}
 double* hc_solver_field(void* h, const char* name, int64_t* n) { return field(static_cast<Solver*>(h), name, n); }
 void hc_solver_apply_all_bc(void* h) { static_cast<Solver*>(h)->apply_all_bc(); }
 int hc_solver_project_once(void* h, double dt) { return static_cast<Solver*>(h)->project_once(dt); }

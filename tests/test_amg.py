@@ -161,7 +161,9 @@ def test_product_hierarchy_and_cycle_match_the_statement(name):
     assert np.array_equal(agg, H.aggs[0])
     zo = H.vcycle(r)
     err = np.linalg.norm(z - zo) / np.linalg.norm(zo)
-    assert err < 2e-5, err  # FP32 cycle against the float64 statement
+    # the product's cycle runs in FP32 with its couplings stored as binary16 (relative rounding
+    # 2^-12 on average, 2^-11 at most) against the float64 statement
+    assert err < 5e-4, err
 
 
 @pytest.mark.parametrize("mixed", [False, True])
@@ -196,7 +198,7 @@ def test_aggressive_first_coarsening_on_flat_cells():
     z, agg, sizes = amg_vcycle(nbr, coef, diag, r, cfg=AMG_PRODUCT)
     assert sizes == H.sizes and np.array_equal(agg, line)  # agg_out is the theta = 0.08 aggregation
     zo = H.vcycle(r)
-    assert np.linalg.norm(z - zo) <= 2e-5 * np.linalg.norm(zo)
+    assert np.linalg.norm(z - zo) <= 5e-4 * np.linalg.norm(zo)  # FP32 cycle, binary16 couplings
     b = r - r.mean()
     rc, x, its, res, sizes2, _ = amg_solve(nbr, coef, diag, b, 1e-8, nullspace=ns, mixed=True, cfg=AMG_PRODUCT)
     assert rc == 0 and res <= 1e-8 * np.linalg.norm(b) and sizes2 == H.sizes
@@ -207,3 +209,19 @@ def test_channel_keeps_the_strength_threshold():
     nbr, coef, diag, ns = operator(cases.small_channel(32, 24, 16))
     H = om.Hierarchy(nbr, coef, diag, *AMG_PRODUCT)
     assert not H.aggressive and H.alpha == 1.5
+
+
+def test_binary16_coupling_storage_matches_ieee_half():
+    """The couplings' binary16 storage rounds to nearest even as IEEE 754 half precision does
+    (numpy float16): normals, subnormals, ties, overflow to infinity."""
+    from hostcheck import coef_roundtrip
+    rng = np.random.default_rng(16)
+    v = np.concatenate([rng.random(20000), rng.random(2000) * 1e-5, rng.random(2000) * 1e-7,
+                        -rng.random(1000), np.array([0.0, -0.0, 1.0, 65504.0, 65519.0, 65520.0, 1e6, 2.0 ** -24,
+                                                     2.0 ** -25, 3 * 2.0 ** -26, 1.0 + 2.0 ** -11,
+                                                     1.0 + 3 * 2.0 ** -11])]).astype(np.float32)
+    h, back = coef_roundtrip(v)
+    with np.errstate(over="ignore"):  # 65520 and 1e6 overflow to infinity on purpose
+        want = v.astype(np.float16)
+    assert np.array_equal(h, want.view(np.uint16))
+    assert np.array_equal(back, want.astype(np.float32))
