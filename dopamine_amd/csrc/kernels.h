@@ -1316,7 +1316,7 @@ struct ReconFaceScalarN {
   const double* pb[RHS_MAX_FIELDS];
   const double* gphi[RHS_MAX_FIELDS];
   double* phiF[RHS_MAX_FIELDS];
-  DM_HD void operator()(int f) const {
+  DM_HD void values(int f, double* out) const {
     const DevMesh& M = base.R.M;
     const int o = M.fown[f], nb = M.fnb[f];
     double Fn;
@@ -1344,19 +1344,90 @@ struct ReconFaceScalarN {
         v = pb[i][b];
       }
       if (!isfinite(v)) v = 0.0;
-      phiF[i][f] = v;
+      out[i] = v;
+    }
+  }
+  DM_HD void operator()(int f) const {
+    double v[NF];
+    values(f, v);
+#pragma unroll
+    for (int i = 0; i < NF; ++i) phiF[i][f] = v[i];
+  }
+};
+
+// Scalar transport of NF fields that share a scheme, face half: for each interior face the
+// convective flux F phi_f and the diffusive flux of every field, as ExplicitRhsTN's per-cell sums
+// use them (computed once per face instead of once from each side), into FT: plane 2q the
+// convective, plane 2q + 1 the diffusive flux of field q, nint values each. Boundary faces of a
+// reconstruction scheme get their face values phiF, as ReconFaceScalarN writes them; with CDS
+// (cds = 1) the launch covers the interior faces only.
+template <int SCHEME, int NF>
+struct FaceRhsTN {
+  ReconFaceScalarN<SCHEME, NF> rec;  // face values of the reconstruction scheme (cds = 0)
+  const double* nus;
+  int cds;
+  double alpha[RHS_MAX_FIELDS], prt[RHS_MAX_FIELDS];
+  double* FT;
+  DM_HD void operator()(int f) const {
+    const DevMesh& M = rec.base.R.M;
+    const int nb = M.fnb[f];
+    if (nb < 0) {
+      if (!cds) rec(f);
+      return;
+    }
+    const int o = M.fown[f];
+    const size_t ni = size_t(M.nint);
+    double pf[NF];
+    if (!cds) rec.values(f, pf);
+    const double* sf = &M.sf[3 * size_t(f)];
+    const double gc = M.gc[f];
+    double d_on[3], dnp[3];
+    for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
+    const double dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+    const double sdd = dot3(sf, d_on) / dl;
+    for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dl) * d_on[i];
+    const double ow = ortho_weight(sdd, M.area[f]);
+    const double nuf = gc * nus[o] + (1.0 - gc) * nus[nb];
+    const double Fn0 = rec.base.R.UF[f];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      const double* Tq = rec.phi[q];
+      const double* gq = rec.gphi[q];
+      double Fn = Fn0;
+      double phif;
+      if (cds) {
+        const double pl = gc * Tq[o] + (1.0 - gc) * Tq[nb];
+        double gm[3];
+        for (int i = 0; i < 3; ++i) gm[i] = gc * gq[3 * size_t(o) + i] + (1.0 - gc) * gq[3 * size_t(nb) + i];
+        phif = pl + dot3(gm, &M.eps[3 * size_t(f)]);
+        const double w = M.wbl[f];
+        if (w < 1.0) phif = pl + w * (phif - pl);
+        if (!isfinite(phif)) phif = (Fn >= 0.0) ? Tq[o] : Tq[nb];
+        if (!isfinite(Fn)) Fn = 0.0;
+        if (!isfinite(phif)) phif = 0.0;
+      } else {
+        if (!isfinite(Fn)) Fn = 0.0;
+        phif = pf[q];
+      }
+      FT[2 * q * ni + f] = Fn * phif;
+      const double ae = alpha[q] + nuf / prt[q];
+      double gf[3];
+      for (int i = 0; i < 3; ++i) gf[i] = gc * gq[3 * size_t(o) + i] + (1.0 - gc) * gq[3 * size_t(nb) + i];
+      FT[(2 * q + 1) * ni + f] = ae * ((Tq[nb] - Tq[o]) * ow / dl + dot3(gf, dnp));
     }
   }
 };
 
-// Scalar transport of NF fields that share a scheme, in one pass over the cell's faces: each field
-// as ExplicitRhsT computes it (same expressions, same order), the face geometry read once.
+// Scalar transport of NF fields that share a scheme, cell half: each field's sums as ExplicitRhsT
+// forms them (same expressions, same order), the interior faces' fluxes read from FaceRhsTN's FT,
+// the boundary faces computed here (one side only).
 template <int NF>
 struct ExplicitRhsTN {
   DevMesh M;
   const double *UF, *nus;
   double dt, a, bcoef;
   int cds;
+  const double* FT;  // interior-face fluxes from FaceRhsTN
   const double *T[RHS_MAX_FIELDS], *Tb[RHS_MAX_FIELDS], *gT[RHS_MAX_FIELDS], *HT0[RHS_MAX_FIELDS],
       *phiF[RHS_MAX_FIELDS], *qw[RHS_MAX_FIELDS];
   double *HT[RHS_MAX_FIELDS], *Tnew[RHS_MAX_FIELDS];
@@ -1373,42 +1444,12 @@ struct ExplicitRhsTN {
       const int e = M.aface[s], f = e >> 1, side = e & 1, oth = M.aoth[s];
       const double* sf = &M.sf[3 * size_t(f)];
       if (oth >= 0) {
-        const int o = side ? oth : c, nb = side ? c : oth;
-        const double gc = M.gc[f];
-        double d_on[3], dnp[3];
-        for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
-        const double dl = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
-        const double sdd = dot3(sf, d_on) / dl;
-        for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dl) * d_on[i];
-        const double ow = ortho_weight(sdd, M.area[f]);
-        const double nuf = gc * nus[o] + (1.0 - gc) * nus[nb];
-        const double Fn0 = UF[f];
+        const size_t ni = size_t(M.nint);
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
-          const double* Tq = T[q];
-          const double* gq = gT[q];
-          double Fn = Fn0;
-          double phif;
-          if (cds) {
-            const double pl = gc * Tq[o] + (1.0 - gc) * Tq[nb];
-            double gm[3];
-            for (int i = 0; i < 3; ++i) gm[i] = gc * gq[3 * size_t(o) + i] + (1.0 - gc) * gq[3 * size_t(nb) + i];
-            phif = pl + dot3(gm, &M.eps[3 * size_t(f)]);
-            const double w = M.wbl[f];
-            if (w < 1.0) phif = pl + w * (phif - pl);
-            if (!isfinite(phif)) phif = (Fn >= 0.0) ? Tq[o] : Tq[nb];
-            if (!isfinite(Fn)) Fn = 0.0;
-            if (!isfinite(phif)) phif = 0.0;
-          } else {
-            if (!isfinite(Fn)) Fn = 0.0;
-            phif = phiF[q][f];
-          }
-          cv[q] = side ? cv[q] + Fn * phif * iv : cv[q] - Fn * phif * iv;
-          const double ae = alpha[q] + nuf / prt[q];
-          double gf[3];
-          for (int i = 0; i < 3; ++i) gf[i] = gc * gq[3 * size_t(o) + i] + (1.0 - gc) * gq[3 * size_t(nb) + i];
-          const double flux = ae * ((Tq[nb] - Tq[o]) * ow / dl + dot3(gf, dnp));
-          vs[q] = side ? vs[q] - flux * iv : vs[q] + flux * iv;
+          const double cf = FT[2 * q * ni + f], vf = FT[(2 * q + 1) * ni + f];
+          cv[q] = side ? cv[q] + cf * iv : cv[q] - cf * iv;
+          vs[q] = side ? vs[q] - vf * iv : vs[q] + vf * iv;
         }
       } else {
         const int b = -1 - oth, bc = M.bbc[b];
@@ -1620,10 +1661,61 @@ struct ExplicitRhsT {
 
 // ------------------------------------------------------------------ explicit right-hand side
 
+// Momentum right-hand side, face half: for each interior face the viscous stress flux of the three
+// components (viscous_rhs_vector's face term, src/numerics/mod_viscous.f90:37-283) into planes 0-2
+// of FV (nint values each) and, with CDS, the convective flux F u_f into planes 3-5 — computed once
+// per face; ExplicitRhs adds them to both cells. KEP and the reconstruction schemes keep their
+// convective term in the cell pass (it reads only U of the neighbour, or the face values).
+template <int SCHEME>
+struct FaceRhsU {
+  DevMesh M;
+  const double *U, *UF, *grad, *nus;
+  double nu;
+  double* FV;
+  DM_HD void operator()(int f) const {
+    const int o = M.fown[f], nb = M.fnb[f];
+    const size_t ni = size_t(M.nint);
+    const double* Uo = &U[3 * size_t(o)];
+    const double* Un = &U[3 * size_t(nb)];
+    const double* sf = &M.sf[3 * size_t(f)];
+    if (SCHEME == 3) {
+      const double Fn = UF[f];
+      const double gcf = M.gc[f];
+      const double* ep = &M.eps[3 * size_t(f)];
+      const double w = M.wbl[f];
+      for (int q = 0; q < 3; ++q) {
+        const double pl = gcf * Uo[q] + (1.0 - gcf) * Un[q];
+        double gm[3];
+        for (int i = 0; i < 3; ++i) gm[i] = gcf * grad[DM_G(o, i, q)] + (1.0 - gcf) * grad[DM_G(nb, i, q)];
+        double phi = pl + dot3(gm, ep);
+        if (w < 1.0) phi = pl + w * (phi - pl);
+        if (!isfinite(phi)) phi = 0.0;
+        FV[(3 + q) * ni + f] = Fn * phi;
+      }
+    }
+    const double gc = M.gc[f];
+    const double nue = nu + gc * nus[o] + (1.0 - gc) * nus[nb];
+    double gf[9];
+    for (int q = 0; q < 9; ++q) gf[q] = gc * grad[9 * size_t(o) + q] + (1.0 - gc) * grad[9 * size_t(nb) + q];
+    double d_on[3], dnp[3];
+    for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
+    const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+    const double sdd = dot3(sf, d_on) / dlen;
+    for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dlen) * d_on[i];
+    const double wo = ortho_weight(sdd, M.area[f]);
+    for (int q = 0; q < 3; ++q) {
+      const double col = gf[3 * q] * dnp[0] + gf[3 * q + 1] * dnp[1] + gf[3 * q + 2] * dnp[2];
+      const double row = gf[q] * sf[0] + gf[3 + q] * sf[1] + gf[6 + q] * sf[2];
+      FV[q * ni + f] = nue * ((Un[q] - Uo[q]) * wo / dlen + col + row);
+    }
+  }
+};
+
 // advection_rhs_vector_{kep,cds}_soa (src/numerics/mod_advection.f90:1013-1392) + viscous_rhs_vector
 // (src/numerics/mod_viscous.f90:37-283) + compute_explicit_rhs assembly with body force and wall
 // traction (src/solver/mod_fv_solver.f90:1617-1711, 2238-2275) + the RK3W predictor
-// u* = u + dt (alpha H + beta H_prev) (:1326-1331), for one cell. Writes H and u* (to Unew).
+// u* = u + dt (alpha H + beta H_prev) (:1326-1331), for one cell (interior faces' viscous, and
+// CDS convective, fluxes from FaceRhsU's FV). Writes H and u* (to Unew).
 template <int SCHEME>
 struct ExplicitRhs {
   DevMesh M;
@@ -1637,6 +1729,7 @@ struct ExplicitRhs {
   int rot;             // Coriolis -2 om x U (src/solver/mod_fv_solver.f90:1694-1698)
   double om0, om1, om2;
   int part = 0;        // 0: H and the predictor; 1: only the advection term into H; 2: only the viscous term
+  const double* FV = nullptr;  // interior-face fluxes from FaceRhsU
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -1650,6 +1743,7 @@ struct ExplicitRhs {
       const double* sf = &M.sf[3 * size_t(f)];
       if (oth >= 0) {
         const int o = side ? oth : c, nb = side ? c : oth;
+        const size_t ni = size_t(M.nint);
         const double* Uo = &U[3 * size_t(o)];
         const double* Un = &U[3 * size_t(nb)];
         const double Fn = UF[f];
@@ -1667,34 +1761,14 @@ struct ExplicitRhs {
             cv[q] = side ? cv[q] + Fr * phi * iv : cv[q] - Fr * phi * iv;
           }
         } else {
-          const double gcf = M.gc[f];
-          const double* ep = &M.eps[3 * size_t(f)];
-          const double w = M.wbl[f];
           for (int q = 0; q < 3; ++q) {
-            const double pl = gcf * Uo[q] + (1.0 - gcf) * Un[q];
-            double gm[3];
-            for (int i = 0; i < 3; ++i) gm[i] = gcf * grad[DM_G(o, i, q)] + (1.0 - gcf) * grad[DM_G(nb, i, q)];
-            double phi = pl + dot3(gm, ep);
-            if (w < 1.0) phi = pl + w * (phi - pl);
-            if (!isfinite(phi)) phi = 0.0;
-            cv[q] = side ? cv[q] + Fn * phi * iv : cv[q] - Fn * phi * iv;
+            const double fl = FV[(3 + q) * ni + f];
+            cv[q] = side ? cv[q] + fl * iv : cv[q] - fl * iv;
           }
         }
         // viscous stress
-        const double gc = M.gc[f];
-        const double nue = nu + gc * nus[o] + (1.0 - gc) * nus[nb];
-        double gf[9];
-        for (int q = 0; q < 9; ++q) gf[q] = gc * grad[9 * size_t(o) + q] + (1.0 - gc) * grad[9 * size_t(nb) + q];
-        double d_on[3], dnp[3];
-        for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
-        const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
-        const double sdd = dot3(sf, d_on) / dlen;
-        for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dlen) * d_on[i];
-        const double wo = ortho_weight(sdd, M.area[f]);
         for (int q = 0; q < 3; ++q) {
-          const double col = gf[3 * q] * dnp[0] + gf[3 * q + 1] * dnp[1] + gf[3 * q + 2] * dnp[2];
-          const double row = gf[q] * sf[0] + gf[3 + q] * sf[1] + gf[6 + q] * sf[2];
-          const double st = nue * ((Un[q] - Uo[q]) * wo / dlen + col + row);
+          const double st = FV[q * ni + f];
           vs[q] = side ? vs[q] - st * iv : vs[q] + st * iv;
         }
       } else {
@@ -1815,16 +1889,57 @@ struct ExplicitRhs {
 
 // ------------------------------------------------------------------ projection
 
-// compute_face_fluxes_and_rhs (src/solver/mod_fv_solver.f90:2389-2752) for one cell: provisional
-// face fluxes with the Rhie-Chow term (written by the owner side), rhs_p = -(1/dt) sum q + deferred
+// compute_face_fluxes_and_rhs (src/solver/mod_fv_solver.f90:2389-2752), first half: the
+// provisional flux of each pressure-operator face (interior faces and canonical periodic edges)
+// with its Rhie-Chow term, computed once per face (one lane per face) into ff[fm].
+struct FaceFluxRC {
+  DevMesh M;
+  const double *U, *grad, *gprc, *phi;
+  double* ff;
+  double dt;
+  int rck;  // Rhie-Chow subtracts the k-vector (non-orthogonal mesh, rhie_chow_dual off)
+  DM_HD void operator()(int i) const {
+    const int o = M.pown[i];
+    if (o < 0) return;  // skipped face (no finite operator coefficient)
+    const int nb = M.pnbr[i], fm = M.pfm[i];
+    const double* sf = &M.sf[3 * size_t(fm)];
+    const double gcf = M.rcgc[i];
+    double Upl[3], Uf[3];
+    for (int q = 0; q < 3; ++q) Upl[q] = gcf * U[3 * size_t(o) + q] + (1.0 - gcf) * U[3 * size_t(nb) + q];
+    if (fm < M.nint) {
+      const double wb = M.wbl[fm], g = M.gc[fm];
+      const double* ep = &M.eps[3 * size_t(fm)];
+      for (int j = 0; j < 3; ++j) {
+        double ue = g * U[3 * size_t(o) + j] + (1.0 - g) * U[3 * size_t(nb) + j];
+        double gm[3];
+        for (int k = 0; k < 3; ++k) gm[k] = g * grad[DM_G(o, k, j)] + (1.0 - g) * grad[DM_G(nb, k, j)];
+        ue = ue + gm[0] * ep[0] + gm[1] * ep[1] + gm[2] * ep[2];
+        Uf[j] = Upl[j] + wb * (ue - Upl[j]);
+      }
+    } else {
+      Uf[0] = Upl[0]; Uf[1] = Upl[1]; Uf[2] = Upl[2];
+    }
+    double flux = Uf[0] * sf[0] + Uf[1] * sf[1] + Uf[2] * sf[2];
+    double gm[3], smk[3];
+    const double* kr = &M.kint[3 * size_t(i)];
+    for (int k = 0; k < 3; ++k) {
+      gm[k] = gcf * gprc[3 * size_t(o) + k] + (1.0 - gcf) * gprc[3 * size_t(nb) + k];
+      smk[k] = sf[k] - (rck ? kr[k] : 0.0);
+    }
+    const double rc = dot3(gm, smk) - M.pcoef[i] * (phi[nb] - phi[o]);
+    ff[fm] = flux + dt * rc;
+  }
+};
+
+// Second half, one cell: the boundary-face fluxes (written here), rhs_p = -(1/dt) sum q over the
+// cell's list in its order (operator-face fluxes read from ff, as FaceFluxRC left them) + deferred
 // non-orthogonal cross terms. Returns |rhs_p| for the max reduction.
 struct FluxRhs {
   DevMesh M;
-  const double *U, *Ub, *grad, *gprc, *phi, *gp;
+  const double *U, *Ub, *grad, *phi, *gp;
   double *ff, *rhs;
   double dt;
   int nonortho;  // deferred cross terms on (mesh not orthogonal)
-  int rck;       // Rhie-Chow subtracts the k-vector (non-orthogonal mesh, rhie_chow_dual off)
   const double* pob;  // cell_projection EXACT: outlet faces carry -dt c_b (p_b - phi_o) (:2629-2636)
   int cproj;
   DM_HD Red<1> operator()(int c) const {
@@ -1836,38 +1951,8 @@ struct FluxRhs {
       const int e = M.bface[s], fm = e >> 1, side = e & 1, oth = M.both[s], idx = M.bidx[s];
       const double* sf = &M.sf[3 * size_t(fm)];
       if (idx >= 0) {
-        const int o = side ? oth : c, nb = side ? c : oth;
-        const double gcf = M.rcgc[idx];
-        double Upl[3], Uf[3];
-        for (int q = 0; q < 3; ++q) Upl[q] = gcf * U[3 * size_t(o) + q] + (1.0 - gcf) * U[3 * size_t(nb) + q];
-        if (fm < M.nint) {
-          const double wb = M.wbl[fm], g = M.gc[fm];
-          const double* ep = &M.eps[3 * size_t(fm)];
-          for (int j = 0; j < 3; ++j) {
-            double ue = g * U[3 * size_t(o) + j] + (1.0 - g) * U[3 * size_t(nb) + j];
-            double gm[3];
-            for (int i = 0; i < 3; ++i) gm[i] = g * grad[DM_G(o, i, j)] + (1.0 - g) * grad[DM_G(nb, i, j)];
-            ue = ue + gm[0] * ep[0] + gm[1] * ep[1] + gm[2] * ep[2];
-            Uf[j] = Upl[j] + wb * (ue - Upl[j]);
-          }
-        } else {
-          Uf[0] = Upl[0]; Uf[1] = Upl[1]; Uf[2] = Upl[2];
-        }
-        double flux = Uf[0] * sf[0] + Uf[1] * sf[1] + Uf[2] * sf[2];
-        double gm[3], smk[3];
-        const double* kr = &M.kint[3 * size_t(idx)];
-        for (int i = 0; i < 3; ++i) {
-          gm[i] = gcf * gprc[3 * size_t(o) + i] + (1.0 - gcf) * gprc[3 * size_t(nb) + i];
-          smk[i] = sf[i] - (rck ? kr[i] : 0.0);
-        }
-        const double rc = dot3(gm, smk) - M.pcoef[idx] * (phi[nb] - phi[o]);
-        flux = flux + dt * rc;
-        if (side) {
-          acc = acc + flux;
-        } else {
-          ff[fm] = flux;
-          acc = acc - flux;
-        }
+        const double flux = ff[fm];
+        acc = side ? acc + flux : acc - flux;
       } else {
         const int b = -1 - oth;
         const int bc = M.bbc[b];

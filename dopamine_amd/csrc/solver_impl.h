@@ -165,6 +165,8 @@ class SolverT {
   double *Cs[4] = {}, *Cs2[4] = {}, *Cb[4] = {}, *gC[4] = {}, *HC[4] = {}, *HC0[4] = {};
   double *phiC = nullptr, *gCprev = nullptr;
   double* phiCk[4] = {};  // each scalar's face values, for the fused transport of T and the scalars
+  double* faceT = nullptr;  // the fused transport's interior-face fluxes, 2 per field and face
+  double* faceU = nullptr;  // the momentum RHS's interior-face fluxes (3 per face, 6 with CDS)
   int* s_mode = nullptr;
   double *s_val = nullptr, *s_flux = nullptr, *s_om = nullptr, *s_ph = nullptr;
   // MOST wall model: heat flux and Obukhov length per boundary face, the configuration's L
@@ -739,6 +741,8 @@ class SolverT {
       }
       s_mode = up(hm); s_val = up(hv); s_flux = up(hf); s_om = up(ho); s_ph = up(hp);
     }
+    if (fused_scalars()) faceT = fresh<double>(2 * size_t((bous ? 1 : 0) + ns) * size_t(std::max(1, nint)));
+    faceU = fresh<double>((p.scheme == SCHEME_CDS ? 6 : 3) * size_t(std::max(1, nint)));
     cproj = p.cell_projection == 1;
     if (cproj) {
       pj_b = fresh<double>(nc); pj_x = fresh<double>(nc); pj_r = fresh<double>(nc); pj_z = fresh<double>(nc);
@@ -963,10 +967,13 @@ class SolverT {
   void explicit_rhs(double dt, int k) {
     double fb[3];
     body_force_now(fb);
-    x.for_each(nc, ExplicitRhs<SCHEME>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, fb[0], fb[1], fb[2], dt,
-                                       RK3_A[k], RK3_B[k], n_wm > 0, phiF, bous ? T : nullptr, prm.beta, prm.T_ref,
-                                       prm.g[0], prm.g[1], prm.g[2], prm.use_rotation ? 1 : 0, prm.omega_rot[0],
-                                       prm.omega_rot[1], prm.omega_rot[2]});
+    x.for_each(nint, FaceRhsU<SCHEME>{M, U, UF, grad, nu_sgs, prm.nu, faceU});
+    ExplicitRhs<SCHEME> e{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, fb[0], fb[1], fb[2], dt,
+                          RK3_A[k], RK3_B[k], n_wm > 0, phiF, bous ? T : nullptr, prm.beta, prm.T_ref,
+                          prm.g[0], prm.g[1], prm.g[2], prm.use_rotation ? 1 : 0, prm.omega_rot[0],
+                          prm.omega_rot[1], prm.omega_rot[2]};
+    e.FV = faceU;
+    x.for_each(nc, e);
   }
   // body_force_now (src/solver/mod_fv_solver.f90:2904-2923): constant or mass-flux force plus the
   // harmonic forcing at the current (stage) time
@@ -1022,25 +1029,29 @@ class SolverT {
       fa[n] = prm.scalar_diffusivity[q]; fp[n] = prm.scalar_pr_t[q];
     }
     const int scheme = bous ? prm.t_scheme : prm.scalar_scheme;
+    const int cds = scheme == SCHEME_CDS ? 1 : 0;
     const auto run = [&](auto nfc) {
       constexpr int NF = decltype(nfc)::value;
-      const auto recon = [&](auto sch) {
+      // face half: interior-face fluxes into faceT (and the boundary face values)
+      const auto faces = [&](auto sch) {
         constexpr int S = decltype(sch)::value;
-        ReconFaceScalarN<S, NF> r{ReconFaceScalar<S>{ReconFace<S>{M, U, U_bnd, UF, grad, nullptr}, fv[0], fb[0], fg[0],
-                                                     fphi[0]},
-                                  {}, {}, {}, {}};
+        FaceRhsTN<S, NF> r{{ReconFaceScalar<S>{ReconFace<S>{M, U, U_bnd, UF, grad, nullptr}, fv[0], fb[0], fg[0], fphi[0]},
+                            {}, {}, {}, {}},
+                           nu_sgs, cds, {}, {}, faceT};
         for (int i = 0; i < RHS_MAX_FIELDS; ++i) {
-          r.phi[i] = fv[i];
-          r.pb[i] = fb[i];
-          r.gphi[i] = fg[i];
-          r.phiF[i] = fphi[i];
+          r.rec.phi[i] = fv[i];
+          r.rec.pb[i] = fb[i];
+          r.rec.gphi[i] = fg[i];
+          r.rec.phiF[i] = fphi[i];
+          r.alpha[i] = fa[i];
+          r.prt[i] = fp[i];
         }
-        x.for_each(nf, r);
+        x.for_each(cds ? nint : nf, r);
       };
-      if (scheme == SCHEME_QUICK) recon(std::integral_constant<int, SCHEME_QUICK>{});
-      if (scheme == SCHEME_VANLEER) recon(std::integral_constant<int, SCHEME_VANLEER>{});
-      if (scheme == SCHEME_KOREN) recon(std::integral_constant<int, SCHEME_KOREN>{});
-      ExplicitRhsTN<NF> e{M, UF, nu_sgs, dt, RK3_A[k], RK3_B[k], scheme == SCHEME_CDS ? 1 : 0};
+      if (scheme == SCHEME_QUICK) faces(std::integral_constant<int, SCHEME_QUICK>{});
+      else if (scheme == SCHEME_KOREN) faces(std::integral_constant<int, SCHEME_KOREN>{});
+      else faces(std::integral_constant<int, SCHEME_VANLEER>{});  // van Leer, or CDS (no reconstruction)
+      ExplicitRhsTN<NF> e{M, UF, nu_sgs, dt, RK3_A[k], RK3_B[k], cds, faceT};
       for (int i = 0; i < RHS_MAX_FIELDS; ++i) {
         e.T[i] = fv[i]; e.Tb[i] = fb[i]; e.gT[i] = fg[i]; e.HT0[i] = f0[i]; e.phiF[i] = fphi[i]; e.qw[i] = fq[i];
         e.HT[i] = fh[i]; e.Tnew[i] = fn[i]; e.alpha[i] = fa[i]; e.prt[i] = fp[i];
@@ -1193,7 +1204,8 @@ class SolverT {
         scalar_gradient(P, nullptr, p_bnd, nullptr);
     }
     const int rck = (!skip_nonortho && !prm.rhie_chow_dual) ? 1 : 0;
-    x.template reduce<1, true>(nc, FluxRhs{M, U, U_bnd, grad, gp_rc, phi_prev, gp, ff, rhs, dt, skip_nonortho ? 0 : 1, rck,
+    x.for_each(npf, FaceFluxRC{M, U, grad, gp_rc, phi_prev, ff, dt, rck});
+    x.template reduce<1, true>(nc, FluxRhs{M, U, U_bnd, grad, phi_prev, gp, ff, rhs, dt, skip_nonortho ? 0 : 1,
                                            outlet_pb, cproj},
                                sk::DivFin{sc});
     x.for_each(nb, PeriodicAntisym{M, ff});

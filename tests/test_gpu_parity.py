@@ -301,13 +301,14 @@ def test_gpu_forcing_matches_oracle():
     assert du < 1e-10 and dp < 1e-10
 
 
-def test_gpu_passive_scalars_match_oracle():
+@pytest.mark.parametrize("scheme", ["VANLEER_TVD", "CDS", "QUICK", "KOREN_TVD"])
+def test_gpu_passive_scalars_match_oracle(scheme):
     """Two passive scalars (Dirichlet and Neumann walls, one flux oscillating) in the channel
-    against the oracle after four steps."""
+    against the oracle after four steps, for each scalar scheme (the multi-field pass)."""
     from test_scalars import scalar_channel
-    case = scalar_channel()
+    case = scalar_channel(scheme)
     mesh = case.make_mesh()
-    print("[gpu] passive scalars", flush=True)
+    print(f"[gpu] passive scalars {scheme}", flush=True)
     g, o = FVSolver(mesh, case.params), oracle.Solver(mesh, case.params)
     y = mesh.cell_centroid[:, 1]
     for s in (g, o):
