@@ -363,6 +363,13 @@ struct LsGradSca {
   }
 };
 
+// LsGradSca on a list of cells only.
+struct LsGradScaAt {
+  LsGradSca g;
+  const int* cells;
+  DM_HD void operator()(int i) const { g(cells[i]); }
+};
+
 // green_gauss_gradient_vector (src/numerics/mod_gradient.f90:172-304), gprev null on the first sweep.
 struct GgGradVec {
   DevMesh M;
@@ -1661,61 +1668,10 @@ struct ExplicitRhsT {
 
 // ------------------------------------------------------------------ explicit right-hand side
 
-// Momentum right-hand side, face half: for each interior face the viscous stress flux of the three
-// components (viscous_rhs_vector's face term, src/numerics/mod_viscous.f90:37-283) into planes 0-2
-// of FV (nint values each) and, with CDS, the convective flux F u_f into planes 3-5 — computed once
-// per face; ExplicitRhs adds them to both cells. KEP and the reconstruction schemes keep their
-// convective term in the cell pass (it reads only U of the neighbour, or the face values).
-template <int SCHEME>
-struct FaceRhsU {
-  DevMesh M;
-  const double *U, *UF, *grad, *nus;
-  double nu;
-  double* FV;
-  DM_HD void operator()(int f) const {
-    const int o = M.fown[f], nb = M.fnb[f];
-    const size_t ni = size_t(M.nint);
-    const double* Uo = &U[3 * size_t(o)];
-    const double* Un = &U[3 * size_t(nb)];
-    const double* sf = &M.sf[3 * size_t(f)];
-    if (SCHEME == 3) {
-      const double Fn = UF[f];
-      const double gcf = M.gc[f];
-      const double* ep = &M.eps[3 * size_t(f)];
-      const double w = M.wbl[f];
-      for (int q = 0; q < 3; ++q) {
-        const double pl = gcf * Uo[q] + (1.0 - gcf) * Un[q];
-        double gm[3];
-        for (int i = 0; i < 3; ++i) gm[i] = gcf * grad[DM_G(o, i, q)] + (1.0 - gcf) * grad[DM_G(nb, i, q)];
-        double phi = pl + dot3(gm, ep);
-        if (w < 1.0) phi = pl + w * (phi - pl);
-        if (!isfinite(phi)) phi = 0.0;
-        FV[(3 + q) * ni + f] = Fn * phi;
-      }
-    }
-    const double gc = M.gc[f];
-    const double nue = nu + gc * nus[o] + (1.0 - gc) * nus[nb];
-    double gf[9];
-    for (int q = 0; q < 9; ++q) gf[q] = gc * grad[9 * size_t(o) + q] + (1.0 - gc) * grad[9 * size_t(nb) + q];
-    double d_on[3], dnp[3];
-    for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
-    const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
-    const double sdd = dot3(sf, d_on) / dlen;
-    for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dlen) * d_on[i];
-    const double wo = ortho_weight(sdd, M.area[f]);
-    for (int q = 0; q < 3; ++q) {
-      const double col = gf[3 * q] * dnp[0] + gf[3 * q + 1] * dnp[1] + gf[3 * q + 2] * dnp[2];
-      const double row = gf[q] * sf[0] + gf[3 + q] * sf[1] + gf[6 + q] * sf[2];
-      FV[q * ni + f] = nue * ((Un[q] - Uo[q]) * wo / dlen + col + row);
-    }
-  }
-};
-
 // advection_rhs_vector_{kep,cds}_soa (src/numerics/mod_advection.f90:1013-1392) + viscous_rhs_vector
 // (src/numerics/mod_viscous.f90:37-283) + compute_explicit_rhs assembly with body force and wall
 // traction (src/solver/mod_fv_solver.f90:1617-1711, 2238-2275) + the RK3W predictor
-// u* = u + dt (alpha H + beta H_prev) (:1326-1331), for one cell (interior faces' viscous, and
-// CDS convective, fluxes from FaceRhsU's FV). Writes H and u* (to Unew).
+// u* = u + dt (alpha H + beta H_prev) (:1326-1331), for one cell. Writes H and u* (to Unew).
 template <int SCHEME>
 struct ExplicitRhs {
   DevMesh M;
@@ -1729,7 +1685,6 @@ struct ExplicitRhs {
   int rot;             // Coriolis -2 om x U (src/solver/mod_fv_solver.f90:1694-1698)
   double om0, om1, om2;
   int part = 0;        // 0: H and the predictor; 1: only the advection term into H; 2: only the viscous term
-  const double* FV = nullptr;  // interior-face fluxes from FaceRhsU
   DM_HD void operator()(int c) const {
     const int nc = M.nc;
     const double iv = M.iv[c];
@@ -1743,7 +1698,6 @@ struct ExplicitRhs {
       const double* sf = &M.sf[3 * size_t(f)];
       if (oth >= 0) {
         const int o = side ? oth : c, nb = side ? c : oth;
-        const size_t ni = size_t(M.nint);
         const double* Uo = &U[3 * size_t(o)];
         const double* Un = &U[3 * size_t(nb)];
         const double Fn = UF[f];
@@ -1761,14 +1715,34 @@ struct ExplicitRhs {
             cv[q] = side ? cv[q] + Fr * phi * iv : cv[q] - Fr * phi * iv;
           }
         } else {
+          const double gcf = M.gc[f];
+          const double* ep = &M.eps[3 * size_t(f)];
+          const double w = M.wbl[f];
           for (int q = 0; q < 3; ++q) {
-            const double fl = FV[(3 + q) * ni + f];
-            cv[q] = side ? cv[q] + fl * iv : cv[q] - fl * iv;
+            const double pl = gcf * Uo[q] + (1.0 - gcf) * Un[q];
+            double gm[3];
+            for (int i = 0; i < 3; ++i) gm[i] = gcf * grad[DM_G(o, i, q)] + (1.0 - gcf) * grad[DM_G(nb, i, q)];
+            double phi = pl + dot3(gm, ep);
+            if (w < 1.0) phi = pl + w * (phi - pl);
+            if (!isfinite(phi)) phi = 0.0;
+            cv[q] = side ? cv[q] + Fn * phi * iv : cv[q] - Fn * phi * iv;
           }
         }
         // viscous stress
+        const double gc = M.gc[f];
+        const double nue = nu + gc * nus[o] + (1.0 - gc) * nus[nb];
+        double gf[9];
+        for (int q = 0; q < 9; ++q) gf[q] = gc * grad[9 * size_t(o) + q] + (1.0 - gc) * grad[9 * size_t(nb) + q];
+        double d_on[3], dnp[3];
+        for (int i = 0; i < 3; ++i) d_on[i] = M.ctr[3 * size_t(nb) + i] - M.ctr[3 * size_t(o) + i];
+        const double dlen = sqrt(d_on[0] * d_on[0] + d_on[1] * d_on[1] + d_on[2] * d_on[2]) + SMALL;
+        const double sdd = dot3(sf, d_on) / dlen;
+        for (int i = 0; i < 3; ++i) dnp[i] = sf[i] - (sdd / dlen) * d_on[i];
+        const double wo = ortho_weight(sdd, M.area[f]);
         for (int q = 0; q < 3; ++q) {
-          const double st = FV[q * ni + f];
+          const double col = gf[3 * q] * dnp[0] + gf[3 * q + 1] * dnp[1] + gf[3 * q + 2] * dnp[2];
+          const double row = gf[q] * sf[0] + gf[3 + q] * sf[1] + gf[6 + q] * sf[2];
+          const double st = nue * ((Un[q] - Uo[q]) * wo / dlen + col + row);
           vs[q] = side ? vs[q] - st * iv : vs[q] + st * iv;
         }
       } else {

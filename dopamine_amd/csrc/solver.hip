@@ -565,14 +565,11 @@ static int op_rhs_vector(void* h, int part, int scheme, double nu, const double*
     double* H = b.zeros(3 * z.nc);
     double* Un = b.zeros(3 * z.nc);
     double* phiF = (part == 1 && scheme != dm::SCHEME_CDS && scheme != dm::SCHEME_KEP) ? b.zeros(3 * z.nf) : nullptr;
-    double* FV = b.zeros(6 * size_t(std::max(1, s->nint)));  // interior-face fluxes (FaceRhsU)
     const auto run = [&](auto sch) {
       constexpr int S = decltype(sch)::value;
       if (phiF) s->x.for_each(s->nf, dm::ReconFace<S>{s->M, u, ub, uf, g, phiF});
-      s->x.for_each(s->nint, dm::FaceRhsU<S>{s->M, u, uf, g, nus, part == 2 ? nu : 0.0, FV});
       dm::ExplicitRhs<S> f{s->M, u, ub, uf, g, nus, nullptr, H, H, Un, part == 2 ? nu : 0.0, 0.0, 0.0, 0.0, 0.0,
                            0.0, 0.0, 0, phiF, nullptr, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, 0.0, part};
-      f.FV = FV;
       s->x.for_each(s->nc, f);
     };
     switch (part == 2 ? int(dm::SCHEME_CDS) : scheme) {

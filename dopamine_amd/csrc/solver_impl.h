@@ -166,7 +166,6 @@ class SolverT {
   double *phiC = nullptr, *gCprev = nullptr;
   double* phiCk[4] = {};  // each scalar's face values, for the fused transport of T and the scalars
   double* faceT = nullptr;  // the fused transport's interior-face fluxes, 2 per field and face
-  double* faceU = nullptr;  // the momentum RHS's interior-face fluxes (3 per face, 6 with CDS)
   int* s_mode = nullptr;
   double *s_val = nullptr, *s_flux = nullptr, *s_om = nullptr, *s_ph = nullptr;
   // MOST wall model: heat flux and Obukhov length per boundary face, the configuration's L
@@ -186,6 +185,8 @@ class SolverT {
          *dyn_lmh = nullptr;
   int n_amdw = 0;
   int *amdw_cell = nullptr, *amdw_off = nullptr, *amdw_face = nullptr;
+  int* tbc_cell = nullptr;  // owners of the boundary faces whose temperature extrapolates with grad T
+  int n_tbc = 0;
   // relative residual reduction of one FP32 sweep (DOPAMINE_MIXED_TAU overrides, for tuning)
   // Multigrid set-up: AmgCfg's defaults; DOPAMINE_AMG_THETA / _THETA0 / _ALPHA / _COARSE override
   // the strength thresholds (all levels / the first), the coarse-correction scaling and the
@@ -667,6 +668,21 @@ class SolverT {
         amdw_face = up(faces);
       }
     }
+    // cells whose temperature gradient a boundary value reads (BcTemperature extrapolates on every
+    // face that is not an inlet, a fixed temperature or periodic), each once
+    {
+      std::vector<char> mark(nc, 0);
+      std::vector<int> cells;
+      for (int b = 0; b < nb; ++b) {
+        const int bc = bbc[b];
+        if (bc == BC_INLET || bc == BC_FIXEDTEMP || bc == BC_PERIODIC || mark[bown[b]]) continue;
+        mark[bown[b]] = 1;
+        cells.push_back(bown[b]);
+      }
+      std::sort(cells.begin(), cells.end());
+      n_tbc = int(cells.size());
+      if (n_tbc) tbc_cell = up(cells);
+    }
     // outlet pressure per boundary face (patch p_val; no backflow traction)
     std::vector<double> opb(std::max(1, nb), 0.0);
     for (int b = 0; b < nb; ++b) opb[b] = patchP_h[bpatch[b]];
@@ -742,7 +758,6 @@ class SolverT {
       s_mode = up(hm); s_val = up(hv); s_flux = up(hf); s_om = up(ho); s_ph = up(hp);
     }
     if (fused_scalars()) faceT = fresh<double>(2 * size_t((bous ? 1 : 0) + ns) * size_t(std::max(1, nint)));
-    faceU = fresh<double>((p.scheme == SCHEME_CDS ? 6 : 3) * size_t(std::max(1, nint)));
     cproj = p.cell_projection == 1;
     if (cproj) {
       pj_b = fresh<double>(nc); pj_x = fresh<double>(nc); pj_r = fresh<double>(nc); pj_z = fresh<double>(nc);
@@ -967,13 +982,10 @@ class SolverT {
   void explicit_rhs(double dt, int k) {
     double fb[3];
     body_force_now(fb);
-    x.for_each(nint, FaceRhsU<SCHEME>{M, U, UF, grad, nu_sgs, prm.nu, faceU});
-    ExplicitRhs<SCHEME> e{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, fb[0], fb[1], fb[2], dt,
-                          RK3_A[k], RK3_B[k], n_wm > 0, phiF, bous ? T : nullptr, prm.beta, prm.T_ref,
-                          prm.g[0], prm.g[1], prm.g[2], prm.use_rotation ? 1 : 0, prm.omega_rot[0],
-                          prm.omega_rot[1], prm.omega_rot[2]};
-    e.FV = faceU;
-    x.for_each(nc, e);
+    x.for_each(nc, ExplicitRhs<SCHEME>{M, U, U_bnd, UF, grad, nu_sgs, tau, H0, H, U2, prm.nu, fb[0], fb[1], fb[2], dt,
+                                       RK3_A[k], RK3_B[k], n_wm > 0, phiF, bous ? T : nullptr, prm.beta, prm.T_ref,
+                                       prm.g[0], prm.g[1], prm.g[2], prm.use_rotation ? 1 : 0, prm.omega_rot[0],
+                                       prm.omega_rot[1], prm.omega_rot[2]});
   }
   // body_force_now (src/solver/mod_fv_solver.f90:2904-2923): constant or mass-flux force plus the
   // harmonic forcing at the current (stage) time
@@ -1094,9 +1106,20 @@ class SolverT {
     x.for_each(nc, DynNu{M, dyn_lmh, grad, nu_sgs, dt_stab});
   }
 
-  void gradients(bool with_sgs, double sgs_dt) {
+  // bc_only (least squares, no SGS): the velocity gradient everywhere and the temperature gradient
+  // only where a boundary value reads it before the next full call; the mid-stage call of the
+  // first two stages, whose scalar gradients nothing else reads before the next stage recomputes
+  // them (the last stage computes every field, so the state after a step is complete).
+  void gradients(bool with_sgs, double sgs_dt, bool bc_only = false) {
     const bool dyn = with_sgs && prm.sgs == 4;
     const int sgs = (with_sgs && !dyn) ? prm.sgs : -1;
+    if (prm.gradient == 0 && bc_only && !with_sgs) {
+      x.tbegin(K_LS_GRAD);
+      x.for_each(nc, LsGradAll<0>{M, U, U_bnd, grad, nu_sgs, -1, 0.0, 0.0, 0.0, 0.0, {}, {}, {}});
+      x.tend(K_LS_GRAD);
+      if (bous && n_tbc) x.for_each(n_tbc, LsGradScaAt{LsGradSca{M, T, nullptr, T_bnd, nullptr, gT}, tbc_cell});
+      return;
+    }
     if (prm.gradient == 0) {  // least squares: every field in one pass (LsGradAll)
       const double* sa[LS_MAX_SCALARS] = {};
       const double* sb[LS_MAX_SCALARS] = {};
@@ -1489,7 +1512,7 @@ class SolverT {
       else
         correct_velocity(gdt, false, true, gp_rc);
       velocity_bc_both();
-      gradients(false, 0.0);
+      gradients(false, 0.0, k < 2);
       x.copy(phi_prev, P, nc);
       x.copy(phi_prev_bnd, p_bnd, std::max(1, nb));
       {
