@@ -743,6 +743,10 @@ struct PressurePolicy {
   virtual ~PressurePolicy() = default;
   virtual int solve(const EllOp& A, const double* b, double* x, double rtol, double atol, const double* atol_dev,
                     int maxit, bool nullspace, int* its, double* resid) = 0;
+  // A loose inner solve used as a preconditioner (the EXACT cell projection's): the policy's
+  // current preconditioner, without its bookkeeping (no samples, no fallbacks, no regression count).
+  virtual int precondition(const EllOp& A, const double* b, double* x, double rtol, int maxit, bool nullspace,
+                           int* its, double* resid) = 0;
 };
 
 // Receives the residual of a running Jacobi CG solve at its batch boundaries (r[c * stride]).

@@ -792,10 +792,11 @@ DM_INLINE double most_psi_h(double zeta) {
   return 2.0 * log((1.0 + sqrt(1.0 - 16.0 * zeta)) / 2.0);
 }
 // most_residuals (mod_abl_wall.f90:260-284)
+// (log(y / z0m) and log(y / z0h), fixed along the iteration, come in as lym and lyh)
 DM_INLINE void most_residuals(double us, double ts, double L, double u_par, double T_o, double T_s, double y,
-                              double z0m, double z0h, double* Fu, double* FT) {
-  double dm = log(y / z0m) - most_psi_m(y / L) + most_psi_m(z0m / L);
-  double dh = log(y / z0h) - most_psi_h(y / L) + most_psi_h(z0h / L);
+                              double z0m, double z0h, double lym, double lyh, double* Fu, double* FT) {
+  double dm = lym - most_psi_m(y / L) + most_psi_m(z0m / L);
+  double dh = lyh - most_psi_h(y / L) + most_psi_h(z0h / L);
   if (fabs(dm) < SMALL) dm = copysign(SMALL, dm);
   if (fabs(dh) < SMALL) dh = copysign(SMALL, dh);
   *Fu = us - MOST_KAPPA * u_par / dm;
@@ -812,19 +813,20 @@ DM_INLINE double most_L_from(double us, double ts, double T_ref) {
 // through L(u*, T*), at most 20 iterations, stop when |Fu| and |FT| < 1e-8.
 DM_INLINE void most_iterate(double u_par, double T_o, double T_s, double y, double z0m, double z0h, double L_init,
                             double* us_out, double* ts_out, double* L_out) {
-  double us = fmax(SMALL, MOST_KAPPA * u_par / fmax(SMALL, log(y / z0m)));
+  const double lym = log(y / z0m), lyh = log(y / z0h);
+  double us = fmax(SMALL, MOST_KAPPA * u_par / fmax(SMALL, lym));
   double ts = 0.1 * (T_o - T_s);
   double L = fabs(L_init) < SMALL ? MOST_L_CLIP : L_init;
   for (int it = 0; it < 20; ++it) {
     L = copysign(fmax(fabs(L), 0.001), L);
     if (fabs(L) > MOST_L_CLIP) L = copysign(MOST_L_CLIP, L);
     double Fu, FT, Fp, Tp;
-    most_residuals(us, ts, L, u_par, T_o, T_s, y, z0m, z0h, &Fu, &FT);
+    most_residuals(us, ts, L, u_par, T_o, T_s, y, z0m, z0h, lym, lyh, &Fu, &FT);
     if (fabs(Fu) < 1.0e-8 && fabs(FT) < 1.0e-8) break;
     const double du = fmax(1.0e-6, 1.0e-6 * fabs(us)), dT = fmax(1.0e-6, 1.0e-6 * fabs(ts));
-    most_residuals(us + du, ts, most_L_from(us + du, ts, T_o), u_par, T_o, T_s, y, z0m, z0h, &Fp, &Tp);
+    most_residuals(us + du, ts, most_L_from(us + du, ts, T_o), u_par, T_o, T_s, y, z0m, z0h, lym, lyh, &Fp, &Tp);
     const double a11 = (Fp - Fu) / du, a21 = (Tp - FT) / du;
-    most_residuals(us, ts + dT, most_L_from(us, ts + dT, T_o), u_par, T_o, T_s, y, z0m, z0h, &Fp, &Tp);
+    most_residuals(us, ts + dT, most_L_from(us, ts + dT, T_o), u_par, T_o, T_s, y, z0m, z0h, lym, lyh, &Fp, &Tp);
     const double a12 = (Fp - Fu) / dT, a22 = (Tp - FT) / dT;
     const double den = a11 * a22 - a12 * a21;
     if (fabs(den) < SMALL) break;
@@ -894,6 +896,15 @@ enum { SCHEME_VANLEER = 1, SCHEME_KOREN = 2, SCHEME_CDS = 3, SCHEME_KEP = 4, SCH
 struct ReconCtx {
   int up, down;
   double x_up[3], x_down[3], up_shift[3], dir_hat[3], s_face, s_down;
+};
+
+// What a TVD face value needs of the face's upwind context, which depends on the mesh and on the
+// sign of the face flux only: up / down cells, the far-upwind cell, the face's and the downwind
+// cell's distance from the upwind cell along the upwind direction and the far-upwind cell's
+// distance behind it (tvd1's s_up). ok = 0: no context (the boundary value is the face value).
+struct TvdCtx {
+  int up, down, far, ok;
+  double s_face, s_down, s_up;
 };
 
 template <int SCHEME>
@@ -1267,6 +1278,24 @@ struct ReconFaceScalar {
     }
     return ReconFace<SCHEME>::clampv(pU + a1 * c.s_face + a2 * c.s_face * c.s_face, fmin(pU, pD), fmax(pU, pD));
   }
+  // tvd1 from a precomputed context (the same arithmetic on the same distances)
+  DM_HD double tvd1p(const TvdCtx& t) const {
+    const double pU = phi[t.up], pD = phi[t.down];
+    const double slope_dn = (pD - pU) / fmax(t.s_down, SMALL);
+    double psi = 0.0;
+    if (t.far >= 0 && fabs(slope_dn) > SMALL) {
+      if (t.s_up > SMALL) {
+        const double r = ((pU - phi[t.far]) / t.s_up) / slope_dn;
+        if (SCHEME == SCHEME_KOREN) {
+          psi = fmax(0.0, fmin(fmin(2.0 * r, (1.0 + 2.0 * r) / 3.0), 2.0));
+        } else {
+          psi = (r + fabs(r)) / (1.0 + fabs(r));
+          psi = fmax(0.0, fmin(psi, 2.0));
+        }
+      }
+    }
+    return ReconFace<SCHEME>::clampv(pU + psi * slope_dn * t.s_face, fmin(pU, pD), fmax(pU, pD));
+  }
   DM_HD double tvd1(const ReconCtx& c, int far_up) const {
     const DevMesh& M = R.M;
     const double pU = phi[c.up], pD = phi[c.down];
@@ -1323,6 +1352,7 @@ struct ReconFaceScalarN {
   const double* pb[RHS_MAX_FIELDS];
   const double* gphi[RHS_MAX_FIELDS];
   double* phiF[RHS_MAX_FIELDS];
+  const TvdCtx* tctx = nullptr;  // TVD schemes: the precomputed face contexts (TvdCtxBuild)
   DM_HD void values(int f, double* out) const {
     const DevMesh& M = base.R.M;
     const int o = M.fown[f], nb = M.fnb[f];
@@ -1333,6 +1363,24 @@ struct ReconFaceScalarN {
     } else {
       b = f - M.nint;
       Fn = is_solid(M.bbc[b]) ? 0.0 : base.R.UF[f];
+    }
+    if (SCHEME != SCHEME_QUICK && tctx) {
+      const TvdCtx t = tctx[2 * size_t(f) + (Fn >= 0.0 ? 0 : 1)];
+#pragma unroll
+      for (int i = 0; i < NF; ++i) {
+        ReconFaceScalar<SCHEME> g = base;
+        g.phi = phi[i];
+        double v;
+        if (t.ok) {
+          v = g.tvd1p(t);
+          if (!isfinite(v)) v = (Fn >= 0.0 || nb < 0) ? phi[i][o] : phi[i][nb];
+        } else {
+          v = pb[i][b];
+        }
+        if (!isfinite(v)) v = 0.0;
+        out[i] = v;
+      }
+      return;
     }
     ReconCtx c;
     const bool ok = base.R.context(f, Fn, c);
@@ -1359,6 +1407,35 @@ struct ReconFaceScalarN {
     values(f, v);
 #pragma unroll
     for (int i = 0; i < NF; ++i) phiF[i][f] = v[i];
+  }
+};
+
+// The TVD face contexts of every face for both flux signs (ctx[2 f]: F >= 0, ctx[2 f + 1]:
+// F < 0), from ReconFace's context and far-upwind search; built once per mesh.
+template <int SCHEME>
+struct TvdCtxBuild {
+  ReconFace<SCHEME> R;
+  TvdCtx* ctx;
+  DM_HD void operator()(int f) const {
+    const DevMesh& M = R.M;
+    for (int sg = 0; sg < 2; ++sg) {
+      TvdCtx t{0, 0, -1, 0, 0.0, 0.0, 0.0};
+      ReconCtx c;
+      if (R.context(f, sg == 0 ? 1.0 : -1.0, c)) {
+        t.ok = 1;
+        t.up = c.up;
+        t.down = c.down;
+        t.s_face = c.s_face;
+        t.s_down = c.s_down;
+        t.far = R.far_upwind(c);
+        if (t.far >= 0) {
+          double d[3];
+          for (int i = 0; i < 3; ++i) d[i] = M.ctr[3 * size_t(t.far) + i] + c.up_shift[i] - c.x_up[i];
+          t.s_up = -dot3(d, c.dir_hat);
+        }
+      }
+      ctx[2 * size_t(f) + sg] = t;
+    }
   }
 };
 
@@ -2118,12 +2195,13 @@ struct ProjApply {
   }
 };
 
-// r_old = r; x += alpha p; r -= alpha q; returns |r| for max|r|.
+// r_old = r; x += alpha p; r -= alpha q; returns |r| for max|r| (alpha read from the device).
 struct ProjUpdate {
   const double *p, *q;
   double *x, *r, *rold;
-  double alpha;
+  const double* alpha_dev;
   DM_HD Red<1> operator()(int c) const {
+    const double alpha = *alpha_dev;
     rold[c] = r[c];
     x[c] = x[c] + alpha * p[c];
     const double rc = r[c] - alpha * q[c];
@@ -2145,12 +2223,12 @@ struct ProjBeta {
   }
 };
 
-// p = z + beta p.
+// p = z + beta p (beta read from the device).
 struct ProjDir {
   const double* z;
   double* p;
-  double beta;
-  DM_HD void operator()(int c) const { p[c] = z[c] + beta * p[c]; }
+  const double* beta_dev;
+  DM_HD void operator()(int c) const { p[c] = z[c] + *beta_dev * p[c]; }
 };
 
 // u -= V^-1 B^T x.

@@ -1035,6 +1035,16 @@ class LearnedPolicy : public PressurePolicy {
     }
     return rc;
   }
+  int precondition(const EllOp& A, const double* b, double* x, double rtol, int maxit, bool nullspace, int* its,
+                   double* resid) override {
+    if (phase != USE) return cg_->solve(A, b, x, rtol, 1.0e-50, nullptr, maxit, nullspace, its, resid);
+    cg_->set_fcg(&pc_);
+    const int rc = cg_->solve(A, b, x, rtol, 1.0e-50, nullptr, maxit, nullspace, its, resid);
+    cg_->set_fcg(nullptr);
+    ++precond_solves;
+    precond_its += *its;
+    return rc;
+  }
   void train_now() {
     loss_hist.assign(size_t(std::max(0, cfg_.iters)), 0.0);
     hipEvent_t e0, e1;
@@ -1056,6 +1066,7 @@ class LearnedPolicy : public PressurePolicy {
   Phase phase = COLLECT;
   int collected = 0, learned_solves = 0, fallbacks = 0, regress_run = 0;
   int64_t learned_its = 0;
+  int64_t precond_solves = 0, precond_its = 0;  // precondition() calls that ran the learned FCG
   double ref_its = 0.0, train_ms = 0.0;
   std::vector<double> loss_hist;
 

@@ -289,6 +289,16 @@ int dopamine_solver_learned_info(void* h, double out[8]) {
   });
 }
 
+int dopamine_solver_learned_precond_info(void* h, int64_t out[2]) {
+  return guard([&] {
+    auto* lp = dynamic_cast<dm::mp::LearnedPolicy*>(static_cast<Solver*>(h)->policy.get());
+    if (!lp) return -1;
+    out[0] = lp->precond_solves;
+    out[1] = lp->precond_its;
+    return 0;
+  });
+}
+
 int dopamine_solver_multigrid_info(void* h, int64_t* out, int max) {
   return guard([&] {
     auto* s = static_cast<Solver*>(h);

@@ -34,6 +34,9 @@ struct SolverScalars {
   double st[12];
   double vmin, vtot, gdt, pad;
   double pj[4];  // exact cell projection: reductions of the current iteration
+  // exact cell projection: the flexible PCG's scalars, kept and updated on the device
+  double pj_rz, pj_en_tot, pj_en_new, pj_bn, pj_alpha, pj_beta;
+  int pj_done, pj_pad;
 };
 
 namespace sk {
@@ -47,6 +50,53 @@ struct ProjFin {  // exact cell projection: store K reduction results from pj[at
   int at;
   DM_HD void operator()(const Red<K>& t) const {
     for (int i = 0; i < K; ++i) s->pj[at + i] = t.v[i];
+  }
+};
+// exact cell projection: b = B u + bconst, |b|_inf (the relative test's scale)
+struct ProjBnFin {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<1>& t) const {
+    s->pj[0] = t.v[0];
+    s->pj_bn = t.v[0];
+    s->pj_done = 0;
+    s->pj_en_tot = 0.0;
+  }
+};
+// r.z of the first direction
+struct ProjRzFin {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<1>& t) const { s->pj_rz = t.v[0]; }
+};
+// p.q: alpha = (r.z) / (p.q) and the energy norm of the step; a vanishing p.q ends the loop
+struct ProjAlphaFin {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<1>& t) const {
+    const double pq = t.v[0];
+    if (fabs(pq) <= DBL_MIN) {
+      s->pj_done = 1;
+      return;
+    }
+    s->pj_alpha = s->pj_rz / pq;
+    s->pj_en_new = s->pj_alpha * s->pj_alpha * pq;
+    s->pj_en_tot = s->pj_en_tot + s->pj_en_new;
+  }
+};
+// max|r| after the update: stop at 1e-12 |b|_inf, or (from the second iteration) when the step's
+// energy falls below tol^2 of the total
+struct ProjStopFin {
+  SolverScalars* s;
+  int it;
+  double tol;
+  DM_HD void operator()(const Red<1>& t) const {
+    if (t.v[0] <= 1.0e-12 * s->pj_bn || (it >= 2 && s->pj_en_new <= tol * tol * s->pj_en_tot)) s->pj_done = 1;
+  }
+};
+// flexible beta = max(0, (r - r_old).z / (r.z)_old) and the new r.z
+struct ProjBetaFin {
+  SolverScalars* s;
+  DM_HD void operator()(const Red<2>& t) const {
+    s->pj_beta = fmax(0.0, t.v[0] / s->pj_rz);
+    s->pj_rz = t.v[1];
   }
 };
 struct AtolFin {
@@ -166,6 +216,7 @@ class SolverT {
   double *phiC = nullptr, *gCprev = nullptr;
   double* phiCk[4] = {};  // each scalar's face values, for the fused transport of T and the scalars
   double* faceT = nullptr;  // the fused transport's interior-face fluxes, 2 per field and face
+  TvdCtx* tvd_ctx = nullptr;  // the fused transport's TVD face contexts (both flux signs)
   int* s_mode = nullptr;
   double *s_val = nullptr, *s_flux = nullptr, *s_om = nullptr, *s_ph = nullptr;
   // MOST wall model: heat flux and Obukhov length per boundary face, the configuration's L
@@ -757,7 +808,16 @@ class SolverT {
       }
       s_mode = up(hm); s_val = up(hv); s_flux = up(hf); s_om = up(ho); s_ph = up(hp);
     }
-    if (fused_scalars()) faceT = fresh<double>(2 * size_t((bous ? 1 : 0) + ns) * size_t(std::max(1, nint)));
+    if (fused_scalars()) {
+      faceT = fresh<double>(2 * size_t((bous ? 1 : 0) + ns) * size_t(std::max(1, nint)));
+      const int sch = bous ? p.t_scheme : p.scalar_scheme;
+      if (sch == SCHEME_VANLEER || sch == SCHEME_KOREN) {
+        tvd_ctx = fresh<TvdCtx>(2 * size_t(nf));
+        x.for_each(nf, TvdCtxBuild<SCHEME_VANLEER>{ReconFace<SCHEME_VANLEER>{M, nullptr, nullptr, nullptr, nullptr,
+                                                                             nullptr},
+                                                   tvd_ctx});
+      }
+    }
     cproj = p.cell_projection == 1;
     if (cproj) {
       pj_b = fresh<double>(nc); pj_x = fresh<double>(nc); pj_r = fresh<double>(nc); pj_z = fresh<double>(nc);
@@ -1048,7 +1108,7 @@ class SolverT {
       const auto faces = [&](auto sch) {
         constexpr int S = decltype(sch)::value;
         FaceRhsTN<S, NF> r{{ReconFaceScalar<S>{ReconFace<S>{M, U, U_bnd, UF, grad, nullptr}, fv[0], fb[0], fg[0], fphi[0]},
-                            {}, {}, {}, {}},
+                            {}, {}, {}, {}, tvd_ctx},
                            nu_sgs, cds, {}, {}, faceT};
         for (int i = 0; i < RHS_MAX_FIELDS; ++i) {
           r.rec.phi[i] = fv[i];
@@ -1305,38 +1365,40 @@ class SolverT {
     x.zero(pj_z, nc);
     int its = 0;
     double res = 0.0;
-    cg.solve(A, pj_r, pj_z, 0.05, 1.0e-50, nullptr, 50, n_dir == 0, &its, &res);
+    if (policy)
+      policy->precondition(A, pj_r, pj_z, 0.05, 50, n_dir == 0, &its, &res);
+    else
+      cg.solve(A, pj_r, pj_z, 0.05, 1.0e-50, nullptr, 50, n_dir == 0, &its, &res);
+  }
+  // The loop's scalars (r.z, alpha, beta, the energy norms, the stop flag) live on the device: the
+  // reductions' final kernels update them and the passes read them, so the host reads one flag per
+  // iteration, to decide whether to run the preconditioner solve.
+  int pj_done_flag() {
+    SolverScalars h;
+    x.download(&h, sc, 1);
+    return h.pj_done;
   }
   void project_exact(double dt) {
     x.for_each(nc, ProjBoundary{M, U_bnd, outlet_pb, U, pj_bc, dt});
-    x.template reduce<1, true>(nc, ProjInit{M, U, pj_bc, pj_b, pj_r}, sk::ProjFin<1>{sc, 0});
+    x.template reduce<1, true>(nc, ProjInit{M, U, pj_bc, pj_b, pj_r}, sk::ProjBnFin{sc});
     const double bn = pj_get(0);
     int n_it = 0;
     if (bn > DBL_MIN) {
       x.zero(pj_x, nc);
       pj_precond();
-      x.template reduce<1, false>(nc, ProjFirstDir{pj_r, pj_z, pj_p}, sk::ProjFin<1>{sc, 0});
-      double rz = pj_get(0), en_tot = 0.0;
+      x.template reduce<1, false>(nc, ProjFirstDir{pj_r, pj_z, pj_p}, sk::ProjRzFin{sc});
       const double tol = prm.cell_projection_tol;
+      const int* done = &sc->pj_done;
       for (int it = 1; it <= 100; ++it) {
         n_it = it;
         x.for_each(nc, ProjAdjoint{M, pj_p, pj_v});
-        x.template reduce<1, false>(nc, ProjApply{M, pj_v, pj_p, pj_q}, sk::ProjFin<1>{sc, 0});
-        const double pq = pj_get(0);
-        if (std::fabs(pq) <= DBL_MIN) break;
-        const double alpha = rz / pq;
-        const double en_new = alpha * alpha * pq;
-        en_tot = en_tot + en_new;
-        x.template reduce<1, true>(nc, ProjUpdate{pj_p, pj_q, pj_x, pj_r, pj_ro, alpha}, sk::ProjFin<1>{sc, 0});
-        const double rn = pj_get(0);
-        if (rn <= 1.0e-12 * bn || (it >= 2 && en_new <= tol * tol * en_tot)) break;
+        x.template reduce<1, false>(nc, ProjApply{M, pj_v, pj_p, pj_q}, sk::ProjAlphaFin{sc});
+        x.template reduce<1, true>(nc, ProjUpdate{pj_p, pj_q, pj_x, pj_r, pj_ro, &sc->pj_alpha},
+                                   sk::ProjStopFin{sc, it, tol}, done);
+        if (pj_done_flag()) break;
         pj_precond();
-        x.template reduce<2, false>(nc, ProjBeta{pj_r, pj_ro, pj_z}, sk::ProjFin<2>{sc, 0});
-        SolverScalars h;
-        x.download(&h, sc, 1);
-        const double beta = std::fmax(0.0, h.pj[0] / rz);
-        rz = h.pj[1];
-        x.for_each(nc, ProjDir{pj_z, pj_p, beta});
+        x.template reduce<2, false>(nc, ProjBeta{pj_r, pj_ro, pj_z}, sk::ProjBetaFin{sc});
+        x.for_each(nc, ProjDir{pj_z, pj_p, &sc->pj_beta});
       }
       x.for_each(nc, ProjCorrect{M, pj_x, U});
     }

@@ -109,6 +109,7 @@ def _bind(L):
     L.dopamine_solver_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.dopamine_solver_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _P(_i64), _P(_d)]
     L.dopamine_solver_learned_info.argtypes = [ctypes.c_void_p, _d * 8]
+    L.dopamine_solver_learned_precond_info.argtypes = [ctypes.c_void_p, _i64 * 2]
     L.dopamine_solver_projection_info.argtypes = [ctypes.c_void_p, _i64 * 2]
     L.dopamine_solver_multigrid_info.argtypes = [ctypes.c_void_p, _P(_i64), ctypes.c_int]
     L.dopamine_solver_cell_divergence.argtypes = [ctypes.c_void_p, _d * 2]
@@ -304,6 +305,14 @@ class FVSolver:
         d = dict(zip(self.LEARNED_INFO, list(out)))
         d["phase"] = ("collecting", "in use", "off")[int(d["phase"])]
         return d
+
+    def learned_precond_info(self):
+        """Exact cell projection with precond LEARNED: its preconditioner solves that ran the learned
+        flexible CG and their iterations, or None."""
+        out = (_i64 * 2)()
+        if self.L.dopamine_solver_learned_precond_info(self.h, out) != 0:
+            return None
+        return {"solves": out[0], "iterations": out[1]}
 
     def proj_counters(self):
         """Exact cell projection: projections and flexible-PCG iterations so far."""

@@ -41,7 +41,7 @@ typedef struct dopamine_params {
   int32_t scheme;            /* 1 VANLEER_TVD, 2 KOREN_TVD, 3 CDS, 4 KEP, 5 QUICK (src/mod_parameters.f90:54-60) */
   int32_t gradient;          /* 0 LEAST_SQUARES, 1 GREEN_GAUSS */
   int32_t gg_iterations;
-  int32_t sgs;               /* 0 none, 1 Vreman, 2 Smagorinsky, 3 AMD */
+  int32_t sgs;               /* 0 none, 1 Vreman, 2 Smagorinsky, 3 AMD, 4 dynamic Smagorinsky */
   double c_vreman;           /* reference C_VREMAN = 0.025 */
   double cs_smag;            /* 0.1 */
   int32_t wall_model;        /* 0 log-law (Reichardt), 1 none, 2 MOST (Monin-Obukhov, rough wall) */
@@ -71,7 +71,8 @@ typedef struct dopamine_params {
    * pressure solves run Jacobi CG and keep residual snapshots (learn_capacity of them), the
    * message-passing network is then trained (learn_iters Adam steps of learn_batch residuals,
    * rate learn_lr, seed learn_seed) and later solves run flexible CG with it, falling back to
-   * Jacobi CG when a solve fails or needs more iterations than the Jacobi solves did. */
+   * Jacobi CG when a solve fails or needs more iterations than the Jacobi solves did; 2 the
+   * aggregation-multigrid V-cycle in flexible CG (in FP32 inside FP64 refinement with precision 1). */
   int32_t precond;
   int32_t learn_collect, learn_capacity, learn_iters, learn_batch, learn_seed;
   double learn_lr;
@@ -186,6 +187,10 @@ int dopamine_solver_nonortho_info(void *solver, int32_t out[3]);
  * collected, residual samples kept, learned solves, their CG iterations, fallbacks to Jacobi CG,
  * mean Jacobi CG iterations per solve, training time in ms}. -1 when precond != 1. */
 int dopamine_solver_learned_info(void *solver, double out[8]);
+/* Preconditioner solves of the exact cell projection that ran the learned flexible CG (the policy
+ * in use): {solves, their CG iterations}. -1 when precond != 1. No reference counterpart (the
+ * reference's projection preconditions with its PETSc pressure solve). */
+int dopamine_solver_learned_precond_info(void *solver, int64_t out[2]);
 /* Exact cell projection counters: out = {projections, flexible-PCG iterations} so far. */
 int dopamine_solver_projection_info(void *solver, int64_t out[2]);
 /* Multigrid hierarchy of the pressure solve (precond 2): out[0] levels L, out[1] 1 when the first
