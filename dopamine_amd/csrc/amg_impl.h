@@ -251,8 +251,8 @@ inline void amg_dense_inverse(const AmgOp& C, std::vector<float>& inv, bool* sin
 
 struct AmgCfg {
   double theta = 0.08;       // strength threshold
-  double omega = 2.0 / 3.0;  // Jacobi damping
-  double alpha = 1.5;        // coarse-correction scaling
+  double omega = 0.8;        // Jacobi damping
+  double alpha = 1.8;        // coarse-correction scaling
   // Aggressive first coarsening: when theta leaves the first level with fewer than min_ratio0
   // fine cells per aggregate (strongly anisotropic cells whose weak couplings all fall below
   // theta, so the aggregates are thin lines), the first level is aggregated again with theta0
@@ -260,7 +260,7 @@ struct AmgCfg {
   // aggregates need. theta0 < 0 turns this off.
   double theta0 = 0.02;
   double min_ratio0 = 3.5;
-  double alpha_aggr = 2.0;
+  double alpha_aggr = 2.4;
   int coarse_max = 400;      // cells of the densely solved level
   int max_levels = 16;
 };

@@ -247,6 +247,8 @@ class SolverT {
     if (const char* e = std::getenv("DOPAMINE_AMG_THETA")) c.theta = std::atof(e);
     if (const char* e = std::getenv("DOPAMINE_AMG_THETA0")) c.theta0 = std::atof(e);
     if (const char* e = std::getenv("DOPAMINE_AMG_ALPHA")) c.alpha = std::atof(e);
+    if (const char* e = std::getenv("DOPAMINE_AMG_ALPHA_AGGR")) c.alpha_aggr = std::atof(e);
+    if (const char* e = std::getenv("DOPAMINE_AMG_OMEGA")) c.omega = std::atof(e);
     if (const char* e = std::getenv("DOPAMINE_AMG_COARSE")) c.coarse_max = std::max(1, std::atoi(e));
     return c;
   }

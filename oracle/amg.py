@@ -127,8 +127,8 @@ class Hierarchy:
     """The multigrid levels of an ELL operator (theta, omega, alpha, coarse_max as the product's
     AmgCfg)."""
 
-    def __init__(self, nbr, coef, diag, theta=0.08, omega=2.0 / 3.0, alpha=1.5, coarse_max=400, theta0=0.02,
-                 min_ratio0=3.5, alpha_aggr=2.0, max_levels=16):
+    def __init__(self, nbr, coef, diag, theta=0.08, omega=0.8, alpha=1.8, coarse_max=400, theta0=0.02,
+                 min_ratio0=3.5, alpha_aggr=2.4, max_levels=16):
         self.omega, self.alpha = omega, alpha
         self.aggressive = False
         self.A, self.P, self.aggs = [], [], []
