@@ -8,7 +8,7 @@
  *     src/solver/mod_fv_solver.f90:270-285 (init / advance / project_once / compute_dt /
  *     to_device / sync_host / divergence_metrics) for a Fortran host that keeps its own Mesh.
  *
- * All mesh indices are 0-based here; the Fortran binding subtracts 1.
+ * All mesh indices are 0-based here; a Fortran binding subtracts 1 from its 1-based arrays.
  */
 #ifndef DOPAMINE_HIP_H
 #define DOPAMINE_HIP_H
