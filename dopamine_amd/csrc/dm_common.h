@@ -115,6 +115,10 @@ struct HostExec {
     for (int k = 0; k < NOUT; ++k) out_part[k] = acc.v[k];
   }
   void sync() {}
+  // one sequence of work on the host: the second stream's work simply runs in order
+  void side_begin() {}
+  void side_end() {}
+  void join() {}
   // kernel timing is a device feature; on the host these are no-ops
   void tbegin(int) {}
   void tend(int) {}

@@ -251,10 +251,40 @@ struct DeviceExec {
     }
     if (part) (void)hipFree(part);
     if (stream) (void)hipStreamDestroy(stream);
+    if (part_side) (void)hipFree(part_side);
+    if (side) (void)hipStreamDestroy(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     kt = nullptr;
-    part = nullptr;
-    stream = nullptr;
+    part = part_side = nullptr;
+    stream = side = nullptr;
+    ev_fork = ev_join = nullptr;
   }
+  // A second stream (default priority, created on first use) for work independent of what the
+  // main stream runs meanwhile: launches between side_begin() and side_end() go to it, after
+  // everything queued on the main stream so far, with a partials buffer of their own; join()
+  // makes the main stream wait for them.
+  hipStream_t side = nullptr;
+  double* part_side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void side_begin() {
+    if (!side) {
+      DM_CHECK(hipStreamCreate(&side));  // default priority
+      DM_CHECK(hipMalloc(&part_side, sizeof(double) * 8 * size_t(part_blocks)));
+      DM_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+      DM_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
+    DM_CHECK(hipEventRecord(ev_fork, stream));
+    DM_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+    std::swap(stream, side);
+    std::swap(part, part_side);
+  }
+  void side_end() {
+    std::swap(stream, side);
+    std::swap(part, part_side);
+    DM_CHECK(hipEventRecord(ev_join, side));
+  }
+  void join() { DM_CHECK(hipStreamWaitEvent(stream, ev_join, 0)); }
   void timing(int c, bool enable) {
     if (c < 0 || c >= K_NCLASS) throw std::runtime_error("unknown kernel class");
     if (enable) {

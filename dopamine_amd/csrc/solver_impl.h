@@ -1537,13 +1537,21 @@ class SolverT {
       const double gdt = RK3_G[k] * dt;
       time = time_n + RK3_C[k] * dt;
       refresh_bc();
-      gradients(true, gdt);
-      if (prm.wall_model == 1)
-        x.zero(tau, 3 * size_t(std::max(1, nb)));
-      else if (n_wm && prm.wall_model == 2)
+      // the MOST wall model (latency-bound, few faces) reads only U, T and its own state: it runs on
+      // the second stream while the gradients stream on the main one; the explicit RHS joins it
+      const bool most = n_wm && prm.wall_model == 2;
+      if (most) {
+        x.side_begin();
         x.template reduce<2, false>(nb, MostWall{M, U, bous ? T : nullptr, prm.T_ref, prm.T_surface,
                                                 L_cfg, L_bnd, tau, q_w},
                                     MostLFin{L_cfg, (prm.compute_L && k == 0) ? 1 : 0});
+        x.side_end();
+      }
+      gradients(true, gdt);
+      if (prm.wall_model == 1)
+        x.zero(tau, 3 * size_t(std::max(1, nb)));
+      else if (most)
+        x.join();
       else if (n_wm)
         wall_model(k == 0 ? dt : 0.0);
       if (scheme == SCHEME_QUICK) x.for_each(nf, ReconFace<SCHEME_QUICK>{M, U, U_bnd, UF, grad, phiF});
