@@ -38,12 +38,6 @@ struct EllOp {
   const double* diag = nullptr;  // n (already repaired: non-finite or tiny -> 1)
 };
 
-// Preconditioned residual and search direction of the FP32 flexible CG, packed so one gather
-// fetches both (z of this iteration in x, the previous direction in y).
-struct alignas(8) ZP {
-  float x, y;
-};
-
 // Packed per-cell state of the single-reduction recurrence: one 32-byte (16-byte in FP32) load
 // fetches everything a neighbour contributes.
 struct alignas(32) St4 {
@@ -914,13 +908,14 @@ struct FcgStart32 {
 };
 
 // z -= mean(z) on a singular operator; beta = max(0, z.(r - r_old) / (z.r)_old);
-// p = z + beta p_old; q = A p (FP32 operator); partial p.q
+// p = z + beta p_old; q = A p (FP32 operator); partial p.q. z and the directions are plain FP32
+// arrays (the new direction and q are whole-word stores).
 template <int WC>
 struct FcgDir32T {
   static constexpr int kSlots = WC;
   EllOp32 A;
-  const ZP* cur;  // z (x) and the previous direction (y)
-  ZP* nxt;        // the new direction goes to nxt[c].y
+  const float *z, *pold;
+  float* pnew;
   float* q;
   CgScalars* sc;
   int k;
@@ -936,22 +931,22 @@ struct FcgDir32T {
     if (leader) sc->rz_hist[k & 1] = zr;
     return true;
   }
-  DM_HD static float dir(const ZP& v, float zm, float beta, bool first) {
-    return first ? v.x - zm : (v.x - zm) + beta * v.y;
+  DM_HD static float dir(float zv, float pv, float zm, float beta, bool first) {
+    return first ? zv - zm : (zv - zm) + beta * pv;
   }
   DM_HD Red<1> operator()(int c, const Scal& s) const {
     const int n = A.n;
     const float zm = float(s.zm), beta = float(s.beta);
     const bool first = k == 0;
-    const float pc = dir(cur[c], zm, beta, first);
+    const float pc = dir(z[c], first ? 0.0f : pold[c], zm, beta, first);
     float qc = A.diag[c] * pc;
     for (int kk = 0; kk < A.w; ++kk) {
       const size_t q_ = size_t(kk) * n + c;
       const int j = A.nbr[q_];
       if (j < 0) continue;
-      qc = qc - A.coef[q_] * dir(cur[j], zm, beta, first);
+      qc = qc - A.coef[q_] * dir(z[j], first ? 0.0f : pold[j], zm, beta, first);
     }
-    nxt[c].y = pc;
+    pnew[c] = pc;
     q[c] = qc;
     Red<1> o;
     o.v[0] = double(pc) * double(qc);
@@ -967,7 +962,7 @@ struct FcgDir32T {
     for (int c = lo + t; c < hi; c += nt) {
       int jj[WC > 0 ? WC : 1];
       float cf[WC > 0 ? WC : 1];
-      ZP vj[WC > 0 ? WC : 1];
+      float zj[WC > 0 ? WC : 1], pj[WC > 0 ? WC : 1];
 #pragma unroll
       for (int kk = 0; kk < WC; ++kk) {
         const bool live = kk < w;
@@ -978,12 +973,15 @@ struct FcgDir32T {
         cf[kk] = (live && j >= 0) ? h : 0.0f;
       }
 #pragma unroll
-      for (int kk = 0; kk < WC; ++kk) vj[kk] = cur[jj[kk]];
-      const float pc = dir(cur[c], zm, beta, first);
+      for (int kk = 0; kk < WC; ++kk) {
+        zj[kk] = z[jj[kk]];
+        pj[kk] = first ? 0.0f : pold[jj[kk]];
+      }
+      const float pc = dir(z[c], first ? 0.0f : pold[c], zm, beta, first);
       float qc = A.diag[c] * pc;
 #pragma unroll
-      for (int kk = 0; kk < WC; ++kk) qc = qc - cf[kk] * dir(vj[kk], zm, beta, first);
-      nxt[c].y = pc;
+      for (int kk = 0; kk < WC; ++kk) qc = qc - cf[kk] * dir(zj[kk], pj[kk], zm, beta, first);
+      pnew[c] = pc;
       q[c] = qc;
       acc.v[0] = acc.v[0] + double(pc) * double(qc);
     }
@@ -995,7 +993,7 @@ using FcgDir32 = FcgDir32T<0>;
 // alpha = (z.r) / (p.q); e += alpha p; r_old = r; r -= alpha q; partial r.r
 struct FcgUpd32 {
   float *e, *r, *rold;
-  const ZP* p;  // the direction in y
+  const float* p;  // the direction
   const float* q;
   CgScalars* sc;
   int k;
@@ -1009,7 +1007,7 @@ struct FcgUpd32 {
   }
   DM_HD Red<1> operator()(int c, const Scal& s) const {
     const float a = float(s.alpha);
-    e[c] = e[c] + a * p[c].y;
+    e[c] = e[c] + a * p[c];
     const float rc = r[c];
     rold[c] = rc;
     const float rn = rc - a * q[c];
@@ -1118,8 +1116,8 @@ class CgT {
     if (!pc) return;
     for (float** v : {&r32_, &ro32_, &q32_})
       if (!*v) *v = x_->template alloc<float>(n_);
-    for (ZP** v : {&zp32_[0], &zp32_[1]})
-      if (!*v) *v = x_->template alloc<ZP>(n_);
+    for (float** v : {&z32_, &p32_[0], &p32_[1]})
+      if (!*v) *v = x_->template alloc<float>(n_);
     if (fpart32_g_ < pc->out_grid32(n_)) {
       if (fpart32_) x_->release(fpart32_);
       fpart32_g_ = pc->out_grid32(n_);
@@ -1148,11 +1146,11 @@ class CgT {
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
                     (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_, (void*)e32_, (void*)r32_,
-                    (void*)ro32_, (void*)zp32_[0], (void*)zp32_[1], (void*)q32_, (void*)fpart32_,
+                    (void*)ro32_, (void*)z32_, (void*)p32_[0], (void*)p32_[1], (void*)q32_, (void*)fpart32_,
                     (void*)coef32_, (void*)diag32_})
       if (p) x_->release(p);
     r32_ = ro32_ = q32_ = coef32_ = diag32_ = nullptr;
-    zp32_[0] = zp32_[1] = nullptr;
+    z32_ = p32_[0] = p32_[1] = nullptr;
     fpart32_ = nullptr;
     fpart32_g_ = 0;
     pc32_ = nullptr;
@@ -1234,17 +1232,17 @@ class CgT {
         int ibatch = std::max(4, std::min(maxit + 1, predict_sw_[std::min(sweep, MAX_SWEEPS)]));
         for (;;) {
           for (int i = 0; i < ibatch; ++i, ++kin) {
-            ZP* cur = zp32_[kin & 1];
-            ZP* nxt = zp32_[(kin + 1) & 1];
-            pc32->apply32(kin, r32_, ro32_, &cur->x, 2, sc_, RR, g, fpart32_);
+            const float* pold = p32_[kin & 1];
+            float* pnew = p32_[(kin + 1) & 1];
+            pc32->apply32(kin, r32_, ro32_, z32_, 1, sc_, RR, g, fpart32_);
             xx.tbegin(K_CG_DIR);
             by_width(A32.w, [&](auto wc) {
-              xx.template pass<5, 1>(n, FcgDir32T<decltype(wc)::value>{A32, cur, nxt, q32_, sc_, kin}, fpart32_, gt,
-                                     D);
+              xx.template pass<5, 1>(n, FcgDir32T<decltype(wc)::value>{A32, z32_, pold, pnew, q32_, sc_, kin}, fpart32_,
+                                     gt, D);
             });
             xx.tend(K_CG_DIR);
             xx.tbegin(K_CG_UPD);
-            xx.template pass<1, 1>(n, FcgUpd32{e32_, r32_, ro32_, nxt, q32_, sc_, kin}, D, g, RR);
+            xx.template pass<1, 1>(n, FcgUpd32{e32_, r32_, ro32_, pnew, q32_, sc_, kin}, D, g, RR);
             xx.tend(K_CG_UPD);
           }
           xx.download(&h, sc_, 1);
@@ -1444,7 +1442,8 @@ class CgT {
   FcgPc32* pc32_ = nullptr;
   bool plain_ = false;
   float *r32_ = nullptr, *ro32_ = nullptr, *q32_ = nullptr;
-  ZP* zp32_[2] = {nullptr, nullptr};
+  float* z32_ = nullptr;                   // FP32 flexible CG: preconditioned residual
+  float* p32_[2] = {nullptr, nullptr};     // and its directions (previous, new)
   double* fpart32_ = nullptr;
   int fpart32_g_ = 0;
   CgCapture* capture_ = nullptr;
