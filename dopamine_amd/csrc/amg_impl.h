@@ -685,12 +685,13 @@ struct Coarsest {
 };
 
 // level 0 post-smoothing as a chained pass writing z and the flexible-CG partials
-// z.r, z.r_old, sum z, sum r, sum r_old
+// z.r, z.v, sum z, sum r, sum v (v = r_old in the FP64 flexible CG, q of the last step in the FP32
+// sweeps, which form z.r_old from it)
 template <class T, int WC = 0>
 struct Up0 {
   static constexpr int kSlots = WC;
   AmgLevelDev L;
-  const T *r, *rold;
+  const T *r, *rold;  // rold: the v of the partials
   T* z;  // z[c * zs]
   int zs;
   const CgScalars* sc;
@@ -891,9 +892,9 @@ class AmgPc : public FcgPc, public FcgPc32 {
   }
   // ---- FcgPc32 (FP32 sweeps)
   int out_grid32(int n) const override { return x_->pass_grid(n); }
-  void apply32(int k, const float* r, const float* rold, float* z, int zstride, CgScalars* sc, const double* rr_part,
+  void apply32(int k, const float* r, const float* v, float* z, int zstride, CgScalars* sc, const double* rr_part,
                int rr_g, double* out_part) override {
-    cycle<float>(k, r, rold, z, zstride, sc, rr_part, rr_g, out_part, amgk::INNER);
+    cycle<float>(k, r, v, z, zstride, sc, rr_part, rr_g, out_part, amgk::INNER);
   }
 
   // One V-cycle z = M r outside a solve (tests): no gate, no convergence test.

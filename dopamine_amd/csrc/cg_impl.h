@@ -29,6 +29,7 @@ struct CgScalars {
   double inner_tol2;                    // mixed precision: inner stop on ||r_in||^2
   int its, done, maxit, nullspace;
   int inner_done, inner_its;            // mixed precision: inner solve state
+  double alpha32;                       // FP32 flexible CG: the last update's alpha
 };
 
 struct EllOp {
@@ -721,12 +722,14 @@ struct FcgPc {
   virtual int out_grid(int n) const = 0;
 };
 
-// Preconditioner of the FP32 sweeps of mixed precision: apply32(k, r, r_old, z, zstride, ...) (z[c * zstride]) with the
-// contract of FcgPc on FP32 vectors, the sweep's stopping test (rr <= inner_tol2 or the iteration
-// limit, setting inner_done) in its first pass.
+// Preconditioner of the FP32 sweeps of mixed precision: apply32(k, r, v, z, zstride, ...) (z[c * zstride]) with the
+// contract of FcgPc on FP32 vectors — the five partials z.r, z.v, sum z, sum r, sum v, where the
+// FP32 flexible CG passes v = q = A p of the last step (it forms z.r_old as z.r + alpha z.q) —
+// and the sweep's stopping test (rr <= inner_tol2 or the iteration limit, setting inner_done) in
+// its first pass.
 struct FcgPc32 {
   virtual ~FcgPc32() = default;
-  virtual void apply32(int k, const float* r, const float* rold, float* z, int zstride, CgScalars* sc,
+  virtual void apply32(int k, const float* r, const float* v, float* z, int zstride, CgScalars* sc,
                        const double* rr_part, int rr_g, double* out_part) = 0;
   virtual int out_grid32(int n) const = 0;
 };
@@ -873,7 +876,7 @@ struct FcgUpd {
 // e = 0; partial r.r of the FP32 residual.
 struct FcgStart32 {
   const double* r64;
-  float *r, *rold;
+  float* r;
   double* x;
   float* e;
   CgScalars* sc;
@@ -898,7 +901,6 @@ struct FcgStart32 {
   DM_HD Red<1> operator()(int c, const Scal&) const {
     const float rc = float(r64[c]);
     r[c] = rc;
-    rold[c] = rc;
     x[c] = x[c] + double(e[c]);
     e[c] = 0.0f;
     Red<1> o;
@@ -909,7 +911,9 @@ struct FcgStart32 {
 
 // z -= mean(z) on a singular operator; beta = max(0, z.(r - r_old) / (z.r)_old);
 // p = z + beta p_old; q = A p (FP32 operator); partial p.q. z and the directions are plain FP32
-// arrays (the new direction and q are whole-word stores).
+// arrays (the new direction and q are whole-word stores). The partials come as z.r, z.q, sum z,
+// sum r, sum q with q of the last step: r_old = r + alpha q, so z.r_old = z.r + alpha z.q and
+// sum r_old = sum r + alpha sum q (no r_old stored, and no cancellation in z.(r - r_old)).
 template <int WC>
 struct FcgDir32T {
   static constexpr int kSlots = WC;
@@ -926,7 +930,8 @@ struct FcgDir32T {
     if (sc->done || sc->inner_done) return false;
     s.zm = sc->nullspace ? t.v[2] / double(A.n) : 0.0;
     const double zr = t.v[0] - s.zm * t.v[3];
-    const double zro = t.v[1] - s.zm * t.v[4];
+    const double al = sc->alpha32;
+    const double zro = (t.v[0] + al * t.v[1]) - s.zm * (t.v[3] + al * t.v[4]);
     s.beta = k == 0 ? 0.0 : fmax(0.0, (zr - zro) / sc->rz_hist[(k + 1) & 1]);
     if (leader) sc->rz_hist[k & 1] = zr;
     return true;
@@ -990,9 +995,10 @@ struct FcgDir32T {
 };
 using FcgDir32 = FcgDir32T<0>;
 
-// alpha = (z.r) / (p.q); e += alpha p; r_old = r; r -= alpha q; partial r.r
+// alpha = (z.r) / (p.q) (kept for the next direction's beta); e += alpha p; r -= alpha q;
+// partial r.r
 struct FcgUpd32 {
-  float *e, *r, *rold;
+  float *e, *r;
   const float* p;  // the direction
   const float* q;
   CgScalars* sc;
@@ -1000,16 +1006,16 @@ struct FcgUpd32 {
   struct Scal {
     double alpha;
   };
-  DM_HD bool setup(const Red<1>& t, bool, Scal& s) const {
+  DM_HD bool setup(const Red<1>& t, bool leader, Scal& s) const {
     if (sc->done || sc->inner_done) return false;
     s.alpha = sc->rz_hist[k & 1] / t.v[0];
+    if (leader) sc->alpha32 = s.alpha;
     return true;
   }
   DM_HD Red<1> operator()(int c, const Scal& s) const {
     const float a = float(s.alpha);
     e[c] = e[c] + a * p[c];
     const float rc = r[c];
-    rold[c] = rc;
     const float rn = rc - a * q[c];
     r[c] = rn;
     Red<1> o;
@@ -1114,7 +1120,7 @@ class CgT {
   void set_pc32(FcgPc32* pc) {
     pc32_ = pc;
     if (!pc) return;
-    for (float** v : {&r32_, &ro32_, &q32_})
+    for (float** v : {&r32_, &q32_})
       if (!*v) *v = x_->template alloc<float>(n_);
     for (float** v : {&z32_, &p32_[0], &p32_[1]})
       if (!*v) *v = x_->template alloc<float>(n_);
@@ -1146,10 +1152,10 @@ class CgT {
     for (void* p : {(void*)bw_, (void*)r_, (void*)z_, (void*)pa_, (void*)pb_, (void*)q_, (void*)sc_, (void*)part_,
                     (void*)st_[0], (void*)st_[1], (void*)st32_[0], (void*)st32_[1], (void*)chat_, (void*)dinv_,
                     (void*)chat32_, (void*)dinv32_, (void*)ro_, (void*)fpart_, (void*)e32_, (void*)r32_,
-                    (void*)ro32_, (void*)z32_, (void*)p32_[0], (void*)p32_[1], (void*)q32_, (void*)fpart32_,
+                    (void*)z32_, (void*)p32_[0], (void*)p32_[1], (void*)q32_, (void*)fpart32_,
                     (void*)coef32_, (void*)diag32_})
       if (p) x_->release(p);
-    r32_ = ro32_ = q32_ = coef32_ = diag32_ = nullptr;
+    r32_ = q32_ = coef32_ = diag32_ = nullptr;
     z32_ = p32_[0] = p32_[1] = nullptr;
     fpart32_ = nullptr;
     fpart32_g_ = 0;
@@ -1227,14 +1233,14 @@ class CgT {
       int total = 0;
       xx.zero(e32_, size_t(n));
       for (int sweep = 0;; ++sweep) {
-        xx.template pass<2, 1>(n, FcgStart32{r_, r32_, ro32_, x, e32_, sc_, sweep, MAX_SWEEPS, tau_}, in, g, RR);
+        xx.template pass<2, 1>(n, FcgStart32{r_, r32_, x, e32_, sc_, sweep, MAX_SWEEPS, tau_}, in, g, RR);
         int kin = 0;
         int ibatch = std::max(4, std::min(maxit + 1, predict_sw_[std::min(sweep, MAX_SWEEPS)]));
         for (;;) {
           for (int i = 0; i < ibatch; ++i, ++kin) {
             const float* pold = p32_[kin & 1];
             float* pnew = p32_[(kin + 1) & 1];
-            pc32->apply32(kin, r32_, ro32_, z32_, 1, sc_, RR, g, fpart32_);
+            pc32->apply32(kin, r32_, q32_, z32_, 1, sc_, RR, g, fpart32_);
             xx.tbegin(K_CG_DIR);
             by_width(A32.w, [&](auto wc) {
               xx.template pass<5, 1>(n, FcgDir32T<decltype(wc)::value>{A32, z32_, pold, pnew, q32_, sc_, kin}, fpart32_,
@@ -1242,7 +1248,7 @@ class CgT {
             });
             xx.tend(K_CG_DIR);
             xx.tbegin(K_CG_UPD);
-            xx.template pass<1, 1>(n, FcgUpd32{e32_, r32_, ro32_, pnew, q32_, sc_, kin}, D, g, RR);
+            xx.template pass<1, 1>(n, FcgUpd32{e32_, r32_, pnew, q32_, sc_, kin}, D, g, RR);
             xx.tend(K_CG_UPD);
           }
           xx.download(&h, sc_, 1);
@@ -1441,7 +1447,7 @@ class CgT {
   FcgPc* fcg_ = nullptr;
   FcgPc32* pc32_ = nullptr;
   bool plain_ = false;
-  float *r32_ = nullptr, *ro32_ = nullptr, *q32_ = nullptr;
+  float *r32_ = nullptr, *q32_ = nullptr;
   float* z32_ = nullptr;                   // FP32 flexible CG: preconditioned residual
   float* p32_[2] = {nullptr, nullptr};     // and its directions (previous, new)
   double* fpart32_ = nullptr;
