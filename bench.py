@@ -95,17 +95,18 @@ def multigrid_bytes(mesh, mixed):
     cg_dir, the flexible CG's direction pass FcgDir32T: one FP32 coefficient and two int32 indices
       per face (12 nf); per cell the diagonal, z and the previous direction read, the direction
       and q = A p written (5 x 4 B); FP64 (FcgDirT): 16 nf + 5 x 8 B per cell;
-    cg_upd, FcgUpd32: e, r read and written, r_old, p, q (7 x 4 B per cell; 7 x 8 B in FP64);
+    cg_upd, FcgUpd32: e, r read and written, p, q read (6 x 4 B per cell); FP64 (FcgUpd): x, r read
+      and written, r_old written, p, q read (7 x 8 B);
     mg_down, the pre-smoothing pass Down0: the column-scaled binary16 coupling of each face side
       and two int32 indices (12 nf); per cell b, 1/diag read, x, r written (4 x 4 B; b FP64 in
       FP64 mode);
     mg_up, the post-smoothing pass Up0: the row-scaled binary16 coupling of each face side and two
-      int32 indices (12 nf); per cell x, 1/diag, r, r_old read, z written (5 x 4 B; r, r_old, z
-      FP64 in FP64 mode)."""
+      int32 indices (12 nf); per cell x, 1/diag, r, q read, z written (5 x 4 B; r, r_old, z FP64 in
+      FP64 mode)."""
     nf, nc = n_op_faces(mesh), mesh.n_cells
     v = 4 if mixed else 8
     return {"cg_dir": 12 * nf + 20 * nc if mixed else 16 * nf + 40 * nc,
-            "cg_upd": 28 * nc if mixed else 56 * nc,
+            "cg_upd": 24 * nc if mixed else 56 * nc,
             "mg_down": 12 * nf + (v + 12) * nc,
             "mg_up": 12 * nf + (8 + 3 * v) * nc}
 
