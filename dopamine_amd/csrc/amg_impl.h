@@ -261,7 +261,7 @@ struct AmgCfg {
   double theta0 = 0.02;
   double min_ratio0 = 3.5;
   double alpha_aggr = 2.4;
-  int coarse_max = 400;      // cells of the densely solved level
+  int coarse_max = 1500;     // cells of the densely solved level
   int max_levels = 16;
 };
 

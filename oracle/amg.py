@@ -127,7 +127,7 @@ class Hierarchy:
     """The multigrid levels of an ELL operator (theta, omega, alpha, coarse_max as the product's
     AmgCfg)."""
 
-    def __init__(self, nbr, coef, diag, theta=0.08, omega=0.8, alpha=1.8, coarse_max=400, theta0=0.02,
+    def __init__(self, nbr, coef, diag, theta=0.08, omega=0.8, alpha=1.8, coarse_max=1500, theta0=0.02,
                  min_ratio0=3.5, alpha_aggr=2.4, max_levels=16):
         self.omega, self.alpha = omega, alpha
         self.aggressive = False

@@ -193,7 +193,9 @@ def cg_solve_retry(nbr, coef, diag, b, x0, rtol, atol, maxit, nullspace=False, s
     return rc, x, its.value, resid.value, total.value, retried.value
 
 
-AMG_DEFAULTS = (0.08, 0.8, 1.8, 400, -1.0)  # theta, omega, alpha, coarse_max, theta0 (< 0: off)
+# theta, omega, alpha, coarse_max, theta0 (< 0: off): the product's defaults except a coarsest
+# level of 400 cells (the product's 1500), so the small test operators keep several levels
+AMG_DEFAULTS = (0.08, 0.8, 1.8, 400, -1.0)
 
 
 def amg_solve(nbr, coef, diag, b, rtol, atol=1e-50, maxit=2000, nullspace=False, mixed=False, cfg=AMG_DEFAULTS):

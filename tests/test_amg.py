@@ -53,7 +53,7 @@ OPERATORS = {
     "tets": lambda: cases.cavity_tets(6, rcm=True),             # jittered tets, singular
     "full_stack": lambda: cases.full_stack(16, 16, 8, n_obstacles=3, seed=5),  # flat cells, obstacles
 }
-# the product's defaults, with the aggressive first coarsening on
+# the test configuration with the product's aggressive first coarsening on
 AMG_PRODUCT = AMG_DEFAULTS[:4] + (0.02,)
 
 

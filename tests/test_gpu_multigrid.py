@@ -22,13 +22,19 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("coarse", [None, 64])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("name", list(CASES))
-def test_gpu_multigrid_tracks_the_oracle(name, precision):
+def test_gpu_multigrid_tracks_the_oracle(name, precision, coarse, monkeypatch):
     """Four steps at p_tol 1e-8 against the oracle's Jacobi-CG run at p_tol 1e-12: u within 1e-7
     and p within 1e-6 relative L2 (the bar of the mixed-precision Jacobi path); against the same
     path with Jacobi CG at p_tol 1e-8 on the GPU: the projected face-flux divergence within 10x of
-    its value and at most half of its CG iterations (on these meshes of 1-4 thousand cells; the gap grows with the mesh)."""
+    its value and at most half of its CG iterations (on these meshes of 1-4 thousand cells; the gap grows with the mesh).
+    coarse = 64: the densely solved level capped at 64 cells (DOPAMINE_AMG_COARSE), so these small
+    meshes run the deeper cycle (restriction, smoothing and prolongation on the coarse levels) that
+    the default 1,500-cell coarsest level leaves to the large meshes."""
+    if coarse is not None:
+        monkeypatch.setenv("DOPAMINE_AMG_COARSE", str(coarse))
     case = CASES[name]()
     mesh = case.make_mesh()
     case.params.p_tol = 1e-12
@@ -41,6 +47,10 @@ def test_gpu_multigrid_tracks_the_oracle(name, precision):
     case.params.precond = "MULTIGRID"
     g = FVSolver(mesh, case.params, precision=precision)
     run_case(g, case, mesh, nsteps=4, stats_every=4)
+    levels = g.multigrid_info()["levels"]
+    print(f"[gpu] multigrid {name} p{precision} coarse {coarse}: {levels} levels", flush=True)
+    if coarse is not None:
+        assert levels >= 3
     du, dp = rel_l2(o.U, g.U), rel_l2(o.p, g.p)
     ci, cj = g.counters(), j.counters()
     dg, dj = g.stats()["max_abs_divU"], j.stats()["max_abs_divU"]
