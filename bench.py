@@ -34,7 +34,8 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s per RK3 step at p-residual 1e-8: learned/mixed vs plain FP64"
-WORKLOAD_KEYS = {"full": "full stack 250x210x160", "channel": "channel 128x96x64", "tets": "tets 6x55^3 rcm"}
+WORKLOAD_KEYS = {"full": "full stack 250x210x160", "channel": "channel 128x96x64", "tets": "tets 6x55^3 rcm",
+                 "rb": "rb 256x256x32"}
 
 
 def parse():
@@ -42,9 +43,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("full", "channel", "tets"), default="full",
+    ap.add_argument("--workload", choices=("full", "channel", "tets", "rb"), default="full",
                     help="full: the full-stack obstacle box (the line's workload); channel: the turbulent channel; "
-                         "tets: the lid-driven cavity on 6 x 55^3 jittered, shuffled, RCM-renumbered tets")
+                         "tets: the lid-driven cavity on 6 x 55^3 jittered, shuffled, RCM-renumbered tets; "
+                         "rb: Rayleigh-Benard convection 256 x 256 x 32 (the learned preconditioner's configuration)")
     ap.add_argument("--mesh", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
                     help="mesh of the full-stack or channel workload (defaults 250 210 160 and 128 96 64)")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed",
@@ -198,6 +200,10 @@ def main():
         dims = tuple(args.mesh) if args.mesh else (128, 96, 64)
         case_fn = cases.channel
         case = case_fn(*dims)
+    elif wl == "rb":
+        dims = tuple(args.mesh) if args.mesh else (256, 256, 32)
+        case_fn = cases.rayleigh_benard
+        case = case_fn(*dims)
     else:
         dims = None
         case_fn = None
@@ -312,6 +318,9 @@ def main():
     elif wl == "channel":
         workload_text = (f"turbulent channel {dims[0]}x{dims[1]}x{dims[2]} ({nc} hex cells), Re_tau 395, KEP + "
                          "Vreman + log-law wall model, pressure to 1e-8")
+    elif wl == "rb":
+        workload_text = (f"Rayleigh-Benard convection 4 x 4 x 1 box {dims[0]}x{dims[1]}x{dims[2]} ({nc} wall-clustered "
+                         "hex cells), Ra 1e7, Pr 0.7, Boussinesq, KEP + van Leer TVD temperature, pressure to 1e-8")
     else:
         workload_text = (f"lid-driven cavity Re 3200 on {nc} jittered shuffled tets renumbered by reverse "
                          "Cuthill-McKee, Koren TVD + AMD, Jasak non-orthogonal correction, pressure to 1e-8")
@@ -337,6 +346,7 @@ def main():
         "dtype": "f32+f64" if mixed else ("f64+bf16" if learned else "f64"),
         "data": ("synthetic: seeded obstacles and temperature noise on a generated mesh" if wl == "full" else
                  "synthetic: fluid at rest, moving lid, on a generated seeded jittered mesh" if wl == "tets" else
+                 "synthetic: fluid at rest with seeded 1e-3 temperature noise on a generated mesh" if wl == "rb" else
                  "synthetic: seeded 10% uniform perturbation of the laminar profile on a generated mesh"),
         "config": {"workload": workload_text + "; " + path_text, "precision": "mixed" if mixed else "fp64",
                    "precond": args.precond, "cg_variant": args.cg_variant, "dt": case.params.dt},
