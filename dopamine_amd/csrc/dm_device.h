@@ -68,6 +68,21 @@ __global__ void __launch_bounds__(BLOCK) k_for(int n, F f) {
   const int i = xcd_block(gridDim.x) * BLOCK + threadIdx.x;
   if (i < n) f(i);
 }
+// the same with a wavefronts-per-SIMD floor W for the register allocator (functors with a static
+// kWaves: gather-latency-bound cell kernels whose registers would otherwise cap the occupancy)
+template <class F, int W>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W))) k_for_w(int n, F f) {
+  const int i = xcd_block(gridDim.x) * BLOCK + threadIdx.x;
+  if (i < n) f(i);
+}
+template <class F, class = void>
+struct ForWaves {
+  static constexpr int value = 0;
+};
+template <class F>
+struct ForWaves<F, std::void_t<decltype(F::kWaves)>> {
+  static constexpr int value = F::kWaves;
+};
 
 // Grid reductions are two launches: every block stores its partials (layout [quantity][block]),
 // then one block sums them in block order and runs the finaliser. The kernel boundary orders the
@@ -351,7 +366,10 @@ struct DeviceExec {
   template <class F>
   void for_each(int n, const F& f) {
     if (n <= 0) return;
-    launch(k_for<F>, nblocks(n), BLOCK, n, f);
+    if constexpr (ForWaves<F>::value > 0)
+      launch(k_for_w<F, ForWaves<F>::value>, nblocks(n), BLOCK, n, f);
+    else
+      launch(k_for<F>, nblocks(n), BLOCK, n, f);
   }
   template <int N, bool MAX, class F, class Fin>
   void reduce(int n, const F& f, const Fin& fin, const int* gate = nullptr) {
