@@ -351,6 +351,10 @@ struct AmgLevelDev {
   const int* nbr = nullptr;                      // level 0: ELL neighbours [slot][cell]
   const int *sptr = nullptr, *col = nullptr;     // coarse levels: sliced-ELL rows
   const AmgCoef *cs = nullptr, *cr = nullptr;
+  // level 0, when its neighbour offsets fit 16 bits (EllOff16): each slot's coupling and offset
+  // in one word (binary16 coupling in the low half, offset - obase[slot] in the high half)
+  const uint32_t *hs = nullptr, *hr = nullptr;
+  int obase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const float* dinv = nullptr;
   const int *kptr = nullptr, *kidx = nullptr;    // children (cells of this level) of the next coarser level's cells, sliced ELL
   float *b = nullptr, *x = nullptr, *r = nullptr;
@@ -481,6 +485,17 @@ DM_INLINE void slots0(const AmgLevelDev& L, const AmgCoef* cf, int c, int* jj, f
   }
 }
 
+// the same from the packed words: every slot live, an empty one the cell itself with coupling 0
+template <int WC>
+DM_INLINE void slots0p(const AmgLevelDev& L, const uint32_t* hw, int c, int* jj, float* h) {
+#pragma unroll
+  for (int kk = 0; kk < WC; ++kk) {
+    const uint32_t v = hw[size_t(kk) * L.n + c];
+    jj[kk] = off16_nbr(c, L.obase[kk], v >> 16);
+    h[kk] = coef_f(AmgCoef(v & 0xffffu));
+  }
+}
+
 // level 0 pre-smoothing as a chained pass: its setup is the convergence test of iteration k
 template <class T, int WC = 0>
 struct Down0 {
@@ -502,7 +517,10 @@ struct Down0 {
     for (int c = lo + t; c < hi; c += nt) {
       int jj[WC > 0 ? WC : 1];
       float h[WC > 0 ? WC : 1], bj[WC > 0 ? WC : 1];
-      slots0<WC>(L, L.cs, c, jj, h);
+      if (L.hs)
+        slots0p<WC>(L, L.hs, c, jj, h);
+      else
+        slots0<WC>(L, L.cs, c, jj, h);
 #pragma unroll
       for (int kk = 0; kk < WC; ++kk) bj[kk] = float(b[jj[kk]]);
       const float bc = float(b[c]);
@@ -717,7 +735,10 @@ struct Up0 {
     for (int c = lo + t; c < hi; c += nt) {
       int jj[WC > 0 ? WC : 1];
       float h[WC > 0 ? WC : 1], xj[WC > 0 ? WC : 1];
-      slots0<WC>(L, L.cr, c, jj, h);
+      if (L.hr)
+        slots0p<WC>(L, L.hr, c, jj, h);
+      else
+        slots0<WC>(L, L.cr, c, jj, h);
 #pragma unroll
       for (int kk = 0; kk < WC; ++kk) xj[kk] = L.x[jj[kk]];
       const float xc = L.x[c];
@@ -783,8 +804,23 @@ class AmgPc : public FcgPc, public FcgPc32 {
             const size_t q = size_t(k) * cur.n + c;
             scaled(c, q, &cs[q], &cr[q]);
           }
-        L.cs = up(to_coef(cs));
-        L.cr = up(to_coef(cr));
+        const std::vector<AmgCoef> hcs = to_coef(cs), hcr = to_coef(cr);
+        L.cs = up(hcs);
+        L.cr = up(hcr);
+        EllOff16 o16;
+        if (off16_enabled() && ell_off16(cur.nbr.data(), cur.n, cur.w, o16)) {
+          std::vector<uint32_t> ws(size_t(o16.W) * cur.n), wr(ws.size());
+          for (size_t q = 0; q < ws.size(); ++q) {
+            const int s = o16.src[q];
+            const size_t q0 = size_t(s) * cur.n + q % size_t(cur.n);
+            const uint32_t hi = uint32_t(uint16_t(o16.d[q])) << 16;
+            ws[q] = hi | (s < 0 ? 0u : uint32_t(hcs[q0]));
+            wr[q] = hi | (s < 0 ? 0u : uint32_t(hcr[q0]));
+          }
+          L.hs = up(ws);
+          L.hr = up(wr);
+          for (int k = 0; k < 8; ++k) L.obase[k] = o16.base[k];
+        }
       } else {  // the non-empty entries, sliced ELL
         std::vector<int> rptr(size_t(cur.n) + 1, 0), col;
         std::vector<float> cs, cr;

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <stdexcept>
 #include <type_traits>
 #include <vector>
@@ -37,7 +38,123 @@ struct EllOp {
   const int* nbr = nullptr;      // w*n, -1 = empty slot
   const double* coef = nullptr;  // w*n
   const double* diag = nullptr;  // n (already repaired: non-finite or tiny -> 1)
+  // optional compact copy for the batched FP32 passes (see EllOff16): 16-bit offsets two to a
+  // word [W/2][n], the FP32 coefficients in the re-seated slot order [W][n], the slot bases
+  const uint32_t* opair = nullptr;
+  const float* ocoef = nullptr;
+  int obase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
+
+// Neighbour offsets of an ELL operator in 16 bits, for the batched passes that stream the whole
+// operator every iteration (an int32 index is a third of their bytes). The listed neighbours of
+// each cell are re-seated so that one slot holds offsets j - c of similar size across the cells:
+// the cells with the most neighbours give each slot its typical offset (the median of their k-th
+// smallest offset; on a box mesh in natural order -plane, -row, -1, +1, +row, +plane), and every
+// cell places its sorted offsets into the slots, in order, at the least total distance from the
+// typical ones. Slot k of cell c then stores j - c - base[k] as a signed 16-bit value; an empty
+// slot stores the offset 0 (the cell itself) and carries a zero coefficient. src[k * n + c] is
+// the original slot (-1: empty). The re-seating changes only the order of a row's terms. Returns
+// false (the int32 lists are used) when the padded width W exceeds 8 or a slot's offsets span
+// more than 65,536 values (long periodic wraps, unbanded numberings).
+struct EllOff16 {
+  int W = 0;
+  int base[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<int16_t> d;  // [W][n]
+  std::vector<int> src;    // [W][n]
+};
+inline bool ell_off16(const int* nbr, int n, int w, EllOff16& o) {
+  const int W = w <= 4 ? 4 : w <= 6 ? 6 : w <= 8 ? 8 : 0;
+  if (W == 0 || n <= 0) return false;
+  o.W = W;
+  o.src.assign(size_t(W) * n, -1);
+  // each cell's listed neighbours as (offset, original slot), ascending
+  std::vector<std::pair<int, int>> ent(size_t(w) * n);
+  std::vector<int> cnt(n, 0);
+  int mmax = 0;
+  for (int c = 0; c < n; ++c) {
+    std::pair<int, int>* e = &ent[size_t(c) * w];
+    for (int k = 0; k < w; ++k) {
+      const int j = nbr[size_t(k) * n + c];
+      if (j >= 0) e[cnt[c]++] = {j - c, k};
+    }
+    std::sort(e, e + cnt[c]);
+    mmax = std::max(mmax, cnt[c]);
+  }
+  if (mmax == 0) return false;
+  double typ[8];
+  for (int k = 0; k < mmax; ++k) {
+    std::vector<int> v;
+    for (int c = 0; c < n; ++c)
+      if (cnt[c] == mmax) v.push_back(ent[size_t(c) * w + k].first);
+    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+    typ[k] = double(v[v.size() / 2]);
+  }
+  // least-distance monotone placement of m sorted offsets into the mmax slots (dynamic program)
+  double cost[9][9];
+  int from[9][9];
+  for (int c = 0; c < n; ++c) {
+    const int m = cnt[c];
+    const std::pair<int, int>* e = &ent[size_t(c) * w];
+    for (int i = 0; i <= m; ++i)
+      for (int k = 0; k <= mmax; ++k) cost[i][k] = 1e300;
+    for (int k = 0; k <= mmax; ++k) cost[0][k] = 0.0;
+    for (int i = 1; i <= m; ++i)
+      for (int k = i; k <= mmax; ++k) {
+        const double place = cost[i - 1][k - 1] + std::fabs(double(e[i - 1].first) - typ[k - 1]);
+        const double skip = cost[i][k - 1];
+        cost[i][k] = place <= skip ? place : skip;
+        from[i][k] = place <= skip ? 1 : 0;
+      }
+    for (int i = m, k = mmax; i > 0; --k)
+      if (from[i][k]) {
+        o.src[size_t(k - 1) * n + c] = e[i - 1].second;
+        --i;
+      }
+  }
+  auto off = [&](int k, int c) {
+    const int s = o.src[size_t(k) * n + c];
+    return s < 0 ? 0 : nbr[size_t(s) * n + c] - c;
+  };
+  for (int k = 0; k < W; ++k) {
+    int lo = off(k, 0), hi = lo;
+    for (int c = 1; c < n; ++c) {
+      const int v = off(k, c);
+      lo = std::min(lo, v);
+      hi = std::max(hi, v);
+    }
+    if (int64_t(hi) - int64_t(lo) > 65535) return false;
+    o.base[k] = lo + 32768;
+  }
+  o.d.resize(size_t(W) * n);
+  for (int k = 0; k < W; ++k)
+    for (int c = 0; c < n; ++c) o.d[size_t(k) * n + c] = int16_t(off(k, c) - o.base[k]);
+  return true;
+}
+DM_INLINE int off16_nbr(int c, int base, uint32_t bits16) { return c + base + int(int16_t(uint16_t(bits16))); }
+// The direction pass's compact operator: offsets two to a word [W/2][n] and the FP32
+// coefficients in the re-seated order [W][n]; false when the offsets do not fit (ell_off16).
+inline bool ell_pack32(const int* nbr, const double* coef, int n, int w, std::vector<uint32_t>& pr,
+                       std::vector<float>& oc, int base[8]) {
+  EllOff16 o;
+  if (!ell_off16(nbr, n, w, o)) return false;
+  const size_t W = size_t(o.W);
+  pr.assign(W / 2 * n, 0u);
+  oc.assign(W * n, 0.0f);
+  for (size_t k = 0; k < W; ++k)
+    for (int c = 0; c < n; ++c) {
+      const size_t q = k * n + c;
+      const int s = o.src[q];
+      oc[q] = s < 0 ? 0.0f : float(coef[size_t(s) * n + c]);
+      pr[k / 2 * n + c] |= uint32_t(uint16_t(o.d[q])) << (16 * (k & 1));
+    }
+  for (int k = 0; k < 8; ++k) base[k] = o.base[k];
+  return true;
+}
+// DOPAMINE_ELL_OFF16=0 keeps the int32 lists everywhere (for comparison)
+inline bool off16_enabled() {
+  const char* e = std::getenv("DOPAMINE_ELL_OFF16");
+  return !(e && e[0] == '0');
+}
 
 // Packed per-cell state of the single-reduction recurrence: one 32-byte (16-byte in FP32) load
 // fetches everything a neighbour contributes.
@@ -114,6 +231,9 @@ struct EllOp32 {
   const int* nbr = nullptr;
   const float* coef = nullptr;
   const float* diag = nullptr;
+  const uint32_t* opair = nullptr;  // as EllOp: the compact copy, or null
+  const float* ocoef = nullptr;
+  int obase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // Chained passes of one solve. Each reads the totals of the previous pass's partials (identical
@@ -968,14 +1088,25 @@ struct FcgDir32T {
       int jj[WC > 0 ? WC : 1];
       float cf[WC > 0 ? WC : 1];
       float zj[WC > 0 ? WC : 1], pj[WC > 0 ? WC : 1];
+      if (A.opair) {  // 16-bit offsets two to a word, every slot live (EllOff16)
 #pragma unroll
-      for (int kk = 0; kk < WC; ++kk) {
-        const bool live = kk < w;
-        const size_t q_ = size_t(live ? kk : 0) * n + c;
-        const int j = A.nbr[q_];
-        const float h = A.coef[q_];
-        jj[kk] = (live && j >= 0) ? j : c;
-        cf[kk] = (live && j >= 0) ? h : 0.0f;
+        for (int m = 0; m < WC / 2; ++m) {
+          const uint32_t pr = A.opair[size_t(m) * n + c];
+          jj[2 * m] = off16_nbr(c, A.obase[2 * m], pr & 0xffffu);
+          jj[2 * m + 1] = off16_nbr(c, A.obase[2 * m + 1], pr >> 16);
+        }
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) cf[kk] = A.ocoef[size_t(kk) * n + c];
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < WC; ++kk) {
+          const bool live = kk < w;
+          const size_t q_ = size_t(live ? kk : 0) * n + c;
+          const int j = A.nbr[q_];
+          const float h = A.coef[q_];
+          jj[kk] = (live && j >= 0) ? j : c;
+          cf[kk] = (live && j >= 0) ? h : 0.0f;
+        }
       }
 #pragma unroll
       for (int kk = 0; kk < WC; ++kk) {
@@ -1420,6 +1551,9 @@ class CgT {
     o.nbr = A.nbr;
     o.coef = coef32_;
     o.diag = diag32_;
+    o.opair = A.opair;
+    o.ocoef = A.ocoef;
+    for (int k = 0; k < 8; ++k) o.obase[k] = A.obase[k];
     return o;
   }
   float *coef32_ = nullptr, *diag32_ = nullptr;

@@ -692,6 +692,12 @@ class SolverT {
       A32.nbr = A.nbr;
       A32.coef = up(c32);
       A32.diag = up(d32);
+      std::vector<uint32_t> pr;  // the compact copy the batched FP32 direction pass streams
+      std::vector<float> oc;
+      if (off16_enabled() && ell_pack32(anbr.data(), acoef.data(), nc, wP, pr, oc, A.obase)) {
+        A.opair = up(pr);
+        A.ocoef = up(oc);
+      }
     }
     if (n_dir) {
       dir_rows = up(dir_rows_h);

@@ -220,6 +220,21 @@ def amg_solve(nbr, coef, diag, b, rtol, atol=1e-50, maxit=2000, nullspace=False,
     return rc, x, its.value, resid.value, list(info[1:1 + info[0]]), info[15]
 
 
+def ell_off16(nbr):
+    """The 16-bit offset form of an ELL neighbour list (host-check build): (W, base[W], d[W, n],
+    src[W, n]), W = 0 when the offsets do not fit."""
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    w, n = nbr.shape
+    base = (ctypes.c_int * 8)()
+    d = np.zeros((8, n), dtype=np.int16)
+    src = np.zeros((8, n), dtype=np.int32)
+    L = lib()
+    L.hc_ell_off16.argtypes = [ctypes.c_int, ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_void_p,
+                               _P(ctypes.c_int)]
+    W = L.hc_ell_off16(n, w, nbr.ctypes.data_as(_P(ctypes.c_int)), base, d.ctypes.data, src.ctypes.data_as(_P(ctypes.c_int)))
+    return W, np.array(base[:W]), d.reshape(-1)[: W * n].reshape(W, n), src.reshape(-1)[: W * n].reshape(W, n)
+
+
 def amg_vcycle(nbr, coef, diag, r, cfg=AMG_DEFAULTS):
     """One V-cycle z = M r (host-check build); returns (z, level-0 aggregates, level sizes)."""
     nbr = np.ascontiguousarray(nbr, dtype=np.int32)

@@ -129,6 +129,16 @@ int hc_cg_solve_retry(int n, int w, const int* nbr, const double* coef, const do
 // The product's flexible CG with the aggregation multigrid (AmgPc) on an ELL operator: FP64
 // flexible CG (mixed = 0) or FP32 sweeps in FP64 refinement (mixed = 1). cfg = {theta, omega,
 // alpha, coarse_max}; info receives {levels, sizes of up to 14 levels...}.
+// The 16-bit offset form of an ELL neighbour list (dm::ell_off16): returns the padded width W
+// (0: does not fit); base receives 8 slot bases, d and src W * n entries each (room for 8 * n).
+int hc_ell_off16(int n, int w, const int* nbr, int* base, int16_t* d, int* src) {
+  dm::EllOff16 o;
+  if (!dm::ell_off16(nbr, n, w, o)) return 0;
+  std::copy(o.base, o.base + 8, base);
+  std::copy(o.d.begin(), o.d.end(), d);
+  std::copy(o.src.begin(), o.src.end(), src);
+  return o.W;
+}
 int hc_amg_solve(int n, int w, const int* nbr, const double* coef, const double* diag, const double* b, double* xv,
                  double rtol, double atol, int maxit, int nullspace, int mixed, const double cfg[5], int* its,
                  double* resid, int* info) {
@@ -164,6 +174,12 @@ int hc_amg_solve(int n, int w, const int* nbr, const double* coef, const double*
     A.nbr = nbr;
     A.coef = coef;
     A.diag = diag;
+    std::vector<uint32_t> pr;  // the direction pass's compact operator, as the solver attaches it
+    std::vector<float> oc;
+    if (mixed && dm::off16_enabled() && dm::ell_pack32(nbr, coef, n, w, pr, oc, A.obase)) {
+      A.opair = pr.data();
+      A.ocoef = oc.data();
+    }
     const int rc = cg.solve(A, b, xv, rtol, atol, nullptr, maxit, nullspace != 0, its, resid);
     info[15] = cg.last_sweeps();
     cg.destroy();

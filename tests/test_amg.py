@@ -7,7 +7,7 @@ import scipy.sparse.linalg as spla
 
 import oracle.amg as om
 from dopamine_amd import cases
-from hostcheck import AMG_DEFAULTS, amg_solve, amg_vcycle
+from hostcheck import AMG_DEFAULTS, amg_solve, amg_vcycle, ell_off16
 
 
 def chain(n, c=1.0, dirichlet=0.0):
@@ -226,3 +226,88 @@ def test_binary16_coupling_storage_matches_ieee_half():
         want = v.astype(np.float16)
     assert np.array_equal(h, want.view(np.uint16))
     assert np.array_equal(back, want.astype(np.float32))
+
+
+# ---- the 16-bit neighbour offsets of the level-0 passes (dm::ell_off16)
+
+def box_ell(nx, ny, nz, seed=0):
+    """7-point box Laplacian neighbour lists in natural order, each cell's slots shuffled."""
+    n = nx * ny * nz
+    c = np.arange(n)
+    i, j, k = c % nx, (c // nx) % ny, c // (nx * ny)
+    rows = []
+    for ok, off in ((i > 0, -1), (i < nx - 1, 1), (j > 0, -nx), (j < ny - 1, nx), (k > 0, -nx * ny),
+                    (k < nz - 1, nx * ny)):
+        rows.append(np.where(ok, c + off, -1))
+    nbr = np.array(rows)
+    rng = np.random.default_rng(seed)
+    for cc in range(n):
+        nbr[:, cc] = nbr[rng.permutation(6), cc]
+    return nbr
+
+
+def decode(nbr, W, base, d, src):
+    n = nbr.shape[1]
+    j = np.arange(n)[None, :] + base[:, None] + d.astype(np.int64)
+    live = src >= 0
+    assert np.array_equal(j[~live], np.broadcast_to(np.arange(n), (W, n))[~live])  # empty slot: the cell
+    return j, live
+
+
+def test_off16_box_slots_are_seated_by_direction():
+    nx, ny, nz = 9, 7, 5
+    nbr = box_ell(nx, ny, nz)
+    W, base, d, src = ell_off16(nbr)
+    assert W == 6
+    j, live = decode(nbr, W, base, d, src)
+    c = 4 + nx * (3 + ny * 2)  # an interior cell
+    assert list(j[:, c] - c) == [-nx * ny, -nx, -1, 1, nx, nx * ny]
+    c0 = 0  # the corner: its three (positive) neighbours in the upper slots, the lower ones empty
+    assert list(live[:, c0]) == [False] * 3 + [True] * 3
+    assert list(j[3:, c0]) == [1, nx, nx * ny]
+
+
+@pytest.mark.parametrize("name", list(OPERATORS))
+def test_off16_rows_hold_the_same_neighbours(name):
+    nbr = operator(OPERATORS[name]())[0]
+    W, base, d, src = ell_off16(nbr)
+    if W == 0:
+        pytest.skip("offsets do not fit 16 bits on this operator (the int32 lists are used)")
+    j, live = decode(nbr, W, base, d, src)
+    n = nbr.shape[1]
+    for c in range(n):
+        s = src[live[:, c], c]
+        assert sorted(s) == [k for k in range(nbr.shape[0]) if nbr[k, c] >= 0]
+        assert np.array_equal(j[live[:, c], c], nbr[s, c])
+        off = (j[:, c] - c)[live[:, c]]
+        assert np.all(np.diff(off) > 0)  # the slots keep the offsets in ascending order
+
+
+def test_off16_falls_back_when_offsets_do_not_fit():
+    def periodic_chain(n):
+        nbr, _, _ = chain(n)
+        nbr[0, 0], nbr[1, n - 1] = n - 1, 0
+        return nbr
+    assert ell_off16(periodic_chain(60_000))[0] == 4
+    assert ell_off16(periodic_chain(70_000))[0] == 0  # the wrap spans more than 65,536 offsets
+    nbr = -np.ones((9, 10), dtype=np.int64)  # nine slots: wider than the compact form's eight
+    nbr[:, 9] = np.arange(9)
+    nbr[0, :9] = 9
+    assert ell_off16(nbr)[0] == 0
+
+
+@pytest.mark.parametrize("name", ["full_stack", "channel"])
+def test_off16_solve_matches_the_int32_lists(name, monkeypatch):
+    """The compact and the int32 forms differ only in the order of each row's terms."""
+    nbr, coef, diag, ns = operator(OPERATORS[name]())
+    assert ell_off16(nbr)[0] > 0
+    b = np.random.default_rng(4).standard_normal(diag.size)
+    if ns:
+        b -= b.mean()
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("DOPAMINE_ELL_OFF16", flag)
+        out[flag] = amg_solve(nbr, coef, diag, b, 1e-8, nullspace=ns, mixed=True)
+    (rc1, x1, it1, r1, *_), (rc0, x0, it0, r0, *_) = out["1"], out["0"]
+    assert rc1 == rc0 == 0 and abs(it1 - it0) <= 1
+    assert np.linalg.norm(x1 - x0) <= 1e-6 * np.linalg.norm(x0)
