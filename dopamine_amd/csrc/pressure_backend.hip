@@ -34,6 +34,8 @@ struct Ctx {
   int* nbr = nullptr;
   double *coef = nullptr, *diag = nullptr, *b = nullptr, *sol = nullptr;
   float *coef32 = nullptr, *diag32 = nullptr;
+  uint32_t* opair = nullptr;  // the direction pass's 16-bit offsets and their FP32 coefficients
+  float* ocoef = nullptr;
   bool mixed = false;
   std::string selector;
   std::unique_ptr<dm::mp::LearnedPolicy> learned;  // selector "fcg-learned"
@@ -42,7 +44,8 @@ struct Ctx {
     learned.reset();
     amg.reset();
     cg.destroy();
-    for (void* p : {(void*)nbr, (void*)coef, (void*)diag, (void*)b, (void*)sol, (void*)coef32, (void*)diag32})
+    for (void* p : {(void*)nbr, (void*)coef, (void*)diag, (void*)b, (void*)sol, (void*)coef32, (void*)diag32, (void*)opair,
+                     (void*)ocoef})
       x.release(p);
     x.destroy();
   }
@@ -125,6 +128,16 @@ int build_ctx(int n, int n_global, int n_faces, const int* l2g, const int* owner
     ctx->A32.diag = ctx->diag32;
     ctx->cg.set_mixed(ctx->A32, 1.0e-4);
     ctx->mixed = true;
+    std::vector<uint32_t> pr;
+    std::vector<float> oc;
+    if (dm::off16_enabled() && dm::ell_pack32(nbr.data(), cf.data(), n, w, pr, oc, ctx->A.obase)) {
+      ctx->opair = ctx->x.alloc<uint32_t>(pr.size());
+      ctx->ocoef = ctx->x.alloc<float>(oc.size());
+      ctx->x.upload(ctx->opair, pr.data(), pr.size());
+      ctx->x.upload(ctx->ocoef, oc.data(), oc.size());
+      ctx->A.opair = ctx->opair;
+      ctx->A.ocoef = ctx->ocoef;
+    }
   }
   if (multigrid_selector(g_solver)) {
     dm::AmgOp op;

@@ -27,13 +27,15 @@ def test_byte_model_and_committed_traffic():
     assert bench.cg_pass_bytes(m, 0, False)[0] == 40 * nc + 16 * nfp
     # interior faces plus one per periodic pair
     assert nfp == m.n_int_faces + int((m.periodic_pair >= 0).sum()) // 2
-    # the measured traffic of the benchmark's finest-level passes is committed under profiles/ and
-    # is at least their algorithmic bytes (full stack: 7,943,987 cells, 23,726,143 operator faces)
+    # the measured traffic of the benchmark's finest-level passes is committed under profiles/;
+    # the byte model prices each operator index at 4 bytes, while these passes read 16-bit offsets
+    # where they fit, so the smoothing passes move as little as 0.8x the model (full stack:
+    # 7,943,987 cells, 23,726,143 operator faces)
     nc_f, nf_f = 7_943_987, 23_726_143
     model = {"FcgDir32T<6>": 12 * nf_f + 20 * nc_f, "Up0<float, 6>": 12 * nf_f + 20 * nc_f,
              "Down0<float, 6>": 12 * nf_f + 16 * nc_f, "FcgUpd32": 24 * nc_f}
     for k, alg in model.items():
-        assert 0.98 * alg <= bench.measured_traffic(k, "full stack 250x210x160") <= 2.0 * alg, k
+        assert 0.7 * alg <= bench.measured_traffic(k, "full stack 250x210x160") <= 2.0 * alg, k
     assert bench.measured_traffic("FcgDir32T<6>", "no such workload") is None
 
 
