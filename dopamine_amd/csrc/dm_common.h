@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -64,6 +65,18 @@ DM_INLINE Red<N> red_identity() {
 
 inline int nblocks(int64_t n, int b = BLOCK) { return int((n + b - 1) / b); }
 
+// A pass functor with a positive static kSlots streams its chunk itself (run(lo, hi, thread,
+// threads, scalars) returns the thread's partials, accumulated in the same cell order as the
+// generic loop); the others are called cell by cell.
+template <class F, class = void>
+struct PassSlots {
+  static constexpr int value = 0;
+};
+template <class F>
+struct PassSlots<F, std::void_t<decltype(F::kSlots)>> {
+  static constexpr int value = F::kSlots;
+};
+
 // ------------------------------------------------------------------ host execution (tests only)
 struct HostExec {
   static constexpr bool device = false;
@@ -108,9 +121,13 @@ struct HostExec {
     typename F::Scal sc;
     if (!f.setup(tot, true, sc)) return;
     Red<NOUT> acc = red_identity<NOUT, false>();
-    for (int c = 0; c < n; ++c) {
-      const Red<NOUT> v = f(c, sc);
-      for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
+    if constexpr (PassSlots<F>::value > 0) {  // the batched form the device runs, one thread
+      acc = f.run(0, n, 0, 1, sc);
+    } else {
+      for (int c = 0; c < n; ++c) {
+        const Red<NOUT> v = f(c, sc);
+        for (int k = 0; k < NOUT; ++k) acc.v[k] = acc.v[k] + v.v[k];
+      }
     }
     for (int k = 0; k < NOUT; ++k) out_part[k] = acc.v[k];
   }

@@ -126,17 +126,6 @@ __global__ void __launch_bounds__(BLOCK) k_final(int g, Fin fin, const double* p
 constexpr int PBLOCK = DM_PBLOCK;
 constexpr int PGRID_MAX = DM_PGRID_MAX;  // four blocks of four waves per CU
 
-// A pass functor with a positive static kSlots streams its chunk itself (run(lo, hi, thread,
-// threads, scalars) returns the thread's partials, accumulated in the same cell order as the
-// generic loop); the others are called cell by cell.
-template <class F, class = void>
-struct PassSlots {
-  static constexpr int value = 0;
-};
-template <class F>
-struct PassSlots<F, std::void_t<decltype(F::kSlots)>> {
-  static constexpr int value = F::kSlots;
-};
 
 template <int NIN, int NOUT, class F>
 __global__ void __launch_bounds__(PBLOCK) k_pass(int n, int chunk, F f, const double* in_part, int in_g,
